@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Closest-hit throughput on the ~1.0M-triangle displaced icosphere with 2^22 rays (one
+MI355X), next to the reference's own BVH timed on the host cores.
+
+One "step" = one closest-hit traversal launch over the whole 2^22-ray batch, rays and BVH
+already resident in HBM.  Prints ONE JSON line (rank 0).  `--gpus N` with N > 1 exits: this
+path runs on one GPU (the design for several is a replicated BVH with sharded rays).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ray-based-rendering_amd"))
+
+HBM_PEAK_GBS = 8000.0
+SEED = 0xC1
+N_SUB = 224            # 20 * 224^2 = 1,003,520 triangles
+N_RAYS = 1 << 22
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(tris, rays, gpu_hits, seconds: float) -> dict:
+    """The reference's BVH::build + BVH::intersect (compiled from its sources into
+    oracle/_ref/libref.so), OpenMP over rays on every host core, on a bounded ray sample
+    sized to about `seconds` of CPU work.  Falls back to the oracle restatement ("port")
+    where the reference build is absent.  Also checks the GPU ids on that sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+
+    threads = O.max_threads()
+    if O.ref_available():
+        w = O.RefWorld(tris=tris)
+        build_ms = w.build_bvh()
+        kind = "reference"
+
+        def run(r):
+            h, sec = w.bvh_closest(r, threads=threads)
+            return h, sec
+    else:
+        w = O.OracleScene(tris=tris)
+        t0 = time.perf_counter()
+        w.build_bvh()
+        build_ms = (time.perf_counter() - t0) * 1e3
+        kind = "port"
+
+        def run(r):
+            t0 = time.perf_counter()
+            h = w.closest(r, threads=threads)
+            return h, time.perf_counter() - t0
+    probe = min(len(rays), 1 << 16)
+    _, sec = run(rays[:probe])
+    n = int(min(len(rays), max(probe, probe * seconds / max(sec, 1e-6))))
+    h, sec = run(rays[:n])
+    ids_equal = int((h["id"] == gpu_hits["id"][:n]).sum())
+    hit = h["id"] != 0xFFFFFFFF
+    t_equal = bool(np.array_equal(h["t"][hit], gpu_hits["t"][:n][hit]))
+    return {
+        "value": n / sec / 1e6, "unit": "Mrays/s", "cores": threads, "kind": kind,
+        "sample": f"first {n} of the {len(rays)} closest-hit rays, same mesh; BVH build {build_ms:.0f} ms (serial)",
+        "cpu": platform.processor() or platform.machine(),
+        "parity_on_sample": {"rays": n, "ids_equal": ids_equal, "t_bit_equal_on_hits": t_equal},
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--width", type=int, default=4)
+    ap.add_argument("--leaf", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this path runs on one GPU; --gpus {args.gpus} is not supported "
+              "(several GPUs would replicate the BVH and shard the rays; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import rbr_amd as R
+
+    log("bench: generating mesh and rays")
+    tris = R.icosphere(N_SUB, 100.0, 0.1, SEED)
+    lo, hi = R.mesh_bounds(tris)
+    rays = R.rays_uniform(lo, hi, N_RAYS, SEED + 1)
+    shadow = R.rays_shadow(lo, hi, N_RAYS, SEED + 2)
+
+    ctx = R.Context(0)
+    dev_name, cus = ctx.device_info()
+    sc = R.Scene(ctx)
+    sc.set_geometry(tris=tris)
+    log("bench: building BVH")
+    sc.build_bvh(args.width, args.leaf)
+    st = sc.stats()
+
+    d_rays = ctx.alloc(rays.nbytes)
+    d_shadow = ctx.alloc(shadow.nbytes)
+    d_hits = ctx.alloc(N_RAYS * 8)
+    d_occ = ctx.alloc(N_RAYS)
+    d_rays.upload(rays)
+    d_shadow.upload(shadow)
+
+    log("bench: warmup")
+    for _ in range(args.warmup):
+        sc.trace_closest_device(d_rays, N_RAYS, d_hits)
+        sc.trace_any_device(d_shadow, N_RAYS, d_occ)
+    ctx.synchronize()
+
+    log("bench: timed closest-hit")
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ms_closest = sc.time_trace(0, d_rays, N_RAYS, d_hits, args.steps)
+    ctx.synchronize()
+    wall_s = time.perf_counter() - t0
+    ms_any = sc.time_trace(1, d_shadow, N_RAYS, d_occ, max(1, args.steps // 2))
+
+    hits = np.empty(N_RAYS, R.HIT_DTYPE)
+    d_hits.download(hits)
+
+    # Algorithmic bytes per ray from the host traversal of the same layout on a sample.
+    sample = rays[:: max(1, N_RAYS // (1 << 16))]
+    _, nv, pt = sc.host_trace_closest(sample)
+    node_b = 128 if args.width == 4 else 256
+    nodes_per_ray, prims_per_ray = nv / len(sample), pt / len(sample)
+    bytes_compulsory = 32 + 8
+    bytes_per_ray = bytes_compulsory + nodes_per_ray * node_b + prims_per_ray * 48
+    rays_per_s = N_RAYS / (ms_closest * 1e-3)
+    achieved = rays_per_s * bytes_per_ray / 1e9
+
+    rec = {
+        "metric": "closest-hit Mrays/s (1.0M-tri displaced icosphere, 2^22 incoherent rays)",
+        "value": rays_per_s / 1e6,
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_closest,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded icosphere + seeded rays)",
+        "config": {
+            "workload": "closest-hit, icosphere 20*224^2 = %d tris r=100 displaced 0.1r, 2^22 rays origins in bbox*1.1, "
+                        "dirs uniform on S^2, tmin 1e-4, tmax inf" % len(tris),
+            "bvh": f"BVH{args.width}, leaf<={args.leaf}", "nodes": st["node_count"], "max_depth": st["max_depth"],
+            "seed": SEED,
+        },
+        "wall_s_timed_region": wall_s,
+        "any_hit_mrays_s": N_RAYS / (ms_any * 1e-3) / 1e6,
+        "hit_fraction": float((hits["id"] != R.MISS).mean()),
+        "device": dev_name, "cus": cus,
+        "bvh_build_ms": st["build_ms"],
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "bytes_per_ray": bytes_per_ray, "nodes_per_ray": nodes_per_ray, "prims_per_ray": prims_per_ray,
+            "frac_compulsory_only": rays_per_s * bytes_compulsory / 1e9 / HBM_PEAK_GBS,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        log("bench: CPU baseline")
+        rec["cpu_baseline"] = cpu_baseline(tris, rays, hits, args.cpu_seconds)
+    print(json.dumps(rec), flush=True)
+    for b in (d_rays, d_shadow, d_hits, d_occ):
+        b.free()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,238 @@
+/*
+ * rbr.h — C ABI of the MI355X (gfx950) ray-traversal and path-tracing library.
+ *
+ * Everything here is plain C: opaque handles, caller-owned host arrays, int status codes
+ * (RBR_OK == 0) and a per-context error string.  No C++ types, exceptions or torch types
+ * cross this boundary.  One context per host thread (calls on one context are not
+ * reentrant); several contexts may share a device.
+ *
+ * Each entry point names the reference interface it stands behind (HuynhNguyenPhuc/
+ * Ray-Based-Rendering, C++17).  The C++ adapter classes that a maintainer adds to the
+ * reference (GpuBVH : Accelerator, GpuPathTracingIntegrator : Integrator) call only these.
+ */
+#ifndef RBR_H
+#define RBR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+enum {
+    RBR_OK = 0,
+    RBR_ERR_INVALID = 1,   /* bad argument (null pointer, size out of range, bad enum) */
+    RBR_ERR_HIP = 2,       /* a HIP runtime call failed; rbr_last_error() has the text   */
+    RBR_ERR_STATE = 3,     /* call out of order (e.g. trace before a BVH was built)      */
+    RBR_ERR_NOMEM = 4,     /* device or host allocation failed                           */
+    RBR_ERR_UNSUPPORTED = 5/* feature not handled by the GPU path (caller falls back)    */
+};
+
+#define RBR_MISS 0xFFFFFFFFu
+
+/* ---------------------------------------------------------------- plain data */
+
+/* One ray, 32 bytes.  The direction is used as given: it is never re-normalized on the
+ * device (the reference's Ray constructor normalizes, ray.h:19-23; callers that build
+ * rays that way pass the normalized direction).  A hit is accepted for t in [tmin, tmax],
+ * both ends inclusive, as Triangle::intersect does (triangle.cpp:24, with tmin = EPSILON). */
+typedef struct rbr_ray {
+    float o[3];
+    float d[3];
+    float tmin;
+    float tmax;
+} rbr_ray;
+
+/* Closest hit, 8 bytes: id is the primitive id given at upload (RBR_MISS on a miss);
+ * among hits with equal t the lowest id wins. */
+typedef struct rbr_hit {
+    uint32_t id;
+    float t;
+} rbr_hit;
+
+typedef struct rbr_bvh_stats {
+    uint32_t width;        /* 4 or 8                                     */
+    uint32_t node_count;   /* wide nodes                                  */
+    uint32_t leaf_count;   /* leaf references                              */
+    uint32_t prim_count;   /* bounded primitives in the tree               */
+    uint32_t unbounded;    /* infinite planes tested linearly              */
+    uint32_t max_depth;    /* wide-tree depth                              */
+    uint64_t node_bytes;   /* device bytes of the node array               */
+    uint64_t prim_bytes;   /* device bytes of the leaf primitive records   */
+    double sah_cost;       /* SAH cost of the binary tree (Ct = 1, Ci = 1) */
+    double build_ms;       /* host build time                              */
+} rbr_bvh_stats;
+
+/* Material with the reference's six parameters (material.h:30-35). */
+typedef struct rbr_material {
+    float base_color[3];
+    float metallic;
+    float roughness;
+    float ior;
+    float transmission;
+    float emission[3];
+} rbr_material;
+
+enum { RBR_LIGHT_AREA = 0, RBR_LIGHT_POINT = 1, RBR_LIGHT_DIRECTIONAL = 2 };
+
+/* Light (light.h:48-63).  Area: quad centred at position with edges u, v; normal is
+ * normalize(cross(u, v)) as AreaLight's constructor computes it (area_light.cpp:10).
+ * Directional: direction is the normalized direction FROM the light. */
+typedef struct rbr_light {
+    int32_t type;
+    float color[3];
+    float intensity;
+    float position[3];
+    float u[3];
+    float v[3];
+    float normal[3];
+    float direction[3];
+} rbr_light;
+
+enum { RBR_MEDIUM_NONE = 0, RBR_MEDIUM_HOMOGENEOUS = 1, RBR_MEDIUM_EXPONENTIAL = 2 };
+
+/* Participating medium (medium.h:52-108) with a Henyey-Greenstein phase of parameter g. */
+typedef struct rbr_medium {
+    int32_t type;
+    float sigma_a[3];
+    float sigma_s[3];
+    float falloff;
+    float base_height;
+    float g;
+} rbr_medium;
+
+/* Pinhole camera: CameraExtrinsics (camera_extrinsics.h:18-45) + PerspectiveIntrinsics
+ * (perspective_intrinsics.h:20-45).  Raster y points down. */
+typedef struct rbr_camera {
+    float position[3];
+    float forward[3];
+    float right[3];
+    float up[3];
+    float fx, fy, cx, cy;
+    int32_t width, height;
+} rbr_camera;
+
+enum { RBR_INTEGRATOR_PATH = 0, RBR_INTEGRATOR_VOLPATH = 1 };
+
+/* IntegratorConfig (integrator.h:22-28) plus the GPU stream seed. */
+typedef struct rbr_render_config {
+    int32_t integrator;     /* RBR_INTEGRATOR_*                  */
+    int32_t spp;
+    int32_t max_depth;
+    int32_t use_nee;
+    int32_t use_mis;
+    int32_t guided;         /* 1 = MLP path guiding (PATH only)  */
+    uint64_t seed;
+    uint32_t sample_offset; /* first sample index (progressive)  */
+    uint32_t pad;
+} rbr_render_config;
+
+typedef struct rbr_ctx rbr_ctx;
+typedef struct rbr_scene rbr_scene;
+
+/* One render job: its scene, camera, settings and the caller's film (RGB float, row-major
+ * y*width+x, 3 floats per pixel).  The film receives the per-pixel mean over the job's
+ * samples, as Integrator::render's film.addSample(x, y, sum / spp) does (integrator.cpp:36). */
+typedef struct rbr_job {
+    rbr_scene* scene;
+    rbr_camera camera;
+    rbr_render_config config;
+    float* film_rgb;
+} rbr_job;
+
+/* ---------------------------------------------------------------- context */
+
+/* Creates a context on HIP device `device` with its own stream.  device == -1 gives a
+ * host-only context: geometry, BVH build and the host traversal work; kernels do not. */
+int rbr_ctx_create(int device, rbr_ctx** out);
+void rbr_ctx_destroy(rbr_ctx* ctx);
+/* Text of the last error on this context ("" if none).  Valid until the next call. */
+const char* rbr_last_error(const rbr_ctx* ctx);
+/* Name of the device the context runs on, and its compute-unit count. */
+int rbr_device_info(rbr_ctx* ctx, char* name, size_t name_len, int* cu_count);
+
+/* Device buffers for callers that keep rays/hits resident in HBM. */
+int rbr_device_alloc(rbr_ctx* ctx, size_t bytes, void** dptr);
+int rbr_device_free(rbr_ctx* ctx, void* dptr);
+int rbr_copy_to_device(rbr_ctx* ctx, void* dst, const void* src, size_t bytes);
+int rbr_copy_to_host(rbr_ctx* ctx, void* dst, const void* src, size_t bytes);
+int rbr_synchronize(rbr_ctx* ctx);
+
+/* ---------------------------------------------------------------- geometry + BVH
+ * Replaces Accelerator::build (accelerator.h:25) for the BVH (bvh.cpp:10-31).  The
+ * primitive types are those of geometry/: Triangle (triangle.h:13-22), Sphere, Quad
+ * (quad.cpp:6-10) and the unbounded Plane (plane.cpp:6-10).  Ids are the caller's
+ * primitive ids (the index in the vector passed to build()); a NULL id array numbers the
+ * primitives of that type consecutively after the previous types (tris, spheres, quads,
+ * planes).  Rebuilding replaces the previous contents (test_accelerators.cpp:148-158). */
+int rbr_scene_create(rbr_ctx* ctx, rbr_scene** out);
+void rbr_scene_destroy(rbr_scene* scene);
+
+int rbr_scene_set_geometry(rbr_scene* scene,
+                           const float* tris /* 9*n: v0 v1 v2 */, uint32_t ntri, const uint32_t* tri_ids,
+                           const float* spheres /* 4*n: center radius */, uint32_t nsph, const uint32_t* sph_ids,
+                           const float* quads /* 12*n: origin u v normal */, uint32_t nquad, const uint32_t* quad_ids,
+                           const float* planes /* 4*n: normal d */, uint32_t nplane, const uint32_t* plane_ids);
+
+/* Builds the wide BVH (width 4 or 8) over the bounded primitives, leaves of at most
+ * leaf_max primitives (1..8), and uploads it. */
+int rbr_scene_build_bvh(rbr_scene* scene, int width, int leaf_max);
+int rbr_scene_bvh_stats(const rbr_scene* scene, rbr_bvh_stats* out);
+/* Accelerator::worldBounds (accelerator.h:34): empty box (lo=+FLT_MAX, hi=-FLT_MAX) when
+ * nothing bounded was built. */
+int rbr_scene_world_bounds(const rbr_scene* scene, float lo[3], float hi[3]);
+
+/* ---------------------------------------------------------------- traversal
+ * Batched Accelerator::intersect / intersectP (accelerator.h:28-31; bvh.cpp:33-41).
+ * Host-array variants copy in, trace and copy out synchronously. */
+int rbr_trace_closest(rbr_scene* scene, const rbr_ray* rays, size_t n, rbr_hit* hits);
+int rbr_trace_any(rbr_scene* scene, const rbr_ray* rays, size_t n, uint8_t* occluded);
+/* Device-pointer variants: asynchronous on the context stream. */
+int rbr_trace_closest_device(rbr_scene* scene, const rbr_ray* d_rays, size_t n, rbr_hit* d_hits);
+int rbr_trace_any_device(rbr_scene* scene, const rbr_ray* d_rays, size_t n, uint8_t* d_occluded);
+/* Launches `iters` back-to-back traversals (kind 0 closest, 1 any) over device-resident rays
+ * and returns the mean kernel time measured with HIP events on the launch stream. */
+int rbr_time_trace(rbr_scene* scene, int kind, const rbr_ray* d_rays, size_t n, void* d_out,
+                   int iters, float* mean_ms);
+
+/* Per-ray host traversal of the same flat wide BVH with the same acceptance rule — the path
+ * the Accelerator adapter's per-ray intersect()/intersectP() take, since those are called
+ * one ray at a time from the CPU integrators (integrator.cpp:12-16) and must not launch a
+ * kernel.  Optional counters return the wide-node visits and primitive tests summed over
+ * the batch (the traversal-inclusive byte count of the bandwidth roofline uses them). */
+int rbr_host_trace_closest(const rbr_scene* scene, const rbr_ray* rays, size_t n, rbr_hit* hits,
+                           uint64_t* node_visits, uint64_t* prim_tests);
+int rbr_host_trace_any(const rbr_scene* scene, const rbr_ray* rays, size_t n, uint8_t* occluded,
+                       uint64_t* node_visits, uint64_t* prim_tests);
+
+/* ---------------------------------------------------------------- shading + rendering
+ * Scene shading tables used by the wavefront integrators: one material per primitive id
+ * (Primitive::material, primitive.h:21; -1 = no material, which ends the path as
+ * PathTracingIntegrator::Li does, path_tracing_integrator.cpp:19-20), the lights
+ * (Scene::lights, scene.h:92), the medium (Scene::medium) and the background colour. */
+int rbr_scene_set_shading(rbr_scene* scene,
+                          const rbr_material* materials, uint32_t nmat,
+                          const int32_t* prim_material, uint32_t nprim_ids,
+                          const rbr_light* lights, uint32_t nlights,
+                          const rbr_medium* medium,
+                          const float background[3]);
+
+/* Integrator::render (integrator.cpp:7-50) for PathTracingIntegrator::Li
+ * (path_tracing_integrator.cpp:5-68) or VolumetricPathIntegrator::Li
+ * (volumetric_path_integrator.cpp:6-89), for one or many jobs sharing one device ray pool. */
+int rbr_render(rbr_ctx* ctx, const rbr_job* jobs, int njobs);
+
+/* Per-pixel sums for variance estimates: adds each sample's radiance and its square to
+ * caller arrays (3 doubles per pixel each); used by the convergence tooling. */
+int rbr_render_moments(rbr_ctx* ctx, const rbr_job* job, double* sum_rgb, double* sumsq_rgb);
+
+/* Time of the last rbr_render call: total ms and per-stage ms (extend, shade, shadow). */
+int rbr_last_render_timing(rbr_ctx* ctx, double* total_ms, double* stage_ms /* 4 */);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBR_H */

@@ -1,0 +1,186 @@
+"""ctypes binding of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+liboracle.so : the repository's restatement of the reference (orc_*.cpp)
+libref.so    : the reference's own sources compiled from where they lie (ref_shim.cpp)
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module,
+and only as the checker / the baseline — never as the measured or shipped path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref.so")
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("tmin", "<f4"), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("id", "<u4"), ("t", "<f4")])
+
+_vp = C.c_void_p
+_u32 = C.c_uint32
+_sz = C.c_size_t
+
+
+def _sig(fn, args, res=None):
+    fn.argtypes = args
+    fn.restype = res
+    return fn
+
+
+def _load_oracle():
+    if not os.path.exists(ORACLE_SO):
+        raise ImportError(f"{ORACLE_SO} not built (make -C oracle)")
+    lib = C.CDLL(ORACLE_SO)
+    _sig(lib.orc_scene_create, [_vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp], _vp)
+    _sig(lib.orc_scene_destroy, [_vp])
+    _sig(lib.orc_scene_build_bvh, [_vp])
+    for name in ("orc_closest_brute", "orc_closest_bvh", "orc_any_brute", "orc_any_bvh"):
+        _sig(getattr(lib, name), [_vp, _vp, _sz, _vp, C.c_int])
+    _sig(lib.orc_triangle_t, [_vp, _vp, C.POINTER(C.c_float)], C.c_int)
+    _sig(lib.orc_max_threads, [], C.c_int)
+    return lib
+
+
+_orc = _load_oracle()
+_ref = None
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def _load_ref():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise ImportError(f"{REF_SO} not built (make -C oracle ref, where the reference tree exists)")
+        lib = C.CDLL(REF_SO)
+        _sig(lib.ref_world_create, [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32], _vp)
+        _sig(lib.ref_world_destroy, [_vp])
+        _sig(lib.ref_bvh_build, [_vp], C.c_double)
+        _sig(lib.ref_bvh_closest, [_vp, _vp, _sz, _vp, C.c_int], C.c_double)
+        _sig(lib.ref_bvh_any, [_vp, _vp, _sz, _vp, C.c_int], C.c_double)
+        _sig(lib.ref_brute_closest, [_vp, _vp, _sz, _vp, C.c_int])
+        _sig(lib.ref_brute_any, [_vp, _vp, _sz, _vp, C.c_int])
+        _sig(lib.ref_max_threads, [], C.c_int)
+        _ref = lib
+    return _ref
+
+
+def _arr(a, cols):
+    if a is None:
+        return None, 0
+    a = np.ascontiguousarray(a, np.float32).reshape(-1, cols)
+    return a, len(a)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def max_threads() -> int:
+    return _orc.orc_max_threads()
+
+
+class OracleScene:
+    """The oracle's primitive set: lowest-id closest hit by brute force or via its BVH."""
+
+    def __init__(self, tris=None, tri_ids=None, spheres=None, sphere_ids=None, quads=None, quad_ids=None,
+                 planes=None, plane_ids=None):
+        t, nt = _arr(tris, 9)
+        s, ns = _arr(spheres, 4)
+        q, nq = _arr(quads, 12)
+        p, np_ = _arr(planes, 4)
+        ids = [None if x is None else np.ascontiguousarray(x, np.uint32) for x in (tri_ids, sphere_ids, quad_ids, plane_ids)]
+        self._keep = (t, s, q, p, ids)
+        self._h = _orc.orc_scene_create(_p(t), nt, _p(ids[0]), _p(s), ns, _p(ids[1]), _p(q), nq, _p(ids[2]),
+                                        _p(p), np_, _p(ids[3]))
+        self._bvh = False
+
+    def close(self):
+        if self._h:
+            _orc.orc_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def build_bvh(self):
+        _orc.orc_scene_build_bvh(self._h)
+        self._bvh = True
+
+    def closest(self, rays, brute: bool = False, threads: int = 0) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.empty(len(rays), HIT_DTYPE)
+        fn = _orc.orc_closest_brute if (brute or not self._bvh) else _orc.orc_closest_bvh
+        fn(self._h, rays.ctypes.data, len(rays), hits.ctypes.data, threads)
+        return hits
+
+    def any(self, rays, brute: bool = False, threads: int = 0) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        occ = np.empty(len(rays), np.uint8)
+        fn = _orc.orc_any_brute if (brute or not self._bvh) else _orc.orc_any_bvh
+        fn(self._h, rays.ctypes.data, len(rays), occ.ctypes.data, threads)
+        return occ
+
+
+def triangle_t(tri9, ray) -> float | None:
+    tri = np.ascontiguousarray(tri9, np.float32).reshape(9)
+    r = np.ascontiguousarray(np.asarray(ray, RAY_DTYPE).reshape(1))
+    t = C.c_float(0.0)
+    ok = _orc.orc_triangle_t(tri.ctypes.data, r.ctypes.data, C.byref(t))
+    return t.value if ok else None
+
+
+class RefWorld:
+    """The reference's own Triangle/Sphere/Quad/Plane + BVH, compiled from its sources.
+    Primitive ids follow the input order (tris, spheres, quads, planes)."""
+
+    def __init__(self, tris=None, spheres=None, quads=None, planes=None):
+        lib = _load_ref()
+        t, nt = _arr(tris, 9)
+        s, ns = _arr(spheres, 4)
+        q, nq = _arr(quads, 12)
+        p, np_ = _arr(planes, 4)
+        self._keep = (t, s, q, p)
+        self._lib = lib
+        self._h = lib.ref_world_create(_p(t), nt, _p(s), ns, _p(q), nq, _p(p), np_)
+
+    def close(self):
+        if self._h:
+            self._lib.ref_world_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def build_bvh(self) -> float:
+        return self._lib.ref_bvh_build(self._h)
+
+    def bvh_closest(self, rays, threads: int = 0):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.empty(len(rays), HIT_DTYPE)
+        sec = self._lib.ref_bvh_closest(self._h, rays.ctypes.data, len(rays), hits.ctypes.data, threads)
+        return hits, sec
+
+    def bvh_any(self, rays, threads: int = 0):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        occ = np.empty(len(rays), np.uint8)
+        sec = self._lib.ref_bvh_any(self._h, rays.ctypes.data, len(rays), occ.ctypes.data, threads)
+        return occ, sec
+
+    def brute_closest(self, rays, threads: int = 0):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.empty(len(rays), HIT_DTYPE)
+        self._lib.ref_brute_closest(self._h, rays.ctypes.data, len(rays), hits.ctypes.data, threads)
+        return hits
+
+    def brute_any(self, rays, threads: int = 0):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        occ = np.empty(len(rays), np.uint8)
+        self._lib.ref_brute_any(self._h, rays.ctypes.data, len(rays), occ.ctypes.data, threads)
+        return occ

@@ -1,0 +1,485 @@
+// Host construction of the wide BVH: binned SAH over all three axes (32 bins), built in
+// parallel with OpenMP tasks, then collapsed greedily (open the largest-area interior
+// child until the node has `width` children).  Replaces BVH::build / buildRecursive /
+// findBestSplit (bvh.cpp:10-31, 47-209): same input (every bounded primitive, id = caller's
+// index), different tree; results do not depend on the tree because every traversal
+// returns the lowest-id closest hit.
+#include "bvh.h"
+#include "prim_tests.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace rbr {
+
+namespace {
+
+struct BPrim {
+    float lo[3], hi[3], c[3];
+    uint32_t kind, index;
+};
+
+struct BNode {
+    float lo[3], hi[3];
+    int32_t left = -1, right = -1;
+    uint32_t start = 0, count = 0;
+    bool leaf() const { return left < 0; }
+};
+
+inline float area(const float lo[3], const float hi[3]) {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    if (dx < 0 || dy < 0 || dz < 0) return 0.0f;
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+struct Builder {
+    std::vector<BPrim>& p;
+    std::vector<BNode>& nodes;
+    std::atomic<int32_t>& nextNode;
+    int leafMax;
+    static constexpr int kBins = 32;
+
+    void leaf(BNode& nd, uint32_t s, uint32_t e) {
+        nd.left = nd.right = -1;
+        nd.start = s;
+        nd.count = e - s;
+    }
+
+    void build(int32_t ni, uint32_t s, uint32_t e, int depth) {
+        BNode& nd = nodes[ni];
+        float clo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, chi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+        for (int k = 0; k < 3; ++k) { nd.lo[k] = 3.4e38f; nd.hi[k] = -3.4e38f; }
+        for (uint32_t i = s; i < e; ++i) {
+            for (int k = 0; k < 3; ++k) {
+                nd.lo[k] = std::min(nd.lo[k], p[i].lo[k]);
+                nd.hi[k] = std::max(nd.hi[k], p[i].hi[k]);
+                clo[k] = std::min(clo[k], p[i].c[k]);
+                chi[k] = std::max(chi[k], p[i].c[k]);
+            }
+        }
+        const uint32_t n = e - s;
+        if (n <= 1 || depth > 60) { leaf(nd, s, e); if (n > (uint32_t)leafMax) forceSplit(ni, s, e, depth); return; }
+
+        // Binned SAH on every axis.
+        float bestCost = 3.4e38f;
+        int bestAxis = -1, bestBin = -1;
+        const float parentArea = area(nd.lo, nd.hi);
+        for (int ax = 0; ax < 3; ++ax) {
+            float ext = chi[ax] - clo[ax];
+            if (!(ext > 0.0f)) continue;
+            float scale = float(kBins) * (1.0f - 1e-6f) / ext;
+            int cnt[kBins] = {0};
+            float blo[kBins][3], bhi[kBins][3];
+            for (int b = 0; b < kBins; ++b)
+                for (int k = 0; k < 3; ++k) { blo[b][k] = 3.4e38f; bhi[b][k] = -3.4e38f; }
+            for (uint32_t i = s; i < e; ++i) {
+                int b = std::min(kBins - 1, int((p[i].c[ax] - clo[ax]) * scale));
+                cnt[b]++;
+                for (int k = 0; k < 3; ++k) {
+                    blo[b][k] = std::min(blo[b][k], p[i].lo[k]);
+                    bhi[b][k] = std::max(bhi[b][k], p[i].hi[k]);
+                }
+            }
+            float rightArea[kBins];
+            int rightCnt[kBins];
+            float rlo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, rhi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+            int rc = 0;
+            for (int b = kBins - 1; b > 0; --b) {
+                rc += cnt[b];
+                for (int k = 0; k < 3; ++k) { rlo[k] = std::min(rlo[k], blo[b][k]); rhi[k] = std::max(rhi[k], bhi[b][k]); }
+                rightArea[b] = area(rlo, rhi);
+                rightCnt[b] = rc;
+            }
+            float llo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, lhi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+            int lc = 0;
+            for (int b = 0; b < kBins - 1; ++b) {
+                lc += cnt[b];
+                for (int k = 0; k < 3; ++k) { llo[k] = std::min(llo[k], blo[b][k]); lhi[k] = std::max(lhi[k], bhi[b][k]); }
+                if (lc == 0 || rightCnt[b + 1] == 0) continue;
+                float cost = 1.0f + (area(llo, lhi) * lc + rightArea[b + 1] * rightCnt[b + 1]) / std::max(parentArea, 1e-30f);
+                if (cost < bestCost) { bestCost = cost; bestAxis = ax; bestBin = b; }
+            }
+        }
+        if (bestAxis < 0) {   // all centroids coincide
+            leaf(nd, s, e);
+            if (n > (uint32_t)leafMax) forceSplit(ni, s, e, depth);
+            return;
+        }
+        if (n <= (uint32_t)leafMax && float(n) <= bestCost) { leaf(nd, s, e); return; }
+
+        const float ext = chi[bestAxis] - clo[bestAxis];
+        const float scale = float(kBins) * (1.0f - 1e-6f) / ext;
+        const float cmin = clo[bestAxis];
+        auto mid = std::partition(p.begin() + s, p.begin() + e, [&](const BPrim& q) {
+            return std::min(kBins - 1, int((q.c[bestAxis] - cmin) * scale)) <= bestBin;
+        });
+        uint32_t m = uint32_t(mid - p.begin());
+        if (m == s || m == e) m = s + n / 2;
+        split(ni, s, m, e, depth);
+    }
+
+    void forceSplit(int32_t ni, uint32_t s, uint32_t e, int depth) {
+        split(ni, s, s + (e - s) / 2, e, depth);
+    }
+
+    void split(int32_t ni, uint32_t s, uint32_t m, uint32_t e, int depth) {
+        int32_t l = nextNode.fetch_add(2);
+        nodes[ni].left = l;
+        nodes[ni].right = l + 1;
+        nodes[ni].start = s;
+        nodes[ni].count = e - s;
+        if (e - s > 16384) {
+#pragma omp task default(shared) firstprivate(l, s, m, depth)
+            build(l, s, m, depth + 1);
+#pragma omp task default(shared) firstprivate(l, m, e, depth)
+            build(l + 1, m, e, depth + 1);
+#pragma omp taskwait
+        } else {
+            build(l, s, m, depth + 1);
+            build(l + 1, m, e, depth + 1);
+        }
+    }
+};
+
+struct Collapser {
+    const std::vector<BNode>& bn;
+    WideBVH& out;
+    int W;
+    float pad;
+
+    void padBox(const float lo[3], const float hi[3], float plo[3], float phi[3]) const {
+        for (int k = 0; k < 3; ++k) {
+            plo[k] = lo[k] - (pad + std::fabs(lo[k]) * 1e-6f);
+            phi[k] = hi[k] + (pad + std::fabs(hi[k]) * 1e-6f);
+        }
+    }
+
+    // Emits the wide node whose children are drawn from binary node `b` (interior, or a
+    // leaf root) and returns its index.
+    uint32_t emit(int32_t b, uint32_t depth) {
+        out.maxDepth = std::max(out.maxDepth, depth);
+        int32_t kids[8];
+        int nk = 0;
+        if (bn[b].leaf()) {
+            kids[nk++] = b;
+        } else {
+            kids[nk++] = bn[b].left;
+            kids[nk++] = bn[b].right;
+            while (nk < W) {
+                int best = -1;
+                float bestA = -1.0f;
+                for (int i = 0; i < nk; ++i) {
+                    const BNode& c = bn[kids[i]];
+                    if (c.leaf()) continue;
+                    float a = area(c.lo, c.hi);
+                    if (a > bestA) { bestA = a; best = i; }
+                }
+                if (best < 0) break;
+                int32_t open = kids[best];
+                kids[best] = bn[open].left;
+                kids[nk++] = bn[open].right;
+            }
+        }
+        const uint32_t idx = out.nodeCount++;
+        const int per = out.f4PerNode();
+        out.nodes.resize(size_t(out.nodeCount) * per);
+        // Fill children (recursion may reallocate out.nodes, so write through indices).
+        uint32_t child[8];
+        float blo[8][3], bhi[8][3];
+        for (int i = 0; i < W; ++i) {
+            child[i] = kEmpty;
+            for (int k = 0; k < 3; ++k) { blo[i][k] = INFINITY; bhi[i][k] = -INFINITY; }
+        }
+        for (int i = 0; i < nk; ++i) {
+            const BNode& c = bn[kids[i]];
+            padBox(c.lo, c.hi, blo[i], bhi[i]);
+            if (c.leaf()) {
+                child[i] = makeLeaf(c.start, c.count);
+                out.leafCount++;
+            } else {
+                child[i] = emit(kids[i], depth + 1);
+            }
+        }
+        float* f = reinterpret_cast<float*>(&out.nodes[size_t(idx) * per]);
+        for (int i = 0; i < W; ++i) {
+            f[0 * W + i] = blo[i][0];
+            f[1 * W + i] = bhi[i][0];
+            f[2 * W + i] = blo[i][1];
+            f[3 * W + i] = bhi[i][1];
+            f[4 * W + i] = blo[i][2];
+            f[5 * W + i] = bhi[i][2];
+            std::memcpy(&f[6 * W + i], &child[i], 4);
+        }
+        return idx;
+    }
+};
+
+inline void minmax3(const float* a, const float* b, const float* c, float lo[3], float hi[3]) {
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(std::min(a[k], b[k]), c[k]);
+        hi[k] = std::max(std::max(a[k], b[k]), c[k]);
+    }
+}
+
+}  // namespace
+
+void buildWideBVH(const HostGeometry& g, int width, int leafMax, WideBVH& out) {
+    auto t0 = std::chrono::steady_clock::now();
+    out = WideBVH();
+    out.width = width;
+    const uint32_t ntri = uint32_t(g.triIds.size()), nsph = uint32_t(g.sphIds.size()), nquad = uint32_t(g.quadIds.size());
+    const uint32_t n = ntri + nsph + nquad;
+    out.primCount = n;
+    if (n == 0) return;
+
+    std::vector<BPrim> p(n);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)ntri; ++i) {
+        const float* t = &g.tris[9 * size_t(i)];
+        BPrim& q = p[i];
+        minmax3(t, t + 3, t + 6, q.lo, q.hi);
+        q.kind = KIND_TRI;
+        q.index = uint32_t(i);
+    }
+    for (uint32_t i = 0; i < nsph; ++i) {
+        const float* s = &g.sphs[4 * size_t(i)];
+        BPrim& q = p[ntri + i];
+        for (int k = 0; k < 3; ++k) { q.lo[k] = s[k] - s[3]; q.hi[k] = s[k] + s[3]; }
+        q.kind = KIND_SPHERE;
+        q.index = i;
+    }
+    for (uint32_t i = 0; i < nquad; ++i) {
+        const float* s = &g.quads[12 * size_t(i)];
+        BPrim& q = p[ntri + nsph + i];
+        float c1[3], c2[3], c3[3];
+        for (int k = 0; k < 3; ++k) { c1[k] = s[k] + s[3 + k]; c2[k] = s[k] + s[6 + k]; c3[k] = s[k] + s[3 + k] + s[6 + k]; }
+        minmax3(s, c1, c2, q.lo, q.hi);
+        for (int k = 0; k < 3; ++k) { q.lo[k] = std::min(q.lo[k], c3[k]); q.hi[k] = std::max(q.hi[k], c3[k]); }
+        q.kind = KIND_QUAD;
+        q.index = i;
+    }
+    float wlo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, whi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            p[i].c[k] = 0.5f * (p[i].lo[k] + p[i].hi[k]);
+            wlo[k] = std::min(wlo[k], p[i].lo[k]);
+            whi[k] = std::max(whi[k], p[i].hi[k]);
+        }
+    }
+    for (int k = 0; k < 3; ++k) { out.worldLo[k] = wlo[k]; out.worldHi[k] = whi[k]; }
+
+    std::vector<BNode> bn(size_t(2) * n + 1);
+    std::atomic<int32_t> next{1};
+    Builder b{p, bn, next, std::max(1, std::min(leafMax, int(kMaxLeaf)))};
+#pragma omp parallel
+#pragma omp single
+    b.build(0, 0, n, 0);
+    bn.resize(next.load());
+
+    // SAH cost of the binary tree (Ct = 1, Ci = 1), relative to the root area.
+    const float rootA = std::max(area(bn[0].lo, bn[0].hi), 1e-30f);
+    double sah = 0.0;
+    for (const BNode& x : bn) sah += double(area(x.lo, x.hi)) / rootA * (x.leaf() ? double(x.count) : 1.0);
+    out.sahCost = sah;
+
+    const float ext = std::max({whi[0] - wlo[0], whi[1] - wlo[1], whi[2] - wlo[2]});
+    Collapser c{bn, out, width, 1e-5f * ext + 1e-6f};
+    out.nodes.reserve(size_t(n / 2 + 8) * out.f4PerNode());
+    c.emit(0, 1);
+
+    // Leaf-ordered primitive records.
+    out.prims.resize(size_t(n) * 3);
+    for (uint32_t i = 0; i < n; ++i) {
+        const BPrim& q = p[i];
+        F4* r = &out.prims[size_t(i) * 3];
+        float idf, kf;
+        uint32_t kind = q.kind;
+        std::memcpy(&kf, &kind, 4);
+        if (q.kind == KIND_TRI) {
+            const float* t = &g.tris[9 * size_t(q.index)];
+            uint32_t id = g.triIds[q.index];
+            std::memcpy(&idf, &id, 4);
+            r[0] = {t[0], t[1], t[2], idf};
+            r[1] = {t[3] - t[0], t[4] - t[1], t[5] - t[2], kf};
+            r[2] = {t[6] - t[0], t[7] - t[1], t[8] - t[2], 0.0f};
+        } else if (q.kind == KIND_SPHERE) {
+            const float* s = &g.sphs[4 * size_t(q.index)];
+            uint32_t id = g.sphIds[q.index];
+            std::memcpy(&idf, &id, 4);
+            r[0] = {s[0], s[1], s[2], idf};
+            r[1] = {s[3], 0.0f, 0.0f, kf};
+            r[2] = {0.0f, 0.0f, 0.0f, 0.0f};
+        } else {
+            const float* s = &g.quads[12 * size_t(q.index)];
+            uint32_t id = g.quadIds[q.index];
+            std::memcpy(&idf, &id, 4);
+            float qi;
+            uint32_t qidx = q.index;
+            std::memcpy(&qi, &qidx, 4);
+            r[0] = {s[0], s[1], s[2], idf};
+            r[1] = {s[3], s[4], s[5], kf};
+            r[2] = {s[6], s[7], s[8], qi};
+        }
+    }
+    out.quadNormals.resize(nquad);
+    for (uint32_t i = 0; i < nquad; ++i) {
+        const float* s = &g.quads[12 * size_t(i)];
+        out.quadNormals[i] = {s[9], s[10], s[11], 0.0f};
+    }
+    out.buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ---------------------------------------------------------------- host traversal
+
+namespace {
+
+inline uint32_t asU(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+
+inline bool testRecord(const F4* r, const WideBVH& b, V o, V d, float tmin, float tmax, float& t) {
+    const uint32_t kind = asU(r[1].w);
+    if (kind == KIND_TRI)
+        return triTest(o, d, V{r[0].x, r[0].y, r[0].z}, V{r[1].x, r[1].y, r[1].z}, V{r[2].x, r[2].y, r[2].z}, tmin, tmax, t);
+    if (kind == KIND_SPHERE) return sphereTest(o, d, V{r[0].x, r[0].y, r[0].z}, r[1].x, tmin, tmax, t);
+    const F4& n = b.quadNormals[asU(r[2].w)];
+    return quadTest(o, d, V{r[0].x, r[0].y, r[0].z}, V{r[1].x, r[1].y, r[1].z}, V{r[2].x, r[2].y, r[2].z},
+                    V{n.x, n.y, n.z}, tmin, tmax, t);
+}
+
+struct HostRay {
+    V o, d, inv;
+    int neg[3];
+};
+
+inline HostRay makeRay(const float o[3], const float d[3]) {
+    HostRay r;
+    r.o = {o[0], o[1], o[2]};
+    r.d = {d[0], d[1], d[2]};
+    float dd[3] = {d[0], d[1], d[2]};
+    float inv[3];
+    for (int k = 0; k < 3; ++k) {
+        float x = dd[k];
+        if (std::fabs(x) < 1e-30f) x = std::copysign(1e-30f, x);
+        inv[k] = 1.0f / x;
+        r.neg[k] = inv[k] < 0.0f;
+    }
+    r.inv = {inv[0], inv[1], inv[2]};
+    return r;
+}
+
+// Slab test of child i of a node against [0, tfar].
+inline bool childHit(const float* f, int W, int i, const HostRay& r, float tfar, float& tn) {
+    const float ox = r.o.x, oy = r.o.y, oz = r.o.z;
+    float lx = f[0 * W + i], hx = f[1 * W + i], ly = f[2 * W + i], hy = f[3 * W + i], lz = f[4 * W + i], hz = f[5 * W + i];
+    float nx = r.neg[0] ? hx : lx, fx = r.neg[0] ? lx : hx;
+    float ny = r.neg[1] ? hy : ly, fy = r.neg[1] ? ly : hy;
+    float nz = r.neg[2] ? hz : lz, fz = r.neg[2] ? lz : hz;
+    float t0 = std::max(std::max((nx - ox) * r.inv.x, (ny - oy) * r.inv.y), std::max((nz - oz) * r.inv.z, 0.0f));
+    float t1 = std::min(std::min((fx - ox) * r.inv.x, (fy - oy) * r.inv.y), std::min((fz - oz) * r.inv.z, tfar));
+    tn = t0;
+    return t0 <= t1;
+}
+
+}  // namespace
+
+uint32_t hostClosest(const WideBVH& b, const HostGeometry& g, const float o[3], const float d[3], float tmin,
+                     float tmax, float* tHit, TraceCounters* cnt) {
+    uint32_t bestId = 0xFFFFFFFFu;
+    float bestT = tmax;
+    HostRay r = makeRay(o, d);
+    if (!b.empty()) {
+        const int W = b.width, per = b.f4PerNode();
+        uint32_t stk[256];
+        float stt[256];
+        int sp = 0;
+        stk[sp] = 0;
+        stt[sp] = 0.0f;
+        sp++;
+        while (sp > 0) {
+            --sp;
+            uint32_t ref = stk[sp];
+            if (stt[sp] > bestT) continue;
+            if (isLeaf(ref)) {
+                const uint32_t s = leafStart(ref), c = leafCount(ref);
+                for (uint32_t k = 0; k < c; ++k) {
+                    const F4* rec = &b.prims[size_t(s + k) * 3];
+                    float t;
+                    if (cnt) cnt->prims++;
+                    if (testRecord(rec, b, r.o, r.d, tmin, bestT, t)) {
+                        uint32_t id = asU(rec[0].w);
+                        if (t < bestT || id < bestId) { bestT = t; bestId = id; }
+                    }
+                }
+                continue;
+            }
+            if (cnt) cnt->nodes++;
+            const float* f = reinterpret_cast<const float*>(&b.nodes[size_t(ref) * per]);
+            const uint32_t* ch = reinterpret_cast<const uint32_t*>(f + 6 * W);
+            uint32_t hr[8];
+            float ht[8];
+            int nh = 0;
+            for (int i = 0; i < W; ++i) {
+                float tn;
+                if (ch[i] != kEmpty && childHit(f, W, i, r, bestT, tn)) { hr[nh] = ch[i]; ht[nh] = tn; nh++; }
+            }
+            // Push farthest first so the nearest is popped next.
+            for (int i = 1; i < nh; ++i)
+                for (int j = i; j > 0 && ht[j] > ht[j - 1]; --j) { std::swap(ht[j], ht[j - 1]); std::swap(hr[j], hr[j - 1]); }
+            for (int i = 0; i < nh; ++i) { stk[sp] = hr[i]; stt[sp] = ht[i]; sp++; }
+        }
+    }
+    const size_t np = g.planeIds.size();
+    for (size_t k = 0; k < np; ++k) {
+        const float* pl = &g.planes[4 * k];
+        float t;
+        if (planeTest(r.o, r.d, V{pl[0], pl[1], pl[2]}, pl[3], tmin, bestT, t)) {
+            uint32_t id = g.planeIds[k];
+            if (t < bestT || id < bestId) { bestT = t; bestId = id; }
+        }
+    }
+    if (tHit) *tHit = bestT;
+    return bestId;
+}
+
+bool hostAny(const WideBVH& b, const HostGeometry& g, const float o[3], const float d[3], float tmin, float tmax,
+             TraceCounters* cnt) {
+    HostRay r = makeRay(o, d);
+    if (!b.empty()) {
+        const int W = b.width, per = b.f4PerNode();
+        uint32_t stk[256];
+        int sp = 0;
+        stk[sp++] = 0;
+        while (sp > 0) {
+            uint32_t ref = stk[--sp];
+            if (isLeaf(ref)) {
+                const uint32_t s = leafStart(ref), c = leafCount(ref);
+                for (uint32_t k = 0; k < c; ++k) {
+                    float t;
+                    if (cnt) cnt->prims++;
+                    if (testRecord(&b.prims[size_t(s + k) * 3], b, r.o, r.d, tmin, tmax, t)) return true;
+                }
+                continue;
+            }
+            if (cnt) cnt->nodes++;
+            const float* f = reinterpret_cast<const float*>(&b.nodes[size_t(ref) * per]);
+            const uint32_t* ch = reinterpret_cast<const uint32_t*>(f + 6 * W);
+            for (int i = 0; i < W; ++i) {
+                float tn;
+                if (ch[i] != kEmpty && childHit(f, W, i, r, tmax, tn)) stk[sp++] = ch[i];
+            }
+        }
+    }
+    const size_t np = g.planeIds.size();
+    for (size_t k = 0; k < np; ++k) {
+        const float* pl = &g.planes[4 * k];
+        float t;
+        if (planeTest(r.o, r.d, V{pl[0], pl[1], pl[2]}, pl[3], tmin, tmax, t)) return true;
+    }
+    return false;
+}
+
+}  // namespace rbr

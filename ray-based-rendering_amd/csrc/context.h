@@ -1,0 +1,64 @@
+// Internal definitions behind the opaque rbr_ctx / rbr_scene handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/rbr.h"
+#include "bvh.h"
+#include "trace.h"
+
+namespace rbr {
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct RenderState;   // render.hip
+
+}  // namespace rbr
+
+struct rbr_ctx {
+    int device = 0;
+    int cuCount = 0;
+    std::string name;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    rbr::RenderState* render = nullptr;
+};
+
+struct rbr_scene {
+    rbr_ctx* ctx = nullptr;
+    rbr::HostGeometry geom;
+    rbr::WideBVH bvh;
+    bool built = false;
+    rbr::DevBuf dNodes, dPrims, dQuadN, dPlanes, dPlaneIds;
+    rbr::DevBuf dRays, dOut;   // staging for the host-array trace entry points
+    // Shading tables (render.hip).
+    bool shaded = false;
+    std::vector<rbr_material> materials;
+    std::vector<int32_t> primMaterial;
+    std::vector<rbr_light> lights;
+    rbr_medium medium{};
+    float background[3] = {0.0f, 0.0f, 0.0f};
+    rbr::DevBuf dMaterials, dPrimMaterial, dLights;
+};
+
+namespace rbr {
+
+int fail(rbr_ctx* ctx, int code, const std::string& msg);
+int failHip(rbr_ctx* ctx, hipError_t e, const char* what);
+int ensureBuffer(rbr_ctx* ctx, DevBuf& b, size_t bytes);
+void freeBuffer(DevBuf& b);
+int uploadBuffer(rbr_ctx* ctx, DevBuf& b, const void* src, size_t bytes);
+int useDevice(rbr_ctx* ctx);
+TraceArgs sceneTraceArgs(const rbr_scene* s);
+
+void freeShading(rbr_scene* s);          // render.hip
+void destroyRenderState(rbr_ctx* ctx);   // render.hip
+
+}  // namespace rbr

@@ -1,0 +1,402 @@
+// C ABI (include/rbr.h): contexts, scenes, geometry upload, BVH build and traversal.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "context.h"
+#include "trace.h"
+
+using namespace rbr;
+
+// ---------------------------------------------------------------- helpers
+
+int rbr::fail(rbr_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int rbr::failHip(rbr_ctx* ctx, hipError_t e, const char* what) {
+    return fail(ctx, RBR_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int rbr::ensureBuffer(rbr_ctx* ctx, DevBuf& b, size_t bytes) {
+    if (b.bytes >= bytes && b.ptr) return RBR_OK;
+    if (b.ptr) {
+        hipError_t e = hipFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+        if (e != hipSuccess) return failHip(ctx, e, "hipFree");
+    }
+    if (bytes == 0) return RBR_OK;
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        return failHip(ctx, e, "hipMalloc");
+    }
+    b.bytes = bytes;
+    return RBR_OK;
+}
+
+void rbr::freeBuffer(DevBuf& b) {
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+int rbr::uploadBuffer(rbr_ctx* ctx, DevBuf& b, const void* src, size_t bytes) {
+    int rc = ensureBuffer(ctx, b, bytes);
+    if (rc) return rc;
+    if (bytes == 0) return RBR_OK;
+    hipError_t e = hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return failHip(ctx, e, "hipMemcpyAsync H2D");
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return failHip(ctx, e, "hipStreamSynchronize");
+    return RBR_OK;
+}
+
+int rbr::useDevice(rbr_ctx* ctx) {
+    if (ctx->device < 0) return fail(ctx, RBR_ERR_STATE, "host-only context (device -1) cannot run kernels");
+    hipError_t e = hipSetDevice(ctx->device);
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "hipSetDevice");
+}
+
+static void freeSceneDevice(rbr_scene* s) {
+    freeBuffer(s->dNodes);
+    freeBuffer(s->dPrims);
+    freeBuffer(s->dQuadN);
+    freeBuffer(s->dPlanes);
+    freeBuffer(s->dPlaneIds);
+    freeBuffer(s->dRays);
+    freeBuffer(s->dOut);
+    freeShading(s);
+}
+
+static TraceArgs traceArgs(const rbr_scene* s) {
+    TraceArgs a{};
+    a.nodes = s->bvh.empty() ? nullptr : static_cast<const float4*>(s->dNodes.ptr);
+    a.prims = static_cast<const float4*>(s->dPrims.ptr);
+    a.quadN = static_cast<const float4*>(s->dQuadN.ptr);
+    a.planes = static_cast<const float4*>(s->dPlanes.ptr);
+    a.planeIds = static_cast<const uint32_t*>(s->dPlaneIds.ptr);
+    a.nplanes = uint32_t(s->geom.planeIds.size());
+    a.mixed = (s->geom.sphIds.size() + s->geom.quadIds.size()) > 0 ? 1u : 0u;
+    return a;
+}
+
+TraceArgs rbr::sceneTraceArgs(const rbr_scene* s) { return traceArgs(s); }
+
+// ---------------------------------------------------------------- context
+
+extern "C" {
+
+int rbr_ctx_create(int device, rbr_ctx** out) {
+    if (!out) return RBR_ERR_INVALID;
+    *out = nullptr;
+    if (device == -1) {   // host-only context: BVH build and host traversal, no kernels
+        auto* ctx = new (std::nothrow) rbr_ctx();
+        if (!ctx) return RBR_ERR_NOMEM;
+        ctx->device = -1;
+        ctx->name = "host";
+        *out = ctx;
+        return RBR_OK;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return RBR_ERR_INVALID;
+    auto* ctx = new (std::nothrow) rbr_ctx();
+    if (!ctx) return RBR_ERR_NOMEM;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        delete ctx;
+        return RBR_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        ctx->cuCount = prop.multiProcessorCount;
+        ctx->name = prop.gcnArchName;
+        ctx->name += " / ";
+        ctx->name += prop.name;
+    }
+    *out = ctx;
+    return RBR_OK;
+}
+
+void rbr_ctx_destroy(rbr_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->device < 0) { delete ctx; return; }
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    destroyRenderState(ctx);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* rbr_last_error(const rbr_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rbr_device_info(rbr_ctx* ctx, char* name, size_t name_len, int* cu_count) {
+    if (!ctx) return RBR_ERR_INVALID;
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s", ctx->name.c_str());
+    }
+    if (cu_count) *cu_count = ctx->cuCount;
+    return RBR_OK;
+}
+
+int rbr_device_alloc(rbr_ctx* ctx, size_t bytes, void** dptr) {
+    if (!ctx || !dptr) return RBR_ERR_INVALID;
+    if (int rc = useDevice(ctx)) return rc;
+    hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+    if (e != hipSuccess) { *dptr = nullptr; return failHip(ctx, e, "hipMalloc"); }
+    return RBR_OK;
+}
+
+int rbr_device_free(rbr_ctx* ctx, void* dptr) {
+    if (!ctx) return RBR_ERR_INVALID;
+    if (!dptr) return RBR_OK;
+    if (int rc = useDevice(ctx)) return rc;
+    hipError_t e = hipFree(dptr);
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "hipFree");
+}
+
+int rbr_copy_to_device(rbr_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return RBR_ERR_INVALID;
+    if (int rc = useDevice(ctx)) return rc;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "copy to device");
+}
+
+int rbr_copy_to_host(rbr_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return RBR_ERR_INVALID;
+    if (int rc = useDevice(ctx)) return rc;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "copy to host");
+}
+
+int rbr_synchronize(rbr_ctx* ctx) {
+    if (!ctx) return RBR_ERR_INVALID;
+    if (int rc = useDevice(ctx)) return rc;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "hipStreamSynchronize");
+}
+
+// ---------------------------------------------------------------- scenes
+
+int rbr_scene_create(rbr_ctx* ctx, rbr_scene** out) {
+    if (!ctx || !out) return RBR_ERR_INVALID;
+    auto* s = new (std::nothrow) rbr_scene();
+    if (!s) return fail(ctx, RBR_ERR_NOMEM, "scene allocation");
+    s->ctx = ctx;
+    *out = s;
+    return RBR_OK;
+}
+
+void rbr_scene_destroy(rbr_scene* s) {
+    if (!s) return;
+    if (s->ctx->device < 0) { delete s; return; }
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    freeSceneDevice(s);
+    delete s;
+}
+
+int rbr_scene_set_geometry(rbr_scene* s, const float* tris, uint32_t ntri, const uint32_t* tri_ids,
+                           const float* spheres, uint32_t nsph, const uint32_t* sph_ids, const float* quads,
+                           uint32_t nquad, const uint32_t* quad_ids, const float* planes, uint32_t nplane,
+                           const uint32_t* plane_ids) {
+    if (!s) return RBR_ERR_INVALID;
+    rbr_ctx* ctx = s->ctx;
+    if ((ntri && !tris) || (nsph && !spheres) || (nquad && !quads) || (nplane && !planes))
+        return fail(ctx, RBR_ERR_INVALID, "null primitive array with nonzero count");
+    const uint64_t total = uint64_t(ntri) + nsph + nquad + nplane;
+    if (total >= (1ull << 27)) return fail(ctx, RBR_ERR_INVALID, "too many primitives (limit 2^27)");
+    HostGeometry g;
+    uint32_t next = 0;
+    auto ids = [&](std::vector<uint32_t>& dst, const uint32_t* src, uint32_t n) {
+        dst.resize(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            dst[i] = src ? src[i] : next + i;
+            if (dst[i] == RBR_MISS) return false;
+            g.maxId = std::max(g.maxId, dst[i]);
+            g.anyId = true;
+        }
+        next += n;
+        return true;
+    };
+    g.tris.assign(tris, tris + size_t(ntri) * 9);
+    g.sphs.assign(spheres, spheres + size_t(nsph) * 4);
+    g.quads.assign(quads, quads + size_t(nquad) * 12);
+    g.planes.assign(planes, planes + size_t(nplane) * 4);
+    if (!ids(g.triIds, tri_ids, ntri) || !ids(g.sphIds, sph_ids, nsph) || !ids(g.quadIds, quad_ids, nquad) ||
+        !ids(g.planeIds, plane_ids, nplane))
+        return fail(ctx, RBR_ERR_INVALID, "primitive id 0xFFFFFFFF is reserved for a miss");
+    s->geom = std::move(g);
+    s->built = false;
+    s->bvh = WideBVH();
+    return RBR_OK;
+}
+
+int rbr_scene_build_bvh(rbr_scene* s, int width, int leaf_max) {
+    if (!s) return RBR_ERR_INVALID;
+    rbr_ctx* ctx = s->ctx;
+    if (width != 4 && width != 8) return fail(ctx, RBR_ERR_INVALID, "BVH width must be 4 or 8");
+    if (leaf_max < 1 || leaf_max > 8) return fail(ctx, RBR_ERR_INVALID, "leaf_max must be in 1..8");
+    s->built = false;
+    buildWideBVH(s->geom, width, leaf_max, s->bvh);
+    if (!s->bvh.empty() && uint64_t(width - 1) * s->bvh.maxDepth + 1 > uint64_t(kMaxStack))
+        return fail(ctx, RBR_ERR_UNSUPPORTED, "BVH too deep for the traversal stack");
+    if (ctx->device < 0) { s->built = true; return RBR_OK; }
+    if (int rc = useDevice(ctx)) return rc;
+    int rc;
+    if ((rc = uploadBuffer(ctx, s->dNodes, s->bvh.nodes.data(), s->bvh.nodes.size() * sizeof(F4)))) return rc;
+    if ((rc = uploadBuffer(ctx, s->dPrims, s->bvh.prims.data(), s->bvh.prims.size() * sizeof(F4)))) return rc;
+    if ((rc = uploadBuffer(ctx, s->dQuadN, s->bvh.quadNormals.data(), s->bvh.quadNormals.size() * sizeof(F4)))) return rc;
+    if ((rc = uploadBuffer(ctx, s->dPlanes, s->geom.planes.data(), s->geom.planes.size() * sizeof(float)))) return rc;
+    if ((rc = uploadBuffer(ctx, s->dPlaneIds, s->geom.planeIds.data(), s->geom.planeIds.size() * sizeof(uint32_t)))) return rc;
+    s->built = true;
+    return RBR_OK;
+}
+
+int rbr_scene_bvh_stats(const rbr_scene* s, rbr_bvh_stats* out) {
+    if (!s || !out) return RBR_ERR_INVALID;
+    if (!s->built) return fail(s->ctx, RBR_ERR_STATE, "BVH not built");
+    out->width = uint32_t(s->bvh.width);
+    out->node_count = s->bvh.nodeCount;
+    out->leaf_count = s->bvh.leafCount;
+    out->prim_count = s->bvh.primCount;
+    out->unbounded = uint32_t(s->geom.planeIds.size());
+    out->max_depth = s->bvh.maxDepth;
+    out->node_bytes = uint64_t(s->bvh.nodes.size()) * sizeof(F4);
+    out->prim_bytes = uint64_t(s->bvh.prims.size()) * sizeof(F4);
+    out->sah_cost = s->bvh.sahCost;
+    out->build_ms = s->bvh.buildMs;
+    return RBR_OK;
+}
+
+int rbr_scene_world_bounds(const rbr_scene* s, float lo[3], float hi[3]) {
+    if (!s || !lo || !hi) return RBR_ERR_INVALID;
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = s->built && !s->bvh.empty() ? s->bvh.worldLo[k] : 3.40282347e38f;
+        hi[k] = s->built && !s->bvh.empty() ? s->bvh.worldHi[k] : -3.40282347e38f;
+    }
+    return RBR_OK;
+}
+
+// ---------------------------------------------------------------- traversal
+
+static int traceDevice(rbr_scene* s, const rbr_ray* dRays, size_t n, void* dOut, bool any) {
+    rbr_ctx* ctx = s->ctx;
+    if (!s->built) return fail(ctx, RBR_ERR_STATE, "trace before rbr_scene_build_bvh");
+    if (n > 0x80000000ull) return fail(ctx, RBR_ERR_INVALID, "ray batch too large (limit 2^31)");
+    if (n && (!dRays || !dOut)) return fail(ctx, RBR_ERR_INVALID, "null ray or result pointer");
+    TraceArgs a = traceArgs(s);
+    a.rays = reinterpret_cast<const float4*>(dRays);
+    a.n = uint32_t(n);
+    a.hits = any ? nullptr : static_cast<uint2*>(dOut);
+    a.occ = any ? static_cast<uint8_t*>(dOut) : nullptr;
+    hipError_t e = launchTrace(a, s->bvh.width, any, ctx->stream);
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "trace launch");
+}
+
+static int traceHost(rbr_scene* s, const rbr_ray* rays, size_t n, void* out, bool any) {
+    if (!s) return RBR_ERR_INVALID;
+    rbr_ctx* ctx = s->ctx;
+    if (n == 0) return RBR_OK;
+    if (!rays || !out) return fail(ctx, RBR_ERR_INVALID, "null ray or result array");
+    if (int rc = useDevice(ctx)) return rc;
+    const size_t outBytes = n * (any ? sizeof(uint8_t) : sizeof(rbr_hit));
+    int rc;
+    if ((rc = ensureBuffer(ctx, s->dRays, n * sizeof(rbr_ray)))) return rc;
+    if ((rc = ensureBuffer(ctx, s->dOut, outBytes))) return rc;
+    hipError_t e = hipMemcpyAsync(s->dRays.ptr, rays, n * sizeof(rbr_ray), hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return failHip(ctx, e, "ray upload");
+    if ((rc = traceDevice(s, static_cast<const rbr_ray*>(s->dRays.ptr), n, s->dOut.ptr, any))) return rc;
+    e = hipMemcpyAsync(out, s->dOut.ptr, outBytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) return failHip(ctx, e, "result download");
+    e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "trace");
+}
+
+int rbr_trace_closest(rbr_scene* s, const rbr_ray* rays, size_t n, rbr_hit* hits) {
+    return traceHost(s, rays, n, hits, false);
+}
+
+int rbr_trace_any(rbr_scene* s, const rbr_ray* rays, size_t n, uint8_t* occluded) {
+    return traceHost(s, rays, n, occluded, true);
+}
+
+int rbr_trace_closest_device(rbr_scene* s, const rbr_ray* d_rays, size_t n, rbr_hit* d_hits) {
+    if (!s) return RBR_ERR_INVALID;
+    if (int rc = useDevice(s->ctx)) return rc;
+    return traceDevice(s, d_rays, n, d_hits, false);
+}
+
+int rbr_trace_any_device(rbr_scene* s, const rbr_ray* d_rays, size_t n, uint8_t* d_occluded) {
+    if (!s) return RBR_ERR_INVALID;
+    if (int rc = useDevice(s->ctx)) return rc;
+    return traceDevice(s, d_rays, n, d_occluded, true);
+}
+
+int rbr_host_trace_closest(const rbr_scene* s, const rbr_ray* rays, size_t n, rbr_hit* hits, uint64_t* nodeVisits,
+                           uint64_t* primTests) {
+    if (!s) return RBR_ERR_INVALID;
+    if (!s->built) return fail(s->ctx, RBR_ERR_STATE, "trace before rbr_scene_build_bvh");
+    if (n && (!rays || !hits)) return fail(s->ctx, RBR_ERR_INVALID, "null ray or result array");
+    TraceCounters total;
+    for (size_t i = 0; i < n; ++i) {
+        TraceCounters c;
+        float t;
+        uint32_t id = hostClosest(s->bvh, s->geom, rays[i].o, rays[i].d, rays[i].tmin, rays[i].tmax, &t, &c);
+        hits[i].id = id;
+        hits[i].t = id == RBR_MISS ? 0.0f : t;
+        total.nodes += c.nodes;
+        total.prims += c.prims;
+    }
+    if (nodeVisits) *nodeVisits = total.nodes;
+    if (primTests) *primTests = total.prims;
+    return RBR_OK;
+}
+
+int rbr_host_trace_any(const rbr_scene* s, const rbr_ray* rays, size_t n, uint8_t* occ, uint64_t* nodeVisits,
+                       uint64_t* primTests) {
+    if (!s) return RBR_ERR_INVALID;
+    if (!s->built) return fail(s->ctx, RBR_ERR_STATE, "trace before rbr_scene_build_bvh");
+    if (n && (!rays || !occ)) return fail(s->ctx, RBR_ERR_INVALID, "null ray or result array");
+    TraceCounters total;
+    for (size_t i = 0; i < n; ++i) {
+        TraceCounters c;
+        occ[i] = hostAny(s->bvh, s->geom, rays[i].o, rays[i].d, rays[i].tmin, rays[i].tmax, &c) ? 1 : 0;
+        total.nodes += c.nodes;
+        total.prims += c.prims;
+    }
+    if (nodeVisits) *nodeVisits = total.nodes;
+    if (primTests) *primTests = total.prims;
+    return RBR_OK;
+}
+
+int rbr_time_trace(rbr_scene* s, int kind, const rbr_ray* d_rays, size_t n, void* d_out, int iters, float* mean_ms) {
+    if (!s || !mean_ms || iters < 1 || (kind != 0 && kind != 1)) return RBR_ERR_INVALID;
+    rbr_ctx* ctx = s->ctx;
+    if (int rc = useDevice(ctx)) return rc;
+    hipError_t e = hipEventRecord(ctx->ev0, ctx->stream);
+    if (e != hipSuccess) return failHip(ctx, e, "hipEventRecord");
+    for (int it = 0; it < iters; ++it) {
+        if (int rc = traceDevice(s, d_rays, n, d_out, kind == 1)) return rc;
+    }
+    e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    if (e != hipSuccess) return failHip(ctx, e, "timing");
+    float ms = 0.0f;
+    e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    if (e != hipSuccess) return failHip(ctx, e, "hipEventElapsedTime");
+    *mean_ms = ms / float(iters);
+    return RBR_OK;
+}
+
+}  // extern "C"

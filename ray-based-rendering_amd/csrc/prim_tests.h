@@ -1,0 +1,89 @@
+// Exact primitive tests shared by the host traversal and the HIP kernels (__host__
+// __device__).  Every operation keeps the reference's operand order; the translation
+// units that include this file are compiled with FP contraction off, so host and device
+// produce bit-identical t values.
+//   Triangle::intersect  triangle.cpp:6-24   (e1, e2 precomputed with the same subtraction)
+//   Sphere::intersect    sphere.cpp:6-21
+//   Quad::intersect      quad.cpp:12-31
+//   Plane::intersect     plane.cpp:12-17
+#pragma once
+
+#ifdef __HIPCC__
+#define RBR_HD __host__ __device__ __forceinline__
+#else
+#define RBR_HD inline
+#endif
+
+#include <cmath>
+
+namespace rbr {
+
+constexpr float kEps = 1e-4f;   // EPSILON, common.h:24
+
+struct V { float x, y, z; };
+
+RBR_HD V vsub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+RBR_HD float vdot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+RBR_HD V vcross(V a, V b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// Returns true and t when the triangle (v0, e1, e2) is hit with t in [tmin, tmax].
+RBR_HD bool triTest(V o, V d, V v0, V e1, V e2, float tmin, float tmax, float& tout) {
+    V h = vcross(d, e2);
+    float a = vdot(e1, h);
+    if (fabsf(a) < kEps) return false;
+    float f = 1.0f / a;
+    V s = vsub(o, v0);
+    float u = f * vdot(s, h);
+    if (u < 0.0f || u > 1.0f) return false;
+    V q = vcross(s, e1);
+    float v = f * vdot(d, q);
+    if (v < 0.0f || u + v > 1.0f) return false;
+    float t = f * vdot(e2, q);
+    if (t < tmin || t > tmax) return false;
+    tout = t;
+    return true;
+}
+
+RBR_HD bool sphereTest(V o, V d, V c, float r, float tmin, float tmax, float& tout) {
+    V oc = vsub(o, c);
+    float a = vdot(d, d);
+    float halfB = vdot(oc, d);
+    float cc = vdot(oc, oc) - r * r;
+    float disc = halfB * halfB - a * cc;
+    if (disc < 0.0f) return false;
+    float sq = sqrtf(disc);
+    float t = (-halfB - sq) / a;
+    if (t < tmin || t > tmax) {
+        t = (-halfB + sq) / a;
+        if (t < tmin || t > tmax) return false;
+    }
+    tout = t;
+    return true;
+}
+
+RBR_HD bool quadTest(V o, V d, V org, V u, V vv, V n, float tmin, float tmax, float& tout) {
+    float denom = vdot(n, d);
+    if (fabsf(denom) < kEps) return false;
+    float t = vdot(vsub(org, o), n) / denom;
+    if (t < tmin || t > tmax) return false;
+    V hp = {o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
+    V lp = vsub(hp, org);
+    float uLen2 = vdot(u, u);
+    float vLen2 = vdot(vv, vv);
+    float uc = vdot(lp, u) / uLen2;
+    float vc = vdot(lp, vv) / vLen2;
+    if (uc < 0.0f || uc > 1.0f || vc < 0.0f || vc > 1.0f) return false;
+    tout = t;
+    return true;
+}
+
+RBR_HD bool planeTest(V o, V d, V n, float pd, float tmin, float tmax, float& tout) {
+    float denom = vdot(n, d);
+    if (fabsf(denom) < kEps) return false;
+    float t = -(vdot(n, o) + pd) / denom;
+    if (t < tmin || t > tmax) return false;
+    tout = t;
+    return true;
+}
+
+}  // namespace rbr

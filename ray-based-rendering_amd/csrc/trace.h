@@ -1,0 +1,36 @@
+// Launch interface of the traversal kernels (trace.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rbr {
+
+constexpr uint32_t RBR_MISS_ID = 0xFFFFFFFFu;
+constexpr uint32_t kEmptyD = 0xFFFFFFFFu;
+constexpr int kBlock = 64;        // one wave per workgroup
+constexpr int kLdsDepth = 16;     // LDS stack entries per lane (8 B each)
+constexpr int kMaxStack = 192;    // total stack entries per lane (LDS + private spill)
+
+__host__ __device__ __forceinline__ bool isLeafD(uint32_t c) { return (c & 0x80000000u) != 0u; }
+__host__ __device__ __forceinline__ uint32_t leafCountD(uint32_t c) { return ((c >> 27) & 7u) + 1u; }
+__host__ __device__ __forceinline__ uint32_t leafStartD(uint32_t c) { return c & ((1u << 27) - 1u); }
+
+struct TraceArgs {
+    const float4* nodes;      // null when no bounded primitive exists
+    const float4* prims;
+    const float4* quadN;
+    const float4* planes;
+    const uint32_t* planeIds;
+    uint32_t nplanes;
+    uint32_t mixed;           // any non-triangle bounded primitive
+    const float4* rays;       // 2 float4 per ray
+    uint32_t n;
+    uint2* hits;              // closest: (id, t bits)
+    uint8_t* occ;             // any
+};
+
+hipError_t launchTrace(const TraceArgs& a, int width, bool any, hipStream_t s);
+
+}  // namespace rbr

@@ -1,0 +1,264 @@
+// Wide-BVH closest-hit / any-hit traversal kernels for gfx950.
+//
+// Replaces BVH::intersect / intersectP (bvh.cpp:33-41, 211-267): instead of a recursive,
+// pointer-chasing walk per ray with a virtual call per primitive, one lane owns one ray and
+// walks the flat wide BVH (bvh.h) with a short stack:
+//   * node boxes are read as aligned float4 rows (one 128-byte line per BVH4 node);
+//   * children are tested with the sign-selected FMA slab test against [0, t_best] and
+//     visited nearest-first; pushed entries carry their entry distance so entries that
+//     became farther than the current closest hit are dropped on pop without a fetch;
+//   * the stack lives in LDS (kLdsDepth entries per lane, [depth][lane] so a wave's
+//     pushes hit distinct banks) and spills to per-lane private memory beyond that;
+//   * primitives are tested with the exact Moller-Trumbore arithmetic of the reference
+//     (prim_tests.h, contraction off) and the lowest id wins among equal t.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bvh.h"
+#include "prim_tests.h"
+#include "trace.h"
+
+#pragma clang fp contract(off)
+
+namespace rbr {
+
+namespace {
+
+__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
+
+struct DRay {
+    V o, d;
+    float ix, iy, iz;      // 1/d with zero components replaced by +-1e-30
+    float oix, oiy, oiz;   // o * inv
+    bool nx, ny, nz;       // sign of inv
+    float tmin, tmax;
+};
+
+__device__ __forceinline__ float safeInv(float x) {
+    float y = fabsf(x) < 1e-30f ? copysignf(1e-30f, x) : x;
+    return __builtin_amdgcn_rcpf(y);
+}
+
+__device__ __forceinline__ DRay loadRay(const float4* rays, uint32_t i) {
+    float4 a = rays[2 * size_t(i)];
+    float4 b = rays[2 * size_t(i) + 1];
+    DRay r;
+    r.o = V{a.x, a.y, a.z};
+    r.d = V{a.w, b.x, b.y};
+    r.tmin = b.z;
+    r.tmax = b.w;
+    r.ix = safeInv(r.d.x);
+    r.iy = safeInv(r.d.y);
+    r.iz = safeInv(r.d.z);
+    r.oix = r.o.x * r.ix;
+    r.oiy = r.o.y * r.iy;
+    r.oiz = r.o.z * r.iz;
+    r.nx = r.ix < 0.0f;
+    r.ny = r.iy < 0.0f;
+    r.nz = r.iz < 0.0f;
+    return r;
+}
+
+// Slab test of one child; lo/hi per axis from the node rows.  Uses fused multiply-adds on
+// purpose (the boxes are padded, so the rounding cannot cull a real hit).
+__device__ __forceinline__ float slab(const DRay& r, float lx, float hx, float ly, float hy, float lz, float hz,
+                                      float tfar, bool& hit) {
+    float nx = r.nx ? hx : lx, fx = r.nx ? lx : hx;
+    float ny = r.ny ? hy : ly, fy = r.ny ? ly : hy;
+    float nz = r.nz ? hz : lz, fz = r.nz ? lz : hz;
+    float t0x = __builtin_fmaf(nx, r.ix, -r.oix), t1x = __builtin_fmaf(fx, r.ix, -r.oix);
+    float t0y = __builtin_fmaf(ny, r.iy, -r.oiy), t1y = __builtin_fmaf(fy, r.iy, -r.oiy);
+    float t0z = __builtin_fmaf(nz, r.iz, -r.oiz), t1z = __builtin_fmaf(fz, r.iz, -r.oiz);
+    float t0 = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, 0.0f));
+    float t1 = fminf(fminf(t1x, t1y), fminf(t1z, tfar));
+    hit = t0 <= t1;
+    return t0;
+}
+
+template <bool MIXED>
+__device__ __forceinline__ bool testPrim(const TraceArgs& a, const float4* rec, const DRay& r, float tmax, float& t,
+                                         uint32_t& id) {
+    float4 p0 = rec[0], p1 = rec[1], p2 = rec[2];
+    id = f2u(p0.w);
+    if (!MIXED) return triTest(r.o, r.d, V{p0.x, p0.y, p0.z}, V{p1.x, p1.y, p1.z}, V{p2.x, p2.y, p2.z}, r.tmin, tmax, t);
+    uint32_t kind = f2u(p1.w);
+    if (kind == KIND_TRI)
+        return triTest(r.o, r.d, V{p0.x, p0.y, p0.z}, V{p1.x, p1.y, p1.z}, V{p2.x, p2.y, p2.z}, r.tmin, tmax, t);
+    if (kind == KIND_SPHERE) return sphereTest(r.o, r.d, V{p0.x, p0.y, p0.z}, p1.x, r.tmin, tmax, t);
+    float4 n = a.quadN[f2u(p2.w)];
+    return quadTest(r.o, r.d, V{p0.x, p0.y, p0.z}, V{p1.x, p1.y, p1.z}, V{p2.x, p2.y, p2.z}, V{n.x, n.y, n.z}, r.tmin,
+                    tmax, t);
+}
+
+template <typename T>
+__device__ __forceinline__ void cswap(float& ta, T& ra, float& tb, T& rb) {
+    bool s = tb < ta;
+    float tt = s ? tb : ta;
+    tb = s ? ta : tb;
+    ta = tt;
+    T rr = s ? rb : ra;
+    rb = s ? ra : rb;
+    ra = rr;
+}
+
+// Two-level stack: LDS rows [depth][lane] for the first kLds entries, private memory
+// beyond (sized for the deepest tree the host accepts).
+template <int BLOCK, int LDS>
+struct Stack {
+    uint2* lds;          // base of this block's LDS stack
+    uint2 spill[kMaxStack - LDS];
+    int sp = 0;
+    __device__ __forceinline__ void push(uint32_t ref, float t) {
+        uint2 e = make_uint2(ref, f2u(t));
+        if (sp < LDS) lds[sp * BLOCK + threadIdx.x] = e;
+        else spill[sp - LDS] = e;
+        ++sp;
+    }
+    __device__ __forceinline__ uint2 pop() {
+        --sp;
+        return sp < LDS ? lds[sp * BLOCK + threadIdx.x] : spill[sp - LDS];
+    }
+};
+
+template <int W, int BLOCK, int LDS, bool ANY, bool MIXED>
+__global__ __launch_bounds__(BLOCK) void traceKernel(TraceArgs a) {
+    __shared__ uint2 ldsStack[LDS * BLOCK];
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const DRay r = loadRay(a.rays, i);
+    float bestT = r.tmax;
+    uint32_t bestId = RBR_MISS_ID;
+    bool occluded = false;
+
+    if (a.nodes) {
+        Stack<BLOCK, LDS> st;
+        st.lds = ldsStack;
+        uint32_t ref = 0;   // root: wide node 0
+        while (true) {
+            if (!isLeafD(ref)) {
+                const float4* nd = a.nodes + size_t(ref) * (W == 4 ? 8 : 16);
+                float tc[W];
+                uint32_t rc[W];
+                if (W == 4) {
+                    float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+                    uint4 ch = *reinterpret_cast<const uint4*>(nd + 6);
+                    const float* LX = &lx.x; const float* HX = &hx.x; const float* LY = &ly.x;
+                    const float* HY = &hy.x; const float* LZ = &lz.x; const float* HZ = &hz.x;
+                    const uint32_t* CH = &ch.x;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        bool h;
+                        float t = slab(r, LX[k], HX[k], LY[k], HY[k], LZ[k], HZ[k], bestT, h);
+                        tc[k] = h ? t : INFINITY;
+                        rc[k] = h ? CH[k] : kEmptyD;
+                    }
+                } else {
+#pragma unroll
+                    for (int half = 0; half < 2; ++half) {
+                        float4 lx = nd[0 + half], hx = nd[2 + half], ly = nd[4 + half], hy = nd[6 + half];
+                        float4 lz = nd[8 + half], hz = nd[10 + half];
+                        uint4 ch = *reinterpret_cast<const uint4*>(nd + 12 + half);
+                        const float* LX = &lx.x; const float* HX = &hx.x; const float* LY = &ly.x;
+                        const float* HY = &hy.x; const float* LZ = &lz.x; const float* HZ = &hz.x;
+                        const uint32_t* CH = &ch.x;
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            bool h;
+                            float t = slab(r, LX[k], HX[k], LY[k], HY[k], LZ[k], HZ[k], bestT, h);
+                            tc[half * 4 + k] = h ? t : INFINITY;
+                            rc[half * 4 + k] = h ? CH[k] : kEmptyD;
+                        }
+                    }
+                }
+                if (!ANY) {
+                    // Sorting network, ascending entry distance (misses sort last as +inf).
+                    if (W == 4) {
+                        cswap(tc[0], rc[0], tc[1], rc[1]); cswap(tc[2], rc[2], tc[3], rc[3]);
+                        cswap(tc[0], rc[0], tc[2], rc[2]); cswap(tc[1], rc[1], tc[3], rc[3]);
+                        cswap(tc[1], rc[1], tc[2], rc[2]);
+                    } else {
+#pragma unroll
+                        for (int p = 0; p < W; ++p)
+#pragma unroll
+                            for (int k = (p & 1); k + 1 < W; k += 2) cswap(tc[k], rc[k], tc[k + 1], rc[k + 1]);
+                    }
+                }
+                int cnt = 0;
+#pragma unroll
+                for (int k = 0; k < W; ++k) cnt += rc[k] != kEmptyD ? 1 : 0;
+                if (cnt > 0) {
+                    if (ANY) {
+                        // Any order: keep the first hit child, push the rest.
+                        uint32_t first = kEmptyD;
+#pragma unroll
+                        for (int k = 0; k < W; ++k) {
+                            if (rc[k] == kEmptyD) continue;
+                            if (first == kEmptyD) first = rc[k];
+                            else st.push(rc[k], tc[k]);
+                        }
+                        ref = first;
+                    } else {
+#pragma unroll
+                        for (int k = W - 1; k >= 1; --k)
+                            if (k < cnt) st.push(rc[k], tc[k]);
+                        ref = rc[0];
+                    }
+                    continue;
+                }
+            } else {
+                const uint32_t s = leafStartD(ref), c = leafCountD(ref);
+                for (uint32_t k = 0; k < c; ++k) {
+                    float t;
+                    uint32_t id;
+                    if (testPrim<MIXED>(a, a.prims + size_t(s + k) * 3, r, bestT, t, id)) {
+                        if (ANY) { occluded = true; break; }
+                        if (t < bestT || id < bestId) { bestT = t; bestId = id; }
+                    }
+                }
+                if (ANY && occluded) break;
+            }
+            // Pop the next entry that can still hold a closer hit.
+            bool found = false;
+            while (st.sp > 0) {
+                uint2 e = st.pop();
+                if (ANY || u2f(e.y) <= bestT) { ref = e.x; found = true; break; }
+            }
+            if (!found) break;
+        }
+    }
+    // Unbounded primitives (infinite planes), tested linearly.
+    for (uint32_t k = 0; k < a.nplanes && !(ANY && occluded); ++k) {
+        float4 pl = a.planes[k];
+        float t;
+        if (planeTest(r.o, r.d, V{pl.x, pl.y, pl.z}, pl.w, r.tmin, bestT, t)) {
+            if (ANY) { occluded = true; break; }
+            uint32_t id = a.planeIds[k];
+            if (t < bestT || id < bestId) { bestT = t; bestId = id; }
+        }
+    }
+    if (ANY) {
+        a.occ[i] = occluded ? 1 : 0;
+    } else {
+        a.hits[i] = make_uint2(bestId, bestId == RBR_MISS_ID ? 0u : f2u(bestT));
+    }
+}
+
+template <int W, int BLOCK, int LDS, bool ANY>
+hipError_t launchT(const TraceArgs& a, hipStream_t s) {
+    dim3 grid((a.n + BLOCK - 1) / BLOCK);
+    if (a.mixed) hipLaunchKernelGGL((traceKernel<W, BLOCK, LDS, ANY, true>), grid, dim3(BLOCK), 0, s, a);
+    else hipLaunchKernelGGL((traceKernel<W, BLOCK, LDS, ANY, false>), grid, dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launchTrace(const TraceArgs& a, int width, bool any, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    if (width == 8) return any ? launchT<8, kBlock, kLdsDepth, true>(a, s) : launchT<8, kBlock, kLdsDepth, false>(a, s);
+    return any ? launchT<4, kBlock, kLdsDepth, true>(a, s) : launchT<4, kBlock, kLdsDepth, false>(a, s);
+}
+
+}  // namespace rbr

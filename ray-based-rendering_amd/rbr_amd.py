@@ -1,0 +1,315 @@
+"""Python host binding of the MI355X path (ctypes over lib/librbr.so, include/rbr.h).
+
+This is the host-side mirror the tests and the benchmark drive.  It mirrors the reference's
+`Accelerator` interface for the traversal path (build / intersect / intersectP /
+worldBounds, include/render/acceleration/accelerator.h:19-34) in batched form, and the
+`Integrator::render` entry for the path tracer.  All compute runs in the HIP library; there is
+no CPU fallback here — if the library is missing, importing this module fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_PKG, "lib")
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("tmin", "<f4"), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("id", "<u4"), ("t", "<f4")])
+MISS = 0xFFFFFFFF
+
+ERRORS = {1: "invalid argument", 2: "HIP error", 3: "bad state", 4: "out of memory", 5: "unsupported"}
+
+
+class RbrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"rbr error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+def _load(name: str) -> C.CDLL:
+    path = os.path.join(LIB_DIR, name)
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run ray-based-rendering_amd/build.py (or __graft_entry__.build())")
+    return C.CDLL(path)
+
+
+_lib = _load("librbr.so")
+_wl = _load("librbr_workloads.so")
+
+_vp = C.c_void_p
+_u32 = C.c_uint32
+_sz = C.c_size_t
+
+
+class BvhStats(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("node_count", C.c_uint32), ("leaf_count", C.c_uint32),
+                ("prim_count", C.c_uint32), ("unbounded", C.c_uint32), ("max_depth", C.c_uint32),
+                ("node_bytes", C.c_uint64), ("prim_bytes", C.c_uint64), ("sah_cost", C.c_double),
+                ("build_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def _sig(fn, args, res=C.c_int):
+    fn.argtypes = args
+    fn.restype = res
+    return fn
+
+
+_sig(_lib.rbr_ctx_create, [C.c_int, C.POINTER(_vp)])
+_sig(_lib.rbr_ctx_destroy, [_vp], None)
+_sig(_lib.rbr_last_error, [_vp], C.c_char_p)
+_sig(_lib.rbr_device_info, [_vp, C.c_char_p, _sz, C.POINTER(C.c_int)])
+_sig(_lib.rbr_device_alloc, [_vp, _sz, C.POINTER(_vp)])
+_sig(_lib.rbr_device_free, [_vp, _vp])
+_sig(_lib.rbr_copy_to_device, [_vp, _vp, _vp, _sz])
+_sig(_lib.rbr_copy_to_host, [_vp, _vp, _vp, _sz])
+_sig(_lib.rbr_synchronize, [_vp])
+_sig(_lib.rbr_scene_create, [_vp, C.POINTER(_vp)])
+_sig(_lib.rbr_scene_destroy, [_vp], None)
+_sig(_lib.rbr_scene_set_geometry, [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp])
+_sig(_lib.rbr_scene_build_bvh, [_vp, C.c_int, C.c_int])
+_sig(_lib.rbr_scene_bvh_stats, [_vp, C.POINTER(BvhStats)])
+_sig(_lib.rbr_scene_world_bounds, [_vp, _vp, _vp])
+_sig(_lib.rbr_trace_closest, [_vp, _vp, _sz, _vp])
+_sig(_lib.rbr_trace_any, [_vp, _vp, _sz, _vp])
+_sig(_lib.rbr_trace_closest_device, [_vp, _vp, _sz, _vp])
+_sig(_lib.rbr_trace_any_device, [_vp, _vp, _sz, _vp])
+_sig(_lib.rbr_time_trace, [_vp, C.c_int, _vp, _sz, _vp, C.c_int, C.POINTER(C.c_float)])
+_sig(_lib.rbr_host_trace_closest, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
+_sig(_lib.rbr_host_trace_any, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
+
+_sig(_wl.rbrw_icosphere, [C.c_int, C.c_float, C.c_float, C.c_uint64, _vp], _sz)
+_sig(_wl.rbrw_noise3, [C.c_float, C.c_float, C.c_float, C.c_uint64], C.c_float)
+_sig(_wl.rbrw_rays_uniform, [_vp, _vp, C.c_float, _sz, C.c_uint64, C.c_float, C.c_float, _vp], None)
+_sig(_wl.rbrw_rays_shadow, [_vp, _vp, _sz, C.c_uint64, C.c_float, _vp], None)
+_sig(_wl.rbrw_u01, [C.c_uint64, C.c_uint64], C.c_float)
+
+
+def lib_path() -> str:
+    return os.path.join(LIB_DIR, "librbr.so")
+
+
+def _ptr(a) -> int | None:
+    if a is None:
+        return None
+    return a.ctypes.data
+
+
+def _f32(a, cols: int) -> np.ndarray | None:
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1, cols)
+    return a
+
+
+def _ids(a) -> np.ndarray | None:
+    if a is None:
+        return None
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def rays_array(o, d, tmin=1e-4, tmax=np.inf) -> np.ndarray:
+    """Packs origins/directions (n,3) into the 32-byte ray layout (directions used as given)."""
+    o = np.asarray(o, np.float32).reshape(-1, 3)
+    d = np.asarray(d, np.float32).reshape(-1, 3)
+    r = np.empty(len(o), RAY_DTYPE)
+    r["o"] = o
+    r["d"] = d
+    r["tmin"] = tmin
+    r["tmax"] = tmax
+    return r
+
+
+class Context:
+    """A HIP device + stream (rbr_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        rc = _lib.rbr_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise RbrError(rc, f"rbr_ctx_create(device={device}) failed")
+        self._h = h
+        self._scenes: list = []
+
+    def check(self, rc: int) -> None:
+        if rc != 0:
+            raise RbrError(rc, _lib.rbr_last_error(self._h).decode())
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_info(self) -> tuple[str, int]:
+        buf = C.create_string_buffer(256)
+        cu = C.c_int(0)
+        self.check(_lib.rbr_device_info(self._h, buf, 256, C.byref(cu)))
+        return buf.value.decode(), cu.value
+
+    def alloc(self, nbytes: int) -> "DeviceBuffer":
+        return DeviceBuffer(self, nbytes)
+
+    def synchronize(self) -> None:
+        self.check(_lib.rbr_synchronize(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            for s in self._scenes:
+                s.close()
+            _lib.rbr_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """A device allocation owned by a context (inputs kept resident in HBM)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = _vp()
+        ctx.check(_lib.rbr_device_alloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr: np.ndarray) -> None:
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.ctx.check(_lib.rbr_copy_to_device(self.ctx.handle, self.ptr, arr.ctypes.data, arr.nbytes))
+
+    def download(self, out: np.ndarray) -> np.ndarray:
+        assert out.flags.c_contiguous and out.nbytes <= self.nbytes
+        self.ctx.check(_lib.rbr_copy_to_host(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            self.ctx.check(_lib.rbr_device_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+
+class Scene:
+    """Geometry + wide BVH on the device (rbr_scene): the batched Accelerator."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = _vp()
+        ctx.check(_lib.rbr_scene_create(ctx.handle, C.byref(h)))
+        self._h = h
+        ctx._scenes.append(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.rbr_scene_destroy(self._h)
+            self._h = None
+
+    def set_geometry(self, tris=None, tri_ids=None, spheres=None, sphere_ids=None, quads=None, quad_ids=None,
+                     planes=None, plane_ids=None) -> None:
+        t, s, q, p = _f32(tris, 9), _f32(spheres, 4), _f32(quads, 12), _f32(planes, 4)
+        ti, si, qi, pi = _ids(tri_ids), _ids(sphere_ids), _ids(quad_ids), _ids(plane_ids)
+        self._keep = (t, s, q, p, ti, si, qi, pi)
+        n = lambda a: 0 if a is None else len(a)
+        self.ctx.check(_lib.rbr_scene_set_geometry(self._h, _ptr(t), n(t), _ptr(ti), _ptr(s), n(s), _ptr(si),
+                                                   _ptr(q), n(q), _ptr(qi), _ptr(p), n(p), _ptr(pi)))
+
+    def build_bvh(self, width: int = 4, leaf_max: int = 4) -> None:
+        self.ctx.check(_lib.rbr_scene_build_bvh(self._h, width, leaf_max))
+
+    def stats(self) -> dict:
+        st = BvhStats()
+        self.ctx.check(_lib.rbr_scene_bvh_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def world_bounds(self) -> tuple[np.ndarray, np.ndarray]:
+        lo = np.zeros(3, np.float32)
+        hi = np.zeros(3, np.float32)
+        self.ctx.check(_lib.rbr_scene_world_bounds(self._h, lo.ctypes.data, hi.ctypes.data))
+        return lo, hi
+
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.empty(len(rays), HIT_DTYPE)
+        self.ctx.check(_lib.rbr_trace_closest(self._h, rays.ctypes.data, len(rays), hits.ctypes.data))
+        return hits
+
+    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        occ = np.empty(len(rays), np.uint8)
+        self.ctx.check(_lib.rbr_trace_any(self._h, rays.ctypes.data, len(rays), occ.ctypes.data))
+        return occ
+
+    def trace_closest_device(self, d_rays: DeviceBuffer, n: int, d_hits: DeviceBuffer) -> None:
+        self.ctx.check(_lib.rbr_trace_closest_device(self._h, d_rays.ptr, n, d_hits.ptr))
+
+    def trace_any_device(self, d_rays: DeviceBuffer, n: int, d_occ: DeviceBuffer) -> None:
+        self.ctx.check(_lib.rbr_trace_any_device(self._h, d_rays.ptr, n, d_occ.ptr))
+
+    def time_trace(self, kind: int, d_rays: DeviceBuffer, n: int, d_out: DeviceBuffer, iters: int) -> float:
+        ms = C.c_float(0.0)
+        self.ctx.check(_lib.rbr_time_trace(self._h, kind, d_rays.ptr, n, d_out.ptr, iters, C.byref(ms)))
+        return ms.value
+
+    def host_trace_closest(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.empty(len(rays), HIT_DTYPE)
+        nv, pt = C.c_uint64(0), C.c_uint64(0)
+        self.ctx.check(_lib.rbr_host_trace_closest(self._h, rays.ctypes.data, len(rays), hits.ctypes.data,
+                                                   C.byref(nv), C.byref(pt)))
+        return hits, nv.value, pt.value
+
+    def host_trace_any(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        occ = np.empty(len(rays), np.uint8)
+        nv, pt = C.c_uint64(0), C.c_uint64(0)
+        self.ctx.check(_lib.rbr_host_trace_any(self._h, rays.ctypes.data, len(rays), occ.ctypes.data,
+                                               C.byref(nv), C.byref(pt)))
+        return occ, nv.value, pt.value
+
+
+# ---------------------------------------------------------------- workloads
+
+def icosphere(n: int = 224, radius: float = 100.0, amp: float = 0.1, seed: int = 0xC1) -> np.ndarray:
+    """Displaced geodesic icosphere, 20*n*n triangles, (T, 9) float32."""
+    count = _wl.rbrw_icosphere(n, radius, amp, seed, None)
+    out = np.empty((count, 9), np.float32)
+    _wl.rbrw_icosphere(n, radius, amp, seed, out.ctypes.data)
+    return out
+
+
+def rays_uniform(lo, hi, n: int, seed: int, scale: float = 1.1, tmin: float = 1e-4, tmax: float = np.inf) -> np.ndarray:
+    lo = np.ascontiguousarray(lo, np.float32)
+    hi = np.ascontiguousarray(hi, np.float32)
+    r = np.empty(n, RAY_DTYPE)
+    _wl.rbrw_rays_uniform(lo.ctypes.data, hi.ctypes.data, scale, n, seed, tmin, tmax, r.ctypes.data)
+    return r
+
+
+def rays_shadow(lo, hi, n: int, seed: int, tmin: float = 1e-4) -> np.ndarray:
+    lo = np.ascontiguousarray(lo, np.float32)
+    hi = np.ascontiguousarray(hi, np.float32)
+    r = np.empty(n, RAY_DTYPE)
+    _wl.rbrw_rays_shadow(lo.ctypes.data, hi.ctypes.data, n, seed, tmin, r.ctypes.data)
+    return r
+
+
+def mesh_bounds(tris: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    v = tris.reshape(-1, 3)
+    return v.min(0).astype(np.float32), v.max(0).astype(np.float32)

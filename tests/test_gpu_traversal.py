@@ -1,0 +1,145 @@
+"""GPU parity of the traversal kernels through the C ABI: closest-hit ids and t must be
+bit-identical to the oracle's lowest-id brute force / BVH, any-hit flags identical."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+import rbr_amd as R
+import worlds
+
+pytestmark = pytest.mark.gpu
+GOLDEN = np.load(os.path.join(os.path.dirname(__file__), "golden", "traversal_small.npz"))
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = R.Context(0)
+    yield c
+    c.close()
+
+
+def gpu_scene(ctx, width=4, leaf=4, **kw):
+    sc = R.Scene(ctx)
+    sc.set_geometry(**kw)
+    sc.build_bvh(width, leaf)
+    return sc
+
+
+def assert_same(h, ref):
+    bad = np.nonzero((h["id"] != ref["id"]) | (h["t"] != ref["t"]))[0]
+    assert len(bad) == 0, f"{len(bad)} mismatches, first {bad[:5]}: {h[bad[:5]]} vs {ref[bad[:5]]}"
+
+
+def test_empty_and_single_sphere(ctx):
+    say("case: empty + single sphere")
+    sc = gpu_scene(ctx)
+    r = R.rays_array([[0, 0, 5]], [[0, 0, -1]])
+    assert sc.trace_closest(r)["id"][0] == R.MISS
+    assert sc.trace_any(r)[0] == 0
+    sc = gpu_scene(ctx, spheres=np.array([[0, 0, 0, 1]], np.float32))
+    h = sc.trace_closest(r)
+    assert h["id"][0] == 0 and h["t"][0] == np.float32(4.0)
+    assert sc.trace_any(r)[0] == 1
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_golden_fixture(ctx, width):
+    say(f"case: golden fixture width {width}")
+    rays = GOLDEN["rays"].view(worlds.RAY_DTYPE).reshape(-1)
+    sc = gpu_scene(ctx, width, 4, tris=GOLDEN["tris"], spheres=GOLDEN["spheres"], planes=GOLDEN["planes"])
+    h = sc.trace_closest(rays)
+    ref = O.OracleScene(tris=GOLDEN["tris"], spheres=GOLDEN["spheres"], planes=GOLDEN["planes"]).closest(rays, brute=True)
+    assert_same(h, ref)
+    # Against the reference's own answers: identical t, ids identical up to exact ties.
+    rid, rt = GOLDEN["ref_brute_id"], GOLDEN["ref_brute_t"]
+    hit = rid != R.MISS
+    assert np.array_equal(h["id"] == R.MISS, ~hit) and np.array_equal(h["t"][hit], rt[hit])
+    assert np.array_equal(sc.trace_any(rays), GOLDEN["ref_any"])
+    mrays = GOLDEN["mesh_rays"].view(worlds.RAY_DTYPE).reshape(-1)
+    ms = gpu_scene(ctx, width, 4, tris=GOLDEN["mesh"])
+    mh = ms.trace_closest(mrays)
+    assert np.array_equal(mh["t"], np.where(GOLDEN["mesh_ref_id"] == R.MISS, np.float32(0), GOLDEN["mesh_ref_t"]))
+
+
+@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("cfg", [(30, 30, False, 3000), (400, 1500, False, 6000), (60, 200, True, 3000)])
+def test_reference_worlds(ctx, width, cfg):
+    ns, nt, plane, nr = cfg
+    say(f"case: reference world {cfg} width {width}")
+    rng = np.random.default_rng(4242 + ns)
+    w = worlds.make_world(rng, ns, nt, plane)
+    rays = worlds.make_rays(rng, nr)
+    kw = worlds.geometry_kwargs(w)
+    o = O.OracleScene(**kw)
+    sc = gpu_scene(ctx, width, 4, **kw)
+    assert_same(sc.trace_closest(rays), o.closest(rays, brute=True))
+    assert np.array_equal(sc.trace_any(rays), o.any(rays, brute=True))
+
+
+def test_rebuild_replaces_contents(ctx):
+    say("case: rebuild")
+    rng = np.random.default_rng(1)
+    big = worlds.make_world(rng, 400, 1500, False)
+    small = worlds.make_world(rng, 30, 30, False)
+    sc = gpu_scene(ctx, **worlds.geometry_kwargs(big))
+    sc.set_geometry(**worlds.geometry_kwargs(small))
+    sc.build_bvh()
+    r = R.rays_array([[0, 0, 50]], [[0, 0, -1]])
+    assert_same(sc.trace_closest(r), O.OracleScene(**worlds.geometry_kwargs(small)).closest(r, brute=True))
+
+
+def test_leaf_sizes_and_ties(ctx):
+    say("case: leaf sizes and ties")
+    tri = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32)
+    tris = np.concatenate([tri] * 9)
+    ids = np.array([40, 12, 33, 7, 90, 8, 11, 70, 9], np.uint32)
+    r = R.rays_array([[0.2, 0.2, 1.0]], [[0, 0, -1]])
+    for leaf in range(1, 9):
+        for width in (4, 8):
+            h = gpu_scene(ctx, width, leaf, tris=tris, tri_ids=ids).trace_closest(r)
+            assert h["id"][0] == 7 and h["t"][0] == np.float32(1.0)
+
+
+def test_device_pointer_path_matches_host_path(ctx):
+    say("case: device pointers")
+    tris = R.icosphere(32, 100.0, 0.1, 0xC1)
+    lo, hi = R.mesh_bounds(tris)
+    rays = R.rays_uniform(lo, hi, 50000, 11)
+    sc = gpu_scene(ctx, tris=tris)
+    d_rays = ctx.alloc(rays.nbytes)
+    d_hits = ctx.alloc(len(rays) * 8)
+    d_rays.upload(rays)
+    sc.trace_closest_device(d_rays, len(rays), d_hits)
+    out = np.empty(len(rays), R.HIT_DTYPE)
+    d_hits.download(out)
+    assert_same(out, sc.trace_closest(rays))
+    d_rays.free()
+    d_hits.free()
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_c1_full_size(ctx, width):
+    """~1.0M-triangle displaced icosphere (20*224^2) and 2^22 rays: bit-exact against the
+    oracle BVH (itself equal to brute force), plus the 2^22 shadow batch."""
+    say(f"case: C1 full size width {width}")
+    tris = R.icosphere(224, 100.0, 0.1, 0xC1)
+    lo, hi = R.mesh_bounds(tris)
+    rays = R.rays_uniform(lo, hi, 1 << 22, 0xC1 + 1)
+    shadow = R.rays_shadow(lo, hi, 1 << 22, 0xC1 + 2)
+    sc = gpu_scene(ctx, width, 4, tris=tris)
+    h = sc.trace_closest(rays)
+    occ = sc.trace_any(shadow)
+    say("  gpu done; oracle")
+    o = O.OracleScene(tris=tris)
+    o.build_bvh()
+    assert_same(h, o.closest(rays))
+    assert np.array_equal(occ, o.any(shadow))
+    assert 0.3 < (h["id"] != R.MISS).mean() < 1.0

@@ -1,0 +1,146 @@
+"""Pins the CPU oracle's traversal (oracle/orc_geometry.cpp) before it is trusted:
+known answers of the reference's tests, closed-form Moller-Trumbore cases, the reference's
+own compiled code (golden fixture + live libref.so), and brute force == BVH."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+import worlds
+
+MISS = 0xFFFFFFFF
+GOLDEN = np.load(__file__.replace("test_oracle_traversal.py", "golden/traversal_small.npz"))
+
+
+def ray(o, d, tmin=1e-4, tmax=np.inf):
+    r = np.empty(1, worlds.RAY_DTYPE)
+    r["o"], r["d"], r["tmin"], r["tmax"] = o, d, tmin, tmax
+    return r
+
+
+def test_empty_scene_misses():
+    # test_accelerators.cpp:121-128
+    s = O.OracleScene()
+    s.build_bvh()
+    r = ray((0, 0, 5), (0, 0, -1))
+    assert s.closest(r)["id"][0] == MISS
+    assert s.any(r)[0] == 0
+
+
+def test_single_sphere_t4():
+    # test_accelerators.cpp:130-136: unit sphere at the origin, ray from z=5 -> t = 4
+    s = O.OracleScene(spheres=np.array([[0, 0, 0, 1]], np.float32))
+    s.build_bvh()
+    h = s.closest(ray((0, 0, 5), (0, 0, -1)))
+    assert h["id"][0] == 0 and h["t"][0] == np.float32(4.0)
+
+
+def test_moller_trumbore_closed_forms():
+    tri = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0], np.float32)  # z = 0 plane, unit right triangle
+    # Straight down onto an interior point: t = distance.
+    assert O.triangle_t(tri, ray((0.25, 0.25, 3.0), (0, 0, -1))[0]) == np.float32(3.0)
+    # Outside the triangle (u + v > 1).
+    assert O.triangle_t(tri, ray((0.75, 0.75, 3.0), (0, 0, -1))[0]) is None
+    # Edge u + v == 1 exactly is inside (triangle.cpp:21 rejects only u + v > 1).
+    assert O.triangle_t(tri, ray((0.5, 0.5, 3.0), (0, 0, -1))[0]) == np.float32(3.0)
+    # Vertex hit.
+    assert O.triangle_t(tri, ray((0.0, 0.0, 2.0), (0, 0, -1))[0]) == np.float32(2.0)
+    # Parallel ray: |a| = 0 < EPSILON.
+    assert O.triangle_t(tri, ray((0.25, 0.25, 1.0), (1, 0, 0))[0]) is None
+    # t window is inclusive on both ends (triangle.cpp:24 rejects t < EPSILON or t > tMax).
+    assert O.triangle_t(tri, ray((0.25, 0.25, 3.0), (0, 0, -1), tmax=3.0)[0]) == np.float32(3.0)
+    assert O.triangle_t(tri, ray((0.25, 0.25, 3.0), (0, 0, -1), tmax=2.999)[0]) is None
+    assert O.triangle_t(tri, ray((0.25, 0.25, 1e-4), (0, 0, -1))[0]) == np.float32(1e-4)
+    assert O.triangle_t(tri, ray((0.25, 0.25, 5e-5), (0, 0, -1))[0]) is None
+    # Behind the origin.
+    assert O.triangle_t(tri, ray((0.25, 0.25, -1.0), (0, 0, -1))[0]) is None
+
+
+def test_determinant_cut_is_absolute():
+    # a = e1 . (d x e2) equals twice the triangle area for a perpendicular unit ray; the
+    # reference rejects |a| < 1e-4 in absolute units (triangle.cpp:12).
+    # For legs s in the z = 0 plane and d = (0, 0, -1): h = (s, 0, 0), a = s*s.
+    thr = np.float32(1e-4)
+    for scale in (0.005, 0.0099, 0.01, np.nextafter(np.float32(0.01), np.float32(1)), 0.02):
+        s = np.float32(scale)
+        tri = np.array([0, 0, 0, s, 0, 0, 0, s, 0], np.float32)
+        a = np.float32(s * s)
+        t = O.triangle_t(tri, ray((s / 4, s / 4, 1.0), (0, 0, -1))[0])
+        assert (t is not None) == bool(a >= thr), (scale, a, t)
+
+
+def test_oracle_matches_reference_fixture():
+    """Oracle brute force vs the reference's BVH and brute-force loop (fixture written from
+    the reference's own compiled code).  t must be bit-identical; ids identical except where
+    two primitives give the same t (the reference keeps the last, the oracle the lowest)."""
+    rays = GOLDEN["rays"].view(worlds.RAY_DTYPE).reshape(-1)
+    s = O.OracleScene(tris=GOLDEN["tris"], spheres=GOLDEN["spheres"], planes=GOLDEN["planes"])
+    h = s.closest(rays, brute=True)
+    for key in ("ref_bvh", "ref_brute"):
+        rid, rt = GOLDEN[key + "_id"], GOLDEN[key + "_t"]
+        assert np.array_equal(h["id"] == MISS, rid == MISS), key
+        hit = rid != MISS
+        assert np.array_equal(h["t"][hit], rt[hit]), key
+        diff = np.nonzero(h["id"] != rid)[0]
+        assert len(diff) <= 4, key  # only genuine ties
+        for i in diff:
+            assert h["id"][i] < rid[i]
+    assert np.array_equal(s.any(rays, brute=True), GOLDEN["ref_any"])
+
+
+def test_oracle_matches_reference_mesh_fixture():
+    rays = GOLDEN["mesh_rays"].view(worlds.RAY_DTYPE).reshape(-1)
+    s = O.OracleScene(tris=GOLDEN["mesh"])
+    s.build_bvh()
+    h = s.closest(rays)
+    rid, rt = GOLDEN["mesh_ref_id"], GOLDEN["mesh_ref_t"]
+    assert np.array_equal(h["t"], np.where(rid == MISS, np.float32(0), rt))
+    tie = h["id"] != rid
+    assert tie.sum() <= 2
+
+
+@pytest.mark.parametrize("cfg", [(30, 30, False, 3000), (400, 1500, False, 6000), (60, 200, True, 3000)])
+def test_oracle_bvh_equals_brute(cfg):
+    ns, nt, plane, nr = cfg
+    rng = np.random.default_rng(12345 + ns)
+    w = worlds.make_world(rng, ns, nt, plane)
+    rays = worlds.make_rays(rng, nr)
+    s = O.OracleScene(**worlds.geometry_kwargs(w))
+    b = s.closest(rays, brute=True)
+    s.build_bvh()
+    v = s.closest(rays)
+    assert np.array_equal(b["id"], v["id"]) and np.array_equal(b["t"], v["t"])
+    assert np.array_equal(s.any(rays, brute=True), s.any(rays))
+    assert (b["id"] != MISS).sum() > nr // 20  # test_accelerators.cpp:108
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref/libref.so not built")
+def test_oracle_matches_live_reference():
+    rng = np.random.default_rng(777)
+    w = worlds.make_world(rng, 400, 1500, True)
+    rays = worlds.make_rays(rng, 6000)
+    ref = O.RefWorld(tris=w["tris"], spheres=w["spheres"], planes=w["planes"])
+    rb = ref.brute_closest(rays)
+    s = O.OracleScene(tris=w["tris"], spheres=w["spheres"], planes=w["planes"])
+    h = s.closest(rays, brute=True)
+    assert np.array_equal(h["t"], rb["t"])
+    ties = np.nonzero(h["id"] != rb["id"])[0]
+    assert all(h["id"][i] < rb["id"][i] for i in ties)
+    assert np.array_equal(s.any(rays, brute=True), ref.brute_any(rays))
+    # The reference BVH agrees with its own brute force to the reference test's tolerance
+    # (test_accelerators.cpp:102).
+    ref.build_bvh()
+    rv, _ = ref.bvh_closest(rays)
+    assert np.array_equal(rv["id"] == MISS, rb["id"] == MISS)
+    hit = rb["id"] != MISS
+    assert np.all(np.abs(rv["t"][hit] - rb["t"][hit]) <= 1e-3 * np.maximum(1.0, rb["t"][hit]))
+
+
+def test_tie_rule_lowest_id():
+    # Two coincident triangles: every hit is a tie; the lowest id must win whatever the order.
+    tri = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32)
+    s = O.OracleScene(tris=np.concatenate([tri, tri, tri]), tri_ids=np.array([7, 3, 5], np.uint32))
+    s.build_bvh()
+    h = s.closest(ray((0.2, 0.2, 1.0), (0, 0, -1)))
+    assert h["id"][0] == 3 and h["t"][0] == np.float32(1.0)
