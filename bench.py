@@ -73,6 +73,17 @@ def cpu_baseline(tris, rays, gpu_hits, seconds: float) -> dict:
     }
 
 
+def measured_traffic(width: int):
+    """HBM bytes per closest-hit launch from the committed rocprofv3 --pmc summary of this
+    workload (FETCH_SIZE + WRITE_SIZE, scaled by the same counters' reading for a copy kernel
+    of known size), or None when no such summary exists for this BVH width."""
+    path = os.path.join(ROOT, "profiles", f"traversal_closest_c1_bvh{width}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes_per_launch")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +93,8 @@ def main() -> int:
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 256 MiB streaming copy kernel 3x (byte-counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {args.gpus} is not supported "
@@ -127,6 +140,14 @@ def main() -> int:
 
     hits = np.empty(N_RAYS, R.HIT_DTYPE)
     d_hits.download(hits)
+    if args.calib_copy:
+        nb = 256 << 20
+        src, dst = ctx.alloc(nb), ctx.alloc(nb)
+        for _ in range(3):
+            ctx.check(R._lib.rbr_calib_copy(ctx.handle, dst.ptr, src.ptr, nb))
+        ctx.synchronize()
+        src.free()
+        dst.free()
 
     # Algorithmic bytes per ray from the host traversal of the same layout on a sample.
     sample = rays[:: max(1, N_RAYS // (1 << 16))]
@@ -164,7 +185,7 @@ def main() -> int:
         "bvh_build_ms": st["build_ms"],
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.width),
             "bytes_per_ray": bytes_per_ray, "nodes_per_ray": nodes_per_ray, "prims_per_ray": prims_per_ray,
             "frac_compulsory_only": rays_per_s * bytes_compulsory / 1e9 / HBM_PEAK_GBS,
         },

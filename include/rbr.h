@@ -159,6 +159,10 @@ int rbr_device_free(rbr_ctx* ctx, void* dptr);
 int rbr_copy_to_device(rbr_ctx* ctx, void* dst, const void* src, size_t bytes);
 int rbr_copy_to_host(rbr_ctx* ctx, void* dst, const void* src, size_t bytes);
 int rbr_synchronize(rbr_ctx* ctx);
+/* Profiling helper with no counterpart in the reference: a plain streaming copy kernel of
+ * `bytes` (multiple of 16) on the context stream, used to calibrate hardware byte counters
+ * (FETCH_SIZE / WRITE_SIZE) against a kernel whose traffic is known. */
+int rbr_calib_copy(rbr_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 
 /* ---------------------------------------------------------------- geometry + BVH
  * Replaces Accelerator::build (accelerator.h:25) for the BVH (bvh.cpp:10-31).  The
@@ -196,6 +200,12 @@ int rbr_trace_any_device(rbr_scene* scene, const rbr_ray* d_rays, size_t n, uint
  * and returns the mean kernel time measured with HIP events on the launch stream. */
 int rbr_time_trace(rbr_scene* scene, int kind, const rbr_ray* d_rays, size_t n, void* d_out,
                    int iters, float* mean_ms);
+
+/* Traversal launch tuning (the autotuner's knobs): mode 0 = persistent waves that fetch
+ * 64-ray batches from a device counter (default), 1 = one ray per lane; lds_depth = LDS
+ * stack entries per lane (8, 12, 16, 24, 32; deeper entries spill to a per-wave global
+ * buffer); waves_per_cu = persistent waves launched per compute unit (1..32). */
+int rbr_set_trace_tuning(rbr_ctx* ctx, int mode, int lds_depth, int waves_per_cu);
 
 /* Per-ray host traversal of the same flat wide BVH with the same acceptance rule — the path
  * the Accelerator adapter's per-ray intersect()/intersectP() take, since those are called

@@ -1,0 +1,110 @@
+"""Traversal autotuner: sweeps BVH width, leaf size, launch mode, LDS stack depth and
+persistent waves per CU on a resident ray batch, checks that every configuration returns
+bit-identical hits, and records the fastest in a JSON cache keyed by device and workload.
+
+  python ray-based-rendering_amd/autotune.py [--quick] [--out PATH]
+
+The cache is read by `best_config()`; bench.py and the render path use it when present.
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, PKG)
+import rbr_amd as R  # noqa: E402
+
+CACHE = os.path.join(PKG, "tuning", "traversal.json")
+
+
+def sweep(ctx: R.Context, tris: np.ndarray, rays: np.ndarray, kind: int, widths, leaves, modes, depths, waves,
+          iters: int = 10, log=print) -> list[dict]:
+    n = len(rays)
+    d_rays = ctx.alloc(rays.nbytes)
+    d_out = ctx.alloc(n * (8 if kind == 0 else 1))
+    d_rays.upload(rays)
+    results = []
+    ref = None
+    for width, leaf in itertools.product(widths, leaves):
+        sc = R.Scene(ctx)
+        sc.set_geometry(tris=tris)
+        sc.build_bvh(width, leaf)
+        for mode, depth, wpc in itertools.product(modes, depths, waves):
+            if mode == 1 and wpc != waves[0]:
+                continue  # waves per CU only matters for the persistent kernel
+            ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, depth, wpc))
+            sc.time_trace(kind, d_rays, n, d_out, 2)
+            ms = sc.time_trace(kind, d_rays, n, d_out, iters)
+            out = np.empty(n, R.HIT_DTYPE if kind == 0 else np.uint8)
+            d_out.download(out)
+            same = True
+            if ref is None:
+                ref = out.copy()
+            else:
+                same = bool(np.array_equal(out, ref))
+            rec = {"width": width, "leaf": leaf, "mode": mode, "lds_depth": depth, "waves_per_cu": wpc,
+                   "ms": ms, "mrays_s": n / ms / 1e3, "identical": same}
+            log(json.dumps(rec))
+            if not same:
+                raise RuntimeError(f"configuration {rec} changed the results")
+            results.append(rec)
+        sc.close()
+    ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
+    d_rays.free()
+    d_out.free()
+    return results
+
+
+def best_config(kind: str = "closest", path: str = CACHE) -> dict | None:
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get(kind)
+
+
+def apply(ctx: R.Context, cfg: dict | None) -> None:
+    if cfg:
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, cfg["mode"], cfg["lds_depth"], cfg["waves_per_cu"]))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--out", default=CACHE)
+    ap.add_argument("--n-sub", type=int, default=224)
+    ap.add_argument("--rays", type=int, default=1 << 22)
+    args = ap.parse_args()
+    tris = R.icosphere(args.n_sub, 100.0, 0.1, 0xC1)
+    lo, hi = R.mesh_bounds(tris)
+    rays = R.rays_uniform(lo, hi, args.rays, 0xC1 + 1)
+    shadow = R.rays_shadow(lo, hi, args.rays, 0xC1 + 2)
+    ctx = R.Context(0)
+    name, cus = ctx.device_info()
+    if args.quick:
+        grid = dict(widths=[4, 8], leaves=[4], modes=[0, 1], depths=[8, 16], waves=[8, 16, 32])
+    else:
+        grid = dict(widths=[4, 8], leaves=[2, 4, 8], modes=[0, 1], depths=[8, 12, 16, 24, 32], waves=[4, 8, 16, 24, 32])
+    out = {"device": name, "cus": cus, "workload": f"icosphere n={args.n_sub} ({len(tris)} tris), {args.rays} rays",
+           "time": time.strftime("%Y-%m-%dT%H:%M:%S")}
+    for kind, r in (("closest", rays), ("any", shadow)):
+        res = sweep(ctx, tris, r, 0 if kind == "closest" else 1, **grid, log=lambda s: print(s, flush=True))
+        best = min(res, key=lambda x: x["ms"])
+        out[kind] = best
+        out[kind + "_all"] = res
+    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    with open(args.out, "w") as f:
+        json.dump(out, f, indent=1)
+    print("best closest:", out["closest"], "\nbest any:", out["any"])
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

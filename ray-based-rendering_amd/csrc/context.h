@@ -29,6 +29,12 @@ struct rbr_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     rbr::RenderState* render = nullptr;
+    // Traversal tuning (rbr_set_trace_tuning) and persistent-launch state.
+    int traceMode = 0;                 // 0 persistent waves, 1 one ray per lane
+    int ldsDepth = rbr::kLdsDepthDefault;
+    int wavesPerCu = rbr::kWavesPerCuDefault;
+    rbr::DevBuf dCounter, dSpill;
+    unsigned long long counterBase = 0;
 };
 
 struct rbr_scene {
@@ -57,6 +63,9 @@ void freeBuffer(DevBuf& b);
 int uploadBuffer(rbr_ctx* ctx, DevBuf& b, const void* src, size_t bytes);
 int useDevice(rbr_ctx* ctx);
 TraceArgs sceneTraceArgs(const rbr_scene* s);
+// Launches closest (any=false) or any-hit traversal of `a` (rays/outputs filled in) for a
+// scene whose wide BVH has the given width and stack need, using the context's tuning.
+int launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack, bool any);
 
 void freeShading(rbr_scene* s);          // render.hip
 void destroyRenderState(rbr_ctx* ctx);   // render.hip

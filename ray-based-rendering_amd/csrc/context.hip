@@ -1,6 +1,7 @@
 // C ABI (include/rbr.h): contexts, scenes, geometry upload, BVH build and traversal.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -88,6 +89,32 @@ static TraceArgs traceArgs(const rbr_scene* s) {
 
 TraceArgs rbr::sceneTraceArgs(const rbr_scene* s) { return traceArgs(s); }
 
+int rbr::launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack, bool any) {
+    if (a.n == 0) return RBR_OK;
+    hipError_t e;
+    if (ctx->traceMode == 1) {
+        if (maxStack > uint32_t(kMaxStack)) return fail(ctx, RBR_ERR_UNSUPPORTED, "BVH too deep for the lane stack");
+        e = launchTrace(a, width, any, ctx->ldsDepth, nullptr, 0, ctx->stream);
+        return e == hipSuccess ? RBR_OK : failHip(ctx, e, "trace launch");
+    }
+    const unsigned grid = unsigned(std::max(1, ctx->cuCount) * ctx->wavesPerCu);
+    const uint32_t spillDepth = std::max<uint32_t>(1u, maxStack > uint32_t(ctx->ldsDepth) ? maxStack - ctx->ldsDepth : 1u);
+    if (!ctx->dCounter.ptr) {
+        if (int rc = ensureBuffer(ctx, ctx->dCounter, sizeof(unsigned long long))) return rc;
+        e = hipMemsetAsync(ctx->dCounter.ptr, 0, sizeof(unsigned long long), ctx->stream);
+        if (e != hipSuccess) return failHip(ctx, e, "counter init");
+        ctx->counterBase = 0;
+    }
+    if (int rc = ensureBuffer(ctx, ctx->dSpill, size_t(grid) * spillDepth * 64 * sizeof(uint2))) return rc;
+    PersistArgs p{static_cast<unsigned long long*>(ctx->dCounter.ptr), ctx->counterBase,
+                  static_cast<uint2*>(ctx->dSpill.ptr), spillDepth};
+    e = launchTrace(a, width, any, ctx->ldsDepth, &p, grid, ctx->stream);
+    if (e != hipSuccess) return failHip(ctx, e, "trace launch");
+    // Every batch takes one fetch and every wave one final, failing fetch.
+    ctx->counterBase += 64ull * ((uint64_t(a.n) + 63) / 64 + grid);
+    return RBR_OK;
+}
+
 // ---------------------------------------------------------------- context
 
 extern "C" {
@@ -130,6 +157,8 @@ void rbr_ctx_destroy(rbr_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     destroyRenderState(ctx);
+    freeBuffer(ctx->dCounter);
+    freeBuffer(ctx->dSpill);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     (void)hipStreamDestroy(ctx->stream);
@@ -184,6 +213,35 @@ int rbr_synchronize(rbr_ctx* ctx) {
     if (int rc = useDevice(ctx)) return rc;
     hipError_t e = hipStreamSynchronize(ctx->stream);
     return e == hipSuccess ? RBR_OK : failHip(ctx, e, "hipStreamSynchronize");
+}
+
+__global__ __launch_bounds__(256) void calibCopyKernel(float4* __restrict__ dst, const float4* __restrict__ src,
+                                                       size_t n) {
+    for (size_t i = blockIdx.x * size_t(256) + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) dst[i] = src[i];
+}
+
+int rbr_calib_copy(rbr_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || !dst || !src || bytes % 16) return RBR_ERR_INVALID;
+    if (int rc = useDevice(ctx)) return rc;
+    const size_t n = bytes / 16;
+    const unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, size_t(ctx->cuCount) * 16));
+    if (grid == 0) return RBR_OK;
+    hipLaunchKernelGGL(calibCopyKernel, dim3(grid), dim3(256), 0, ctx->stream, static_cast<float4*>(dst),
+                       static_cast<const float4*>(src), n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "calib copy launch");
+}
+
+int rbr_set_trace_tuning(rbr_ctx* ctx, int mode, int lds_depth, int waves_per_cu) {
+    if (!ctx) return RBR_ERR_INVALID;
+    if (mode != 0 && mode != 1) return fail(ctx, RBR_ERR_INVALID, "mode must be 0 (persistent) or 1 (lane)");
+    if (lds_depth != 8 && lds_depth != 12 && lds_depth != 16 && lds_depth != 24 && lds_depth != 32)
+        return fail(ctx, RBR_ERR_INVALID, "lds_depth must be 8, 12, 16, 24 or 32");
+    if (waves_per_cu < 1 || waves_per_cu > 32) return fail(ctx, RBR_ERR_INVALID, "waves_per_cu must be in 1..32");
+    ctx->traceMode = mode;
+    ctx->ldsDepth = lds_depth;
+    ctx->wavesPerCu = waves_per_cu;
+    return RBR_OK;
 }
 
 // ---------------------------------------------------------------- scenes
@@ -300,8 +358,8 @@ static int traceDevice(rbr_scene* s, const rbr_ray* dRays, size_t n, void* dOut,
     a.n = uint32_t(n);
     a.hits = any ? nullptr : static_cast<uint2*>(dOut);
     a.occ = any ? static_cast<uint8_t*>(dOut) : nullptr;
-    hipError_t e = launchTrace(a, s->bvh.width, any, ctx->stream);
-    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "trace launch");
+    const uint32_t maxStack = s->bvh.empty() ? 1u : uint32_t(s->bvh.width - 1) * s->bvh.maxDepth + 1u;
+    return launchTraversal(ctx, a, s->bvh.width, maxStack, any);
 }
 
 static int traceHost(rbr_scene* s, const rbr_ray* rays, size_t n, void* out, bool any) {

@@ -9,8 +9,8 @@ namespace rbr {
 
 constexpr uint32_t RBR_MISS_ID = 0xFFFFFFFFu;
 constexpr uint32_t kEmptyD = 0xFFFFFFFFu;
-constexpr int kBlock = 64;        // one wave per workgroup
-constexpr int kLdsDepth = 16;     // LDS stack entries per lane (8 B each)
+constexpr int kLdsDepthDefault = 8;    // LDS stack entries per lane (8 B each)
+constexpr int kWavesPerCuDefault = 32;  // persistent waves per compute unit
 constexpr int kMaxStack = 192;    // total stack entries per lane (LDS + private spill)
 
 __host__ __device__ __forceinline__ bool isLeafD(uint32_t c) { return (c & 0x80000000u) != 0u; }
@@ -31,6 +31,17 @@ struct TraceArgs {
     uint8_t* occ;             // any
 };
 
-hipError_t launchTrace(const TraceArgs& a, int width, bool any, hipStream_t s);
+// Persistent-wave launch state: a device batch counter that only grows (launchBase is its
+// value when this launch starts) and the per-wave spill rows of the traversal stack.
+struct PersistArgs {
+    unsigned long long* counter;
+    unsigned long long launchBase;
+    uint2* spill;             // [grid][spillDepth][64]
+    uint32_t spillDepth;
+};
+
+// ldsDepth in {8, 12, 16, 24, 32}; p == nullptr selects the one-ray-per-lane kernel.
+hipError_t launchTrace(const TraceArgs& a, int width, bool any, int ldsDepth, const PersistArgs* p, unsigned grid,
+                       hipStream_t s);
 
 }  // namespace rbr

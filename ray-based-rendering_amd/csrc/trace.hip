@@ -103,47 +103,64 @@ __device__ __forceinline__ void cswap(float& ta, T& ra, float& tb, T& rb) {
     ra = rr;
 }
 
-// Two-level stack: LDS rows [depth][lane] for the first kLds entries, private memory
-// beyond (sized for the deepest tree the host accepts).
-template <int BLOCK, int LDS>
-struct Stack {
-    uint2* lds;          // base of this block's LDS stack
+// Traversal stack: LDS rows [depth][lane] for the first LDS entries of each lane; deeper
+// entries spill to a per-wave region of a global buffer (persistent kernel) or to private
+// memory (one-ray-per-lane kernel).
+template <int LDS>
+struct GlobalSpillStack {
+    uint2* lds;     // this wave's LDS rows
+    uint2* spill;   // this wave's spill rows [depth - LDS][64]
+    int sp = 0;
+    __device__ __forceinline__ void push(uint32_t ref, float t) {
+        uint2 e = make_uint2(ref, f2u(t));
+        if (sp < LDS) lds[sp * 64 + threadIdx.x] = e;
+        else spill[(sp - LDS) * 64 + threadIdx.x] = e;
+        ++sp;
+    }
+    __device__ __forceinline__ uint2 pop() {
+        --sp;
+        return sp < LDS ? lds[sp * 64 + threadIdx.x] : spill[(sp - LDS) * 64 + threadIdx.x];
+    }
+};
+
+template <int LDS>
+struct PrivateSpillStack {
+    uint2* lds;
     uint2 spill[kMaxStack - LDS];
     int sp = 0;
     __device__ __forceinline__ void push(uint32_t ref, float t) {
         uint2 e = make_uint2(ref, f2u(t));
-        if (sp < LDS) lds[sp * BLOCK + threadIdx.x] = e;
+        if (sp < LDS) lds[sp * 64 + threadIdx.x] = e;
         else spill[sp - LDS] = e;
         ++sp;
     }
     __device__ __forceinline__ uint2 pop() {
         --sp;
-        return sp < LDS ? lds[sp * BLOCK + threadIdx.x] : spill[sp - LDS];
+        return sp < LDS ? lds[sp * 64 + threadIdx.x] : spill[sp - LDS];
     }
 };
 
-template <int W, int BLOCK, int LDS, bool ANY, bool MIXED>
-__global__ __launch_bounds__(BLOCK) void traceKernel(TraceArgs a) {
-    __shared__ uint2 ldsStack[LDS * BLOCK];
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= a.n) return;
-    const DRay r = loadRay(a.rays, i);
-    float bestT = r.tmax;
-    uint32_t bestId = RBR_MISS_ID;
-    bool occluded = false;
-
+// One ray through the wide BVH and the unbounded list.  Returns the lowest-id closest hit
+// (bestT/bestId) or, for ANY, whether any primitive is hit in [tmin, tmax].
+template <int W, bool ANY, bool MIXED, typename Stack>
+__device__ __forceinline__ void traceRay(const TraceArgs& a, const DRay& r, Stack& st, float& bestT, uint32_t& bestId,
+                                         bool& occluded) {
+    bestT = r.tmax;
+    bestId = RBR_MISS_ID;
+    occluded = false;
     if (a.nodes) {
-        Stack<BLOCK, LDS> st;
-        st.lds = ldsStack;
         uint32_t ref = 0;   // root: wide node 0
         while (true) {
             if (!isLeafD(ref)) {
                 const float4* nd = a.nodes + size_t(ref) * (W == 4 ? 8 : 16);
                 float tc[W];
                 uint32_t rc[W];
-                if (W == 4) {
-                    float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
-                    uint4 ch = *reinterpret_cast<const uint4*>(nd + 6);
+#pragma unroll
+                for (int half = 0; half < W / 4; ++half) {
+                    const int q = W / 4;
+                    float4 lx = nd[0 * q + half], hx = nd[1 * q + half], ly = nd[2 * q + half];
+                    float4 hy = nd[3 * q + half], lz = nd[4 * q + half], hz = nd[5 * q + half];
+                    uint4 ch = *reinterpret_cast<const uint4*>(nd + 6 * q + half);
                     const float* LX = &lx.x; const float* HX = &hx.x; const float* LY = &ly.x;
                     const float* HY = &hy.x; const float* LZ = &lz.x; const float* HZ = &hz.x;
                     const uint32_t* CH = &ch.x;
@@ -151,25 +168,8 @@ __global__ __launch_bounds__(BLOCK) void traceKernel(TraceArgs a) {
                     for (int k = 0; k < 4; ++k) {
                         bool h;
                         float t = slab(r, LX[k], HX[k], LY[k], HY[k], LZ[k], HZ[k], bestT, h);
-                        tc[k] = h ? t : INFINITY;
-                        rc[k] = h ? CH[k] : kEmptyD;
-                    }
-                } else {
-#pragma unroll
-                    for (int half = 0; half < 2; ++half) {
-                        float4 lx = nd[0 + half], hx = nd[2 + half], ly = nd[4 + half], hy = nd[6 + half];
-                        float4 lz = nd[8 + half], hz = nd[10 + half];
-                        uint4 ch = *reinterpret_cast<const uint4*>(nd + 12 + half);
-                        const float* LX = &lx.x; const float* HX = &hx.x; const float* LY = &ly.x;
-                        const float* HY = &hy.x; const float* LZ = &lz.x; const float* HZ = &hz.x;
-                        const uint32_t* CH = &ch.x;
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            bool h;
-                            float t = slab(r, LX[k], HX[k], LY[k], HY[k], LZ[k], HZ[k], bestT, h);
-                            tc[half * 4 + k] = h ? t : INFINITY;
-                            rc[half * 4 + k] = h ? CH[k] : kEmptyD;
-                        }
+                        tc[half * 4 + k] = h ? t : INFINITY;
+                        rc[half * 4 + k] = h ? CH[k] : kEmptyD;
                     }
                 }
                 if (!ANY) {
@@ -190,7 +190,6 @@ __global__ __launch_bounds__(BLOCK) void traceKernel(TraceArgs a) {
                 for (int k = 0; k < W; ++k) cnt += rc[k] != kEmptyD ? 1 : 0;
                 if (cnt > 0) {
                     if (ANY) {
-                        // Any order: keep the first hit child, push the rest.
                         uint32_t first = kEmptyD;
 #pragma unroll
                         for (int k = 0; k < W; ++k) {
@@ -238,27 +237,96 @@ __global__ __launch_bounds__(BLOCK) void traceKernel(TraceArgs a) {
             if (t < bestT || id < bestId) { bestT = t; bestId = id; }
         }
     }
-    if (ANY) {
-        a.occ[i] = occluded ? 1 : 0;
-    } else {
-        a.hits[i] = make_uint2(bestId, bestId == RBR_MISS_ID ? 0u : f2u(bestT));
+}
+
+template <bool ANY>
+__device__ __forceinline__ void storeResult(const TraceArgs& a, uint32_t i, float bestT, uint32_t bestId, bool occ) {
+    if (ANY) a.occ[i] = occ ? 1 : 0;
+    else a.hits[i] = make_uint2(bestId, bestId == RBR_MISS_ID ? 0u : f2u(bestT));
+}
+
+// Persistent waves: each 64-lane workgroup takes batches of 64 consecutive rays from a
+// device counter until the batch is exhausted, so slow rays do not idle a whole launch
+// slot and every resident wave stays busy to the end.  The counter is never reset: the host
+// knows how far each launch advances it (one fetch per batch plus one final fetch per wave).
+template <int W, int LDS, bool ANY, bool MIXED>
+__global__ __launch_bounds__(64) void traceKernelPersistent(TraceArgs a, PersistArgs p) {
+    __shared__ uint2 ldsStack[LDS * 64];
+    __shared__ unsigned long long batch;
+    GlobalSpillStack<LDS> st;
+    st.lds = ldsStack;
+    st.spill = p.spill + size_t(blockIdx.x) * size_t(p.spillDepth) * 64;
+    while (true) {
+        if (threadIdx.x == 0) batch = atomicAdd(p.counter, 64ull);
+        __syncthreads();
+        const unsigned long long local = batch - p.launchBase;
+        __syncthreads();
+        if (local >= a.n) break;
+        const uint32_t i = uint32_t(local) + threadIdx.x;
+        if (i < a.n) {
+            const DRay r = loadRay(a.rays, i);
+            float bestT;
+            uint32_t bestId;
+            bool occ;
+            st.sp = 0;
+            traceRay<W, ANY, MIXED>(a, r, st, bestT, bestId, occ);
+            storeResult<ANY>(a, i, bestT, bestId, occ);
+        }
     }
 }
 
-template <int W, int BLOCK, int LDS, bool ANY>
-hipError_t launchT(const TraceArgs& a, hipStream_t s) {
-    dim3 grid((a.n + BLOCK - 1) / BLOCK);
-    if (a.mixed) hipLaunchKernelGGL((traceKernel<W, BLOCK, LDS, ANY, true>), grid, dim3(BLOCK), 0, s, a);
-    else hipLaunchKernelGGL((traceKernel<W, BLOCK, LDS, ANY, false>), grid, dim3(BLOCK), 0, s, a);
+// One ray per lane, stack overflow in private memory (kept for comparison and for callers
+// that cannot provide the spill buffer).
+template <int W, int LDS, bool ANY, bool MIXED>
+__global__ __launch_bounds__(64) void traceKernelLane(TraceArgs a) {
+    __shared__ uint2 ldsStack[LDS * 64];
+    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= a.n) return;
+    PrivateSpillStack<LDS> st;
+    st.lds = ldsStack;
+    const DRay r = loadRay(a.rays, i);
+    float bestT;
+    uint32_t bestId;
+    bool occ;
+    traceRay<W, ANY, MIXED>(a, r, st, bestT, bestId, occ);
+    storeResult<ANY>(a, i, bestT, bestId, occ);
+}
+
+template <int W, int LDS, bool ANY>
+hipError_t launchT(const TraceArgs& a, const PersistArgs* p, unsigned grid, hipStream_t s) {
+    if (p) {
+        if (a.mixed) hipLaunchKernelGGL((traceKernelPersistent<W, LDS, ANY, true>), dim3(grid), dim3(64), 0, s, a, *p);
+        else hipLaunchKernelGGL((traceKernelPersistent<W, LDS, ANY, false>), dim3(grid), dim3(64), 0, s, a, *p);
+    } else {
+        dim3 g((a.n + 63) / 64);
+        if (a.mixed) hipLaunchKernelGGL((traceKernelLane<W, LDS, ANY, true>), g, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((traceKernelLane<W, LDS, ANY, false>), g, dim3(64), 0, s, a);
+    }
     return hipGetLastError();
+}
+
+template <int W, int LDS>
+hipError_t launchW(const TraceArgs& a, const PersistArgs* p, unsigned grid, bool any, hipStream_t s) {
+    return any ? launchT<W, LDS, true>(a, p, grid, s) : launchT<W, LDS, false>(a, p, grid, s);
+}
+
+template <int LDS>
+hipError_t launchL(const TraceArgs& a, const PersistArgs* p, unsigned grid, int width, bool any, hipStream_t s) {
+    return width == 8 ? launchW<8, LDS>(a, p, grid, any, s) : launchW<4, LDS>(a, p, grid, any, s);
 }
 
 }  // namespace
 
-hipError_t launchTrace(const TraceArgs& a, int width, bool any, hipStream_t s) {
+hipError_t launchTrace(const TraceArgs& a, int width, bool any, int ldsDepth, const PersistArgs* p, unsigned grid,
+                       hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    if (width == 8) return any ? launchT<8, kBlock, kLdsDepth, true>(a, s) : launchT<8, kBlock, kLdsDepth, false>(a, s);
-    return any ? launchT<4, kBlock, kLdsDepth, true>(a, s) : launchT<4, kBlock, kLdsDepth, false>(a, s);
+    switch (ldsDepth) {
+        case 8: return launchL<8>(a, p, grid, width, any, s);
+        case 12: return launchL<12>(a, p, grid, width, any, s);
+        case 24: return launchL<24>(a, p, grid, width, any, s);
+        case 32: return launchL<32>(a, p, grid, width, any, s);
+        default: return launchL<16>(a, p, grid, width, any, s);
+    }
 }
 
 }  // namespace rbr

@@ -69,6 +69,7 @@ _sig(_lib.rbr_device_free, [_vp, _vp])
 _sig(_lib.rbr_copy_to_device, [_vp, _vp, _vp, _sz])
 _sig(_lib.rbr_copy_to_host, [_vp, _vp, _vp, _sz])
 _sig(_lib.rbr_synchronize, [_vp])
+_sig(_lib.rbr_calib_copy, [_vp, _vp, _vp, _sz])
 _sig(_lib.rbr_scene_create, [_vp, C.POINTER(_vp)])
 _sig(_lib.rbr_scene_destroy, [_vp], None)
 _sig(_lib.rbr_scene_set_geometry, [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp])
@@ -79,6 +80,7 @@ _sig(_lib.rbr_trace_closest, [_vp, _vp, _sz, _vp])
 _sig(_lib.rbr_trace_any, [_vp, _vp, _sz, _vp])
 _sig(_lib.rbr_trace_closest_device, [_vp, _vp, _sz, _vp])
 _sig(_lib.rbr_trace_any_device, [_vp, _vp, _sz, _vp])
+_sig(_lib.rbr_set_trace_tuning, [_vp, C.c_int, C.c_int, C.c_int])
 _sig(_lib.rbr_time_trace, [_vp, C.c_int, _vp, _sz, _vp, C.c_int, C.POINTER(C.c_float)])
 _sig(_lib.rbr_host_trace_closest, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
 _sig(_lib.rbr_host_trace_any, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])

@@ -143,3 +143,24 @@ def test_c1_full_size(ctx, width):
     assert_same(h, o.closest(rays))
     assert np.array_equal(occ, o.any(shadow))
     assert 0.3 < (h["id"] != R.MISS).mean() < 1.0
+
+
+@pytest.mark.parametrize("mode,depth,waves", [(0, 8, 1), (0, 12, 32), (0, 32, 16), (1, 8, 16), (1, 24, 16)])
+def test_launch_tunings_identical(ctx, mode, depth, waves):
+    """Every launch configuration (persistent waves with tiny/large LDS stacks and so
+    deep global spills, or one ray per lane) returns the same bit-exact hits."""
+    say(f"case: tuning mode {mode} lds {depth} waves {waves}")
+    tris = R.icosphere(48, 100.0, 0.1, 0xC1)
+    lo, hi = R.mesh_bounds(tris)
+    rays = R.rays_uniform(lo, hi, 100000, 21)
+    o = O.OracleScene(tris=tris)
+    o.build_bvh()
+    ref = o.closest(rays)
+    ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, depth, waves))
+    try:
+        for width in (4, 8):
+            sc = gpu_scene(ctx, width, 4, tris=tris)
+            assert_same(sc.trace_closest(rays), ref)
+            assert np.array_equal(sc.trace_any(rays), o.any(rays))
+    finally:
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
