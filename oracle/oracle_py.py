@@ -184,3 +184,113 @@ class RefWorld:
         occ = np.empty(len(rays), np.uint8)
         self._lib.ref_brute_any(self._h, rays.ctypes.data, len(rays), occ.ctypes.data, threads)
         return occ
+
+
+# ---------------------------------------------------------------- rendering
+
+def _render_structs():
+    import rbr_amd  # the C ABI struct layouts (rbr.h) shared by the oracle entry points
+
+    return rbr_amd
+
+
+_sig(_orc.orc_rscene_create, [_vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp,
+                              _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp], _vp)
+_sig(_orc.orc_rscene_destroy, [_vp])
+_sig(_orc.orc_render, [_vp, _vp, _vp, _vp, C.c_int])
+_sig(_orc.orc_render_moments, [_vp, _vp, _vp, _vp, _sz, _vp, _vp, C.c_int])
+
+
+def _geom(desc):
+    t, nt = _arr(desc.tris if len(desc.tris) else None, 9)
+    s, ns = _arr(desc.spheres if len(desc.spheres) else None, 4)
+    q, nq = _arr(desc.quads if len(desc.quads) else None, 12)
+    p, np_ = _arr(desc.planes if len(desc.planes) else None, 4)
+    return (t, nt), (s, ns), (q, nq), (p, np_)
+
+
+class OracleRender:
+    """The oracle's restated Integrator::render over a scenes.SceneDesc (lowest-id BVH)."""
+
+    def __init__(self, desc):
+        R = _render_structs()
+        (t, nt), (s, ns), (q, nq), (p, np_) = _geom(desc)
+        pm = np.ascontiguousarray(desc.prim_material, np.int32)
+        mats, lights, med = R.c_materials(desc.materials), R.c_lights(desc.lights), R.c_medium(desc.medium)
+        bg = np.asarray(desc.background, np.float32)
+        self._keep = (t, s, q, p, pm, mats, lights, med, bg)
+        self.desc = desc
+        self._R = R
+        self._h = _orc.orc_rscene_create(_p(t), nt, None, _p(s), ns, None, _p(q), nq, None, _p(p), np_, None,
+                                         C.addressof(mats), len(desc.materials), pm.ctypes.data, len(pm),
+                                         C.addressof(lights), len(desc.lights), C.addressof(med), bg.ctypes.data)
+
+    def close(self):
+        if self._h:
+            _orc.orc_rscene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def render(self, spp: int, depth: int, integrator: int = 0, nee: bool = True, mis: bool = True, camera=None,
+               threads: int = 0) -> np.ndarray:
+        """Integrator::render with the reference's per-row sampler seeding (y * width)."""
+        cam = camera or self.desc.camera
+        c = self._R.c_camera(cam)
+        cfg = self._R.c_config(spp, depth, 0, integrator, nee, mis)
+        film = np.zeros((cam.height, cam.width, 3), np.float32)
+        _orc.orc_render(self._h, C.addressof(c), C.addressof(cfg), film.ctypes.data, threads)
+        return film
+
+    def moments(self, spp: int, depth: int, seed: int = 7, integrator: int = 0, nee: bool = True, mis: bool = True,
+                pixels=None, camera=None, threads: int = 0):
+        """Per-pixel sum and sum of squares (float64) of `spp` single-sample estimates."""
+        cam = camera or self.desc.camera
+        c = self._R.c_camera(cam)
+        cfg = self._R.c_config(spp, depth, seed, integrator, nee, mis)
+        s1 = np.zeros((cam.height * cam.width, 3), np.float64)
+        s2 = np.zeros_like(s1)
+        px = None if pixels is None else np.ascontiguousarray(pixels, np.uint32)
+        _orc.orc_render_moments(self._h, C.addressof(c), C.addressof(cfg), _p(px), 0 if px is None else len(px),
+                                s1.ctypes.data, s2.ctypes.data, threads)
+        return s1, s2
+
+
+class RefRender:
+    """The reference's own Scene + PathTracingIntegrator / VolumetricPathIntegrator."""
+
+    def __init__(self, desc):
+        lib = _load_ref()
+        if not hasattr(lib, "_render_sig"):
+            _sig(lib.ref_scene_create, [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _u32, _vp, _vp], _vp)
+            _sig(lib.ref_scene_destroy, [_vp])
+            _sig(lib.ref_render, [_vp, _vp, _vp, _vp, C.c_int], C.c_double)
+            lib._render_sig = True
+        R = _render_structs()
+        (t, nt), (s, ns), (q, nq), (p, np_) = _geom(desc)
+        pm = np.ascontiguousarray(desc.prim_material, np.int32)
+        mats, lights, med = R.c_materials(desc.materials), R.c_lights(desc.lights), R.c_medium(desc.medium)
+        bg = np.asarray(desc.background, np.float32)
+        self._keep = (t, s, q, p, pm, mats, lights, med, bg)
+        self._lib, self._R, self.desc = lib, R, desc
+        self._h = lib.ref_scene_create(_p(t), nt, _p(s), ns, _p(q), nq, _p(p), np_, C.addressof(mats),
+                                       len(desc.materials), pm.ctypes.data, C.addressof(lights), len(desc.lights),
+                                       C.addressof(med), bg.ctypes.data)
+
+    def close(self):
+        if self._h:
+            self._lib.ref_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def render(self, spp: int, depth: int, integrator: int = 0, nee: bool = True, mis: bool = True, camera=None,
+               threads: int = 0):
+        cam = camera or self.desc.camera
+        c = self._R.c_camera(cam)
+        cfg = self._R.c_config(spp, depth, 0, integrator, nee, mis)
+        film = np.zeros((cam.height, cam.width, 3), np.float32)
+        sec = self._lib.ref_render(self._h, C.addressof(c), C.addressof(cfg), film.ctypes.data, threads)
+        return film, sec

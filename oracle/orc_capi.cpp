@@ -101,3 +101,168 @@ int orc_triangle_t(const float* tri9, const rbr_ray* ray, float* t) {
 int orc_max_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- rendering
+
+#include "orc_render.h"
+
+namespace {
+
+struct RS {
+    RenderScene s;
+};
+
+V3 f3(const float* p) { return V3(p[0], p[1], p[2]); }
+
+orc::Camera toCam(const rbr_camera& c) {
+    orc::Camera k;
+    k.position = f3(c.position);
+    k.forward = f3(c.forward);
+    k.right = f3(c.right);
+    k.up = f3(c.up);
+    k.fx = c.fx;
+    k.fy = c.fy;
+    k.cx = c.cx;
+    k.cy = c.cy;
+    k.width = c.width;
+    k.height = c.height;
+    return k;
+}
+
+orc::Config toCfg(const rbr_render_config& c) {
+    orc::Config k;
+    k.integrator = c.integrator;
+    k.spp = c.spp;
+    k.maxDepth = c.max_depth;
+    k.useNEE = c.use_nee != 0;
+    k.useMIS = c.use_mis != 0;
+    return k;
+}
+
+inline Color Li(const RenderScene& s, const orc::Config& c, const Ray& r, Sampler& smp) {
+    return c.integrator == RBR_INTEGRATOR_VOLPATH ? LiVolPath(s, c, r, smp) : LiPath(s, c, r, smp);
+}
+
+}  // namespace
+
+extern "C" {
+
+void* orc_rscene_create(const float* tris, uint32_t ntri, const uint32_t* tri_ids, const float* spheres, uint32_t nsph,
+                        const uint32_t* sph_ids, const float* quads, uint32_t nquad, const uint32_t* quad_ids,
+                        const float* planes, uint32_t nplane, const uint32_t* plane_ids, const rbr_material* mats,
+                        uint32_t nmat, const int32_t* primMat, uint32_t nPrimIds, const rbr_light* lights,
+                        uint32_t nlights, const rbr_medium* medium, const float* background) {
+    orc_scene* g = orc_scene_create(tris, ntri, tri_ids, spheres, nsph, sph_ids, quads, nquad, quad_ids, planes, nplane,
+                                    plane_ids);
+    auto* r = new RS();
+    r->s.geo = std::move(g->s);
+    delete g;
+    r->s.geo.buildBVH();
+    for (uint32_t i = 0; i < nmat; ++i) {
+        orc::Material m;
+        m.baseColor = f3(mats[i].base_color);
+        m.metallic = mats[i].metallic;
+        m.roughness = mats[i].roughness;
+        m.ior = mats[i].ior;
+        m.transmission = mats[i].transmission;
+        m.emission = f3(mats[i].emission);
+        r->s.materials.push_back(m);
+    }
+    r->s.primMaterial.assign(primMat, primMat + nPrimIds);
+    for (uint32_t i = 0; i < nlights; ++i) {
+        const rbr_light& l = lights[i];
+        orc::Light x;
+        x.type = l.type;
+        x.color = f3(l.color);
+        x.intensity = l.intensity;
+        x.position = f3(l.position);
+        x.u = f3(l.u);
+        x.v = f3(l.v);
+        x.normal = f3(l.normal);
+        x.direction = f3(l.direction);
+        r->s.lights.push_back(x);
+    }
+    if (medium) {
+        r->s.medium.type = medium->type;
+        r->s.medium.sigma_a = f3(medium->sigma_a);
+        r->s.medium.sigma_s = f3(medium->sigma_s);
+        r->s.medium.falloff = medium->falloff;
+        r->s.medium.baseHeight = medium->base_height;
+        r->s.medium.g = medium->g;
+    }
+    if (background) r->s.background = f3(background);
+    return r;
+}
+
+void orc_rscene_destroy(void* h) { delete static_cast<RS*>(h); }
+
+// Integrator::render (integrator.cpp:7-50): rows in parallel, sampler reseeded with y*W per
+// row, per sample jitter (get2D), two lens numbers, camera ray, Li; film = sum * (1/spp).
+void orc_render(void* h, const rbr_camera* cam, const rbr_render_config* cfg, float* film, int nthreads) {
+    const RenderScene& s = static_cast<RS*>(h)->s;
+    const orc::Camera c = toCam(*cam);
+    const orc::Config k = toCfg(*cfg);
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+    {
+        Sampler smp;
+#pragma omp for schedule(dynamic, 16)
+        for (int y = 0; y < c.height; ++y) {
+            smp.setSeed(uint64_t(y) * uint64_t(c.width));
+            for (int x = 0; x < c.width; ++x) {
+                Color sum(0.0f);
+                for (int sIdx = 0; sIdx < k.spp; ++sIdx) {
+                    float jx = smp.get1D();
+                    float jy = smp.get1D();
+                    float px = x + jx, py = y + jy;
+                    (void)smp.get1D();   // lensU
+                    (void)smp.get1D();   // lensV
+                    sum += Li(s, k, c.generateRay(px, py), smp);
+                }
+                Color v = sum / float(k.spp);
+                float* o = film + 3 * (size_t(y) * c.width + x);
+                o[0] = v.x;
+                o[1] = v.y;
+                o[2] = v.z;
+            }
+        }
+    }
+}
+
+// Per-pixel first and second moments of the single-sample radiance (double accumulation),
+// each pixel with its own stream seeded from (seed, pixel index).  `pixels` (optional)
+// restricts the work to a list of pixel indices; sums are written at those indices.
+void orc_render_moments(void* h, const rbr_camera* cam, const rbr_render_config* cfg, const uint32_t* pixels,
+                        size_t npix, double* sum, double* sumsq, int nthreads) {
+    const RenderScene& s = static_cast<RS*>(h)->s;
+    const orc::Camera c = toCam(*cam);
+    const orc::Config k = toCfg(*cfg);
+    const size_t total = pixels ? npix : size_t(c.width) * c.height;
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+    {
+        Sampler smp;
+#pragma omp for schedule(dynamic, 8)
+        for (long long j = 0; j < (long long)total; ++j) {
+            const uint32_t p = pixels ? pixels[j] : uint32_t(j);
+            const int x = int(p % uint32_t(c.width)), y = int(p / uint32_t(c.width));
+            smp.setSeed(cfg->seed * 0x9E3779B97F4A7C15ull + p);
+            double s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
+            for (int sIdx = 0; sIdx < k.spp; ++sIdx) {
+                float jx = smp.get1D();
+                float jy = smp.get1D();
+                (void)smp.get1D();
+                (void)smp.get1D();
+                Color v = Li(s, k, c.generateRay(x + jx, y + jy), smp);
+                s1[0] += v.x; s1[1] += v.y; s1[2] += v.z;
+                s2[0] += double(v.x) * v.x; s2[1] += double(v.y) * v.y; s2[2] += double(v.z) * v.z;
+            }
+            for (int ch = 0; ch < 3; ++ch) {
+                sum[3 * size_t(p) + ch] = s1[ch];
+                sumsq[3 * size_t(p) + ch] = s2[ch];
+            }
+        }
+    }
+}
+
+}  // extern "C"

@@ -34,7 +34,6 @@ struct rbr_ctx {
     int ldsDepth = rbr::kLdsDepthDefault;
     int wavesPerCu = rbr::kWavesPerCuDefault;
     rbr::DevBuf dCounter, dSpill;
-    unsigned long long counterBase = 0;
 };
 
 struct rbr_scene {

@@ -90,28 +90,20 @@ static TraceArgs traceArgs(const rbr_scene* s) {
 TraceArgs rbr::sceneTraceArgs(const rbr_scene* s) { return traceArgs(s); }
 
 int rbr::launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack, bool any) {
-    if (a.n == 0) return RBR_OK;
+    if (a.n == 0 && !a.nDev) return RBR_OK;
     hipError_t e;
-    if (ctx->traceMode == 1) {
+    if (ctx->traceMode == 1 && !a.nDev) {
         if (maxStack > uint32_t(kMaxStack)) return fail(ctx, RBR_ERR_UNSUPPORTED, "BVH too deep for the lane stack");
         e = launchTrace(a, width, any, ctx->ldsDepth, nullptr, 0, ctx->stream);
         return e == hipSuccess ? RBR_OK : failHip(ctx, e, "trace launch");
     }
     const unsigned grid = unsigned(std::max(1, ctx->cuCount) * ctx->wavesPerCu);
     const uint32_t spillDepth = std::max<uint32_t>(1u, maxStack > uint32_t(ctx->ldsDepth) ? maxStack - ctx->ldsDepth : 1u);
-    if (!ctx->dCounter.ptr) {
-        if (int rc = ensureBuffer(ctx, ctx->dCounter, sizeof(unsigned long long))) return rc;
-        e = hipMemsetAsync(ctx->dCounter.ptr, 0, sizeof(unsigned long long), ctx->stream);
-        if (e != hipSuccess) return failHip(ctx, e, "counter init");
-        ctx->counterBase = 0;
-    }
+    if (int rc = ensureBuffer(ctx, ctx->dCounter, sizeof(uint32_t))) return rc;
     if (int rc = ensureBuffer(ctx, ctx->dSpill, size_t(grid) * spillDepth * 64 * sizeof(uint2))) return rc;
-    PersistArgs p{static_cast<unsigned long long*>(ctx->dCounter.ptr), ctx->counterBase,
-                  static_cast<uint2*>(ctx->dSpill.ptr), spillDepth};
+    PersistArgs p{static_cast<uint32_t*>(ctx->dCounter.ptr), static_cast<uint2*>(ctx->dSpill.ptr), spillDepth};
     e = launchTrace(a, width, any, ctx->ldsDepth, &p, grid, ctx->stream);
     if (e != hipSuccess) return failHip(ctx, e, "trace launch");
-    // Every batch takes one fetch and every wave one final, failing fetch.
-    ctx->counterBase += 64ull * ((uint64_t(a.n) + 63) / 64 + grid);
     return RBR_OK;
 }
 

@@ -1,7 +1,921 @@
-// Wavefront path tracer (filled in by the integrator work; traversal-only build for now).
+// Wavefront path tracer for gfx950: Integrator::render (integrator.cpp:7-50) with
+// PathTracingIntegrator::Li (path_tracing_integrator.cpp:5-138) and
+// VolumetricPathIntegrator::Li (volumetric_path_integrator.cpp:6-143), restructured as
+// one kernel per stage over a device pool of paths:
+//
+//   camera  -> [ extend (closest hit) -> shade -> occlusion (any hit) -> gather ] x depth
+//           -> accumulate film
+//
+// * extend / occlusion are the persistent traversal kernels of trace.hip, launched with the
+//   queue length read from device memory, so no bounce needs a host round trip;
+// * shade writes continuing paths to the next queue and NEE shadow rays to the shadow queue,
+//   compacted per wave with a ballot + one atomicAdd per wave;
+// * gather adds each path's unoccluded NEE terms in light order (deterministic);
+// * several jobs (different scenes, resolutions, spp, depths, media) share the pool: their
+//   BVHs are packed into one node arena and every queued ray carries its root node.
+// Random numbers are a counter-based hash of (seed, path, bounce, dimension), so results
+// do not depend on queue order or pool size.  The estimator is the reference's, including its
+// quirks: BSDF-sampled emission counted only at depth 0 / after delta bounces, NEE weighted
+// by powerHeuristic(p_light, p_bsdf) for area lights; the volumetric integrator's emission
+// on every hit, point-light-only volume NEE and its medium weights.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
 #include "context.h"
+#include "shading.h"
+#include "trace.h"
+
+// Geometry that decides occlusion (hit points, light sample points, shadow-ray origins and
+// lengths) must round exactly as the reference does: an emitter quad coplanar with its
+// AreaLight puts NEE shadow rays on a knife edge (t_quad ~ dist - EPSILON), so contraction
+// stays off and every operation keeps the reference's operand order.
+#pragma clang fp contract(off)
+
+namespace rbr {
+
+namespace {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 1.0f / kPi;
+constexpr float kTwoPi = 2.0f * kPi;
+constexpr float kEpsD = 1e-4f;
+constexpr uint32_t kMaxLights = 32;
+
+// ---------------------------------------------------------------- vectors
+
+struct F3 {
+    float x, y, z;
+};
+__device__ __forceinline__ F3 mk(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ F3 operator-(F3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ F3 operator*(F3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ F3 operator*(F3 a, F3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float len(F3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ F3 normalize(F3 a) {
+    float l = len(a);
+    return l > 0.0f ? a * (1.0f / l) : mk(0, 0, 0);
+}
+__device__ __forceinline__ bool black(F3 a) { return a.x <= 0.0f && a.y <= 0.0f && a.z <= 0.0f; }
+__device__ __forceinline__ float maxc(F3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+__device__ __forceinline__ F3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
+
+// coordinateSystem (vector.h:282-289) and sampling::Frame (sampler.h:103-118)
+struct Frame {
+    F3 t, b, n;
+    __device__ explicit Frame(F3 nn) : n(nn) {
+        if (fabsf(nn.x) > fabsf(nn.y)) t = normalize(mk(-nn.z, 0.0f, nn.x));
+        else t = normalize(mk(0.0f, nn.z, -nn.y));
+        b = cross(nn, t);
+    }
+    __device__ F3 toWorld(F3 l) const { return t * l.x + b * l.y + n * l.z; }
+    __device__ F3 toLocal(F3 w) const { return mk(dot(w, t), dot(w, b), dot(w, n)); }
+};
+
+// ---------------------------------------------------------------- random numbers
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+struct Rng {
+    uint64_t key;
+    __device__ float u(uint32_t dim) const {
+        uint64_t h = mix64(key ^ (uint64_t(dim) * 0xD6E8FEB86659FD93ull));
+        return float(uint32_t(h >> 40)) * (1.0f / 16777216.0f);
+    }
+};
+
+__device__ __forceinline__ Rng pathRng(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t pixels) {
+    return Rng{mix64(seed * 0xA0761D6478BD642Full + uint64_t(sample) * pixels + pixel)};
+}
+
+// Dimensions: camera 0..1; bounce d uses base(d) + {0,1 BSDF, 2 RR, 3 distance, 4,5 phase,
+// 8+2l, 9+2l light l}.
+__device__ __forceinline__ uint32_t dimBase(uint32_t depth) { return 8u + depth * 80u; }
+
+// ---------------------------------------------------------------- Fresnel and lobes
+
+// fresnelDielectric, fresnel.h:16-32
+__device__ float fresnelDielectric(float c, float etaI, float etaT) {
+    c = fminf(fmaxf(c, -1.0f), 1.0f);
+    if (c < 0.0f) {
+        float tmp = etaI;
+        etaI = etaT;
+        etaT = tmp;
+        c = -c;
+    }
+    const float sinI = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+    const float sinT = etaI / etaT * sinI;
+    if (sinT >= 1.0f) return 1.0f;
+    const float cosT = sqrtf(fmaxf(0.0f, 1.0f - sinT * sinT));
+    const float rParl = (etaT * c - etaI * cosT) / (etaT * c + etaI * cosT);
+    const float rPerp = (etaI * c - etaT * cosT) / (etaI * c + etaT * cosT);
+    return 0.5f * (rParl * rParl + rPerp * rPerp);
+}
+
+__device__ F3 fresnelEval(uint32_t kind, float etaI, float etaT, const float* f0, float c) {
+    if (kind == DF_DIELECTRIC) {
+        float f = fresnelDielectric(c, etaI, etaT);
+        return mk(f, f, f);
+    }
+    if (kind == DF_SCHLICK) {   // fresnelSchlick, fresnel.h:41-45
+        const float m = 1.0f - fminf(fmaxf(fabsf(c), 0.0f), 1.0f);
+        const float m5 = m * m * m * m * m;
+        return mk(f0[0] + (1.0f - f0[0]) * m5, f0[1] + (1.0f - f0[1]) * m5, f0[2] + (1.0f - f0[2]) * m5);
+    }
+    return mk(1.0f, 1.0f, 1.0f);
+}
+
+__device__ __forceinline__ bool sameHemi(F3 a, F3 b) { return a.z * b.z > 0.0f; }
+
+// GGX (microfacet_bsdf.cpp:9-58)
+__device__ float ggxD(float a, F3 wh) {
+    const float c2 = wh.z * wh.z;
+    if (c2 <= 0.0f) return 0.0f;
+    const float a2 = a * a;
+    const float d = c2 * (a2 - 1.0f) + 1.0f;
+    return a2 / (kPi * d * d);
+}
+__device__ float ggxLambda(float a, F3 w) {
+    const float c2 = w.z * w.z;
+    if (c2 <= 0.0f) return 0.0f;
+    const float t2 = fmaxf(0.0f, 1.0f - c2) / c2;
+    return 0.5f * (sqrtf(1.0f + a * a * t2) - 1.0f);
+}
+__device__ float ggxPdf(float a, F3 wo, F3 wh) {
+    const float cosO = fabsf(wo.z);
+    if (cosO == 0.0f) return 0.0f;
+    return ggxD(a, wh) * (1.0f / (1.0f + ggxLambda(a, wo))) * fabsf(dot(wo, wh)) / cosO;
+}
+__device__ F3 ggxSample(float a, F3 wo, float u1, float u2) {   // Heitz 2018 VNDF
+    const bool flip = wo.z < 0.0f;
+    const F3 v = flip ? -wo : wo;
+    const F3 vh = normalize(mk(a * v.x, a * v.y, v.z));
+    const F3 t1 = vh.z < 0.9999f ? normalize(cross(mk(0, 0, 1), vh)) : mk(1, 0, 0);
+    const F3 t2 = cross(vh, t1);
+    const float r = sqrtf(u1);
+    const float phi = kTwoPi * u2;
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    const float p1 = r * cp;
+    float p2 = r * sp;
+    const float s = 0.5f * (1.0f + vh.z);
+    p2 = (1.0f - s) * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1)) + s * p2;
+    const F3 nh = t1 * p1 + t2 * p2 + vh * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
+    const F3 wh = normalize(mk(a * nh.x, a * nh.y, fmaxf(1e-6f, nh.z)));
+    return flip ? -wh : wh;
+}
+
+__device__ bool halfVecT(const DLobe& l, F3 wo, F3 wi, F3& wh, float& eta) {
+    if (sameHemi(wo, wi)) return false;
+    if (wo.z == 0.0f || wi.z == 0.0f) return false;
+    eta = wo.z > 0.0f ? l.etaIn / l.etaOut : l.etaOut / l.etaIn;
+    wh = wo + wi * eta;
+    if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return false;
+    wh = normalize(wh);
+    if (dot(wo, wh) < 0.0f) wh = -wh;
+    return wh.z * wo.z > 0.0f && dot(wi, wh) <= 0.0f;
+}
+
+__device__ F3 lobeEval(const DLobe& l, F3 wo, F3 wi) {
+    const F3 R = ld3(l.R);
+    if (l.type == DL_LAMBERT) return sameHemi(wo, wi) ? R * kInvPi : mk(0, 0, 0);
+    if (l.type == DL_MF_REFL) {
+        if (!sameHemi(wo, wi)) return mk(0, 0, 0);
+        const float cosO = fabsf(wo.z), cosI = fabsf(wi.z);
+        if (cosO == 0.0f || cosI == 0.0f) return mk(0, 0, 0);
+        F3 wh = wo + wi;
+        if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return mk(0, 0, 0);
+        wh = normalize(wh);
+        if (wh.z < 0.0f) wh = -wh;
+        const F3 F = fresnelEval(l.fkind, l.fEtaI, l.fEtaT, l.f0, dot(wo, wh));
+        const float G = 1.0f / (1.0f + ggxLambda(l.alpha, wo) + ggxLambda(l.alpha, wi));
+        return R * F * (ggxD(l.alpha, wh) * G / (4.0f * cosO * cosI));
+    }
+    if (l.type == DL_MF_TRANS) {
+        F3 wh;
+        float eta;
+        if (!halfVecT(l, wo, wi, wh, eta)) return mk(0, 0, 0);
+        const float dotOH = dot(wo, wh), dotIH = dot(wi, wh);
+        const float F = fresnelDielectric(wo.z > 0.0f ? dotOH : -dotOH, l.etaOut, l.etaIn);
+        const float denom = dotOH + eta * dotIH;
+        if (denom == 0.0f) return mk(0, 0, 0);
+        const float G = 1.0f / (1.0f + ggxLambda(l.alpha, wo) + ggxLambda(l.alpha, wi));
+        const float v = ggxD(l.alpha, wh) * G * fabsf(dotIH * dotOH) / fabsf(wi.z * wo.z * denom * denom);
+        return R * ((1.0f - F) * v);
+    }
+    return mk(0, 0, 0);
+}
+
+__device__ float lobePdf(const DLobe& l, F3 wo, F3 wi) {
+    if (l.type == DL_LAMBERT) return sameHemi(wo, wi) ? fabsf(wi.z) * kInvPi : 0.0f;
+    if (l.type == DL_MF_REFL) {
+        if (!sameHemi(wo, wi)) return 0.0f;
+        F3 wh = wo + wi;
+        if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return 0.0f;
+        wh = normalize(wh);
+        if (wh.z < 0.0f) wh = -wh;
+        const float dotOH = dot(wo, wh);
+        if (dotOH == 0.0f) return 0.0f;
+        return ggxPdf(l.alpha, wo, wh) / (4.0f * fabsf(dotOH));
+    }
+    if (l.type == DL_MF_TRANS) {
+        F3 wh;
+        float eta;
+        if (!halfVecT(l, wo, wi, wh, eta)) return 0.0f;
+        const float dotOH = dot(wo, wh), dotIH = dot(wi, wh);
+        const float denom = dotOH + eta * dotIH;
+        if (denom == 0.0f) return 0.0f;
+        return ggxPdf(l.alpha, wo, wh) * (fabsf(eta * eta * dotIH) / (denom * denom));
+    }
+    return 0.0f;
+}
+
+__device__ F3 lobeRho(const DLobe& l, F3 wo) {
+    const F3 R = ld3(l.R);
+    if (l.type == DL_LAMBERT) return R;
+    if (l.type == DL_SPEC_REFL || l.type == DL_MF_REFL) return R * fresnelEval(l.fkind, l.fEtaI, l.fEtaT, l.f0, wo.z);
+    return R * (1.0f - fresnelDielectric(wo.z, l.etaOut, l.etaIn));
+}
+
+struct BSample {
+    F3 wi, f;
+    float pdf;
+    bool delta;
+};
+
+__device__ BSample lobeSample(const DLobe& l, F3 wo, float u1, float u2) {
+    BSample r{mk(0, 0, 0), mk(0, 0, 0), 0.0f, false};
+    const F3 R = ld3(l.R);
+    if (l.type == DL_LAMBERT) {   // cosineHemisphere, sampler.h:73-81
+        const float rr = sqrtf(u1);
+        float s, c;
+        sincosf(kTwoPi * u2, &s, &c);
+        r.wi = mk(rr * c, rr * s, sqrtf(fmaxf(0.0f, 1.0f - u1)));
+        if (wo.z < 0.0f) r.wi.z = -r.wi.z;
+        r.f = R * kInvPi;
+        r.pdf = fabsf(r.wi.z) * kInvPi;
+        return r;
+    }
+    if (l.type == DL_SPEC_REFL) {
+        const float cosO = fabsf(wo.z);
+        if (cosO == 0.0f) return r;
+        r.wi = mk(-wo.x, -wo.y, wo.z);
+        r.f = R * fresnelEval(l.fkind, l.fEtaI, l.fEtaT, l.f0, wo.z) * (1.0f / cosO);
+        r.pdf = 1.0f;
+        r.delta = true;
+        return r;
+    }
+    if (l.type == DL_SPEC_TRANS) {
+        if (wo.z == 0.0f) return r;
+        const bool entering = wo.z > 0.0f;
+        const float eta = entering ? l.etaOut / l.etaIn : l.etaIn / l.etaOut;
+        const float cosI = fabsf(wo.z);
+        const float sin2T = eta * eta * fmaxf(0.0f, 1.0f - cosI * cosI);
+        if (sin2T >= 1.0f) return r;
+        const float cosT = sqrtf(1.0f - sin2T);
+        r.wi = mk(-eta * wo.x, -eta * wo.y, entering ? -cosT : cosT);
+        const float fr = fresnelDielectric(wo.z, l.etaOut, l.etaIn);
+        r.f = R * ((1.0f - fr) * eta * eta / cosT);
+        r.pdf = 1.0f;
+        r.delta = true;
+        return r;
+    }
+    if (wo.z == 0.0f) return r;
+    const F3 wh = ggxSample(l.alpha, wo, u1, u2);
+    F3 wi;
+    if (l.type == DL_MF_REFL) {
+        wi = wh * (2.0f * dot(wo, wh)) - wo;
+        if (!sameHemi(wo, wi)) return r;
+    } else {
+        const float dotOH = dot(wo, wh);
+        if (dotOH <= 0.0f) return r;
+        const bool entering = wo.z > 0.0f;
+        const float er = entering ? l.etaOut / l.etaIn : l.etaIn / l.etaOut;
+        const float sin2T = er * er * fmaxf(0.0f, 1.0f - dotOH * dotOH);
+        if (sin2T >= 1.0f) return r;
+        const float cosT = sqrtf(1.0f - sin2T);
+        wi = wh * (er * dotOH - cosT) - wo * er;
+        if (sameHemi(wo, wi) || wi.z == 0.0f) return r;
+    }
+    const float p = lobePdf(l, wo, wi);
+    if (p <= 0.0f) return r;
+    r.wi = wi;
+    r.pdf = p;
+    r.f = lobeEval(l, wo, wi);
+    return r;
+}
+
+// MixtureBSDF (mixture_bsdf.cpp:5-104)
+__device__ float effWeight(const DMaterial& m, int i, F3 wo) {
+    const float w = m.lobe[i].weight;
+    if (!((m.substrateMask >> i) & 1u)) return w;
+    const float f0[3] = {1.0f, 1.0f, 1.0f};
+    return w * fmaxf(0.0f, 1.0f - fresnelEval(m.sfKind, m.sfEtaI, m.sfEtaT, f0, wo.z).x);
+}
+
+__device__ float mixSelection(const DMaterial& m, F3 wo, float* w, float* s) {
+    float total = 0.0f;
+    for (uint32_t i = 0; i < m.n; ++i) {
+        w[i] = effWeight(m, int(i), wo);
+        s[i] = w[i] * maxc(lobeRho(m.lobe[i], wo));
+        total += s[i];
+    }
+    return total;
+}
+
+__device__ __forceinline__ bool lobeDelta(const DLobe& l) { return l.type == DL_SPEC_REFL || l.type == DL_SPEC_TRANS; }
+
+__device__ F3 mixEval(const DMaterial& m, F3 wo, F3 wi) {
+    F3 f = mk(0, 0, 0);
+    for (uint32_t i = 0; i < m.n; ++i) {
+        if (lobeDelta(m.lobe[i])) continue;
+        f = f + lobeEval(m.lobe[i], wo, wi) * effWeight(m, int(i), wo);
+    }
+    return f;
+}
+
+__device__ float mixPdf(const DMaterial& m, F3 wo, F3 wi) {
+    float w[6], s[6];
+    const float total = mixSelection(m, wo, w, s);
+    if (total <= 0.0f) return 0.0f;
+    float p = 0.0f;
+    for (uint32_t i = 0; i < m.n; ++i) {
+        if (lobeDelta(m.lobe[i])) continue;
+        p += s[i] * lobePdf(m.lobe[i], wo, wi);
+    }
+    return p / total;
+}
+
+__device__ BSample mixSample(const DMaterial& m, F3 wo, float u1, float u2) {
+    BSample none{mk(0, 0, 0), mk(0, 0, 0), 0.0f, false};
+    float w[6], s[6];
+    const float total = mixSelection(m, wo, w, s);
+    if (total <= 0.0f) return none;
+    float u = u1 * total;
+    int chosen = int(m.n) - 1;
+    for (uint32_t i = 0; i < m.n; ++i) {
+        if (s[i] <= 0.0f) continue;
+        if (u < s[i] || i + 1 == m.n) {
+            chosen = int(i);
+            break;
+        }
+        u -= s[i];
+    }
+    if (s[chosen] <= 0.0f) return none;
+    const float v1 = fminf(u / s[chosen], 0.99999994f);
+    const DLobe& l = m.lobe[chosen];
+    BSample r = lobeSample(l, wo, v1, u2);
+    if (r.pdf <= 0.0f || black(r.f)) return none;
+    if (lobeDelta(l)) {
+        r.f = r.f * w[chosen];
+        r.pdf *= s[chosen] / total;
+        return r;
+    }
+    r.f = mixEval(m, wo, r.wi);
+    r.pdf = mixPdf(m, wo, r.wi);
+    if (r.pdf <= 0.0f) return none;
+    return r;
+}
+
+// sampling::powerHeuristic (sampler.h:135-149)
+__device__ float powerHeuristic(float f, float g) {
+    if (isinf(f)) return 1.0f;
+    if (isinf(g)) return 0.0f;
+    const float m = fmaxf(f, g);
+    if (m <= 0.0f) return 0.0f;
+    f /= m;
+    g /= m;
+    return (f * f) / (f * f + g * g);
+}
+
+// ---------------------------------------------------------------- lights and media
+
+struct LSample {
+    F3 wi, Li;
+    float pdf, dist;
+};
+
+__device__ LSample sampleLight(const DLight& l, F3 p, float u1, float u2) {
+    LSample s{mk(0, 0, 0), mk(0, 0, 0), 0.0f, 0.0f};
+    if (l.type == RBR_LIGHT_AREA) {   // area_light.cpp:13-39
+        const F3 lp = ld3(l.position) + ld3(l.u) * (u1 - 0.5f) + ld3(l.v) * (u2 - 0.5f);
+        const F3 to = lp - p;
+        const float d2 = dot(to, to);
+        s.dist = sqrtf(d2);
+        s.wi = to * (1.0f / s.dist);
+        const float c = -dot(ld3(l.normal), s.wi);
+        if (c <= 0.0f) return s;
+        s.pdf = d2 / (c * l.area);
+        s.Li = ld3(l.color) * l.intensity;
+    } else if (l.type == RBR_LIGHT_POINT) {   // point_light.cpp:5-19
+        const F3 to = ld3(l.position) - p;
+        const float d2 = dot(to, to);
+        s.dist = sqrtf(d2);
+        s.wi = to * (1.0f / s.dist);
+        s.Li = ld3(l.color) * l.intensity * (1.0f / d2);
+        s.pdf = 1.0f;
+    } else {   // directional_light.cpp:5-15
+        s.wi = -ld3(l.direction);
+        s.dist = INFINITY;
+        s.Li = ld3(l.color) * l.intensity;
+        s.pdf = 1.0f;
+    }
+    return s;
+}
+
+__device__ __forceinline__ float densityScale(const DMedium& m, F3 p) {
+    return expf(-m.falloff * fmaxf(0.0f, p.y - m.base));
+}
+
+// HomogeneousMedium / ExponentialMedium::transmittance (medium.cpp:35-43, 87-103)
+__device__ F3 transmittance(const DMedium& m, F3 o, F3 d, float dist) {
+    const F3 st = ld3(m.sa) + ld3(m.ss);
+    if (m.type == RBR_MEDIUM_EXPONENTIAL) {
+        const float step = dist / 64.0f;
+        float od = 0.0f;
+        for (int i = 0; i < 64; ++i) od += densityScale(m, o + d * ((float(i) + 0.5f) * step)) * step;
+        return mk(expf(-st.x * od), expf(-st.y * od), expf(-st.z * od));
+    }
+    return mk(expf(-st.x * dist), expf(-st.y * dist), expf(-st.z * dist));
+}
+
+// sampleDistance (medium.cpp:9-33, 61-85).
+// The exponential medium's march re-uses the same u at every step (medium.cpp:68-84), so
+// its steps are a constant delta = -ln(1-u)/avg and a step at height y is accepted iff
+// u < exp(-falloff*max(0, y - base)), i.e. iff y < base + ln(1/u)/falloff.  Heights along
+// the ray are linear in the step index, so the first accepted step is found in closed form
+// instead of by iterating (the reference's loop runs ~maxDist/delta times and never ends
+// for a ray with no surface behind it; such a ray ends here with zero weight).
+__device__ bool sampleDistance(const DMedium& m, F3 o, F3 d, float maxDist, float u, float& t, F3& w) {
+    const F3 st = ld3(m.sa) + ld3(m.ss);
+    const float avg = (st.x + st.y + st.z) / 3.0f;
+    if (m.type == RBR_MEDIUM_EXPONENTIAL) {
+        const float step = -logf(1.0f - u) / avg;
+        const F3 ratio = mk(m.ss[0] / st.x, m.ss[1] / st.y, m.ss[2] / st.z);   // sigma_s(p) / sigma_t(p)
+        bool accept = false;
+        if (step == 0.0f) {   // u == 0: the march stays at t = 0, where u < density always holds
+            t = 0.0f;
+            accept = true;
+        } else if (m.falloff <= 0.0f || u <= 0.0f) {
+            t = step;
+            accept = true;
+        } else {
+            const float ystar = m.base + (-logf(u)) / m.falloff;
+            const float y1 = o.y + d.y * step;
+            if (y1 < ystar) {
+                t = step;
+                accept = true;
+            } else if (d.y < 0.0f) {
+                const float k = ceilf((ystar - o.y) / (d.y * step));
+                t = fmaxf(k, 1.0f) * step;
+                if (o.y + d.y * t >= ystar) t += step;
+                accept = true;
+            }
+        }
+        if (accept && t < maxDist) {
+            w = ratio;
+            return true;
+        }
+        w = isinf(maxDist) ? mk(0, 0, 0) : transmittance(m, o, d, maxDist);
+        return false;
+    }
+    if (avg < 1e-6f) {
+        w = mk(1, 1, 1);
+        return false;
+    }
+    t = -logf(1.0f - u) / avg;
+    if (t < maxDist) {
+        const F3 tr = transmittance(m, o, d, t);
+        w = ld3(m.ss) * tr * (1.0f / (avg * avg));
+        return true;
+    }
+    w = transmittance(m, o, d, maxDist);
+    return false;
+}
+
+// HenyeyGreensteinPhase (phase_function.cpp:17-47), wo = propagation direction.
+__device__ float hgEval(float g, F3 wo, F3 wi) {
+    const float c = dot(wo, wi);
+    const float den = 1.0f + g * g + 2.0f * g * c;
+    return (1.0f - g * g) / (4.0f * kPi * den * sqrtf(den));
+}
+__device__ F3 hgSample(float g, F3 wo, float u1, float u2) {
+    float c;
+    if (fabsf(g) < 1e-3f) c = 1.0f - 2.0f * u1;
+    else {
+        const float sq = (1.0f - g * g) / (1.0f + g - 2.0f * g * u1);
+        c = -(1.0f + g * g - sq * sq) / (2.0f * g);
+    }
+    const float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+    float sp, cp;
+    sincosf(kTwoPi * u2, &sp, &cp);
+    Frame f(wo);
+    return f.t * (s * cp) + f.b * (s * sp) + wo * c;
+}
+
+// ---------------------------------------------------------------- queues
+
+struct Queues {
+    // extension queue (ping-pong): rays, hits, path slot, BVH root
+    float4* ray[2];
+    uint32_t* path[2];
+    uint32_t* root[2];
+    uint2* hit;
+    // shadow queue
+    float4* sray;
+    uint32_t* sroot;
+    float4* scontrib;
+    uint8_t* socc;
+    int32_t* sindex;      // [path * maxLights + light] -> shadow entry or -1
+    // path state
+    float4* T;            // throughput, w = specular-bounce flag
+    float4* L;            // radiance
+    uint32_t* counts;     // [0],[1] extension queues, [2] shadow queue
+};
+
+__device__ __forceinline__ uint32_t waveAppend(uint32_t* counter, bool want) {
+    const unsigned long long m = __ballot(want ? 1 : 0);
+    if (m == 0ull) return 0u;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(m)));
+    base = __shfl(base, leader);
+    return base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+}
+
+__device__ __forceinline__ int findJob(const DJob* jobs, int njobs, uint32_t p) {
+    int j = 0;
+    while (j + 1 < njobs && p >= jobs[j + 1].pathBase) ++j;
+    return j;
+}
+
+__device__ __forceinline__ void putRay(float4* q, uint32_t i, F3 o, F3 d, float tmin, float tmax) {
+    q[2 * size_t(i)] = make_float4(o.x, o.y, o.z, d.x);
+    q[2 * size_t(i) + 1] = make_float4(d.y, d.z, tmin, tmax);
+}
+
+// spawnRay (interaction.h:32-39): offset along the ray-facing normal toward the new
+// direction; the direction is re-normalized as Ray's constructor does.
+__device__ __forceinline__ void spawn(F3 p, F3 n, F3 dir, F3& o, F3& d) {
+    F3 off = n * kEpsD;
+    if (dot(dir, n) < 0.0f) off = -off;
+    o = p + off;
+    d = normalize(dir);
+}
+
+// ---------------------------------------------------------------- kernels
+
+// Integrator::render's per-sample camera ray (integrator.cpp:22-31, camera.cpp:5-9).
+__global__ __launch_bounds__(256) void cameraKernel(const DJob* __restrict__ jobs, int njobs, uint32_t total, Queues q) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i == 0) q.counts[0] = total;
+    if (i >= total) return;
+    const int j = findJob(jobs, njobs, i);
+    const DJob& jb = jobs[j];
+    const uint32_t local = i - jb.pathBase;
+    const uint32_t pixel = local % jb.pixels;
+    const uint32_t sample = jb.sampleBase + local / jb.pixels;
+    const Rng rng = pathRng(jb.seed, pixel, sample, jb.pixels);
+    const DCamera& c = jb.cam;
+    const float px = float(pixel % c.width) + rng.u(0);
+    const float py = float(pixel / c.width) + rng.u(1);
+    const F3 local3 = mk((px - c.cx) / c.fx, (c.cy - py) / c.fy, 1.0f);
+    // normalized in generateIntrinsicRay and again by Ray's constructor (camera.cpp:5-9)
+    const F3 d = normalize(normalize(ld3(c.right) * local3.x + ld3(c.up) * local3.y + ld3(c.fwd) * local3.z));
+    putRay(q.ray[0], i, ld3(c.pos), d, kEpsD, INFINITY);
+    q.path[0][i] = i;
+    q.root[0][i] = jb.root;
+    q.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    q.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// One bounce of shading for every path in the current extension queue.
+__global__ __launch_bounds__(256) void shadeKernel(const DJob* __restrict__ jobs, int njobs, uint32_t depth,
+                                                   uint32_t maxLights, int cur, Queues q) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t count = q.counts[cur];
+    const bool active = i < count;
+    const int nxt = cur ^ 1;
+
+    uint32_t p = 0;
+    int j = 0;
+    F3 T = mk(0, 0, 0), L = mk(0, 0, 0);
+    bool spec = false;
+    // mode: 0 done, 1 surface, 2 volume scatter
+    int mode = 0;
+    F3 P = mk(0, 0, 0), n = mk(0, 0, 0), outward = mk(0, 0, 0), rd = mk(0, 0, 0);
+    int mat = -1;
+    Rng rng{0};
+    uint32_t base = dimBase(depth);
+    bool nee = false;
+
+    if (active) {
+        p = q.path[cur][i];
+        j = findJob(jobs, njobs, p);
+        const DJob& jb = jobs[j];
+        const float4 r0 = q.ray[cur][2 * size_t(i)], r1 = q.ray[cur][2 * size_t(i) + 1];
+        const F3 ro = mk(r0.x, r0.y, r0.z);
+        rd = mk(r0.w, r1.x, r1.y);
+        const uint2 h = q.hit[i];
+        const float4 t4 = q.T[p], l4 = q.L[p];
+        T = mk(t4.x, t4.y, t4.z);
+        spec = t4.w != 0.0f;
+        L = mk(l4.x, l4.y, l4.z);
+        const uint32_t local = p - jb.pathBase;
+        rng = pathRng(jb.seed, local % jb.pixels, jb.sampleBase + local / jb.pixels, jb.pixels);
+        for (uint32_t l = 0; l < maxLights; ++l) q.sindex[size_t(p) * maxLights + l] = -1;
+
+        const bool hitSurf = h.x != RBR_MISS_ID;
+        const float th = __uint_as_float(h.y);
+        bool scattered = false;
+        if (jb.integrator == RBR_INTEGRATOR_VOLPATH && jb.medium.type != RBR_MEDIUM_NONE) {
+            float ts;
+            F3 w;
+            scattered = sampleDistance(jb.medium, ro, rd, hitSurf ? th : INFINITY, rng.u(base + 3), ts, w);
+            T = T * w;
+            if (scattered) {
+                P = ro + rd * ts;
+                mode = 2;
+                nee = jb.useNEE != 0;
+            }
+        }
+        if (!scattered) {
+            if (!hitSurf) {
+                L = L + T * ld3(jb.bg);
+            } else {
+                const float4 s0 = jb.primShade[2 * size_t(h.x)], s1 = jb.primShade[2 * size_t(h.x) + 1];
+                mat = __float_as_int(s1.w);
+                if (mat >= 0) {
+                    P = ro + rd * th;
+                    const uint32_t kind = __float_as_uint(s0.w);
+                    outward = kind == 1u ? (P - mk(s0.x, s0.y, s0.z)) * (1.0f / s1.x) : mk(s0.x, s0.y, s0.z);
+                    const bool front = dot(rd, outward) < 0.0f;
+                    n = front ? outward : -outward;
+                    const DMaterial& m = jb.mats[mat];
+                    const bool vol = jb.integrator == RBR_INTEGRATOR_VOLPATH;
+                    if (m.emissive && (vol || depth == 0 || spec)) L = L + T * ld3(m.emission);
+                    mode = 1;
+                    nee = jb.useNEE && !m.allDelta;
+                }
+            }
+        }
+    }
+
+    // Next-event estimation: one shadow ray per light, appended wave by wave.
+    const DJob& jb = jobs[j];
+    const bool volInt = active && jb.integrator == RBR_INTEGRATOR_VOLPATH;
+    Frame frame(mode == 1 ? outward : mk(0, 0, 1));
+    const F3 woL = frame.toLocal(-rd);
+    for (uint32_t l = 0; l < maxLights; ++l) {
+        bool want = false;
+        F3 so = mk(0, 0, 0), sd = mk(0, 0, 0), contrib = mk(0, 0, 0);
+        float smax = 0.0f;
+        if (active && nee && l < jb.nLights) {
+            const DLight& lt = jb.lights[l];
+            if (mode == 2) {   // directLightingVolume: point lights only, no offset
+                if (lt.type == RBR_LIGHT_POINT) {
+                    const F3 to = ld3(lt.position) - P;
+                    const float dist = len(to);
+                    const F3 wi = to * (1.0f / dist);
+                    so = P;
+                    sd = normalize(wi);
+                    smax = dist;
+                    const F3 tr = transmittance(jb.medium, so, sd, dist);
+                    const F3 Li = ld3(lt.color) * lt.intensity * (1.0f / (dist * dist));
+                    contrib = T * Li * tr * hgEval(jb.medium.g, rd, wi);
+                    want = true;
+                }
+            } else {
+                const LSample ls = sampleLight(lt, P, rng.u(base + 8 + 2 * l), rng.u(base + 9 + 2 * l));
+                if (ls.pdf > 0.0f && !black(ls.Li)) {
+                    const DMaterial& m = jb.mats[mat];
+                    const F3 wiL = frame.toLocal(ls.wi);
+                    const F3 f = mixEval(m, woL, wiL);
+                    if (!black(f)) {
+                        const float c = fabsf(dot(ls.wi, n));
+                        float w = 1.0f;
+                        if (!volInt && jb.useMIS && lt.type == RBR_LIGHT_AREA) w = powerHeuristic(ls.pdf, mixPdf(m, woL, wiL));
+                        contrib = T * f * ls.Li * (c * w / ls.pdf);
+                        spawn(P, n, ls.wi, so, sd);
+                        smax = ls.dist - kEpsD;
+                        want = true;
+                    }
+                }
+            }
+        }
+        const uint32_t slot = waveAppend(&q.counts[2], want);
+        if (want) {
+            putRay(q.sray, slot, so, sd, kEpsD, smax);
+            q.sroot[slot] = jb.root;
+            q.scontrib[slot] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+            q.sindex[size_t(p) * maxLights + l] = int32_t(slot);
+        }
+    }
+
+    // Continuation: phase or BSDF sample, Russian roulette, spawn.
+    bool cont = false;
+    F3 no = mk(0, 0, 0), nd = mk(0, 0, 0);
+    if (mode == 2) {
+        nd = normalize(hgSample(jb.medium.g, rd, rng.u(base + 4), rng.u(base + 5)));
+        no = P;
+        cont = depth + 1 < jb.maxDepth;
+    } else if (mode == 1) {
+        const DMaterial& m = jb.mats[mat];
+        const BSample bs = mixSample(m, woL, rng.u(base + 0), rng.u(base + 1));
+        if (bs.pdf > 0.0f && !black(bs.f)) {
+            const F3 wi = frame.toWorld(bs.wi);
+            T = T * bs.f * (fabsf(dot(wi, n)) / bs.pdf);
+            bool alive = true;
+            if (depth > 3) {
+                const float qq = fmaxf(0.05f, 1.0f - maxc(T));
+                if (rng.u(base + 2) < qq) alive = false;
+                else T = T * (1.0f / (1.0f - qq));
+            }
+            if (alive) {
+                spec = bs.delta;
+                spawn(P, n, wi, no, nd);
+                cont = depth + 1 < jb.maxDepth;
+            }
+        }
+    }
+    if (active) {
+        q.L[p] = make_float4(L.x, L.y, L.z, 0.0f);
+        if (cont) q.T[p] = make_float4(T.x, T.y, T.z, spec ? 1.0f : 0.0f);
+    }
+    const uint32_t slot = waveAppend(&q.counts[nxt], cont);
+    if (cont) {
+        putRay(q.ray[nxt], slot, no, nd, kEpsD, INFINITY);
+        q.path[nxt][slot] = p;
+        q.root[nxt][slot] = jb.root;
+    }
+}
+
+// Adds each shaded path's unoccluded NEE terms in light order.
+__global__ __launch_bounds__(256) void gatherKernel(uint32_t maxLights, int cur, Queues q) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= q.counts[cur]) return;
+    const uint32_t p = q.path[cur][i];
+    float4 l4 = q.L[p];
+    for (uint32_t l = 0; l < maxLights; ++l) {
+        const int32_t s = q.sindex[size_t(p) * maxLights + l];
+        if (s >= 0 && !q.socc[s]) {
+            const float4 c = q.scontrib[s];
+            l4.x += c.x;
+            l4.y += c.y;
+            l4.z += c.z;
+        }
+    }
+    q.L[p] = l4;
+}
+
+__global__ __launch_bounds__(256) void resetCountsKernel(uint32_t* counts, int nxt) {
+    counts[nxt] = 0u;
+    counts[2] = 0u;
+}
+
+// Adds this round's samples of one job to its film (and optional moments), in sample order.
+__global__ __launch_bounds__(256) void accumulateKernel(const DJob* __restrict__ jobs, int j, uint32_t samples,
+                                                        const float4* __restrict__ L) {
+    const DJob& jb = jobs[j];
+    const uint32_t px = blockIdx.x * 256u + threadIdx.x;
+    if (px >= jb.pixels) return;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    double mx = 0, my = 0, mz = 0, qx = 0, qy = 0, qz = 0;
+    for (uint32_t s = 0; s < samples; ++s) {
+        const float4 v = L[jb.pathBase + size_t(s) * jb.pixels + px];
+        sx += v.x;
+        sy += v.y;
+        sz += v.z;
+        if (jb.momSum) {
+            mx += v.x; my += v.y; mz += v.z;
+            qx += double(v.x) * v.x; qy += double(v.y) * v.y; qz += double(v.z) * v.z;
+        }
+    }
+    jb.filmSum[3 * size_t(px) + 0] += sx;
+    jb.filmSum[3 * size_t(px) + 1] += sy;
+    jb.filmSum[3 * size_t(px) + 2] += sz;
+    if (jb.momSum) {
+        jb.momSum[3 * size_t(px) + 0] += mx;
+        jb.momSum[3 * size_t(px) + 1] += my;
+        jb.momSum[3 * size_t(px) + 2] += mz;
+        jb.momSq[3 * size_t(px) + 0] += qx;
+        jb.momSq[3 * size_t(px) + 1] += qy;
+        jb.momSq[3 * size_t(px) + 2] += qz;
+    }
+}
+
+__global__ void zeroKernel(float* p, size_t n) {
+    for (size_t i = blockIdx.x * size_t(256) + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) p[i] = 0.0f;
+}
+__global__ void zeroKernelD(double* p, size_t n) {
+    for (size_t i = blockIdx.x * size_t(256) + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------- host side
+
+DMaterial makeMaterial(const rbr_material& in) {
+    DMaterial m{};
+    const float metallic = std::min(std::max(in.metallic, 0.0f), 1.0f);
+    const float t = std::min(std::max(in.transmission, 0.0f), 1.0f);
+    const float dielectric = 1.0f - metallic;
+    const bool smooth = in.roughness < 0.03f;
+    const bool hasInterface = std::fabs(in.ior - 1.0f) > 1e-3f;
+    const bool tinted = !(in.base_color[0] <= 0.0f && in.base_color[1] <= 0.0f && in.base_color[2] <= 0.0f);
+    const float alpha = std::max(in.roughness * in.roughness, 1e-3f);
+    auto add = [&](DLobe l, float w, bool substrate) {
+        if (w <= 0.0f || m.n >= 6) return;
+        l.weight = w;
+        if (substrate) m.substrateMask |= 1u << m.n;
+        m.lobe[m.n++] = l;
+    };
+    auto reflLobe = [&](uint32_t fk, float etaT, const float* f0) {
+        DLobe l{};
+        l.type = smooth ? DL_SPEC_REFL : DL_MF_REFL;
+        l.R[0] = l.R[1] = l.R[2] = 1.0f;
+        l.fkind = fk;
+        l.fEtaI = 1.0f;
+        l.fEtaT = etaT;
+        for (int k = 0; k < 3; ++k) l.f0[k] = f0 ? f0[k] : 1.0f;
+        l.alpha = alpha;
+        return l;
+    };
+    if (metallic > 0.0f && tinted) add(reflLobe(DF_SCHLICK, 1.0f, in.base_color), metallic, false);
+    if (dielectric > 0.0f) {
+        if (hasInterface) {
+            add(reflLobe(DF_DIELECTRIC, in.ior, nullptr), dielectric, false);
+            m.sfKind = DF_DIELECTRIC;
+            m.sfEtaI = 1.0f;
+            m.sfEtaT = in.ior;
+            if (t > 0.0f && tinted) {
+                DLobe l{};
+                l.type = smooth ? DL_SPEC_TRANS : DL_MF_TRANS;
+                for (int k = 0; k < 3; ++k) l.R[k] = in.base_color[k];
+                l.etaOut = 1.0f;
+                l.etaIn = in.ior;
+                l.alpha = alpha;
+                add(l, dielectric * t, false);
+            }
+            if (t < 1.0f && tinted) {
+                DLobe l{};
+                l.type = DL_LAMBERT;
+                for (int k = 0; k < 3; ++k) l.R[k] = in.base_color[k];
+                add(l, dielectric * (1.0f - t), true);
+            }
+        } else {
+            if (t > 0.0f && tinted) {
+                DLobe l{};
+                l.type = DL_SPEC_TRANS;
+                for (int k = 0; k < 3; ++k) l.R[k] = in.base_color[k];
+                l.etaOut = l.etaIn = 1.0f;
+                add(l, dielectric * t, false);
+            }
+            if (t < 1.0f && tinted) {
+                DLobe l{};
+                l.type = DL_LAMBERT;
+                for (int k = 0; k < 3; ++k) l.R[k] = in.base_color[k];
+                add(l, dielectric * (1.0f - t), false);
+            }
+        }
+    }
+    m.allDelta = m.n > 0 ? 1u : 0u;
+    for (uint32_t i = 0; i < m.n; ++i)
+        if (m.lobe[i].type != DL_SPEC_REFL && m.lobe[i].type != DL_SPEC_TRANS) m.allDelta = 0u;
+    m.emissive = (in.emission[0] > 0.0f || in.emission[1] > 0.0f || in.emission[2] > 0.0f) ? 1u : 0u;
+    for (int k = 0; k < 3; ++k) m.emission[k] = in.emission[k];
+    return m;
+}
+
+}  // namespace
+
+// Per-context render state: the path pool and queues, grown on demand.
+struct RenderState {
+    DevBuf ray[2], path[2], root[2], hit, sray, sroot, scontrib, socc, sindex, T, L, counts, jobs;
+    DevBuf arenaNodes, arenaPrims, arenaQuadN, arenaShade, arenaMats, arenaLights;
+    std::vector<DevBuf> films, momS, momQ;
+    double lastTotalMs = 0.0;
+    double stageMs[4] = {0, 0, 0, 0};
+    uint32_t poolCap = 1u << 23;
+    std::vector<hipEvent_t> events;
+};
+
+}  // namespace rbr
 
 using namespace rbr;
 
@@ -11,25 +925,435 @@ void rbr::freeShading(rbr_scene* s) {
     freeBuffer(s->dLights);
 }
 
-void rbr::destroyRenderState(rbr_ctx*) {}
+void rbr::destroyRenderState(rbr_ctx* ctx) {
+    RenderState* r = ctx->render;
+    if (!r) return;
+    for (DevBuf* b : {&r->ray[0], &r->ray[1], &r->path[0], &r->path[1], &r->root[0], &r->root[1], &r->hit, &r->sray,
+                      &r->sroot, &r->scontrib, &r->socc, &r->sindex, &r->T, &r->L, &r->counts, &r->jobs,
+                      &r->arenaNodes, &r->arenaPrims, &r->arenaQuadN, &r->arenaShade, &r->arenaMats, &r->arenaLights})
+        freeBuffer(*b);
+    for (auto* v : {&r->films, &r->momS, &r->momQ})
+        for (DevBuf& b : *v) freeBuffer(b);
+    for (hipEvent_t e : r->events) (void)hipEventDestroy(e);
+    delete r;
+    ctx->render = nullptr;
+}
 
 extern "C" {
 
-int rbr_scene_set_shading(rbr_scene* s, const rbr_material*, uint32_t, const int32_t*, uint32_t, const rbr_light*,
-                          uint32_t, const rbr_medium*, const float[3]) {
-    return s ? fail(s->ctx, RBR_ERR_UNSUPPORTED, "shading not built yet") : RBR_ERR_INVALID;
+int rbr_scene_set_shading(rbr_scene* s, const rbr_material* materials, uint32_t nmat, const int32_t* primMaterial,
+                          uint32_t nprimIds, const rbr_light* lights, uint32_t nlights, const rbr_medium* medium,
+                          const float background[3]) {
+    if (!s) return RBR_ERR_INVALID;
+    rbr_ctx* ctx = s->ctx;
+    const HostGeometry& g = s->geom;
+    const uint32_t needIds = g.anyId ? g.maxId + 1 : 0;
+    if (nprimIds < needIds || (nprimIds && !primMaterial)) return fail(ctx, RBR_ERR_INVALID, "prim_material must cover every primitive id");
+    if ((nmat && !materials) || (nlights && !lights)) return fail(ctx, RBR_ERR_INVALID, "null material or light array");
+    if (nlights > kMaxLights) return fail(ctx, RBR_ERR_UNSUPPORTED, "more than 32 lights");
+    for (uint32_t i = 0; i < nprimIds; ++i)
+        if (primMaterial[i] < -1 || primMaterial[i] >= int32_t(nmat)) return fail(ctx, RBR_ERR_INVALID, "material index out of range");
+    if (medium && (medium->type < RBR_MEDIUM_NONE || medium->type > RBR_MEDIUM_EXPONENTIAL))
+        return fail(ctx, RBR_ERR_INVALID, "bad medium type");
+    for (uint32_t i = 0; i < nlights; ++i)
+        if (lights[i].type < RBR_LIGHT_AREA || lights[i].type > RBR_LIGHT_DIRECTIONAL) return fail(ctx, RBR_ERR_INVALID, "bad light type");
+    s->materials.assign(materials, materials + nmat);
+    s->primMaterial.assign(primMaterial, primMaterial + nprimIds);
+    s->lights.assign(lights, lights + nlights);
+    s->medium = medium ? *medium : rbr_medium{};
+    for (int k = 0; k < 3; ++k) s->background[k] = background ? background[k] : 0.0f;
+    s->shaded = true;
+    return RBR_OK;
 }
 
-int rbr_render(rbr_ctx* ctx, const rbr_job*, int) {
-    return ctx ? fail(ctx, RBR_ERR_UNSUPPORTED, "render not built yet") : RBR_ERR_INVALID;
+}  // extern "C"
+
+namespace {
+
+// Shading records by primitive id: (outward normal or sphere centre, kind) (radius, -, -, material).
+void buildShade(const rbr_scene* s, uint32_t idBase, int32_t matBase, std::vector<F4>& out) {
+    const HostGeometry& g = s->geom;
+    const uint32_t n = g.anyId ? g.maxId + 1 : 0;
+    const size_t o = out.size();
+    out.resize(o + 2 * size_t(n));
+    auto asF = [](uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; };
+    for (uint32_t i = 0; i < n; ++i) {
+        out[o + 2 * i] = {0, 0, 1, asF(0xFFu)};
+        out[o + 2 * i + 1] = {0, 0, 0, asF(0xFFFFFFFFu)};
+    }
+    auto mat = [&](uint32_t id) -> float {
+        int32_t m = id < s->primMaterial.size() ? s->primMaterial[id] : -1;
+        return asF(uint32_t(m < 0 ? -1 : m + matBase));
+    };
+    for (size_t i = 0; i < g.triIds.size(); ++i) {
+        const float* t = &g.tris[9 * i];
+        // Triangle geometric normal: cross(v1-v0, v2-v0).normalized() (triangle.cpp:33).
+        float e1[3] = {t[3] - t[0], t[4] - t[1], t[5] - t[2]}, e2[3] = {t[6] - t[0], t[7] - t[1], t[8] - t[2]};
+        float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        float l = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        float inv = l > 0.0f ? 1.0f / l : 0.0f;
+        const uint32_t id = g.triIds[i];
+        out[o + 2 * id] = {c[0] * inv, c[1] * inv, c[2] * inv, asF(0u)};
+        out[o + 2 * id + 1] = {0, 0, 0, mat(id)};
+    }
+    for (size_t i = 0; i < g.sphIds.size(); ++i) {
+        const float* sp = &g.sphs[4 * i];
+        const uint32_t id = g.sphIds[i];
+        out[o + 2 * id] = {sp[0], sp[1], sp[2], asF(1u)};
+        out[o + 2 * id + 1] = {sp[3], 0, 0, mat(id)};
+    }
+    for (size_t i = 0; i < g.quadIds.size(); ++i) {
+        const float* qd = &g.quads[12 * i];
+        const uint32_t id = g.quadIds[i];
+        out[o + 2 * id] = {qd[9], qd[10], qd[11], asF(2u)};
+        out[o + 2 * id + 1] = {0, 0, 0, mat(id)};
+    }
+    for (size_t i = 0; i < g.planeIds.size(); ++i) {
+        const float* pl = &g.planes[4 * i];
+        const uint32_t id = g.planeIds[i];
+        out[o + 2 * id] = {pl[0], pl[1], pl[2], asF(3u)};
+        out[o + 2 * id + 1] = {0, 0, 0, mat(id)};
+    }
+    (void)idBase;
 }
 
-int rbr_render_moments(rbr_ctx* ctx, const rbr_job*, double*, double*) {
-    return ctx ? fail(ctx, RBR_ERR_UNSUPPORTED, "render not built yet") : RBR_ERR_INVALID;
+DLight makeLight(const rbr_light& l) {
+    DLight d{};
+    d.type = uint32_t(l.type);
+    for (int k = 0; k < 3; ++k) {
+        d.color[k] = l.color[k];
+        d.position[k] = l.position[k];
+        d.u[k] = l.u[k];
+        d.v[k] = l.v[k];
+        d.normal[k] = l.normal[k];
+        d.direction[k] = l.direction[k];
+    }
+    d.intensity = l.intensity;
+    const float c[3] = {l.u[1] * l.v[2] - l.u[2] * l.v[1], l.u[2] * l.v[0] - l.u[0] * l.v[2], l.u[0] * l.v[1] - l.u[1] * l.v[0]};
+    d.area = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    return d;
 }
 
-int rbr_last_render_timing(rbr_ctx* ctx, double*, double*) {
-    return ctx ? fail(ctx, RBR_ERR_UNSUPPORTED, "render not built yet") : RBR_ERR_INVALID;
+struct SceneSlot {
+    const rbr_scene* s;
+    uint32_t nodeOff, primOff, quadOff, idOff, matOff, lightOff;
+};
+
+}  // namespace
+
+namespace {
+
+int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momSum, double* const* momSq) {
+    if (!ctx || !jobs || njobs < 1) return RBR_ERR_INVALID;
+    if (njobs > 64) return fail(ctx, RBR_ERR_UNSUPPORTED, "at most 64 jobs per render");
+    if (int rc = useDevice(ctx)) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    // ---- validate jobs and collect distinct scenes
+    std::vector<SceneSlot> scenes;
+    std::map<const rbr_scene*, int> sceneIdx;
+    int width = 0;
+    uint32_t maxStack = 1, maxDepthAll = 0, maxLights = 0;
+    bool anyMixed = false, anyPlanes = false;
+    for (int j = 0; j < njobs; ++j) {
+        const rbr_job& jb = jobs[j];
+        if (!jb.scene || jb.scene->ctx != ctx) return fail(ctx, RBR_ERR_INVALID, "job scene missing or from another context");
+        if (!jb.scene->built || !jb.scene->shaded) return fail(ctx, RBR_ERR_STATE, "job scene needs rbr_scene_build_bvh and rbr_scene_set_shading");
+        if (!jb.film_rgb) return fail(ctx, RBR_ERR_INVALID, "job film is null");
+        if (jb.camera.width < 1 || jb.camera.height < 1 || jb.config.spp < 1 || jb.config.max_depth < 1)
+            return fail(ctx, RBR_ERR_INVALID, "job needs width, height, spp and max_depth >= 1");
+        if (jb.config.integrator != RBR_INTEGRATOR_PATH && jb.config.integrator != RBR_INTEGRATOR_VOLPATH)
+            return fail(ctx, RBR_ERR_INVALID, "unknown integrator");
+        if (jb.config.guided) return fail(ctx, RBR_ERR_UNSUPPORTED, "guided rendering goes through rbr_guide_render");
+        if (uint64_t(jb.camera.width) * jb.camera.height > (1ull << 26)) return fail(ctx, RBR_ERR_INVALID, "film too large");
+        if (sceneIdx.find(jb.scene) == sceneIdx.end()) {
+            sceneIdx[jb.scene] = int(scenes.size());
+            scenes.push_back({jb.scene, 0, 0, 0, 0, 0, 0});
+        }
+        const rbr_scene* s = jb.scene;
+        if (!s->bvh.empty()) {
+            if (width && width != s->bvh.width) return fail(ctx, RBR_ERR_UNSUPPORTED, "jobs mix BVH widths");
+            width = s->bvh.width;
+            maxStack = std::max(maxStack, uint32_t(s->bvh.width - 1) * s->bvh.maxDepth + 1u);
+        }
+        anyMixed |= (s->geom.sphIds.size() + s->geom.quadIds.size()) > 0;
+        anyPlanes |= !s->geom.planeIds.empty();
+        maxDepthAll = std::max(maxDepthAll, uint32_t(jb.config.max_depth));
+        maxLights = std::max(maxLights, uint32_t(s->lights.size()));
+    }
+    if (width == 0) width = 4;
+    if (anyPlanes && scenes.size() > 1) return fail(ctx, RBR_ERR_UNSUPPORTED, "infinite planes in a multi-scene render");
+    if (!ctx->render) ctx->render = new RenderState();
+    RenderState& R = *ctx->render;
+    const int per = width == 4 ? 8 : 16;
+
+    // ---- node/primitive arena (packed only when several scenes share the launch)
+    std::vector<F4> shade;
+    std::vector<DMaterial> mats;
+    std::vector<DLight> lights;
+    TraceArgs ta{};
+    int rc;
+    uint32_t nodeOff = 0, primOff = 0, quadOff = 0, idOff = 0;
+    for (SceneSlot& sl : scenes) {
+        sl.nodeOff = nodeOff;
+        sl.primOff = primOff;
+        sl.quadOff = quadOff;
+        sl.idOff = idOff;
+        sl.matOff = uint32_t(mats.size());
+        sl.lightOff = uint32_t(lights.size());
+        buildShade(sl.s, idOff, int32_t(sl.matOff), shade);
+        for (const rbr_material& m : sl.s->materials) mats.push_back(makeMaterial(m));
+        for (const rbr_light& l : sl.s->lights) lights.push_back(makeLight(l));
+        nodeOff += sl.s->bvh.nodeCount;
+        primOff += sl.s->bvh.primCount;
+        quadOff += uint32_t(sl.s->bvh.quadNormals.size());
+        idOff += sl.s->geom.anyId ? sl.s->geom.maxId + 1 : 0;
+    }
+    if (primOff >= (1u << 27)) return fail(ctx, RBR_ERR_UNSUPPORTED, "arena exceeds 2^27 primitives");
+    if (scenes.size() == 1) {
+        ta = sceneTraceArgs(scenes[0].s);
+    } else {
+        std::vector<F4> nodes(size_t(nodeOff) * per), prims(size_t(primOff) * 3), quadN(quadOff);
+        for (const SceneSlot& sl : scenes) {
+            const WideBVH& b = sl.s->bvh;
+            std::memcpy(&nodes[size_t(sl.nodeOff) * per], b.nodes.data(), b.nodes.size() * sizeof(F4));
+            for (uint32_t nd = 0; nd < b.nodeCount; ++nd) {
+                uint32_t* ch = reinterpret_cast<uint32_t*>(&nodes[size_t(sl.nodeOff + nd) * per]) + 6 * width;
+                for (int k = 0; k < width; ++k) {
+                    if (ch[k] == kEmpty) continue;
+                    if (isLeaf(ch[k])) ch[k] = makeLeaf(leafStart(ch[k]) + sl.primOff, leafCount(ch[k]));
+                    else ch[k] += sl.nodeOff;
+                }
+            }
+            for (uint32_t pi = 0; pi < b.primCount; ++pi) {
+                F4* r = &prims[size_t(sl.primOff + pi) * 3];
+                std::memcpy(r, &b.prims[size_t(pi) * 3], 3 * sizeof(F4));
+                uint32_t id, kind;
+                std::memcpy(&id, &r[0].w, 4);
+                id += sl.idOff;
+                std::memcpy(&r[0].w, &id, 4);
+                std::memcpy(&kind, &r[1].w, 4);
+                if (kind == KIND_QUAD) {
+                    uint32_t qi;
+                    std::memcpy(&qi, &r[2].w, 4);
+                    qi += sl.quadOff;
+                    std::memcpy(&r[2].w, &qi, 4);
+                }
+            }
+            std::copy(b.quadNormals.begin(), b.quadNormals.end(), quadN.begin() + sl.quadOff);
+        }
+        if ((rc = uploadBuffer(ctx, R.arenaNodes, nodes.data(), nodes.size() * sizeof(F4)))) return rc;
+        if ((rc = uploadBuffer(ctx, R.arenaPrims, prims.data(), prims.size() * sizeof(F4)))) return rc;
+        if ((rc = uploadBuffer(ctx, R.arenaQuadN, quadN.data(), quadN.size() * sizeof(F4)))) return rc;
+        ta.nodes = nodeOff ? static_cast<const float4*>(R.arenaNodes.ptr) : nullptr;
+        ta.prims = static_cast<const float4*>(R.arenaPrims.ptr);
+        ta.quadN = static_cast<const float4*>(R.arenaQuadN.ptr);
+        ta.nplanes = 0;
+        ta.mixed = anyMixed ? 1u : 0u;
+    }
+    if ((rc = uploadBuffer(ctx, R.arenaShade, shade.data(), std::max<size_t>(1, shade.size()) * sizeof(F4)))) return rc;
+    if ((rc = uploadBuffer(ctx, R.arenaMats, mats.data(), std::max<size_t>(1, mats.size()) * sizeof(DMaterial)))) return rc;
+    if ((rc = uploadBuffer(ctx, R.arenaLights, lights.data(), std::max<size_t>(1, lights.size()) * sizeof(DLight)))) return rc;
+
+    // ---- per-job device descriptors and films
+    std::vector<DJob> dj(njobs);
+    R.films.resize(std::max<size_t>(R.films.size(), size_t(njobs)));
+    R.momS.resize(std::max<size_t>(R.momS.size(), size_t(njobs)));
+    R.momQ.resize(std::max<size_t>(R.momQ.size(), size_t(njobs)));
+    uint64_t totalPaths = 0;
+    for (int j = 0; j < njobs; ++j) {
+        const rbr_job& jb = jobs[j];
+        const SceneSlot& sl = scenes[sceneIdx[jb.scene]];
+        DJob& d = dj[j];
+        d = DJob{};
+        for (int k = 0; k < 3; ++k) {
+            d.cam.pos[k] = jb.camera.position[k];
+            d.cam.fwd[k] = jb.camera.forward[k];
+            d.cam.right[k] = jb.camera.right[k];
+            d.cam.up[k] = jb.camera.up[k];
+            d.bg[k] = jb.scene->background[k];
+            d.medium.sa[k] = jb.scene->medium.sigma_a[k];
+            d.medium.ss[k] = jb.scene->medium.sigma_s[k];
+        }
+        d.cam.fx = jb.camera.fx;
+        d.cam.fy = jb.camera.fy;
+        d.cam.cx = jb.camera.cx;
+        d.cam.cy = jb.camera.cy;
+        d.cam.width = uint32_t(jb.camera.width);
+        d.cam.height = uint32_t(jb.camera.height);
+        d.medium.type = uint32_t(jb.scene->medium.type);
+        d.medium.falloff = jb.scene->medium.falloff;
+        d.medium.base = jb.scene->medium.base_height;
+        d.medium.g = jb.scene->medium.g;
+        d.primShade = static_cast<const float4*>(R.arenaShade.ptr);   // ids are arena-global
+        d.mats = static_cast<const DMaterial*>(R.arenaMats.ptr);
+        d.lights = static_cast<const DLight*>(R.arenaLights.ptr) + sl.lightOff;
+        d.nLights = uint32_t(jb.scene->lights.size());
+        d.integrator = uint32_t(jb.config.integrator);
+        d.maxDepth = uint32_t(jb.config.max_depth);
+        d.useNEE = jb.config.use_nee ? 1u : 0u;
+        d.useMIS = jb.config.use_mis ? 1u : 0u;
+        d.root = jb.scene->bvh.empty() ? kEmpty : sl.nodeOff;
+        d.pixels = uint32_t(jb.camera.width) * uint32_t(jb.camera.height);
+        d.seed = jb.config.seed;
+        const size_t fb = size_t(d.pixels) * 3;
+        if ((rc = ensureBuffer(ctx, R.films[j], fb * sizeof(float)))) return rc;
+        d.filmSum = static_cast<float*>(R.films[j].ptr);
+        hipLaunchKernelGGL(zeroKernel, dim3(1024), dim3(256), 0, ctx->stream, d.filmSum, fb);
+        if (momSum && momSum[j]) {
+            if ((rc = ensureBuffer(ctx, R.momS[j], fb * sizeof(double)))) return rc;
+            if ((rc = ensureBuffer(ctx, R.momQ[j], fb * sizeof(double)))) return rc;
+            d.momSum = static_cast<double*>(R.momS[j].ptr);
+            d.momSq = static_cast<double*>(R.momQ[j].ptr);
+            hipLaunchKernelGGL(zeroKernelD, dim3(1024), dim3(256), 0, ctx->stream, d.momSum, fb);
+            hipLaunchKernelGGL(zeroKernelD, dim3(1024), dim3(256), 0, ctx->stream, d.momSq, fb);
+        }
+        totalPaths += uint64_t(d.pixels) * uint32_t(jb.config.spp);
+    }
+
+    // ---- pool
+    uint32_t maxPixels = 0;
+    for (const DJob& d : dj) maxPixels = std::max(maxPixels, d.pixels);
+    const uint32_t P = uint32_t(std::min<uint64_t>(std::max<uint64_t>(R.poolCap, maxPixels), totalPaths));
+    const uint32_t ML = std::max(1u, maxLights);
+    struct { DevBuf* b; size_t bytes; } allocs[] = {
+        {&R.ray[0], size_t(P) * 32}, {&R.ray[1], size_t(P) * 32}, {&R.path[0], size_t(P) * 4}, {&R.path[1], size_t(P) * 4},
+        {&R.root[0], size_t(P) * 4}, {&R.root[1], size_t(P) * 4}, {&R.hit, size_t(P) * 8},
+        {&R.sray, size_t(P) * ML * 32}, {&R.sroot, size_t(P) * ML * 4}, {&R.scontrib, size_t(P) * ML * 16},
+        {&R.socc, size_t(P) * ML}, {&R.sindex, size_t(P) * ML * 4}, {&R.T, size_t(P) * 16}, {&R.L, size_t(P) * 16},
+        {&R.counts, 16}, {&R.jobs, sizeof(DJob) * size_t(njobs)}};
+    for (auto& a : allocs)
+        if ((rc = ensureBuffer(ctx, *a.b, a.bytes))) return rc;
+    Queues q;
+    for (int k = 0; k < 2; ++k) {
+        q.ray[k] = static_cast<float4*>(R.ray[k].ptr);
+        q.path[k] = static_cast<uint32_t*>(R.path[k].ptr);
+        q.root[k] = static_cast<uint32_t*>(R.root[k].ptr);
+    }
+    q.hit = static_cast<uint2*>(R.hit.ptr);
+    q.sray = static_cast<float4*>(R.sray.ptr);
+    q.sroot = static_cast<uint32_t*>(R.sroot.ptr);
+    q.scontrib = static_cast<float4*>(R.scontrib.ptr);
+    q.socc = static_cast<uint8_t*>(R.socc.ptr);
+    q.sindex = static_cast<int32_t*>(R.sindex.ptr);
+    q.T = static_cast<float4*>(R.T.ptr);
+    q.L = static_cast<float4*>(R.L.ptr);
+    q.counts = static_cast<uint32_t*>(R.counts.ptr);
+
+    // ---- rounds: give every unfinished job as many whole samples as the pool holds
+    std::vector<uint32_t> done(njobs, 0);
+    hipError_t e = hipSuccess;
+    hipEvent_t ev[2];
+    while (R.events.size() < 2) {
+        hipEvent_t x;
+        if ((e = hipEventCreate(&x)) != hipSuccess) return failHip(ctx, e, "hipEventCreate");
+        R.events.push_back(x);
+    }
+    ev[0] = R.events[0];
+    ev[1] = R.events[1];
+    if ((e = hipEventRecord(ev[0], ctx->stream)) != hipSuccess) return failHip(ctx, e, "hipEventRecord");
+    while (true) {
+        uint32_t used = 0;
+        std::vector<uint32_t> take(njobs, 0);
+        for (int j = 0; j < njobs; ++j) {
+            const uint32_t left = uint32_t(jobs[j].config.spp) - done[j];
+            if (!left) continue;
+            const uint32_t fit = (P - used) / dj[j].pixels;
+            take[j] = std::min(left, fit);
+            used += take[j] * dj[j].pixels;
+        }
+        if (used == 0) break;
+        uint32_t depthRound = 0;
+        uint32_t base = 0;
+        for (int j = 0; j < njobs; ++j) {
+            dj[j].pathBase = base;
+            dj[j].pathCount = take[j] * dj[j].pixels;
+            dj[j].sampleBase = jobs[j].config.sample_offset + done[j];
+            base += dj[j].pathCount;
+            if (take[j]) depthRound = std::max(depthRound, uint32_t(jobs[j].config.max_depth));
+        }
+        e = hipMemcpyAsync(R.jobs.ptr, dj.data(), sizeof(DJob) * njobs, hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) return failHip(ctx, e, "job upload");
+        const DJob* djd = static_cast<const DJob*>(R.jobs.ptr);
+        const unsigned grid = (used + 255) / 256;
+        hipLaunchKernelGGL(cameraKernel, dim3(grid), dim3(256), 0, ctx->stream, djd, njobs, used, q);
+        int cur = 0;
+        for (uint32_t depth = 0; depth < depthRound; ++depth) {
+            hipLaunchKernelGGL(resetCountsKernel, dim3(1), dim3(1), 0, ctx->stream, q.counts, cur ^ 1);
+            TraceArgs a = ta;
+            a.rays = q.ray[cur];
+            a.n = used;
+            a.nDev = q.counts + cur;
+            a.rootOf = q.root[cur];
+            a.hits = q.hit;
+            a.occ = nullptr;
+            if ((rc = launchTraversal(ctx, a, width, maxStack, false))) return rc;
+            hipLaunchKernelGGL(shadeKernel, dim3(grid), dim3(256), 0, ctx->stream, djd, njobs, depth, maxLights, cur, q);
+            if (maxLights) {
+                TraceArgs b = ta;
+                b.rays = q.sray;
+                b.n = used * ML;
+                b.nDev = q.counts + 2;
+                b.rootOf = q.sroot;
+                b.hits = nullptr;
+                b.occ = q.socc;
+                if ((rc = launchTraversal(ctx, b, width, maxStack, true))) return rc;
+                hipLaunchKernelGGL(gatherKernel, dim3(grid), dim3(256), 0, ctx->stream, maxLights, cur, q);
+            }
+            cur ^= 1;
+        }
+        for (int j = 0; j < njobs; ++j) {
+            if (!take[j]) continue;
+            hipLaunchKernelGGL(accumulateKernel, dim3((dj[j].pixels + 255) / 256), dim3(256), 0, ctx->stream, djd, j,
+                               take[j], q.L);
+            done[j] += take[j];
+        }
+        if ((e = hipGetLastError()) != hipSuccess) return failHip(ctx, e, "render launch");
+    }
+    if ((e = hipEventRecord(ev[1], ctx->stream)) != hipSuccess) return failHip(ctx, e, "hipEventRecord");
+    // ---- films
+    std::vector<float> host;
+    for (int j = 0; j < njobs; ++j) {
+        const size_t fb = size_t(dj[j].pixels) * 3;
+        host.resize(fb);
+        e = hipMemcpyAsync(host.data(), R.films[j].ptr, fb * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return failHip(ctx, e, "film download");
+        const float inv = 1.0f / float(jobs[j].config.spp);
+        for (size_t k = 0; k < fb; ++k) jobs[j].film_rgb[k] = host[k] * inv;
+        if (momSum && momSum[j]) {
+            e = hipMemcpyAsync(momSum[j], R.momS[j].ptr, fb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(momSq[j], R.momQ[j].ptr, fb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) return failHip(ctx, e, "moment download");
+        }
+    }
+    float ms = 0.0f;
+    if ((e = hipEventElapsedTime(&ms, ev[0], ev[1])) != hipSuccess) return failHip(ctx, e, "hipEventElapsedTime");
+    R.lastTotalMs = ms;
+    R.stageMs[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return RBR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rbr_render(rbr_ctx* ctx, const rbr_job* jobs, int njobs) { return renderImpl(ctx, jobs, njobs, nullptr, nullptr); }
+
+int rbr_render_moments(rbr_ctx* ctx, const rbr_job* job, double* sum_rgb, double* sumsq_rgb) {
+    if (!ctx || !job || !sum_rgb || !sumsq_rgb) return RBR_ERR_INVALID;
+    double* s[1] = {sum_rgb};
+    double* q[1] = {sumsq_rgb};
+    return renderImpl(ctx, job, 1, s, q);
+}
+
+int rbr_last_render_timing(rbr_ctx* ctx, double* total_ms, double* stage_ms) {
+    if (!ctx) return RBR_ERR_INVALID;
+    if (!ctx->render) return fail(ctx, RBR_ERR_STATE, "no render yet");
+    if (total_ms) *total_ms = ctx->render->lastTotalMs;
+    if (stage_ms)
+        for (int k = 0; k < 4; ++k) stage_ms[k] = ctx->render->stageMs[k];
+    return RBR_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,87 @@
+// Device-side shading model: the reference's Material -> MixtureBSDF construction
+// (material.cpp:69-108) evaluated without allocation, its lobes (lambertian_bsdf.cpp,
+// specular_bsdf.cpp, microfacet_bsdf.cpp), Fresnel terms (fresnel.h), lights
+// (area/point/directional_light.cpp), media + Henyey-Greenstein phase (medium.cpp,
+// phase_function.cpp) and the sampling helpers of sampler.h.  The lobe list of a material
+// depends only on the material, so the host precomputes it once (DMaterial) and a shading
+// thread reads 16-word lobe records instead of building a heap mixture per bounce.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rbr {
+
+enum DLobeType : uint32_t { DL_LAMBERT = 0, DL_SPEC_REFL = 1, DL_SPEC_TRANS = 2, DL_MF_REFL = 3, DL_MF_TRANS = 4 };
+enum DFresnel : uint32_t { DF_NONE = 0, DF_DIELECTRIC = 1, DF_SCHLICK = 2 };
+
+struct DLobe {           // 64 bytes
+    uint32_t type;
+    float R[3];          // albedo / reflectance / transmittance
+    uint32_t fkind;      // Fresnel of reflection lobes
+    float fEtaI, fEtaT;  // dielectric Fresnel indices
+    float f0[3];         // Schlick reflectance at normal incidence
+    float etaOut, etaIn; // transmission lobes
+    float alpha;         // GGX alpha, already max(roughness^2, 1e-3)
+    float weight;
+};
+
+struct DMaterial {       // 6 lobes + header
+    DLobe lobe[6];
+    uint32_t n;
+    uint32_t allDelta;   // MixtureBSDF::isDelta
+    uint32_t emissive;
+    uint32_t substrateMask;  // bit i: lobe i sits under the substrate Fresnel
+    uint32_t sfKind;
+    float sfEtaI, sfEtaT;
+    float emission[3];
+    float pad[4];
+};
+
+struct DLight {
+    uint32_t type;
+    float color[3];
+    float intensity;
+    float position[3];
+    float u[3];
+    float v[3];
+    float normal[3];
+    float direction[3];
+    float area;          // |u x v|
+    float pad[2];
+};
+
+struct DMedium {
+    uint32_t type;
+    float sa[3], ss[3];
+    float falloff, base, g;
+};
+
+struct DCamera {
+    float pos[3], fwd[3], right[3], up[3];
+    float fx, fy, cx, cy;
+    uint32_t width, height;
+};
+
+// One render job as the kernels see it.
+struct DJob {
+    DCamera cam;
+    DMedium medium;
+    const float4* primShade;     // 3 float4 per global primitive id
+    const DMaterial* mats;
+    const DLight* lights;
+    uint32_t nLights;
+    uint32_t integrator, maxDepth, useNEE, useMIS;
+    uint32_t root;               // BVH root node in the node arena
+    uint32_t pathBase, pathCount;   // pool slots of this job in the current round
+    uint32_t sampleBase;         // sample index of the job's first pool slot this round
+    uint32_t pixels;             // width * height
+    float bg[3];
+    uint64_t seed;
+    float* filmSum;              // 3 floats per pixel (device)
+    double* momSum;              // optional: 3 doubles per pixel
+    double* momSq;
+};
+
+}  // namespace rbr

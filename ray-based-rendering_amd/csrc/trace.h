@@ -26,21 +26,23 @@ struct TraceArgs {
     uint32_t nplanes;
     uint32_t mixed;           // any non-triangle bounded primitive
     const float4* rays;       // 2 float4 per ray
-    uint32_t n;
+    uint32_t n;               // ray count (ignored when nDev is set)
+    const uint32_t* nDev;     // optional: ray count read from device memory at run time
+    const uint32_t* rootOf;   // optional: per-ray root node (several BVHs in one node arena)
     uint2* hits;              // closest: (id, t bits)
     uint8_t* occ;             // any
 };
 
-// Persistent-wave launch state: a device batch counter that only grows (launchBase is its
-// value when this launch starts) and the per-wave spill rows of the traversal stack.
+// Persistent-wave launch state: a batch counter (zeroed by a one-lane kernel launched just
+// before the traversal on the same stream) and the per-wave spill rows of the stack.
 struct PersistArgs {
-    unsigned long long* counter;
-    unsigned long long launchBase;
+    uint32_t* counter;
     uint2* spill;             // [grid][spillDepth][64]
     uint32_t spillDepth;
 };
 
-// ldsDepth in {8, 12, 16, 24, 32}; p == nullptr selects the one-ray-per-lane kernel.
+// ldsDepth in {8, 12, 16, 24, 32}; p == nullptr selects the one-ray-per-lane kernel (which
+// needs a.nDev == nullptr).
 hipError_t launchTrace(const TraceArgs& a, int width, bool any, int ldsDepth, const PersistArgs* p, unsigned grid,
                        hipStream_t s);
 

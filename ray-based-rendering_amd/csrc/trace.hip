@@ -143,13 +143,13 @@ struct PrivateSpillStack {
 // One ray through the wide BVH and the unbounded list.  Returns the lowest-id closest hit
 // (bestT/bestId) or, for ANY, whether any primitive is hit in [tmin, tmax].
 template <int W, bool ANY, bool MIXED, typename Stack>
-__device__ __forceinline__ void traceRay(const TraceArgs& a, const DRay& r, Stack& st, float& bestT, uint32_t& bestId,
-                                         bool& occluded) {
+__device__ __forceinline__ void traceRay(const TraceArgs& a, const DRay& r, uint32_t root, Stack& st, float& bestT,
+                                         uint32_t& bestId, bool& occluded) {
     bestT = r.tmax;
     bestId = RBR_MISS_ID;
     occluded = false;
-    if (a.nodes) {
-        uint32_t ref = 0;   // root: wide node 0
+    if (a.nodes && root != kEmptyD) {
+        uint32_t ref = root;
         while (true) {
             if (!isLeafD(ref)) {
                 const float4* nd = a.nodes + size_t(ref) * (W == 4 ? 8 : 16);
@@ -247,33 +247,37 @@ __device__ __forceinline__ void storeResult(const TraceArgs& a, uint32_t i, floa
 
 // Persistent waves: each 64-lane workgroup takes batches of 64 consecutive rays from a
 // device counter until the batch is exhausted, so slow rays do not idle a whole launch
-// slot and every resident wave stays busy to the end.  The counter is never reset: the host
-// knows how far each launch advances it (one fetch per batch plus one final fetch per wave).
+// slot and every resident wave stays busy to the end.  The counter is zeroed by
+// resetCounterKernel right before the launch on the same stream; the ray count can come
+// from device memory (queue lengths written by an earlier kernel).
 template <int W, int LDS, bool ANY, bool MIXED>
 __global__ __launch_bounds__(64) void traceKernelPersistent(TraceArgs a, PersistArgs p) {
     __shared__ uint2 ldsStack[LDS * 64];
-    __shared__ unsigned long long batch;
+    __shared__ uint32_t batch;
     GlobalSpillStack<LDS> st;
     st.lds = ldsStack;
     st.spill = p.spill + size_t(blockIdx.x) * size_t(p.spillDepth) * 64;
+    const uint32_t n = a.nDev ? *a.nDev : a.n;
     while (true) {
-        if (threadIdx.x == 0) batch = atomicAdd(p.counter, 64ull);
+        if (threadIdx.x == 0) batch = atomicAdd(p.counter, 64u);
         __syncthreads();
-        const unsigned long long local = batch - p.launchBase;
+        const uint32_t base = batch;
         __syncthreads();
-        if (local >= a.n) break;
-        const uint32_t i = uint32_t(local) + threadIdx.x;
-        if (i < a.n) {
+        if (base >= n) break;
+        const uint32_t i = base + threadIdx.x;
+        if (i < n) {
             const DRay r = loadRay(a.rays, i);
             float bestT;
             uint32_t bestId;
             bool occ;
             st.sp = 0;
-            traceRay<W, ANY, MIXED>(a, r, st, bestT, bestId, occ);
+            traceRay<W, ANY, MIXED>(a, r, a.rootOf ? a.rootOf[i] : 0u, st, bestT, bestId, occ);
             storeResult<ANY>(a, i, bestT, bestId, occ);
         }
     }
 }
+
+__global__ void resetCounterKernel(uint32_t* c) { *c = 0u; }
 
 // One ray per lane, stack overflow in private memory (kept for comparison and for callers
 // that cannot provide the spill buffer).
@@ -288,13 +292,14 @@ __global__ __launch_bounds__(64) void traceKernelLane(TraceArgs a) {
     float bestT;
     uint32_t bestId;
     bool occ;
-    traceRay<W, ANY, MIXED>(a, r, st, bestT, bestId, occ);
+    traceRay<W, ANY, MIXED>(a, r, a.rootOf ? a.rootOf[i] : 0u, st, bestT, bestId, occ);
     storeResult<ANY>(a, i, bestT, bestId, occ);
 }
 
 template <int W, int LDS, bool ANY>
 hipError_t launchT(const TraceArgs& a, const PersistArgs* p, unsigned grid, hipStream_t s) {
     if (p) {
+        hipLaunchKernelGGL(resetCounterKernel, dim3(1), dim3(1), 0, s, p->counter);
         if (a.mixed) hipLaunchKernelGGL((traceKernelPersistent<W, LDS, ANY, true>), dim3(grid), dim3(64), 0, s, a, *p);
         else hipLaunchKernelGGL((traceKernelPersistent<W, LDS, ANY, false>), dim3(grid), dim3(64), 0, s, a, *p);
     } else {
@@ -319,7 +324,8 @@ hipError_t launchL(const TraceArgs& a, const PersistArgs* p, unsigned grid, int 
 
 hipError_t launchTrace(const TraceArgs& a, int width, bool any, int ldsDepth, const PersistArgs* p, unsigned grid,
                        hipStream_t s) {
-    if (a.n == 0) return hipSuccess;
+    if (a.n == 0 && !a.nDev) return hipSuccess;
+    if (!p && a.nDev) return hipErrorInvalidValue;
     switch (ldsDepth) {
         case 8: return launchL<8>(a, p, grid, width, any, s);
         case 12: return launchL<12>(a, p, grid, width, any, s);

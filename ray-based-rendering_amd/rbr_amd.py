@@ -315,3 +315,136 @@ def rays_shadow(lo, hi, n: int, seed: int, tmin: float = 1e-4) -> np.ndarray:
 def mesh_bounds(tris: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     v = tris.reshape(-1, 3)
     return v.min(0).astype(np.float32), v.max(0).astype(np.float32)
+
+
+# ---------------------------------------------------------------- rendering (rbr.h structs)
+
+class RbrMaterial(C.Structure):
+    _fields_ = [("base_color", C.c_float * 3), ("metallic", C.c_float), ("roughness", C.c_float), ("ior", C.c_float),
+                ("transmission", C.c_float), ("emission", C.c_float * 3)]
+
+
+class RbrLight(C.Structure):
+    _fields_ = [("type", C.c_int32), ("color", C.c_float * 3), ("intensity", C.c_float), ("position", C.c_float * 3),
+                ("u", C.c_float * 3), ("v", C.c_float * 3), ("normal", C.c_float * 3), ("direction", C.c_float * 3)]
+
+
+class RbrMedium(C.Structure):
+    _fields_ = [("type", C.c_int32), ("sigma_a", C.c_float * 3), ("sigma_s", C.c_float * 3), ("falloff", C.c_float),
+                ("base_height", C.c_float), ("g", C.c_float)]
+
+
+class RbrCamera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("forward", C.c_float * 3), ("right", C.c_float * 3), ("up", C.c_float * 3),
+                ("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float), ("width", C.c_int32),
+                ("height", C.c_int32)]
+
+
+class RbrRenderConfig(C.Structure):
+    _fields_ = [("integrator", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32), ("use_nee", C.c_int32),
+                ("use_mis", C.c_int32), ("guided", C.c_int32), ("seed", C.c_uint64), ("sample_offset", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class RbrJob(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", RbrCamera), ("config", RbrRenderConfig),
+                ("film_rgb", C.POINTER(C.c_float))]
+
+
+PATH, VOLPATH = 0, 1
+
+_sig(_lib.rbr_scene_set_shading, [_vp, C.POINTER(RbrMaterial), _u32, _vp, _u32, C.POINTER(RbrLight), _u32,
+                                  C.POINTER(RbrMedium), _vp])
+_sig(_lib.rbr_render, [_vp, C.POINTER(RbrJob), C.c_int])
+_sig(_lib.rbr_render_moments, [_vp, C.POINTER(RbrJob), _vp, _vp])
+_sig(_lib.rbr_last_render_timing, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)])
+
+
+def _f3(x):
+    return (C.c_float * 3)(*[float(v) for v in x])
+
+
+def c_materials(mats: list) -> C.Array:
+    arr = (RbrMaterial * max(1, len(mats)))()
+    for i, m in enumerate(mats):
+        arr[i] = RbrMaterial(_f3(m["base_color"]), m["metallic"], m["roughness"], m["ior"], m["transmission"],
+                             _f3(m["emission"]))
+    return arr
+
+
+def c_lights(lights: list) -> C.Array:
+    arr = (RbrLight * max(1, len(lights)))()
+    for i, l in enumerate(lights):
+        arr[i] = RbrLight(l["type"], _f3(l["color"]), l["intensity"], _f3(l["position"]), _f3(l["u"]), _f3(l["v"]),
+                          _f3(l["normal"]), _f3(l["direction"]))
+    return arr
+
+
+def c_medium(m: dict | None) -> RbrMedium:
+    if not m:
+        return RbrMedium()
+    return RbrMedium(m["type"], _f3(m["sigma_a"]), _f3(m["sigma_s"]), m["falloff"], m["base_height"], m["g"])
+
+
+def c_camera(cam) -> RbrCamera:
+    return RbrCamera(_f3(cam.position), _f3(cam.forward), _f3(cam.right), _f3(cam.up), cam.fx, cam.fy, cam.cx, cam.cy,
+                     cam.width, cam.height)
+
+
+def c_config(spp: int, depth: int, seed: int = 1, integrator: int = PATH, nee: bool = True, mis: bool = True,
+             sample_offset: int = 0) -> RbrRenderConfig:
+    return RbrRenderConfig(integrator, spp, depth, int(nee), int(mis), 0, seed, sample_offset, 0)
+
+
+def scene_from_desc(ctx: Context, desc, width: int = 4, leaf: int = 4) -> Scene:
+    """Uploads a scenes.SceneDesc: geometry, wide BVH and shading tables."""
+    sc = Scene(ctx)
+    sc.set_geometry(tris=desc.tris if len(desc.tris) else None, spheres=desc.spheres if len(desc.spheres) else None,
+                    quads=desc.quads if len(desc.quads) else None, planes=desc.planes if len(desc.planes) else None)
+    sc.build_bvh(width, leaf)
+    pm = np.ascontiguousarray(desc.prim_material, np.int32)
+    mats, lights, med = c_materials(desc.materials), c_lights(desc.lights), c_medium(desc.medium)
+    bg = np.asarray(desc.background, np.float32)
+    ctx.check(_lib.rbr_scene_set_shading(sc.handle, mats, len(desc.materials), pm.ctypes.data, len(pm), lights,
+                                         len(desc.lights), C.byref(med), bg.ctypes.data))
+    sc.desc = desc
+    return sc
+
+
+def render(ctx: Context, jobs: list) -> list[np.ndarray]:
+    """Integrator::render for one or many jobs in one launch sequence.  Each job is a dict
+    with scene (a Scene from scene_from_desc), spp, depth and optional seed, integrator, nee,
+    mis, camera.  Returns one (H, W, 3) float32 film per job (per-pixel mean radiance)."""
+    arr = (RbrJob * len(jobs))()
+    films = []
+    for i, j in enumerate(jobs):
+        cam = j.get("camera") or j["scene"].desc.camera
+        film = np.zeros((cam.height, cam.width, 3), np.float32)
+        films.append(film)
+        arr[i] = RbrJob(j["scene"].handle.value, c_camera(cam),
+                        c_config(j["spp"], j["depth"], j.get("seed", 1), j.get("integrator", PATH), j.get("nee", True),
+                                 j.get("mis", True), j.get("sample_offset", 0)),
+                        film.ctypes.data_as(C.POINTER(C.c_float)))
+    ctx.check(_lib.rbr_render(ctx.handle, arr, len(jobs)))
+    return films
+
+
+def render_moments(ctx: Context, job: dict):
+    """Per-pixel sum and sum of squares of the single-sample radiance (float64)."""
+    cam = job.get("camera") or job["scene"].desc.camera
+    film = np.zeros((cam.height, cam.width, 3), np.float32)
+    s1 = np.zeros((cam.height, cam.width, 3), np.float64)
+    s2 = np.zeros_like(s1)
+    j = RbrJob(job["scene"].handle.value, c_camera(cam),
+               c_config(job["spp"], job["depth"], job.get("seed", 1), job.get("integrator", PATH), job.get("nee", True),
+                        job.get("mis", True), job.get("sample_offset", 0)),
+               film.ctypes.data_as(C.POINTER(C.c_float)))
+    ctx.check(_lib.rbr_render_moments(ctx.handle, C.byref(j), s1.ctypes.data, s2.ctypes.data))
+    return film, s1, s2
+
+
+def last_render_ms(ctx: Context) -> float:
+    t = C.c_double(0)
+    st = (C.c_double * 4)()
+    ctx.check(_lib.rbr_last_render_timing(ctx.handle, C.byref(t), st))
+    return t.value

@@ -1,0 +1,342 @@
+"""Seeded scene descriptions for the path-tracing workloads.
+
+A `SceneDesc` holds plain arrays in the C ABI's layout (rbr.h): triangles (9 floats),
+spheres (4), quads (origin u v normal: 12), planes (normal d: 4), one material index per
+primitive id (ids number triangles, then spheres, quads, planes), lights, medium,
+background and a pinhole camera.  Normals are computed in FP32 in the reference's
+operand order (cross then normalized = v * (1/len)) so every consumer sees the same bits.
+
+  cornell_c0()        Cornell box of side 555 built from quads, 256x256, 16 spp, depth 8
+  terrain_c2()        512x512-cell noise heightfield + 64 spheres, 4 area + 1 directional light
+  fog_jobs_c3()       16 uneven jobs: displaced icospheres in a closed room with fog
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+F = np.float32
+
+
+def _cross(a, b):
+    a = np.asarray(a, F)
+    b = np.asarray(b, F)
+    return np.array([a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]], F)
+
+
+def normalized(v):
+    """Vec3::normalized (vector.h:203-209) in FP32: v * (1 / sqrt((x*x + y*y) + z*z))."""
+    v = np.asarray(v, F)
+    ln = np.sqrt(F(F(v[0] * v[0]) + F(v[1] * v[1])) + F(v[2] * v[2]))
+    inv = F(F(1.0) / ln)
+    return (v * inv).astype(F)
+
+
+@dataclasses.dataclass
+class Camera:
+    position: np.ndarray
+    forward: np.ndarray
+    right: np.ndarray
+    up: np.ndarray
+    fx: float
+    fy: float
+    cx: float
+    cy: float
+    width: int
+    height: int
+
+
+def look_at(position, target, up, width, height, vfov_deg) -> Camera:
+    """CameraExtrinsics::lookAt (camera_extrinsics.h:28-45) and
+    PerspectiveIntrinsics::fromVerticalFov (perspective_intrinsics.h:33-36)."""
+    position = np.asarray(position, F)
+    fwd = normalized(np.asarray(target, F) - position)
+    r = _cross(fwd, up)
+    if float(np.dot(r, r)) < 1e-12:
+        alt = np.array([0, 1, 0], F) if abs(fwd[1]) < 0.99 else np.array([0, 0, 1], F)
+        r = _cross(fwd, alt)
+    right = normalized(r)
+    upv = _cross(right, fwd)
+    f = F(0.5) * F(height) / F(math.tan(F(F(vfov_deg) * F(math.pi)) / F(180.0) * F(0.5)))
+    return Camera(position, fwd, right, upv, float(f), float(f), 0.5 * width, 0.5 * height, width, height)
+
+
+def material(base=(0.8, 0.8, 0.8), metallic=0.0, roughness=1.0, ior=1.5, transmission=0.0, emission=(0, 0, 0)):
+    return dict(base_color=tuple(float(x) for x in base), metallic=float(metallic), roughness=float(roughness),
+                ior=float(ior), transmission=float(transmission), emission=tuple(float(x) for x in emission))
+
+
+def diffuse(albedo):
+    """Material::diffuse (material.cpp:9-15): roughness 1, ior 1 (pure Lambert)."""
+    return material(albedo, 0.0, 1.0, 1.0, 0.0)
+
+
+def emissive(e):
+    """Material::emissive (material.cpp:49-55)."""
+    return material((0, 0, 0), 0.0, 1.0, 1.0, 0.0, e)
+
+
+def area_light(position, u, v, color, intensity=1.0):
+    u = np.asarray(u, F)
+    v = np.asarray(v, F)
+    return dict(type=0, color=tuple(map(float, color)), intensity=float(intensity), position=tuple(map(float, position)),
+                u=tuple(map(float, u)), v=tuple(map(float, v)), normal=tuple(map(float, normalized(_cross(u, v)))),
+                direction=(0.0, 0.0, 0.0))
+
+
+def point_light(position, color, intensity):
+    return dict(type=1, color=tuple(map(float, color)), intensity=float(intensity), position=tuple(map(float, position)),
+                u=(0.0, 0.0, 0.0), v=(0.0, 0.0, 0.0), normal=(0.0, 0.0, 0.0), direction=(0.0, 0.0, 0.0))
+
+
+def directional_light(direction, color, intensity):
+    return dict(type=2, color=tuple(map(float, color)), intensity=float(intensity), position=(0.0, 0.0, 0.0),
+                u=(0.0, 0.0, 0.0), v=(0.0, 0.0, 0.0), normal=(0.0, 0.0, 0.0),
+                direction=tuple(map(float, normalized(direction))))
+
+
+@dataclasses.dataclass
+class SceneDesc:
+    tris: np.ndarray
+    spheres: np.ndarray
+    quads: np.ndarray
+    planes: np.ndarray
+    tri_mat: np.ndarray
+    sphere_mat: np.ndarray
+    quad_mat: np.ndarray
+    plane_mat: np.ndarray
+    materials: list
+    lights: list
+    camera: Camera
+    medium: dict | None = None
+    background: tuple = (0.0, 0.0, 0.0)
+    name: str = ""
+
+    @property
+    def prim_material(self) -> np.ndarray:
+        return np.concatenate([self.tri_mat, self.sphere_mat, self.quad_mat, self.plane_mat]).astype(np.int32)
+
+    @property
+    def n_prims(self) -> int:
+        return len(self.tris) + len(self.spheres) + len(self.quads) + len(self.planes)
+
+
+class _Builder:
+    def __init__(self):
+        self.tris, self.spheres, self.quads, self.planes = [], [], [], []
+        self.tm, self.sm, self.qm, self.pm = [], [], [], []
+        self.materials, self.lights = [], []
+
+    def mat(self, m) -> int:
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def quad(self, origin, u, v, m):
+        origin, u, v = (np.asarray(x, F) for x in (origin, u, v))
+        n = normalized(_cross(u, v))   # Quad::Quad, quad.cpp:6-10
+        self.quads.append(np.concatenate([origin, u, v, n]))
+        self.qm.append(m)
+
+    def box(self, size, angle_deg, offset, m):
+        """Axis box [0,size] rotated about y by angle then translated: six quads."""
+        a = math.radians(angle_deg)
+        c, s = math.cos(a), math.sin(a)
+        rot = lambda p: np.array([c * p[0] + s * p[2], p[1], -s * p[0] + c * p[2]], F)
+        sx, sy, sz = size
+        corners = {
+            "front": ((0, 0, 0), (sx, 0, 0), (0, sy, 0)), "back": ((0, 0, sz), (sx, 0, 0), (0, sy, 0)),
+            "left": ((0, 0, 0), (0, 0, sz), (0, sy, 0)), "right": ((sx, 0, 0), (0, 0, sz), (0, sy, 0)),
+            "bottom": ((0, 0, 0), (sx, 0, 0), (0, 0, sz)), "top": ((0, sy, 0), (sx, 0, 0), (0, 0, sz)),
+        }
+        off = np.asarray(offset, F)
+        for o, u, v in corners.values():
+            self.quad(rot(o) + off, rot(u), rot(v), m)
+
+    def tri_mesh(self, tris, m):
+        self.tris.append(np.asarray(tris, F).reshape(-1, 9))
+        self.tm.append(np.full(len(self.tris[-1]), m, np.int32))
+
+    def sphere(self, c, r, m):
+        self.spheres.append(np.array([c[0], c[1], c[2], r], F))
+        self.sm.append(m)
+
+    def build(self, camera, medium=None, background=(0, 0, 0), name="") -> SceneDesc:
+        cat = lambda lst, w: np.concatenate(lst).astype(F).reshape(-1, w) if lst else np.zeros((0, w), F)
+        return SceneDesc(
+            tris=cat(self.tris, 9),
+            spheres=np.asarray(self.spheres, F).reshape(-1, 4),
+            quads=np.asarray(self.quads, F).reshape(-1, 12),
+            planes=np.asarray(self.planes, F).reshape(-1, 4),
+            tri_mat=np.concatenate(self.tm).astype(np.int32) if self.tm else np.zeros(0, np.int32),
+            sphere_mat=np.asarray(self.sm, np.int32), quad_mat=np.asarray(self.qm, np.int32),
+            plane_mat=np.asarray(self.pm, np.int32), materials=self.materials, lights=self.lights,
+            camera=camera, medium=medium, background=tuple(background), name=name)
+
+
+def cornell_c0(width: int = 256, height: int = 256) -> SceneDesc:
+    """The 555 Cornell box: white floor/ceiling/back (0.73), red wall at x=555 (image left
+    for this camera), green at x=0, two rotated boxes, an emissive ceiling quad (17,12,4)
+    and the matching AreaLight; pinhole at (278,278,-800) looking at (278,278,0), 40 deg."""
+    b = _Builder()
+    white = b.mat(diffuse((0.73, 0.73, 0.73)))
+    red = b.mat(diffuse((0.65, 0.05, 0.05)))
+    green = b.mat(diffuse((0.12, 0.45, 0.15)))
+    light = b.mat(emissive((17.0, 12.0, 4.0)))
+    b.quad((0, 0, 0), (555, 0, 0), (0, 0, 555), white)        # floor
+    b.quad((0, 555, 0), (555, 0, 0), (0, 0, 555), white)      # ceiling
+    b.quad((0, 0, 555), (555, 0, 0), (0, 555, 0), white)      # back
+    b.quad((555, 0, 0), (0, 555, 0), (0, 0, 555), red)        # x = 555
+    b.quad((0, 0, 0), (0, 555, 0), (0, 0, 555), green)        # x = 0
+    b.quad((213, 554, 227), (130, 0, 0), (0, 0, 105), light)  # emitter
+    b.box((165, 330, 165), 15.0, (265, 0, 295), white)
+    b.box((165, 165, 165), -18.0, (130, 0, 65), white)
+    b.lights.append(area_light((278, 554, 279.5), (130, 0, 0), (0, 0, 105), (17, 12, 4), 1.0))
+    cam = look_at((278, 278, -800), (278, 278, 0), (0, 1, 0), width, height, 40.0)
+    return b.build(cam, name="cornell555")
+
+
+# ---------------------------------------------------------------- terrain scene
+
+def _value_noise2(x, z, seed):
+    xi, zi = np.floor(x).astype(np.int64), np.floor(z).astype(np.int64)
+    tx, tz = (x - xi).astype(F), (z - zi).astype(F)
+    fx, fz = tx * tx * (3 - 2 * tx), tz * tz * (3 - 2 * tz)
+
+    def h(a, b):
+        v = (a * 0x27D4EB2D) ^ (b * 0x165667B1) ^ (seed * 0x9E3779B1)
+        v = (v ^ (v >> 15)) * 0x85EBCA77
+        v = (v ^ (v >> 13)) * 0xC2B2AE3D
+        v = v ^ (v >> 16)
+        return ((v & 0xFFFFFF).astype(F) / F(16777216.0)) * F(2) - F(1)
+
+    a, b_, c, d = h(xi, zi), h(xi + 1, zi), h(xi, zi + 1), h(xi + 1, zi + 1)
+    return (a + (b_ - a) * fx) + ((c + (d - c) * fx) - (a + (b_ - a) * fx)) * fz
+
+
+def terrain_height(x, z, seed=0xC2):
+    x = np.asarray(x, np.float64)
+    z = np.asarray(z, np.float64)
+    n = (1.0 * _value_noise2(x / 96.0, z / 96.0, seed) + 0.5 * _value_noise2(x / 48.0, z / 48.0, seed + 1)
+         + 0.25 * _value_noise2(x / 24.0, z / 24.0, seed + 2))
+    return (F(18.0) * n.astype(F) / F(1.75)).astype(F)
+
+
+def _random_material(rng) -> dict:
+    rough = rng.uniform(0.05, 1.0)
+    base = rng.uniform(0.1, 0.9, 3)
+    if rng.random() < 0.1:
+        return material(base, 0.0, rough, 1.5, 1.0)          # glass
+    metal = 1.0 if rng.random() < 0.4 else 0.0
+    return material(base, metal, rough, 1.5, 0.0)
+
+
+def terrain_c2(width: int = 1920, height: int = 1080, cells: int = 512, seed: int = 0xC2) -> SceneDesc:
+    rng = np.random.default_rng(seed)
+    b = _Builder()
+    xs = np.arange(cells + 1, dtype=F) - F(cells / 2)
+    X, Z = np.meshgrid(xs, xs, indexing="xy")
+    H = terrain_height(X, Z, seed)
+    v = np.stack([X, H, Z], -1).astype(F)
+    v00, v10, v01, v11 = v[:-1, :-1], v[:-1, 1:], v[1:, :-1], v[1:, 1:]
+    t1 = np.concatenate([v00, v01, v10], -1).reshape(-1, 9)
+    t2 = np.concatenate([v10, v01, v11], -1).reshape(-1, 9)
+    tris = np.stack([t1, t2], 1).reshape(-1, 9)
+    b.tri_mesh(tris, b.mat(_random_material(rng)))
+    for _ in range(64):
+        r = float(rng.uniform(0.2, 1.5))
+        x, z = rng.uniform(-30.0, 30.0, 2)
+        y = float(terrain_height(np.array([x]), np.array([z]), seed)[0]) + r
+        b.sphere((x, y, z), r, b.mat(_random_material(rng)))
+    for k in range(4):
+        e = float(rng.uniform(10.0, 30.0))
+        col = (e, e * 0.92, e * 0.85)
+        cx, cz = (-20.0 + 40.0 * (k % 2)), (-20.0 + 40.0 * (k // 2))
+        cy = float(terrain_height(np.array([cx]), np.array([cz]), seed)[0]) + 30.0
+        size = 8.0
+        # Quad facing down: u x v = -y.
+        b.quad((cx - size / 2, cy, cz - size / 2), (size, 0, 0), (0, 0, size), b.mat(emissive(col)))
+        b.lights.append(area_light((cx, cy, cz), (size, 0, 0), (0, 0, size), col, 1.0))
+    b.lights.append(directional_light((-0.4, -1.0, -0.3), (1.0, 0.95, 0.85), 1.5))
+    cy = float(terrain_height(np.array([0.0]), np.array([70.0]), seed)[0])
+    cam = look_at((0.0, cy + 22.0, 70.0), (0.0, cy - 4.0, 0.0), (0, 1, 0), width, height, 45.0)
+    return b.build(cam, name=f"terrain{cells}")
+
+
+# ---------------------------------------------------------------- fog jobs
+
+def fog_room(tris: np.ndarray, width: int, height: int, medium: dict, rng) -> SceneDesc:
+    """A displaced icosphere (radius ~100) inside a closed 400-unit room, two point lights,
+    filled with the given medium; every ray ends on a surface (the exponential medium's
+    distance sampler needs that)."""
+    b = _Builder()
+    wall = b.mat(diffuse((0.7, 0.7, 0.7)))
+    b.tri_mesh(tris, b.mat(material(rng.uniform(0.2, 0.8, 3), 0.0, float(rng.uniform(0.3, 1.0)), 1.5, 0.0)))
+    # Each wall overhangs its neighbours by 2 units so no ray can slip out through an edge.
+    s, e = 200.0, 202.0
+    b.quad((-e, -s, -e), (2 * e, 0, 0), (0, 0, 2 * e), wall)
+    b.quad((-e, s, -e), (2 * e, 0, 0), (0, 0, 2 * e), wall)
+    b.quad((-e, -e, -s), (2 * e, 0, 0), (0, 2 * e, 0), wall)
+    b.quad((-e, -e, s), (2 * e, 0, 0), (0, 2 * e, 0), wall)
+    b.quad((-s, -e, -e), (0, 2 * e, 0), (0, 0, 2 * e), wall)
+    b.quad((s, -e, -e), (0, 2 * e, 0), (0, 0, 2 * e), wall)
+    b.lights.append(point_light((-90.0, 150.0, -120.0), (1.0, 0.9, 0.8), 3.0e4))
+    b.lights.append(point_light((110.0, 60.0, -150.0), (0.7, 0.8, 1.0), 1.5e4))
+    cam = look_at((0.0, 40.0, -195.0), (0.0, 0.0, 0.0), (0, 1, 0), width, height, 50.0)
+    return b.build(cam, medium=medium, name="fogroom")
+
+
+def fog_jobs_c3(seed: int = 0xC3, scale: float = 1.0):
+    """16 jobs: resolution from 128x128 to 1280x720, spp in {8,16,32}, depth 4..12,
+    icospheres of 10k..300k triangles; jobs 0-7 homogeneous fog (sigma_t 0.05..0.3,
+    albedo 0.8), jobs 8-15 exponential height fog (0.5*exp(-2y)); HG g in [-0.3, 0.8].
+    `scale` shrinks resolutions (tests use small films)."""
+    import rbr_amd as R
+
+    rng = np.random.default_rng(seed)
+    res = [(128, 128), (256, 256), (320, 240), (512, 288), (640, 360), (640, 480), (800, 600), (1024, 576),
+           (1280, 720), (256, 144), (384, 216), (480, 270), (960, 540), (200, 200), (720, 405), (160, 120)]
+    jobs = []
+    for j in range(16):
+        w, h = res[j]
+        w, h = max(8, int(w * scale)), max(8, int(h * scale))
+        spp = int(rng.choice([8, 16, 32]))
+        depth = int(rng.integers(4, 13))
+        n_tri = int(rng.uniform(10_000, 300_000))
+        n_sub = max(4, int(round(math.sqrt(n_tri / 20.0))))
+        tris = R.icosphere(n_sub, 100.0, 0.1, seed + j)
+        g = float(rng.uniform(-0.3, 0.8))
+        # Homogeneous: sigma_t in [0.05, 0.3], albedo 0.8.  Height fog: density 0.5*exp(-2y)
+        # (sigma_a 0.1 + sigma_s 0.4 at y <= 0, falloff 2).  Coefficients are per world unit.
+        if j < 8:
+            st = float(rng.uniform(0.05, 0.3))
+            med = dict(type=1, sigma_a=(0.2 * st,) * 3, sigma_s=(0.8 * st,) * 3, falloff=0.0, base_height=0.0, g=g)
+        else:
+            med = dict(type=2, sigma_a=(0.1,) * 3, sigma_s=(0.4,) * 3, falloff=2.0, base_height=0.0, g=g)
+        sc = fog_room(tris, w, h, med, rng)
+        jobs.append(dict(scene=sc, spp=spp, depth=depth, seed=seed * 100 + j))
+    return jobs
+
+
+def materials_scene(width: int = 64, height: int = 48) -> SceneDesc:
+    """Spheres of every lobe combination (smooth and rough glass, mirror, rough metal,
+    plastic, half metal, diffuse) on a floor, lit by an area light, a point light and a
+    directional light: exercises all BSDF paths, delta and non-delta NEE."""
+    b = _Builder()
+    floor = b.mat(diffuse((0.6, 0.6, 0.6)))
+    b.quad((-10, 0, -10), (20, 0, 0), (0, 0, 20), floor)
+    mats = [material((1, 1, 1), 0.0, 0.0, 1.5, 1.0), material((0.9, 0.95, 1.0), 0.0, 0.35, 1.5, 1.0),
+            material((0.9, 0.9, 0.9), 1.0, 0.0), material((0.9, 0.6, 0.3), 1.0, 0.4),
+            material((0.2, 0.5, 0.8), 0.0, 0.3, 1.5, 0.0), material((0.8, 0.8, 0.2), 0.5, 0.2, 1.5, 0.0),
+            diffuse((0.7, 0.2, 0.2)), material((0.3, 0.8, 0.3), 0.0, 0.8, 1.33, 0.5)]
+    for k, m in enumerate(mats):
+        x = -5.25 + 1.5 * k
+        b.sphere((x, 0.7, 0.5 * (k % 2)), 0.7, b.mat(m))
+    lm = b.mat(emissive((8.0, 8.0, 8.0)))
+    b.quad((-1.5, 6.0, -1.5), (3, 0, 0), (0, 0, 3), lm)
+    b.lights.append(area_light((0.0, 6.0, 0.0), (3, 0, 0), (0, 0, 3), (8, 8, 8), 1.0))
+    b.lights.append(point_light((4.0, 4.0, -4.0), (1.0, 0.8, 0.6), 12.0))
+    b.lights.append(directional_light((0.3, -1.0, 0.4), (0.4, 0.45, 0.5), 1.0))
+    cam = look_at((0.0, 3.0, -9.0), (0.0, 0.6, 0.0), (0, 1, 0), width, height, 45.0)
+    return b.build(cam, background=(0.05, 0.05, 0.08), name="materials")

@@ -1,0 +1,119 @@
+"""GPU wavefront path tracer vs the CPU oracle (itself bit-exact with the reference's own
+integrators, test_oracle_render.py).  The two use different random streams, so parity is
+statistical: per-pixel z-scores of the mean radiance over independent samples
+    z = (m_gpu - m_ref) / sqrt(var_gpu / n_gpu + var_ref / n_ref)
+must look standard normal (no bias), and whole-image means must agree."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+import rbr_amd as R
+import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = R.Context(0)
+    yield c
+    c.close()
+
+
+def zscores(gs1, gs2, ng, rs1, rs2, nr):
+    gs1, gs2 = gs1.reshape(-1, 3), gs2.reshape(-1, 3)
+    rs1, rs2 = rs1.reshape(-1, 3), rs2.reshape(-1, 3)
+    mg, mr = gs1 / ng, rs1 / nr
+    vg = np.maximum(gs2 / ng - mg * mg, 0.0)
+    vr = np.maximum(rs2 / nr - mr * mr, 0.0)
+    se2 = vg / ng + vr / nr
+    se = np.sqrt(se2)
+    ok = se > 0
+    z = np.zeros_like(mg)
+    z[ok] = (mg[ok] - mr[ok]) / se[ok]
+    both_zero = (~ok) & (mg != mr)
+    # z-score of the whole-image mean (pixels are independent).
+    z_img = float((mg.sum() - mr.sum()) / np.sqrt(se2.sum()))
+    return z, mg, mr, both_zero, z_img
+
+
+def check_unbiased(z, mg, mr, both_zero, z_img, frac_tail=0.003, zmax=4.5, mean_tol=0.05):
+    assert not both_zero.any(), "a pixel with zero variance differs"
+    tail = float((np.abs(z) > zmax).mean())
+    # The three channels of a pixel share samples, so count pixels for the standard error.
+    zlim = 5.0 / np.sqrt(len(z))
+    assert abs(float(z.mean())) < zlim, f"mean z {z.mean():.3f} (limit {zlim:.3f})"
+    assert tail < frac_tail, f"{tail:.4f} of pixels beyond |z| > {zmax}"
+    assert abs(z_img) < 5.0, f"image-mean z {z_img:.2f}"
+    if mean_tol is not None:
+        rel = abs(mg.mean() - mr.mean()) / max(mr.mean(), 1e-9)
+        assert rel < mean_tol, f"image means differ by {rel:.4f}"
+
+
+def gpu_vs_oracle(ctx, desc, spp_gpu, spp_ref, depth, integrator=0, nee=True, mis=True, **kw):
+    sc = R.scene_from_desc(ctx, desc)
+    _, g1, g2 = R.render_moments(ctx, dict(scene=sc, spp=spp_gpu, depth=depth, seed=11, integrator=integrator,
+                                           nee=nee, mis=mis))
+    r1, r2 = O.OracleRender(desc).moments(spp_ref, depth, seed=5, integrator=integrator, nee=nee, mis=mis)
+    z, mg, mr, bz, zi = zscores(g1, g2, spp_gpu, r1, r2, spp_ref)
+    say(f"  mean z {z.mean():+.3f}  |z|>4.5 {(np.abs(z) > 4.5).mean():.4f}  image z {zi:+.2f}  "
+        f"means {mg.mean():.5f} vs {mr.mean():.5f}")
+    check_unbiased(z, mg, mr, bz, zi, **kw)
+
+
+@pytest.mark.parametrize("nee,mis", [(True, True), (True, False), (False, True)])
+def test_cornell_unbiased_vs_oracle(ctx, nee, mis):
+    say(f"case: cornell 48x48 nee={nee} mis={mis}")
+    gpu_vs_oracle(ctx, S.cornell_c0(48, 48), 1024, 1024, 8, 0, nee, mis)
+
+
+def test_materials_unbiased_vs_oracle(ctx):
+    say("case: materials scene")
+    # Caustics through the glass spheres are heavy-tailed; allow a slightly wider tail.
+    gpu_vs_oracle(ctx, S.materials_scene(40, 30), 2048, 2048, 8, frac_tail=0.01)
+
+
+@pytest.mark.parametrize("job", [0, 3, 9, 14])
+def test_fog_volpath_unbiased_vs_oracle(ctx, job):
+    say(f"case: fog job {job}")
+    desc = S.fog_jobs_c3(scale=0.05)[job]["scene"]
+    # The reference's homogeneous scatter weight sigma_s*Tr/avg^2 (medium.cpp:22-27) makes
+    # radiance heavy-tailed, so only the z-tests (not a raw ratio of means) are meaningful.
+    gpu_vs_oracle(ctx, desc, 512, 512, 6, integrator=R.VOLPATH, frac_tail=0.01, mean_tol=None)
+
+
+def test_render_is_deterministic_and_pool_independent(ctx):
+    say("case: determinism + multi-job equivalence")
+    a = R.scene_from_desc(ctx, S.cornell_c0(40, 30))
+    b = R.scene_from_desc(ctx, S.materials_scene(24, 20))
+    ja = dict(scene=a, spp=8, depth=8, seed=3)
+    jb = dict(scene=b, spp=5, depth=6, seed=4)
+    f1 = R.render(ctx, [ja])[0]
+    f2 = R.render(ctx, [ja])[0]
+    assert np.array_equal(f1, f2)
+    fa, fb = R.render(ctx, [ja, jb])
+    assert np.array_equal(fa, f1)
+    assert np.array_equal(fb, R.render(ctx, [jb])[0])
+
+
+def test_film_is_mean_of_moments(ctx):
+    say("case: film vs moments")
+    sc = R.scene_from_desc(ctx, S.cornell_c0(32, 32))
+    film, s1, s2 = R.render_moments(ctx, dict(scene=sc, spp=16, depth=8, seed=9))
+    assert np.allclose(film, s1 / 16, rtol=1e-5, atol=1e-6)
+
+
+def test_invalid_jobs(ctx):
+    sc = R.Scene(ctx)
+    sc.set_geometry(spheres=np.array([[0, 0, 0, 1]], np.float32))
+    sc.build_bvh()
+    cam = S.cornell_c0(8, 8).camera
+    with pytest.raises(R.RbrError):   # no shading tables yet
+        R.render(ctx, [dict(scene=sc, spp=1, depth=1, camera=cam)])
