@@ -1,0 +1,99 @@
+"""Condenses rocprofv3 CSV output into the short summaries kept under profiles/.
+
+  python ray-based-rendering_amd/profile_summary.py --stats DIR/x_kernel_stats.csv \
+      [--fetch DIR/x_counter_collection.csv] [--write DIR/x_counter_collection.csv] \
+      [--l2 DIR/x_counter_collection.csv] --name NAME --command "..." [--kernel SUBSTR] \
+      [--copy-bytes 268435456]
+
+Writes profiles/NAME.md (kernel table, counters) and, for traversal, profiles/NAME_pmc.json
+with the calibrated HBM bytes per launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are
+scaled by the ratio of the known bytes of the calibration copy kernel to what the same
+counter reads for it.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import json
+import os
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def counters(path: str) -> dict:
+    agg = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        agg[(r["Kernel_Name"], r["Counter_Name"])].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in agg.items()}
+
+
+def short(name: str) -> str:
+    return name.replace("void rbr::(anonymous namespace)::", "").replace("(rbr::TraceArgs, rbr::PersistArgs)", "")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--stats", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--l2")
+    ap.add_argument("--name", required=True)
+    ap.add_argument("--command", required=True)
+    ap.add_argument("--kernel", default="traceKernelPersistent<4, 8, false")
+    ap.add_argument("--copy-bytes", type=float, default=256 * 1024 * 1024)
+    args = ap.parse_args()
+    rows = list(csv.DictReader(open(args.stats)))
+    lines = [f"# {args.name}", "", "Command (from the repository root, on one MI355X):", "", "```", args.command, "```", "",
+             "| kernel | calls | avg us | min us | max us | % time |", "|---|---|---|---|---|---|"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))[:12]:
+        lines.append(f"| `{short(r['Name'])[:90]}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    out = {"name": args.name, "command": args.command}
+    tgt = [r for r in rows if args.kernel in r["Name"]]
+    if tgt:
+        out["kernel"] = short(tgt[0]["Name"])
+        out["avg_us"] = float(tgt[0]["AverageNs"]) / 1e3
+    if args.fetch and args.write:
+        f = counters(args.fetch)
+        w = counters(args.write)
+        cf = [v for (k, c), v in f.items() if "calibCopy" in k]
+        cw = [v for (k, c), v in w.items() if "calibCopy" in k]
+        kib = args.copy_bytes / 1024.0
+        sf = kib / cf[0] if cf else 1.0
+        sw = kib / cw[0] if cw else 1.0
+        lines += ["", "Byte counters (KiB per dispatch, then calibrated with the 256 MiB copy kernel: "
+                  f"FETCH x{sf:.3f}, WRITE x{sw:.3f}):", "", "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM MB/launch |",
+                  "|---|---|---|---|"]
+        for (k, c), v in sorted(f.items()):
+            wv = w.get((k, "WRITE_SIZE"), 0.0)
+            mb = (v * sf + wv * sw) * 1024 / 1e6
+            lines.append(f"| `{short(k)[:90]}` | {v:.0f} | {wv:.0f} | {mb:.1f} |")
+            if args.kernel in k:
+                out["fetch_kib"] = v
+                out["write_kib"] = wv
+                out["fetch_scale"] = sf
+                out["write_scale"] = sw
+                out["traffic_bytes_per_launch"] = (v * sf + wv * sw) * 1024
+    if args.l2:
+        l2 = counters(args.l2)
+        lines += ["", "| kernel | TCC_HIT | TCC_MISS | L2 hit rate |", "|---|---|---|---|"]
+        kernels = sorted({k for k, _ in l2})
+        for k in kernels:
+            h, m = l2.get((k, "TCC_HIT_sum"), 0.0), l2.get((k, "TCC_MISS_sum"), 0.0)
+            if h + m:
+                lines.append(f"| `{short(k)[:90]}` | {h:.0f} | {m:.0f} | {h / (h + m):.3f} |")
+                if args.kernel in k:
+                    out["l2_hit_rate"] = h / (h + m)
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    with open(os.path.join(ROOT, "profiles", args.name + ".md"), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    if "traffic_bytes_per_launch" in out:
+        with open(os.path.join(ROOT, "profiles", args.name + "_pmc.json"), "w") as fh:
+            json.dump(out, fh, indent=1)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
