@@ -2,6 +2,8 @@
 
   lib/librbr.so            HIP kernels for gfx950 + the C ABI of include/rbr.h
   lib/librbr_workloads.so  seeded synthetic inputs (include/rbr_workloads.h)
+  lib/librbr_adapter.so, lib/rbr_adapter_tests
+                           GpuBVH / GpuPathTracingIntegrator against the reference headers
   ../oracle/_build, _ref   the CPU oracle (test infrastructure; built by oracle/Makefile)
 
 Run:  python ray-based-rendering_amd/build.py [--no-oracle]
@@ -66,9 +68,11 @@ def build(oracle: bool = True, jobs: int = 8) -> None:
     out = os.path.join(LIB, "librbr.so")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out + ".tmp", *objs, "-lgomp"])
     os.replace(out + ".tmp", out)
+    make = shutil.which("make") or "make"
     if oracle:
-        make = shutil.which("make") or "make"
         _run([make, "-s", "-j", str(jobs), "-C", os.path.join(ROOT, "oracle"), "all"])
+        # Drop-in adapters against the reference's headers (kept prebuilt where it is absent).
+        _run([make, "-s", "-C", os.path.join(PKG, "adapter"), "all"])
 
 
 if __name__ == "__main__":
