@@ -241,6 +241,26 @@ int rbr_render_moments(rbr_ctx* ctx, const rbr_job* job, double* sum_rgb, double
 /* Time of the last rbr_render call: total ms and per-stage ms (extend, shade, shadow). */
 int rbr_last_render_timing(rbr_ctx* ctx, double* total_ms, double* stage_ms /* 4 */);
 
+/* ---------------------------------------------------------------- path guiding
+ * Extension without a counterpart in the reference, whose integrators sample the BSDF only
+ * (path_tracing_integrator.cpp:40-60).  A per-context MLP maps a shading point (position
+ * scaled by the box lo..hi, outward normal, wo) to a mixture of 8 von Mises-Fisher lobes over
+ * incident directions.  Layout of the network and its parameters: rbr_guide_param_count()
+ * floats, W0[64][224] W1[64][64] W2[64][64] W3[64][64] W4[48][64] b0[64] b1[64] b2[64] b3[64]
+ * b4[48] (W[out][in]; input features 216..223 and outputs 40..47 are padding).  Inference
+ * and training run on the matrix cores in BF16 with FP32 accumulation. */
+int rbr_guide_param_count(void);
+int rbr_guide_init(rbr_ctx* ctx, uint64_t seed, const float lo[3], const float hi[3]);
+int rbr_guide_get_params(rbr_ctx* ctx, float* params, size_t n);
+int rbr_guide_set_params(rbr_ctx* ctx, const float* params, size_t n);
+/* queries: 9 floats each (position, normal, wo); lobes: 40 floats each, 8 x (mu 3, kappa, weight). */
+int rbr_guide_eval(rbr_ctx* ctx, const float* queries, size_t n, float* lobes);
+/* samples: 13 floats each (position 3, normal 3, wo 3, direction 3, weight c >= 0).
+ * loss = (1/n) sum -c log p(direction); grad = its gradient (rbr_guide_param_count floats). */
+int rbr_guide_loss_grad(rbr_ctx* ctx, const float* samples, size_t n, double* loss, float* grad);
+/* One Adam step (beta 0.9 / 0.999) on the loss above; loss (optional) is the pre-step value. */
+int rbr_guide_train_step(rbr_ctx* ctx, const float* samples, size_t n, float lr, double* loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,7 @@ struct DevBuf {
 };
 
 struct RenderState;   // render.hip
+struct GuideState;    // guide_dev.h
 
 }  // namespace rbr
 
@@ -29,6 +30,7 @@ struct rbr_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     rbr::RenderState* render = nullptr;
+    rbr::GuideState* guide = nullptr;
     // Traversal tuning (rbr_set_trace_tuning) and persistent-launch state.
     int traceMode = 0;                 // 0 persistent waves, 1 one ray per lane
     int ldsDepth = rbr::kLdsDepthDefault;
@@ -68,5 +70,6 @@ int launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack, boo
 
 void freeShading(rbr_scene* s);          // render.hip
 void destroyRenderState(rbr_ctx* ctx);   // render.hip
+void destroyGuideState(rbr_ctx* ctx);    // guide.hip
 
 }  // namespace rbr

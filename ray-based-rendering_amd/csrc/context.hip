@@ -149,6 +149,7 @@ void rbr_ctx_destroy(rbr_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     destroyRenderState(ctx);
+    destroyGuideState(ctx);
     freeBuffer(ctx->dCounter);
     freeBuffer(ctx->dSpill);
     (void)hipEventDestroy(ctx->ev0);

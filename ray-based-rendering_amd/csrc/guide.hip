@@ -1,0 +1,611 @@
+// Path-guiding MLP on the matrix cores of gfx950 (BF16 inputs, FP32 accumulation).
+//
+// Kernels:
+//   guideInferKernel     encoding + 5-layer forward for 16 queries per wave, output turned
+//                        into 8 vMF lobes (unit mean, kappa, mixture weight) per query
+//   guideFwdBwdKernel    training tile of 16 samples per wave: forward (activations kept in
+//                        LDS), loss -c*log p(dir) and its gradient w.r.t. the 40 outputs,
+//                        backward through the layers for the activation gradients; writes the
+//                        layer inputs A_l and output gradients dZ_l feature-major to HBM
+//   guideWeightGradKernel  dW_l = dZ_l^T A_l over the batch, one 16x16 block per wave over a
+//                        chunk of samples (MFMA), atomically added into the FP32 gradient
+//   guideBiasGradKernel  db_l = column sums of dZ_l
+//   guideAdamKernel      Adam on the FP32 master weights, refreshed BF16 copy
+// The MFMA used is v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4*(l/16)+i],
+// B[4*(l/16)+i][l%16] and D[4*(l/16)+i][l%16], i = 0..3.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "context.h"
+#include "guide.h"
+#include "guide_dev.h"
+
+namespace rbr {
+
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kWaves = 4;                     // waves per workgroup
+constexpr int kTile = 16;                     // queries / samples per wave tile
+constexpr int kLdsWeights = kGWeights;        // BF16 weights in LDS
+
+struct WaveBuf {
+    uint16_t X[kTile][kGInPad];               // encoded input
+    uint16_t H[4][kTile][kGHid];              // layer inputs A_1..A_4 (post-ReLU)
+    uint16_t G[kTile][kGHid];                 // current dZ (BF16) for backward
+    float O[kTile][kGOutPad];                 // network output (FP32)
+};
+
+__device__ __forceinline__ v4s ld4(const uint16_t* p) { return *reinterpret_cast<const v4s*>(p); }
+
+// Orders a wave's LDS writes before its later LDS reads by other lanes (no block barrier: the
+// waves of a workgroup run different numbers of tiles).
+__device__ __forceinline__ void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// C[16 x N] = A[16 x K] * W^T with W stored [N][K] (row per output unit), + bias, optional
+// ReLU; BF16 result to H (row stride ldh) and/or FP32 result to O.
+template <int K, int N, bool RELU>
+__device__ __forceinline__ void layerForward(const uint16_t* A, int lda, const uint16_t* W, const float* bias,
+                                             uint16_t* H, int ldh, float* O) {
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int nb = 0; nb < N / 16; ++nb) {
+        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int k0 = 0; k0 < K; k0 += 16) {
+            const v4s a = ld4(A + r * lda + k0 + 4 * g);
+            const v4s b = ld4(W + (nb * 16 + r) * K + k0 + 4 * g);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        }
+        const int col = nb * 16 + r;
+        const float bb = bias[col];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float v = acc[i] + bb;
+            if (RELU) v = fmaxf(v, 0.0f);
+            const int row = 4 * g + i;
+            if (H) H[row * ldh + col] = f2bf(v);
+            if (O) O[row * N + col] = v;
+        }
+    }
+}
+
+// dA[16 x K] = dZ[16 x N] * W (W stored [N][K]); then dZ_prev = dA * (Aprev > 0), BF16.
+template <int K, int N>
+__device__ __forceinline__ void layerBackwardData(const uint16_t* dZ, int ldz, const uint16_t* W,
+                                                  const uint16_t* Aprev, uint16_t* dZprev) {
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int kb = 0; kb < K / 16; ++kb) {
+        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int n0 = 0; n0 < N; n0 += 16) {
+            const v4s a = ld4(dZ + r * ldz + n0 + 4 * g);
+            v4s b;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) b[i] = short(W[(n0 + 4 * g + i) * K + kb * 16 + r]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        }
+        const int col = kb * 16 + r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 4 * g + i;
+            const bool on = bf2f(Aprev[row * K + col]) > 0.0f;
+            dZprev[row * K + col] = f2bf(on ? acc[i] : 0.0f);
+        }
+    }
+}
+
+__device__ __forceinline__ void loadWeights(uint16_t* lw, float* lb, const uint16_t* w, const float* b) {
+    for (int i = threadIdx.x; i < kGWeights / 4; i += blockDim.x)
+        reinterpret_cast<uint2*>(lw)[i] = reinterpret_cast<const uint2*>(w)[i];
+    for (int i = threadIdx.x; i < kGBias; i += blockDim.x) lb[i] = b[i];
+}
+
+__device__ __forceinline__ void forwardTile(WaveBuf& wb, const uint16_t* lw, const float* lb) {
+    layerForward<kGInPad, kGHid, true>(&wb.X[0][0], kGInPad, lw + kGOffW[0], lb + 0, &wb.H[0][0][0], kGHid, nullptr);
+    waveSync();
+    layerForward<kGHid, kGHid, true>(&wb.H[0][0][0], kGHid, lw + kGOffW[1], lb + 64, &wb.H[1][0][0], kGHid, nullptr);
+    waveSync();
+    layerForward<kGHid, kGHid, true>(&wb.H[1][0][0], kGHid, lw + kGOffW[2], lb + 128, &wb.H[2][0][0], kGHid, nullptr);
+    waveSync();
+    layerForward<kGHid, kGHid, true>(&wb.H[2][0][0], kGHid, lw + kGOffW[3], lb + 192, &wb.H[3][0][0], kGHid, nullptr);
+    waveSync();
+    layerForward<kGHid, kGOutPad, false>(&wb.H[3][0][0], kGHid, lw + kGOffW[4], lb + 256, nullptr, 0, &wb.O[0][0]);
+}
+
+// Encodes query q of the tile (4 lanes per query split the 9 scalars x 12 frequencies).
+__device__ __forceinline__ void encodeTile(WaveBuf& wb, const float* s9, bool valid) {
+    const int lane = threadIdx.x & 63, q = lane & 15, part = lane >> 4;
+    for (int f = part; f < kGIn / 2; f += 4) {
+        float sv = 0.0f, cv = 0.0f;
+        if (valid) guideFeature(s9[f / kGFreq], f % kGFreq, sv, cv);
+        wb.X[q][2 * f] = f2bf(sv);
+        wb.X[q][2 * f + 1] = f2bf(cv);
+    }
+    if (part == 0)
+        for (int f = kGIn; f < kGInPad; ++f) wb.X[q][f] = 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- inference
+
+// Queries come either from an explicit array (9 floats: position, normal, wo) or from the
+// render's extension queue (hit point, outward normal, -ray direction).
+__global__ __launch_bounds__(256) void guideInferKernel(GuideInferArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint16_t* lw = reinterpret_cast<uint16_t*>(smem);
+    float* lb = reinterpret_cast<float*>(smem + kLdsWeights * 2);
+    WaveBuf* bufs = reinterpret_cast<WaveBuf*>(smem + kLdsWeights * 2 + kGBias * 4 + 64);
+    loadWeights(lw, lb, a.g.w, a.g.b);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveBuf& wb = bufs[wave];
+    const uint32_t n = a.nDev ? *a.nDev : a.n;
+    const uint32_t tiles = (n + kTile - 1) / kTile;
+    for (uint32_t t = blockIdx.x * kWaves + wave; t < tiles; t += gridDim.x * kWaves) {
+        const uint32_t qi = t * kTile + (lane & 15);
+        float s9[9];
+        const bool valid = qi < n && guideQuery(a, qi, s9);
+        encodeTile(wb, s9, valid);
+        waveSync();
+        forwardTile(wb, lw, lb);
+        waveSync();
+        // Outputs -> lobes: lanes (q, part) write lobes 2*part, 2*part+1 of query q.
+        if (qi < n) {
+            const float* o = &wb.O[lane & 15][0];
+            float* out = a.out + size_t(qi) * (kGLobes * 5);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int k = 2 * (lane >> 4) + j;
+                GLobe lb1;
+                decodeLobe(o, k, lb1);
+                out[5 * k + 0] = valid ? lb1.mu[0] : 0.0f;
+                out[5 * k + 1] = valid ? lb1.mu[1] : 0.0f;
+                out[5 * k + 2] = valid ? lb1.mu[2] : 1.0f;
+                out[5 * k + 3] = valid ? lb1.kappa : 1.0f;
+                out[5 * k + 4] = valid ? lb1.w : 0.0f;
+            }
+        }
+        waveSync();
+    }
+}
+
+// ---------------------------------------------------------------- training
+
+__global__ __launch_bounds__(256) void guideFwdBwdKernel(GuideTrainArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint16_t* lw = reinterpret_cast<uint16_t*>(smem);
+    float* lb = reinterpret_cast<float*>(smem + kLdsWeights * 2);
+    WaveBuf* bufs = reinterpret_cast<WaveBuf*>(smem + kLdsWeights * 2 + kGBias * 4 + 64);
+    loadWeights(lw, lb, a.g.w, a.g.b);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane & 15, part = lane >> 4;
+    WaveBuf& wb = bufs[wave];
+    const uint32_t n = a.n;
+    const uint32_t tiles = (n + kTile - 1) / kTile;
+    float lossAcc = 0.0f;
+    for (uint32_t t = blockIdx.x * kWaves + wave; t < tiles; t += gridDim.x * kWaves) {
+        const uint32_t si = t * kTile + q;
+        const bool valid = si < n;
+        float s9[9], dir[3] = {0, 0, 1}, c = 0.0f;
+        if (valid) guideSample(a, si, s9, dir, c);
+        encodeTile(wb, s9, valid);
+        waveSync();
+        forwardTile(wb, lw, lb);
+        waveSync();
+        // Output gradient: each lane evaluates the 8 lobes of its sample and writes the
+        // gradient of lobes 2*part, 2*part+1.
+        float grad[10];
+        const float l = outputGrad(&wb.O[q][0], dir, valid ? c * a.scale : 0.0f, part, grad);
+        if (part == 0 && valid) lossAcc += l;
+        for (int j = 0; j < 10; ++j) wb.G[q][10 * part + j] = f2bf(grad[j]);
+        if (part == 0)
+            for (int j = kGOut; j < kGOutPad; ++j) wb.G[q][j] = 0;
+        waveSync();
+        // Store A_4 and dZ_4, then walk the layers down.
+        const size_t S = a.cap;
+        const uint32_t s0 = t * kTile;
+        auto storeT = [&](uint16_t* dst, const uint16_t* src, int cols, int ld) {
+            for (int idx = lane; idx < cols * kTile; idx += 64) {
+                const int col = idx / kTile, row = idx % kTile;
+                dst[size_t(col) * S + s0 + row] = src[row * ld + col];
+            }
+        };
+        storeT(a.dz[4], &wb.G[0][0], kGOutPad, kGHid);
+        storeT(a.act[4], &wb.H[3][0][0], kGHid, kGHid);
+        waveSync();
+        // A_0 (the input encoding) is needed for dW_0: store it before X is reused as scratch.
+        storeT(a.act[0], &wb.X[0][0], kGInPad, kGInPad);
+        storeT(a.act[1], &wb.H[0][0][0], kGHid, kGHid);
+        storeT(a.act[2], &wb.H[1][0][0], kGHid, kGHid);
+        storeT(a.act[3], &wb.H[2][0][0], kGHid, kGHid);
+        waveSync();
+        uint16_t* scratch = &wb.X[0][0];   // [16][64] BF16 dZ_prev buffer
+        // layer 4 -> dZ_3 (N = 48 outputs, K = 64 inputs)
+        layerBackwardData<kGHid, kGOutPad>(&wb.G[0][0], kGHid, lw + kGOffW[4], &wb.H[3][0][0], scratch);
+        waveSync();
+        storeT(a.dz[3], scratch, kGHid, kGHid);
+        for (int i = lane; i < kTile * kGHid; i += 64) (&wb.G[0][0])[i] = scratch[i];
+        waveSync();
+        layerBackwardData<kGHid, kGHid>(&wb.G[0][0], kGHid, lw + kGOffW[3], &wb.H[2][0][0], scratch);
+        waveSync();
+        storeT(a.dz[2], scratch, kGHid, kGHid);
+        for (int i = lane; i < kTile * kGHid; i += 64) (&wb.G[0][0])[i] = scratch[i];
+        waveSync();
+        layerBackwardData<kGHid, kGHid>(&wb.G[0][0], kGHid, lw + kGOffW[2], &wb.H[1][0][0], scratch);
+        waveSync();
+        storeT(a.dz[1], scratch, kGHid, kGHid);
+        for (int i = lane; i < kTile * kGHid; i += 64) (&wb.G[0][0])[i] = scratch[i];
+        waveSync();
+        layerBackwardData<kGHid, kGHid>(&wb.G[0][0], kGHid, lw + kGOffW[1], &wb.H[0][0][0], scratch);
+        waveSync();
+        storeT(a.dz[0], scratch, kGHid, kGHid);
+        waveSync();
+    }
+    // Loss: one atomic per wave.
+    for (int off = 32; off > 0; off >>= 1) lossAcc += __shfl_down(lossAcc, off);
+    if (lane == 0) atomicAdd(a.loss, double(lossAcc));
+}
+
+// dW_l[n][k] += sum_s dZ_l[n][s] * A_l[k][s] over one chunk of samples; one 16x16 block per
+// wave.  Blocks are enumerated over all five layers.
+__global__ __launch_bounds__(256) void guideWeightGradKernel(GuideTrainArgs a, uint32_t chunk) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const uint32_t blockId = blockIdx.x * kWaves + wave;
+    // Block tables: layer 0: 4 x 14, layers 1-3: 4 x 4, layer 4: 3 x 4.
+    int layer, nb, kb;
+    uint32_t b = blockId;
+    if (b < 56) { layer = 0; nb = b / 14; kb = b % 14; }
+    else if (b < 56 + 48) { b -= 56; layer = 1 + b / 16; nb = (b % 16) / 4; kb = b % 4; }
+    else if (b < 116) { b -= 104; layer = 4; nb = b / 4; kb = b % 4; }
+    else return;
+    const int K = layer == 0 ? kGInPad : kGHid;
+    const size_t S = a.cap;
+    const uint32_t s0 = blockIdx.y * chunk;
+    const uint32_t s1 = min(a.n, s0 + chunk);
+    const uint16_t* dz = a.dz[layer] + size_t(nb * 16 + r) * S;
+    const uint16_t* act = a.act[layer] + size_t(kb * 16 + r) * S;
+    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t s = s0; s < s1; s += 16) {
+        const v4s av = ld4(dz + s + 4 * g);
+        const v4s bv = ld4(act + s + 4 * g);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int nrow = nb * 16 + 4 * g + i, kcol = kb * 16 + r;
+        atomicAdd(&a.grad[kGOffW[layer] + nrow * K + kcol], acc[i]);
+    }
+}
+
+__global__ __launch_bounds__(256) void guideBiasGradKernel(GuideTrainArgs a) {
+    __shared__ float red[256];
+    const int row = blockIdx.x;   // 0..303
+    const int layer = row < 256 ? row / 64 : 4;
+    const int nIdx = row < 256 ? row % 64 : row - 256;
+    const uint16_t* dz = a.dz[layer] + size_t(nIdx) * a.cap;
+    float s = 0.0f;
+    for (uint32_t i = threadIdx.x; i < a.n; i += 256) s += bf2f(dz[i]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.grad[kGOffB[layer] + nIdx] = red[0];
+}
+
+__global__ __launch_bounds__(256) void guideAdamKernel(float* __restrict__ p, float* __restrict__ m,
+                                                       float* __restrict__ v, const float* __restrict__ g,
+                                                       uint16_t* __restrict__ wbf, float lr, float c1, float c2) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= kGParams) return;
+    const float gi = g[i];
+    const float mi = 0.9f * m[i] + 0.1f * gi;
+    const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi * c1) / (sqrtf(vi * c2) + 1e-8f);
+    p[i] = pi;
+    if (i < kGWeights) wbf[i] = f2bf(pi);
+}
+
+__global__ void guideZeroKernel(float* p, int n, double* loss) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+    if (i == 0 && loss) *loss = 0.0;
+}
+
+// ---------------------------------------------------------------- host side
+
+namespace {
+
+GuideState& state(rbr_ctx* ctx) {
+    if (!ctx->guide) ctx->guide = new GuideState();
+    return *ctx->guide;
+}
+
+size_t inferSmem() { return size_t(kLdsWeights) * 2 + kGBias * 4 + 64 + sizeof(WaveBuf) * kWaves; }
+
+int refreshBf16(rbr_ctx* ctx, GuideState& g) {
+    std::vector<uint16_t> bf(kGWeights);
+    for (int i = 0; i < kGWeights; ++i) {
+        uint32_t u;
+        std::memcpy(&u, &g.master[i], 4);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        bf[i] = uint16_t(u >> 16);
+    }
+    int rc;
+    if ((rc = uploadBuffer(ctx, g.dW, bf.data(), bf.size() * 2))) return rc;
+    if ((rc = uploadBuffer(ctx, g.dMaster, g.master.data(), g.master.size() * 4))) return rc;
+    return RBR_OK;
+}
+
+int ensureTrain(rbr_ctx* ctx, GuideState& g, size_t n) {
+    const size_t cap = (n + 15) / 16 * 16;
+    if (cap > g.cap) {
+        for (int l = 0; l < 5; ++l) {
+            const int K = l == 0 ? kGInPad : kGHid;
+            const int N = l == 4 ? kGOutPad : kGHid;
+            if (int rc = ensureBuffer(ctx, g.act[l], cap * K * 2)) return rc;
+            if (int rc = ensureBuffer(ctx, g.dz[l], cap * N * 2)) return rc;
+        }
+        g.cap = cap;
+    }
+    int rc;
+    if ((rc = ensureBuffer(ctx, g.dGrad, size_t(kGParams) * 4))) return rc;
+    if ((rc = ensureBuffer(ctx, g.dM, size_t(kGParams) * 4))) return rc;
+    if ((rc = ensureBuffer(ctx, g.dV, size_t(kGParams) * 4))) return rc;
+    if ((rc = ensureBuffer(ctx, g.dLoss, 8))) return rc;
+    return RBR_OK;
+}
+
+GuideTrainArgs trainArgs(rbr_ctx* ctx, GuideState& g, const float4* samples, uint32_t stride, uint32_t n, float scale) {
+    GuideTrainArgs a{};
+    a.g = guideDev(g);
+    a.samples = samples;
+    a.stride = stride;
+    a.n = n;
+    a.cap = g.cap;
+    a.scale = scale;
+    for (int l = 0; l < 5; ++l) {
+        a.act[l] = static_cast<uint16_t*>(g.act[l].ptr);
+        a.dz[l] = static_cast<uint16_t*>(g.dz[l].ptr);
+    }
+    a.grad = static_cast<float*>(g.dGrad.ptr);
+    a.loss = static_cast<double*>(g.dLoss.ptr);
+    (void)ctx;
+    return a;
+}
+
+int launchLossGrad(rbr_ctx* ctx, GuideState& g, const GuideTrainArgs& a) {
+    hipLaunchKernelGGL(guideZeroKernel, dim3((kGParams + 255) / 256), dim3(256), 0, ctx->stream, a.grad, kGParams, a.loss);
+    const unsigned tiles = (a.n + kTile - 1) / kTile;
+    const unsigned grid = std::max(1u, std::min<unsigned>((tiles + kWaves - 1) / kWaves, unsigned(ctx->cuCount) * 2));
+    hipLaunchKernelGGL(guideFwdBwdKernel, dim3(grid), dim3(256), inferSmem(), ctx->stream, a);
+    const uint32_t chunk = 2048;
+    hipLaunchKernelGGL(guideWeightGradKernel, dim3((116 + kWaves - 1) / kWaves, (a.n + chunk - 1) / chunk), dim3(256), 0,
+                       ctx->stream, a, chunk);
+    hipLaunchKernelGGL(guideBiasGradKernel, dim3(kGBias), dim3(256), 0, ctx->stream, a);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "guide training launch");
+}
+
+}  // namespace
+
+GuideDev guideDev(const GuideState& g) {
+    GuideDev d{};
+    d.w = static_cast<const uint16_t*>(g.dW.ptr);
+    d.b = static_cast<const float*>(g.dMaster.ptr) + kGWeights;
+    for (int k = 0; k < 3; ++k) {
+        d.lo[k] = g.lo[k];
+        d.inv[k] = g.inv[k];
+    }
+    return d;
+}
+
+int guideSetAttributes(rbr_ctx* ctx) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(guideInferKernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(inferSmem()));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(guideFwdBwdKernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(inferSmem()));
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "guide LDS attribute");
+}
+
+int launchGuideInfer(rbr_ctx* ctx, const GuideInferArgs& a, uint32_t maxN) {
+    if (int rc = guideSetAttributes(ctx)) return rc;
+    const unsigned tiles = (maxN + kTile - 1) / kTile;
+    const unsigned grid = std::max(1u, std::min<unsigned>((tiles + kWaves - 1) / kWaves, unsigned(ctx->cuCount)));
+    hipLaunchKernelGGL(guideInferKernel, dim3(grid), dim3(256), inferSmem(), ctx->stream, a);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "guide inference launch");
+}
+
+int guideTrainDevice(rbr_ctx* ctx, const float4* samples, uint32_t stride, uint32_t n, float scale, float lr,
+                     double* lossOut) {
+    GuideState& g = state(ctx);
+    if (!g.ready) return fail(ctx, RBR_ERR_STATE, "guide not initialised (rbr_guide_init)");
+    if (int rc = ensureTrain(ctx, g, n)) return rc;
+    if (!g.adamInit) {
+        hipLaunchKernelGGL(guideZeroKernel, dim3((kGParams + 255) / 256), dim3(256), 0, ctx->stream,
+                           static_cast<float*>(g.dM.ptr), kGParams, nullptr);
+        hipLaunchKernelGGL(guideZeroKernel, dim3((kGParams + 255) / 256), dim3(256), 0, ctx->stream,
+                           static_cast<float*>(g.dV.ptr), kGParams, nullptr);
+        g.adamInit = true;
+        g.step = 0;
+    }
+    if (int rc = guideSetAttributes(ctx)) return rc;
+    GuideTrainArgs a = trainArgs(ctx, g, samples, stride, n, scale);
+    if (int rc = launchLossGrad(ctx, g, a)) return rc;
+    g.step += 1;
+    const float c1 = 1.0f / (1.0f - std::pow(0.9f, float(g.step)));
+    const float c2 = 1.0f / (1.0f - std::pow(0.999f, float(g.step)));
+    hipLaunchKernelGGL(guideAdamKernel, dim3((kGParams + 255) / 256), dim3(256), 0, ctx->stream,
+                       static_cast<float*>(g.dMaster.ptr), static_cast<float*>(g.dM.ptr), static_cast<float*>(g.dV.ptr),
+                       static_cast<const float*>(g.dGrad.ptr), static_cast<uint16_t*>(g.dW.ptr), lr, c1, c2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return failHip(ctx, e, "guide Adam launch");
+    g.masterDirty = true;
+    if (lossOut) {
+        double l = 0.0;
+        e = hipMemcpyAsync(&l, a.loss, 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return failHip(ctx, e, "guide loss download");
+        *lossOut = l;
+    }
+    return RBR_OK;
+}
+
+void destroyGuideState(rbr_ctx* ctx) {
+    GuideState* g = ctx->guide;
+    if (!g) return;
+    for (DevBuf* b : {&g->dW, &g->dMaster, &g->dM, &g->dV, &g->dGrad, &g->dLoss, &g->dQ, &g->dOut, &g->dRec, &g->dRecCount,
+                      &g->dCsum})
+        freeBuffer(*b);
+    for (int l = 0; l < 5; ++l) {
+        freeBuffer(g->act[l]);
+        freeBuffer(g->dz[l]);
+    }
+    delete g;
+    ctx->guide = nullptr;
+}
+
+}  // namespace rbr
+
+using namespace rbr;
+
+extern "C" {
+
+int rbr_guide_init(rbr_ctx* ctx, uint64_t seed, const float lo[3], const float hi[3]) {
+    if (!ctx || !lo || !hi) return RBR_ERR_INVALID;
+    if (int rc = useDevice(ctx)) return rc;
+    GuideState& g = state(ctx);
+    g.master.assign(kGParams, 0.0f);
+    uint64_t s = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto u01 = [&]() {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return float(z >> 40) * (1.0f / 16777216.0f);
+    };
+    // He-uniform initialisation of the real (unpadded) weights; biases zero.
+    const int ins[5] = {kGIn, kGHid, kGHid, kGHid, kGHid};
+    const int Ks[5] = {kGInPad, kGHid, kGHid, kGHid, kGHid};
+    const int outs[5] = {kGHid, kGHid, kGHid, kGHid, kGOut};
+    for (int l = 0; l < 5; ++l) {
+        const float lim = std::sqrt(6.0f / float(ins[l])) * (l == 4 ? 0.1f : 1.0f);
+        for (int o = 0; o < outs[l]; ++o)
+            for (int i = 0; i < ins[l]; ++i) g.master[kGOffW[l] + o * Ks[l] + i] = (2.0f * u01() - 1.0f) * lim;
+    }
+    for (int k = 0; k < 3; ++k) {
+        g.lo[k] = lo[k];
+        const float ext = hi[k] - lo[k];
+        g.inv[k] = ext > 0.0f ? 1.0f / ext : 1.0f;
+    }
+    g.adamInit = false;
+    g.step = 0;
+    g.ready = true;
+    return refreshBf16(ctx, g);
+}
+
+int rbr_guide_get_params(rbr_ctx* ctx, float* params, size_t n) {
+    if (!ctx || !params || n != size_t(kGParams)) return ctx ? fail(ctx, RBR_ERR_INVALID, "params size") : RBR_ERR_INVALID;
+    GuideState& g = state(ctx);
+    if (!g.ready) return fail(ctx, RBR_ERR_STATE, "guide not initialised");
+    if (int rc = useDevice(ctx)) return rc;
+    hipError_t e = hipMemcpyAsync(params, g.dMaster.ptr, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return failHip(ctx, e, "guide params download");
+    g.master.assign(params, params + n);
+    return RBR_OK;
+}
+
+int rbr_guide_set_params(rbr_ctx* ctx, const float* params, size_t n) {
+    if (!ctx || !params || n != size_t(kGParams)) return ctx ? fail(ctx, RBR_ERR_INVALID, "params size") : RBR_ERR_INVALID;
+    GuideState& g = state(ctx);
+    if (!g.ready) return fail(ctx, RBR_ERR_STATE, "guide not initialised");
+    if (int rc = useDevice(ctx)) return rc;
+    g.master.assign(params, params + n);
+    g.adamInit = false;
+    return refreshBf16(ctx, g);
+}
+
+int rbr_guide_param_count(void) { return kGParams; }
+
+int rbr_guide_eval(rbr_ctx* ctx, const float* queries, size_t n, float* lobes) {
+    if (!ctx || (n && (!queries || !lobes))) return RBR_ERR_INVALID;
+    GuideState& g = state(ctx);
+    if (!g.ready) return fail(ctx, RBR_ERR_STATE, "guide not initialised");
+    if (int rc = useDevice(ctx)) return rc;
+    if (n == 0) return RBR_OK;
+    int rc;
+    if ((rc = ensureBuffer(ctx, g.dQ, n * 9 * 4))) return rc;
+    if ((rc = ensureBuffer(ctx, g.dOut, n * kGLobes * 5 * 4))) return rc;
+    hipError_t e = hipMemcpyAsync(g.dQ.ptr, queries, n * 9 * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return failHip(ctx, e, "query upload");
+    GuideInferArgs a{};
+    a.g = guideDev(g);
+    a.queries = static_cast<const float*>(g.dQ.ptr);
+    a.n = uint32_t(n);
+    a.out = static_cast<float*>(g.dOut.ptr);
+    if ((rc = launchGuideInfer(ctx, a, uint32_t(n)))) return rc;
+    e = hipMemcpyAsync(lobes, g.dOut.ptr, n * kGLobes * 5 * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "guide eval");
+}
+
+// samples: 13 floats each (position 3, normal 3, wo 3, direction 3, weight c).
+static int uploadSamples(rbr_ctx* ctx, GuideState& g, const float* samples, size_t n) {
+    std::vector<float4> packed(n * 4);
+    for (size_t i = 0; i < n; ++i) {
+        const float* s = samples + 13 * i;
+        packed[4 * i + 0] = make_float4(s[0], s[1], s[2], s[12]);
+        packed[4 * i + 1] = make_float4(s[3], s[4], s[5], 0.0f);
+        packed[4 * i + 2] = make_float4(s[6], s[7], s[8], 0.0f);
+        packed[4 * i + 3] = make_float4(s[9], s[10], s[11], 0.0f);
+    }
+    return uploadBuffer(ctx, g.dQ, packed.data(), packed.size() * sizeof(float4));
+}
+
+int rbr_guide_loss_grad(rbr_ctx* ctx, const float* samples, size_t n, double* loss, float* grad) {
+    if (!ctx || !samples || n == 0 || n > (1u << 26)) return RBR_ERR_INVALID;
+    GuideState& g = state(ctx);
+    if (!g.ready) return fail(ctx, RBR_ERR_STATE, "guide not initialised");
+    if (int rc = useDevice(ctx)) return rc;
+    int rc;
+    if ((rc = uploadSamples(ctx, g, samples, n))) return rc;
+    if ((rc = ensureTrain(ctx, g, n))) return rc;
+    if ((rc = guideSetAttributes(ctx))) return rc;
+    GuideTrainArgs a = trainArgs(ctx, g, static_cast<const float4*>(g.dQ.ptr), 4, uint32_t(n), 1.0f / float(n));
+    if ((rc = launchLossGrad(ctx, g, a))) return rc;
+    hipError_t e = hipSuccess;
+    if (grad) e = hipMemcpyAsync(grad, g.dGrad.ptr, size_t(kGParams) * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && loss) e = hipMemcpyAsync(loss, g.dLoss.ptr, 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "guide gradient download");
+}
+
+int rbr_guide_train_step(rbr_ctx* ctx, const float* samples, size_t n, float lr, double* loss) {
+    if (!ctx || !samples || n == 0 || n > (1u << 26)) return RBR_ERR_INVALID;
+    GuideState& g = state(ctx);
+    if (!g.ready) return fail(ctx, RBR_ERR_STATE, "guide not initialised");
+    if (int rc = useDevice(ctx)) return rc;
+    if (int rc = uploadSamples(ctx, g, samples, n)) return rc;
+    return guideTrainDevice(ctx, static_cast<const float4*>(g.dQ.ptr), 4, uint32_t(n), 1.0f / float(n), lr, loss);
+}
+
+}  // extern "C"

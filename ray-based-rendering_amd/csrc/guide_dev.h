@@ -1,0 +1,209 @@
+// Device helpers and host state of the guiding network (guide.hip, render.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "context.h"
+#include "guide.h"
+#include "shading.h"
+
+namespace rbr {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
+    return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+
+// sin/cos(2^k * pi * x) = sin/cos(2*pi * frac(2^(k-1) * x)); the scaling by a power of two and
+// the fraction are exact, so the feature is accurate at every frequency.
+__device__ __forceinline__ void guideFeature(float x, int k, float& s, float& c) {
+    const float r = x * __builtin_ldexpf(1.0f, k - 1);
+    const float f = r - floorf(r);
+    s = __builtin_amdgcn_sinf(f);
+    c = __builtin_amdgcn_cosf(f);
+}
+
+struct GuideInferArgs {
+    GuideDev g;
+    // explicit queries: 9 floats each (world position, outward normal, wo)
+    const float* queries;
+    uint32_t n;
+    const uint32_t* nDev;
+    // render queue: rays, hits, path slots, jobs
+    const float4* rays;
+    const uint2* hits;
+    const uint32_t* paths;
+    const DJob* jobs;
+    int njobs;
+    float* out;               // 40 floats per query: 8 x (mu 3, kappa, weight)
+};
+
+struct GuideTrainArgs {
+    GuideDev g;
+    const float4* samples;    // per sample: (pos, c) (normal, -) (wo, -) (dir, -) [+ stride padding]
+    uint32_t stride;          // float4 per sample
+    uint32_t n;
+    size_t cap;               // row length of the feature-major activation arrays
+    float scale;              // multiplies c (1 / (batch * mean c))
+    uint16_t* act[5];         // A_l, feature-major [K_l][cap]
+    uint16_t* dz[5];          // dZ_l, feature-major [N_l][cap]
+    float* grad;              // FP32 gradient [kGParams]
+    double* loss;
+};
+
+__device__ __forceinline__ int findJobD(const DJob* jobs, int njobs, uint32_t p) {
+    int j = 0;
+    while (j + 1 < njobs && p >= jobs[j + 1].pathBase) ++j;
+    return j;
+}
+
+__device__ __forceinline__ void scaleQuery(const GuideDev& g, const float* pos, const float* n, const float* wo, float* s9) {
+    for (int k = 0; k < 3; ++k) {
+        s9[k] = (pos[k] - g.lo[k]) * g.inv[k];
+        s9[3 + k] = n[k];
+        s9[6 + k] = wo[k];
+    }
+}
+
+__device__ __forceinline__ bool guideQuery(const GuideInferArgs& a, uint32_t i, float* s9) {
+    if (a.queries) {
+        const float* q = a.queries + size_t(i) * 9;
+        scaleQuery(a.g, q, q + 3, q + 6, s9);
+        return true;
+    }
+    const uint32_t p = a.paths[i];
+    const DJob& jb = a.jobs[findJobD(a.jobs, a.njobs, p)];
+    if (!(jb.guided & 1u)) return false;
+    const uint2 h = a.hits[i];
+    if (h.x == 0xFFFFFFFFu) return false;
+    const float4 s0 = jb.primShade[2 * size_t(h.x)], s1 = jb.primShade[2 * size_t(h.x) + 1];
+    if (__float_as_int(s1.w) < 0) return false;
+    const float4 r0 = a.rays[2 * size_t(i)], r1 = a.rays[2 * size_t(i) + 1];
+    const float t = __uint_as_float(h.y);
+    const float P[3] = {r0.x + r0.w * t, r0.y + r1.x * t, r0.z + r1.y * t};
+    float n[3];
+    if (__float_as_uint(s0.w) == 1u) {
+        const float inv = 1.0f / s1.x;
+        n[0] = (P[0] - s0.x) * inv;
+        n[1] = (P[1] - s0.y) * inv;
+        n[2] = (P[2] - s0.z) * inv;
+    } else {
+        n[0] = s0.x;
+        n[1] = s0.y;
+        n[2] = s0.z;
+    }
+    const float wo[3] = {-r0.w, -r1.x, -r1.y};
+    scaleQuery(a.g, P, n, wo, s9);
+    return true;
+}
+
+__device__ __forceinline__ void guideSample(const GuideTrainArgs& a, uint32_t i, float* s9, float* dir, float& c) {
+    const float4* s = a.samples + size_t(i) * a.stride;
+    const float4 p = s[0], n = s[1], wo = s[2], d = s[3];
+    const float pp[3] = {p.x, p.y, p.z}, nn[3] = {n.x, n.y, n.z}, ww[3] = {wo.x, wo.y, wo.z};
+    scaleQuery(a.g, pp, nn, ww, s9);
+    dir[0] = d.x;
+    dir[1] = d.y;
+    dir[2] = d.z;
+    c = isfinite(p.w) ? fmaxf(p.w, 0.0f) : 0.0f;
+}
+
+// log(1 - exp(-x)) for x > 0
+__device__ __forceinline__ float log1mexp(float x) { return x < 0.693f ? logf(-expm1f(-x)) : log1pf(-expf(-x)); }
+
+constexpr float kLog2Pi = 1.8378770664093453f;
+
+// Lobe k of the network output o (per lobe: mean 3, log kappa, logit): unit mean, kappa
+// clamped to [e^-5, e^8], softmax weight.
+__device__ __forceinline__ void decodeLobe(const float* o, int k, GLobe& lobe) {
+    float mx = -INFINITY;
+    for (int j = 0; j < kGLobes; ++j) mx = fmaxf(mx, o[5 * j + 4]);
+    float se = 0.0f;
+    for (int j = 0; j < kGLobes; ++j) se += expf(o[5 * j + 4] - mx);
+    const float mx0 = o[5 * k], my = o[5 * k + 1], mz = o[5 * k + 2];
+    const float ln = fmaxf(sqrtf(mx0 * mx0 + my * my + mz * mz), 1e-6f);
+    lobe.mu[0] = mx0 / ln;
+    lobe.mu[1] = my / ln;
+    lobe.mu[2] = mz / ln;
+    lobe.kappa = expf(fminf(fmaxf(o[5 * k + 3], kGLogKappaMin), kGLogKappaMax));
+    lobe.w = expf(o[5 * k + 4] - mx) / se;
+}
+
+// Loss -c*log p(dir) of one sample and its gradient w.r.t. the 10 outputs of lobes
+// 2*part and 2*part+1 (layout per lobe: mu 3, log kappa, logit).  Returns the loss.
+__device__ __forceinline__ float outputGrad(const float* o, const float* dir, float c, int part, float* grad) {
+    float a[kGLobes], wlog[kGLobes];
+    float mx = -INFINITY;
+    for (int k = 0; k < kGLobes; ++k) mx = fmaxf(mx, o[5 * k + 4]);
+    float se = 0.0f;
+    for (int k = 0; k < kGLobes; ++k) se += expf(o[5 * k + 4] - mx);
+    const float lse = mx + logf(se);
+    float amax = -INFINITY;
+    for (int k = 0; k < kGLobes; ++k) {
+        const float ln = fmaxf(sqrtf(o[5 * k] * o[5 * k] + o[5 * k + 1] * o[5 * k + 1] + o[5 * k + 2] * o[5 * k + 2]), 1e-6f);
+        const float d = (o[5 * k] * dir[0] + o[5 * k + 1] * dir[1] + o[5 * k + 2] * dir[2]) / ln;
+        const float l = fminf(fmaxf(o[5 * k + 3], kGLogKappaMin), kGLogKappaMax);
+        const float kap = expf(l);
+        wlog[k] = o[5 * k + 4] - lse;
+        a[k] = wlog[k] + l - kLog2Pi - log1mexp(2.0f * kap) + kap * (d - 1.0f);
+        amax = fmaxf(amax, a[k]);
+    }
+    float sa = 0.0f;
+    for (int k = 0; k < kGLobes; ++k) sa += expf(a[k] - amax);
+    const float logp = amax + logf(sa);
+    for (int j = 0; j < 2; ++j) {
+        const int k = 2 * part + j;
+        const float r = expf(a[k] - logp);
+        const float w = expf(wlog[k]);
+        const float ln = fmaxf(sqrtf(o[5 * k] * o[5 * k] + o[5 * k + 1] * o[5 * k + 1] + o[5 * k + 2] * o[5 * k + 2]), 1e-6f);
+        const float mu[3] = {o[5 * k] / ln, o[5 * k + 1] / ln, o[5 * k + 2] / ln};
+        const float d = mu[0] * dir[0] + mu[1] * dir[1] + mu[2] * dir[2];
+        const float lraw = o[5 * k + 3];
+        const bool clamped = lraw < kGLogKappaMin || lraw > kGLogKappaMax;
+        const float kap = expf(fminf(fmaxf(lraw, kGLogKappaMin), kGLogKappaMax));
+        const float x = 2.0f * kap;
+        const float ratio = x < 1e-4f ? 1.0f - 0.5f * x : x / expm1f(x);   // 2k e^{-2k} / (1 - e^{-2k})
+        const float dl = clamped ? 0.0f : -c * r * (1.0f - ratio + kap * (d - 1.0f));
+        const float gm = -c * r * kap / ln;
+        for (int q = 0; q < 3; ++q) grad[5 * j + q] = gm * (dir[q] - d * mu[q]);
+        grad[5 * j + 3] = dl;
+        grad[5 * j + 4] = c * (w - r);
+    }
+    return isfinite(logp) ? -c * logp : 0.0f;
+}
+
+// vMF density of direction w for lobe (mu, kappa): kappa / (2 pi (1 - e^{-2 kappa})) e^{kappa (mu.w - 1)}
+__device__ __forceinline__ float vmfPdf(const float* mu, float kappa, const float* w) {
+    const float d = mu[0] * w[0] + mu[1] * w[1] + mu[2] * w[2];
+    return expf(logf(kappa) - kLog2Pi - log1mexp(2.0f * kappa) + kappa * (d - 1.0f));
+}
+
+struct GuideState {
+    bool ready = false, adamInit = false, masterDirty = false;
+    int step = 0;
+    float lo[3] = {0, 0, 0}, inv[3] = {1, 1, 1};
+    std::vector<float> master;
+    DevBuf dW, dMaster, dM, dV, dGrad, dLoss, dQ, dOut;
+    DevBuf act[5], dz[5];
+    size_t cap = 0;
+    // samples recorded by the renderer: 6 float4 per record, a device counter and the sum of c
+    DevBuf dRec, dRecCount, dCsum;
+    uint32_t recCap = 0;
+    uint64_t trainRng = 0x1234567ull;
+};
+
+GuideDev guideDev(const GuideState& g);
+int launchGuideInfer(rbr_ctx* ctx, const GuideInferArgs& a, uint32_t maxN);
+int guideTrainDevice(rbr_ctx* ctx, const float4* samples, uint32_t stride, uint32_t n, float scale, float lr,
+                     double* lossOut);
+int guideSetAttributes(rbr_ctx* ctx);
+
+}  // namespace rbr

@@ -392,8 +392,13 @@ def c_camera(cam) -> RbrCamera:
 
 
 def c_config(spp: int, depth: int, seed: int = 1, integrator: int = PATH, nee: bool = True, mis: bool = True,
-             sample_offset: int = 0) -> RbrRenderConfig:
-    return RbrRenderConfig(integrator, spp, depth, int(nee), int(mis), 0, seed, sample_offset, 0)
+             sample_offset: int = 0, guided: int = 0) -> RbrRenderConfig:
+    return RbrRenderConfig(integrator, spp, depth, int(nee), int(mis), int(guided), seed, sample_offset, 0)
+
+
+def _job_config(j: dict) -> RbrRenderConfig:
+    return c_config(j["spp"], j["depth"], j.get("seed", 1), j.get("integrator", PATH), j.get("nee", True),
+                    j.get("mis", True), j.get("sample_offset", 0), j.get("guided", 0))
 
 
 def scene_from_desc(ctx: Context, desc, width: int = 4, leaf: int = 4) -> Scene:
@@ -414,16 +419,15 @@ def scene_from_desc(ctx: Context, desc, width: int = 4, leaf: int = 4) -> Scene:
 def render(ctx: Context, jobs: list) -> list[np.ndarray]:
     """Integrator::render for one or many jobs in one launch sequence.  Each job is a dict
     with scene (a Scene from scene_from_desc), spp, depth and optional seed, integrator, nee,
-    mis, camera.  Returns one (H, W, 3) float32 film per job (per-pixel mean radiance)."""
+    mis, camera, sample_offset, guided (1: sample the context's guide, 3: also record training
+    samples).  Returns one (H, W, 3) float32 film per job (per-pixel mean radiance)."""
     arr = (RbrJob * len(jobs))()
     films = []
     for i, j in enumerate(jobs):
         cam = j.get("camera") or j["scene"].desc.camera
         film = np.zeros((cam.height, cam.width, 3), np.float32)
         films.append(film)
-        arr[i] = RbrJob(j["scene"].handle.value, c_camera(cam),
-                        c_config(j["spp"], j["depth"], j.get("seed", 1), j.get("integrator", PATH), j.get("nee", True),
-                                 j.get("mis", True), j.get("sample_offset", 0)),
+        arr[i] = RbrJob(j["scene"].handle.value, c_camera(cam), _job_config(j),
                         film.ctypes.data_as(C.POINTER(C.c_float)))
     ctx.check(_lib.rbr_render(ctx.handle, arr, len(jobs)))
     return films
@@ -435,10 +439,7 @@ def render_moments(ctx: Context, job: dict):
     film = np.zeros((cam.height, cam.width, 3), np.float32)
     s1 = np.zeros((cam.height, cam.width, 3), np.float64)
     s2 = np.zeros_like(s1)
-    j = RbrJob(job["scene"].handle.value, c_camera(cam),
-               c_config(job["spp"], job["depth"], job.get("seed", 1), job.get("integrator", PATH), job.get("nee", True),
-                        job.get("mis", True), job.get("sample_offset", 0)),
-               film.ctypes.data_as(C.POINTER(C.c_float)))
+    j = RbrJob(job["scene"].handle.value, c_camera(cam), _job_config(job), film.ctypes.data_as(C.POINTER(C.c_float)))
     ctx.check(_lib.rbr_render_moments(ctx.handle, C.byref(j), s1.ctypes.data, s2.ctypes.data))
     return film, s1, s2
 
@@ -448,3 +449,60 @@ def last_render_ms(ctx: Context) -> float:
     st = (C.c_double * 4)()
     ctx.check(_lib.rbr_last_render_timing(ctx.handle, C.byref(t), st))
     return t.value
+
+
+# ---------------------------------------------------------------- path guiding
+
+_sig(_lib.rbr_guide_param_count, [])
+_sig(_lib.rbr_guide_init, [_vp, C.c_uint64, _vp, _vp])
+_sig(_lib.rbr_guide_get_params, [_vp, _vp, _sz])
+_sig(_lib.rbr_guide_set_params, [_vp, _vp, _sz])
+_sig(_lib.rbr_guide_eval, [_vp, _vp, _sz, _vp])
+_sig(_lib.rbr_guide_loss_grad, [_vp, _vp, _sz, C.POINTER(C.c_double), _vp])
+_sig(_lib.rbr_guide_train_step, [_vp, _vp, _sz, C.c_float, C.POINTER(C.c_double)])
+
+GUIDE_PARAMS = _lib.rbr_guide_param_count()
+
+
+class Guide:
+    """The context's path-guiding network (rbr_guide_*)."""
+
+    def __init__(self, ctx: Context, lo, hi, seed: int = 1):
+        self.ctx = ctx
+        lo = np.ascontiguousarray(lo, np.float32)
+        hi = np.ascontiguousarray(hi, np.float32)
+        ctx.check(_lib.rbr_guide_init(ctx.handle, seed, lo.ctypes.data, hi.ctypes.data))
+        self.lo, self.hi = lo, hi
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.zeros(GUIDE_PARAMS, np.float32)
+        self.ctx.check(_lib.rbr_guide_get_params(self.ctx.handle, p.ctypes.data, GUIDE_PARAMS))
+        return p
+
+    @params.setter
+    def params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (GUIDE_PARAMS,)
+        self.ctx.check(_lib.rbr_guide_set_params(self.ctx.handle, p.ctypes.data, GUIDE_PARAMS))
+
+    def eval(self, queries) -> np.ndarray:
+        """queries (n, 9): position, normal, wo -> (n, 8, 5) lobes (mu 3, kappa, weight)."""
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 9)
+        out = np.zeros((len(q), 8, 5), np.float32)
+        self.ctx.check(_lib.rbr_guide_eval(self.ctx.handle, q.ctypes.data, len(q), out.ctypes.data))
+        return out
+
+    def loss_grad(self, samples) -> tuple[float, np.ndarray]:
+        """samples (n, 13): position, normal, wo, direction, weight -> (mean loss, gradient)."""
+        s = np.ascontiguousarray(samples, np.float32).reshape(-1, 13)
+        g = np.zeros(GUIDE_PARAMS, np.float32)
+        loss = C.c_double(0)
+        self.ctx.check(_lib.rbr_guide_loss_grad(self.ctx.handle, s.ctypes.data, len(s), C.byref(loss), g.ctypes.data))
+        return loss.value, g
+
+    def train_step(self, samples, lr: float = 1e-3) -> float:
+        s = np.ascontiguousarray(samples, np.float32).reshape(-1, 13)
+        loss = C.c_double(0)
+        self.ctx.check(_lib.rbr_guide_train_step(self.ctx.handle, s.ctypes.data, len(s), lr, C.byref(loss)))
+        return loss.value
