@@ -253,13 +253,29 @@ int rbr_guide_param_count(void);
 int rbr_guide_init(rbr_ctx* ctx, uint64_t seed, const float lo[3], const float hi[3]);
 int rbr_guide_get_params(rbr_ctx* ctx, float* params, size_t n);
 int rbr_guide_set_params(rbr_ctx* ctx, const float* params, size_t n);
-/* queries: 9 floats each (position, normal, wo); lobes: 40 floats each, 8 x (mu 3, kappa, weight). */
+/* queries: 9 floats each (position, normal, wo); lobes: 40 floats each, 8 x (mu 3, kappa, weight),
+ * at the FP16 precision the renderer samples them with. */
 int rbr_guide_eval(rbr_ctx* ctx, const float* queries, size_t n, float* lobes);
+/* Draws per_query directions from each query's mixture with the renderer's sampler (u: 3
+ * uniforms per direction; lobe choice, polar angle, azimuth) and returns them with their
+ * density: dirs 3 floats and pdfs 1 float per direction, query-major. */
+int rbr_guide_sample(rbr_ctx* ctx, const float* queries, size_t nq, size_t per_query, const float* u, float* dirs,
+                     float* pdfs);
 /* samples: 13 floats each (position 3, normal 3, wo 3, direction 3, weight c >= 0).
  * loss = (1/n) sum -c log p(direction); grad = its gradient (rbr_guide_param_count floats). */
 int rbr_guide_loss_grad(rbr_ctx* ctx, const float* samples, size_t n, double* loss, float* grad);
 /* One Adam step (beta 0.9 / 0.999) on the loss above; loss (optional) is the pre-step value. */
 int rbr_guide_train_step(rbr_ctx* ctx, const float* samples, size_t n, float lr, double* loss);
+
+/* Guided rendering: rbr_render jobs with config.guided = 1 draw each non-delta continuation
+ * direction from 0.5 x guide + 0.5 x BSDF (one-sample MIS; NEE and its weights unchanged, so
+ * the image converges to the plain integrator's); guided = 3 also records one training
+ * sample per guided vertex.  Records stay on the device until cleared. */
+int rbr_guide_reserve(rbr_ctx* ctx, uint32_t capacity);      /* record capacity, default 2^22 */
+int rbr_guide_collected(rbr_ctx* ctx, uint64_t* count, double* weight_sum);
+/* steps Adam steps, each on `batch` records drawn without repetition (random permutation). */
+int rbr_guide_train_collected(rbr_ctx* ctx, int steps, uint32_t batch, float lr, double* loss);
+int rbr_guide_clear_collected(rbr_ctx* ctx);
 
 #ifdef __cplusplus
 }

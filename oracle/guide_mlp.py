@@ -15,10 +15,10 @@ pins this restatement instead (tests/test_oracle_guide.py):
 
 Network (all shapes as the device stores them):
   input  : 9 scalars (position scaled to [0,1]^3 by the box lo..hi, outward normal, wo)
-           feature f = 12*j + k of scalar j at frequency k: x[2f] = sin(2^k pi s_j),
-           x[2f+1] = cos(2^k pi s_j); 216 features, padded with zeros to 224
+           108 (sin, cos) pairs sin/cos(2^k pi s_j), j < 9, k < 12, in the order of
+           feature_pairs(), plus 4 zero pairs: 224 features
   layers : W0[64][224] W1..W3[64][64] W4[48][64], biases b0..b3[64], b4[48]; ReLU after
-           layers 0..3; outputs 40..47 are padding
+           layers 0..3; the padding features and outputs 40..47 carry no information
   output : 8 lobes x (mean 3, log kappa, logit); mean normalised, log kappa clamped to
            [-5, 8] (zero gradient outside), weights = softmax(logits)
   density: p(w) = sum_k w_k kappa_k / (2 pi (1 - e^{-2 kappa_k})) e^{kappa_k (mu_k . w - 1)}
@@ -81,18 +81,40 @@ def scale_queries(q9: np.ndarray, lo, hi) -> np.ndarray:
     return q.astype(np.float64)
 
 
+def feature_pairs() -> list:
+    """Scalar j and frequency k of each of the 112 (sin, cos) feature pairs, None for the 4
+    zero padding pairs.  The order is the device's operand layout (guide.h guidePair): pair p
+    sits in k-block p // 16 and slot group g = (p % 8) // 2, and lane group g owns the local
+    pairs q = 4 (p // 16) + 2 ((p % 16) // 8) + p % 2 = scalar q // 3 at frequency 3g + q % 3."""
+    out = []
+    for p in range(N_IN_PAD // 2):
+        kb, rem = divmod(p, 16)
+        half, g, t = rem // 8, (rem % 8) // 2, rem % 2
+        q = 4 * kb + 2 * half + t
+        out.append(None if q >= 27 else (q // 3, 3 * g + q % 3))
+    return out
+
+
 def encode(s9: np.ndarray) -> np.ndarray:
-    """(n, 9) scalars -> (n, 224) features.  sin/cos(2^k pi x) = sin/cos(2 pi frac(2^(k-1) x))."""
+    """(n, 9) scalars -> (n, 224) features: for pair p = (j, k), x[2p] = sin(2^k pi s_j),
+    x[2p+1] = cos(2^k pi s_j), evaluated as sin/cos(2 pi frac(2^(k-1) s_j))."""
     s = np.asarray(s9, np.float64)
     n = s.shape[0]
     X = np.zeros((n, N_IN_PAD))
-    for j in range(9):
-        for k in range(FREQ):
-            r = s[:, j] * 2.0 ** (k - 1)
-            f = r - np.floor(r)
-            X[:, 2 * (12 * j + k)] = np.sin(2.0 * np.pi * f)
-            X[:, 2 * (12 * j + k) + 1] = np.cos(2.0 * np.pi * f)
+    for p, jk in enumerate(feature_pairs()):
+        if jk is None:
+            continue
+        j, k = jk
+        r = s[:, j] * 2.0 ** (k - 1)
+        f = r - np.floor(r)
+        X[:, 2 * p] = np.sin(2.0 * np.pi * f)
+        X[:, 2 * p + 1] = np.cos(2.0 * np.pi * f)
     return X
+
+
+def padding_features() -> np.ndarray:
+    """Indices of the 8 zero input features."""
+    return np.array([2 * p + t for p, jk in enumerate(feature_pairs()) if jk is None for t in (0, 1)])
 
 
 def forward(params: np.ndarray, X: np.ndarray, quant: bool = False):
@@ -225,11 +247,13 @@ def init_params(seed: int) -> np.ndarray:
     ins = [N_IN, 64, 64, 64, 64]
     outs = [64, 64, 64, 64, N_OUT]
     offs = np.cumsum([0] + [a * b for a, b in W_SHAPES])
+    real0 = [2 * p + t for p, jk in enumerate(feature_pairs()) if jk is not None for t in (0, 1)]
     for l in range(5):
         lim = np.float32(np.sqrt(np.float32(6.0) / np.float32(ins[l]))) * np.float32(0.1 if l == 4 else 1.0)
         K = W_SHAPES[l][1]
+        cols = real0 if l == 0 else list(range(ins[l]))
         for o in range(outs[l]):
-            for i in range(ins[l]):
+            for i in cols:
                 s = (s + 0x9E3779B97F4A7C15) & mask
                 z = s
                 z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask

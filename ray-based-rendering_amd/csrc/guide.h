@@ -3,7 +3,8 @@
 // A small MLP maps a shading point to a mixture of 8 von Mises-Fisher lobes over incident
 // directions:
 //   input  : 9 scalars (position scaled to [0,1]^3 by the scene box, outward normal, wo),
-//            each encoded as sin/cos(2^k * pi * x) for k = 0..11  -> 216 features (padded to 224)
+//            each encoded as sin/cos(2^k * pi * x) for k = 0..11  -> 216 features (padded to
+//            224, in the order of guidePair)
 //   hidden : 4 layers of 64 ReLU units
 //   output : 40 = 8 lobes x (mean 3, log kappa 1, logit 1)
 // Weights live as W[out][in] in BF16 for the matrix cores (with an FP32 master copy and
@@ -27,6 +28,31 @@ constexpr int kGOffW[5] = {0, kGW0, kGW0 + kGW1, kGW0 + 2 * kGW1, kGW0 + 3 * kGW
 constexpr int kGOffB[5] = {kGWeights, kGWeights + 64, kGWeights + 128, kGWeights + 192, kGWeights + 256};
 constexpr float kGLogKappaMin = -5.0f, kGLogKappaMax = 8.0f;
 
+// Input feature order.  Pair p (0..111) is features 2p = sin(2^k pi s_j), 2p+1 = cos(...);
+// guidePair gives (j, k) for it, or returns false for the 4 zero padding pairs.  The order
+// follows the matrix-core operand layout: with k-block kb = p / 16 and slot group
+// g = (p % 8) / 2, the lane group g owns local pairs q = 4 kb + 2 ((p % 16) / 8) + p % 2 and
+// produces scalar j = q / 3 at frequencies 3g, 3g+1, 3g+2 from one sine/cosine by doubling.
+__host__ __device__ inline bool guidePair(int p, int& j, int& k) {
+    const int kb = p >> 4, rem = p & 15, half = rem >> 3, g = (rem & 7) >> 1, t = rem & 1;
+    const int q = 4 * kb + 2 * half + t;
+    if (q >= 27) return false;
+    j = q / 3;
+    k = 3 * g + q % 3;
+    return true;
+}
+
+// Matrix-core fragment order of the BF16 weights used by inference: layer l, output row,
+// input column -> element index.  Fragment (l, mb, kb) = 64 lanes x 8 BF16; lane r + 16 g,
+// slot i holds W_l[16 mb + r][32 kb + 4 g + (i & 3) + 16 (i >> 2)].
+constexpr int kGFragKB[5] = {7, 2, 2, 2, 2};
+constexpr int kGFragOff[5] = {0, 28, 36, 44, 52};   // 58 fragments in all
+__host__ __device__ inline int guideFragPos(int layer, int row, int col) {
+    const int mb = row >> 4, r = row & 15, kb = col >> 5, c = col & 31;
+    const int g = (c & 15) >> 2, i = 4 * (c >> 4) + (c & 3);
+    return ((kGFragOff[layer] + mb * kGFragKB[layer] + kb) * 64 + r + 16 * g) * 8 + i;
+}
+
 // One vMF lobe as the samplers use it.
 struct GLobe {
     float mu[3];
@@ -35,7 +61,8 @@ struct GLobe {
 };
 
 struct GuideDev {
-    const uint16_t* w;     // BF16 weights [kGWeights]
+    const uint16_t* w;     // BF16 weights [kGWeights], row-major W[out][in]
+    const uint16_t* wf;    // the same in matrix-core fragment order (guideFragPos)
     const float* b;        // FP32 biases [kGBias]
     float lo[3], inv[3];   // position scaling to [0,1]
 };

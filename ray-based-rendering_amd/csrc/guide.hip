@@ -1,8 +1,13 @@
 // Path-guiding MLP on the matrix cores of gfx950 (BF16 inputs, FP32 accumulation).
 //
 // Kernels:
-//   guideInferKernel     encoding + 5-layer forward for 16 queries per wave, output turned
-//                        into 8 vMF lobes (unit mean, kappa, mixture weight) per query
+//   guideInferKernel     encoding + 5-layer forward, register resident: every layer is
+//                        computed transposed (H^T = W X^T, 16x16x32 MFMA with the weights as
+//                        the A operand), so a layer's accumulators are already the next
+//                        layer's B operand; weights are read from LDS in fragment order (one
+//                        16-byte read per lane per fragment, shared by the wave's 4 tiles of 16
+//                        queries); outputs go through a small LDS stage to be turned into 8
+//                        vMF lobes per query (FP16 record)
 //   guideFwdBwdKernel    training tile of 16 samples per wave: forward (activations kept in
 //                        LDS), loss -c*log p(dir) and its gradient w.r.t. the 40 outputs,
 //                        backward through the layers for the activation gradients; writes the
@@ -10,9 +15,11 @@
 //   guideWeightGradKernel  dW_l = dZ_l^T A_l over the batch, one 16x16 block per wave over a
 //                        chunk of samples (MFMA), atomically added into the FP32 gradient
 //   guideBiasGradKernel  db_l = column sums of dZ_l
-//   guideAdamKernel      Adam on the FP32 master weights, refreshed BF16 copy
-// The MFMA used is v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4*(l/16)+i],
-// B[4*(l/16)+i][l%16] and D[4*(l/16)+i][l%16], i = 0..3.
+//   guideAdamKernel      Adam on the FP32 master weights, refreshed BF16 copies
+// The training kernels use v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4*(l/16)+i],
+// B[4*(l/16)+i][l%16] and D[4*(l/16)+i][l%16], i = 0..3.  For the 16x16x32 form only the M/N
+// placement matters (as above); which K index a (lane group, slot) stands for is the
+// caller's choice as long as A and B agree, and guideFragPos / guideEncodeBlock fix it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -123,61 +130,187 @@ __device__ __forceinline__ void forwardTile(WaveBuf& wb, const uint16_t* lw, con
     layerForward<kGHid, kGOutPad, false>(&wb.H[3][0][0], kGHid, lw + kGOffW[4], lb + 256, nullptr, 0, &wb.O[0][0]);
 }
 
-// Encodes query q of the tile (4 lanes per query split the 9 scalars x 12 frequencies).
+// Encodes the tile's queries into X: lane (q, g) produces the pairs of lane group g of
+// query q (guidePair's order), the same values the inference kernel computes.
 __device__ __forceinline__ void encodeTile(WaveBuf& wb, const float* s9, bool valid) {
-    const int lane = threadIdx.x & 63, q = lane & 15, part = lane >> 4;
-    for (int f = part; f < kGIn / 2; f += 4) {
-        float sv = 0.0f, cv = 0.0f;
-        if (valid) guideFeature(s9[f / kGFreq], f % kGFreq, sv, cv);
-        wb.X[q][2 * f] = f2bf(sv);
-        wb.X[q][2 * f + 1] = f2bf(cv);
+    const int lane = threadIdx.x & 63, q = lane & 15, g = lane >> 4;
+    float z9[9];
+    for (int j = 0; j < 9; ++j) z9[j] = valid ? s9[j] : 0.0f;
+    float2 st = make_float2(0.0f, 1.0f);
+#pragma unroll
+    for (int kb = 0; kb < 7; ++kb) {
+        const u32x4 v = guideEncodeBlock(z9, g, kb, st);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const int p = 16 * kb + 8 * (w >> 1) + 2 * g + (w & 1);
+            *reinterpret_cast<uint32_t*>(&wb.X[q][2 * p]) = valid ? v[w] : 0u;
+        }
     }
-    if (part == 0)
-        for (int f = kGIn; f < kGInPad; ++f) wb.X[q][f] = 0;
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------- inference
 
-// Queries come either from an explicit array (9 floats: position, normal, wo) or from the
-// render's extension queue (hit point, outward normal, -ray direction).
-__global__ __launch_bounds__(256) void guideInferKernel(GuideInferArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    uint16_t* lw = reinterpret_cast<uint16_t*>(smem);
-    float* lb = reinterpret_cast<float*>(smem + kLdsWeights * 2);
-    WaveBuf* bufs = reinterpret_cast<WaveBuf*>(smem + kLdsWeights * 2 + kGBias * 4 + 64);
-    loadWeights(lw, lb, a.g.w, a.g.b);
-    __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    WaveBuf& wb = bufs[wave];
-    const uint32_t n = a.nDev ? *a.nDev : a.n;
-    const uint32_t tiles = (n + kTile - 1) / kTile;
-    for (uint32_t t = blockIdx.x * kWaves + wave; t < tiles; t += gridDim.x * kWaves) {
-        const uint32_t qi = t * kTile + (lane & 15);
-        float s9[9];
-        const bool valid = qi < n && guideQuery(a, qi, s9);
-        encodeTile(wb, s9, valid);
-        waveSync();
-        forwardTile(wb, lw, lb);
-        waveSync();
-        // Outputs -> lobes: lanes (q, part) write lobes 2*part, 2*part+1 of query q.
-        if (qi < n) {
-            const float* o = &wb.O[lane & 15][0];
-            float* out = a.out + size_t(qi) * (kGLobes * 5);
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kIT = 4;                         // 16-query tiles per wave
+constexpr int kInferStage = kTile * kGOutPad;  // floats of LDS output stage per wave
+
+__device__ __forceinline__ v4f mfma32(u32x4 a, u32x4 b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+// ReLU + BF16 of output blocks 2kb, 2kb+1 -> the next layer's B operand for k-block kb.
+__device__ __forceinline__ u32x4 reluPack(const v4f& lo, const v4f& hi) {
+    u32x4 r;
+    r[0] = packBf2(fmaxf(lo[0], 0.0f), fmaxf(lo[1], 0.0f));
+    r[1] = packBf2(fmaxf(lo[2], 0.0f), fmaxf(lo[3], 0.0f));
+    r[2] = packBf2(fmaxf(hi[0], 0.0f), fmaxf(hi[1], 0.0f));
+    r[3] = packBf2(fmaxf(hi[2], 0.0f), fmaxf(hi[3], 0.0f));
+    return r;
+}
+
+__device__ __forceinline__ v4f biasInit(const float* lb, int off, int mb, int g) {
+    const float4 b = *reinterpret_cast<const float4*>(lb + off + mb * 16 + 4 * g);
+    return v4f{b.x, b.y, b.z, b.w};
+}
+
+// One hidden layer (64 -> 64) on all tiles: acc holds the previous layer's pre-activations.
+template <int L>
+__device__ __forceinline__ void hiddenLayer(v4f (&acc)[kIT][4], const u32x4* lwf, const float* lb, int lane, int g) {
+    u32x4 h[kIT][2];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int k = 2 * (lane >> 4) + j;
-                GLobe lb1;
-                decodeLobe(o, k, lb1);
-                out[5 * k + 0] = valid ? lb1.mu[0] : 0.0f;
-                out[5 * k + 1] = valid ? lb1.mu[1] : 0.0f;
-                out[5 * k + 2] = valid ? lb1.mu[2] : 1.0f;
-                out[5 * k + 3] = valid ? lb1.kappa : 1.0f;
-                out[5 * k + 4] = valid ? lb1.w : 0.0f;
+    for (int t = 0; t < kIT; ++t) {
+        h[t][0] = reluPack(acc[t][0], acc[t][1]);
+        h[t][1] = reluPack(acc[t][2], acc[t][3]);
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) acc[t][mb] = biasInit(lb, 64 * L, mb, g);
+    }
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            const u32x4 af = lwf[(kGFragOff[L] + mb * 2 + kb) * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < kIT; ++t) acc[t][mb] = mfma32(af, h[t][kb], acc[t][mb]);
+        }
+}
+
+}  // namespace
+
+// Queries come either from an explicit array (9 floats: position, normal, wo) or from the
+// render's extension queue (hit point, outward normal, -ray direction); each wave takes
+// kIT * 16 queries at a time.  Queries that are not guided are skipped (no output).
+__global__ __launch_bounds__(256, 2) void guideInferKernel(GuideInferArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    u32x4* lwf = reinterpret_cast<u32x4*>(smem);
+    float* lb = reinterpret_cast<float*>(smem + kGWeights * 2);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    float* stage = reinterpret_cast<float*>(smem + kGWeights * 2 + kGBias * 4) + wave * kInferStage;
+    for (int i = threadIdx.x; i < kGWeights / 8; i += blockDim.x) lwf[i] = reinterpret_cast<const u32x4*>(a.g.wf)[i];
+    for (int i = threadIdx.x; i < kGBias; i += blockDim.x) lb[i] = a.g.b[i];
+    __syncthreads();
+    const uint32_t n = a.nDev ? *a.nDev : a.n;
+    const uint32_t groups = (n + kTile * kIT - 1) / (kTile * kIT);
+    for (uint32_t grp = blockIdx.x * kWaves + wave; grp < groups; grp += gridDim.x * kWaves) {
+        float s9[kIT][9];
+        bool valid[kIT];
+        bool any = false;
+#pragma unroll
+        for (int t = 0; t < kIT; ++t) {
+            const uint32_t qi = (grp * kIT + t) * kTile + r;
+            valid[t] = qi < n && guideQuery(a, qi, s9[t]);
+            if (!valid[t])
+                for (int j = 0; j < 9; ++j) s9[t][j] = 0.0f;
+            any |= valid[t];
+        }
+        if (__ballot(any ? 1 : 0) == 0ull) continue;
+        // layer 0: features are produced one k-block at a time
+        v4f acc[kIT][4];
+        float2 st[kIT];
+#pragma unroll
+        for (int t = 0; t < kIT; ++t) {
+            st[t] = make_float2(0.0f, 1.0f);
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb) acc[t][mb] = biasInit(lb, 0, mb, g);
+        }
+#pragma unroll
+        for (int kb = 0; kb < 7; ++kb) {
+            u32x4 bx[kIT];
+#pragma unroll
+            for (int t = 0; t < kIT; ++t) bx[t] = guideEncodeBlock(s9[t], g, kb, st[t]);
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb) {
+                const u32x4 af = lwf[(kGFragOff[0] + mb * 7 + kb) * 64 + lane];
+#pragma unroll
+                for (int t = 0; t < kIT; ++t) acc[t][mb] = mfma32(af, bx[t], acc[t][mb]);
             }
         }
-        waveSync();
+        hiddenLayer<1>(acc, lwf, lb, lane, g);
+        hiddenLayer<2>(acc, lwf, lb, lane, g);
+        hiddenLayer<3>(acc, lwf, lb, lane, g);
+        // layer 4: 64 -> 48
+        v4f o[kIT][3];
+        {
+            u32x4 h[kIT][2];
+#pragma unroll
+            for (int t = 0; t < kIT; ++t) {
+                h[t][0] = reluPack(acc[t][0], acc[t][1]);
+                h[t][1] = reluPack(acc[t][2], acc[t][3]);
+#pragma unroll
+                for (int mb = 0; mb < 3; ++mb) o[t][mb] = biasInit(lb, 256, mb, g);
+            }
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+                for (int mb = 0; mb < 3; ++mb) {
+                    const u32x4 af = lwf[(kGFragOff[4] + mb * 2 + kb) * 64 + lane];
+#pragma unroll
+                    for (int t = 0; t < kIT; ++t) o[t][mb] = mfma32(af, h[t][kb], o[t][mb]);
+                }
+        }
+        // outputs -> lobes, one tile at a time through the wave's LDS stage: lane (r, g)
+        // writes lobes 2g, 2g+1 of query r.
+#pragma unroll
+        for (int t = 0; t < kIT; ++t) {
+#pragma unroll
+            for (int mb = 0; mb < 3; ++mb)
+                *reinterpret_cast<float4*>(stage + r * kGOutPad + mb * 16 + 4 * g) =
+                    make_float4(o[t][mb][0], o[t][mb][1], o[t][mb][2], o[t][mb][3]);
+            waveSync();
+            if (valid[t]) {
+                const float* ov = stage + r * kGOutPad;
+                float mx = -INFINITY;
+#pragma unroll
+                for (int k = 0; k < kGLobes; ++k) mx = fmaxf(mx, ov[5 * k + 4]);
+                float se = 0.0f;
+#pragma unroll
+                for (int k = 0; k < kGLobes; ++k) se += expf(ov[5 * k + 4] - mx);
+                uint32_t words[5];
+#pragma unroll
+                for (int a5 = 0; a5 < 5; ++a5) words[a5] = 0u;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int k = 2 * g + j;
+                    const float m0 = ov[5 * k], m1 = ov[5 * k + 1], m2 = ov[5 * k + 2];
+                    const float ln = fmaxf(sqrtf(m0 * m0 + m1 * m1 + m2 * m2), 1e-6f);
+                    const float vals[5] = {m0 / ln, m1 / ln, m2 / ln,
+                                           expf(fminf(fmaxf(ov[5 * k + 3], kGLogKappaMin), kGLogKappaMax)),
+                                           expf(ov[5 * k + 4] - mx) / se};
+#pragma unroll
+                    for (int a5 = 0; a5 < 5; ++a5)
+                        words[a5] |= uint32_t(__half_as_ushort(__float2half_rn(vals[a5]))) << (16 * j);
+                }
+                const uint32_t qi = (grp * kIT + t) * kTile + r;
+                uint32_t* dst = reinterpret_cast<uint32_t*>(a.out + size_t(qi) * 5);
+#pragma unroll
+                for (int a5 = 0; a5 < 5; ++a5) dst[4 * a5 + g] = words[a5];
+            }
+            waveSync();
+        }
     }
 }
 
@@ -306,9 +439,18 @@ __global__ __launch_bounds__(256) void guideBiasGradKernel(GuideTrainArgs a) {
     if (threadIdx.x == 0) a.grad[kGOffB[layer] + nIdx] = red[0];
 }
 
+// Element index of weight i (row-major parameter layout) in the fragment-ordered copy.
+__host__ __device__ inline int fragIndexOf(int i) {
+    const int l = i < kGOffW[1] ? 0 : i < kGOffW[2] ? 1 : i < kGOffW[3] ? 2 : i < kGOffW[4] ? 3 : 4;
+    const int K = l == 0 ? kGInPad : kGHid;
+    const int o = i - kGOffW[l];
+    return guideFragPos(l, o / K, o % K);
+}
+
 __global__ __launch_bounds__(256) void guideAdamKernel(float* __restrict__ p, float* __restrict__ m,
                                                        float* __restrict__ v, const float* __restrict__ g,
-                                                       uint16_t* __restrict__ wbf, float lr, float c1, float c2) {
+                                                       uint16_t* __restrict__ wbf, uint16_t* __restrict__ wfrag,
+                                                       float lr, float c1, float c2) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= kGParams) return;
     const float gi = g[i];
@@ -318,7 +460,35 @@ __global__ __launch_bounds__(256) void guideAdamKernel(float* __restrict__ p, fl
     v[i] = vi;
     const float pi = p[i] - lr * (mi * c1) / (sqrtf(vi * c2) + 1e-8f);
     p[i] = pi;
-    if (i < kGWeights) wbf[i] = f2bf(pi);
+    if (i < kGWeights) {
+        const uint16_t b = f2bf(pi);
+        wbf[i] = b;
+        wfrag[fragIndexOf(i)] = b;
+    }
+}
+
+// Direction i drawn from query i's mixture with the renderer's sampler, and its density.
+__global__ __launch_bounds__(256) void guideSampleKernel(const uint4* __restrict__ rec, const float* __restrict__ u,
+                                                         uint32_t n, uint32_t perQuery, float* __restrict__ dirs,
+                                                         float* __restrict__ pdfs) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    uint4 r[5];
+    for (int k = 0; k < 5; ++k) r[k] = rec[size_t(i / perQuery) * 5 + k];
+    float w[3];
+    guideSampleDir(r, u[3 * size_t(i)], u[3 * size_t(i) + 1], u[3 * size_t(i) + 2], w);
+    dirs[3 * size_t(i)] = w[0];
+    dirs[3 * size_t(i) + 1] = w[1];
+    dirs[3 * size_t(i) + 2] = w[2];
+    pdfs[i] = guideMixPdf(r, w);
+}
+
+__global__ void guideClearRecordsKernel(uint32_t* count, double* csum) {
+    if (threadIdx.x == 0) {
+        count[0] = 0u;
+        count[1] = 0u;
+        *csum = 0.0;
+    }
 }
 
 __global__ void guideZeroKernel(float* p, int n, double* loss) {
@@ -336,18 +506,21 @@ GuideState& state(rbr_ctx* ctx) {
     return *ctx->guide;
 }
 
-size_t inferSmem() { return size_t(kLdsWeights) * 2 + kGBias * 4 + 64 + sizeof(WaveBuf) * kWaves; }
+size_t trainSmem() { return size_t(kLdsWeights) * 2 + kGBias * 4 + 64 + sizeof(WaveBuf) * kWaves; }
+size_t inferSmem() { return size_t(kGWeights) * 2 + kGBias * 4 + size_t(kInferStage) * 4 * kWaves; }
 
 int refreshBf16(rbr_ctx* ctx, GuideState& g) {
-    std::vector<uint16_t> bf(kGWeights);
+    std::vector<uint16_t> bf(kGWeights), frag(kGWeights);
     for (int i = 0; i < kGWeights; ++i) {
         uint32_t u;
         std::memcpy(&u, &g.master[i], 4);
         u += 0x7FFFu + ((u >> 16) & 1u);
         bf[i] = uint16_t(u >> 16);
+        frag[fragIndexOf(i)] = bf[i];
     }
     int rc;
     if ((rc = uploadBuffer(ctx, g.dW, bf.data(), bf.size() * 2))) return rc;
+    if ((rc = uploadBuffer(ctx, g.dWf, frag.data(), frag.size() * 2))) return rc;
     if ((rc = uploadBuffer(ctx, g.dMaster, g.master.data(), g.master.size() * 4))) return rc;
     return RBR_OK;
 }
@@ -393,7 +566,7 @@ int launchLossGrad(rbr_ctx* ctx, GuideState& g, const GuideTrainArgs& a) {
     hipLaunchKernelGGL(guideZeroKernel, dim3((kGParams + 255) / 256), dim3(256), 0, ctx->stream, a.grad, kGParams, a.loss);
     const unsigned tiles = (a.n + kTile - 1) / kTile;
     const unsigned grid = std::max(1u, std::min<unsigned>((tiles + kWaves - 1) / kWaves, unsigned(ctx->cuCount) * 2));
-    hipLaunchKernelGGL(guideFwdBwdKernel, dim3(grid), dim3(256), inferSmem(), ctx->stream, a);
+    hipLaunchKernelGGL(guideFwdBwdKernel, dim3(grid), dim3(256), trainSmem(), ctx->stream, a);
     const uint32_t chunk = 2048;
     hipLaunchKernelGGL(guideWeightGradKernel, dim3((116 + kWaves - 1) / kWaves, (a.n + chunk - 1) / chunk), dim3(256), 0,
                        ctx->stream, a, chunk);
@@ -407,6 +580,7 @@ int launchLossGrad(rbr_ctx* ctx, GuideState& g, const GuideTrainArgs& a) {
 GuideDev guideDev(const GuideState& g) {
     GuideDev d{};
     d.w = static_cast<const uint16_t*>(g.dW.ptr);
+    d.wf = static_cast<const uint16_t*>(g.dWf.ptr);
     d.b = static_cast<const float*>(g.dMaster.ptr) + kGWeights;
     for (int k = 0; k < 3; ++k) {
         d.lo[k] = g.lo[k];
@@ -420,23 +594,24 @@ int guideSetAttributes(rbr_ctx* ctx) {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(inferSmem()));
     if (e == hipSuccess)
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(guideFwdBwdKernel),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(inferSmem()));
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(trainSmem()));
     return e == hipSuccess ? RBR_OK : failHip(ctx, e, "guide LDS attribute");
 }
 
 int launchGuideInfer(rbr_ctx* ctx, const GuideInferArgs& a, uint32_t maxN) {
     if (int rc = guideSetAttributes(ctx)) return rc;
-    const unsigned tiles = (maxN + kTile - 1) / kTile;
-    const unsigned grid = std::max(1u, std::min<unsigned>((tiles + kWaves - 1) / kWaves, unsigned(ctx->cuCount)));
+    const unsigned groups = (maxN + kTile * kIT - 1) / (kTile * kIT);
+    const unsigned grid = std::max(1u, std::min<unsigned>((groups + kWaves - 1) / kWaves, unsigned(ctx->cuCount) * 2));
     hipLaunchKernelGGL(guideInferKernel, dim3(grid), dim3(256), inferSmem(), ctx->stream, a);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? RBR_OK : failHip(ctx, e, "guide inference launch");
 }
 
-int guideTrainDevice(rbr_ctx* ctx, const float4* samples, uint32_t stride, uint32_t n, float scale, float lr,
-                     double* lossOut) {
+int guideTrainDevice(rbr_ctx* ctx, const float4* samples, uint32_t stride, uint32_t n, uint32_t total, float scale,
+                     float lr, double* lossOut) {
     GuideState& g = state(ctx);
     if (!g.ready) return fail(ctx, RBR_ERR_STATE, "guide not initialised (rbr_guide_init)");
+    if (total && n > total) return fail(ctx, RBR_ERR_INVALID, "batch larger than the record set");
     if (int rc = ensureTrain(ctx, g, n)) return rc;
     if (!g.adamInit) {
         hipLaunchKernelGGL(guideZeroKernel, dim3((kGParams + 255) / 256), dim3(256), 0, ctx->stream,
@@ -448,13 +623,22 @@ int guideTrainDevice(rbr_ctx* ctx, const float4* samples, uint32_t stride, uint3
     }
     if (int rc = guideSetAttributes(ctx)) return rc;
     GuideTrainArgs a = trainArgs(ctx, g, samples, stride, n, scale);
+    if (total) {
+        // xxHash's 32-bit primes; total < 2^27 is never a multiple of one
+        static const uint32_t primes[5] = {2654435761u, 2246822519u, 3266489917u, 668265263u, 374761393u};
+        g.trainRng = g.trainRng * 6364136223846793005ull + 1442695040888963407ull;
+        a.total = total;
+        a.permMul = primes[(g.trainRng >> 33) % 5];
+        a.permAdd = uint32_t((g.trainRng >> 11) % total);
+    }
     if (int rc = launchLossGrad(ctx, g, a)) return rc;
     g.step += 1;
     const float c1 = 1.0f / (1.0f - std::pow(0.9f, float(g.step)));
     const float c2 = 1.0f / (1.0f - std::pow(0.999f, float(g.step)));
     hipLaunchKernelGGL(guideAdamKernel, dim3((kGParams + 255) / 256), dim3(256), 0, ctx->stream,
                        static_cast<float*>(g.dMaster.ptr), static_cast<float*>(g.dM.ptr), static_cast<float*>(g.dV.ptr),
-                       static_cast<const float*>(g.dGrad.ptr), static_cast<uint16_t*>(g.dW.ptr), lr, c1, c2);
+                       static_cast<const float*>(g.dGrad.ptr), static_cast<uint16_t*>(g.dW.ptr),
+                       static_cast<uint16_t*>(g.dWf.ptr), lr, c1, c2);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return failHip(ctx, e, "guide Adam launch");
     g.masterDirty = true;
@@ -468,10 +652,23 @@ int guideTrainDevice(rbr_ctx* ctx, const float4* samples, uint32_t stride, uint3
     return RBR_OK;
 }
 
+int guideEnsureRecords(rbr_ctx* ctx, GuideState& g) {
+    if (g.recCap) return RBR_OK;
+    int rc;
+    if ((rc = ensureBuffer(ctx, g.dRec, size_t(g.recReserve) * kRecStride * sizeof(float4)))) return rc;
+    if ((rc = ensureBuffer(ctx, g.dRecCount, 8))) return rc;
+    if ((rc = ensureBuffer(ctx, g.dCsum, 8))) return rc;
+    g.recCap = g.recReserve;
+    hipLaunchKernelGGL(guideClearRecordsKernel, dim3(1), dim3(64), 0, ctx->stream, static_cast<uint32_t*>(g.dRecCount.ptr),
+                       static_cast<double*>(g.dCsum.ptr));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "record clear");
+}
+
 void destroyGuideState(rbr_ctx* ctx) {
     GuideState* g = ctx->guide;
     if (!g) return;
-    for (DevBuf* b : {&g->dW, &g->dMaster, &g->dM, &g->dV, &g->dGrad, &g->dLoss, &g->dQ, &g->dOut, &g->dRec, &g->dRecCount,
+    for (DevBuf* b : {&g->dW, &g->dWf, &g->dMaster, &g->dM, &g->dV, &g->dGrad, &g->dLoss, &g->dQ, &g->dOut, &g->dRec, &g->dRecCount,
                       &g->dCsum})
         freeBuffer(*b);
     for (int l = 0; l < 5; ++l) {
@@ -485,6 +682,28 @@ void destroyGuideState(rbr_ctx* ctx) {
 }  // namespace rbr
 
 using namespace rbr;
+
+// IEEE binary16 -> float (host side of the guide records).
+static float halfToFloat(uint16_t h) {
+    const uint32_t sign = uint32_t(h & 0x8000u) << 16, ex = (h >> 10) & 0x1Fu, man = h & 0x3FFu;
+    uint32_t u;
+    if (ex == 0) {
+        if (man == 0) u = sign;
+        else {   // subnormal: normalise
+            int e = -1;
+            uint32_t m = man;
+            do {
+                ++e;
+                m <<= 1;
+            } while (!(m & 0x400u));
+            u = sign | uint32_t(127 - 15 - e) << 23 | (m & 0x3FFu) << 13;
+        }
+    } else if (ex == 31) u = sign | 0x7F800000u | man << 13;
+    else u = sign | (ex + 112u) << 23 | man << 13;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
 
 extern "C" {
 
@@ -506,10 +725,21 @@ int rbr_guide_init(rbr_ctx* ctx, uint64_t seed, const float lo[3], const float h
     const int ins[5] = {kGIn, kGHid, kGHid, kGHid, kGHid};
     const int Ks[5] = {kGInPad, kGHid, kGHid, kGHid, kGHid};
     const int outs[5] = {kGHid, kGHid, kGHid, kGHid, kGOut};
+    std::vector<int> real0;   // input columns that carry a feature (guidePair order)
+    for (int p = 0; p < kGInPad / 2; ++p) {
+        int j, k;
+        if (guidePair(p, j, k)) {
+            real0.push_back(2 * p);
+            real0.push_back(2 * p + 1);
+        }
+    }
     for (int l = 0; l < 5; ++l) {
         const float lim = std::sqrt(6.0f / float(ins[l])) * (l == 4 ? 0.1f : 1.0f);
         for (int o = 0; o < outs[l]; ++o)
-            for (int i = 0; i < ins[l]; ++i) g.master[kGOffW[l] + o * Ks[l] + i] = (2.0f * u01() - 1.0f) * lim;
+            for (int c = 0; c < ins[l]; ++c) {
+                const int i = l == 0 ? real0[c] : c;
+                g.master[kGOffW[l] + o * Ks[l] + i] = (2.0f * u01() - 1.0f) * lim;
+            }
     }
     for (int k = 0; k < 3; ++k) {
         g.lo[k] = lo[k];
@@ -554,18 +784,65 @@ int rbr_guide_eval(rbr_ctx* ctx, const float* queries, size_t n, float* lobes) {
     if (n == 0) return RBR_OK;
     int rc;
     if ((rc = ensureBuffer(ctx, g.dQ, n * 9 * 4))) return rc;
-    if ((rc = ensureBuffer(ctx, g.dOut, n * kGLobes * 5 * 4))) return rc;
+    if ((rc = ensureBuffer(ctx, g.dOut, n * kGuideRecBytes))) return rc;
     hipError_t e = hipMemcpyAsync(g.dQ.ptr, queries, n * 9 * 4, hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) return failHip(ctx, e, "query upload");
     GuideInferArgs a{};
     a.g = guideDev(g);
     a.queries = static_cast<const float*>(g.dQ.ptr);
     a.n = uint32_t(n);
-    a.out = static_cast<float*>(g.dOut.ptr);
+    a.out = static_cast<uint4*>(g.dOut.ptr);
     if ((rc = launchGuideInfer(ctx, a, uint32_t(n)))) return rc;
-    e = hipMemcpyAsync(lobes, g.dOut.ptr, n * kGLobes * 5 * 4, hipMemcpyDeviceToHost, ctx->stream);
+    std::vector<uint16_t> rec(n * kGuideRecBytes / 2);
+    e = hipMemcpyAsync(rec.data(), g.dOut.ptr, n * kGuideRecBytes, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "guide eval");
+    if (e != hipSuccess) return failHip(ctx, e, "guide eval");
+    for (size_t q = 0; q < n; ++q)
+        for (int k = 0; k < kGLobes; ++k)
+            for (int v = 0; v < 5; ++v) lobes[q * 40 + 5 * k + v] = halfToFloat(rec[q * 40 + 8 * v + k]);
+    return RBR_OK;
+}
+
+int rbr_guide_sample(rbr_ctx* ctx, const float* queries, size_t nq, size_t per_query, const float* u, float* dirs,
+                     float* pdfs) {
+    if (!ctx || (nq && (!queries || !u || !dirs || !pdfs)) || per_query == 0) return RBR_ERR_INVALID;
+    const size_t n = nq * per_query;
+    if (n > (1u << 28)) return fail(ctx, RBR_ERR_INVALID, "too many samples");
+    GuideState& g = state(ctx);
+    if (!g.ready) return fail(ctx, RBR_ERR_STATE, "guide not initialised");
+    if (int rc = useDevice(ctx)) return rc;
+    if (n == 0) return RBR_OK;
+    int rc;
+    DevBuf du, dd, dp;
+    if ((rc = ensureBuffer(ctx, g.dQ, nq * 9 * 4))) return rc;
+    if ((rc = ensureBuffer(ctx, g.dOut, nq * kGuideRecBytes))) return rc;
+    if ((rc = uploadBuffer(ctx, du, u, n * 12))) return rc;
+    if ((rc = ensureBuffer(ctx, dd, n * 12)) || (rc = ensureBuffer(ctx, dp, n * 4))) {
+        freeBuffer(du);
+        freeBuffer(dd);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(g.dQ.ptr, queries, nq * 9 * 4, hipMemcpyHostToDevice, ctx->stream);
+    GuideInferArgs a{};
+    a.g = guideDev(g);
+    a.queries = static_cast<const float*>(g.dQ.ptr);
+    a.n = uint32_t(nq);
+    a.out = static_cast<uint4*>(g.dOut.ptr);
+    if (e == hipSuccess) rc = launchGuideInfer(ctx, a, uint32_t(nq));
+    if (e == hipSuccess && !rc) {
+        hipLaunchKernelGGL(guideSampleKernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                           static_cast<const uint4*>(g.dOut.ptr), static_cast<const float*>(du.ptr), uint32_t(n),
+                           uint32_t(per_query), static_cast<float*>(dd.ptr), static_cast<float*>(dp.ptr));
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(dirs, dd.ptr, n * 12, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(pdfs, dp.ptr, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    }
+    freeBuffer(du);
+    freeBuffer(dd);
+    freeBuffer(dp);
+    if (rc) return rc;
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "guide sample");
 }
 
 // samples: 13 floats each (position 3, normal 3, wo 3, direction 3, weight c).
@@ -605,7 +882,67 @@ int rbr_guide_train_step(rbr_ctx* ctx, const float* samples, size_t n, float lr,
     if (!g.ready) return fail(ctx, RBR_ERR_STATE, "guide not initialised");
     if (int rc = useDevice(ctx)) return rc;
     if (int rc = uploadSamples(ctx, g, samples, n)) return rc;
-    return guideTrainDevice(ctx, static_cast<const float4*>(g.dQ.ptr), 4, uint32_t(n), 1.0f / float(n), lr, loss);
+    return guideTrainDevice(ctx, static_cast<const float4*>(g.dQ.ptr), 4, uint32_t(n), 0, 1.0f / float(n), lr, loss);
+}
+
+int rbr_guide_reserve(rbr_ctx* ctx, uint32_t capacity) {
+    if (!ctx || capacity == 0 || capacity > (1u << 27)) return ctx ? fail(ctx, RBR_ERR_INVALID, "record capacity") : RBR_ERR_INVALID;
+    GuideState& g = state(ctx);
+    if (int rc = useDevice(ctx)) return rc;
+    g.recReserve = capacity;
+    if (g.recCap && g.recCap != capacity) {
+        freeBuffer(g.dRec);
+        g.recCap = 0;
+    }
+    return guideEnsureRecords(ctx, g);
+}
+
+int rbr_guide_collected(rbr_ctx* ctx, uint64_t* count, double* weight_sum) {
+    if (!ctx) return RBR_ERR_INVALID;
+    GuideState& g = state(ctx);
+    if (int rc = useDevice(ctx)) return rc;
+    uint32_t c[2] = {0, 0};
+    double cs = 0.0;
+    if (g.recCap) {
+        hipError_t e = hipMemcpyAsync(c, g.dRecCount.ptr, 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&cs, g.dCsum.ptr, 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return failHip(ctx, e, "record count download");
+    }
+    if (count) *count = c[1];
+    if (weight_sum) *weight_sum = cs;
+    return RBR_OK;
+}
+
+int rbr_guide_clear_collected(rbr_ctx* ctx) {
+    if (!ctx) return RBR_ERR_INVALID;
+    GuideState& g = state(ctx);
+    if (int rc = useDevice(ctx)) return rc;
+    if (!g.recCap) return RBR_OK;
+    hipLaunchKernelGGL(guideClearRecordsKernel, dim3(1), dim3(64), 0, ctx->stream, static_cast<uint32_t*>(g.dRecCount.ptr),
+                       static_cast<double*>(g.dCsum.ptr));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "record clear");
+}
+
+int rbr_guide_train_collected(rbr_ctx* ctx, int steps, uint32_t batch, float lr, double* loss) {
+    if (!ctx || steps < 0 || batch == 0 || batch > (1u << 26) || !(lr > 0.0f)) return ctx ? fail(ctx, RBR_ERR_INVALID, "train arguments") : RBR_ERR_INVALID;
+    GuideState& g = state(ctx);
+    if (!g.ready) return fail(ctx, RBR_ERR_STATE, "guide not initialised");
+    uint64_t n = 0;
+    double cs = 0.0;
+    if (int rc = rbr_guide_collected(ctx, &n, &cs)) return rc;
+    if (loss) *loss = 0.0;
+    if (n == 0 || !(cs > 0.0) || steps == 0) return RBR_OK;
+    const uint32_t B = uint32_t(std::min<uint64_t>(batch, n));
+    // c is normalised by its mean over the records, so the loss scale does not depend on the
+    // scene's brightness.
+    const float scale = float(double(n) / (cs * double(B)));
+    for (int s = 0; s < steps; ++s)
+        if (int rc = guideTrainDevice(ctx, static_cast<const float4*>(g.dRec.ptr), kRecStride, B, uint32_t(n), scale, lr,
+                                      s + 1 == steps ? loss : nullptr))
+            return rc;
+    return RBR_OK;
 }
 
 }  // extern "C"

@@ -1,6 +1,7 @@
 // Device helpers and host state of the guiding network (guide.hip, render.hip).
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -22,14 +23,45 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 
-// sin/cos(2^k * pi * x) = sin/cos(2*pi * frac(2^(k-1) * x)); the scaling by a power of two and
-// the fraction are exact, so the feature is accurate at every frequency.
-__device__ __forceinline__ void guideFeature(float x, int k, float& s, float& c) {
-    const float r = x * __builtin_ldexpf(1.0f, k - 1);
-    const float f = r - floorf(r);
-    s = __builtin_amdgcn_sinf(f);
-    c = __builtin_amdgcn_cosf(f);
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t packBf2(float lo, float hi) { return uint32_t(f2bf(lo)) | (uint32_t(f2bf(hi)) << 16); }
+
+// Features of one 32-wide k-block for lane group g (0..3) of a query with scalars s9: word w
+// is the BF16 (sin, cos) pair of local pair q = 4 kb + w (guidePair's order).  Scalar j = q / 3
+// runs over frequencies 3g, 3g+1, 3g+2: the first is evaluated directly as
+// sin/cos(2 pi frac(2^(k-1) x)) (scaling by a power of two and the fraction are exact; the
+// hardware sine takes revolutions), the next two by angle doubling,
+//   sin 2a = 2 sin a cos a,  cos 2a = (cos a - sin a)(cos a + sin a),
+// carrying (sin, cos) in st between calls.  kb must be a compile-time constant after unrolling.
+__device__ __forceinline__ u32x4 guideEncodeBlock(const float* s9, int g, int kb, float2& st) {
+    u32x4 out;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const int q = 4 * kb + w;
+        if (q >= 27) {
+            out[w] = 0u;
+            continue;
+        }
+        if (q % 3 == 0) {
+            const float r = s9[q / 3] * __builtin_ldexpf(1.0f, 3 * g - 1);
+            const float f = r - floorf(r);
+            st.x = __builtin_amdgcn_sinf(f);
+            st.y = __builtin_amdgcn_cosf(f);
+        } else {
+            const float sn = 2.0f * st.x * st.y;
+            const float cs = (st.y - st.x) * (st.y + st.x);
+            st.x = sn;
+            st.y = cs;
+        }
+        out[w] = packBf2(st.x, st.y);
+    }
+    return out;
 }
+
+// Per-query guide record written by inference for the shading kernel: 5 x 8 FP16,
+// [mu.x of lobes 0..7][mu.y][mu.z][kappa][weight].
+constexpr uint32_t kGuideRecBytes = 80;
 
 struct GuideInferArgs {
     GuideDev g;
@@ -43,14 +75,17 @@ struct GuideInferArgs {
     const uint32_t* paths;
     const DJob* jobs;
     int njobs;
-    float* out;               // 40 floats per query: 8 x (mu 3, kappa, weight)
+    uint4* out;               // kGuideRecBytes per query (5 x uint4)
 };
 
 struct GuideTrainArgs {
     GuideDev g;
     const float4* samples;    // per sample: (pos, c) (normal, -) (wo, -) (dir, -) [+ stride padding]
     uint32_t stride;          // float4 per sample
-    uint32_t n;
+    uint32_t n;               // samples in this batch
+    // Batch sample s reads record (permAdd + s * permMul) mod total; permMul is a prime that
+    // does not divide total, so a batch never repeats a record.  total = 0: record s.
+    uint32_t total, permMul, permAdd;
     size_t cap;               // row length of the feature-major activation arrays
     float scale;              // multiplies c (1 / (batch * mean c))
     uint16_t* act[5];         // A_l, feature-major [K_l][cap]
@@ -106,7 +141,8 @@ __device__ __forceinline__ bool guideQuery(const GuideInferArgs& a, uint32_t i, 
 }
 
 __device__ __forceinline__ void guideSample(const GuideTrainArgs& a, uint32_t i, float* s9, float* dir, float& c) {
-    const float4* s = a.samples + size_t(i) * a.stride;
+    const uint32_t r = a.total ? uint32_t((uint64_t(a.permAdd) + uint64_t(i) * a.permMul) % a.total) : i;
+    const float4* s = a.samples + size_t(r) * a.stride;
     const float4 p = s[0], n = s[1], wo = s[2], d = s[3];
     const float pp[3] = {p.x, p.y, p.z}, nn[3] = {n.x, n.y, n.z}, ww[3] = {wo.x, wo.y, wo.z};
     scaleQuery(a.g, pp, nn, ww, s9);
@@ -180,10 +216,87 @@ __device__ __forceinline__ float outputGrad(const float* o, const float* dir, fl
     return isfinite(logp) ? -c * logp : 0.0f;
 }
 
-// vMF density of direction w for lobe (mu, kappa): kappa / (2 pi (1 - e^{-2 kappa})) e^{kappa (mu.w - 1)}
-__device__ __forceinline__ float vmfPdf(const float* mu, float kappa, const float* w) {
+// ---------------------------------------------------------------- sampling the mixture
+
+constexpr float kGuideFrac = 0.5f;     // one-sample MIS: probability of drawing from the guide
+constexpr uint32_t kRecStride = 6;     // float4 per recorded training sample
+
+// Lobe k of array a (0..2 mean, 3 kappa, 4 weight) of a guide record; k static after unrolling.
+__device__ __forceinline__ float guideVal(const uint4* rec, int a, int k) {
+    const uint4 v = rec[a];
+    const uint32_t w = k < 2 ? v.x : k < 4 ? v.y : k < 6 ? v.z : v.w;
+    return __half2float(__ushort_as_half(uint16_t((k & 1) ? (w >> 16) : (w & 0xFFFFu))));
+}
+
+// Lobe parameters as both the sampler and the density use them: the FP16 mean renormalised
+// in FP32, kappa and the weight as stored.
+__device__ __forceinline__ void guideLobe(const uint4* rec, int k, float* mu, float& kap, float& w) {
+    const float x = guideVal(rec, 0, k), y = guideVal(rec, 1, k), z = guideVal(rec, 2, k);
+    const float inv = 1.0f / sqrtf(fmaxf(x * x + y * y + z * z, 1e-30f));
+    mu[0] = x * inv;
+    mu[1] = y * inv;
+    mu[2] = z * inv;
+    kap = guideVal(rec, 3, k);
+    w = guideVal(rec, 4, k);
+}
+
+// vMF density kappa e^{kappa (mu.w - 1)} / (2 pi (1 - e^{-2 kappa})).
+__device__ __forceinline__ float vmfDensity(const float* mu, float kap, const float* w) {
     const float d = mu[0] * w[0] + mu[1] * w[1] + mu[2] * w[2];
-    return expf(logf(kappa) - kLog2Pi - log1mexp(2.0f * kappa) + kappa * (d - 1.0f));
+    return kap * expf(kap * (d - 1.0f)) / (6.28318530717958647f * -expm1f(-2.0f * kap));
+}
+
+// Mixture density at unit direction w, weights renormalised by their float sum exactly as
+// guideSampleDir selects lobes.
+__device__ __forceinline__ float guideMixPdf(const uint4* rec, const float* w) {
+    float p = 0.0f, tot = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kGLobes; ++k) {
+        float mu[3], kap, wk;
+        guideLobe(rec, k, mu, kap, wk);
+        tot += wk;
+        if (wk > 0.0f) p += wk * vmfDensity(mu, kap, w);
+    }
+    return tot > 0.0f ? p / tot : 0.0f;
+}
+
+// Draws a unit direction from the mixture: u0 picks the lobe, (u1, u2) the direction within
+// it.  Polar angle by the numerically stable inversion of Jakob 2012 ("Numerically stable
+// sampling of the von Mises Fisher distribution on S^2"):
+//   1 - cos = -log(u1 + (1 - u1) e^{-2 kappa}) / kappa,  sin from (1 - cos)(1 + cos);
+// tangent frame of Duff et al. 2017 ("Building an orthonormal basis, revisited").
+__device__ __forceinline__ void guideSampleDir(const uint4* rec, float u0, float u1, float u2, float* w) {
+    float tot = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kGLobes; ++k) tot += guideVal(rec, 4, k);
+    float u = u0 * tot;
+    int pick = -1, last = 0;
+#pragma unroll
+    for (int k = 0; k < kGLobes; ++k) {
+        const float wk = guideVal(rec, 4, k);
+        if (wk > 0.0f) {
+            last = k;
+            if (pick < 0 && u < wk) pick = k;
+        }
+        u -= wk;
+    }
+    if (pick < 0) pick = last;   // rounding at the top of the cumulative sum
+    float mu[3] = {0.0f, 0.0f, 1.0f}, kap = 1.0f, wsel = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kGLobes; ++k)
+        if (k == pick) guideLobe(rec, k, mu, kap, wsel);
+    float d = -logf(fmaxf(u1 + (1.0f - u1) * expf(-2.0f * kap), 1e-38f)) / kap;   // 1 - cos
+    d = fminf(fmaxf(d, 0.0f), 2.0f);
+    const float c = 1.0f - d;
+    const float s = sqrtf(fmaxf(0.0f, d * (2.0f - d)));
+    float sp, cp;
+    sincosf(6.28318530717958647f * u2, &sp, &cp);
+    const float sg = mu[2] >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sg + mu[2]);
+    const float b = mu[0] * mu[1] * a;
+    const float t[3] = {1.0f + sg * mu[0] * mu[0] * a, sg * b, -sg * mu[0]};
+    const float bb[3] = {b, sg + mu[1] * mu[1] * a, -mu[1]};
+    for (int q = 0; q < 3; ++q) w[q] = c * mu[q] + s * (cp * t[q] + sp * bb[q]);
 }
 
 struct GuideState {
@@ -191,19 +304,25 @@ struct GuideState {
     int step = 0;
     float lo[3] = {0, 0, 0}, inv[3] = {1, 1, 1};
     std::vector<float> master;
-    DevBuf dW, dMaster, dM, dV, dGrad, dLoss, dQ, dOut;
+    DevBuf dW, dWf, dMaster, dM, dV, dGrad, dLoss, dQ, dOut;
     DevBuf act[5], dz[5];
     size_t cap = 0;
-    // samples recorded by the renderer: 6 float4 per record, a device counter and the sum of c
+    // Samples recorded by the renderer, kRecStride float4 each:
+    //   (position, c) (outward normal, path slot) (wo, -) (direction, -) (throughput, -) (L, -)
+    // dRecCount holds {appended, finalised}; dCsum the sum of c over finalised records.
     DevBuf dRec, dRecCount, dCsum;
-    uint32_t recCap = 0;
+    uint32_t recCap = 0, recReserve = 1u << 22;
     uint64_t trainRng = 0x1234567ull;
 };
 
 GuideDev guideDev(const GuideState& g);
 int launchGuideInfer(rbr_ctx* ctx, const GuideInferArgs& a, uint32_t maxN);
-int guideTrainDevice(rbr_ctx* ctx, const float4* samples, uint32_t stride, uint32_t n, float scale, float lr,
-                     double* lossOut);
+// One Adam step on n samples.  total = 0: samples[0..n); total > 0: a batch of n records
+// drawn without repetition from samples[0..total) by a fresh random permutation.
+int guideTrainDevice(rbr_ctx* ctx, const float4* samples, uint32_t stride, uint32_t n, uint32_t total, float scale,
+                     float lr, double* lossOut);
 int guideSetAttributes(rbr_ctx* ctx);
+// Allocates the record buffers (recReserve records) if absent.
+int guideEnsureRecords(rbr_ctx* ctx, GuideState& g);
 
 }  // namespace rbr

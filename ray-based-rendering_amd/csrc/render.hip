@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "context.h"
+#include "guide_dev.h"
 #include "shading.h"
 #include "trace.h"
 
@@ -104,8 +105,8 @@ __device__ __forceinline__ Rng pathRng(uint64_t seed, uint32_t pixel, uint32_t s
     return Rng{mix64(seed * 0xA0761D6478BD642Full + uint64_t(sample) * pixels + pixel)};
 }
 
-// Dimensions: camera 0..1; bounce d uses base(d) + {0,1 BSDF, 2 RR, 3 distance, 4,5 phase,
-// 8+2l, 9+2l light l}.
+// Dimensions: camera 0..1; bounce d uses base(d) + {0,1 BSDF, 2 RR, 3 distance, 4,5 phase
+// (or the guide's direction), 6 guide-or-BSDF choice, 7 guide lobe, 8+2l, 9+2l light l}.
 __device__ __forceinline__ uint32_t dimBase(uint32_t depth) { return 8u + depth * 80u; }
 
 // ---------------------------------------------------------------- Fresnel and lobes
@@ -547,6 +548,12 @@ struct Queues {
     float4* T;            // throughput, w = specular-bounce flag
     float4* L;            // radiance
     uint32_t* counts;     // [0],[1] extension queues, [2] shadow queue
+    // path guiding (null when no job is guided / records)
+    const uint4* guide;   // per extension-queue entry: guide record (5 x uint4, FP16 lobes)
+    float4* rec;          // training records, kRecStride float4 each
+    uint32_t* recOf;      // [path] record written at this bounce or ~0
+    uint32_t* recCount;   // {appended, finalised}
+    uint32_t recCap;
 };
 
 __device__ __forceinline__ uint32_t waveAppend(uint32_t* counter, bool want) {
@@ -730,18 +737,48 @@ __global__ __launch_bounds__(256) void shadeKernel(const DJob* __restrict__ jobs
     }
 
     // Continuation: phase or BSDF sample, Russian roulette, spawn.
-    bool cont = false;
-    F3 no = mk(0, 0, 0), nd = mk(0, 0, 0);
+    bool cont = false, record = false;
+    F3 no = mk(0, 0, 0), nd = mk(0, 0, 0), wiRec = mk(0, 0, 0), Tin = T;
     if (mode == 2) {
         nd = normalize(hgSample(jb.medium.g, rd, rng.u(base + 4), rng.u(base + 5)));
         no = P;
         cont = depth + 1 < jb.maxDepth;
     } else if (mode == 1) {
         const DMaterial& m = jb.mats[mat];
-        const BSample bs = mixSample(m, woL, rng.u(base + 0), rng.u(base + 1));
-        if (bs.pdf > 0.0f && !black(bs.f)) {
-            const F3 wi = frame.toWorld(bs.wi);
-            T = T * bs.f * (fabsf(dot(wi, n)) / bs.pdf);
+        // Guided jobs: one-sample MIS between the guide's vMF mixture (probability
+        // kGuideFrac) and the BSDF; the direction's density is the mixture of both, a delta
+        // BSDF sample keeps its discrete probability.  All-delta materials are never guided.
+        const bool guided = (jb.guided & 1u) && !m.allDelta;
+        uint4 gl[5];
+        if (guided)
+            for (int k = 0; k < 5; ++k) gl[k] = q.guide[size_t(i) * 5 + k];
+        const bool fromGuide = guided && rng.u(base + 6) < kGuideFrac;
+        BSample bs{mk(0, 0, 0), mk(0, 0, 0), 0.0f, false};
+        F3 wi = mk(0, 0, 0);
+        float pdf = 0.0f;
+        if (!fromGuide) {
+            bs = mixSample(m, woL, rng.u(base + 0), rng.u(base + 1));
+            if (bs.pdf > 0.0f && !black(bs.f)) {
+                wi = frame.toWorld(bs.wi);
+                pdf = bs.pdf;
+                if (guided) {
+                    const float wa[3] = {wi.x, wi.y, wi.z};
+                    pdf = bs.delta ? (1.0f - kGuideFrac) * bs.pdf
+                                   : kGuideFrac * guideMixPdf(gl, wa) + (1.0f - kGuideFrac) * bs.pdf;
+                }
+            }
+        } else {
+            float w3[3];
+            guideSampleDir(gl, rng.u(base + 7), rng.u(base + 4), rng.u(base + 5), w3);
+            wi = normalize(mk(w3[0], w3[1], w3[2]));
+            const F3 wiL = frame.toLocal(wi);
+            bs.f = mixEval(m, woL, wiL);
+            const float wa[3] = {wi.x, wi.y, wi.z};
+            pdf = kGuideFrac * guideMixPdf(gl, wa) + (1.0f - kGuideFrac) * mixPdf(m, woL, wiL);
+            if (black(bs.f)) pdf = 0.0f;
+        }
+        if (pdf > 0.0f) {
+            T = T * bs.f * (fabsf(dot(wi, n)) / pdf);
             bool alive = true;
             if (depth > 3) {
                 const float qq = fmaxf(0.05f, 1.0f - maxc(T));
@@ -752,12 +789,30 @@ __global__ __launch_bounds__(256) void shadeKernel(const DJob* __restrict__ jobs
                 spec = bs.delta;
                 spawn(P, n, wi, no, nd);
                 cont = depth + 1 < jb.maxDepth;
+                record = guided && (jb.guided & 2u) && !bs.delta && cont;
+                wiRec = wi;
             }
         }
     }
     if (active) {
         q.L[p] = make_float4(L.x, L.y, L.z, 0.0f);
         if (cont) q.T[p] = make_float4(T.x, T.y, T.z, spec ? 1.0f : 0.0f);
+    }
+    if (q.recCount) {
+        // Training record of this vertex: the radiance that arrives later along wi is
+        // L_final - L(after this vertex's NEE), the gather kernel stores the latter.
+        const uint32_t r = waveAppend(q.recCount, record);
+        const bool keep = record && r < q.recCap;
+        if (active) q.recOf[p] = keep ? r : 0xFFFFFFFFu;
+        if (keep) {
+            float4* o = q.rec + size_t(r) * kRecStride;
+            o[0] = make_float4(P.x, P.y, P.z, 0.0f);
+            o[1] = make_float4(outward.x, outward.y, outward.z, __uint_as_float(p));
+            o[2] = make_float4(-rd.x, -rd.y, -rd.z, 0.0f);
+            o[3] = make_float4(wiRec.x, wiRec.y, wiRec.z, 0.0f);
+            o[4] = make_float4(Tin.x, Tin.y, Tin.z, 0.0f);
+            o[5] = make_float4(L.x, L.y, L.z, 0.0f);
+        }
     }
     const uint32_t slot = waveAppend(&q.counts[nxt], cont);
     if (cont) {
@@ -783,7 +838,37 @@ __global__ __launch_bounds__(256) void gatherKernel(uint32_t maxLights, int cur,
         }
     }
     q.L[p] = l4;
+    if (q.recOf) {
+        const uint32_t r = q.recOf[p];
+        if (r != 0xFFFFFFFFu) q.rec[size_t(r) * kRecStride + 5] = l4;
+    }
 }
+
+// Turns this round's records into training targets once their paths have finished:
+// c = lum((L_final - L_vertex) / T_vertex) = lum(f cos L_i) / pdf of the sampled direction,
+// so that minimising E[-c log p(wi)] over wi ~ pdf drives p towards f cos L_i.
+__global__ __launch_bounds__(256) void guideFinalizeKernel(float4* __restrict__ rec, const float4* __restrict__ L,
+                                                           const uint32_t* __restrict__ counts, uint32_t cap,
+                                                           double* csum) {
+    const uint32_t lo = counts[1], hi = min(counts[0], cap);
+    float acc = 0.0f;
+    for (uint32_t r = lo + blockIdx.x * 256u + threadIdx.x; r < hi; r += gridDim.x * 256u) {
+        float4* o = rec + size_t(r) * kRecStride;
+        const uint32_t p = __float_as_uint(o[1].w);
+        const float4 lf = L[p], ls = o[5], t = o[4];
+        const float dx = t.x > 0.0f ? (lf.x - ls.x) / t.x : 0.0f;
+        const float dy = t.y > 0.0f ? (lf.y - ls.y) / t.y : 0.0f;
+        const float dz = t.z > 0.0f ? (lf.z - ls.z) / t.z : 0.0f;
+        float c = 0.2126f * dx + 0.7152f * dy + 0.0722f * dz;
+        if (!(c > 0.0f) || !isfinite(c)) c = 0.0f;
+        o[0].w = c;
+        acc += c;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+    if ((threadIdx.x & 63) == 0 && acc > 0.0f) atomicAdd(csum, double(acc));
+}
+
+__global__ void guideFinalizeEndKernel(uint32_t* counts, uint32_t cap) { counts[1] = min(counts[0], cap); }
 
 __global__ __launch_bounds__(256) void resetCountsKernel(uint32_t* counts, int nxt) {
     counts[nxt] = 0u;
@@ -906,7 +991,7 @@ DMaterial makeMaterial(const rbr_material& in) {
 
 // Per-context render state: the path pool and queues, grown on demand.
 struct RenderState {
-    DevBuf ray[2], path[2], root[2], hit, sray, sroot, scontrib, socc, sindex, T, L, counts, jobs;
+    DevBuf ray[2], path[2], root[2], hit, sray, sroot, scontrib, socc, sindex, T, L, counts, jobs, guideOut, recOf;
     DevBuf arenaNodes, arenaPrims, arenaQuadN, arenaShade, arenaMats, arenaLights;
     std::vector<DevBuf> films, momS, momQ;
     double lastTotalMs = 0.0;
@@ -930,6 +1015,7 @@ void rbr::destroyRenderState(rbr_ctx* ctx) {
     if (!r) return;
     for (DevBuf* b : {&r->ray[0], &r->ray[1], &r->path[0], &r->path[1], &r->root[0], &r->root[1], &r->hit, &r->sray,
                       &r->sroot, &r->scontrib, &r->socc, &r->sindex, &r->T, &r->L, &r->counts, &r->jobs,
+                      &r->guideOut, &r->recOf,
                       &r->arenaNodes, &r->arenaPrims, &r->arenaQuadN, &r->arenaShade, &r->arenaMats, &r->arenaLights})
         freeBuffer(*b);
     for (auto* v : {&r->films, &r->momS, &r->momQ})
@@ -1053,7 +1139,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     std::map<const rbr_scene*, int> sceneIdx;
     int width = 0;
     uint32_t maxStack = 1, maxDepthAll = 0, maxLights = 0;
-    bool anyMixed = false, anyPlanes = false;
+    bool anyMixed = false, anyPlanes = false, anyGuided = false, anyRecord = false;
     for (int j = 0; j < njobs; ++j) {
         const rbr_job& jb = jobs[j];
         if (!jb.scene || jb.scene->ctx != ctx) return fail(ctx, RBR_ERR_INVALID, "job scene missing or from another context");
@@ -1063,7 +1149,14 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
             return fail(ctx, RBR_ERR_INVALID, "job needs width, height, spp and max_depth >= 1");
         if (jb.config.integrator != RBR_INTEGRATOR_PATH && jb.config.integrator != RBR_INTEGRATOR_VOLPATH)
             return fail(ctx, RBR_ERR_INVALID, "unknown integrator");
-        if (jb.config.guided) return fail(ctx, RBR_ERR_UNSUPPORTED, "guided rendering goes through rbr_guide_render");
+        if (jb.config.guided < 0 || jb.config.guided > 3) return fail(ctx, RBR_ERR_INVALID, "guided must be 0..3");
+        if (jb.config.guided) {
+            if (jb.config.integrator != RBR_INTEGRATOR_PATH)
+                return fail(ctx, RBR_ERR_UNSUPPORTED, "guiding is implemented for the path integrator");
+            if (!ctx->guide || !ctx->guide->ready) return fail(ctx, RBR_ERR_STATE, "guided job needs rbr_guide_init");
+            anyGuided = true;
+            anyRecord |= (jb.config.guided & 2) != 0;
+        }
         if (uint64_t(jb.camera.width) * jb.camera.height > (1ull << 26)) return fail(ctx, RBR_ERR_INVALID, "film too large");
         if (sceneIdx.find(jb.scene) == sceneIdx.end()) {
             sceneIdx[jb.scene] = int(scenes.size());
@@ -1192,6 +1285,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         d.maxDepth = uint32_t(jb.config.max_depth);
         d.useNEE = jb.config.use_nee ? 1u : 0u;
         d.useMIS = jb.config.use_mis ? 1u : 0u;
+        d.guided = uint32_t(jb.config.guided);
         d.root = jb.scene->bvh.empty() ? kEmpty : sl.nodeOff;
         d.pixels = uint32_t(jb.camera.width) * uint32_t(jb.camera.height);
         d.seed = jb.config.seed;
@@ -1223,6 +1317,11 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         {&R.counts, 16}, {&R.jobs, sizeof(DJob) * size_t(njobs)}};
     for (auto& a : allocs)
         if ((rc = ensureBuffer(ctx, *a.b, a.bytes))) return rc;
+    if (anyGuided && (rc = ensureBuffer(ctx, R.guideOut, size_t(P) * kGuideRecBytes))) return rc;
+    if (anyRecord) {
+        if ((rc = ensureBuffer(ctx, R.recOf, size_t(P) * 4))) return rc;
+        if ((rc = guideEnsureRecords(ctx, *ctx->guide))) return rc;
+    }
     Queues q;
     for (int k = 0; k < 2; ++k) {
         q.ray[k] = static_cast<float4*>(R.ray[k].ptr);
@@ -1238,6 +1337,11 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     q.T = static_cast<float4*>(R.T.ptr);
     q.L = static_cast<float4*>(R.L.ptr);
     q.counts = static_cast<uint32_t*>(R.counts.ptr);
+    q.guide = anyGuided ? static_cast<const uint4*>(R.guideOut.ptr) : nullptr;
+    q.rec = anyRecord ? static_cast<float4*>(ctx->guide->dRec.ptr) : nullptr;
+    q.recOf = anyRecord ? static_cast<uint32_t*>(R.recOf.ptr) : nullptr;
+    q.recCount = anyRecord ? static_cast<uint32_t*>(ctx->guide->dRecCount.ptr) : nullptr;
+    q.recCap = anyRecord ? ctx->guide->recCap : 0u;
 
     // ---- rounds: give every unfinished job as many whole samples as the pool holds
     std::vector<uint32_t> done(njobs, 0);
@@ -1287,6 +1391,19 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
             a.hits = q.hit;
             a.occ = nullptr;
             if ((rc = launchTraversal(ctx, a, width, maxStack, false))) return rc;
+            if (anyGuided) {
+                GuideInferArgs gi{};
+                gi.g = guideDev(*ctx->guide);
+                gi.n = used;
+                gi.nDev = q.counts + cur;
+                gi.rays = q.ray[cur];
+                gi.hits = q.hit;
+                gi.paths = q.path[cur];
+                gi.jobs = djd;
+                gi.njobs = njobs;
+                gi.out = static_cast<uint4*>(R.guideOut.ptr);
+                if ((rc = launchGuideInfer(ctx, gi, used))) return rc;
+            }
             hipLaunchKernelGGL(shadeKernel, dim3(grid), dim3(256), 0, ctx->stream, djd, njobs, depth, maxLights, cur, q);
             if (maxLights) {
                 TraceArgs b = ta;
@@ -1300,6 +1417,13 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
                 hipLaunchKernelGGL(gatherKernel, dim3(grid), dim3(256), 0, ctx->stream, maxLights, cur, q);
             }
             cur ^= 1;
+        }
+        if (anyRecord) {
+            GuideState& gs = *ctx->guide;
+            hipLaunchKernelGGL(guideFinalizeKernel, dim3(1024), dim3(256), 0, ctx->stream, q.rec, q.L,
+                               static_cast<const uint32_t*>(gs.dRecCount.ptr), gs.recCap, static_cast<double*>(gs.dCsum.ptr));
+            hipLaunchKernelGGL(guideFinalizeEndKernel, dim3(1), dim3(1), 0, ctx->stream, static_cast<uint32_t*>(gs.dRecCount.ptr),
+                               gs.recCap);
         }
         for (int j = 0; j < njobs; ++j) {
             if (!take[j]) continue;

@@ -461,6 +461,12 @@ _sig(_lib.rbr_guide_eval, [_vp, _vp, _sz, _vp])
 _sig(_lib.rbr_guide_loss_grad, [_vp, _vp, _sz, C.POINTER(C.c_double), _vp])
 _sig(_lib.rbr_guide_train_step, [_vp, _vp, _sz, C.c_float, C.POINTER(C.c_double)])
 
+_sig(_lib.rbr_guide_sample, [_vp, _vp, _sz, _sz, _vp, _vp, _vp])
+_sig(_lib.rbr_guide_reserve, [_vp, C.c_uint32])
+_sig(_lib.rbr_guide_collected, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_double)])
+_sig(_lib.rbr_guide_train_collected, [_vp, C.c_int, C.c_uint32, C.c_float, C.POINTER(C.c_double)])
+_sig(_lib.rbr_guide_clear_collected, [_vp])
+
 GUIDE_PARAMS = _lib.rbr_guide_param_count()
 
 
@@ -493,6 +499,18 @@ class Guide:
         self.ctx.check(_lib.rbr_guide_eval(self.ctx.handle, q.ctypes.data, len(q), out.ctypes.data))
         return out
 
+    def sample(self, queries, u) -> tuple[np.ndarray, np.ndarray]:
+        """Directions drawn from each query's mixture: u (nq, m, 3) uniforms -> dirs (nq, m, 3),
+        pdfs (nq, m), as the renderer samples them."""
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 9)
+        uu = np.ascontiguousarray(u, np.float32).reshape(len(q), -1, 3)
+        m = uu.shape[1]
+        dirs = np.zeros((len(q), m, 3), np.float32)
+        pdfs = np.zeros((len(q), m), np.float32)
+        self.ctx.check(_lib.rbr_guide_sample(self.ctx.handle, q.ctypes.data, len(q), m, uu.ctypes.data,
+                                             dirs.ctypes.data, pdfs.ctypes.data))
+        return dirs, pdfs
+
     def loss_grad(self, samples) -> tuple[float, np.ndarray]:
         """samples (n, 13): position, normal, wo, direction, weight -> (mean loss, gradient)."""
         s = np.ascontiguousarray(samples, np.float32).reshape(-1, 13)
@@ -506,3 +524,20 @@ class Guide:
         loss = C.c_double(0)
         self.ctx.check(_lib.rbr_guide_train_step(self.ctx.handle, s.ctypes.data, len(s), lr, C.byref(loss)))
         return loss.value
+
+    def reserve(self, capacity: int) -> None:
+        self.ctx.check(_lib.rbr_guide_reserve(self.ctx.handle, capacity))
+
+    def collected(self) -> tuple[int, float]:
+        n = C.c_uint64(0)
+        cs = C.c_double(0)
+        self.ctx.check(_lib.rbr_guide_collected(self.ctx.handle, C.byref(n), C.byref(cs)))
+        return n.value, cs.value
+
+    def train_collected(self, steps: int, batch: int = 1 << 18, lr: float = 1e-3) -> float:
+        loss = C.c_double(0)
+        self.ctx.check(_lib.rbr_guide_train_collected(self.ctx.handle, steps, batch, lr, C.byref(loss)))
+        return loss.value
+
+    def clear_collected(self) -> None:
+        self.ctx.check(_lib.rbr_guide_clear_collected(self.ctx.handle))

@@ -34,27 +34,33 @@ def test_param_layout_and_init_match_oracle(ctx):
     np.testing.assert_array_equal(g.params, G.init_params(5))
 
 
-@pytest.mark.parametrize("n", [1, 15, 16, 17, 1000, 4099])
+def fp16(a):
+    return np.asarray(a, np.float32).astype(np.float16).astype(np.float64)
+
+
+@pytest.mark.parametrize("n", [1, 15, 16, 17, 63, 64, 65, 1000, 4099])
 def test_eval_matches_oracle(ctx, n):
     params = random_params(100 + n).astype(np.float32)
     samples, lo, hi = random_samples(200 + n, n)
     g = R.Guide(ctx, lo, hi, seed=1)
     g.params = params
     say(f"guide eval n={n}")
-    out = g.eval(samples[:, :9])
+    out = g.eval(samples[:, :9]).astype(np.float64)
     mu, kappa, w = G.eval_lobes(params.astype(np.float64), samples[:, :9], lo, hi, quant=True)
     assert np.all(np.isfinite(out))
-    # Most queries agree to float32 rounding.  The hardware sine (error ~1e-6) sometimes puts an
-    # encoded feature on the other side of a BF16 rounding boundary: that query's outputs then
-    # move by ~1e-3 (about 5% of queries).  A wrong operand layout moves every query by O(1).
-    e_q = np.maximum.reduce([np.abs(out[:, :, :3] - mu).max(axis=(1, 2)),
-                             np.abs(np.log(out[:, :, 3]) - np.log(kappa)).max(axis=1),
-                             np.abs(out[:, :, 4] - w).max(axis=1)])
-    flipped = int(np.sum(e_q > 1e-4))
-    say(f"  per-query max error: median {np.median(e_q):.2e} max {e_q.max():.2e}; {flipped}/{n} above 1e-4")
-    assert np.median(e_q) < 1e-5
+    # The record is FP16 (2^-11 relative).  Beyond that, most queries agree to float32
+    # rounding; the hardware sine and angle doubling (error ~1e-6) sometimes put an encoded
+    # feature on the other side of a BF16 rounding boundary, which moves that query's outputs
+    # by ~1e-3.  A wrong operand layout or feature order moves every query by O(1).
+    e_q = np.maximum.reduce([np.abs(out[:, :, :3] - fp16(mu)).max(axis=(1, 2)),
+                             np.abs(np.log(out[:, :, 3]) - np.log(fp16(kappa))).max(axis=1),
+                             np.abs(out[:, :, 4] - fp16(w)).max(axis=1)])
+    flipped = int(np.sum(e_q > 2e-3))
+    say(f"  per-query max error: median {np.median(e_q):.2e} max {e_q.max():.2e}; {flipped}/{n} above 2e-3")
+    assert np.median(e_q) < 1e-3
     assert flipped <= max(1, int(0.15 * n))
-    assert e_q.max() < 3e-2
+    # a flipped query's unit mean moves further when its raw mean vector is short
+    assert e_q.max() < 0.2
     # against the unquantised network: BF16 error budget
     mu_x, kappa_x, w_x = G.eval_lobes(params.astype(np.float64), samples[:, :9], lo, hi, quant=False)
     assert np.median(np.abs(out[:, :, :3] - mu_x)) < 2e-2
@@ -63,10 +69,67 @@ def test_eval_matches_oracle(ctx, n):
     rng = np.random.default_rng(n)
     d = rng.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
-    pg = G.mixture_pdf(out[:, :, :3].astype(np.float64), out[:, :, 3].astype(np.float64),
-                       out[:, :, 4].astype(np.float64), d)[:, 0]
+    pg = G.mixture_pdf(out[:, :, :3], out[:, :, 3], out[:, :, 4], d)[:, 0]
     po = G.mixture_pdf(mu, kappa, w, d)[:, 0]
-    assert np.quantile(np.abs(pg / po - 1.0), 0.99) < 1e-2
+    assert np.quantile(np.abs(pg / po - 1.0), 0.99) < 2e-2
+
+
+def sphere_bins(dirs, nz=32, nphi=64):
+    """Equal-area bins: uniform in z and in azimuth."""
+    z = np.clip(dirs[..., 2], -1.0, 1.0 - 1e-12)
+    phi = np.mod(np.arctan2(dirs[..., 1], dirs[..., 0]), 2 * np.pi)
+    iz = np.minimum(((z + 1.0) * 0.5 * nz).astype(int), nz - 1)
+    ip = np.minimum((phi / (2 * np.pi) * nphi).astype(int), nphi - 1)
+    return iz * nphi + ip
+
+
+def test_sampler_matches_its_density(ctx):
+    """Directions drawn by the renderer's guide sampler follow the mixture density it reports
+    (chi-square over 2048 equal-area bins, expected counts by quadrature of the oracle's
+    mixture density at the record's FP16 lobes), and the reported density equals that mixture
+    at the drawn direction.  A sampler/density mismatch would bias guided rendering."""
+    nq, m = 3, 1 << 20
+    # concentrations kept to kappa <~ 100 so that 8x8 sub-cell quadrature resolves every lobe
+    p64 = random_params(77, scale=2.0)
+    Ws, bs = G.unpack(p64)
+    prng = np.random.default_rng(76)
+    for k in range(8):
+        Ws[4][5 * k + 3] *= 0.2
+        bs[4][5 * k + 3] = prng.uniform(-1.0, 3.5)
+    params = G.pack(Ws, bs).astype(np.float32)
+    samples, lo, hi = random_samples(78, nq)
+    g = R.Guide(ctx, lo, hi, seed=1)
+    g.params = params
+    lobes = g.eval(samples[:, :9]).astype(np.float64)
+    say(f"guide sampler: kappa range {lobes[:, :, 3].min():.3g}..{lobes[:, :, 3].max():.3g}")
+    rng = np.random.default_rng(79)
+    u = rng.random((nq, m, 3), dtype=np.float32)
+    dirs, pdfs = g.sample(samples[:, :9], u)
+    assert np.all(np.isfinite(dirs)) and np.all(pdfs > 0)
+    np.testing.assert_allclose(np.linalg.norm(dirs, axis=2), 1.0, atol=1e-5)
+    mu = lobes[:, :, :3] / np.linalg.norm(lobes[:, :, :3], axis=2, keepdims=True)
+    kap, w = lobes[:, :, 3], lobes[:, :, 4] / lobes[:, :, 4].sum(axis=1, keepdims=True)
+    pref = G.mixture_pdf(mu, kap, w, dirs.astype(np.float64))
+    assert np.quantile(np.abs(pdfs / pref - 1.0), 0.999) < 1e-3
+    # bin probabilities by 8x8 sub-cell quadrature
+    nz, nphi, sub = 32, 64, 8
+    zc = -1.0 + (np.arange(nz * sub) + 0.5) * 2.0 / (nz * sub)
+    pc = (np.arange(nphi * sub) + 0.5) * 2 * np.pi / (nphi * sub)
+    Z, P = np.meshgrid(zc, pc, indexing="ij")
+    rxy = np.sqrt(1 - Z * Z)
+    grid = np.stack([rxy * np.cos(P), rxy * np.sin(P), Z], -1).reshape(-1, 3)
+    cell = 4 * np.pi / grid.shape[0]
+    for qi in range(nq):
+        dens = G.mixture_pdf(mu[qi:qi + 1], kap[qi:qi + 1], w[qi:qi + 1], grid[None])[0]
+        prob = (dens * cell).reshape(nz, sub, nphi, sub).sum(axis=(1, 3)).reshape(-1)
+        prob /= prob.sum()
+        counts = np.bincount(sphere_bins(dirs[qi].astype(np.float64)), minlength=nz * nphi)
+        expct = prob * m
+        ok = expct >= 20
+        chi2 = float(np.sum((counts[ok] - expct[ok]) ** 2 / expct[ok]))
+        dof = int(ok.sum()) - 1
+        say(f"  query {qi}: chi2/dof {chi2 / dof:.3f} over {dof} bins, mass outside {1 - prob[ok].sum():.2e}")
+        assert chi2 / dof < 1.0 + 6.0 * np.sqrt(2.0 / dof)
 
 
 def test_eval_empty_and_errors(ctx):

@@ -17,7 +17,7 @@ def random_params(seed, scale=1.0):
         b[:] = rng.normal(0.0, 0.1 * scale, b.shape)
     Ws[4][:] = rng.normal(0.0, 0.3 * scale, Ws[4].shape)
     Ws[4][40:] = 0.0
-    Ws[0][:, 216:] = 0.0
+    Ws[0][:, G.padding_features()] = 0.0
     return G.pack(Ws, bs)
 
 
@@ -44,12 +44,13 @@ def torch_loss(params, samples, lo, hi):
     s9, dirs, c = G.split_samples(samples, lo, hi)
     x = torch.tensor(s9)
     feats = []
-    for j in range(9):
-        for k in range(12):
-            ang = (2.0 ** k) * np.pi * x[:, j]
+    for jk in G.feature_pairs():
+        if jk is None:
+            feats += [torch.zeros(len(x), dtype=torch.float64)] * 2
+        else:
+            ang = (2.0 ** jk[1]) * np.pi * x[:, jk[0]]
             feats += [torch.sin(ang), torch.cos(ang)]
     X = torch.stack(feats, dim=1)
-    X = torch.cat([X, torch.zeros(len(X), 8, dtype=torch.float64)], dim=1)
     o = 0
     Ws = []
     for (r, cc) in G.W_SHAPES:
@@ -77,16 +78,29 @@ def torch_loss(params, samples, lo, hi):
     return loss.item(), p.grad.numpy()
 
 
+def test_feature_order_covers_every_scalar_and_frequency_once():
+    pairs = G.feature_pairs()
+    assert len(pairs) == 112
+    real = [jk for jk in pairs if jk is not None]
+    assert sorted(real) == [(j, k) for j in range(9) for k in range(12)]
+    # lane group g of a k-block holds pairs p with (p % 8) // 2 == g: its frequencies are 3g..3g+2
+    for p, jk in enumerate(pairs):
+        if jk is not None:
+            assert jk[1] // 3 == (p % 8) // 2
+    assert len(G.padding_features()) == 8
+
+
 def test_encoding_matches_direct_sin_cos():
     rng = np.random.default_rng(3)
     s9 = rng.uniform(-1.5, 1.5, (64, 9))
     X = G.encode(s9)
-    for j in range(9):
-        for k in range(12):
-            f = 12 * j + k
-            np.testing.assert_allclose(X[:, 2 * f], np.sin(2.0 ** k * np.pi * s9[:, j]), atol=1e-9)
-            np.testing.assert_allclose(X[:, 2 * f + 1], np.cos(2.0 ** k * np.pi * s9[:, j]), atol=1e-9)
-    assert np.all(X[:, 216:] == 0.0)
+    for p, jk in enumerate(G.feature_pairs()):
+        if jk is None:
+            assert np.all(X[:, 2 * p:2 * p + 2] == 0.0)
+            continue
+        j, k = jk
+        np.testing.assert_allclose(X[:, 2 * p], np.sin(2.0 ** k * np.pi * s9[:, j]), atol=1e-9)
+        np.testing.assert_allclose(X[:, 2 * p + 1], np.cos(2.0 ** k * np.pi * s9[:, j]), atol=1e-9)
 
 
 def test_forward_and_gradient_match_torch_autograd():
@@ -108,8 +122,9 @@ def test_gradient_matches_finite_differences():
     # a few entries of every weight matrix and bias vector, largest-gradient ones included
     idx = list(np.argsort(-np.abs(g))[:12])
     for l, (r, c) in enumerate(G.W_SHAPES):
+        real = np.setdiff1d(np.arange(c), G.padding_features()) if l == 0 else np.arange(c)
         for _ in range(4):
-            idx.append(offs[l] + rng.integers(r if l < 4 else 40) * c + rng.integers(c if l else 216))
+            idx.append(offs[l] + rng.integers(r if l < 4 else 40) * c + rng.choice(real))
     b0 = G.N_WEIGHTS
     for nb in G.B_SIZES:
         idx.append(b0 + rng.integers(min(nb, 40)))
