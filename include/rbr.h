@@ -123,10 +123,10 @@ typedef struct rbr_render_config {
     int32_t max_depth;
     int32_t use_nee;
     int32_t use_mis;
-    int32_t guided;         /* 1 = MLP path guiding (PATH only)  */
+    int32_t guided;         /* 1 = MLP path guiding, 3 = also record training samples (PATH only) */
     uint64_t seed;
     uint32_t sample_offset; /* first sample index (progressive)  */
-    uint32_t pad;
+    uint32_t guide_depth;   /* guided jobs: bounces 0..guide_depth-1 draw from the guide (0 = all) */
 } rbr_render_config;
 
 typedef struct rbr_ctx rbr_ctx;

@@ -53,6 +53,9 @@ struct rbr_scene {
     rbr_medium medium{};
     float background[3] = {0.0f, 0.0f, 0.0f};
     rbr::DevBuf dMaterials, dPrimMaterial, dLights;
+    // Changes whenever geometry, BVH or shading change (unique across scenes), so render
+    // state built from the scene can be reused while it is unchanged.
+    uint64_t version = 0;
 };
 
 namespace rbr {
@@ -68,6 +71,8 @@ TraceArgs sceneTraceArgs(const rbr_scene* s);
 // scene whose wide BVH has the given width and stack need, using the context's tuning.
 int launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack, bool any);
 
+// Gives the scene a new version number (after any change to it).
+void bumpVersion(rbr_scene* s);
 void freeShading(rbr_scene* s);          // render.hip
 void destroyRenderState(rbr_ctx* ctx);   // render.hip
 void destroyGuideState(rbr_ctx* ctx);    // guide.hip

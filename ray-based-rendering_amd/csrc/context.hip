@@ -1,6 +1,8 @@
 // C ABI (include/rbr.h): contexts, scenes, geometry upload, BVH build and traversal.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -56,6 +58,11 @@ int rbr::uploadBuffer(rbr_ctx* ctx, DevBuf& b, const void* src, size_t bytes) {
     e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return failHip(ctx, e, "hipStreamSynchronize");
     return RBR_OK;
+}
+
+void rbr::bumpVersion(rbr_scene* s) {
+    static std::atomic<uint64_t> counter{0};
+    s->version = ++counter;
 }
 
 int rbr::useDevice(rbr_ctx* ctx) {
@@ -290,6 +297,7 @@ int rbr_scene_set_geometry(rbr_scene* s, const float* tris, uint32_t ntri, const
     s->geom = std::move(g);
     s->built = false;
     s->bvh = WideBVH();
+    bumpVersion(s);
     return RBR_OK;
 }
 
@@ -299,6 +307,7 @@ int rbr_scene_build_bvh(rbr_scene* s, int width, int leaf_max) {
     if (width != 4 && width != 8) return fail(ctx, RBR_ERR_INVALID, "BVH width must be 4 or 8");
     if (leaf_max < 1 || leaf_max > 8) return fail(ctx, RBR_ERR_INVALID, "leaf_max must be in 1..8");
     s->built = false;
+    bumpVersion(s);
     buildWideBVH(s->geom, width, leaf_max, s->bvh);
     if (!s->bvh.empty() && uint64_t(width - 1) * s->bvh.maxDepth + 1 > uint64_t(kMaxStack))
         return fail(ctx, RBR_ERR_UNSUPPORTED, "BVH too deep for the traversal stack");

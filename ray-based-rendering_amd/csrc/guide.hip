@@ -552,6 +552,7 @@ GuideTrainArgs trainArgs(rbr_ctx* ctx, GuideState& g, const float4* samples, uin
     a.n = n;
     a.cap = g.cap;
     a.scale = scale;
+    a.cmax = INFINITY;
     for (int l = 0; l < 5; ++l) {
         a.act[l] = static_cast<uint16_t*>(g.act[l].ptr);
         a.dz[l] = static_cast<uint16_t*>(g.dz[l].ptr);
@@ -608,7 +609,7 @@ int launchGuideInfer(rbr_ctx* ctx, const GuideInferArgs& a, uint32_t maxN) {
 }
 
 int guideTrainDevice(rbr_ctx* ctx, const float4* samples, uint32_t stride, uint32_t n, uint32_t total, float scale,
-                     float lr, double* lossOut) {
+                     float cmax, float lr, double* lossOut) {
     GuideState& g = state(ctx);
     if (!g.ready) return fail(ctx, RBR_ERR_STATE, "guide not initialised (rbr_guide_init)");
     if (total && n > total) return fail(ctx, RBR_ERR_INVALID, "batch larger than the record set");
@@ -623,6 +624,7 @@ int guideTrainDevice(rbr_ctx* ctx, const float4* samples, uint32_t stride, uint3
     }
     if (int rc = guideSetAttributes(ctx)) return rc;
     GuideTrainArgs a = trainArgs(ctx, g, samples, stride, n, scale);
+    a.cmax = cmax;
     if (total) {
         // xxHash's 32-bit primes; total < 2^27 is never a multiple of one
         static const uint32_t primes[5] = {2654435761u, 2246822519u, 3266489917u, 668265263u, 374761393u};
@@ -882,7 +884,8 @@ int rbr_guide_train_step(rbr_ctx* ctx, const float* samples, size_t n, float lr,
     if (!g.ready) return fail(ctx, RBR_ERR_STATE, "guide not initialised");
     if (int rc = useDevice(ctx)) return rc;
     if (int rc = uploadSamples(ctx, g, samples, n)) return rc;
-    return guideTrainDevice(ctx, static_cast<const float4*>(g.dQ.ptr), 4, uint32_t(n), 0, 1.0f / float(n), lr, loss);
+    return guideTrainDevice(ctx, static_cast<const float4*>(g.dQ.ptr), 4, uint32_t(n), 0, 1.0f / float(n), INFINITY, lr,
+                            loss);
 }
 
 int rbr_guide_reserve(rbr_ctx* ctx, uint32_t capacity) {
@@ -936,11 +939,15 @@ int rbr_guide_train_collected(rbr_ctx* ctx, int steps, uint32_t batch, float lr,
     if (n == 0 || !(cs > 0.0) || steps == 0) return RBR_OK;
     const uint32_t B = uint32_t(std::min<uint64_t>(batch, n));
     // c is normalised by its mean over the records, so the loss scale does not depend on the
-    // scene's brightness.
-    const float scale = float(double(n) / (cs * double(B)));
+    // scene's brightness, and clamped at kTargetClamp times that mean so that a few extreme
+    // path samples do not steer the fit (the target only shapes the guide; the estimator
+    // stays unbiased whatever the guide is).
+    const double cmean = cs / double(n);
+    const float scale = float(1.0 / (cmean * double(B)));
+    const float cmax = float(kTargetClamp * cmean);
     for (int s = 0; s < steps; ++s)
-        if (int rc = guideTrainDevice(ctx, static_cast<const float4*>(g.dRec.ptr), kRecStride, B, uint32_t(n), scale, lr,
-                                      s + 1 == steps ? loss : nullptr))
+        if (int rc = guideTrainDevice(ctx, static_cast<const float4*>(g.dRec.ptr), kRecStride, B, uint32_t(n), scale, cmax,
+                                      lr, s + 1 == steps ? loss : nullptr))
             return rc;
     return RBR_OK;
 }

@@ -75,6 +75,7 @@ struct GuideInferArgs {
     const uint32_t* paths;
     const DJob* jobs;
     int njobs;
+    uint32_t depth;           // bounce of the queue (jobs guide bounces below their guideDepth)
     uint4* out;               // kGuideRecBytes per query (5 x uint4)
 };
 
@@ -86,6 +87,7 @@ struct GuideTrainArgs {
     // Batch sample s reads record (permAdd + s * permMul) mod total; permMul is a prime that
     // does not divide total, so a batch never repeats a record.  total = 0: record s.
     uint32_t total, permMul, permAdd;
+    float cmax;               // training targets are clamped to cmax before scaling
     size_t cap;               // row length of the feature-major activation arrays
     float scale;              // multiplies c (1 / (batch * mean c))
     uint16_t* act[5];         // A_l, feature-major [K_l][cap]
@@ -116,7 +118,7 @@ __device__ __forceinline__ bool guideQuery(const GuideInferArgs& a, uint32_t i, 
     }
     const uint32_t p = a.paths[i];
     const DJob& jb = a.jobs[findJobD(a.jobs, a.njobs, p)];
-    if (!(jb.guided & 1u)) return false;
+    if (!(jb.guided & 1u) || a.depth >= jb.guideDepth) return false;
     const uint2 h = a.hits[i];
     if (h.x == 0xFFFFFFFFu) return false;
     const float4 s0 = jb.primShade[2 * size_t(h.x)], s1 = jb.primShade[2 * size_t(h.x) + 1];
@@ -149,7 +151,7 @@ __device__ __forceinline__ void guideSample(const GuideTrainArgs& a, uint32_t i,
     dir[0] = d.x;
     dir[1] = d.y;
     dir[2] = d.z;
-    c = isfinite(p.w) ? fmaxf(p.w, 0.0f) : 0.0f;
+    c = isfinite(p.w) ? fminf(fmaxf(p.w, 0.0f), a.cmax) : 0.0f;
 }
 
 // log(1 - exp(-x)) for x > 0
@@ -220,6 +222,7 @@ __device__ __forceinline__ float outputGrad(const float* o, const float* dir, fl
 
 constexpr float kGuideFrac = 0.5f;     // one-sample MIS: probability of drawing from the guide
 constexpr uint32_t kRecStride = 6;     // float4 per recorded training sample
+constexpr double kTargetClamp = 50.0;  // recorded targets above this multiple of their mean are clamped
 
 // Lobe k of array a (0..2 mean, 3 kappa, 4 weight) of a guide record; k static after unrolling.
 __device__ __forceinline__ float guideVal(const uint4* rec, int a, int k) {
@@ -320,7 +323,7 @@ int launchGuideInfer(rbr_ctx* ctx, const GuideInferArgs& a, uint32_t maxN);
 // One Adam step on n samples.  total = 0: samples[0..n); total > 0: a batch of n records
 // drawn without repetition from samples[0..total) by a fresh random permutation.
 int guideTrainDevice(rbr_ctx* ctx, const float4* samples, uint32_t stride, uint32_t n, uint32_t total, float scale,
-                     float lr, double* lossOut);
+                     float cmax, float lr, double* lossOut);
 int guideSetAttributes(rbr_ctx* ctx);
 // Allocates the record buffers (recReserve records) if absent.
 int guideEnsureRecords(rbr_ctx* ctx, GuideState& g);

@@ -748,7 +748,7 @@ __global__ __launch_bounds__(256) void shadeKernel(const DJob* __restrict__ jobs
         // Guided jobs: one-sample MIS between the guide's vMF mixture (probability
         // kGuideFrac) and the BSDF; the direction's density is the mixture of both, a delta
         // BSDF sample keeps its discrete probability.  All-delta materials are never guided.
-        const bool guided = (jb.guided & 1u) && !m.allDelta;
+        const bool guided = (jb.guided & 1u) && !m.allDelta && depth < jb.guideDepth;
         uint4 gl[5];
         if (guided)
             for (int k = 0; k < 5; ++k) gl[k] = q.guide[size_t(i) * 5 + k];
@@ -998,6 +998,10 @@ struct RenderState {
     double stageMs[4] = {0, 0, 0, 0};
     uint32_t poolCap = 1u << 23;
     std::vector<hipEvent_t> events;
+    // Arena of the last render: (scene, version) per slot, slot offsets and traversal args.
+    std::vector<std::pair<const rbr_scene*, uint64_t>> arenaKey;
+    std::vector<uint32_t> arenaLightOff, arenaNodeOff;
+    TraceArgs arenaTA{};
 };
 
 }  // namespace rbr
@@ -1049,6 +1053,7 @@ int rbr_scene_set_shading(rbr_scene* s, const rbr_material* materials, uint32_t 
     s->medium = medium ? *medium : rbr_medium{};
     for (int k = 0; k < 3; ++k) s->background[k] = background ? background[k] : 0.0f;
     s->shaded = true;
+    bumpVersion(s);
     return RBR_OK;
 }
 
@@ -1179,14 +1184,26 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     RenderState& R = *ctx->render;
     const int per = width == 4 ? 8 : 16;
 
-    // ---- node/primitive arena (packed only when several scenes share the launch)
+    // ---- node/primitive arena (packed only when several scenes share the launch); reused
+    // while the same scenes, unchanged, are rendered again
     std::vector<F4> shade;
     std::vector<DMaterial> mats;
     std::vector<DLight> lights;
     TraceArgs ta{};
     int rc;
+    std::vector<std::pair<const rbr_scene*, uint64_t>> key;
+    for (const SceneSlot& sl : scenes) key.emplace_back(sl.s, sl.s->version);
+    const bool arenaValid = key == R.arenaKey;
+    if (arenaValid) {
+        for (size_t k = 0; k < scenes.size(); ++k) {
+            scenes[k].lightOff = R.arenaLightOff[k];
+            scenes[k].nodeOff = R.arenaNodeOff[k];
+        }
+        ta = R.arenaTA;
+    }
     uint32_t nodeOff = 0, primOff = 0, quadOff = 0, idOff = 0;
     for (SceneSlot& sl : scenes) {
+        if (arenaValid) break;
         sl.nodeOff = nodeOff;
         sl.primOff = primOff;
         sl.quadOff = quadOff;
@@ -1202,7 +1219,9 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         idOff += sl.s->geom.anyId ? sl.s->geom.maxId + 1 : 0;
     }
     if (primOff >= (1u << 27)) return fail(ctx, RBR_ERR_UNSUPPORTED, "arena exceeds 2^27 primitives");
-    if (scenes.size() == 1) {
+    if (arenaValid) {
+        // device arena, shading records, materials and lights are current
+    } else if (scenes.size() == 1) {
         ta = sceneTraceArgs(scenes[0].s);
     } else {
         std::vector<F4> nodes(size_t(nodeOff) * per), prims(size_t(primOff) * 3), quadN(quadOff);
@@ -1243,9 +1262,20 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         ta.nplanes = 0;
         ta.mixed = anyMixed ? 1u : 0u;
     }
-    if ((rc = uploadBuffer(ctx, R.arenaShade, shade.data(), std::max<size_t>(1, shade.size()) * sizeof(F4)))) return rc;
-    if ((rc = uploadBuffer(ctx, R.arenaMats, mats.data(), std::max<size_t>(1, mats.size()) * sizeof(DMaterial)))) return rc;
-    if ((rc = uploadBuffer(ctx, R.arenaLights, lights.data(), std::max<size_t>(1, lights.size()) * sizeof(DLight)))) return rc;
+    if (!arenaValid) {
+        R.arenaKey.clear();   // stays invalid if an upload below fails
+        if ((rc = uploadBuffer(ctx, R.arenaShade, shade.data(), std::max<size_t>(1, shade.size()) * sizeof(F4)))) return rc;
+        if ((rc = uploadBuffer(ctx, R.arenaMats, mats.data(), std::max<size_t>(1, mats.size()) * sizeof(DMaterial)))) return rc;
+        if ((rc = uploadBuffer(ctx, R.arenaLights, lights.data(), std::max<size_t>(1, lights.size()) * sizeof(DLight)))) return rc;
+        R.arenaKey = key;
+        R.arenaTA = ta;
+        R.arenaLightOff.clear();
+        R.arenaNodeOff.clear();
+        for (const SceneSlot& sl : scenes) {
+            R.arenaLightOff.push_back(sl.lightOff);
+            R.arenaNodeOff.push_back(sl.nodeOff);
+        }
+    }
 
     // ---- per-job device descriptors and films
     std::vector<DJob> dj(njobs);
@@ -1286,6 +1316,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         d.useNEE = jb.config.use_nee ? 1u : 0u;
         d.useMIS = jb.config.use_mis ? 1u : 0u;
         d.guided = uint32_t(jb.config.guided);
+        d.guideDepth = jb.config.guide_depth ? jb.config.guide_depth : 0xFFFFFFFFu;
         d.root = jb.scene->bvh.empty() ? kEmpty : sl.nodeOff;
         d.pixels = uint32_t(jb.camera.width) * uint32_t(jb.camera.height);
         d.seed = jb.config.seed;
@@ -1401,6 +1432,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
                 gi.paths = q.path[cur];
                 gi.jobs = djd;
                 gi.njobs = njobs;
+                gi.depth = depth;
                 gi.out = static_cast<uint4*>(R.guideOut.ptr);
                 if ((rc = launchGuideInfer(ctx, gi, used))) return rc;
             }

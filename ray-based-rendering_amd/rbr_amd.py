@@ -343,7 +343,7 @@ class RbrCamera(C.Structure):
 class RbrRenderConfig(C.Structure):
     _fields_ = [("integrator", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32), ("use_nee", C.c_int32),
                 ("use_mis", C.c_int32), ("guided", C.c_int32), ("seed", C.c_uint64), ("sample_offset", C.c_uint32),
-                ("pad", C.c_uint32)]
+                ("guide_depth", C.c_uint32)]
 
 
 class RbrJob(C.Structure):
@@ -392,13 +392,13 @@ def c_camera(cam) -> RbrCamera:
 
 
 def c_config(spp: int, depth: int, seed: int = 1, integrator: int = PATH, nee: bool = True, mis: bool = True,
-             sample_offset: int = 0, guided: int = 0) -> RbrRenderConfig:
-    return RbrRenderConfig(integrator, spp, depth, int(nee), int(mis), int(guided), seed, sample_offset, 0)
+             sample_offset: int = 0, guided: int = 0, guide_depth: int = 0) -> RbrRenderConfig:
+    return RbrRenderConfig(integrator, spp, depth, int(nee), int(mis), int(guided), seed, sample_offset, guide_depth)
 
 
 def _job_config(j: dict) -> RbrRenderConfig:
     return c_config(j["spp"], j["depth"], j.get("seed", 1), j.get("integrator", PATH), j.get("nee", True),
-                    j.get("mis", True), j.get("sample_offset", 0), j.get("guided", 0))
+                    j.get("mis", True), j.get("sample_offset", 0), j.get("guided", 0), j.get("guide_depth", 0))
 
 
 def scene_from_desc(ctx: Context, desc, width: int = 4, leaf: int = 4) -> Scene:
