@@ -238,6 +238,16 @@ int rbr_render(rbr_ctx* ctx, const rbr_job* jobs, int njobs);
  * caller arrays (3 doubles per pixel each); used by the convergence tooling. */
 int rbr_render_moments(rbr_ctx* ctx, const rbr_job* job, double* sum_rgb, double* sumsq_rgb);
 
+/* Render launch tuning (the autotuner's knobs; no counterpart in the reference):
+ * pool_paths = paths in flight, i.e. the queue capacity (2^16..2^26, default 2^24; a job
+ * never gets fewer than one sample's worth); use_graphs = 1 records each round's launch
+ * sequence as a hipGraph and replays it while nothing it depends on changes (default 1);
+ * sort_materials = 1 shades each bounce's queue in material order, misses grouped apart
+ * (default 1).  Images do
+ * not depend on use_graphs or sort_materials; a pool that holds fewer of a job's samples
+ * only changes the float summation order of its per-pixel means. */
+int rbr_set_render_tuning(rbr_ctx* ctx, uint32_t pool_paths, int use_graphs, int sort_materials);
+
 /* Time of the last rbr_render call: total ms and per-stage ms (extend, shade, shadow). */
 int rbr_last_render_timing(rbr_ctx* ctx, double* total_ms, double* stage_ms /* 4 */);
 

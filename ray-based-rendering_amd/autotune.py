@@ -1,10 +1,19 @@
-"""Traversal autotuner: sweeps BVH width, leaf size, launch mode, LDS stack depth and
-persistent waves per CU on a resident ray batch, checks that every configuration returns
-bit-identical hits, and records the fastest in a JSON cache keyed by device and workload.
+"""Autotuner.
+
+Traversal (default): sweeps BVH width, leaf size, launch mode, LDS stack depth and persistent
+waves per CU on a resident ray batch, checks that every configuration returns bit-identical
+hits, and records the fastest in tuning/traversal.json (read by `best_config()`; bench.py
+uses it when present).
 
   python ray-based-rendering_amd/autotune.py [--quick] [--out PATH]
 
-The cache is read by `best_config()`; bench.py and the render path use it when present.
+Render: for each path-tracing workload, sweeps the path pool (queue capacity), material-
+ordered shading, hipGraph replay and the persistent traversal waves per CU; checks that every
+configuration gives the same images (bit-identical at equal pool, float summation order
+otherwise) and records the fastest per workload in tuning/render.json (read by
+`render_config()`; bench_paths.py uses it when present).
+
+  python ray-based-rendering_amd/autotune.py render [--only c0,c2,c3] [--quick]
 """
 from __future__ import annotations
 
@@ -22,6 +31,7 @@ sys.path.insert(0, PKG)
 import rbr_amd as R  # noqa: E402
 
 CACHE = os.path.join(PKG, "tuning", "traversal.json")
+RENDER_CACHE = os.path.join(PKG, "tuning", "render.json")
 
 
 def sweep(ctx: R.Context, tris: np.ndarray, rays: np.ndarray, kind: int, widths, leaves, modes, depths, waves,
@@ -74,7 +84,94 @@ def apply(ctx: R.Context, cfg: dict | None) -> None:
         ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, cfg["mode"], cfg["lds_depth"], cfg["waves_per_cu"]))
 
 
+# ---------------------------------------------------------------- render tuning
+
+def render_sweep(ctx: R.Context, rj: list, paths: int, pools, sorts, graphs, waves, reps: int = 2,
+                 log=print) -> list[dict]:
+    results = []
+    ref = None
+    for pool, sort, graph, wpc in itertools.product(pools, sorts, graphs, waves):
+        R.set_render_tuning(ctx, pool, graph, sort)
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, wpc))
+        films = R.render(ctx, rj)   # warm-up; records the round graphs when on
+        times = []
+        for _ in range(reps):
+            R.render(ctx, rj)
+            times.append(R.last_render_ms(ctx))
+        ms = min(times)
+        if ref is None:
+            ref = films
+        else:
+            for a, b in zip(films, ref):
+                if not np.allclose(a, b, rtol=2e-6, atol=1e-7):
+                    raise RuntimeError(f"pool={pool} sort={sort} graphs={graph} waves={wpc} changed the image")
+        rec = {"pool_paths": pool, "sort_materials": int(sort), "use_graphs": int(graph), "waves_per_cu": wpc,
+               "ms": ms, "mpaths_s": paths / ms / 1e3}
+        log(json.dumps(rec))
+        results.append(rec)
+    R.set_render_tuning(ctx)
+    ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
+    return results
+
+
+def render_config(workload: str, path: str = RENDER_CACHE) -> dict | None:
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("workloads", {}).get(workload, {}).get("best")
+
+
+def apply_render(ctx: R.Context, cfg: dict | None) -> None:
+    """Applies a render_config() entry (None: library defaults)."""
+    if cfg:
+        R.set_render_tuning(ctx, cfg["pool_paths"], bool(cfg["use_graphs"]), bool(cfg["sort_materials"]))
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, cfg["waves_per_cu"]))
+    else:
+        R.set_render_tuning(ctx)
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
+
+
+def main_render(argv) -> int:
+    import bench_paths as B
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", default="c0,c2,c3")
+    ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--out", default=RENDER_CACHE)
+    args = ap.parse_args(argv)
+    ctx = R.Context(0)
+    name, cus = ctx.device_info()
+    out = {"device": name, "cus": cus, "time": time.strftime("%Y-%m-%dT%H:%M:%S"), "workloads": {}}
+    if os.path.exists(args.out):
+        with open(args.out) as f:
+            out["workloads"] = json.load(f).get("workloads", {})
+    if args.quick:
+        grid = dict(pools=[1 << 23, 1 << 24], sorts=[False, True], graphs=[True], waves=[32])
+    else:
+        grid = dict(pools=[1 << 22, 1 << 23, 1 << 24, 1 << 25], sorts=[False, True], graphs=[False, True],
+                    waves=[16, 32])
+    for key in args.only.split(","):
+        jobs = B.workload_jobs(key)
+        scs = [R.scene_from_desc(ctx, j["desc"]) for j in jobs]
+        rj = [dict(scene=s, spp=j["spp"], depth=j["depth"], seed=j.get("seed", 1), integrator=j.get("integrator", R.PATH))
+              for s, j in zip(scs, jobs)]
+        paths = sum(j["desc"].camera.width * j["desc"].camera.height * j["spp"] for j in jobs)
+        print(f"workload {key}: {B.WORKLOADS[key]}", flush=True)
+        res = render_sweep(ctx, rj, paths, **grid, log=lambda s: print("  " + s, flush=True))
+        best = min(res, key=lambda x: x["ms"])
+        out["workloads"][key] = {"name": B.WORKLOADS[key], "best": best, "all": res}
+        print(f"  best: {best}", flush=True)
+        for s in scs:
+            s.close()
+    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    with open(args.out, "w") as f:
+        json.dump(out, f, indent=1)
+    ctx.close()
+    return 0
+
+
 def main() -> int:
+    if len(sys.argv) > 1 and sys.argv[1] == "render":
+        return main_render(sys.argv[2:])
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--out", default=CACHE)

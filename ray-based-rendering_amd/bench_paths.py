@@ -50,7 +50,7 @@ def shrink(cam, w, h):
     return S.Camera(cam.position, cam.forward, cam.right, cam.up, cam.fx * s, cam.fy * s, 0.5 * w, 0.5 * h, w, h)
 
 
-def run(ctx, name, jobs, cpu, seconds):
+def run(ctx, name, jobs, cpu, seconds, tuning=None):
     scs = [R.scene_from_desc(ctx, j["desc"]) for j in jobs]
     rj = [dict(scene=s, spp=j["spp"], depth=j["depth"], seed=j.get("seed", 1), integrator=j.get("integrator", R.PATH))
           for s, j in zip(scs, jobs)]
@@ -62,7 +62,8 @@ def run(ctx, name, jobs, cpu, seconds):
     paths = sum(j["desc"].camera.width * j["desc"].camera.height * j["spp"] for j in jobs)
     rec = {"workload": name, "value": paths / ms / 1e3, "unit": "Mpaths/s (path samples/s / 1e6)",
            "gpu_ms": ms, "wall_s": wall, "paths": paths, "jobs": len(jobs),
-           "mean_radiance": [float(sum(f.mean() for f in films) / len(films))], "cpu_baseline": None}
+           "mean_radiance": [float(sum(f.mean() for f in films) / len(films))], "cpu_baseline": None,
+           "tuning": tuning or "library defaults"}
     if cpu:
         j0 = jobs[0]
         cam = j0["desc"].camera
@@ -74,23 +75,42 @@ def run(ctx, name, jobs, cpu, seconds):
         s.close()
 
 
+WORKLOADS = {
+    "c0": "c0 cornell 256x256 16spp depth 8",
+    "c2": "c2 terrain 1920x1080 64spp depth 8",
+    "c3": "c3 16 fog jobs (volumetric)",
+}
+
+
+def workload_jobs(key: str) -> list:
+    """The path-tracing workloads as job lists (scene descriptions + settings)."""
+    if key == "c0":
+        return [dict(desc=S.cornell_c0(), spp=16, depth=8)]
+    if key == "c2":
+        return [dict(desc=S.terrain_c2(), spp=64, depth=8)]
+    if key == "c3":
+        return [dict(desc=j["scene"], spp=j["spp"], depth=j["depth"], seed=j["seed"], integrator=R.VOLPATH)
+                for j in S.fog_jobs_c3()]
+    raise KeyError(key)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", default="c0,c2,c3")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tuning", action="store_true", help="library defaults instead of tuning/render.json")
     args = ap.parse_args()
     want = set(args.only.split(","))
     ctx = R.Context(0)
     cpu = not args.no_cpu
-    if "c0" in want:
-        run(ctx, "c0 cornell 256x256 16spp depth 8", [dict(desc=S.cornell_c0(), spp=16, depth=8)], cpu, args.cpu_seconds)
-    if "c2" in want:
-        run(ctx, "c2 terrain 1920x1080 64spp depth 8", [dict(desc=S.terrain_c2(), spp=64, depth=8)], cpu, args.cpu_seconds)
-    if "c3" in want:
-        jobs = [dict(desc=j["scene"], spp=j["spp"], depth=j["depth"], seed=j["seed"], integrator=R.VOLPATH)
-                for j in S.fog_jobs_c3()]
-        run(ctx, "c3 16 fog jobs (volumetric)", jobs, cpu, args.cpu_seconds)
+    import autotune  # noqa: E402  (tuning cache reader)
+    for key in ("c0", "c2", "c3"):
+        if key not in want:
+            continue
+        cfg = None if args.no_tuning else autotune.render_config(key)
+        autotune.apply_render(ctx, cfg)
+        run(ctx, WORKLOADS[key], workload_jobs(key), cpu, args.cpu_seconds, tuning=cfg)
     ctx.close()
     return 0
 

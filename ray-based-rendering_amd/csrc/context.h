@@ -36,6 +36,10 @@ struct rbr_ctx {
     int ldsDepth = rbr::kLdsDepthDefault;
     int wavesPerCu = rbr::kWavesPerCuDefault;
     rbr::DevBuf dCounter, dSpill;
+    // Render tuning (rbr_set_render_tuning).
+    uint32_t renderPool = 1u << 24;    // paths in flight (queue capacity)
+    int renderGraphs = 1;              // replay each round's launch sequence as a hipGraph
+    int renderSort = 1;                // shade the extension queue in material order
 };
 
 struct rbr_scene {
@@ -70,6 +74,9 @@ TraceArgs sceneTraceArgs(const rbr_scene* s);
 // Launches closest (any=false) or any-hit traversal of `a` (rays/outputs filled in) for a
 // scene whose wide BVH has the given width and stack need, using the context's tuning.
 int launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack, bool any);
+// Allocates what launchTraversal needs for a BVH of this width and stack need, so that the
+// launches themselves allocate nothing (stream capture).
+int prepareTraversal(rbr_ctx* ctx, uint32_t maxStack);
 
 // Gives the scene a new version number (after any change to it).
 void bumpVersion(rbr_scene* s);

@@ -114,6 +114,13 @@ int rbr::launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack
     return RBR_OK;
 }
 
+int rbr::prepareTraversal(rbr_ctx* ctx, uint32_t maxStack) {
+    const unsigned grid = unsigned(std::max(1, ctx->cuCount) * ctx->wavesPerCu);
+    const uint32_t spillDepth = std::max<uint32_t>(1u, maxStack > uint32_t(ctx->ldsDepth) ? maxStack - ctx->ldsDepth : 1u);
+    if (int rc = ensureBuffer(ctx, ctx->dCounter, sizeof(uint32_t))) return rc;
+    return ensureBuffer(ctx, ctx->dSpill, size_t(grid) * spillDepth * 64 * sizeof(uint2));
+}
+
 // ---------------------------------------------------------------- context
 
 extern "C" {

@@ -591,12 +591,16 @@ GuideDev guideDev(const GuideState& g) {
 }
 
 int guideSetAttributes(rbr_ctx* ctx) {
+    GuideState& g = state(ctx);
+    if (g.attrSet) return RBR_OK;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(guideInferKernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(inferSmem()));
     if (e == hipSuccess)
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(guideFwdBwdKernel),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(trainSmem()));
-    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "guide LDS attribute");
+    if (e != hipSuccess) return failHip(ctx, e, "guide LDS attribute");
+    g.attrSet = true;
+    return RBR_OK;
 }
 
 int launchGuideInfer(rbr_ctx* ctx, const GuideInferArgs& a, uint32_t maxN) {

@@ -303,7 +303,7 @@ __device__ __forceinline__ void guideSampleDir(const uint4* rec, float u0, float
 }
 
 struct GuideState {
-    bool ready = false, adamInit = false, masterDirty = false;
+    bool ready = false, adamInit = false, masterDirty = false, attrSet = false;
     int step = 0;
     float lo[3] = {0, 0, 0}, inv[3] = {1, 1, 1};
     std::vector<float> master;

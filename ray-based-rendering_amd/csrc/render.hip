@@ -111,6 +111,17 @@ __device__ __forceinline__ uint32_t dimBase(uint32_t depth) { return 8u + depth 
 
 // ---------------------------------------------------------------- Fresnel and lobes
 
+// Weight arithmetic (BSDF values and densities, Fresnel terms, MIS weights) uses the hardware
+// reciprocal and square root (1 ulp).  Geometry that decides visibility -- hit points, light
+// sample points, shadow-ray origins and lengths -- keeps IEEE division and square root.
+__device__ __forceinline__ float frcp(float b) { return __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ F3 fnormalize(F3 a) {
+    const float l2 = dot(a, a);
+    return l2 > 0.0f ? a * __builtin_amdgcn_rsqf(l2) : mk(0, 0, 0);
+}
+
 // fresnelDielectric, fresnel.h:16-32
 __device__ float fresnelDielectric(float c, float etaI, float etaT) {
     c = fminf(fmaxf(c, -1.0f), 1.0f);
@@ -120,12 +131,12 @@ __device__ float fresnelDielectric(float c, float etaI, float etaT) {
         etaT = tmp;
         c = -c;
     }
-    const float sinI = sqrtf(fmaxf(0.0f, 1.0f - c * c));
-    const float sinT = etaI / etaT * sinI;
+    const float sinI = fsqrt(fmaxf(0.0f, 1.0f - c * c));
+    const float sinT = fdiv(etaI, etaT) * sinI;
     if (sinT >= 1.0f) return 1.0f;
-    const float cosT = sqrtf(fmaxf(0.0f, 1.0f - sinT * sinT));
-    const float rParl = (etaT * c - etaI * cosT) / (etaT * c + etaI * cosT);
-    const float rPerp = (etaI * c - etaT * cosT) / (etaI * c + etaT * cosT);
+    const float cosT = fsqrt(fmaxf(0.0f, 1.0f - sinT * sinT));
+    const float rParl = fdiv(etaT * c - etaI * cosT, etaT * c + etaI * cosT);
+    const float rPerp = fdiv(etaI * c - etaT * cosT, etaI * c + etaT * cosT);
     return 0.5f * (rParl * rParl + rPerp * rPerp);
 }
 
@@ -150,45 +161,45 @@ __device__ float ggxD(float a, F3 wh) {
     if (c2 <= 0.0f) return 0.0f;
     const float a2 = a * a;
     const float d = c2 * (a2 - 1.0f) + 1.0f;
-    return a2 / (kPi * d * d);
+    return fdiv(a2, kPi * d * d);
 }
 __device__ float ggxLambda(float a, F3 w) {
     const float c2 = w.z * w.z;
     if (c2 <= 0.0f) return 0.0f;
-    const float t2 = fmaxf(0.0f, 1.0f - c2) / c2;
-    return 0.5f * (sqrtf(1.0f + a * a * t2) - 1.0f);
+    const float t2 = fdiv(fmaxf(0.0f, 1.0f - c2), c2);
+    return 0.5f * (fsqrt(1.0f + a * a * t2) - 1.0f);
 }
 __device__ float ggxPdf(float a, F3 wo, F3 wh) {
     const float cosO = fabsf(wo.z);
     if (cosO == 0.0f) return 0.0f;
-    return ggxD(a, wh) * (1.0f / (1.0f + ggxLambda(a, wo))) * fabsf(dot(wo, wh)) / cosO;
+    return fdiv(ggxD(a, wh) * frcp(1.0f + ggxLambda(a, wo)) * fabsf(dot(wo, wh)), cosO);
 }
 __device__ F3 ggxSample(float a, F3 wo, float u1, float u2) {   // Heitz 2018 VNDF
     const bool flip = wo.z < 0.0f;
     const F3 v = flip ? -wo : wo;
-    const F3 vh = normalize(mk(a * v.x, a * v.y, v.z));
-    const F3 t1 = vh.z < 0.9999f ? normalize(cross(mk(0, 0, 1), vh)) : mk(1, 0, 0);
+    const F3 vh = fnormalize(mk(a * v.x, a * v.y, v.z));
+    const F3 t1 = vh.z < 0.9999f ? fnormalize(cross(mk(0, 0, 1), vh)) : mk(1, 0, 0);
     const F3 t2 = cross(vh, t1);
-    const float r = sqrtf(u1);
+    const float r = fsqrt(u1);
     const float phi = kTwoPi * u2;
     float sp, cp;
     sincosf(phi, &sp, &cp);
     const float p1 = r * cp;
     float p2 = r * sp;
     const float s = 0.5f * (1.0f + vh.z);
-    p2 = (1.0f - s) * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1)) + s * p2;
-    const F3 nh = t1 * p1 + t2 * p2 + vh * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
-    const F3 wh = normalize(mk(a * nh.x, a * nh.y, fmaxf(1e-6f, nh.z)));
+    p2 = (1.0f - s) * fsqrt(fmaxf(0.0f, 1.0f - p1 * p1)) + s * p2;
+    const F3 nh = t1 * p1 + t2 * p2 + vh * fsqrt(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
+    const F3 wh = fnormalize(mk(a * nh.x, a * nh.y, fmaxf(1e-6f, nh.z)));
     return flip ? -wh : wh;
 }
 
 __device__ bool halfVecT(const DLobe& l, F3 wo, F3 wi, F3& wh, float& eta) {
     if (sameHemi(wo, wi)) return false;
     if (wo.z == 0.0f || wi.z == 0.0f) return false;
-    eta = wo.z > 0.0f ? l.etaIn / l.etaOut : l.etaOut / l.etaIn;
+    eta = wo.z > 0.0f ? fdiv(l.etaIn, l.etaOut) : fdiv(l.etaOut, l.etaIn);
     wh = wo + wi * eta;
     if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return false;
-    wh = normalize(wh);
+    wh = fnormalize(wh);
     if (dot(wo, wh) < 0.0f) wh = -wh;
     return wh.z * wo.z > 0.0f && dot(wi, wh) <= 0.0f;
 }
@@ -202,11 +213,11 @@ __device__ F3 lobeEval(const DLobe& l, F3 wo, F3 wi) {
         if (cosO == 0.0f || cosI == 0.0f) return mk(0, 0, 0);
         F3 wh = wo + wi;
         if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return mk(0, 0, 0);
-        wh = normalize(wh);
+        wh = fnormalize(wh);
         if (wh.z < 0.0f) wh = -wh;
         const F3 F = fresnelEval(l.fkind, l.fEtaI, l.fEtaT, l.f0, dot(wo, wh));
-        const float G = 1.0f / (1.0f + ggxLambda(l.alpha, wo) + ggxLambda(l.alpha, wi));
-        return R * F * (ggxD(l.alpha, wh) * G / (4.0f * cosO * cosI));
+        const float G = frcp(1.0f + ggxLambda(l.alpha, wo) + ggxLambda(l.alpha, wi));
+        return R * F * fdiv(ggxD(l.alpha, wh) * G, 4.0f * cosO * cosI);
     }
     if (l.type == DL_MF_TRANS) {
         F3 wh;
@@ -216,8 +227,8 @@ __device__ F3 lobeEval(const DLobe& l, F3 wo, F3 wi) {
         const float F = fresnelDielectric(wo.z > 0.0f ? dotOH : -dotOH, l.etaOut, l.etaIn);
         const float denom = dotOH + eta * dotIH;
         if (denom == 0.0f) return mk(0, 0, 0);
-        const float G = 1.0f / (1.0f + ggxLambda(l.alpha, wo) + ggxLambda(l.alpha, wi));
-        const float v = ggxD(l.alpha, wh) * G * fabsf(dotIH * dotOH) / fabsf(wi.z * wo.z * denom * denom);
+        const float G = frcp(1.0f + ggxLambda(l.alpha, wo) + ggxLambda(l.alpha, wi));
+        const float v = fdiv(ggxD(l.alpha, wh) * G * fabsf(dotIH * dotOH), fabsf(wi.z * wo.z * denom * denom));
         return R * ((1.0f - F) * v);
     }
     return mk(0, 0, 0);
@@ -229,11 +240,11 @@ __device__ float lobePdf(const DLobe& l, F3 wo, F3 wi) {
         if (!sameHemi(wo, wi)) return 0.0f;
         F3 wh = wo + wi;
         if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return 0.0f;
-        wh = normalize(wh);
+        wh = fnormalize(wh);
         if (wh.z < 0.0f) wh = -wh;
         const float dotOH = dot(wo, wh);
         if (dotOH == 0.0f) return 0.0f;
-        return ggxPdf(l.alpha, wo, wh) / (4.0f * fabsf(dotOH));
+        return fdiv(ggxPdf(l.alpha, wo, wh), 4.0f * fabsf(dotOH));
     }
     if (l.type == DL_MF_TRANS) {
         F3 wh;
@@ -242,7 +253,7 @@ __device__ float lobePdf(const DLobe& l, F3 wo, F3 wi) {
         const float dotOH = dot(wo, wh), dotIH = dot(wi, wh);
         const float denom = dotOH + eta * dotIH;
         if (denom == 0.0f) return 0.0f;
-        return ggxPdf(l.alpha, wo, wh) * (fabsf(eta * eta * dotIH) / (denom * denom));
+        return ggxPdf(l.alpha, wo, wh) * fdiv(fabsf(eta * eta * dotIH), denom * denom);
     }
     return 0.0f;
 }
@@ -264,10 +275,10 @@ __device__ BSample lobeSample(const DLobe& l, F3 wo, float u1, float u2) {
     BSample r{mk(0, 0, 0), mk(0, 0, 0), 0.0f, false};
     const F3 R = ld3(l.R);
     if (l.type == DL_LAMBERT) {   // cosineHemisphere, sampler.h:73-81
-        const float rr = sqrtf(u1);
+        const float rr = fsqrt(u1);
         float s, c;
         sincosf(kTwoPi * u2, &s, &c);
-        r.wi = mk(rr * c, rr * s, sqrtf(fmaxf(0.0f, 1.0f - u1)));
+        r.wi = mk(rr * c, rr * s, fsqrt(fmaxf(0.0f, 1.0f - u1)));
         if (wo.z < 0.0f) r.wi.z = -r.wi.z;
         r.f = R * kInvPi;
         r.pdf = fabsf(r.wi.z) * kInvPi;
@@ -277,7 +288,7 @@ __device__ BSample lobeSample(const DLobe& l, F3 wo, float u1, float u2) {
         const float cosO = fabsf(wo.z);
         if (cosO == 0.0f) return r;
         r.wi = mk(-wo.x, -wo.y, wo.z);
-        r.f = R * fresnelEval(l.fkind, l.fEtaI, l.fEtaT, l.f0, wo.z) * (1.0f / cosO);
+        r.f = R * fresnelEval(l.fkind, l.fEtaI, l.fEtaT, l.f0, wo.z) * frcp(cosO);
         r.pdf = 1.0f;
         r.delta = true;
         return r;
@@ -285,14 +296,14 @@ __device__ BSample lobeSample(const DLobe& l, F3 wo, float u1, float u2) {
     if (l.type == DL_SPEC_TRANS) {
         if (wo.z == 0.0f) return r;
         const bool entering = wo.z > 0.0f;
-        const float eta = entering ? l.etaOut / l.etaIn : l.etaIn / l.etaOut;
+        const float eta = entering ? fdiv(l.etaOut, l.etaIn) : fdiv(l.etaIn, l.etaOut);
         const float cosI = fabsf(wo.z);
         const float sin2T = eta * eta * fmaxf(0.0f, 1.0f - cosI * cosI);
         if (sin2T >= 1.0f) return r;
-        const float cosT = sqrtf(1.0f - sin2T);
+        const float cosT = fsqrt(1.0f - sin2T);
         r.wi = mk(-eta * wo.x, -eta * wo.y, entering ? -cosT : cosT);
         const float fr = fresnelDielectric(wo.z, l.etaOut, l.etaIn);
-        r.f = R * ((1.0f - fr) * eta * eta / cosT);
+        r.f = R * fdiv((1.0f - fr) * eta * eta, cosT);
         r.pdf = 1.0f;
         r.delta = true;
         return r;
@@ -307,10 +318,10 @@ __device__ BSample lobeSample(const DLobe& l, F3 wo, float u1, float u2) {
         const float dotOH = dot(wo, wh);
         if (dotOH <= 0.0f) return r;
         const bool entering = wo.z > 0.0f;
-        const float er = entering ? l.etaOut / l.etaIn : l.etaIn / l.etaOut;
+        const float er = entering ? fdiv(l.etaOut, l.etaIn) : fdiv(l.etaIn, l.etaOut);
         const float sin2T = er * er * fmaxf(0.0f, 1.0f - dotOH * dotOH);
         if (sin2T >= 1.0f) return r;
-        const float cosT = sqrtf(1.0f - sin2T);
+        const float cosT = fsqrt(1.0f - sin2T);
         wi = wh * (er * dotOH - cosT) - wo * er;
         if (sameHemi(wo, wi) || wi.z == 0.0f) return r;
     }
@@ -330,66 +341,78 @@ __device__ float effWeight(const DMaterial& m, int i, F3 wo) {
     return w * fmaxf(0.0f, 1.0f - fresnelEval(m.sfKind, m.sfEtaI, m.sfEtaT, f0, wo.z).x);
 }
 
-__device__ float mixSelection(const DMaterial& m, F3 wo, float* w, float* s) {
+// The mixture's per-vertex state for one wo: effective lobe weights w (substrate lobes
+// attenuated by the coat's Fresnel) and selection scores s = w * max(rho), total = sum s.  The
+// shading kernel computes it once per vertex and keeps it in LDS (element i at [i * kShadeBlock])
+// for all of the vertex's light samples and its BSDF sample.
+constexpr int kShadeBlock = 256;
+
+struct MixW {
+    float* w;
+    float* s;
+    float total;
+};
+
+__device__ MixW mixWeights(const DMaterial& m, F3 wo, float* w, float* s) {
     float total = 0.0f;
     for (uint32_t i = 0; i < m.n; ++i) {
-        w[i] = effWeight(m, int(i), wo);
-        s[i] = w[i] * maxc(lobeRho(m.lobe[i], wo));
-        total += s[i];
+        const float wi = effWeight(m, int(i), wo);
+        const float si = wi * maxc(lobeRho(m.lobe[i], wo));
+        w[i * kShadeBlock] = wi;
+        s[i * kShadeBlock] = si;
+        total += si;
     }
-    return total;
+    return MixW{w, s, total};
 }
 
 __device__ __forceinline__ bool lobeDelta(const DLobe& l) { return l.type == DL_SPEC_REFL || l.type == DL_SPEC_TRANS; }
 
-__device__ F3 mixEval(const DMaterial& m, F3 wo, F3 wi) {
+__device__ F3 mixEval(const DMaterial& m, const MixW& x, F3 wo, F3 wi) {
     F3 f = mk(0, 0, 0);
     for (uint32_t i = 0; i < m.n; ++i) {
         if (lobeDelta(m.lobe[i])) continue;
-        f = f + lobeEval(m.lobe[i], wo, wi) * effWeight(m, int(i), wo);
+        f = f + lobeEval(m.lobe[i], wo, wi) * x.w[i * kShadeBlock];
     }
     return f;
 }
 
-__device__ float mixPdf(const DMaterial& m, F3 wo, F3 wi) {
-    float w[6], s[6];
-    const float total = mixSelection(m, wo, w, s);
-    if (total <= 0.0f) return 0.0f;
+__device__ float mixPdf(const DMaterial& m, const MixW& x, F3 wo, F3 wi) {
+    if (x.total <= 0.0f) return 0.0f;
     float p = 0.0f;
     for (uint32_t i = 0; i < m.n; ++i) {
         if (lobeDelta(m.lobe[i])) continue;
-        p += s[i] * lobePdf(m.lobe[i], wo, wi);
+        p += x.s[i * kShadeBlock] * lobePdf(m.lobe[i], wo, wi);
     }
-    return p / total;
+    return fdiv(p, x.total);
 }
 
-__device__ BSample mixSample(const DMaterial& m, F3 wo, float u1, float u2) {
+__device__ BSample mixSample(const DMaterial& m, const MixW& x, F3 wo, float u1, float u2) {
     BSample none{mk(0, 0, 0), mk(0, 0, 0), 0.0f, false};
-    float w[6], s[6];
-    const float total = mixSelection(m, wo, w, s);
-    if (total <= 0.0f) return none;
-    float u = u1 * total;
+    if (x.total <= 0.0f) return none;
+    float u = u1 * x.total;
     int chosen = int(m.n) - 1;
     for (uint32_t i = 0; i < m.n; ++i) {
-        if (s[i] <= 0.0f) continue;
-        if (u < s[i] || i + 1 == m.n) {
+        const float si = x.s[i * kShadeBlock];
+        if (si <= 0.0f) continue;
+        if (u < si || i + 1 == m.n) {
             chosen = int(i);
             break;
         }
-        u -= s[i];
+        u -= si;
     }
-    if (s[chosen] <= 0.0f) return none;
-    const float v1 = fminf(u / s[chosen], 0.99999994f);
+    const float sc = x.s[chosen * kShadeBlock];
+    if (sc <= 0.0f) return none;
+    const float v1 = fminf(fdiv(u, sc), 0.99999994f);
     const DLobe& l = m.lobe[chosen];
     BSample r = lobeSample(l, wo, v1, u2);
     if (r.pdf <= 0.0f || black(r.f)) return none;
     if (lobeDelta(l)) {
-        r.f = r.f * w[chosen];
-        r.pdf *= s[chosen] / total;
+        r.f = r.f * x.w[chosen * kShadeBlock];
+        r.pdf *= fdiv(sc, x.total);
         return r;
     }
-    r.f = mixEval(m, wo, r.wi);
-    r.pdf = mixPdf(m, wo, r.wi);
+    r.f = mixEval(m, x, wo, r.wi);
+    r.pdf = mixPdf(m, x, wo, r.wi);
     if (r.pdf <= 0.0f) return none;
     return r;
 }
@@ -400,9 +423,10 @@ __device__ float powerHeuristic(float f, float g) {
     if (isinf(g)) return 0.0f;
     const float m = fmaxf(f, g);
     if (m <= 0.0f) return 0.0f;
-    f /= m;
-    g /= m;
-    return (f * f) / (f * f + g * g);
+    const float im = frcp(m);
+    f *= im;
+    g *= im;
+    return fdiv(f * f, f * f + g * g);
 }
 
 // ---------------------------------------------------------------- lights and media
@@ -554,7 +578,14 @@ struct Queues {
     uint32_t* recOf;      // [path] record written at this bounce or ~0
     uint32_t* recCount;   // {appended, finalised}
     uint32_t recCap;
+    // material-ordered shading (null when off): shade entry i is queue entry perm[i]
+    uint32_t* perm;
+    uint32_t* sortKey;    // [entry] material bin
+    uint32_t* bins;       // [kSortBins] counts, then running offsets
 };
+
+constexpr uint32_t kSortBins = 1024;   // material bins (last = miss / no material)
+
 
 __device__ __forceinline__ uint32_t waveAppend(uint32_t* counter, bool want) {
     const unsigned long long m = __ballot(want ? 1 : 0);
@@ -613,11 +644,10 @@ __global__ __launch_bounds__(256) void cameraKernel(const DJob* __restrict__ job
     q.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// One bounce of shading for every path in the current extension queue.
-__global__ __launch_bounds__(256) void shadeKernel(const DJob* __restrict__ jobs, int njobs, uint32_t depth,
-                                                   uint32_t maxLights, int cur, Queues q) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t count = q.counts[cur];
+// One bounce of shading for queue entry i (lanes with i >= count only take part in the wave
+// collectives).  mixLds: this block's LDS for the per-vertex mixture weights.
+__device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int njobs, uint32_t depth, uint32_t maxLights,
+                                           int cur, const Queues& q, uint32_t i, uint32_t count, float* mixLds) {
     const bool active = i < count;
     const int nxt = cur ^ 1;
 
@@ -633,14 +663,15 @@ __global__ __launch_bounds__(256) void shadeKernel(const DJob* __restrict__ jobs
     uint32_t base = dimBase(depth);
     bool nee = false;
 
+    const uint32_t src = (active && q.perm) ? q.perm[i] : i;   // queue entry shaded by this lane
     if (active) {
-        p = q.path[cur][i];
+        p = q.path[cur][src];
         j = findJob(jobs, njobs, p);
         const DJob& jb = jobs[j];
-        const float4 r0 = q.ray[cur][2 * size_t(i)], r1 = q.ray[cur][2 * size_t(i) + 1];
+        const float4 r0 = q.ray[cur][2 * size_t(src)], r1 = q.ray[cur][2 * size_t(src) + 1];
         const F3 ro = mk(r0.x, r0.y, r0.z);
         rd = mk(r0.w, r1.x, r1.y);
-        const uint2 h = q.hit[i];
+        const uint2 h = q.hit[src];
         const float4 t4 = q.T[p], l4 = q.L[p];
         T = mk(t4.x, t4.y, t4.z);
         spec = t4.w != 0.0f;
@@ -690,6 +721,8 @@ __global__ __launch_bounds__(256) void shadeKernel(const DJob* __restrict__ jobs
     const bool volInt = active && jb.integrator == RBR_INTEGRATOR_VOLPATH;
     Frame frame(mode == 1 ? outward : mk(0, 0, 1));
     const F3 woL = frame.toLocal(-rd);
+    MixW mx{mixLds + threadIdx.x, mixLds + 6 * kShadeBlock + threadIdx.x, 0.0f};
+    if (mode == 1) mx = mixWeights(jb.mats[mat], woL, mx.w, mx.s);
     for (uint32_t l = 0; l < maxLights; ++l) {
         bool want = false;
         F3 so = mk(0, 0, 0), sd = mk(0, 0, 0), contrib = mk(0, 0, 0);
@@ -714,12 +747,12 @@ __global__ __launch_bounds__(256) void shadeKernel(const DJob* __restrict__ jobs
                 if (ls.pdf > 0.0f && !black(ls.Li)) {
                     const DMaterial& m = jb.mats[mat];
                     const F3 wiL = frame.toLocal(ls.wi);
-                    const F3 f = mixEval(m, woL, wiL);
+                    const F3 f = mixEval(m, mx, woL, wiL);
                     if (!black(f)) {
                         const float c = fabsf(dot(ls.wi, n));
                         float w = 1.0f;
-                        if (!volInt && jb.useMIS && lt.type == RBR_LIGHT_AREA) w = powerHeuristic(ls.pdf, mixPdf(m, woL, wiL));
-                        contrib = T * f * ls.Li * (c * w / ls.pdf);
+                        if (!volInt && jb.useMIS && lt.type == RBR_LIGHT_AREA) w = powerHeuristic(ls.pdf, mixPdf(m, mx, woL, wiL));
+                        contrib = T * f * ls.Li * fdiv(c * w, ls.pdf);
                         spawn(P, n, ls.wi, so, sd);
                         smax = ls.dist - kEpsD;
                         want = true;
@@ -751,13 +784,13 @@ __global__ __launch_bounds__(256) void shadeKernel(const DJob* __restrict__ jobs
         const bool guided = (jb.guided & 1u) && !m.allDelta && depth < jb.guideDepth;
         uint4 gl[5];
         if (guided)
-            for (int k = 0; k < 5; ++k) gl[k] = q.guide[size_t(i) * 5 + k];
+            for (int k = 0; k < 5; ++k) gl[k] = q.guide[size_t(src) * 5 + k];
         const bool fromGuide = guided && rng.u(base + 6) < kGuideFrac;
         BSample bs{mk(0, 0, 0), mk(0, 0, 0), 0.0f, false};
         F3 wi = mk(0, 0, 0);
         float pdf = 0.0f;
         if (!fromGuide) {
-            bs = mixSample(m, woL, rng.u(base + 0), rng.u(base + 1));
+            bs = mixSample(m, mx, woL, rng.u(base + 0), rng.u(base + 1));
             if (bs.pdf > 0.0f && !black(bs.f)) {
                 wi = frame.toWorld(bs.wi);
                 pdf = bs.pdf;
@@ -772,18 +805,18 @@ __global__ __launch_bounds__(256) void shadeKernel(const DJob* __restrict__ jobs
             guideSampleDir(gl, rng.u(base + 7), rng.u(base + 4), rng.u(base + 5), w3);
             wi = normalize(mk(w3[0], w3[1], w3[2]));
             const F3 wiL = frame.toLocal(wi);
-            bs.f = mixEval(m, woL, wiL);
+            bs.f = mixEval(m, mx, woL, wiL);
             const float wa[3] = {wi.x, wi.y, wi.z};
-            pdf = kGuideFrac * guideMixPdf(gl, wa) + (1.0f - kGuideFrac) * mixPdf(m, woL, wiL);
+            pdf = kGuideFrac * guideMixPdf(gl, wa) + (1.0f - kGuideFrac) * mixPdf(m, mx, woL, wiL);
             if (black(bs.f)) pdf = 0.0f;
         }
         if (pdf > 0.0f) {
-            T = T * bs.f * (fabsf(dot(wi, n)) / pdf);
+            T = T * bs.f * fdiv(fabsf(dot(wi, n)), pdf);
             bool alive = true;
             if (depth > 3) {
                 const float qq = fmaxf(0.05f, 1.0f - maxc(T));
                 if (rng.u(base + 2) < qq) alive = false;
-                else T = T * (1.0f / (1.0f - qq));
+                else T = T * frcp(1.0f - qq);
             }
             if (alive) {
                 spec = bs.delta;
@@ -822,11 +855,94 @@ __global__ __launch_bounds__(256) void shadeKernel(const DJob* __restrict__ jobs
     }
 }
 
+// Persistent shading: the grid (sized to the device, not to the queue, whose length is only
+// known on the device) walks the queue in block-sized steps.
+__global__ __launch_bounds__(256, 4) void shadeKernel(const DJob* __restrict__ jobs, int njobs, uint32_t depth,
+                                                   uint32_t maxLights, int cur, Queues q) {
+    __shared__ float mixLds[2 * 6 * kShadeBlock];
+    const uint32_t count = q.counts[cur];
+    for (uint32_t b = blockIdx.x * 256u; b < count; b += gridDim.x * 256u)
+        shadeEntry(jobs, njobs, depth, maxLights, cur, q, b + threadIdx.x, count, mixLds);
+}
+
+// Material-ordered shading: bin of each extension-queue entry (its hit's material; misses and
+// primitives without material share the last bin) and the count per bin, from per-block
+// histograms (one global atomic per block and bin).
+__global__ __launch_bounds__(256) void sortKeyKernel(const DJob* __restrict__ jobs, int cur, Queues q) {
+    __shared__ uint32_t cnt[kSortBins];
+    for (uint32_t t = threadIdx.x; t < kSortBins; t += 256u) cnt[t] = 0u;
+    __syncthreads();
+    const uint32_t count = q.counts[cur];
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) {
+        uint32_t key = kSortBins - 1;
+        const uint2 h = q.hit[i];
+        if (h.x != RBR_MISS_ID) {
+            const int32_t m = __float_as_int(jobs[0].primShade[2 * size_t(h.x) + 1].w);   // arena-global ids
+            if (m >= 0) key = uint32_t(m) % (kSortBins - 1);
+        }
+        q.sortKey[i] = key;
+        atomicAdd(&cnt[key], 1u);
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < kSortBins; t += 256u)
+        if (cnt[t]) atomicAdd(&q.bins[t], cnt[t]);
+}
+
+// Exclusive prefix sum of the bin counts (one block).
+__global__ __launch_bounds__(1024) void sortScanKernel(uint32_t* bins) {
+    __shared__ uint32_t sh[kSortBins];
+    const uint32_t t = threadIdx.x;
+    sh[t] = bins[t];
+    __syncthreads();
+    for (uint32_t off = 1; off < kSortBins; off <<= 1) {
+        const uint32_t v = t >= off ? sh[t - off] : 0u;
+        __syncthreads();
+        sh[t] += v;
+        __syncthreads();
+    }
+    bins[t] = t ? sh[t - 1] : 0u;
+}
+
+// Scatter in chunks of 1024 entries per block: ranks within the chunk from an LDS histogram,
+// one global atomic per (chunk, bin) reserves the chunk's range in the bin.
+constexpr uint32_t kSortChunk = 1024;
+
+__global__ __launch_bounds__(256) void sortScatterKernel(int cur, Queues q) {
+    __shared__ uint32_t cnt[kSortBins];
+    const uint32_t count = q.counts[cur];
+    for (uint32_t c0 = blockIdx.x * kSortChunk; c0 < count; c0 += gridDim.x * kSortChunk) {
+        for (uint32_t t = threadIdx.x; t < kSortBins; t += 256u) cnt[t] = 0u;
+        __syncthreads();
+        uint32_t key[kSortChunk / 256], rank[kSortChunk / 256];
+#pragma unroll
+        for (uint32_t k = 0; k < kSortChunk / 256; ++k) {
+            const uint32_t i = c0 + k * 256u + threadIdx.x;
+            key[k] = 0u;
+            rank[k] = 0u;
+            if (i < count) {
+                key[k] = q.sortKey[i];
+                rank[k] = atomicAdd(&cnt[key[k]], 1u);
+            }
+        }
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < kSortBins; t += 256u)
+            if (cnt[t]) cnt[t] = atomicAdd(&q.bins[t], cnt[t]);   // now the chunk's base in bin t
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kSortChunk / 256; ++k) {
+            const uint32_t i = c0 + k * 256u + threadIdx.x;
+            if (i < count) q.perm[cnt[key[k]] + rank[k]] = i;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void zeroBinsKernel(uint32_t* bins) {
+    for (uint32_t t = threadIdx.x; t < kSortBins; t += 256u) bins[t] = 0u;
+}
+
 // Adds each shaded path's unoccluded NEE terms in light order.
-__global__ __launch_bounds__(256) void gatherKernel(uint32_t maxLights, int cur, Queues q) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= q.counts[cur]) return;
-    const uint32_t p = q.path[cur][i];
+__device__ __forceinline__ void gatherEntry(uint32_t maxLights, const Queues& q, uint32_t p) {
     float4 l4 = q.L[p];
     for (uint32_t l = 0; l < maxLights; ++l) {
         const int32_t s = q.sindex[size_t(p) * maxLights + l];
@@ -842,6 +958,12 @@ __global__ __launch_bounds__(256) void gatherKernel(uint32_t maxLights, int cur,
         const uint32_t r = q.recOf[p];
         if (r != 0xFFFFFFFFu) q.rec[size_t(r) * kRecStride + 5] = l4;
     }
+}
+
+__global__ __launch_bounds__(256) void gatherKernel(uint32_t maxLights, int cur, Queues q) {
+    const uint32_t count = q.counts[cur];
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u)
+        gatherEntry(maxLights, q, q.path[cur][i]);
 }
 
 // Turns this round's records into training targets once their paths have finished:
@@ -996,13 +1118,23 @@ struct RenderState {
     std::vector<DevBuf> films, momS, momQ;
     double lastTotalMs = 0.0;
     double stageMs[4] = {0, 0, 0, 0};
-    uint32_t poolCap = 1u << 23;
     std::vector<hipEvent_t> events;
     // Arena of the last render: (scene, version) per slot, slot offsets and traversal args.
     std::vector<std::pair<const rbr_scene*, uint64_t>> arenaKey;
     std::vector<uint32_t> arenaLightOff, arenaNodeOff;
     TraceArgs arenaTA{};
+    // material sort buffers
+    DevBuf perm, sortKey, bins;
+    // recent rounds' launch sequences as executable graphs, with the values each was built
+    // for (most recently used first)
+    struct Graph {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
 };
+
+constexpr size_t kGraphCache = 8;
 
 }  // namespace rbr
 
@@ -1025,6 +1157,8 @@ void rbr::destroyRenderState(rbr_ctx* ctx) {
     for (auto* v : {&r->films, &r->momS, &r->momQ})
         for (DevBuf& b : *v) freeBuffer(b);
     for (hipEvent_t e : r->events) (void)hipEventDestroy(e);
+    for (DevBuf* b : {&r->perm, &r->sortKey, &r->bins}) freeBuffer(*b);
+    for (auto& g : r->graphs) (void)hipGraphExecDestroy(g.exec);
     delete r;
     ctx->render = nullptr;
 }
@@ -1338,7 +1472,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     // ---- pool
     uint32_t maxPixels = 0;
     for (const DJob& d : dj) maxPixels = std::max(maxPixels, d.pixels);
-    const uint32_t P = uint32_t(std::min<uint64_t>(std::max<uint64_t>(R.poolCap, maxPixels), totalPaths));
+    const uint32_t P = uint32_t(std::min<uint64_t>(std::max<uint64_t>(ctx->renderPool, maxPixels), totalPaths));
     const uint32_t ML = std::max(1u, maxLights);
     struct { DevBuf* b; size_t bytes; } allocs[] = {
         {&R.ray[0], size_t(P) * 32}, {&R.ray[1], size_t(P) * 32}, {&R.path[0], size_t(P) * 4}, {&R.path[1], size_t(P) * 4},
@@ -1373,6 +1507,17 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     q.recOf = anyRecord ? static_cast<uint32_t*>(R.recOf.ptr) : nullptr;
     q.recCount = anyRecord ? static_cast<uint32_t*>(ctx->guide->dRecCount.ptr) : nullptr;
     q.recCap = anyRecord ? ctx->guide->recCap : 0u;
+    q.perm = q.sortKey = q.bins = nullptr;
+    if (ctx->renderSort) {
+        if ((rc = ensureBuffer(ctx, R.perm, size_t(P) * 4)) || (rc = ensureBuffer(ctx, R.sortKey, size_t(P) * 4)) ||
+            (rc = ensureBuffer(ctx, R.bins, kSortBins * 4)))
+            return rc;
+        q.perm = static_cast<uint32_t*>(R.perm.ptr);
+        q.sortKey = static_cast<uint32_t*>(R.sortKey.ptr);
+        q.bins = static_cast<uint32_t*>(R.bins.ptr);
+    }
+    if ((rc = prepareTraversal(ctx, maxStack))) return rc;
+    if (anyGuided && (rc = guideSetAttributes(ctx))) return rc;
 
     // ---- rounds: give every unfinished job as many whole samples as the pool holds
     std::vector<uint32_t> done(njobs, 0);
@@ -1410,60 +1555,129 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         if (e != hipSuccess) return failHip(ctx, e, "job upload");
         const DJob* djd = static_cast<const DJob*>(R.jobs.ptr);
         const unsigned grid = (used + 255) / 256;
-        hipLaunchKernelGGL(cameraKernel, dim3(grid), dim3(256), 0, ctx->stream, djd, njobs, used, q);
-        int cur = 0;
-        for (uint32_t depth = 0; depth < depthRound; ++depth) {
-            hipLaunchKernelGGL(resetCountsKernel, dim3(1), dim3(1), 0, ctx->stream, q.counts, cur ^ 1);
-            TraceArgs a = ta;
-            a.rays = q.ray[cur];
-            a.n = used;
-            a.nDev = q.counts + cur;
-            a.rootOf = q.root[cur];
-            a.hits = q.hit;
-            a.occ = nullptr;
-            if ((rc = launchTraversal(ctx, a, width, maxStack, false))) return rc;
+        // persistent kernels: enough blocks to fill the device, never more than the queue needs
+        const unsigned cus = unsigned(std::max(1, ctx->cuCount));
+        const unsigned gridPersist = std::min(grid, cus * 8u);
+        const unsigned gridSort = std::min((used + kSortChunk - 1) / kSortChunk, cus * 4u);
+        // The round's launch sequence; it allocates nothing, so it can be captured.
+        auto launchRound = [&]() -> int {
+            hipLaunchKernelGGL(cameraKernel, dim3(grid), dim3(256), 0, ctx->stream, djd, njobs, used, q);
+            int cur = 0;
+            for (uint32_t depth = 0; depth < depthRound; ++depth) {
+                hipLaunchKernelGGL(resetCountsKernel, dim3(1), dim3(1), 0, ctx->stream, q.counts, cur ^ 1);
+                TraceArgs a = ta;
+                a.rays = q.ray[cur];
+                a.n = used;
+                a.nDev = q.counts + cur;
+                a.rootOf = q.root[cur];
+                a.hits = q.hit;
+                a.occ = nullptr;
+                if (int r2 = launchTraversal(ctx, a, width, maxStack, false)) return r2;
+                if (anyGuided) {
+                    GuideInferArgs gi{};
+                    gi.g = guideDev(*ctx->guide);
+                    gi.n = used;
+                    gi.nDev = q.counts + cur;
+                    gi.rays = q.ray[cur];
+                    gi.hits = q.hit;
+                    gi.paths = q.path[cur];
+                    gi.jobs = djd;
+                    gi.njobs = njobs;
+                    gi.depth = depth;
+                    gi.out = static_cast<uint4*>(R.guideOut.ptr);
+                    if (int r2 = launchGuideInfer(ctx, gi, used)) return r2;
+                }
+                if (q.perm) {
+                    hipLaunchKernelGGL(zeroBinsKernel, dim3(1), dim3(256), 0, ctx->stream, q.bins);
+                    hipLaunchKernelGGL(sortKeyKernel, dim3(gridSort), dim3(256), 0, ctx->stream, djd, cur, q);
+                    hipLaunchKernelGGL(sortScanKernel, dim3(1), dim3(kSortBins), 0, ctx->stream, q.bins);
+                    hipLaunchKernelGGL(sortScatterKernel, dim3(gridSort), dim3(256), 0, ctx->stream, cur, q);
+                }
+                hipLaunchKernelGGL(shadeKernel, dim3(gridPersist), dim3(256), 0, ctx->stream, djd, njobs, depth, maxLights,
+                                   cur, q);
+                if (maxLights) {
+                    TraceArgs b = ta;
+                    b.rays = q.sray;
+                    b.n = used * ML;
+                    b.nDev = q.counts + 2;
+                    b.rootOf = q.sroot;
+                    b.hits = nullptr;
+                    b.occ = q.socc;
+                    if (int r2 = launchTraversal(ctx, b, width, maxStack, true)) return r2;
+                    hipLaunchKernelGGL(gatherKernel, dim3(gridPersist), dim3(256), 0, ctx->stream, maxLights, cur, q);
+                }
+                cur ^= 1;
+            }
+            if (anyRecord) {
+                GuideState& gs = *ctx->guide;
+                hipLaunchKernelGGL(guideFinalizeKernel, dim3(1024), dim3(256), 0, ctx->stream, q.rec, q.L,
+                                   static_cast<const uint32_t*>(gs.dRecCount.ptr), gs.recCap,
+                                   static_cast<double*>(gs.dCsum.ptr));
+                hipLaunchKernelGGL(guideFinalizeEndKernel, dim3(1), dim3(1), 0, ctx->stream,
+                                   static_cast<uint32_t*>(gs.dRecCount.ptr), gs.recCap);
+            }
+            for (int j = 0; j < njobs; ++j) {
+                if (!take[j]) continue;
+                hipLaunchKernelGGL(accumulateKernel, dim3((dj[j].pixels + 255) / 256), dim3(256), 0, ctx->stream, djd, j,
+                                   take[j], q.L);
+            }
+            hipError_t le = hipGetLastError();
+            return le == hipSuccess ? RBR_OK : failHip(ctx, le, "render launch");
+        };
+        if (!ctx->renderGraphs) {
+            if ((rc = launchRound())) return rc;
+        } else {
+            // Every value a launch of the round depends on; the captured graph is replayed while
+            // it is unchanged (device-side job table and counters are rewritten every round).
+            std::vector<uint64_t> key;
+            auto put = [&](uint64_t v) { key.push_back(v); };
+            auto putp = [&](const void* ptr) { key.push_back(uint64_t(reinterpret_cast<uintptr_t>(ptr))); };
+            auto putf = [&](float f) { uint32_t u; std::memcpy(&u, &f, 4); key.push_back(u); };
+            put(used); put(depthRound); put(maxLights); put(uint64_t(width)); put(maxStack); put(uint64_t(njobs));
+            put(anyGuided); put(anyRecord);
+            for (int j = 0; j < njobs; ++j) { put(take[j]); put(dj[j].pixels); }
+            putp(ta.nodes); putp(ta.prims); putp(ta.quadN); putp(ta.planes); putp(ta.planeIds); put(ta.nplanes); put(ta.mixed);
+            for (int k = 0; k < 2; ++k) { putp(q.ray[k]); putp(q.path[k]); putp(q.root[k]); }
+            putp(q.hit); putp(q.sray); putp(q.sroot); putp(q.scontrib); putp(q.socc); putp(q.sindex); putp(q.T); putp(q.L);
+            putp(q.counts); putp(q.guide); putp(q.rec); putp(q.recOf); putp(q.recCount); put(q.recCap);
+            putp(q.perm); putp(q.sortKey); putp(q.bins); putp(djd);
             if (anyGuided) {
-                GuideInferArgs gi{};
-                gi.g = guideDev(*ctx->guide);
-                gi.n = used;
-                gi.nDev = q.counts + cur;
-                gi.rays = q.ray[cur];
-                gi.hits = q.hit;
-                gi.paths = q.path[cur];
-                gi.jobs = djd;
-                gi.njobs = njobs;
-                gi.depth = depth;
-                gi.out = static_cast<uint4*>(R.guideOut.ptr);
-                if ((rc = launchGuideInfer(ctx, gi, used))) return rc;
+                const GuideDev gd = guideDev(*ctx->guide);
+                putp(gd.w); putp(gd.wf); putp(gd.b);
+                for (int k = 0; k < 3; ++k) { putf(gd.lo[k]); putf(gd.inv[k]); }
             }
-            hipLaunchKernelGGL(shadeKernel, dim3(grid), dim3(256), 0, ctx->stream, djd, njobs, depth, maxLights, cur, q);
-            if (maxLights) {
-                TraceArgs b = ta;
-                b.rays = q.sray;
-                b.n = used * ML;
-                b.nDev = q.counts + 2;
-                b.rootOf = q.sroot;
-                b.hits = nullptr;
-                b.occ = q.socc;
-                if ((rc = launchTraversal(ctx, b, width, maxStack, true))) return rc;
-                hipLaunchKernelGGL(gatherKernel, dim3(grid), dim3(256), 0, ctx->stream, maxLights, cur, q);
+            if (anyRecord) { putp(ctx->guide->dRecCount.ptr); putp(ctx->guide->dCsum.ptr); }
+            put(uint64_t(ctx->traceMode)); put(uint64_t(ctx->ldsDepth)); put(uint64_t(ctx->wavesPerCu));
+            putp(ctx->dCounter.ptr); putp(ctx->dSpill.ptr); put(ctx->dSpill.bytes);
+            size_t hit = R.graphs.size();
+            for (size_t k = 0; k < R.graphs.size(); ++k)
+                if (R.graphs[k].key == key) hit = k;
+            if (hit < R.graphs.size()) {
+                std::rotate(R.graphs.begin(), R.graphs.begin() + hit, R.graphs.begin() + hit + 1);
+            } else {
+                if (R.graphs.size() >= kGraphCache) {
+                    (void)hipGraphExecDestroy(R.graphs.back().exec);
+                    R.graphs.pop_back();
+                }
+                if ((e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+                    return failHip(ctx, e, "hipStreamBeginCapture");
+                const int lrc = launchRound();
+                hipGraph_t graph = nullptr;
+                e = hipStreamEndCapture(ctx->stream, &graph);
+                if (lrc) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return lrc;
+                }
+                if (e != hipSuccess) return failHip(ctx, e, "hipStreamEndCapture");
+                hipGraphExec_t exec = nullptr;
+                e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (e != hipSuccess) return failHip(ctx, e, "hipGraphInstantiate");
+                R.graphs.insert(R.graphs.begin(), RenderState::Graph{key, exec});
             }
-            cur ^= 1;
+            if ((e = hipGraphLaunch(R.graphs.front().exec, ctx->stream)) != hipSuccess) return failHip(ctx, e, "hipGraphLaunch");
         }
-        if (anyRecord) {
-            GuideState& gs = *ctx->guide;
-            hipLaunchKernelGGL(guideFinalizeKernel, dim3(1024), dim3(256), 0, ctx->stream, q.rec, q.L,
-                               static_cast<const uint32_t*>(gs.dRecCount.ptr), gs.recCap, static_cast<double*>(gs.dCsum.ptr));
-            hipLaunchKernelGGL(guideFinalizeEndKernel, dim3(1), dim3(1), 0, ctx->stream, static_cast<uint32_t*>(gs.dRecCount.ptr),
-                               gs.recCap);
-        }
-        for (int j = 0; j < njobs; ++j) {
-            if (!take[j]) continue;
-            hipLaunchKernelGGL(accumulateKernel, dim3((dj[j].pixels + 255) / 256), dim3(256), 0, ctx->stream, djd, j,
-                               take[j], q.L);
-            done[j] += take[j];
-        }
-        if ((e = hipGetLastError()) != hipSuccess) return failHip(ctx, e, "render launch");
+        for (int j = 0; j < njobs; ++j) done[j] += take[j];
     }
     if ((e = hipEventRecord(ev[1], ctx->stream)) != hipSuccess) return failHip(ctx, e, "hipEventRecord");
     // ---- films
@@ -1501,6 +1715,17 @@ int rbr_render_moments(rbr_ctx* ctx, const rbr_job* job, double* sum_rgb, double
     double* s[1] = {sum_rgb};
     double* q[1] = {sumsq_rgb};
     return renderImpl(ctx, job, 1, s, q);
+}
+
+int rbr_set_render_tuning(rbr_ctx* ctx, uint32_t pool_paths, int use_graphs, int sort_materials) {
+    if (!ctx) return RBR_ERR_INVALID;
+    if (pool_paths < (1u << 16) || pool_paths > (1u << 26)) return fail(ctx, RBR_ERR_INVALID, "pool_paths must be in 2^16..2^26");
+    if ((use_graphs != 0 && use_graphs != 1) || (sort_materials != 0 && sort_materials != 1))
+        return fail(ctx, RBR_ERR_INVALID, "use_graphs and sort_materials are 0 or 1");
+    ctx->renderPool = pool_paths;
+    ctx->renderGraphs = use_graphs;
+    ctx->renderSort = sort_materials;
+    return RBR_OK;
 }
 
 int rbr_last_render_timing(rbr_ctx* ctx, double* total_ms, double* stage_ms) {

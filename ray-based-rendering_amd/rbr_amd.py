@@ -358,6 +358,12 @@ _sig(_lib.rbr_scene_set_shading, [_vp, C.POINTER(RbrMaterial), _u32, _vp, _u32, 
 _sig(_lib.rbr_render, [_vp, C.POINTER(RbrJob), C.c_int])
 _sig(_lib.rbr_render_moments, [_vp, C.POINTER(RbrJob), _vp, _vp])
 _sig(_lib.rbr_last_render_timing, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)])
+_sig(_lib.rbr_set_render_tuning, [_vp, C.c_uint32, C.c_int, C.c_int])
+
+
+def set_render_tuning(ctx: Context, pool_paths: int = 1 << 24, graphs: bool = True, sort: bool = True) -> None:
+    """Queue capacity, hipGraph replay of each round and material-ordered shading."""
+    ctx.check(_lib.rbr_set_render_tuning(ctx.handle, pool_paths, int(graphs), int(sort)))
 
 
 def _f3(x):

@@ -141,3 +141,23 @@ def test_trained_guide_reduces_variance(ctx):
     assert losses[-1] < losses[0]
     assert vg < vu
     assert vp / vg > 1.1
+
+
+def test_guided_graph_replay_matches_eager(ctx):
+    """Guided rounds (inference + guided shading + records) replayed as hipGraphs give the
+    eager images; training between replays is picked up (weights are read at run time)."""
+    say("case: guided graph replay")
+    sc = R.scene_from_desc(ctx, S.cornell_c0(32, 32))
+    lo, hi = sc.world_bounds()
+    g = R.Guide(ctx, lo, hi, seed=8)
+    job = dict(scene=sc, spp=4, depth=8, seed=9, guided=1)
+    try:
+        for rep in range(3):
+            R.set_render_tuning(ctx, 1 << 23, False, False)
+            eager = R.render(ctx, [job])[0]
+            R.set_render_tuning(ctx, 1 << 23, True, rep == 2)
+            replay = R.render(ctx, [job])[0]
+            assert np.array_equal(eager, replay), rep
+            train(ctx, sc, g, 1, 4, seed0=77 + rep)
+    finally:
+        R.set_render_tuning(ctx)   # defaults

@@ -117,3 +117,37 @@ def test_invalid_jobs(ctx):
     cam = S.cornell_c0(8, 8).camera
     with pytest.raises(R.RbrError):   # no shading tables yet
         R.render(ctx, [dict(scene=sc, spp=1, depth=1, camera=cam)])
+
+
+@pytest.mark.parametrize("pool,graphs,sort", [(1 << 16, False, False), (1 << 16, True, False), (1 << 16, True, True),
+                                              (1 << 23, True, True), (1 << 23, False, True)])
+def test_render_tuning_does_not_change_images(ctx, pool, graphs, sort):
+    """hipGraph replay and material-ordered shading leave every pixel bit-identical (the
+    random stream is keyed by path, not by queue slot); a smaller queue capacity splits a
+    job's samples over several rounds, which changes only the float summation order of the
+    per-pixel mean.  With graphs on, the rounds are captured on the first call and replayed on
+    the later ones; every replay is checked against the eager result, with eager work in
+    between."""
+    say(f"case: render tuning pool={pool} graphs={graphs} sort={sort}")
+    a = R.scene_from_desc(ctx, S.cornell_c0(40, 30))
+    b = R.scene_from_desc(ctx, S.materials_scene(24, 20))
+    f = R.scene_from_desc(ctx, S.fog_jobs_c3(scale=0.02)[9]["scene"])
+    jobs = [dict(scene=a, spp=60, depth=8, seed=3), dict(scene=b, spp=7, depth=6, seed=4),
+            dict(scene=f, spp=5, depth=5, seed=5, integrator=R.VOLPATH)]
+    R.set_render_tuning(ctx, 1 << 23, False, False)
+    big = R.render(ctx, jobs)
+    R.set_render_tuning(ctx, pool, False, False)
+    eager = R.render(ctx, jobs)
+    for k in range(len(jobs)):
+        np.testing.assert_allclose(eager[k], big[k], rtol=2e-6, atol=1e-7)
+    try:
+        R.set_render_tuning(ctx, pool, graphs, sort)
+        for rep in range(3):
+            out = R.render(ctx, jobs)
+            for k in range(len(jobs)):
+                assert np.array_equal(out[k], eager[k]), (rep, k)
+            R.render(ctx, [dict(scene=b, spp=1, depth=3, seed=rep)])   # other work between replays
+    finally:
+        R.set_render_tuning(ctx)   # defaults
+    with pytest.raises(R.RbrError):
+        R.set_render_tuning(ctx, 1000, True, False)
