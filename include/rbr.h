@@ -277,6 +277,12 @@ int rbr_guide_loss_grad(rbr_ctx* ctx, const float* samples, size_t n, double* lo
 /* One Adam step (beta 0.9 / 0.999) on the loss above; loss (optional) is the pre-step value. */
 int rbr_guide_train_step(rbr_ctx* ctx, const float* samples, size_t n, float lr, double* loss);
 
+/* Mean device time of one launch (HIP events on the context stream) over resident synthetic
+ * inputs: kind 0 = inference of n queries, kind 1 = one training step (forward, backward,
+ * weight gradients, Adam with a zero learning rate, so the weights do not change) on n
+ * samples.  For profiling the matrix-core kernels. */
+int rbr_guide_time(rbr_ctx* ctx, int kind, size_t n, int iters, float* mean_ms);
+
 /* Guided rendering: rbr_render jobs with config.guided = 1 draw each non-delta continuation
  * direction from 0.5 x guide + 0.5 x BSDF (one-sample MIS; NEE and its weights unchanged, so
  * the image converges to the plain integrator's); guided = 3 also records one training

@@ -483,6 +483,44 @@ __global__ __launch_bounds__(256) void guideSampleKernel(const uint4* __restrict
     pdfs[i] = guideMixPdf(r, w);
 }
 
+// Synthetic resident inputs for timing: kind 0 writes n queries (9 floats: position in [0,1]^3,
+// unit normal, unit wo); kind 1 writes n training samples in the packed 4 x float4 layout.
+__device__ __forceinline__ float synthU(uint32_t i, uint32_t d) {
+    uint64_t z = (uint64_t(i) << 8 | d) * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return float(uint32_t((z ^ (z >> 31)) >> 40)) * (1.0f / 16777216.0f);
+}
+__device__ __forceinline__ void synthDir(uint32_t i, uint32_t d, float* w) {
+    const float z = 1.0f - 2.0f * synthU(i, d), ph = 6.2831853f * synthU(i, d + 1);
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    w[0] = r * cosf(ph);
+    w[1] = r * sinf(ph);
+    w[2] = z;
+}
+__global__ __launch_bounds__(256) void guideSynthKernel(float* out, uint32_t n, int kind) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    float p[3] = {synthU(i, 0), synthU(i, 1), synthU(i, 2)}, nn[3], wo[3], d[3];
+    synthDir(i, 3, nn);
+    synthDir(i, 5, wo);
+    synthDir(i, 7, d);
+    if (kind == 0) {
+        float* q = out + size_t(i) * 9;
+        for (int k = 0; k < 3; ++k) {
+            q[k] = p[k];
+            q[3 + k] = nn[k];
+            q[6 + k] = wo[k];
+        }
+    } else {
+        float4* s = reinterpret_cast<float4*>(out) + size_t(i) * 4;
+        s[0] = make_float4(p[0], p[1], p[2], synthU(i, 9));
+        s[1] = make_float4(nn[0], nn[1], nn[2], 0.0f);
+        s[2] = make_float4(wo[0], wo[1], wo[2], 0.0f);
+        s[3] = make_float4(d[0], d[1], d[2], 0.0f);
+    }
+}
+
 __global__ void guideClearRecordsKernel(uint32_t* count, double* csum) {
     if (threadIdx.x == 0) {
         count[0] = 0u;
@@ -890,6 +928,43 @@ int rbr_guide_train_step(rbr_ctx* ctx, const float* samples, size_t n, float lr,
     if (int rc = uploadSamples(ctx, g, samples, n)) return rc;
     return guideTrainDevice(ctx, static_cast<const float4*>(g.dQ.ptr), 4, uint32_t(n), 0, 1.0f / float(n), INFINITY, lr,
                             loss);
+}
+
+int rbr_guide_time(rbr_ctx* ctx, int kind, size_t n, int iters, float* mean_ms) {
+    if (!ctx || !mean_ms || (kind != 0 && kind != 1) || n == 0 || n > (1u << 26) || iters < 1)
+        return ctx ? fail(ctx, RBR_ERR_INVALID, "rbr_guide_time arguments") : RBR_ERR_INVALID;
+    GuideState& g = state(ctx);
+    if (!g.ready) return fail(ctx, RBR_ERR_STATE, "guide not initialised");
+    if (int rc = useDevice(ctx)) return rc;
+    int rc;
+    if ((rc = ensureBuffer(ctx, g.dQ, n * (kind == 0 ? 36 : 64)))) return rc;
+    if (kind == 0 && (rc = ensureBuffer(ctx, g.dOut, n * kGuideRecBytes))) return rc;
+    hipLaunchKernelGGL(guideSynthKernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       static_cast<float*>(g.dQ.ptr), uint32_t(n), kind);
+    auto once = [&]() -> int {
+        if (kind == 0) {
+            GuideInferArgs a{};
+            a.g = guideDev(g);
+            a.queries = static_cast<const float*>(g.dQ.ptr);
+            a.n = uint32_t(n);
+            a.out = static_cast<uint4*>(g.dOut.ptr);
+            return launchGuideInfer(ctx, a, uint32_t(n));
+        }
+        // a full training step with a zero learning rate: every kernel runs, the weights stay
+        return guideTrainDevice(ctx, static_cast<const float4*>(g.dQ.ptr), 4, uint32_t(n), 0, 1.0f / float(n), INFINITY,
+                                0.0f, nullptr);
+    };
+    if ((rc = once())) return rc;
+    hipError_t e = hipEventRecord(ctx->ev0, ctx->stream);
+    for (int k = 0; k < iters && e == hipSuccess; ++k)
+        if ((rc = once())) return rc;
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    if (e != hipSuccess) return failHip(ctx, e, "guide timing");
+    *mean_ms = ms / float(iters);
+    return RBR_OK;
 }
 
 int rbr_guide_reserve(rbr_ctx* ctx, uint32_t capacity) {

@@ -469,6 +469,7 @@ _sig(_lib.rbr_guide_train_step, [_vp, _vp, _sz, C.c_float, C.POINTER(C.c_double)
 
 _sig(_lib.rbr_guide_sample, [_vp, _vp, _sz, _sz, _vp, _vp, _vp])
 _sig(_lib.rbr_guide_reserve, [_vp, C.c_uint32])
+_sig(_lib.rbr_guide_time, [_vp, C.c_int, _sz, C.c_int, C.POINTER(C.c_float)])
 _sig(_lib.rbr_guide_collected, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_double)])
 _sig(_lib.rbr_guide_train_collected, [_vp, C.c_int, C.c_uint32, C.c_float, C.POINTER(C.c_double)])
 _sig(_lib.rbr_guide_clear_collected, [_vp])
@@ -530,6 +531,13 @@ class Guide:
         loss = C.c_double(0)
         self.ctx.check(_lib.rbr_guide_train_step(self.ctx.handle, s.ctypes.data, len(s), lr, C.byref(loss)))
         return loss.value
+
+    def time(self, kind: int, n: int, iters: int = 10) -> float:
+        """Mean ms of one inference launch (kind 0) or training step (kind 1) on n resident
+        synthetic inputs (rbr_guide_time)."""
+        ms = C.c_float(0)
+        self.ctx.check(_lib.rbr_guide_time(self.ctx.handle, kind, n, iters, C.byref(ms)))
+        return ms.value
 
     def reserve(self, capacity: int) -> None:
         self.ctx.check(_lib.rbr_guide_reserve(self.ctx.handle, capacity))
