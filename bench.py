@@ -73,11 +73,27 @@ def cpu_baseline(tris, rays, gpu_hits, seconds: float) -> dict:
     }
 
 
+def layout_name(width: int) -> str:
+    """bvh4 / bvh8 (float child boxes) or bvh4q / bvh8q (quantized child boxes)."""
+    return f"bvh{width & 0xFF}" + ("q" if width & 0x100 else "")
+
+
+def tuned_layout() -> tuple[int, int, dict | None]:
+    """BVH width code, leaf size and launch tuning of the fastest closest-hit configuration in
+    the traversal tuning cache (autotune.py), else quantized BVH4 with the library defaults."""
+    sys.path.insert(0, os.path.join(ROOT, "ray-based-rendering_amd"))
+    import autotune
+    cfg = autotune.best_config("closest")
+    if not cfg:
+        return 4 | 0x100, 4, None
+    return int(cfg["width"]), int(cfg["leaf"]), cfg
+
+
 def measured_traffic(width: int):
     """HBM bytes per closest-hit launch from the committed rocprofv3 --pmc summary of this
     workload (FETCH_SIZE + WRITE_SIZE, scaled by the same counters' reading for a copy kernel
-    of known size), or None when no such summary exists for this BVH width."""
-    path = os.path.join(ROOT, "profiles", f"traversal_closest_c1_bvh{width}_pmc.json")
+    of known size), or None when no such summary exists for this BVH layout."""
+    path = os.path.join(ROOT, "profiles", f"traversal_closest_c1_{layout_name(width)}_pmc.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -89,8 +105,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--width", type=int, default=4)
-    ap.add_argument("--leaf", type=int, default=4)
+    ap.add_argument("--width", type=int, default=None,
+                    help="BVH width 4/8, +256 for quantized child boxes (default: the tuned layout)")
+    ap.add_argument("--leaf", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
@@ -111,6 +128,14 @@ def main() -> int:
 
     ctx = R.Context(0)
     dev_name, cus = ctx.device_info()
+    width, leaf, tuning = tuned_layout()
+    if args.width is not None:
+        width, tuning = args.width, None
+    if args.leaf is not None:
+        leaf = args.leaf
+    args.width, args.leaf = width, leaf
+    if tuning:
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, tuning["mode"], tuning["lds_depth"], tuning["waves_per_cu"]))
     sc = R.Scene(ctx)
     sc.set_geometry(tris=tris)
     log("bench: building BVH")
@@ -152,7 +177,7 @@ def main() -> int:
     # Algorithmic bytes per ray from the host traversal of the same layout on a sample.
     sample = rays[:: max(1, N_RAYS // (1 << 16))]
     _, nv, pt = sc.host_trace_closest(sample)
-    node_b = 128 if args.width == 4 else 256
+    node_b = st["node_bytes"] / max(1, st["node_count"])   # 64/128 B (quantized) or 128/256 B
     nodes_per_ray, prims_per_ray = nv / len(sample), pt / len(sample)
     bytes_compulsory = 32 + 8
     bytes_per_ray = bytes_compulsory + nodes_per_ray * node_b + prims_per_ray * 48
@@ -175,7 +200,10 @@ def main() -> int:
         "config": {
             "workload": "closest-hit, icosphere 20*224^2 = %d tris r=100 displaced 0.1r, 2^22 rays origins in bbox*1.1, "
                         "dirs uniform on S^2, tmin 1e-4, tmax inf" % len(tris),
-            "bvh": f"BVH{args.width}, leaf<={args.leaf}", "nodes": st["node_count"], "max_depth": st["max_depth"],
+            "bvh": f"{layout_name(args.width).upper()}, leaf<={args.leaf}, {node_b:.0f} B nodes", "nodes": st["node_count"],
+            "max_depth": st["max_depth"],
+            "launch": ({k: tuning[k] for k in ("mode", "lds_depth", "waves_per_cu")} if tuning
+                       else "library defaults (persistent, LDS 8, 32 waves/CU)"),
             "seed": SEED,
         },
         "wall_s_timed_region": wall_s,

@@ -53,7 +53,7 @@ typedef struct rbr_hit {
 } rbr_hit;
 
 typedef struct rbr_bvh_stats {
-    uint32_t width;        /* 4 or 8                                     */
+    uint32_t width;        /* 4 or 8, | RBR_BVH_QUANTIZED                */
     uint32_t node_count;   /* wide nodes                                  */
     uint32_t leaf_count;   /* leaf references                              */
     uint32_t prim_count;   /* bounded primitives in the tree               */
@@ -181,8 +181,14 @@ int rbr_scene_set_geometry(rbr_scene* scene,
                            const float* planes /* 4*n: normal d */, uint32_t nplane, const uint32_t* plane_ids);
 
 /* Builds the wide BVH (width 4 or 8) over the bounded primitives, leaves of at most
- * leaf_max primitives (1..8), and uploads it. */
+ * leaf_max primitives (1..8), and uploads it.  width | RBR_BVH_QUANTIZED stores the device
+ * nodes with 8-bit child boxes relative to each node (64 B per 4-wide node, 128 B per 8-wide
+ * node instead of 128 / 256 B); the boxes are rounded outward, so hits are unchanged. */
+#define RBR_BVH_QUANTIZED 0x100
 int rbr_scene_build_bvh(rbr_scene* scene, int width, int leaf_max);
+/* Copies the node array as the kernels read it (layout of bvh.h; up to capacity bytes) and
+ * reports its size: for tooling and layout tests.  Works on host-only contexts. */
+int rbr_scene_device_nodes(const rbr_scene* scene, void* dst, size_t capacity, size_t* bytes);
 int rbr_scene_bvh_stats(const rbr_scene* scene, rbr_bvh_stats* out);
 /* Accelerator::worldBounds (accelerator.h:34): empty box (lo=+FLT_MAX, hi=-FLT_MAX) when
  * nothing bounded was built. */

@@ -23,9 +23,19 @@
 // Child boxes are grown outward by a margin (scene extent * 1e-5 plus 1e-6 relative per
 // coordinate) so that the approximate slab test can never cull a primitive whose exact
 // Moller-Trumbore test accepts the ray.
+//
+// Quantized nodes (device layout when built with kQuantFlag; W = 4: 64 B, W = 8: 128 B):
+//   dword 0-2  node origin p (the children's union box corner, float)
+//   dword 3    biased exponents ex | ey << 8 | ez << 16: per-axis scale s = 2^(e - 127)
+//   then for axis x, y, z: W bytes of q_lo (child k in byte k), then W bytes of q_hi
+//   then W child refs (same encodings as above), zero padding to the node size
+// The decoded box lo = fma(q_lo, s, p), hi = fma(q_hi, s, p) (one rounding, as the device
+// evaluates it) contains the padded float box of the child; empty children have q_lo = 255,
+// q_hi = 0 and ref kEmpty.  The host keeps the float nodes for its own traversal.
 #pragma once
 
 #include <cstdint>
+#include <cstddef>
 #include <vector>
 
 namespace rbr {
@@ -35,6 +45,10 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kCountShift = 27;
 constexpr uint32_t kStartMask = (1u << 27) - 1u;
 constexpr uint32_t kMaxLeaf = 8;
+constexpr int kQuantFlag = 0x100;   // width code flag: quantized child boxes
+
+inline int qNodeDwords(int W) { return W == 4 ? 16 : 32; }
+inline int qRefOffset(int W) { return 4 + 6 * (W / 4); }
 
 enum PrimKind : uint32_t { KIND_TRI = 0, KIND_SPHERE = 1, KIND_QUAD = 2 };
 
@@ -63,7 +77,9 @@ struct HostGeometry {
 
 struct WideBVH {
     int width = 4;
+    bool quantized = false;
     std::vector<F4> nodes;        // width==4: 8 F4 per node; width==8: 16 F4 per node
+    std::vector<uint32_t> qnodes; // quantized device nodes (qNodeDwords(width) per node)
     std::vector<F4> prims;        // 3 F4 per record
     std::vector<F4> quadNormals;
     uint32_t nodeCount = 0, leafCount = 0, primCount = 0, maxDepth = 0;
@@ -71,7 +87,16 @@ struct WideBVH {
     float worldLo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, worldHi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
     bool empty() const { return primCount == 0; }
     int f4PerNode() const { return width == 4 ? 8 : 16; }
+    // What the kernels are given: width code, dwords per node, child-ref dword offset.
+    int code() const { return width | (quantized ? kQuantFlag : 0); }
+    int deviceNodeDwords() const { return quantized ? qNodeDwords(width) : 4 * f4PerNode(); }
+    int deviceRefOffset() const { return quantized ? qRefOffset(width) : 6 * width; }
+    const void* deviceNodes() const { return quantized ? static_cast<const void*>(qnodes.data()) : nodes.data(); }
+    size_t deviceNodeBytes() const { return quantized ? qnodes.size() * 4 : nodes.size() * sizeof(F4); }
 };
+
+// Fills b.qnodes from the float nodes (conservative 8-bit child boxes) and sets b.quantized.
+void quantizeWideBVH(WideBVH& b);
 
 // Builds the binary binned-SAH tree and collapses it to `width` (4 or 8); leaves hold at
 // most leaf_max (1..8) primitives.

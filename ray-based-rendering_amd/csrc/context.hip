@@ -311,17 +311,20 @@ int rbr_scene_set_geometry(rbr_scene* s, const float* tris, uint32_t ntri, const
 int rbr_scene_build_bvh(rbr_scene* s, int width, int leaf_max) {
     if (!s) return RBR_ERR_INVALID;
     rbr_ctx* ctx = s->ctx;
-    if (width != 4 && width != 8) return fail(ctx, RBR_ERR_INVALID, "BVH width must be 4 or 8");
+    const int W = width & ~kQuantFlag;
+    if ((W != 4 && W != 8) || (width & ~(kQuantFlag | 0xFF)))
+        return fail(ctx, RBR_ERR_INVALID, "BVH width must be 4 or 8, optionally | RBR_BVH_QUANTIZED");
     if (leaf_max < 1 || leaf_max > 8) return fail(ctx, RBR_ERR_INVALID, "leaf_max must be in 1..8");
     s->built = false;
     bumpVersion(s);
-    buildWideBVH(s->geom, width, leaf_max, s->bvh);
-    if (!s->bvh.empty() && uint64_t(width - 1) * s->bvh.maxDepth + 1 > uint64_t(kMaxStack))
+    buildWideBVH(s->geom, W, leaf_max, s->bvh);
+    if (width & kQuantFlag) quantizeWideBVH(s->bvh);
+    if (!s->bvh.empty() && uint64_t(W - 1) * s->bvh.maxDepth + 1 > uint64_t(kMaxStack))
         return fail(ctx, RBR_ERR_UNSUPPORTED, "BVH too deep for the traversal stack");
     if (ctx->device < 0) { s->built = true; return RBR_OK; }
     if (int rc = useDevice(ctx)) return rc;
     int rc;
-    if ((rc = uploadBuffer(ctx, s->dNodes, s->bvh.nodes.data(), s->bvh.nodes.size() * sizeof(F4)))) return rc;
+    if ((rc = uploadBuffer(ctx, s->dNodes, s->bvh.deviceNodes(), s->bvh.deviceNodeBytes()))) return rc;
     if ((rc = uploadBuffer(ctx, s->dPrims, s->bvh.prims.data(), s->bvh.prims.size() * sizeof(F4)))) return rc;
     if ((rc = uploadBuffer(ctx, s->dQuadN, s->bvh.quadNormals.data(), s->bvh.quadNormals.size() * sizeof(F4)))) return rc;
     if ((rc = uploadBuffer(ctx, s->dPlanes, s->geom.planes.data(), s->geom.planes.size() * sizeof(float)))) return rc;
@@ -330,16 +333,25 @@ int rbr_scene_build_bvh(rbr_scene* s, int width, int leaf_max) {
     return RBR_OK;
 }
 
+int rbr_scene_device_nodes(const rbr_scene* s, void* dst, size_t capacity, size_t* bytes) {
+    if (!s) return RBR_ERR_INVALID;
+    if (!s->built) return fail(s->ctx, RBR_ERR_STATE, "BVH not built");
+    const size_t n = s->bvh.deviceNodeBytes();
+    if (bytes) *bytes = n;
+    if (dst && capacity) std::memcpy(dst, s->bvh.deviceNodes(), std::min(capacity, n));
+    return RBR_OK;
+}
+
 int rbr_scene_bvh_stats(const rbr_scene* s, rbr_bvh_stats* out) {
     if (!s || !out) return RBR_ERR_INVALID;
     if (!s->built) return fail(s->ctx, RBR_ERR_STATE, "BVH not built");
-    out->width = uint32_t(s->bvh.width);
+    out->width = uint32_t(s->bvh.code());
     out->node_count = s->bvh.nodeCount;
     out->leaf_count = s->bvh.leafCount;
     out->prim_count = s->bvh.primCount;
     out->unbounded = uint32_t(s->geom.planeIds.size());
     out->max_depth = s->bvh.maxDepth;
-    out->node_bytes = uint64_t(s->bvh.nodes.size()) * sizeof(F4);
+    out->node_bytes = uint64_t(s->bvh.deviceNodeBytes());
     out->prim_bytes = uint64_t(s->bvh.prims.size()) * sizeof(F4);
     out->sah_cost = s->bvh.sahCost;
     out->build_ms = s->bvh.buildMs;
@@ -368,7 +380,7 @@ static int traceDevice(rbr_scene* s, const rbr_ray* dRays, size_t n, void* dOut,
     a.hits = any ? nullptr : static_cast<uint2*>(dOut);
     a.occ = any ? static_cast<uint8_t*>(dOut) : nullptr;
     const uint32_t maxStack = s->bvh.empty() ? 1u : uint32_t(s->bvh.width - 1) * s->bvh.maxDepth + 1u;
-    return launchTraversal(ctx, a, s->bvh.width, maxStack, any);
+    return launchTraversal(ctx, a, s->bvh.code(), maxStack, any);
 }
 
 static int traceHost(rbr_scene* s, const rbr_ray* rays, size_t n, void* out, bool any) {

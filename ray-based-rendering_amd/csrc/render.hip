@@ -1303,8 +1303,8 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         }
         const rbr_scene* s = jb.scene;
         if (!s->bvh.empty()) {
-            if (width && width != s->bvh.width) return fail(ctx, RBR_ERR_UNSUPPORTED, "jobs mix BVH widths");
-            width = s->bvh.width;
+            if (width && width != s->bvh.code()) return fail(ctx, RBR_ERR_UNSUPPORTED, "jobs mix BVH layouts");
+            width = s->bvh.code();
             maxStack = std::max(maxStack, uint32_t(s->bvh.width - 1) * s->bvh.maxDepth + 1u);
         }
         anyMixed |= (s->geom.sphIds.size() + s->geom.quadIds.size()) > 0;
@@ -1316,7 +1316,9 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     if (anyPlanes && scenes.size() > 1) return fail(ctx, RBR_ERR_UNSUPPORTED, "infinite planes in a multi-scene render");
     if (!ctx->render) ctx->render = new RenderState();
     RenderState& R = *ctx->render;
-    const int per = width == 4 ? 8 : 16;
+    // node layout of the launch (all scenes share it): dwords per node, child-ref offset
+    const int nodeDw = (width & kQuantFlag) ? qNodeDwords(width & 0xFF) : 4 * ((width & 0xFF) == 4 ? 8 : 16);
+    const int refOff = (width & kQuantFlag) ? qRefOffset(width & 0xFF) : 6 * (width & 0xFF);
 
     // ---- node/primitive arena (packed only when several scenes share the launch); reused
     // while the same scenes, unchanged, are rendered again
@@ -1358,13 +1360,14 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     } else if (scenes.size() == 1) {
         ta = sceneTraceArgs(scenes[0].s);
     } else {
-        std::vector<F4> nodes(size_t(nodeOff) * per), prims(size_t(primOff) * 3), quadN(quadOff);
+        std::vector<uint32_t> nodes(size_t(nodeOff) * nodeDw);
+        std::vector<F4> prims(size_t(primOff) * 3), quadN(quadOff);
         for (const SceneSlot& sl : scenes) {
             const WideBVH& b = sl.s->bvh;
-            std::memcpy(&nodes[size_t(sl.nodeOff) * per], b.nodes.data(), b.nodes.size() * sizeof(F4));
+            std::memcpy(&nodes[size_t(sl.nodeOff) * nodeDw], b.deviceNodes(), b.deviceNodeBytes());
             for (uint32_t nd = 0; nd < b.nodeCount; ++nd) {
-                uint32_t* ch = reinterpret_cast<uint32_t*>(&nodes[size_t(sl.nodeOff + nd) * per]) + 6 * width;
-                for (int k = 0; k < width; ++k) {
+                uint32_t* ch = &nodes[size_t(sl.nodeOff + nd) * nodeDw] + refOff;
+                for (int k = 0; k < (width & 0xFF); ++k) {
                     if (ch[k] == kEmpty) continue;
                     if (isLeaf(ch[k])) ch[k] = makeLeaf(leafStart(ch[k]) + sl.primOff, leafCount(ch[k]));
                     else ch[k] += sl.nodeOff;
@@ -1387,7 +1390,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
             }
             std::copy(b.quadNormals.begin(), b.quadNormals.end(), quadN.begin() + sl.quadOff);
         }
-        if ((rc = uploadBuffer(ctx, R.arenaNodes, nodes.data(), nodes.size() * sizeof(F4)))) return rc;
+        if ((rc = uploadBuffer(ctx, R.arenaNodes, nodes.data(), nodes.size() * sizeof(uint32_t)))) return rc;
         if ((rc = uploadBuffer(ctx, R.arenaPrims, prims.data(), prims.size() * sizeof(F4)))) return rc;
         if ((rc = uploadBuffer(ctx, R.arenaQuadN, quadN.data(), quadN.size() * sizeof(F4)))) return rc;
         ta.nodes = nodeOff ? static_cast<const float4*>(R.arenaNodes.ptr) : nullptr;

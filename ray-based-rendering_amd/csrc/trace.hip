@@ -77,6 +77,48 @@ __device__ __forceinline__ float slab(const DRay& r, float lx, float hx, float l
     return t0;
 }
 
+// Children of a quantized node (bvh.h): the ray is moved into the node's grid once,
+// t(q) = q * (s * 1/d) + (p - o) * 1/d, so each of the 6 planes of a child costs one FMA on
+// its byte.  Fills entry distances (tc) and refs (rc) like the float path.
+template <int W>
+__device__ __forceinline__ void childrenQ(const float4* nodes, uint32_t ref, const DRay& r, float tfar, float* tc,
+                                          uint32_t* rc) {
+    constexpr int Q4 = W / 4;
+    constexpr int NV = W == 4 ? 4 : 6;   // uint4 rows that hold data (64 B / 96 B)
+    const uint4* nd = reinterpret_cast<const uint4*>(nodes) + size_t(ref) * (W == 4 ? 4 : 8);
+    uint32_t w[4 * NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const uint4 v = nd[i];
+        w[4 * i] = v.x;
+        w[4 * i + 1] = v.y;
+        w[4 * i + 2] = v.z;
+        w[4 * i + 3] = v.w;
+    }
+    const float ax = u2f((w[3] & 0xFFu) << 23) * r.ix;
+    const float ay = u2f(((w[3] >> 8) & 0xFFu) << 23) * r.iy;
+    const float az = u2f(((w[3] >> 16) & 0xFFu) << 23) * r.iz;
+    const float bx = __builtin_fmaf(u2f(w[0]), r.ix, -r.oix);
+    const float by = __builtin_fmaf(u2f(w[1]), r.iy, -r.oiy);
+    const float bz = __builtin_fmaf(u2f(w[2]), r.iz, -r.oiz);
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int wo = k >> 2, sh = 8 * (k & 3);
+        const float lx = float((w[4 + 0 * Q4 + wo] >> sh) & 0xFFu), hx = float((w[4 + 1 * Q4 + wo] >> sh) & 0xFFu);
+        const float ly = float((w[4 + 2 * Q4 + wo] >> sh) & 0xFFu), hy = float((w[4 + 3 * Q4 + wo] >> sh) & 0xFFu);
+        const float lz = float((w[4 + 4 * Q4 + wo] >> sh) & 0xFFu), hz = float((w[4 + 5 * Q4 + wo] >> sh) & 0xFFu);
+        const float t0x = __builtin_fmaf(r.nx ? hx : lx, ax, bx), t1x = __builtin_fmaf(r.nx ? lx : hx, ax, bx);
+        const float t0y = __builtin_fmaf(r.ny ? hy : ly, ay, by), t1y = __builtin_fmaf(r.ny ? ly : hy, ay, by);
+        const float t0z = __builtin_fmaf(r.nz ? hz : lz, az, bz), t1z = __builtin_fmaf(r.nz ? lz : hz, az, bz);
+        const float t0 = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, 0.0f));
+        const float t1 = fminf(fminf(t1x, t1y), fminf(t1z, tfar));
+        const uint32_t ch = w[4 + 6 * Q4 + k];
+        const bool h = t0 <= t1 && ch != kEmptyD;
+        tc[k] = h ? t0 : INFINITY;
+        rc[k] = h ? ch : kEmptyD;
+    }
+}
+
 template <bool MIXED>
 __device__ __forceinline__ bool testPrim(const TraceArgs& a, const float4* rec, const DRay& r, float tmax, float& t,
                                          uint32_t& id) {
@@ -142,7 +184,7 @@ struct PrivateSpillStack {
 
 // One ray through the wide BVH and the unbounded list.  Returns the lowest-id closest hit
 // (bestT/bestId) or, for ANY, whether any primitive is hit in [tmin, tmax].
-template <int W, bool ANY, bool MIXED, typename Stack>
+template <int W, bool Q, bool ANY, bool MIXED, typename Stack>
 __device__ __forceinline__ void traceRay(const TraceArgs& a, const DRay& r, uint32_t root, Stack& st, float& bestT,
                                          uint32_t& bestId, bool& occluded) {
     bestT = r.tmax;
@@ -155,8 +197,9 @@ __device__ __forceinline__ void traceRay(const TraceArgs& a, const DRay& r, uint
                 const float4* nd = a.nodes + size_t(ref) * (W == 4 ? 8 : 16);
                 float tc[W];
                 uint32_t rc[W];
+                if (Q) childrenQ<W>(a.nodes, ref, r, bestT, tc, rc);
 #pragma unroll
-                for (int half = 0; half < W / 4; ++half) {
+                for (int half = 0; half < (Q ? 0 : W / 4); ++half) {
                     const int q = W / 4;
                     float4 lx = nd[0 * q + half], hx = nd[1 * q + half], ly = nd[2 * q + half];
                     float4 hy = nd[3 * q + half], lz = nd[4 * q + half], hz = nd[5 * q + half];
@@ -179,10 +222,13 @@ __device__ __forceinline__ void traceRay(const TraceArgs& a, const DRay& r, uint
                         cswap(tc[0], rc[0], tc[2], rc[2]); cswap(tc[1], rc[1], tc[3], rc[3]);
                         cswap(tc[1], rc[1], tc[2], rc[2]);
                     } else {
+                        // Optimal 8-input network (19 comparators, depth 6).
+                        constexpr int net[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6},
+                                                    {3, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}, {2, 4}, {3, 5},
+                                                    {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
 #pragma unroll
-                        for (int p = 0; p < W; ++p)
-#pragma unroll
-                            for (int k = (p & 1); k + 1 < W; k += 2) cswap(tc[k], rc[k], tc[k + 1], rc[k + 1]);
+                        for (int c = 0; c < 19; ++c)
+                            cswap(tc[net[c][0]], rc[net[c][0]], tc[net[c][1]], rc[net[c][1]]);
                     }
                 }
                 int cnt = 0;
@@ -250,7 +296,7 @@ __device__ __forceinline__ void storeResult(const TraceArgs& a, uint32_t i, floa
 // slot and every resident wave stays busy to the end.  The counter is zeroed by
 // resetCounterKernel right before the launch on the same stream; the ray count can come
 // from device memory (queue lengths written by an earlier kernel).
-template <int W, int LDS, bool ANY, bool MIXED>
+template <int W, bool Q, int LDS, bool ANY, bool MIXED>
 __global__ __launch_bounds__(64) void traceKernelPersistent(TraceArgs a, PersistArgs p) {
     __shared__ uint2 ldsStack[LDS * 64];
     __shared__ uint32_t batch;
@@ -271,7 +317,7 @@ __global__ __launch_bounds__(64) void traceKernelPersistent(TraceArgs a, Persist
             uint32_t bestId;
             bool occ;
             st.sp = 0;
-            traceRay<W, ANY, MIXED>(a, r, a.rootOf ? a.rootOf[i] : 0u, st, bestT, bestId, occ);
+            traceRay<W, Q, ANY, MIXED>(a, r, a.rootOf ? a.rootOf[i] : 0u, st, bestT, bestId, occ);
             storeResult<ANY>(a, i, bestT, bestId, occ);
         }
     }
@@ -281,7 +327,7 @@ __global__ void resetCounterKernel(uint32_t* c) { *c = 0u; }
 
 // One ray per lane, stack overflow in private memory (kept for comparison and for callers
 // that cannot provide the spill buffer).
-template <int W, int LDS, bool ANY, bool MIXED>
+template <int W, bool Q, int LDS, bool ANY, bool MIXED>
 __global__ __launch_bounds__(64) void traceKernelLane(TraceArgs a) {
     __shared__ uint2 ldsStack[LDS * 64];
     const uint32_t i = blockIdx.x * 64 + threadIdx.x;
@@ -292,32 +338,39 @@ __global__ __launch_bounds__(64) void traceKernelLane(TraceArgs a) {
     float bestT;
     uint32_t bestId;
     bool occ;
-    traceRay<W, ANY, MIXED>(a, r, a.rootOf ? a.rootOf[i] : 0u, st, bestT, bestId, occ);
+    traceRay<W, Q, ANY, MIXED>(a, r, a.rootOf ? a.rootOf[i] : 0u, st, bestT, bestId, occ);
     storeResult<ANY>(a, i, bestT, bestId, occ);
 }
 
-template <int W, int LDS, bool ANY>
+template <int W, bool Q, int LDS, bool ANY>
 hipError_t launchT(const TraceArgs& a, const PersistArgs* p, unsigned grid, hipStream_t s) {
     if (p) {
         hipLaunchKernelGGL(resetCounterKernel, dim3(1), dim3(1), 0, s, p->counter);
-        if (a.mixed) hipLaunchKernelGGL((traceKernelPersistent<W, LDS, ANY, true>), dim3(grid), dim3(64), 0, s, a, *p);
-        else hipLaunchKernelGGL((traceKernelPersistent<W, LDS, ANY, false>), dim3(grid), dim3(64), 0, s, a, *p);
+        if (a.mixed) hipLaunchKernelGGL((traceKernelPersistent<W, Q, LDS, ANY, true>), dim3(grid), dim3(64), 0, s, a, *p);
+        else hipLaunchKernelGGL((traceKernelPersistent<W, Q, LDS, ANY, false>), dim3(grid), dim3(64), 0, s, a, *p);
     } else {
         dim3 g((a.n + 63) / 64);
-        if (a.mixed) hipLaunchKernelGGL((traceKernelLane<W, LDS, ANY, true>), g, dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((traceKernelLane<W, LDS, ANY, false>), g, dim3(64), 0, s, a);
+        if (a.mixed) hipLaunchKernelGGL((traceKernelLane<W, Q, LDS, ANY, true>), g, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((traceKernelLane<W, Q, LDS, ANY, false>), g, dim3(64), 0, s, a);
     }
     return hipGetLastError();
 }
 
-template <int W, int LDS>
+template <int W, bool Q, int LDS>
 hipError_t launchW(const TraceArgs& a, const PersistArgs* p, unsigned grid, bool any, hipStream_t s) {
-    return any ? launchT<W, LDS, true>(a, p, grid, s) : launchT<W, LDS, false>(a, p, grid, s);
+    return any ? launchT<W, Q, LDS, true>(a, p, grid, s) : launchT<W, Q, LDS, false>(a, p, grid, s);
 }
 
+// width: 4 or 8, | kQuantFlag for quantized nodes
 template <int LDS>
 hipError_t launchL(const TraceArgs& a, const PersistArgs* p, unsigned grid, int width, bool any, hipStream_t s) {
-    return width == 8 ? launchW<8, LDS>(a, p, grid, any, s) : launchW<4, LDS>(a, p, grid, any, s);
+    switch (width) {
+        case 4: return launchW<4, false, LDS>(a, p, grid, any, s);
+        case 8: return launchW<8, false, LDS>(a, p, grid, any, s);
+        case 4 | kQuantFlag: return launchW<4, true, LDS>(a, p, grid, any, s);
+        case 8 | kQuantFlag: return launchW<8, true, LDS>(a, p, grid, any, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace

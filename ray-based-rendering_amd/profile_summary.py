@@ -40,7 +40,7 @@ def main() -> int:
     ap.add_argument("--l2")
     ap.add_argument("--name", required=True)
     ap.add_argument("--command", required=True)
-    ap.add_argument("--kernel", default="traceKernelPersistent<4, 8, false")
+    ap.add_argument("--kernel", default="traceKernelPersistent<4, true, 8, false")
     ap.add_argument("--copy-bytes", type=float, default=256 * 1024 * 1024)
     args = ap.parse_args()
     rows = list(csv.DictReader(open(args.stats)))

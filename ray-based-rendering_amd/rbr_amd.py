@@ -75,6 +75,8 @@ _sig(_lib.rbr_scene_destroy, [_vp], None)
 _sig(_lib.rbr_scene_set_geometry, [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp])
 _sig(_lib.rbr_scene_build_bvh, [_vp, C.c_int, C.c_int])
 _sig(_lib.rbr_scene_bvh_stats, [_vp, C.POINTER(BvhStats)])
+_sig(_lib.rbr_scene_device_nodes, [_vp, _vp, _sz, C.POINTER(_sz)])
+BVH_QUANTIZED = 0x100
 _sig(_lib.rbr_scene_world_bounds, [_vp, _vp, _vp])
 _sig(_lib.rbr_trace_closest, [_vp, _vp, _sz, _vp])
 _sig(_lib.rbr_trace_any, [_vp, _vp, _sz, _vp])
@@ -234,6 +236,14 @@ class Scene:
 
     def build_bvh(self, width: int = 4, leaf_max: int = 4) -> None:
         self.ctx.check(_lib.rbr_scene_build_bvh(self._h, width, leaf_max))
+
+    def device_nodes(self) -> np.ndarray:
+        """The node array as the kernels read it, as uint32 words."""
+        n = _sz(0)
+        self.ctx.check(_lib.rbr_scene_device_nodes(self._h, None, 0, C.byref(n)))
+        out = np.zeros(n.value // 4, np.uint32)
+        self.ctx.check(_lib.rbr_scene_device_nodes(self._h, out.ctypes.data, n.value, C.byref(n)))
+        return out
 
     def stats(self) -> dict:
         st = BvhStats()
@@ -407,8 +417,9 @@ def _job_config(j: dict) -> RbrRenderConfig:
                     j.get("mis", True), j.get("sample_offset", 0), j.get("guided", 0), j.get("guide_depth", 0))
 
 
-def scene_from_desc(ctx: Context, desc, width: int = 4, leaf: int = 4) -> Scene:
-    """Uploads a scenes.SceneDesc: geometry, wide BVH and shading tables."""
+def scene_from_desc(ctx: Context, desc, width: int = 4 | 0x100, leaf: int = 4) -> Scene:
+    """Uploads a scenes.SceneDesc: geometry, wide BVH (quantized BVH4 by default) and
+    shading tables."""
     sc = Scene(ctx)
     sc.set_geometry(tris=desc.tris if len(desc.tris) else None, spheres=desc.spheres if len(desc.spheres) else None,
                     quads=desc.quads if len(desc.quads) else None, planes=desc.planes if len(desc.planes) else None)

@@ -26,6 +26,10 @@ def ctx():
     c.close()
 
 
+# BVH4 / BVH8 with float boxes and with quantized 8-bit child boxes
+WIDTHS = [4, 8, 4 | R.BVH_QUANTIZED, 8 | R.BVH_QUANTIZED]
+
+
 def gpu_scene(ctx, width=4, leaf=4, **kw):
     sc = R.Scene(ctx)
     sc.set_geometry(**kw)
@@ -50,7 +54,7 @@ def test_empty_and_single_sphere(ctx):
     assert sc.trace_any(r)[0] == 1
 
 
-@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("width", WIDTHS)
 def test_golden_fixture(ctx, width):
     say(f"case: golden fixture width {width}")
     rays = GOLDEN["rays"].view(worlds.RAY_DTYPE).reshape(-1)
@@ -69,7 +73,7 @@ def test_golden_fixture(ctx, width):
     assert np.array_equal(mh["t"], np.where(GOLDEN["mesh_ref_id"] == R.MISS, np.float32(0), GOLDEN["mesh_ref_t"]))
 
 
-@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("width", WIDTHS)
 @pytest.mark.parametrize("cfg", [(30, 30, False, 3000), (400, 1500, False, 6000), (60, 200, True, 3000)])
 def test_reference_worlds(ctx, width, cfg):
     ns, nt, plane, nr = cfg
@@ -103,7 +107,7 @@ def test_leaf_sizes_and_ties(ctx):
     ids = np.array([40, 12, 33, 7, 90, 8, 11, 70, 9], np.uint32)
     r = R.rays_array([[0.2, 0.2, 1.0]], [[0, 0, -1]])
     for leaf in range(1, 9):
-        for width in (4, 8):
+        for width in WIDTHS:
             h = gpu_scene(ctx, width, leaf, tris=tris, tri_ids=ids).trace_closest(r)
             assert h["id"][0] == 7 and h["t"][0] == np.float32(1.0)
 
@@ -125,7 +129,7 @@ def test_device_pointer_path_matches_host_path(ctx):
     d_hits.free()
 
 
-@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("width", WIDTHS)
 def test_c1_full_size(ctx, width):
     """~1.0M-triangle displaced icosphere (20*224^2) and 2^22 rays: bit-exact against the
     oracle BVH (itself equal to brute force), plus the 2^22 shadow batch."""
@@ -137,12 +141,18 @@ def test_c1_full_size(ctx, width):
     sc = gpu_scene(ctx, width, 4, tris=tris)
     h = sc.trace_closest(rays)
     occ = sc.trace_any(shadow)
-    say("  gpu done; oracle")
-    o = O.OracleScene(tris=tris)
-    o.build_bvh()
-    assert_same(h, o.closest(rays))
-    assert np.array_equal(occ, o.any(shadow))
+    if "c1" not in _ORACLE:   # the oracle's answer does not depend on the GPU layout
+        say("  gpu done; oracle")
+        o = O.OracleScene(tris=tris)
+        o.build_bvh()
+        _ORACLE["c1"] = (o.closest(rays), o.any(shadow))
+    ref_h, ref_occ = _ORACLE["c1"]
+    assert_same(h, ref_h)
+    assert np.array_equal(occ, ref_occ)
     assert 0.3 < (h["id"] != R.MISS).mean() < 1.0
+
+
+_ORACLE: dict = {}
 
 
 @pytest.mark.parametrize("mode,depth,waves", [(0, 8, 1), (0, 12, 32), (0, 32, 16), (1, 8, 16), (1, 24, 16)])
@@ -158,7 +168,7 @@ def test_launch_tunings_identical(ctx, mode, depth, waves):
     ref = o.closest(rays)
     ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, depth, waves))
     try:
-        for width in (4, 8):
+        for width in WIDTHS:
             sc = gpu_scene(ctx, width, 4, tris=tris)
             assert_same(sc.trace_closest(rays), ref)
             assert np.array_equal(sc.trace_any(rays), o.any(rays))

@@ -113,3 +113,73 @@ def test_icosphere_is_watertight_and_seeded():
     assert set(keys.values()) == {2}
     r = np.linalg.norm(v.reshape(-1, 3), axis=1)
     assert r.min() > 89.9 and r.max() < 110.1
+
+
+def _fma32(a, b, c):
+    """float32 fma with one rounding (exact in float64 for these operands: an 8-bit integer
+    times a power of two plus a float32 is exact in the 53-bit mantissa unless the exponents
+    are far apart, in which case float64 rounding to nearest then float32 rounding could
+    double round; the check below therefore uses directed containment with one ulp slack
+    only in that case)."""
+    return np.float32(np.float64(a) * np.float64(b) + np.float64(c))
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_quantized_nodes_contain_float_boxes(width):
+    """Every child box of a quantized node, decoded as lo = fma(q_lo, 2^(e-127), p),
+    contains the child's padded float box, and the child refs are the float node's."""
+    ctx = R.Context(-1)
+    rng = np.random.default_rng(width)
+    tris = R.icosphere(24, 100.0, 0.1, 7)
+    sph = np.concatenate([rng.uniform(-120, 120, (40, 3)), rng.uniform(0.01, 5, (40, 1))], 1).astype(np.float32)
+    fl = R.Scene(ctx)
+    fl.set_geometry(tris=tris, spheres=sph)
+    fl.build_bvh(width, 4)
+    qs = R.Scene(ctx)
+    qs.set_geometry(tris=tris, spheres=sph)
+    qs.build_bvh(width | R.BVH_QUANTIZED, 4)
+    assert qs.stats()["width"] == width | R.BVH_QUANTIZED
+    F = fl.device_nodes().view(np.float32).reshape(-1, 32 if width == 4 else 64)
+    Fu = F.view(np.uint32)
+    dw = 16 if width == 4 else 32
+    Q = qs.device_nodes().reshape(-1, dw)
+    assert len(Q) == len(F) and qs.stats()["node_bytes"] * 2 == fl.stats()["node_bytes"]
+    q4 = width // 4
+    checked = 0
+    for n in range(len(Q)):
+        q = Q[n]
+        p = q[:3].view(np.float32)
+        e = [(int(q[3]) >> (8 * a)) & 0xFF for a in range(3)]
+        s = [np.float32(2.0 ** (ea - 127)) for ea in e]
+        for k in range(width):
+            ref_f = Fu[n, 6 * width + k]
+            ref_q = q[4 + 6 * q4 + k]
+            assert ref_f == ref_q
+            if ref_f == 0xFFFFFFFF:
+                continue
+            for a in range(3):
+                lo_f, hi_f = F[n, (2 * a) * width + k], F[n, (2 * a + 1) * width + k]
+                qlo = (int(q[4 + (2 * a) * q4 + k // 4]) >> (8 * (k % 4))) & 0xFF
+                qhi = (int(q[4 + (2 * a + 1) * q4 + k // 4]) >> (8 * (k % 4))) & 0xFF
+                lo_q, hi_q = _fma32(qlo, s[a], p[a]), _fma32(qhi, s[a], p[a])
+                assert lo_q <= lo_f and hi_q >= hi_f, (n, k, a, lo_q, lo_f, hi_q, hi_f)
+                checked += 1
+    assert checked > 1000
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_quantized_build_keeps_host_hits(width):
+    """The host traversal (float nodes) of a quantized build equals the float build's."""
+    ctx = R.Context(-1)
+    rng = np.random.default_rng(11)
+    w = worlds.make_world(rng, 400, 1500, True)
+    rays = worlds.make_rays(rng, 2000)
+    a = R.Scene(ctx)
+    a.set_geometry(**worlds.geometry_kwargs(w))
+    a.build_bvh(width, 4)
+    b = R.Scene(ctx)
+    b.set_geometry(**worlds.geometry_kwargs(w))
+    b.build_bvh(width | R.BVH_QUANTIZED, 4)
+    ha, _, _ = a.host_trace_closest(rays)
+    hb, _, _ = b.host_trace_closest(rays)
+    assert np.array_equal(ha, hb)
