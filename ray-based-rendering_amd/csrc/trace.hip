@@ -101,21 +101,56 @@ __device__ __forceinline__ void childrenQ(const float4* nodes, uint32_t ref, con
     const float bx = __builtin_fmaf(u2f(w[0]), r.ix, -r.oix);
     const float by = __builtin_fmaf(u2f(w[1]), r.iy, -r.oiy);
     const float bz = __builtin_fmaf(u2f(w[2]), r.iz, -r.oiz);
+    // near / far plane words per axis: one select covers 4 children
+    uint32_t nw[3][Q4], fw[3][Q4];
+#pragma unroll
+    for (int g = 0; g < Q4; ++g) {
+        nw[0][g] = r.nx ? w[4 + 1 * Q4 + g] : w[4 + 0 * Q4 + g];
+        fw[0][g] = r.nx ? w[4 + 0 * Q4 + g] : w[4 + 1 * Q4 + g];
+        nw[1][g] = r.ny ? w[4 + 3 * Q4 + g] : w[4 + 2 * Q4 + g];
+        fw[1][g] = r.ny ? w[4 + 2 * Q4 + g] : w[4 + 3 * Q4 + g];
+        nw[2][g] = r.nz ? w[4 + 5 * Q4 + g] : w[4 + 4 * Q4 + g];
+        fw[2][g] = r.nz ? w[4 + 4 * Q4 + g] : w[4 + 5 * Q4 + g];
+    }
 #pragma unroll
     for (int k = 0; k < W; ++k) {
-        const int wo = k >> 2, sh = 8 * (k & 3);
-        const float lx = float((w[4 + 0 * Q4 + wo] >> sh) & 0xFFu), hx = float((w[4 + 1 * Q4 + wo] >> sh) & 0xFFu);
-        const float ly = float((w[4 + 2 * Q4 + wo] >> sh) & 0xFFu), hy = float((w[4 + 3 * Q4 + wo] >> sh) & 0xFFu);
-        const float lz = float((w[4 + 4 * Q4 + wo] >> sh) & 0xFFu), hz = float((w[4 + 5 * Q4 + wo] >> sh) & 0xFFu);
-        const float t0x = __builtin_fmaf(r.nx ? hx : lx, ax, bx), t1x = __builtin_fmaf(r.nx ? lx : hx, ax, bx);
-        const float t0y = __builtin_fmaf(r.ny ? hy : ly, ay, by), t1y = __builtin_fmaf(r.ny ? ly : hy, ay, by);
-        const float t0z = __builtin_fmaf(r.nz ? hz : lz, az, bz), t1z = __builtin_fmaf(r.nz ? lz : hz, az, bz);
+        const int g = k >> 2, sh = 8 * (k & 3);
+        const float t0x = __builtin_fmaf(float((nw[0][g] >> sh) & 0xFFu), ax, bx);
+        const float t1x = __builtin_fmaf(float((fw[0][g] >> sh) & 0xFFu), ax, bx);
+        const float t0y = __builtin_fmaf(float((nw[1][g] >> sh) & 0xFFu), ay, by);
+        const float t1y = __builtin_fmaf(float((fw[1][g] >> sh) & 0xFFu), ay, by);
+        const float t0z = __builtin_fmaf(float((nw[2][g] >> sh) & 0xFFu), az, bz);
+        const float t1z = __builtin_fmaf(float((fw[2][g] >> sh) & 0xFFu), az, bz);
         const float t0 = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, 0.0f));
         const float t1 = fminf(fminf(t1x, t1y), fminf(t1z, tfar));
         const uint32_t ch = w[4 + 6 * Q4 + k];
         const bool h = t0 <= t1 && ch != kEmptyD;
         tc[k] = h ? t0 : INFINITY;
         rc[k] = h ? ch : kEmptyD;
+    }
+}
+
+// Children of a float node (bvh.h), same outputs as childrenQ.
+template <int W>
+__device__ __forceinline__ void childrenF(const float4* nodes, uint32_t ref, const DRay& r, float tfar, float* tc,
+                                          uint32_t* rc) {
+    const float4* nd = nodes + size_t(ref) * (W == 4 ? 8 : 16);
+#pragma unroll
+    for (int half = 0; half < W / 4; ++half) {
+        const int q = W / 4;
+        float4 lx = nd[0 * q + half], hx = nd[1 * q + half], ly = nd[2 * q + half];
+        float4 hy = nd[3 * q + half], lz = nd[4 * q + half], hz = nd[5 * q + half];
+        uint4 ch = *reinterpret_cast<const uint4*>(nd + 6 * q + half);
+        const float* LX = &lx.x; const float* HX = &hx.x; const float* LY = &ly.x;
+        const float* HY = &hy.x; const float* LZ = &lz.x; const float* HZ = &hz.x;
+        const uint32_t* CH = &ch.x;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            bool h;
+            float t = slab(r, LX[k], HX[k], LY[k], HY[k], LZ[k], HZ[k], tfar, h);
+            tc[half * 4 + k] = h ? t : INFINITY;
+            rc[half * 4 + k] = h ? CH[k] : kEmptyD;
+        }
     }
 }
 
@@ -184,6 +219,14 @@ struct PrivateSpillStack {
 
 // One ray through the wide BVH and the unbounded list.  Returns the lowest-id closest hit
 // (bestT/bestId) or, for ANY, whether any primitive is hit in [tmin, tmax].
+//
+// Loop structure ("while-while" with postponed leaves, after Aila & Laine 2009): a lane
+// walks interior nodes until it meets a leaf, parks it, and keeps walking until every lane of
+// the wave that still traverses holds a parked leaf; then the wave tests leaves together.
+// Lanes in a wave so spend their iterations on the same kind of work.  Parking a leaf can only
+// add node visits (culling by the closest hit comes later), never change the result: the
+// closest hit is the minimum over every primitive tested, and every entry that can still
+// hold a hit at least as close is visited.
 template <int W, bool Q, bool ANY, bool MIXED, typename Stack>
 __device__ __forceinline__ void traceRay(const TraceArgs& a, const DRay& r, uint32_t root, Stack& st, float& bestT,
                                          uint32_t& bestId, bool& occluded) {
@@ -191,30 +234,21 @@ __device__ __forceinline__ void traceRay(const TraceArgs& a, const DRay& r, uint
     bestId = RBR_MISS_ID;
     occluded = false;
     if (a.nodes && root != kEmptyD) {
-        uint32_t ref = root;
+        // next stack entry that can still hold a hit (closest hit: entry distance <= best t)
+        auto popNext = [&]() -> uint32_t {
+            while (st.sp > 0) {
+                const uint2 e = st.pop();
+                if (ANY || u2f(e.y) <= bestT) return e.x;
+            }
+            return kEmptyD;
+        };
+        uint32_t ref = root, leaf = kEmptyD;
         while (true) {
-            if (!isLeafD(ref)) {
-                const float4* nd = a.nodes + size_t(ref) * (W == 4 ? 8 : 16);
+            while (ref != kEmptyD && !isLeafD(ref)) {
                 float tc[W];
                 uint32_t rc[W];
                 if (Q) childrenQ<W>(a.nodes, ref, r, bestT, tc, rc);
-#pragma unroll
-                for (int half = 0; half < (Q ? 0 : W / 4); ++half) {
-                    const int q = W / 4;
-                    float4 lx = nd[0 * q + half], hx = nd[1 * q + half], ly = nd[2 * q + half];
-                    float4 hy = nd[3 * q + half], lz = nd[4 * q + half], hz = nd[5 * q + half];
-                    uint4 ch = *reinterpret_cast<const uint4*>(nd + 6 * q + half);
-                    const float* LX = &lx.x; const float* HX = &hx.x; const float* LY = &ly.x;
-                    const float* HY = &hy.x; const float* LZ = &lz.x; const float* HZ = &hz.x;
-                    const uint32_t* CH = &ch.x;
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        bool h;
-                        float t = slab(r, LX[k], HX[k], LY[k], HY[k], LZ[k], HZ[k], bestT, h);
-                        tc[half * 4 + k] = h ? t : INFINITY;
-                        rc[half * 4 + k] = h ? CH[k] : kEmptyD;
-                    }
-                }
+                else childrenF<W>(a.nodes, ref, r, bestT, tc, rc);
                 if (!ANY) {
                     // Sorting network, ascending entry distance (misses sort last as +inf).
                     if (W == 4) {
@@ -234,26 +268,32 @@ __device__ __forceinline__ void traceRay(const TraceArgs& a, const DRay& r, uint
                 int cnt = 0;
 #pragma unroll
                 for (int k = 0; k < W; ++k) cnt += rc[k] != kEmptyD ? 1 : 0;
-                if (cnt > 0) {
-                    if (ANY) {
-                        uint32_t first = kEmptyD;
+                if (cnt == 0) {
+                    ref = popNext();
+                } else if (ANY) {
+                    uint32_t first = kEmptyD;
 #pragma unroll
-                        for (int k = 0; k < W; ++k) {
-                            if (rc[k] == kEmptyD) continue;
-                            if (first == kEmptyD) first = rc[k];
-                            else st.push(rc[k], tc[k]);
-                        }
-                        ref = first;
-                    } else {
-#pragma unroll
-                        for (int k = W - 1; k >= 1; --k)
-                            if (k < cnt) st.push(rc[k], tc[k]);
-                        ref = rc[0];
+                    for (int k = 0; k < W; ++k) {
+                        if (rc[k] == kEmptyD) continue;
+                        if (first == kEmptyD) first = rc[k];
+                        else st.push(rc[k], tc[k]);
                     }
-                    continue;
+                    ref = first;
+                } else {
+#pragma unroll
+                    for (int k = W - 1; k >= 1; --k)
+                        if (k < cnt) st.push(rc[k], tc[k]);
+                    ref = rc[0];
                 }
-            } else {
-                const uint32_t s = leafStartD(ref), c = leafCountD(ref);
+                if (ref != kEmptyD && isLeafD(ref) && leaf == kEmptyD) {
+                    leaf = ref;
+                    ref = popNext();
+                }
+                if (__all(leaf != kEmptyD ? 1 : 0)) break;
+            }
+            // Leaves: the parked one, then ref for as long as it is a leaf.
+            while (leaf != kEmptyD) {
+                const uint32_t s = leafStartD(leaf), c = leafCountD(leaf);
                 for (uint32_t k = 0; k < c; ++k) {
                     float t;
                     uint32_t id;
@@ -263,14 +303,13 @@ __device__ __forceinline__ void traceRay(const TraceArgs& a, const DRay& r, uint
                     }
                 }
                 if (ANY && occluded) break;
+                leaf = kEmptyD;
+                if (ref != kEmptyD && isLeafD(ref)) {
+                    leaf = ref;
+                    ref = popNext();
+                }
             }
-            // Pop the next entry that can still hold a closer hit.
-            bool found = false;
-            while (st.sp > 0) {
-                uint2 e = st.pop();
-                if (ANY || u2f(e.y) <= bestT) { ref = e.x; found = true; break; }
-            }
-            if (!found) break;
+            if ((ANY && occluded) || ref == kEmptyD) break;
         }
     }
     // Unbounded primitives (infinite planes), tested linearly.
