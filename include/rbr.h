@@ -123,7 +123,7 @@ typedef struct rbr_render_config {
     int32_t max_depth;
     int32_t use_nee;
     int32_t use_mis;
-    int32_t guided;         /* 1 = MLP path guiding, 3 = also record training samples (PATH only) */
+    int32_t guided;         /* bit 0: MLP path guiding; bit 1: record training samples (PATH only) */
     uint64_t seed;
     uint32_t sample_offset; /* first sample index (progressive)  */
     uint32_t guide_depth;   /* guided jobs: bounces 0..guide_depth-1 draw from the guide (0 = all) */
@@ -292,7 +292,9 @@ int rbr_guide_time(rbr_ctx* ctx, int kind, size_t n, int iters, float* mean_ms);
 /* Guided rendering: rbr_render jobs with config.guided = 1 draw each non-delta continuation
  * direction from 0.5 x guide + 0.5 x BSDF (one-sample MIS; NEE and its weights unchanged, so
  * the image converges to the plain integrator's); guided = 3 also records one training
- * sample per guided vertex.  Records stay on the device until cleared. */
+ * sample per guided vertex; guided = 2 records the same samples from the plain integrator's
+ * BSDF-sampled paths (no guide inference, the image is the plain one bit for bit).  Records
+ * stay on the device until cleared. */
 int rbr_guide_reserve(rbr_ctx* ctx, uint32_t capacity);      /* record capacity, default 2^22 */
 int rbr_guide_collected(rbr_ctx* ctx, uint64_t* count, double* weight_sum);
 /* steps Adam steps, each on `batch` records drawn without repetition (random permutation). */

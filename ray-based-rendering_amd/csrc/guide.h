@@ -53,6 +53,18 @@ __host__ __device__ inline int guideFragPos(int layer, int row, int col) {
     return ((kGFragOff[layer] + mb * kGFragKB[layer] + kb) * 64 + r + 16 * g) * 8 + i;
 }
 
+// Transposed fragments of layers 1-4 for the backward pass (dA^T = W^T dZ^T): fragment
+// (l, mb, kb), mb over the layer's 4 input blocks of 16, kb over its output blocks of 32 (layer
+// 4's 48 outputs padded to 64 with zeros): lane r + 16 g, slot i holds
+// W_l[32 kb + 4 g + (i & 3) + 16 (i >> 2)][16 mb + r].
+constexpr int kGTFrags = 32;
+constexpr int kGTWeights = kGTFrags * 64 * 8;   // 16384 BF16
+__host__ __device__ inline int guideTFragPos(int layer, int row, int col) {
+    const int kb = row >> 5, c = row & 31, g = (c & 15) >> 2, i = 4 * (c >> 4) + (c & 3);
+    const int mb = col >> 4, r = col & 15;
+    return ((8 * (layer - 1) + mb * 2 + kb) * 64 + r + 16 * g) * 8 + i;
+}
+
 // One vMF lobe as the samplers use it.
 struct GLobe {
     float mu[3];
@@ -63,6 +75,7 @@ struct GLobe {
 struct GuideDev {
     const uint16_t* w;     // BF16 weights [kGWeights], row-major W[out][in]
     const uint16_t* wf;    // the same in matrix-core fragment order (guideFragPos)
+    const uint16_t* wt;    // layers 1-4 in transposed fragment order (guideTFragPos)
     const float* b;        // FP32 biases [kGBias]
     float lo[3], inv[3];   // position scaling to [0,1]
 };

@@ -8,18 +8,24 @@
 //                        16-byte read per lane per fragment, shared by the wave's 4 tiles of 16
 //                        queries); outputs go through a small LDS stage to be turned into 8
 //                        vMF lobes per query (FP16 record)
-//   guideFwdBwdKernel    training tile of 16 samples per wave: forward (activations kept in
-//                        LDS), loss -c*log p(dir) and its gradient w.r.t. the 40 outputs,
-//                        backward through the layers for the activation gradients; writes the
-//                        layer inputs A_l and output gradients dZ_l feature-major to HBM
-//   guideWeightGradKernel  dW_l = dZ_l^T A_l over the batch, one 16x16 block per wave over a
-//                        chunk of samples (MFMA), atomically added into the FP32 gradient
-//   guideBiasGradKernel  db_l = column sums of dZ_l
+//   guideFwdBwdKernel    training, 64 samples per wave: the same register-resident transposed
+//                        forward, the loss -c*log p(dir) and its gradient w.r.t. the 40 outputs
+//                        (through a small LDS stage), and the backward pass in the same
+//                        transposed form with a second, transposed fragment copy of the weights
+//                        (dA^T = W^T dZ^T; the ReLU masks are kept as bits).  Every layer input
+//                        A_l and output gradient dZ_l is written feature-major to HBM; the
+//                        register operands (lane = sample) are turned into rows (lane =
+//                        feature) by a product with an identity fragment on the matrix cores,
+//                        which is exact for BF16 values
+//   guideWeightGradKernel  dW_l = dZ_l^T A_l and db_l = column sums of dZ_l: one 64 x 64 tile of
+//                        one layer per wave over a chunk of 1024 samples (16x16x32 MFMA with
+//                        the samples as the reduction index), into a per-chunk partial gradient
+//   guideGradReduceKernel  sums the partial gradients in chunk order (deterministic)
 //   guideAdamKernel      Adam on the FP32 master weights, refreshed BF16 copies
-// The training kernels use v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4*(l/16)+i],
-// B[4*(l/16)+i][l%16] and D[4*(l/16)+i][l%16], i = 0..3.  For the 16x16x32 form only the M/N
-// placement matters (as above); which K index a (lane group, slot) stands for is the
-// caller's choice as long as A and B agree, and guideFragPos / guideEncodeBlock fix it.
+// All matrix-core work uses v_mfma_f32_16x16x32_bf16: lane l holds the A rows / B columns
+// l % 16 and D[4*(l/16)+i][l%16], i = 0..3.  Only the M/N placement is fixed by the hardware;
+// which K index a (lane group, slot) stands for is the caller's choice as long as A and B
+// agree, and guideFragPos / guideTFragPos / guideEncodeBlock fix it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,18 +43,8 @@ namespace {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int kWaves = 4;                     // waves per workgroup
+constexpr int kWaves = 4;                     // waves per inference workgroup
 constexpr int kTile = 16;                     // queries / samples per wave tile
-constexpr int kLdsWeights = kGWeights;        // BF16 weights in LDS
-
-struct WaveBuf {
-    uint16_t X[kTile][kGInPad];               // encoded input
-    uint16_t H[4][kTile][kGHid];              // layer inputs A_1..A_4 (post-ReLU)
-    uint16_t G[kTile][kGHid];                 // current dZ (BF16) for backward
-    float O[kTile][kGOutPad];                 // network output (FP32)
-};
-
-__device__ __forceinline__ v4s ld4(const uint16_t* p) { return *reinterpret_cast<const v4s*>(p); }
 
 // Orders a wave's LDS writes before its later LDS reads by other lanes (no block barrier: the
 // waves of a workgroup run different numbers of tiles).
@@ -56,96 +52,6 @@ __device__ __forceinline__ void waveSync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// C[16 x N] = A[16 x K] * W^T with W stored [N][K] (row per output unit), + bias, optional
-// ReLU; BF16 result to H (row stride ldh) and/or FP32 result to O.
-template <int K, int N, bool RELU>
-__device__ __forceinline__ void layerForward(const uint16_t* A, int lda, const uint16_t* W, const float* bias,
-                                             uint16_t* H, int ldh, float* O) {
-    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-#pragma unroll
-    for (int nb = 0; nb < N / 16; ++nb) {
-        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int k0 = 0; k0 < K; k0 += 16) {
-            const v4s a = ld4(A + r * lda + k0 + 4 * g);
-            const v4s b = ld4(W + (nb * 16 + r) * K + k0 + 4 * g);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-        }
-        const int col = nb * 16 + r;
-        const float bb = bias[col];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            float v = acc[i] + bb;
-            if (RELU) v = fmaxf(v, 0.0f);
-            const int row = 4 * g + i;
-            if (H) H[row * ldh + col] = f2bf(v);
-            if (O) O[row * N + col] = v;
-        }
-    }
-}
-
-// dA[16 x K] = dZ[16 x N] * W (W stored [N][K]); then dZ_prev = dA * (Aprev > 0), BF16.
-template <int K, int N>
-__device__ __forceinline__ void layerBackwardData(const uint16_t* dZ, int ldz, const uint16_t* W,
-                                                  const uint16_t* Aprev, uint16_t* dZprev) {
-    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-#pragma unroll
-    for (int kb = 0; kb < K / 16; ++kb) {
-        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int n0 = 0; n0 < N; n0 += 16) {
-            const v4s a = ld4(dZ + r * ldz + n0 + 4 * g);
-            v4s b;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) b[i] = short(W[(n0 + 4 * g + i) * K + kb * 16 + r]);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-        }
-        const int col = kb * 16 + r;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = 4 * g + i;
-            const bool on = bf2f(Aprev[row * K + col]) > 0.0f;
-            dZprev[row * K + col] = f2bf(on ? acc[i] : 0.0f);
-        }
-    }
-}
-
-__device__ __forceinline__ void loadWeights(uint16_t* lw, float* lb, const uint16_t* w, const float* b) {
-    for (int i = threadIdx.x; i < kGWeights / 4; i += blockDim.x)
-        reinterpret_cast<uint2*>(lw)[i] = reinterpret_cast<const uint2*>(w)[i];
-    for (int i = threadIdx.x; i < kGBias; i += blockDim.x) lb[i] = b[i];
-}
-
-__device__ __forceinline__ void forwardTile(WaveBuf& wb, const uint16_t* lw, const float* lb) {
-    layerForward<kGInPad, kGHid, true>(&wb.X[0][0], kGInPad, lw + kGOffW[0], lb + 0, &wb.H[0][0][0], kGHid, nullptr);
-    waveSync();
-    layerForward<kGHid, kGHid, true>(&wb.H[0][0][0], kGHid, lw + kGOffW[1], lb + 64, &wb.H[1][0][0], kGHid, nullptr);
-    waveSync();
-    layerForward<kGHid, kGHid, true>(&wb.H[1][0][0], kGHid, lw + kGOffW[2], lb + 128, &wb.H[2][0][0], kGHid, nullptr);
-    waveSync();
-    layerForward<kGHid, kGHid, true>(&wb.H[2][0][0], kGHid, lw + kGOffW[3], lb + 192, &wb.H[3][0][0], kGHid, nullptr);
-    waveSync();
-    layerForward<kGHid, kGOutPad, false>(&wb.H[3][0][0], kGHid, lw + kGOffW[4], lb + 256, nullptr, 0, &wb.O[0][0]);
-}
-
-// Encodes the tile's queries into X: lane (q, g) produces the pairs of lane group g of
-// query q (guidePair's order), the same values the inference kernel computes.
-__device__ __forceinline__ void encodeTile(WaveBuf& wb, const float* s9, bool valid) {
-    const int lane = threadIdx.x & 63, q = lane & 15, g = lane >> 4;
-    float z9[9];
-    for (int j = 0; j < 9; ++j) z9[j] = valid ? s9[j] : 0.0f;
-    float2 st = make_float2(0.0f, 1.0f);
-#pragma unroll
-    for (int kb = 0; kb < 7; ++kb) {
-        const u32x4 v = guideEncodeBlock(z9, g, kb, st);
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            const int p = 16 * kb + 8 * (w >> 1) + 2 * g + (w & 1);
-            *reinterpret_cast<uint32_t*>(&wb.X[q][2 * p]) = valid ? v[w] : 0u;
-        }
-    }
 }
 
 }  // namespace
@@ -316,127 +222,324 @@ __global__ __launch_bounds__(256, 2) void guideInferKernel(GuideInferArgs a) {
 
 // ---------------------------------------------------------------- training
 
-__global__ __launch_bounds__(256) void guideFwdBwdKernel(GuideTrainArgs a) {
+namespace {
+
+constexpr int kTT = 2;                 // 16-sample tiles per training wave group: 32 samples
+constexpr int kTrainWaves = 8;         // waves per training workgroup
+constexpr uint32_t kGradChunk = 1024;  // samples per partial weight gradient
+constexpr int kGradTiles = 8;          // weight-gradient tiles: layer 0 in 4 column blocks of 64
+                                       // (the last has 32 columns), layers 1-4 one each
+
+// Identity B operand that turns a forward operand (lane = sample, slots = 32 features of a
+// k-block) into rows: mfma(op, E, 0) = D[sample][feature], lane (n, g) holding feature
+// 16 hi + n of the block for samples 4g..4g+3.  Lane (n, g) carries one BF16 1.0 in slot
+// 4 hi + (n & 3) when g = n / 4 (slot i of lane group g stands for feature 4g + (i & 3) + 16 (i >> 2)).
+__device__ __forceinline__ u32x4 identityOp(int lane, int hi) {
+    const int n = lane & 15, g = lane >> 4;
+    const int i = 4 * hi + (n & 3);
+    const uint32_t one = g == (n >> 2) ? (0x3F80u << (16 * (i & 1))) : 0u;
+    u32x4 r;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) r[w] = w == (i >> 1) ? one : 0u;
+    return r;
+}
+
+// The training arrays A_l and dZ_l are blocked by group: group q holds F rows (features) of
+// its 32 samples, row f at (q F + f) 32.  Writes one 16-feature half of a forward operand of
+// tile t: row f0 + n gets samples 16 t + 4 g .. +3 (8 bytes per lane; the lane part of the
+// address is fixed, the rest is uniform).  The operand holds BF16 values, so the FP32 product
+// with the identity is exact and its upper halves are those values.
+__device__ __forceinline__ void storeRows(uint16_t* grpBase, int f0, int t, u32x4 op, u32x4 id, int lane) {
+    const v4f z = {0.0f, 0.0f, 0.0f, 0.0f};
+    const v4f d = mfma32(op, id, z);
+    uint2 w;
+    w.x = (__float_as_uint(d[0]) >> 16) | (__float_as_uint(d[1]) & 0xFFFF0000u);
+    w.y = (__float_as_uint(d[2]) >> 16) | (__float_as_uint(d[3]) & 0xFFFF0000u);
+    *reinterpret_cast<uint2*>(grpBase + (f0 + (lane & 15)) * 32 + 16 * t + 4 * (lane >> 4)) = w;
+}
+
+// Both halves of k-block kb of a group's operands (F rows per group in dst).
+__device__ __forceinline__ void storeBlock(uint16_t* dst, int F, uint32_t grp, int kb, const u32x4 (&op)[kTT], u32x4 idLo,
+                                           u32x4 idHi, int lane) {
+    uint16_t* base = dst + size_t(grp) * F * 32;
+#pragma unroll
+    for (int t = 0; t < kTT; ++t) storeRows(base, 32 * kb, t, op[t], idLo, lane);
+#pragma unroll
+    for (int t = 0; t < kTT; ++t) storeRows(base, 32 * kb + 16, t, op[t], idHi, lane);
+}
+
+// Bit i set when operand slot i is not +-0 (the ReLU mask of a layer input, slot order).
+__device__ __forceinline__ uint32_t nonzeroBits(u32x4 h) {
+    uint32_t m = 0u;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        m |= ((h[w] & 0x7FFFu) != 0u ? 1u : 0u) << (2 * w);
+        m |= ((h[w] & 0x7FFF0000u) != 0u ? 1u : 0u) << (2 * w + 1);
+    }
+    return m;
+}
+
+// Masked BF16 pack of output blocks 2kb, 2kb+1 into a k-block operand (reluPack's slot order):
+// slot i keeps its value where bit i of `bits` is set.
+__device__ __forceinline__ u32x4 maskPack(const v4f& lo, const v4f& hi, uint32_t bits) {
+    u32x4 r;
+    r[0] = packBf2((bits & 1u) ? lo[0] : 0.0f, (bits & 2u) ? lo[1] : 0.0f);
+    r[1] = packBf2((bits & 4u) ? lo[2] : 0.0f, (bits & 8u) ? lo[3] : 0.0f);
+    r[2] = packBf2((bits & 16u) ? hi[0] : 0.0f, (bits & 32u) ? hi[1] : 0.0f);
+    r[3] = packBf2((bits & 64u) ? hi[2] : 0.0f, (bits & 128u) ? hi[3] : 0.0f);
+    return r;
+}
+
+// Layer L (1..4) of the training forward: acc holds layer L-1's pre-activations on entry and
+// layer L's on return.  Stores A_L, records its ReLU mask (bits 16 (L-1) + 8 kb + slot).
+template <int L, int MB>
+__device__ __forceinline__ void trainLayer(v4f (&acc)[kTT][4], uint64_t (&mask)[kTT], const u32x4* lwf, const float* lb,
+                                           const GuideTrainArgs& a, uint32_t grp, u32x4 idLo, u32x4 idHi, int lane) {
+    const int g = lane >> 4;
+    u32x4 h[2][kTT];
+#pragma unroll
+    for (int t = 0; t < kTT; ++t) {
+        h[0][t] = reluPack(acc[t][0], acc[t][1]);
+        h[1][t] = reluPack(acc[t][2], acc[t][3]);
+        mask[t] |= uint64_t(nonzeroBits(h[0][t]) | (nonzeroBits(h[1][t]) << 8)) << (16 * (L - 1));
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) acc[t][mb] = biasInit(lb, 64 * L, mb, g);
+    }
+    storeBlock(a.act[L], kGHid, grp, 0, h[0], idLo, idHi, lane);
+    storeBlock(a.act[L], kGHid, grp, 1, h[1], idLo, idHi, lane);
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+            const u32x4 af = lwf[(kGFragOff[L] + mb * 2 + kb) * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < kTT; ++t) acc[t][mb] = mfma32(af, h[kb][t], acc[t][mb]);
+        }
+}
+
+// Backward through layer L (4..1): dz holds dZ_L as k-block operands; on return it holds
+// dZ_{L-1} = BF16(W_L^T dZ_L masked by A_L > 0), which is also stored.
+template <int L>
+__device__ __forceinline__ void trainBack(u32x4 (&dz)[2][kTT], v4f (&acc)[kTT][4], const uint64_t (&mask)[kTT],
+                                          const u32x4* lwt, const GuideTrainArgs& a, uint32_t grp, u32x4 idLo,
+                                          u32x4 idHi, int lane) {
+#pragma unroll
+    for (int t = 0; t < kTT; ++t)
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) acc[t][mb] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            const u32x4 at = lwt[(8 * (L - 1) + mb * 2 + kb) * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < kTT; ++t) acc[t][mb] = mfma32(at, dz[kb][t], acc[t][mb]);
+        }
+#pragma unroll
+    for (int t = 0; t < kTT; ++t) {
+        const uint32_t bits = uint32_t(mask[t] >> (16 * (L - 1)));
+        dz[0][t] = maskPack(acc[t][0], acc[t][1], bits & 0xFFu);
+        dz[1][t] = maskPack(acc[t][2], acc[t][3], (bits >> 8) & 0xFFu);
+    }
+    storeBlock(a.dz[L - 1], kGHid, grp, 0, dz[0], idLo, idHi, lane);
+    storeBlock(a.dz[L - 1], kGHid, grp, 1, dz[1], idLo, idHi, lane);
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(512) void guideFwdBwdKernel(GuideTrainArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    uint16_t* lw = reinterpret_cast<uint16_t*>(smem);
-    float* lb = reinterpret_cast<float*>(smem + kLdsWeights * 2);
-    WaveBuf* bufs = reinterpret_cast<WaveBuf*>(smem + kLdsWeights * 2 + kGBias * 4 + 64);
-    loadWeights(lw, lb, a.g.w, a.g.b);
+    u32x4* lwf = reinterpret_cast<u32x4*>(smem);
+    u32x4* lwt = reinterpret_cast<u32x4*>(smem + kGWeights * 2);
+    float* lb = reinterpret_cast<float*>(smem + kGWeights * 2 + kGTWeights * 2);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    float* stage = reinterpret_cast<float*>(smem + kGWeights * 2 + kGTWeights * 2 + kGBias * 4) + wave * kInferStage;
+    for (int i = threadIdx.x; i < kGWeights / 8; i += blockDim.x) lwf[i] = reinterpret_cast<const u32x4*>(a.g.wf)[i];
+    for (int i = threadIdx.x; i < kGTWeights / 8; i += blockDim.x) lwt[i] = reinterpret_cast<const u32x4*>(a.g.wt)[i];
+    for (int i = threadIdx.x; i < kGBias; i += blockDim.x) lb[i] = a.g.b[i];
     __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane & 15, part = lane >> 4;
-    WaveBuf& wb = bufs[wave];
+    const u32x4 idLo = identityOp(lane, 0), idHi = identityOp(lane, 1);
     const uint32_t n = a.n;
-    const uint32_t tiles = (n + kTile - 1) / kTile;
+    const uint32_t groups = (n + kTile * kTT - 1) / (kTile * kTT);
     float lossAcc = 0.0f;
-    for (uint32_t t = blockIdx.x * kWaves + wave; t < tiles; t += gridDim.x * kWaves) {
-        const uint32_t si = t * kTile + q;
-        const bool valid = si < n;
-        float s9[9], dir[3] = {0, 0, 1}, c = 0.0f;
-        if (valid) guideSample(a, si, s9, dir, c);
-        encodeTile(wb, s9, valid);
-        waveSync();
-        forwardTile(wb, lw, lb);
-        waveSync();
-        // Output gradient: each lane evaluates the 8 lobes of its sample and writes the
-        // gradient of lobes 2*part, 2*part+1.
-        float grad[10];
-        const float l = outputGrad(&wb.O[q][0], dir, valid ? c * a.scale : 0.0f, part, grad);
-        if (part == 0 && valid) lossAcc += l;
-        for (int j = 0; j < 10; ++j) wb.G[q][10 * part + j] = f2bf(grad[j]);
-        if (part == 0)
-            for (int j = kGOut; j < kGOutPad; ++j) wb.G[q][j] = 0;
-        waveSync();
-        // Store A_4 and dZ_4, then walk the layers down.
-        const size_t S = a.cap;
-        const uint32_t s0 = t * kTile;
-        auto storeT = [&](uint16_t* dst, const uint16_t* src, int cols, int ld) {
-            for (int idx = lane; idx < cols * kTile; idx += 64) {
-                const int col = idx / kTile, row = idx % kTile;
-                dst[size_t(col) * S + s0 + row] = src[row * ld + col];
+    for (uint32_t grp = blockIdx.x * kTrainWaves + wave; grp < groups; grp += gridDim.x * kTrainWaves) {
+        const uint32_t s0 = grp * kTile * kTT;
+        float s9[kTT][9], dir[kTT][3], c[kTT];
+        bool valid[kTT];
+#pragma unroll
+        for (int t = 0; t < kTT; ++t) {
+            const uint32_t si = s0 + kTile * t + r;
+            valid[t] = si < n;
+            c[t] = 0.0f;
+            dir[t][0] = 0.0f;
+            dir[t][1] = 0.0f;
+            dir[t][2] = 1.0f;
+            for (int j = 0; j < 9; ++j) s9[t][j] = 0.0f;
+            if (valid[t]) guideSample(a, si, s9[t], dir[t], c[t]);
+        }
+        // layer 0: features are produced one k-block at a time and stored as A_0
+        v4f acc[kTT][4];
+        float2 st[kTT];
+        uint64_t mask[kTT];
+#pragma unroll
+        for (int t = 0; t < kTT; ++t) {
+            st[t] = make_float2(0.0f, 1.0f);
+            mask[t] = 0ull;
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb) acc[t][mb] = biasInit(lb, 0, mb, g);
+        }
+#pragma unroll
+        for (int kb = 0; kb < 7; ++kb) {
+            u32x4 bx[kTT];
+#pragma unroll
+            for (int t = 0; t < kTT; ++t) bx[t] = guideEncodeBlock(s9[t], g, kb, st[t]);
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb) {
+                const u32x4 af = lwf[(kGFragOff[0] + mb * 7 + kb) * 64 + lane];
+#pragma unroll
+                for (int t = 0; t < kTT; ++t) acc[t][mb] = mfma32(af, bx[t], acc[t][mb]);
             }
-        };
-        storeT(a.dz[4], &wb.G[0][0], kGOutPad, kGHid);
-        storeT(a.act[4], &wb.H[3][0][0], kGHid, kGHid);
-        waveSync();
-        // A_0 (the input encoding) is needed for dW_0: store it before X is reused as scratch.
-        storeT(a.act[0], &wb.X[0][0], kGInPad, kGInPad);
-        storeT(a.act[1], &wb.H[0][0][0], kGHid, kGHid);
-        storeT(a.act[2], &wb.H[1][0][0], kGHid, kGHid);
-        storeT(a.act[3], &wb.H[2][0][0], kGHid, kGHid);
-        waveSync();
-        uint16_t* scratch = &wb.X[0][0];   // [16][64] BF16 dZ_prev buffer
-        // layer 4 -> dZ_3 (N = 48 outputs, K = 64 inputs)
-        layerBackwardData<kGHid, kGOutPad>(&wb.G[0][0], kGHid, lw + kGOffW[4], &wb.H[3][0][0], scratch);
-        waveSync();
-        storeT(a.dz[3], scratch, kGHid, kGHid);
-        for (int i = lane; i < kTile * kGHid; i += 64) (&wb.G[0][0])[i] = scratch[i];
-        waveSync();
-        layerBackwardData<kGHid, kGHid>(&wb.G[0][0], kGHid, lw + kGOffW[3], &wb.H[2][0][0], scratch);
-        waveSync();
-        storeT(a.dz[2], scratch, kGHid, kGHid);
-        for (int i = lane; i < kTile * kGHid; i += 64) (&wb.G[0][0])[i] = scratch[i];
-        waveSync();
-        layerBackwardData<kGHid, kGHid>(&wb.G[0][0], kGHid, lw + kGOffW[2], &wb.H[1][0][0], scratch);
-        waveSync();
-        storeT(a.dz[1], scratch, kGHid, kGHid);
-        for (int i = lane; i < kTile * kGHid; i += 64) (&wb.G[0][0])[i] = scratch[i];
-        waveSync();
-        layerBackwardData<kGHid, kGHid>(&wb.G[0][0], kGHid, lw + kGOffW[1], &wb.H[0][0][0], scratch);
-        waveSync();
-        storeT(a.dz[0], scratch, kGHid, kGHid);
-        waveSync();
+            storeBlock(a.act[0], kGInPad, grp, kb, bx, idLo, idHi, lane);
+        }
+        trainLayer<1, 4>(acc, mask, lwf, lb, a, grp, idLo, idHi, lane);
+        trainLayer<2, 4>(acc, mask, lwf, lb, a, grp, idLo, idHi, lane);
+        trainLayer<3, 4>(acc, mask, lwf, lb, a, grp, idLo, idHi, lane);
+        trainLayer<4, 3>(acc, mask, lwf, lb, a, grp, idLo, idHi, lane);
+        // Output gradient, one tile at a time through the wave's LDS stage: lane (r, g) takes
+        // lobes 2g, 2g+1 of sample r; the gradient returns as the k-block operands of dZ_4.
+        u32x4 dz[2][kTT];
+#pragma unroll
+        for (int t = 0; t < kTT; ++t) {
+#pragma unroll
+            for (int mb = 0; mb < 3; ++mb)
+                *reinterpret_cast<float4*>(stage + r * kGOutPad + mb * 16 + 4 * g) =
+                    make_float4(acc[t][mb][0], acc[t][mb][1], acc[t][mb][2], acc[t][mb][3]);
+            waveSync();
+            float grad[10];
+            const float l = outputGrad(stage + r * kGOutPad, dir[t], c[t] * a.scale, g, grad);
+            if (g == 0 && valid[t]) lossAcc += l;
+            waveSync();
+#pragma unroll
+            for (int q = 0; q < 10; ++q) stage[r * kGOutPad + 10 * g + q] = grad[q];
+            if (g == 0)
+                for (int q = kGOut; q < kGOutPad; ++q) stage[r * kGOutPad + q] = 0.0f;
+            waveSync();
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb) {
+                float v[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int idx = 32 * kb + 4 * g + (i & 3) + 16 * (i >> 2);
+                    v[i] = idx < kGOutPad ? stage[r * kGOutPad + idx] : 0.0f;
+                }
+                dz[kb][t] = u32x4{packBf2(v[0], v[1]), packBf2(v[2], v[3]), packBf2(v[4], v[5]), packBf2(v[6], v[7])};
+            }
+            waveSync();
+        }
+        // dZ_4 rows 0..47 (k-block 0 both halves, k-block 1 the lower half), then the layers down
+        {
+            uint16_t* base = a.dz[4] + size_t(grp) * kGOutPad * 32;
+#pragma unroll
+            for (int t = 0; t < kTT; ++t) {
+                storeRows(base, 0, t, dz[0][t], idLo, lane);
+                storeRows(base, 16, t, dz[0][t], idHi, lane);
+                storeRows(base, 32, t, dz[1][t], idLo, lane);
+            }
+        }
+        trainBack<4>(dz, acc, mask, lwt, a, grp, idLo, idHi, lane);
+        trainBack<3>(dz, acc, mask, lwt, a, grp, idLo, idHi, lane);
+        trainBack<2>(dz, acc, mask, lwt, a, grp, idLo, idHi, lane);
+        trainBack<1>(dz, acc, mask, lwt, a, grp, idLo, idHi, lane);
     }
     // Loss: one atomic per wave.
     for (int off = 32; off > 0; off >>= 1) lossAcc += __shfl_down(lossAcc, off);
     if (lane == 0) atomicAdd(a.loss, double(lossAcc));
 }
 
-// dW_l[n][k] += sum_s dZ_l[n][s] * A_l[k][s] over one chunk of samples; one 16x16 block per
-// wave.  Blocks are enumerated over all five layers.
-__global__ __launch_bounds__(256) void guideWeightGradKernel(GuideTrainArgs a, uint32_t chunk) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    const uint32_t blockId = blockIdx.x * kWaves + wave;
-    // Block tables: layer 0: 4 x 14, layers 1-3: 4 x 4, layer 4: 3 x 4.
-    int layer, nb, kb;
-    uint32_t b = blockId;
-    if (b < 56) { layer = 0; nb = b / 14; kb = b % 14; }
-    else if (b < 56 + 48) { b -= 56; layer = 1 + b / 16; nb = (b % 16) / 4; kb = b % 4; }
-    else if (b < 116) { b -= 104; layer = 4; nb = b / 4; kb = b % 4; }
-    else return;
+// One (layer, 64-column) tile of dW_l[n][k] = sum_s dZ_l[n][s] A_l[k][s] over one chunk of
+// samples, and for the first column tile of a layer db_l[n] = sum_s dZ_l[n][s]; written to the
+// chunk's partial gradient (every parameter of a chunk is written by exactly one wave).  Per
+// group of 32 samples, lane (r, g) loads samples 8g..8g+7 of row r of each 16-row block (one
+// 16-byte load; a block is 1 KB contiguous), so the sample index is the MFMA's reduction index.  Workgroups are dealt to the 8 XCDs in turn; the
+// mapping keeps the 8 tiles of a chunk on one XCD, whose L2 then serves the dZ_0 rows that the
+// four layer-0 tiles share.
+__global__ __launch_bounds__(64) void guideWeightGradKernel(GuideTrainArgs a, uint32_t chunks, float* __restrict__ partial) {
+    const uint32_t w = blockIdx.x, xcd = w % 8u, slot = w / 8u;
+    const int tile = int(slot % kGradTiles);
+    const uint32_t ch = (slot / kGradTiles) * 8u + xcd;
+    if (ch >= chunks) return;
+    const int layer = tile < 4 ? 0 : tile - 3;
+    const int kt = tile < 4 ? tile : 0;
     const int K = layer == 0 ? kGInPad : kGHid;
-    const size_t S = a.cap;
-    const uint32_t s0 = blockIdx.y * chunk;
-    const uint32_t s1 = min(a.n, s0 + chunk);
-    const uint16_t* dz = a.dz[layer] + size_t(nb * 16 + r) * S;
-    const uint16_t* act = a.act[layer] + size_t(kb * 16 + r) * S;
-    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (uint32_t s = s0; s < s1; s += 16) {
-        const v4s av = ld4(dz + s + 4 * g);
-        const v4s bv = ld4(act + s + 4 * g);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
-    }
+    const int nrb = layer == 4 ? kGOutPad / 16 : 4;
+    const int ncb = min(4, (K - 64 * kt) / 16);
+    const int N = layer == 4 ? kGOutPad : kGHid;
+    const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+    const uint32_t groups = (a.n + 31u) / 32u;
+    const uint32_t q0 = ch * (kGradChunk / 32), q1 = min(groups, q0 + kGradChunk / 32);
+    const uint16_t* dz = a.dz[layer] + r * 32 + 8 * g;
+    const uint16_t* act = a.act[layer] + (64 * kt + r) * 32 + 8 * g;
+    v4f acc[4][4];
+    float bs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int nrow = nb * 16 + 4 * g + i, kcol = kb * 16 + r;
-        atomicAdd(&a.grad[kGOffW[layer] + nrow * K + kcol], acc[i]);
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t q = q0; q < q1; ++q) {
+        const uint16_t* dq = dz + size_t(q) * N * 32;
+        const uint16_t* aq = act + size_t(q) * K * 32;
+        u32x4 A[4], B[4];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+            A[rb] = rb < nrb ? *reinterpret_cast<const u32x4*>(dq + 16 * rb * 32) : u32x4{0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+            B[cb] = cb < ncb ? *reinterpret_cast<const u32x4*>(aq + 16 * cb * 32) : u32x4{0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+                if (rb < nrb && cb < ncb) acc[rb][cb] = mfma32(A[rb], B[cb], acc[rb][cb]);
+        if (kt == 0)
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) bs[rb] += bf2f(uint16_t(A[rb][q] & 0xFFFFu)) + bf2f(uint16_t(A[rb][q] >> 16));
     }
+    float* out = partial + size_t(ch) * kGParams;
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+            if (rb < nrb && cb < ncb)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) out[kGOffW[layer] + (16 * rb + 4 * g + j) * K + 64 * kt + 16 * cb + r] = acc[rb][cb][j];
+    if (kt == 0)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            float v = bs[rb];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (g == 0 && rb < nrb) out[kGOffB[layer] + 16 * rb + r] = v;
+        }
 }
 
-__global__ __launch_bounds__(256) void guideBiasGradKernel(GuideTrainArgs a) {
-    __shared__ float red[256];
-    const int row = blockIdx.x;   // 0..303
-    const int layer = row < 256 ? row / 64 : 4;
-    const int nIdx = row < 256 ? row % 64 : row - 256;
-    const uint16_t* dz = a.dz[layer] + size_t(nIdx) * a.cap;
+// grad[p] = sum over chunks of the partial gradients, in chunk order.
+__global__ __launch_bounds__(256) void guideGradReduceKernel(const float* __restrict__ partial, uint32_t chunks,
+                                                             float* __restrict__ grad) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= kGParams) return;
     float s = 0.0f;
-    for (uint32_t i = threadIdx.x; i < a.n; i += 256) s += bf2f(dz[i]);
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
+    uint32_t c = 0;
+    for (; c + 8 <= chunks; c += 8) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = partial[size_t(c + k) * kGParams + p];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += v[k];
     }
-    if (threadIdx.x == 0) a.grad[kGOffB[layer] + nIdx] = red[0];
+    for (; c < chunks; ++c) s += partial[size_t(c) * kGParams + p];
+    grad[p] = s;
 }
 
 // Element index of weight i (row-major parameter layout) in the fragment-ordered copy.
@@ -447,10 +550,18 @@ __host__ __device__ inline int fragIndexOf(int i) {
     return guideFragPos(l, o / K, o % K);
 }
 
+// Element index of weight i of layers 1-4 in the transposed fragment copy (-1 for layer 0).
+__host__ __device__ inline int tfragIndexOf(int i) {
+    if (i < kGOffW[1]) return -1;
+    const int l = i < kGOffW[2] ? 1 : i < kGOffW[3] ? 2 : i < kGOffW[4] ? 3 : 4;
+    const int o = i - kGOffW[l];
+    return guideTFragPos(l, o / kGHid, o % kGHid);
+}
+
 __global__ __launch_bounds__(256) void guideAdamKernel(float* __restrict__ p, float* __restrict__ m,
                                                        float* __restrict__ v, const float* __restrict__ g,
                                                        uint16_t* __restrict__ wbf, uint16_t* __restrict__ wfrag,
-                                                       float lr, float c1, float c2) {
+                                                       uint16_t* __restrict__ wtrans, float lr, float c1, float c2) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= kGParams) return;
     const float gi = g[i];
@@ -464,6 +575,8 @@ __global__ __launch_bounds__(256) void guideAdamKernel(float* __restrict__ p, fl
         const uint16_t b = f2bf(pi);
         wbf[i] = b;
         wfrag[fragIndexOf(i)] = b;
+        const int ti = tfragIndexOf(i);
+        if (ti >= 0) wtrans[ti] = b;
     }
 }
 
@@ -544,27 +657,34 @@ GuideState& state(rbr_ctx* ctx) {
     return *ctx->guide;
 }
 
-size_t trainSmem() { return size_t(kLdsWeights) * 2 + kGBias * 4 + 64 + sizeof(WaveBuf) * kWaves; }
+size_t trainSmem() {
+    return size_t(kGWeights) * 2 + size_t(kGTWeights) * 2 + kGBias * 4 + size_t(kInferStage) * 4 * kTrainWaves;
+}
 size_t inferSmem() { return size_t(kGWeights) * 2 + kGBias * 4 + size_t(kInferStage) * 4 * kWaves; }
 
 int refreshBf16(rbr_ctx* ctx, GuideState& g) {
-    std::vector<uint16_t> bf(kGWeights), frag(kGWeights);
+    std::vector<uint16_t> bf(kGWeights), frag(kGWeights), trans(kGTWeights, 0);
     for (int i = 0; i < kGWeights; ++i) {
         uint32_t u;
         std::memcpy(&u, &g.master[i], 4);
         u += 0x7FFFu + ((u >> 16) & 1u);
         bf[i] = uint16_t(u >> 16);
         frag[fragIndexOf(i)] = bf[i];
+        const int ti = tfragIndexOf(i);
+        if (ti >= 0) trans[ti] = bf[i];
     }
     int rc;
     if ((rc = uploadBuffer(ctx, g.dW, bf.data(), bf.size() * 2))) return rc;
     if ((rc = uploadBuffer(ctx, g.dWf, frag.data(), frag.size() * 2))) return rc;
+    if ((rc = uploadBuffer(ctx, g.dWt, trans.data(), trans.size() * 2))) return rc;
     if ((rc = uploadBuffer(ctx, g.dMaster, g.master.data(), g.master.size() * 4))) return rc;
     return RBR_OK;
 }
 
+uint32_t gradChunks(size_t n) { return uint32_t(((n + 31) / 32 * 32 + kGradChunk - 1) / kGradChunk); }
+
 int ensureTrain(rbr_ctx* ctx, GuideState& g, size_t n) {
-    const size_t cap = (n + 15) / 16 * 16;
+    const size_t cap = (n + kTile * kTT - 1) / (kTile * kTT) * (kTile * kTT);
     if (cap > g.cap) {
         for (int l = 0; l < 5; ++l) {
             const int K = l == 0 ? kGInPad : kGHid;
@@ -573,6 +693,11 @@ int ensureTrain(rbr_ctx* ctx, GuideState& g, size_t n) {
             if (int rc = ensureBuffer(ctx, g.dz[l], cap * N * 2)) return rc;
         }
         g.cap = cap;
+    }
+    const size_t chunks = gradChunks(n);
+    if (chunks > g.partialChunks) {
+        if (int rc = ensureBuffer(ctx, g.dPartial, chunks * kGParams * 4)) return rc;
+        g.partialChunks = chunks;
     }
     int rc;
     if ((rc = ensureBuffer(ctx, g.dGrad, size_t(kGParams) * 4))) return rc;
@@ -602,14 +727,17 @@ GuideTrainArgs trainArgs(rbr_ctx* ctx, GuideState& g, const float4* samples, uin
 }
 
 int launchLossGrad(rbr_ctx* ctx, GuideState& g, const GuideTrainArgs& a) {
-    hipLaunchKernelGGL(guideZeroKernel, dim3((kGParams + 255) / 256), dim3(256), 0, ctx->stream, a.grad, kGParams, a.loss);
-    const unsigned tiles = (a.n + kTile - 1) / kTile;
-    const unsigned grid = std::max(1u, std::min<unsigned>((tiles + kWaves - 1) / kWaves, unsigned(ctx->cuCount) * 2));
-    hipLaunchKernelGGL(guideFwdBwdKernel, dim3(grid), dim3(256), trainSmem(), ctx->stream, a);
-    const uint32_t chunk = 2048;
-    hipLaunchKernelGGL(guideWeightGradKernel, dim3((116 + kWaves - 1) / kWaves, (a.n + chunk - 1) / chunk), dim3(256), 0,
-                       ctx->stream, a, chunk);
-    hipLaunchKernelGGL(guideBiasGradKernel, dim3(kGBias), dim3(256), 0, ctx->stream, a);
+    // the loss accumulator only: every gradient element is written by the reduction
+    hipLaunchKernelGGL(guideZeroKernel, dim3(1), dim3(64), 0, ctx->stream, a.grad, 0, a.loss);
+    const unsigned groups = (a.n + kTile * kTT - 1) / (kTile * kTT);
+    const unsigned grid = std::max(1u, std::min<unsigned>((groups + kTrainWaves - 1) / kTrainWaves, unsigned(ctx->cuCount)));
+    hipLaunchKernelGGL(guideFwdBwdKernel, dim3(grid), dim3(64 * kTrainWaves), trainSmem(), ctx->stream, a);
+    const uint32_t chunks = gradChunks(a.n);
+    float* partial = static_cast<float*>(g.dPartial.ptr);
+    hipLaunchKernelGGL(guideWeightGradKernel, dim3((chunks + 7) / 8 * 8 * kGradTiles), dim3(64), 0, ctx->stream, a, chunks,
+                       partial);
+    hipLaunchKernelGGL(guideGradReduceKernel, dim3((kGParams + 255) / 256), dim3(256), 0, ctx->stream, partial, chunks,
+                       a.grad);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? RBR_OK : failHip(ctx, e, "guide training launch");
 }
@@ -620,6 +748,7 @@ GuideDev guideDev(const GuideState& g) {
     GuideDev d{};
     d.w = static_cast<const uint16_t*>(g.dW.ptr);
     d.wf = static_cast<const uint16_t*>(g.dWf.ptr);
+    d.wt = static_cast<const uint16_t*>(g.dWt.ptr);
     d.b = static_cast<const float*>(g.dMaster.ptr) + kGWeights;
     for (int k = 0; k < 3; ++k) {
         d.lo[k] = g.lo[k];
@@ -682,7 +811,7 @@ int guideTrainDevice(rbr_ctx* ctx, const float4* samples, uint32_t stride, uint3
     hipLaunchKernelGGL(guideAdamKernel, dim3((kGParams + 255) / 256), dim3(256), 0, ctx->stream,
                        static_cast<float*>(g.dMaster.ptr), static_cast<float*>(g.dM.ptr), static_cast<float*>(g.dV.ptr),
                        static_cast<const float*>(g.dGrad.ptr), static_cast<uint16_t*>(g.dW.ptr),
-                       static_cast<uint16_t*>(g.dWf.ptr), lr, c1, c2);
+                       static_cast<uint16_t*>(g.dWf.ptr), static_cast<uint16_t*>(g.dWt.ptr), lr, c1, c2);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return failHip(ctx, e, "guide Adam launch");
     g.masterDirty = true;
@@ -712,7 +841,7 @@ int guideEnsureRecords(rbr_ctx* ctx, GuideState& g) {
 void destroyGuideState(rbr_ctx* ctx) {
     GuideState* g = ctx->guide;
     if (!g) return;
-    for (DevBuf* b : {&g->dW, &g->dWf, &g->dMaster, &g->dM, &g->dV, &g->dGrad, &g->dLoss, &g->dQ, &g->dOut, &g->dRec, &g->dRecCount,
+    for (DevBuf* b : {&g->dW, &g->dWf, &g->dWt, &g->dPartial, &g->dMaster, &g->dM, &g->dV, &g->dGrad, &g->dLoss, &g->dQ, &g->dOut, &g->dRec, &g->dRecCount,
                       &g->dCsum})
         freeBuffer(*b);
     for (int l = 0; l < 5; ++l) {

@@ -88,10 +88,10 @@ struct GuideTrainArgs {
     // does not divide total, so a batch never repeats a record.  total = 0: record s.
     uint32_t total, permMul, permAdd;
     float cmax;               // training targets are clamped to cmax before scaling
-    size_t cap;               // row length of the feature-major activation arrays
+    size_t cap;               // samples the activation arrays hold (multiple of 32)
     float scale;              // multiplies c (1 / (batch * mean c))
-    uint16_t* act[5];         // A_l, feature-major [K_l][cap]
-    uint16_t* dz[5];          // dZ_l, feature-major [N_l][cap]
+    uint16_t* act[5];         // A_l, BF16, blocked [cap / 32][K_l][32 samples]
+    uint16_t* dz[5];          // dZ_l, BF16, blocked [cap / 32][N_l][32 samples]
     float* grad;              // FP32 gradient [kGParams]
     double* loss;
 };
@@ -307,9 +307,9 @@ struct GuideState {
     int step = 0;
     float lo[3] = {0, 0, 0}, inv[3] = {1, 1, 1};
     std::vector<float> master;
-    DevBuf dW, dWf, dMaster, dM, dV, dGrad, dLoss, dQ, dOut;
-    DevBuf act[5], dz[5];
-    size_t cap = 0;
+    DevBuf dW, dWf, dWt, dMaster, dM, dV, dGrad, dLoss, dQ, dOut;
+    DevBuf act[5], dz[5], dPartial;
+    size_t cap = 0, partialChunks = 0;
     // Samples recorded by the renderer, kRecStride float4 each:
     //   (position, c) (outward normal, path slot) (wo, -) (direction, -) (throughput, -) (L, -)
     // dRecCount holds {appended, finalised}; dCsum the sum of c over finalised records.

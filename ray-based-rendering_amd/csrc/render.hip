@@ -781,7 +781,10 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
         // Guided jobs: one-sample MIS between the guide's vMF mixture (probability
         // kGuideFrac) and the BSDF; the direction's density is the mixture of both, a delta
         // BSDF sample keeps its discrete probability.  All-delta materials are never guided.
-        const bool guided = (jb.guided & 1u) && !m.allDelta && depth < jb.guideDepth;
+        // Recording jobs write a training sample per guidable vertex, whichever density drew
+        // the direction (guided = 2 records plain BSDF-sampled paths).
+        const bool guidable = !m.allDelta && depth < jb.guideDepth;
+        const bool guided = (jb.guided & 1u) && guidable;
         uint4 gl[5];
         if (guided)
             for (int k = 0; k < 5; ++k) gl[k] = q.guide[size_t(src) * 5 + k];
@@ -822,7 +825,7 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
                 spec = bs.delta;
                 spawn(P, n, wi, no, nd);
                 cont = depth + 1 < jb.maxDepth;
-                record = guided && (jb.guided & 2u) && !bs.delta && cont;
+                record = guidable && (jb.guided & 2u) && !bs.delta && cont;
                 wiRec = wi;
             }
         }
@@ -1293,7 +1296,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
             if (jb.config.integrator != RBR_INTEGRATOR_PATH)
                 return fail(ctx, RBR_ERR_UNSUPPORTED, "guiding is implemented for the path integrator");
             if (!ctx->guide || !ctx->guide->ready) return fail(ctx, RBR_ERR_STATE, "guided job needs rbr_guide_init");
-            anyGuided = true;
+            anyGuided |= (jb.config.guided & 1) != 0;
             anyRecord |= (jb.config.guided & 2) != 0;
         }
         if (uint64_t(jb.camera.width) * jb.camera.height > (1ull << 26)) return fail(ctx, RBR_ERR_INVALID, "film too large");

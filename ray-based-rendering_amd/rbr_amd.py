@@ -437,7 +437,7 @@ def render(ctx: Context, jobs: list) -> list[np.ndarray]:
     """Integrator::render for one or many jobs in one launch sequence.  Each job is a dict
     with scene (a Scene from scene_from_desc), spp, depth and optional seed, integrator, nee,
     mis, camera, sample_offset, guided (1: sample the context's guide, 3: also record training
-    samples).  Returns one (H, W, 3) float32 film per job (per-pixel mean radiance)."""
+    samples, 2: record training samples from plain BSDF-sampled paths), guide_depth.  Returns one (H, W, 3) float32 film per job (per-pixel mean radiance)."""
     arr = (RbrJob * len(jobs))()
     films = []
     for i, j in enumerate(jobs):

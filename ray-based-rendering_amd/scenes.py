@@ -7,6 +7,7 @@ background and a pinhole camera.  Normals are computed in FP32 in the reference'
 operand order (cross then normalized = v * (1/len)) so every consumer sees the same bits.
 
   cornell_c0()        Cornell box of side 555 built from quads, 256x256, 16 spp, depth 8
+  cornell_indirect()  the same box lit only indirectly (light panel facing the ceiling)
   terrain_c2()        512x512-cell noise heightfield + 64 spheres, 4 area + 1 directional light
   fog_jobs_c3()       16 uneven jobs: displaced icospheres in a closed room with fog
 """
@@ -195,6 +196,31 @@ def cornell_c0(width: int = 256, height: int = 256) -> SceneDesc:
     b.lights.append(area_light((278, 554, 279.5), (130, 0, 0), (0, 0, 105), (17, 12, 4), 1.0))
     cam = look_at((278, 278, -800), (278, 278, 0), (0, 1, 0), width, height, 40.0)
     return b.build(cam, name="cornell555")
+
+
+def cornell_indirect(width: int = 256, height: int = 256) -> SceneDesc:
+    """The Cornell box of cornell_c0 with its light panel turned to face the ceiling: the
+    panel (130 x 105, emission (17,12,4)) hangs at y = 450 and emits upward only, an opaque
+    white shade of the same size sits just under it.  NEE reaches the panel from the ceiling
+    alone, so everything else in the room is lit indirectly, by the bright ceiling patch
+    above the panel -- light that only the continuation direction can find."""
+    b = _Builder()
+    white = b.mat(diffuse((0.73, 0.73, 0.73)))
+    red = b.mat(diffuse((0.65, 0.05, 0.05)))
+    green = b.mat(diffuse((0.12, 0.45, 0.15)))
+    light = b.mat(emissive((17.0, 12.0, 4.0)))
+    b.quad((0, 0, 0), (555, 0, 0), (0, 0, 555), white)        # floor
+    b.quad((0, 555, 0), (555, 0, 0), (0, 0, 555), white)      # ceiling
+    b.quad((0, 0, 555), (555, 0, 0), (0, 555, 0), white)      # back
+    b.quad((555, 0, 0), (0, 555, 0), (0, 0, 555), red)        # x = 555
+    b.quad((0, 0, 0), (0, 555, 0), (0, 0, 555), green)        # x = 0
+    b.quad((213, 450, 227), (0, 0, 105), (130, 0, 0), light)  # emitter, u x v = +y
+    b.quad((213, 449, 227), (130, 0, 0), (0, 0, 105), white)  # shade under it
+    b.box((165, 330, 165), 15.0, (265, 0, 295), white)
+    b.box((165, 165, 165), -18.0, (130, 0, 65), white)
+    b.lights.append(area_light((278, 450, 279.5), (0, 0, 105), (130, 0, 0), (17, 12, 4), 1.0))
+    cam = look_at((278, 278, -800), (278, 278, 0), (0, 1, 0), width, height, 40.0)
+    return b.build(cam, name="cornell555-indirect")
 
 
 # ---------------------------------------------------------------- terrain scene

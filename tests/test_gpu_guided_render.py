@@ -108,6 +108,28 @@ def test_records_training_and_clear(ctx):
     g.reserve(1 << 22)
 
 
+def test_record_only_is_plain_image(ctx):
+    """guided = 2 records training samples from the plain integrator's paths: the film is the
+    plain one bit for bit, and the records train the guide like guided = 3 records do."""
+    say("case: record-only cornell 48x48")
+    sc = R.scene_from_desc(ctx, S.cornell_c0(48, 48))
+    lo, hi = sc.world_bounds()
+    g = R.Guide(ctx, lo, hi, seed=9)
+    g.clear_collected()
+    plain = R.render(ctx, [dict(scene=sc, spp=4, depth=8, seed=5)])[0]
+    rec = R.render(ctx, [dict(scene=sc, spp=4, depth=8, seed=5, guided=2)])[0]
+    assert np.array_equal(plain, rec)
+    n, cs = g.collected()
+    say(f"  records {n}, mean c {cs / max(n, 1):.4g}")
+    assert 48 * 48 * 4 <= n <= 48 * 48 * 4 * 7 and cs > 0
+    l0 = g.train_collected(1, 4096, 1e-3)
+    for _ in range(20):
+        last = g.train_collected(4, 4096, 1e-3)
+    say(f"  loss {l0:.4f} -> {last:.4f}")
+    assert np.isfinite(l0) and last < l0
+    g.clear_collected()
+
+
 def test_guided_errors(ctx):
     sc = R.scene_from_desc(ctx, S.cornell_c0(16, 16))
     with R.Context(0) as c2:
