@@ -1,23 +1,32 @@
 """Learned path guiding vs the plain NEE/MIS tracer: time to a target relMSE.
 
-  python ray-based-rendering_amd/bench_guiding.py [--width 1920 --height 1080] [--ref-spp 1024]
+  python ray-based-rendering_amd/bench_guiding.py [--scene terrain|cornell_indirect]
+      [--width 1920 --height 1080] [--ref-spp 1024] [--schedules '[{...}, ...]']
 
-Scene: the terrain workload (524,288 triangles, 64 spheres, 4 area lights + sun), depth 8.
+Scenes (depth 8):
+  terrain           the C2 workload: 524,288 triangles, 64 spheres, 4 area lights + sun
+  cornell_indirect  the Cornell box lit only indirectly (light panel facing the ceiling)
 
 * Reference: the plain GPU tracer at --ref-spp samples per pixel, accumulated in FP64 on the
   device (rbr_render_moments).  Its own error is estimated from its per-pixel variance and
   reported ("ref_relmse").
 * relMSE(I) = mean over pixels and RGB channels of (I - R)^2 / (R^2 + 1e-2).
 * Plain: progressive passes of --spp-per-call samples; the image is the running mean.
-* Guided: the guide is trained online while rendering: --train-passes passes of one sample
-  each that sample the guide and record training samples (guided = 3), each followed by
-  --train-steps Adam steps of --batch records; then guided passes (guided = 1).  Every pass,
-  training passes included, is unbiased and enters the running mean.  Training time is
-  part of the guided time.
+* Guided, one run per schedule (a dict; missing keys take the defaults below):
+    train_passes  passes that record training samples, each followed by train_steps Adam
+                  steps of --batch records (then the records are cleared)
+    train_spp     samples per pixel of a training pass
+    train_mode    "guided": the guide is sampled while recording (guided = 3);
+                  "plain": records come from the plain integrator's paths (guided = 2)
+    online_steps  after the training passes, every guided pass also records and is followed
+                  by this many Adam steps (0: guided passes only sample, guided = 1)
+    guide_depth   bounces that draw from the guide (0 = all)
+  Every pass, training passes included, is unbiased and enters the running mean; training
+  time is part of the guided time.  The guide starts from the same initialisation each run.
 * Times are host wall-clock around the render and training calls (each render call returns
   its film, so they include the film download); relMSE evaluation is not timed.
 
-Prints one JSON line with both convergence curves and the time to each target relMSE.
+Prints one JSON line per schedule with both convergence curves and the time to each target.
 """
 from __future__ import annotations
 
@@ -36,6 +45,9 @@ import rbr_amd as R  # noqa: E402
 import scenes as S  # noqa: E402
 
 EPS = 1e-2
+DEFAULT_SCHEDULE = {"train_passes": 16, "train_spp": 1, "train_mode": "guided", "train_steps": 8,
+                    "online_steps": 0, "guide_depth": 0}
+SCENES = {"terrain": S.terrain_c2, "cornell_indirect": S.cornell_indirect}
 
 
 def relmse(img: np.ndarray, ref: np.ndarray) -> float:
@@ -60,56 +72,90 @@ def time_to(curve, target):
     return None
 
 
-def progressive(ctx, sc, ref, max_spp, per_call, seed, guided=0, pre=None, log=None, guide_depth=0):
-    """Runs passes until max_spp; returns the [(seconds, relMSE)] curve and spp reached."""
-    acc = None
-    n = 0
-    t = 0.0
-    curve = []
-    if pre is not None:   # training passes: list of (callable returning (film, spp, seconds))
-        for step in pre:
-            film, k, dt = step()
-            t += dt
-            acc = film.astype(np.float64) * k if acc is None else acc + film.astype(np.float64) * k
-            n += k
-            curve.append((t, relmse(acc / n, ref)))
+class Progressive:
+    """Running mean of passes with its (seconds, relMSE) curve."""
+
+    def __init__(self, ref):
+        self.ref, self.acc, self.n, self.t, self.curve = ref, None, 0, 0.0, []
+
+    def add(self, film, k, dt):
+        self.t += dt
+        f = film.astype(np.float64) * k
+        self.acc = f if self.acc is None else self.acc + f
+        self.n += k
+        e = relmse(self.acc / self.n, self.ref)
+        self.curve.append((self.t, e))
+        return e
+
+
+def run_plain(ctx, sc, ref, args, log):
+    p = Progressive(ref)
     call = 0
-    while n < max_spp:
-        k = min(per_call, max_spp - n)
+    while p.n < args.max_spp:
+        k = min(args.spp_per_call, args.max_spp - p.n)
         t0 = time.perf_counter()
-        film = R.render(ctx, [dict(scene=sc, spp=k, depth=8, seed=seed + call, guided=guided,
-                                   guide_depth=guide_depth)])[0]
-        t += time.perf_counter() - t0
+        film = R.render(ctx, [dict(scene=sc, spp=k, depth=8, seed=1000 + call)])[0]
+        e = p.add(film, k, time.perf_counter() - t0)
         call += 1
-        acc = film.astype(np.float64) * k if acc is None else acc + film.astype(np.float64) * k
-        n += k
-        e = relmse(acc / n, ref)
-        curve.append((t, e))
         if log:
-            log(f"  spp {n:5d}  t {t:8.3f}s  relMSE {e:.5f}")
-    return curve, n
+            log(f"  plain spp {p.n:5d}  t {p.t:8.3f}s  relMSE {e:.5f}")
+    return p
+
+
+def run_guided(ctx, sc, ref, args, sched, log):
+    lo, hi = sc.world_bounds()
+    g = R.Guide(ctx, lo, hi, seed=1)
+    g.clear_collected()
+    losses = []
+    p = Progressive(ref)
+    gd = sched["guide_depth"]
+    mode = 3 if sched["train_mode"] == "guided" else 2
+    for i in range(sched["train_passes"]):
+        k = sched["train_spp"]
+        t0 = time.perf_counter()
+        film = R.render(ctx, [dict(scene=sc, spp=k, depth=8, seed=50000 + i, guided=mode, guide_depth=gd)])[0]
+        losses.append(g.train_collected(sched["train_steps"], args.batch, args.lr))
+        g.clear_collected()
+        e = p.add(film, k, time.perf_counter() - t0)
+        if log:
+            log(f"  train spp {p.n:5d}  t {p.t:8.3f}s  relMSE {e:.5f}  loss {losses[-1]:.4f}")
+    online = sched["online_steps"]
+    call = 0
+    while p.n < args.max_spp:
+        k = min(args.spp_per_call, args.max_spp - p.n)
+        t0 = time.perf_counter()
+        film = R.render(ctx, [dict(scene=sc, spp=k, depth=8, seed=90000 + call, guided=3 if online else 1,
+                                   guide_depth=gd)])[0]
+        if online:
+            losses.append(g.train_collected(online, args.batch, args.lr))
+            g.clear_collected()
+        e = p.add(film, k, time.perf_counter() - t0)
+        call += 1
+        if log:
+            log(f"  guided spp {p.n:5d}  t {p.t:8.3f}s  relMSE {e:.5f}")
+    return p, losses
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--scene", choices=sorted(SCENES), default="terrain")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--ref-spp", type=int, default=1024)
     ap.add_argument("--max-spp", type=int, default=256)
     ap.add_argument("--spp-per-call", type=int, default=4)
-    ap.add_argument("--train-passes", type=int, default=16)
-    ap.add_argument("--train-steps", type=int, default=8)
     ap.add_argument("--batch", type=int, default=1 << 18)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--guide-depth", type=int, default=0, help="bounces that draw from the guide (0 = all)")
+    ap.add_argument("--schedules", default=json.dumps([DEFAULT_SCHEDULE]),
+                    help="JSON list of guided schedules (keys as in the module docstring)")
     ap.add_argument("--targets", default="0.01,0.003,0.001")
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     log = None if args.quiet else (lambda m: print(m, file=sys.stderr, flush=True))
+    schedules = [{**DEFAULT_SCHEDULE, **s} for s in json.loads(args.schedules)]
 
     ctx = R.Context(0)
-    desc = S.terrain_c2(args.width, args.height)
-    sc = R.scene_from_desc(ctx, desc)
+    sc = R.scene_from_desc(ctx, SCENES[args.scene](args.width, args.height))
     R.render(ctx, [dict(scene=sc, spp=1, depth=8, seed=1)])   # warm-up (code objects, pool)
 
     t0 = time.perf_counter()
@@ -120,53 +166,38 @@ def main() -> int:
     ref_err = float(np.mean(var / args.ref_spp / (ref * ref + EPS)))
     if log:
         log(f"reference: {args.ref_spp} spp in {ref_s:.1f} s, estimated relMSE {ref_err:.2e}")
-
-    if log:
-        log("plain:")
-    plain, _ = progressive(ctx, sc, ref, args.max_spp, args.spp_per_call, 1000, 0, log=log)
-
-    lo, hi = sc.world_bounds()
-    g = R.Guide(ctx, lo, hi, seed=1)
-    g.clear_collected()
-    losses = []
-
-    def train_pass(i):
-        def step():
-            t0 = time.perf_counter()
-            film = R.render(ctx, [dict(scene=sc, spp=1, depth=8, seed=50000 + i, guided=3,
-                                       guide_depth=args.guide_depth)])[0]
-            losses.append(g.train_collected(args.train_steps, args.batch, args.lr))
-            g.clear_collected()
-            return film, 1, time.perf_counter() - t0
-        return step
-
-    if log:
-        log("guided:")
-    guided, _ = progressive(ctx, sc, ref, args.max_spp, args.spp_per_call, 90000, 1,
-                            pre=[train_pass(i) for i in range(args.train_passes)], log=log,
-                            guide_depth=args.guide_depth)
+    plain = run_plain(ctx, sc, ref, args, log)
     targets = [float(x) for x in args.targets.split(",")]
-    tt = {}
-    for tgt in targets:
-        tp, tg = time_to(plain, tgt), time_to(guided, tgt)
-        tt[str(tgt)] = {"plain_s": tp, "guided_s": tg,
-                        "speedup": (tp / tg) if (tp is not None and tg is not None and tg > 0) else None}
-    out = {
-        "workload": f"c4 guiding terrain {args.width}x{args.height} depth 8",
-        "metric": "time-to-relMSE guided vs plain GPU PT",
-        "reference": {"spp": args.ref_spp, "accumulation": "fp64", "ref_relmse": ref_err},
-        "training": {"passes": args.train_passes, "steps_per_pass": args.train_steps, "batch": args.batch,
-                     "lr": args.lr, "guide_fraction": 0.5, "guide_depth": args.guide_depth,
-                     "loss_first": losses[0] if losses else None,
-                     "loss_last": losses[-1] if losses else None},
-        "max_spp": args.max_spp,
-        "time_to_target": tt,
-        "final_relmse": {"plain": plain[-1][1], "guided": guided[-1][1]},
-        "final_time_s": {"plain": plain[-1][0], "guided": guided[-1][0]},
-        "curve_plain": [[round(t, 4), e] for t, e in plain],
-        "curve_guided": [[round(t, 4), e] for t, e in guided],
-    }
-    print(json.dumps(out), flush=True)
+    for sched in schedules:
+        # warm-up of the guided code paths (graphs, inference) outside the timed run
+        g0 = R.Guide(ctx, *sc.world_bounds(), seed=1)
+        R.render(ctx, [dict(scene=sc, spp=1, depth=8, seed=7, guided=3, guide_depth=sched["guide_depth"])])
+        g0.train_collected(1, args.batch, args.lr)
+        g0.clear_collected()
+        if log:
+            log(f"guided {sched}:")
+        guided, losses = run_guided(ctx, sc, ref, args, sched, log)
+        tt = {}
+        for tgt in targets:
+            tp, tg = time_to(plain.curve, tgt), time_to(guided.curve, tgt)
+            tt[str(tgt)] = {"plain_s": tp, "guided_s": tg,
+                            "speedup": (tp / tg) if (tp is not None and tg is not None and tg > 0) else None}
+        out = {
+            "workload": f"c4 guiding {args.scene} {args.width}x{args.height} depth 8",
+            "metric": "time-to-relMSE guided vs plain GPU PT",
+            "reference": {"spp": args.ref_spp, "accumulation": "fp64", "ref_relmse": ref_err},
+            "schedule": sched,
+            "training": {"batch": args.batch, "lr": args.lr, "guide_fraction": 0.5,
+                         "loss_first": losses[0] if losses else None,
+                         "loss_last": losses[-1] if losses else None},
+            "max_spp": args.max_spp,
+            "time_to_target": tt,
+            "final_relmse": {"plain": plain.curve[-1][1], "guided": guided.curve[-1][1]},
+            "final_time_s": {"plain": plain.curve[-1][0], "guided": guided.curve[-1][0]},
+            "curve_plain": [[round(t, 4), e] for t, e in plain.curve],
+            "curve_guided": [[round(t, 4), e] for t, e in guided.curve],
+        }
+        print(json.dumps(out), flush=True)
     sc.close()
     ctx.close()
     return 0

@@ -198,17 +198,17 @@ def cornell_c0(width: int = 256, height: int = 256) -> SceneDesc:
     return b.build(cam, name="cornell555")
 
 
-def cornell_indirect(width: int = 256, height: int = 256) -> SceneDesc:
+def cornell_indirect(width: int = 256, height: int = 256, emission=(255.0, 180.0, 60.0)) -> SceneDesc:
     """The Cornell box of cornell_c0 with its light panel turned to face the ceiling: the
-    panel (130 x 105, emission (17,12,4)) hangs at y = 450 and emits upward only, an opaque
-    white shade of the same size sits just under it.  NEE reaches the panel from the ceiling
+    panel (130 x 105, 15x the emission of cornell_c0, so the image is about as bright) hangs
+    at y = 450 and emits upward only, an opaque white shade of the same size sits just under it.  NEE reaches the panel from the ceiling
     alone, so everything else in the room is lit indirectly, by the bright ceiling patch
     above the panel -- light that only the continuation direction can find."""
     b = _Builder()
     white = b.mat(diffuse((0.73, 0.73, 0.73)))
     red = b.mat(diffuse((0.65, 0.05, 0.05)))
     green = b.mat(diffuse((0.12, 0.45, 0.15)))
-    light = b.mat(emissive((17.0, 12.0, 4.0)))
+    light = b.mat(emissive(emission))
     b.quad((0, 0, 0), (555, 0, 0), (0, 0, 555), white)        # floor
     b.quad((0, 555, 0), (555, 0, 0), (0, 0, 555), white)      # ceiling
     b.quad((0, 0, 555), (555, 0, 0), (0, 555, 0), white)      # back
@@ -218,7 +218,7 @@ def cornell_indirect(width: int = 256, height: int = 256) -> SceneDesc:
     b.quad((213, 449, 227), (130, 0, 0), (0, 0, 105), white)  # shade under it
     b.box((165, 330, 165), 15.0, (265, 0, 295), white)
     b.box((165, 165, 165), -18.0, (130, 0, 65), white)
-    b.lights.append(area_light((278, 450, 279.5), (0, 0, 105), (130, 0, 0), (17, 12, 4), 1.0))
+    b.lights.append(area_light((278, 450, 279.5), (0, 0, 105), (130, 0, 0), emission, 1.0))
     cam = look_at((278, 278, -800), (278, 278, 0), (0, 1, 0), width, height, 40.0)
     return b.build(cam, name="cornell555-indirect")
 
