@@ -189,12 +189,14 @@ __global__ __launch_bounds__(256, 2) void guideInferKernel(GuideInferArgs a) {
             waveSync();
             if (valid[t]) {
                 const float* ov = stage + r * kGOutPad;
+                // hardware exp / rsq / rcp: the lobes are stored in FP16
                 float mx = -INFINITY;
 #pragma unroll
                 for (int k = 0; k < kGLobes; ++k) mx = fmaxf(mx, ov[5 * k + 4]);
                 float se = 0.0f;
 #pragma unroll
-                for (int k = 0; k < kGLobes; ++k) se += expf(ov[5 * k + 4] - mx);
+                for (int k = 0; k < kGLobes; ++k) se += __expf(ov[5 * k + 4] - mx);
+                const float rse = __builtin_amdgcn_rcpf(se);
                 uint32_t words[5];
 #pragma unroll
                 for (int a5 = 0; a5 < 5; ++a5) words[a5] = 0u;
@@ -202,10 +204,10 @@ __global__ __launch_bounds__(256, 2) void guideInferKernel(GuideInferArgs a) {
                 for (int j = 0; j < 2; ++j) {
                     const int k = 2 * g + j;
                     const float m0 = ov[5 * k], m1 = ov[5 * k + 1], m2 = ov[5 * k + 2];
-                    const float ln = fmaxf(sqrtf(m0 * m0 + m1 * m1 + m2 * m2), 1e-6f);
-                    const float vals[5] = {m0 / ln, m1 / ln, m2 / ln,
-                                           expf(fminf(fmaxf(ov[5 * k + 3], kGLogKappaMin), kGLogKappaMax)),
-                                           expf(ov[5 * k + 4] - mx) / se};
+                    const float il = __builtin_amdgcn_rsqf(fmaxf(m0 * m0 + m1 * m1 + m2 * m2, 1e-12f));
+                    const float vals[5] = {m0 * il, m1 * il, m2 * il,
+                                           __expf(fminf(fmaxf(ov[5 * k + 3], kGLogKappaMin), kGLogKappaMax)),
+                                           __expf(ov[5 * k + 4] - mx) * rse};
 #pragma unroll
                     for (int a5 = 0; a5 < 5; ++a5)
                         words[a5] |= uint32_t(__half_as_ushort(__float2half_rn(vals[a5]))) << (16 * j);
@@ -253,8 +255,8 @@ __device__ __forceinline__ void storeRows(uint16_t* grpBase, int f0, int t, u32x
     const v4f z = {0.0f, 0.0f, 0.0f, 0.0f};
     const v4f d = mfma32(op, id, z);
     uint2 w;
-    w.x = (__float_as_uint(d[0]) >> 16) | (__float_as_uint(d[1]) & 0xFFFF0000u);
-    w.y = (__float_as_uint(d[2]) >> 16) | (__float_as_uint(d[3]) & 0xFFFF0000u);
+    w.x = packBf2(d[0], d[1]);
+    w.y = packBf2(d[2], d[3]);
     *reinterpret_cast<uint2*>(grpBase + (f0 + (lane & 15)) * 32 + 16 * t + 4 * (lane >> 4)) = w;
 }
 
@@ -801,7 +803,8 @@ int guideTrainDevice(rbr_ctx* ctx, const float4* samples, uint32_t stride, uint3
         static const uint32_t primes[5] = {2654435761u, 2246822519u, 3266489917u, 668265263u, 374761393u};
         g.trainRng = g.trainRng * 6364136223846793005ull + 1442695040888963407ull;
         a.total = total;
-        a.permMul = primes[(g.trainRng >> 33) % 5];
+        a.permMul = primes[(g.trainRng >> 33) % 5] % total;
+        a.invTotal = 1.0 / double(total);
         a.permAdd = uint32_t((g.trainRng >> 11) % total);
     }
     if (int rc = launchLossGrad(ctx, g, a)) return rc;

@@ -16,16 +16,19 @@ namespace rbr {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
-    return uint16_t(u >> 16);
-}
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f)); }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint32_t packBf2(float lo, float hi) { return uint32_t(f2bf(lo)) | (uint32_t(f2bf(hi)) << 16); }
+// Two FP32 values -> BF16 pair (lo in bits 0-15), one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t packBf2(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{lo, hi}, bf16x2));
+}
 
 // Features of one 32-wide k-block for lane group g (0..3) of a query with scalars s9: word w
 // is the BF16 (sin, cos) pair of local pair q = 4 kb + w (guidePair's order).  Scalar j = q / 3
@@ -84,9 +87,11 @@ struct GuideTrainArgs {
     const float4* samples;    // per sample: (pos, c) (normal, -) (wo, -) (dir, -) [+ stride padding]
     uint32_t stride;          // float4 per sample
     uint32_t n;               // samples in this batch
-    // Batch sample s reads record (permAdd + s * permMul) mod total; permMul is a prime that
-    // does not divide total, so a batch never repeats a record.  total = 0: record s.
-    uint32_t total, permMul, permAdd;
+    // Batch sample s reads record (permAdd + s * permMul) mod total; permMul is a prime above
+    // total reduced mod total, coprime to it, so a batch never repeats a record (s < 2^26 and
+    // total <= 2^27 keep s * permMul below 2^53).  total = 0: record s.
+    uint32_t total, permMul, permAdd;   // permMul already reduced mod total
+    double invTotal;                    // 1 / total
     float cmax;               // training targets are clamped to cmax before scaling
     size_t cap;               // samples the activation arrays hold (multiple of 32)
     float scale;              // multiplies c (1 / (batch * mean c))
@@ -143,7 +148,17 @@ __device__ __forceinline__ bool guideQuery(const GuideInferArgs& a, uint32_t i, 
 }
 
 __device__ __forceinline__ void guideSample(const GuideTrainArgs& a, uint32_t i, float* s9, float* dir, float& c) {
-    const uint32_t r = a.total ? uint32_t((uint64_t(a.permAdd) + uint64_t(i) * a.permMul) % a.total) : i;
+    uint32_t r = i;
+    if (a.total) {
+        // i * permMul < 2^53 is exact in FP64; the quotient from the reciprocal is off by at
+        // most one, which the two corrections take back
+        const double x = double(i) * double(a.permMul), tot = double(a.total);
+        double m = x - floor(x * a.invTotal) * tot;
+        if (m < 0.0) m += tot;
+        if (m >= tot) m -= tot;
+        r = uint32_t(m) + a.permAdd;
+        if (r >= a.total) r -= a.total;
+    }
     const float4* s = a.samples + size_t(r) * a.stride;
     const float4 p = s[0], n = s[1], wo = s[2], d = s[3];
     const float pp[3] = {p.x, p.y, p.z}, nn[3] = {n.x, n.y, n.z}, ww[3] = {wo.x, wo.y, wo.z};
@@ -176,41 +191,45 @@ __device__ __forceinline__ void decodeLobe(const float* o, int k, GLobe& lobe) {
 }
 
 // Loss -c*log p(dir) of one sample and its gradient w.r.t. the 10 outputs of lobes
-// 2*part and 2*part+1 (layout per lobe: mu 3, log kappa, logit).  Returns the loss.
+// 2*part and 2*part+1 (layout per lobe: mu 3, log kappa, logit).  Returns the loss.  Hardware
+// exp / log / rsq where only relative accuracy matters (the gradient is rounded to BF16);
+// log(1 - e^-x) keeps expm1 / log1p.
 __device__ __forceinline__ float outputGrad(const float* o, const float* dir, float c, int part, float* grad) {
     float a[kGLobes], wlog[kGLobes];
     float mx = -INFINITY;
     for (int k = 0; k < kGLobes; ++k) mx = fmaxf(mx, o[5 * k + 4]);
     float se = 0.0f;
-    for (int k = 0; k < kGLobes; ++k) se += expf(o[5 * k + 4] - mx);
-    const float lse = mx + logf(se);
+    for (int k = 0; k < kGLobes; ++k) se += __expf(o[5 * k + 4] - mx);
+    const float lse = mx + __logf(se);
     float amax = -INFINITY;
     for (int k = 0; k < kGLobes; ++k) {
-        const float ln = fmaxf(sqrtf(o[5 * k] * o[5 * k] + o[5 * k + 1] * o[5 * k + 1] + o[5 * k + 2] * o[5 * k + 2]), 1e-6f);
-        const float d = (o[5 * k] * dir[0] + o[5 * k + 1] * dir[1] + o[5 * k + 2] * dir[2]) / ln;
+        const float il = __builtin_amdgcn_rsqf(
+            fmaxf(o[5 * k] * o[5 * k] + o[5 * k + 1] * o[5 * k + 1] + o[5 * k + 2] * o[5 * k + 2], 1e-12f));
+        const float d = (o[5 * k] * dir[0] + o[5 * k + 1] * dir[1] + o[5 * k + 2] * dir[2]) * il;
         const float l = fminf(fmaxf(o[5 * k + 3], kGLogKappaMin), kGLogKappaMax);
-        const float kap = expf(l);
+        const float kap = __expf(l);
         wlog[k] = o[5 * k + 4] - lse;
         a[k] = wlog[k] + l - kLog2Pi - log1mexp(2.0f * kap) + kap * (d - 1.0f);
         amax = fmaxf(amax, a[k]);
     }
     float sa = 0.0f;
-    for (int k = 0; k < kGLobes; ++k) sa += expf(a[k] - amax);
-    const float logp = amax + logf(sa);
+    for (int k = 0; k < kGLobes; ++k) sa += __expf(a[k] - amax);
+    const float logp = amax + __logf(sa);
     for (int j = 0; j < 2; ++j) {
         const int k = 2 * part + j;
-        const float r = expf(a[k] - logp);
-        const float w = expf(wlog[k]);
-        const float ln = fmaxf(sqrtf(o[5 * k] * o[5 * k] + o[5 * k + 1] * o[5 * k + 1] + o[5 * k + 2] * o[5 * k + 2]), 1e-6f);
-        const float mu[3] = {o[5 * k] / ln, o[5 * k + 1] / ln, o[5 * k + 2] / ln};
+        const float r = __expf(a[k] - logp);
+        const float w = __expf(wlog[k]);
+        const float il = __builtin_amdgcn_rsqf(
+            fmaxf(o[5 * k] * o[5 * k] + o[5 * k + 1] * o[5 * k + 1] + o[5 * k + 2] * o[5 * k + 2], 1e-12f));
+        const float mu[3] = {o[5 * k] * il, o[5 * k + 1] * il, o[5 * k + 2] * il};
         const float d = mu[0] * dir[0] + mu[1] * dir[1] + mu[2] * dir[2];
         const float lraw = o[5 * k + 3];
         const bool clamped = lraw < kGLogKappaMin || lraw > kGLogKappaMax;
-        const float kap = expf(fminf(fmaxf(lraw, kGLogKappaMin), kGLogKappaMax));
+        const float kap = __expf(fminf(fmaxf(lraw, kGLogKappaMin), kGLogKappaMax));
         const float x = 2.0f * kap;
         const float ratio = x < 1e-4f ? 1.0f - 0.5f * x : x / expm1f(x);   // 2k e^{-2k} / (1 - e^{-2k})
         const float dl = clamped ? 0.0f : -c * r * (1.0f - ratio + kap * (d - 1.0f));
-        const float gm = -c * r * kap / ln;
+        const float gm = -c * r * kap * il;
         for (int q = 0; q < 3; ++q) grad[5 * j + q] = gm * (dir[q] - d * mu[q]);
         grad[5 * j + 3] = dl;
         grad[5 * j + 4] = c * (w - r);
