@@ -23,6 +23,7 @@ Scenes (depth 8):
     guide_depth   bounces that draw from the guide (0 = all)
   Every pass, training passes included, is unbiased and enters the running mean; training
   time is part of the guided time.  The guide starts from the same initialisation each run.
+  The record capacity (--reserve) holds a whole 1080p pass, so every bounce is recorded.
 * Times are host wall-clock around the render and training calls (each render call returns
   its film, so they include the film download); relMSE evaluation is not timed.
 
@@ -45,7 +46,7 @@ import rbr_amd as R  # noqa: E402
 import scenes as S  # noqa: E402
 
 EPS = 1e-2
-DEFAULT_SCHEDULE = {"train_passes": 16, "train_spp": 1, "train_mode": "guided", "train_steps": 8,
+DEFAULT_SCHEDULE = {"train_passes": 8, "train_spp": 2, "train_mode": "plain", "train_steps": 16,
                     "online_steps": 0, "guide_depth": 0}
 SCENES = {"terrain": S.terrain_c2, "cornell_indirect": S.cornell_indirect}
 
@@ -105,6 +106,7 @@ def run_plain(ctx, sc, ref, args, log):
 def run_guided(ctx, sc, ref, args, sched, log):
     lo, hi = sc.world_bounds()
     g = R.Guide(ctx, lo, hi, seed=1)
+    g.reserve(args.reserve)
     g.clear_collected()
     losses = []
     p = Progressive(ref)
@@ -146,6 +148,9 @@ def main() -> int:
     ap.add_argument("--spp-per-call", type=int, default=4)
     ap.add_argument("--batch", type=int, default=1 << 18)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--reserve", type=int, default=1 << 24,
+                    help="record capacity: records of a pass beyond it are dropped, and records are "
+                         "appended bounce by bounce, so it must hold a whole pass to cover deep vertices")
     ap.add_argument("--schedules", default=json.dumps([DEFAULT_SCHEDULE]),
                     help="JSON list of guided schedules (keys as in the module docstring)")
     ap.add_argument("--targets", default="0.01,0.003,0.001")
