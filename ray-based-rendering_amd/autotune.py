@@ -185,10 +185,10 @@ def main() -> int:
     ctx = R.Context(0)
     name, cus = ctx.device_info()
     if args.quick:
-        grid = dict(widths=[4, 8, 4 | R.BVH_QUANTIZED, 8 | R.BVH_QUANTIZED], leaves=[4], modes=[0, 1], depths=[8, 16],
-                    waves=[8, 16, 24, 32])
+        grid = dict(widths=[4, 8, 4 | R.BVH_QUANTIZED, 8 | R.BVH_QUANTIZED], leaves=[4], modes=[0, 2, 3, 4],
+                    depths=[8, 16], waves=[16, 24, 32])
     else:
-        grid = dict(widths=[4, 8, 4 | R.BVH_QUANTIZED, 8 | R.BVH_QUANTIZED], leaves=[2, 4, 8], modes=[0, 1],
+        grid = dict(widths=[4, 8, 4 | R.BVH_QUANTIZED, 8 | R.BVH_QUANTIZED], leaves=[2, 4, 8], modes=[0, 1, 2, 3, 4],
                     depths=[8, 12, 16, 24, 32], waves=[4, 8, 16, 24, 32])
     out = {"device": name, "cus": cus, "workload": f"icosphere n={args.n_sub} ({len(tris)} tris), {args.rays} rays",
            "time": time.strftime("%Y-%m-%dT%H:%M:%S")}

@@ -39,6 +39,7 @@ struct PersistArgs {
     uint32_t* counter;
     uint2* spill;             // [grid][spillDepth][64]
     uint32_t spillDepth;
+    uint32_t refill;          // 0: waves take 64-ray batches; else per-lane fetch once this many lanes idle
 };
 
 // ldsDepth in {8, 12, 16, 24, 32}; p == nullptr selects the one-ray-per-lane kernel (which

@@ -217,102 +217,96 @@ struct PrivateSpillStack {
     }
 };
 
-// One ray through the wide BVH and the unbounded list.  Returns the lowest-id closest hit
-// (bestT/bestId) or, for ANY, whether any primitive is hit in [tmin, tmax].
-//
-// Loop structure ("while-while" with postponed leaves, after Aila & Laine 2009): a lane
-// walks interior nodes until it meets a leaf, parks it, and keeps walking until every lane of
-// the wave that still traverses holds a parked leaf; then the wave tests leaves together.
-// Lanes in a wave so spend their iterations on the same kind of work.  Parking a leaf can only
-// add node visits (culling by the closest hit comes later), never change the result: the
-// closest hit is the minimum over every primitive tested, and every entry that can still
-// hold a hit at least as close is visited.
+// One round of the traversal loop of one ray ("while-while" with postponed leaves, after Aila
+// & Laine 2009): a lane walks interior nodes until it meets a leaf, parks it, and keeps
+// walking until every lane of the wave that still traverses holds a parked leaf; then the
+// wave tests leaves together.  Lanes in a wave so spend their iterations on the same kind of
+// work.  Parking a leaf can only add node visits (culling by the closest hit comes later),
+// never change the result: the closest hit is the minimum over every primitive tested, and
+// every entry that can still hold a hit at least as close is visited.  On return, ref ==
+// kEmptyD (or an any-hit occlusion) means the ray is done.
 template <int W, bool Q, bool ANY, bool MIXED, typename Stack>
-__device__ __forceinline__ void traceRay(const TraceArgs& a, const DRay& r, uint32_t root, Stack& st, float& bestT,
-                                         uint32_t& bestId, bool& occluded) {
-    bestT = r.tmax;
-    bestId = RBR_MISS_ID;
-    occluded = false;
-    if (a.nodes && root != kEmptyD) {
-        // next stack entry that can still hold a hit (closest hit: entry distance <= best t)
-        auto popNext = [&]() -> uint32_t {
-            while (st.sp > 0) {
-                const uint2 e = st.pop();
-                if (ANY || u2f(e.y) <= bestT) return e.x;
+__device__ __forceinline__ void traceRound(const TraceArgs& a, const DRay& r, Stack& st, uint32_t& ref, uint32_t& leaf,
+                                           float& bestT, uint32_t& bestId, bool& occluded) {
+    // next stack entry that can still hold a hit (closest hit: entry distance <= best t)
+    auto popNext = [&]() -> uint32_t {
+        while (st.sp > 0) {
+            const uint2 e = st.pop();
+            if (ANY || u2f(e.y) <= bestT) return e.x;
+        }
+        return kEmptyD;
+    };
+    while (ref != kEmptyD && !isLeafD(ref)) {
+        float tc[W];
+        uint32_t rc[W];
+        if (Q) childrenQ<W>(a.nodes, ref, r, bestT, tc, rc);
+        else childrenF<W>(a.nodes, ref, r, bestT, tc, rc);
+        if (!ANY) {
+            // Sorting network, ascending entry distance (misses sort last as +inf).
+            if (W == 4) {
+                cswap(tc[0], rc[0], tc[1], rc[1]); cswap(tc[2], rc[2], tc[3], rc[3]);
+                cswap(tc[0], rc[0], tc[2], rc[2]); cswap(tc[1], rc[1], tc[3], rc[3]);
+                cswap(tc[1], rc[1], tc[2], rc[2]);
+            } else {
+                // Optimal 8-input network (19 comparators, depth 6).
+                constexpr int net[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6},
+                                            {3, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}, {2, 4}, {3, 5},
+                                            {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+                for (int c = 0; c < 19; ++c)
+                    cswap(tc[net[c][0]], rc[net[c][0]], tc[net[c][1]], rc[net[c][1]]);
             }
-            return kEmptyD;
-        };
-        uint32_t ref = root, leaf = kEmptyD;
-        while (true) {
-            while (ref != kEmptyD && !isLeafD(ref)) {
-                float tc[W];
-                uint32_t rc[W];
-                if (Q) childrenQ<W>(a.nodes, ref, r, bestT, tc, rc);
-                else childrenF<W>(a.nodes, ref, r, bestT, tc, rc);
-                if (!ANY) {
-                    // Sorting network, ascending entry distance (misses sort last as +inf).
-                    if (W == 4) {
-                        cswap(tc[0], rc[0], tc[1], rc[1]); cswap(tc[2], rc[2], tc[3], rc[3]);
-                        cswap(tc[0], rc[0], tc[2], rc[2]); cswap(tc[1], rc[1], tc[3], rc[3]);
-                        cswap(tc[1], rc[1], tc[2], rc[2]);
-                    } else {
-                        // Optimal 8-input network (19 comparators, depth 6).
-                        constexpr int net[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6},
-                                                    {3, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}, {2, 4}, {3, 5},
-                                                    {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
+        }
+        int cnt = 0;
 #pragma unroll
-                        for (int c = 0; c < 19; ++c)
-                            cswap(tc[net[c][0]], rc[net[c][0]], tc[net[c][1]], rc[net[c][1]]);
-                    }
-                }
-                int cnt = 0;
+        for (int k = 0; k < W; ++k) cnt += rc[k] != kEmptyD ? 1 : 0;
+        if (cnt == 0) {
+            ref = popNext();
+        } else if (ANY) {
+            uint32_t first = kEmptyD;
 #pragma unroll
-                for (int k = 0; k < W; ++k) cnt += rc[k] != kEmptyD ? 1 : 0;
-                if (cnt == 0) {
-                    ref = popNext();
-                } else if (ANY) {
-                    uint32_t first = kEmptyD;
-#pragma unroll
-                    for (int k = 0; k < W; ++k) {
-                        if (rc[k] == kEmptyD) continue;
-                        if (first == kEmptyD) first = rc[k];
-                        else st.push(rc[k], tc[k]);
-                    }
-                    ref = first;
-                } else {
-#pragma unroll
-                    for (int k = W - 1; k >= 1; --k)
-                        if (k < cnt) st.push(rc[k], tc[k]);
-                    ref = rc[0];
-                }
-                if (ref != kEmptyD && isLeafD(ref) && leaf == kEmptyD) {
-                    leaf = ref;
-                    ref = popNext();
-                }
-                if (__all(leaf != kEmptyD ? 1 : 0)) break;
+            for (int k = 0; k < W; ++k) {
+                if (rc[k] == kEmptyD) continue;
+                if (first == kEmptyD) first = rc[k];
+                else st.push(rc[k], tc[k]);
             }
-            // Leaves: the parked one, then ref for as long as it is a leaf.
-            while (leaf != kEmptyD) {
-                const uint32_t s = leafStartD(leaf), c = leafCountD(leaf);
-                for (uint32_t k = 0; k < c; ++k) {
-                    float t;
-                    uint32_t id;
-                    if (testPrim<MIXED>(a, a.prims + size_t(s + k) * 3, r, bestT, t, id)) {
-                        if (ANY) { occluded = true; break; }
-                        if (t < bestT || id < bestId) { bestT = t; bestId = id; }
-                    }
-                }
-                if (ANY && occluded) break;
-                leaf = kEmptyD;
-                if (ref != kEmptyD && isLeafD(ref)) {
-                    leaf = ref;
-                    ref = popNext();
-                }
+            ref = first;
+        } else {
+#pragma unroll
+            for (int k = W - 1; k >= 1; --k)
+                if (k < cnt) st.push(rc[k], tc[k]);
+            ref = rc[0];
+        }
+        if (ref != kEmptyD && isLeafD(ref) && leaf == kEmptyD) {
+            leaf = ref;
+            ref = popNext();
+        }
+        if (__all(leaf != kEmptyD ? 1 : 0)) break;
+    }
+    // Leaves: the parked one, then ref for as long as it is a leaf.
+    while (leaf != kEmptyD) {
+        const uint32_t s = leafStartD(leaf), c = leafCountD(leaf);
+        for (uint32_t k = 0; k < c; ++k) {
+            float t;
+            uint32_t id;
+            if (testPrim<MIXED>(a, a.prims + size_t(s + k) * 3, r, bestT, t, id)) {
+                if (ANY) { occluded = true; break; }
+                if (t < bestT || id < bestId) { bestT = t; bestId = id; }
             }
-            if ((ANY && occluded) || ref == kEmptyD) break;
+        }
+        if (ANY && occluded) break;
+        leaf = kEmptyD;
+        if (ref != kEmptyD && isLeafD(ref)) {
+            leaf = ref;
+            ref = popNext();
         }
     }
-    // Unbounded primitives (infinite planes), tested linearly.
+}
+
+// Unbounded primitives (infinite planes), tested linearly after the BVH.
+template <bool ANY>
+__device__ __forceinline__ void tracePlanes(const TraceArgs& a, const DRay& r, float& bestT, uint32_t& bestId,
+                                            bool& occluded) {
     for (uint32_t k = 0; k < a.nplanes && !(ANY && occluded); ++k) {
         float4 pl = a.planes[k];
         float t;
@@ -322,6 +316,23 @@ __device__ __forceinline__ void traceRay(const TraceArgs& a, const DRay& r, uint
             if (t < bestT || id < bestId) { bestT = t; bestId = id; }
         }
     }
+}
+
+// One ray through the wide BVH and the unbounded list.  Returns the lowest-id closest hit
+// (bestT/bestId) or, for ANY, whether any primitive is hit in [tmin, tmax].
+template <int W, bool Q, bool ANY, bool MIXED, typename Stack>
+__device__ __forceinline__ void traceRay(const TraceArgs& a, const DRay& r, uint32_t root, Stack& st, float& bestT,
+                                         uint32_t& bestId, bool& occluded) {
+    bestT = r.tmax;
+    bestId = RBR_MISS_ID;
+    occluded = false;
+    if (a.nodes && root != kEmptyD) {
+        uint32_t ref = root, leaf = kEmptyD;
+        do {
+            traceRound<W, Q, ANY, MIXED>(a, r, st, ref, leaf, bestT, bestId, occluded);
+        } while (!((ANY && occluded) || ref == kEmptyD));
+    }
+    tracePlanes<ANY>(a, r, bestT, bestId, occluded);
 }
 
 template <bool ANY>
@@ -362,6 +373,62 @@ __global__ __launch_bounds__(64) void traceKernelPersistent(TraceArgs a, Persist
     }
 }
 
+// Persistent waves with per-lane ray fetch: a lane whose ray is done goes idle, and once at
+// least p.refill lanes of the wave are idle the wave takes that many new rays from the device
+// counter (one atomic) and hands them to its idle lanes, so a wave no longer waits for its
+// slowest ray before taking work (incoherent rays vary several-fold in node count).  Between
+// refills each live lane runs one traversal round.  Every wave exits once the counter has
+// passed the ray count and all of its lanes are idle.
+template <int W, bool Q, int LDS, bool ANY, bool MIXED>
+__global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArgs p) {
+    __shared__ uint2 ldsStack[LDS * 64];
+    GlobalSpillStack<LDS> st;
+    st.lds = ldsStack;
+    st.spill = p.spill + size_t(blockIdx.x) * size_t(p.spillDepth) * 64;
+    const uint32_t n = a.nDev ? *a.nDev : a.n;
+    const uint32_t lane = threadIdx.x;
+    bool alive = false, more = true, occ = false;
+    uint32_t i = 0, ref = kEmptyD, leaf = kEmptyD, bestId = RBR_MISS_ID;
+    float bestT = 0.0f;
+    DRay r{};
+    while (true) {
+        const uint64_t idle = __ballot(alive ? 0 : 1);
+        const uint32_t nIdle = uint32_t(__popcll(idle));
+        if (more && nIdle >= p.refill) {
+            uint32_t base = 0u;
+            if (lane == 0) base = atomicAdd(p.counter, nIdle);
+            base = __shfl(base, 0);
+            if (base + nIdle >= n) more = false;
+            if (!alive) {
+                const uint32_t idx = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+                if (idx < n) {
+                    i = idx;
+                    r = loadRay(a.rays, idx);
+                    bestT = r.tmax;
+                    bestId = RBR_MISS_ID;
+                    occ = false;
+                    st.sp = 0;
+                    ref = a.nodes ? (a.rootOf ? a.rootOf[idx] : 0u) : kEmptyD;
+                    leaf = kEmptyD;
+                    alive = true;
+                }
+            }
+        }
+        if (!__any(alive ? 1 : 0)) {
+            if (!more) break;
+            continue;
+        }
+        if (alive) {
+            if (ref != kEmptyD || leaf != kEmptyD) traceRound<W, Q, ANY, MIXED>(a, r, st, ref, leaf, bestT, bestId, occ);
+            if ((ANY && occ) || (ref == kEmptyD && leaf == kEmptyD)) {
+                tracePlanes<ANY>(a, r, bestT, bestId, occ);
+                storeResult<ANY>(a, i, bestT, bestId, occ);
+                alive = false;
+            }
+        }
+    }
+}
+
 __global__ void resetCounterKernel(uint32_t* c) { *c = 0u; }
 
 // One ray per lane, stack overflow in private memory (kept for comparison and for callers
@@ -383,7 +450,11 @@ __global__ __launch_bounds__(64) void traceKernelLane(TraceArgs a) {
 
 template <int W, bool Q, int LDS, bool ANY>
 hipError_t launchT(const TraceArgs& a, const PersistArgs* p, unsigned grid, hipStream_t s) {
-    if (p) {
+    if (p && p->refill) {
+        hipLaunchKernelGGL(resetCounterKernel, dim3(1), dim3(1), 0, s, p->counter);
+        if (a.mixed) hipLaunchKernelGGL((traceKernelDynamic<W, Q, LDS, ANY, true>), dim3(grid), dim3(64), 0, s, a, *p);
+        else hipLaunchKernelGGL((traceKernelDynamic<W, Q, LDS, ANY, false>), dim3(grid), dim3(64), 0, s, a, *p);
+    } else if (p) {
         hipLaunchKernelGGL(resetCounterKernel, dim3(1), dim3(1), 0, s, p->counter);
         if (a.mixed) hipLaunchKernelGGL((traceKernelPersistent<W, Q, LDS, ANY, true>), dim3(grid), dim3(64), 0, s, a, *p);
         else hipLaunchKernelGGL((traceKernelPersistent<W, Q, LDS, ANY, false>), dim3(grid), dim3(64), 0, s, a, *p);

@@ -151,3 +151,21 @@ def test_render_tuning_does_not_change_images(ctx, pool, graphs, sort):
         R.set_render_tuning(ctx)   # defaults
     with pytest.raises(R.RbrError):
         R.set_render_tuning(ctx, 1000, True, False)
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+def test_trace_fetch_modes_do_not_change_images(ctx, mode):
+    """The renderer's traversals run on the device-sized queues; with per-lane ray fetch they
+    return the same hits, so every pixel is bit-identical."""
+    say(f"case: render with trace mode {mode}")
+    a = R.scene_from_desc(ctx, S.cornell_c0(40, 30))
+    f = R.scene_from_desc(ctx, S.fog_jobs_c3(scale=0.02)[9]["scene"])
+    jobs = [dict(scene=a, spp=16, depth=8, seed=3), dict(scene=f, spp=4, depth=5, seed=5, integrator=R.VOLPATH)]
+    base = R.render(ctx, jobs)
+    ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, 8, 32))
+    try:
+        out = R.render(ctx, jobs)
+    finally:
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
+    for k in range(len(jobs)):
+        assert np.array_equal(out[k], base[k]), k

@@ -155,10 +155,12 @@ def test_c1_full_size(ctx, width):
 _ORACLE: dict = {}
 
 
-@pytest.mark.parametrize("mode,depth,waves", [(0, 8, 1), (0, 12, 32), (0, 32, 16), (1, 8, 16), (1, 24, 16)])
+@pytest.mark.parametrize("mode,depth,waves", [(0, 8, 1), (0, 12, 32), (0, 32, 16), (1, 8, 16), (1, 24, 16),
+                                               (2, 8, 32), (3, 8, 1), (3, 12, 16), (4, 32, 16)])
 def test_launch_tunings_identical(ctx, mode, depth, waves):
     """Every launch configuration (persistent waves with tiny/large LDS stacks and so
-    deep global spills, or one ray per lane) returns the same bit-exact hits."""
+    deep global spills, one ray per lane, or persistent waves whose idle lanes fetch new
+    rays) returns the same bit-exact hits."""
     say(f"case: tuning mode {mode} lds {depth} waves {waves}")
     tris = R.icosphere(48, 100.0, 0.1, 0xC1)
     lo, hi = R.mesh_bounds(tris)
