@@ -302,6 +302,21 @@ int rbr_guide_collected(rbr_ctx* ctx, uint64_t* count, double* weight_sum);
 int rbr_guide_train_collected(rbr_ctx* ctx, int steps, uint32_t batch, float lr, double* loss);
 int rbr_guide_clear_collected(rbr_ctx* ctx);
 
+/* ---- Film output (host only; films are float RGB, row-major from the top row) ----------- */
+/* 8-bit conversion of a mean-radiance film, as Film::toRGB8 / Film::toRGB8ToneMapped
+ * (film.cpp:27-63): per channel v (times exposure, then v / (v + 1) when tone_map),
+ * pow(v, 1/gamma) clamped to [0, 1] (a NaN becomes 1, as the reference's clamp does),
+ * * 255 + 0.5, truncated.  out holds width * height * 3 bytes. */
+int rbr_film_to_rgb8(const float* rgb, int width, int height, float gamma, int tone_map, float exposure,
+                     unsigned char* out);
+/* PNG of an 8-bit RGB image, the default format of ImageExporter::writeRGB8
+ * (image_exporter.cpp:13-39); stored deflate blocks, any PNG reader decodes it. */
+int rbr_film_write_png(const char* path, const unsigned char* rgb8, int width, int height);
+/* HDR film as PFM (little-endian FP32, rows bottom-up); no reference counterpart, the
+ * reference writes 8-bit images only.  read: rgb == NULL returns the size only. */
+int rbr_film_write_pfm(const char* path, const float* rgb, int width, int height);
+int rbr_film_read_pfm(const char* path, int* width, int* height, float* rgb, size_t capacity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,8 +67,18 @@ def _load_ref():
         _sig(lib.ref_brute_closest, [_vp, _vp, _sz, _vp, C.c_int])
         _sig(lib.ref_brute_any, [_vp, _vp, _sz, _vp, C.c_int])
         _sig(lib.ref_max_threads, [], C.c_int)
+        _sig(lib.ref_film_to_rgb8, [_vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, _vp])
         _ref = lib
     return _ref
+
+
+def ref_film_to_rgb8(film, gamma: float = 2.2, tone_map: bool = False, exposure: float = 1.0):
+    """The reference's Film::toRGB8 / toRGB8ToneMapped (film.cpp:33-63) on an (H, W, 3) film."""
+    import numpy as np
+    f = np.ascontiguousarray(film, np.float32)
+    out = np.empty(f.shape, np.uint8)
+    _load_ref().ref_film_to_rgb8(f.ctypes.data, f.shape[1], f.shape[0], gamma, int(tone_map), exposure, out.ctypes.data)
+    return out
 
 
 def _arr(a, cols):

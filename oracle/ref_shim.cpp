@@ -4,6 +4,7 @@
 // CPU baseline; the product never links it.  No reference source is copied here — this file
 // only instantiates the reference's types through their public headers.
 #include "render/acceleration/bvh.h"
+#include "render/core/film.h"
 #include "render/geometry/plane.h"
 #include "render/geometry/quad.h"
 #include "render/geometry/sphere.h"
@@ -151,5 +152,15 @@ void ref_brute_any(void* h, const rbr_ray* rays, size_t n, uint8_t* occ, int nth
 }
 
 int ref_max_threads(void) { return omp_get_max_threads(); }
+
+// The reference's Film::toRGB8 / toRGB8ToneMapped on a film whose pixels are set by one
+// addSample of weight 1 (getColor returns the value exactly).
+void ref_film_to_rgb8(const float* rgb, int w, int h, float gamma, int tone_map, float exposure, unsigned char* out) {
+    Film film(w, h);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) film.addSample(x, y, v3(rgb + (size_t(y) * w + x) * 3), 1.0f);
+    const std::vector<unsigned char> v = tone_map ? film.toRGB8ToneMapped(exposure, gamma) : film.toRGB8(gamma);
+    for (size_t i = 0; i < v.size(); ++i) out[i] = v[i];
+}
 
 }  // extern "C"

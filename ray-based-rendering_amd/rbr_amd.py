@@ -87,6 +87,11 @@ _sig(_lib.rbr_time_trace, [_vp, C.c_int, _vp, _sz, _vp, C.c_int, C.POINTER(C.c_f
 _sig(_lib.rbr_host_trace_closest, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
 _sig(_lib.rbr_host_trace_any, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
 
+_sig(_lib.rbr_film_to_rgb8, [_vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, _vp])
+_sig(_lib.rbr_film_write_png, [C.c_char_p, _vp, C.c_int, C.c_int])
+_sig(_lib.rbr_film_write_pfm, [C.c_char_p, _vp, C.c_int, C.c_int])
+_sig(_lib.rbr_film_read_pfm, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _vp, _sz])
+
 _sig(_wl.rbrw_icosphere, [C.c_int, C.c_float, C.c_float, C.c_uint64, _vp], _sz)
 _sig(_wl.rbrw_noise3, [C.c_float, C.c_float, C.c_float, C.c_uint64], C.c_float)
 _sig(_wl.rbrw_rays_uniform, [_vp, _vp, C.c_float, _sz, C.c_uint64, C.c_float, C.c_float, _vp], None)
@@ -297,6 +302,48 @@ class Scene:
 
 
 # ---------------------------------------------------------------- workloads
+
+def _film(film) -> np.ndarray:
+    f = np.ascontiguousarray(film, np.float32)
+    if f.ndim != 3 or f.shape[2] != 3 or f.shape[0] < 1 or f.shape[1] < 1:
+        raise ValueError("film must be (H, W, 3)")
+    return f
+
+
+def film_to_rgb8(film, gamma: float = 2.2, tone_map: bool = False, exposure: float = 1.0) -> np.ndarray:
+    """Film::toRGB8 / toRGB8ToneMapped (film.cpp:27-63) of an (H, W, 3) mean-radiance film."""
+    f = _film(film)
+    out = np.empty(f.shape, np.uint8)
+    if _lib.rbr_film_to_rgb8(_ptr(f), f.shape[1], f.shape[0], gamma, int(tone_map), exposure, _ptr(out)):
+        raise ValueError("rbr_film_to_rgb8: invalid arguments")
+    return out
+
+
+def write_png(path: str, rgb8) -> None:
+    """8-bit RGB PNG, ImageExporter::writeRGB8's default format (image_exporter.cpp:13-39)."""
+    a = np.ascontiguousarray(rgb8, np.uint8)
+    if a.ndim != 3 or a.shape[2] != 3:
+        raise ValueError("image must be (H, W, 3) uint8")
+    if _lib.rbr_film_write_png(os.fsencode(path), _ptr(a), a.shape[1], a.shape[0]):
+        raise OSError(f"cannot write {path}")
+
+
+def write_pfm(path: str, film) -> None:
+    """HDR film as PFM (FP32, no reference counterpart)."""
+    f = _film(film)
+    if _lib.rbr_film_write_pfm(os.fsencode(path), _ptr(f), f.shape[1], f.shape[0]):
+        raise OSError(f"cannot write {path}")
+
+
+def read_pfm(path: str) -> np.ndarray:
+    w, h = C.c_int(0), C.c_int(0)
+    if _lib.rbr_film_read_pfm(os.fsencode(path), C.byref(w), C.byref(h), None, 0):
+        raise OSError(f"cannot read {path} as an RGB PFM")
+    out = np.empty((h.value, w.value, 3), np.float32)
+    if _lib.rbr_film_read_pfm(os.fsencode(path), C.byref(w), C.byref(h), _ptr(out), out.size):
+        raise OSError(f"truncated PFM {path}")
+    return out
+
 
 def icosphere(n: int = 224, radius: float = 100.0, amp: float = 0.1, seed: int = 0xC1) -> np.ndarray:
     """Displaced geodesic icosphere, 20*n*n triangles, (T, 9) float32."""
