@@ -186,9 +186,21 @@ int rbr_scene_set_geometry(rbr_scene* scene,
  * node instead of 128 / 256 B); the boxes are rounded outward, so hits are unchanged. */
 #define RBR_BVH_QUANTIZED 0x100
 int rbr_scene_build_bvh(rbr_scene* scene, int width, int leaf_max);
+/* The same layouts built on the GPU (no reference counterpart; the reference builds serially
+ * on the host, bvh.cpp:47-209): Morton-ordered radix tree (LBVH), collapsed to `width` with the
+ * host's largest-area rule, leaves of up to leaf_max Morton-contiguous primitives, quantized by
+ * the host's quantizer.  Builds in milliseconds where the host SAH build takes hundreds; the
+ * tree costs more node visits per ray.  Hits are the same (they do not depend on the tree).
+ * Needs a GPU context; the host copies (host traversal, statistics) are filled as well. */
+int rbr_scene_build_bvh_device(rbr_scene* scene, int width, int leaf_max);
 /* Copies the node array as the kernels read it (layout of bvh.h; up to capacity bytes) and
  * reports its size: for tooling and layout tests.  Works on host-only contexts. */
 int rbr_scene_device_nodes(const rbr_scene* scene, void* dst, size_t capacity, size_t* bytes);
+/* The float wide nodes (bvh.h layout; the host traversal's copy, also for quantized builds)
+ * and the leaf-ordered primitive records (48 B each), for inspection and tests; no reference
+ * counterpart.  Sizes are returned in *node_bytes / *prim_bytes. */
+int rbr_scene_bvh_arrays(const rbr_scene* scene, void* nodes, size_t node_capacity, void* prims, size_t prim_capacity,
+                         size_t* node_bytes, size_t* prim_bytes);
 int rbr_scene_bvh_stats(const rbr_scene* scene, rbr_bvh_stats* out);
 /* Accelerator::worldBounds (accelerator.h:34): empty box (lo=+FLT_MAX, hi=-FLT_MAX) when
  * nothing bounded was built. */

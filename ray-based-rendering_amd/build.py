@@ -36,7 +36,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
              "-I", INC, "-I", CSRC, "-Wall", "-Wno-unused-parameter"]
 
-HIP_SOURCES = ["context.hip", "trace.hip", "render.hip", "guide.hip"]
+HIP_SOURCES = ["context.hip", "trace.hip", "render.hip", "guide.hip", "bvh_gpu.hip"]
 CXX_SOURCES = ["bvh_build.cpp", "film_io.cpp"]
 
 

@@ -5,6 +5,7 @@
 // index), different tree; results do not depend on the tree because every traversal
 // returns the lowest-id closest hit.
 #include "bvh.h"
+#include "bvh_quant.h"
 #include "prim_tests.h"
 
 #include <omp.h>
@@ -486,72 +487,12 @@ bool hostAny(const WideBVH& b, const HostGeometry& g, const float o[3], const fl
 
 namespace rbr {
 
-// Quantization of one axis of one node: the origin is the union box's lower corner; the
-// scale is the smallest power of two with fma(255, s, p) >= the union's upper corner, so
-// every child plane lands in 0..255 after outward rounding (checked with the same single
-// rounding the device uses).
-static void quantAxis(const float* lo, const float* hi, const bool* live, int W, float& p, int& ebias, uint8_t* qlo,
-                      uint8_t* qhi) {
-    float plo = INFINITY, phi = -INFINITY;
-    for (int k = 0; k < W; ++k)
-        if (live[k]) {
-            plo = std::min(plo, lo[k]);
-            phi = std::max(phi, hi[k]);
-        }
-    if (!(plo <= phi)) {   // no live child
-        p = 0.0f;
-        ebias = 127;
-        for (int k = 0; k < W; ++k) { qlo[k] = 255; qhi[k] = 0; }
-        return;
-    }
-    p = plo;
-    const double ext = double(phi) - double(plo);
-    int e = -100;
-    if (ext > 0.0) e = std::max(-100, int(std::ceil(std::log2(ext / 255.0))));
-    while (std::fma(255.0f, std::ldexp(1.0f, e), p) < phi) ++e;
-    const float s = std::ldexp(1.0f, e);
-    ebias = e + 127;
-    for (int k = 0; k < W; ++k) {
-        if (!live[k]) { qlo[k] = 255; qhi[k] = 0; continue; }
-        int a = int(std::floor((double(lo[k]) - double(p)) / double(s)));
-        a = std::min(255, std::max(0, a));
-        while (a > 0 && std::fma(float(a), s, p) > lo[k]) --a;
-        int b = int(std::ceil((double(hi[k]) - double(p)) / double(s)));
-        b = std::min(255, std::max(0, b));
-        while (b < 255 && std::fma(float(b), s, p) < hi[k]) ++b;
-        qlo[k] = uint8_t(a);
-        qhi[k] = uint8_t(b);
-    }
-}
-
 void quantizeWideBVH(WideBVH& b) {
-    const int W = b.width, per = b.f4PerNode(), dw = qNodeDwords(W), q4 = W / 4;
+    const int W = b.width, per = b.f4PerNode(), dw = qNodeDwords(W);
     b.qnodes.assign(size_t(b.nodeCount) * dw, 0u);
-    for (uint32_t n = 0; n < b.nodeCount; ++n) {
-        const float* f = reinterpret_cast<const float*>(&b.nodes[size_t(n) * per]);
-        uint32_t ref[8];
-        bool live[8];
-        for (int k = 0; k < W; ++k) {
-            std::memcpy(&ref[k], &f[6 * W + k], 4);
-            live[k] = ref[k] != kEmpty;
-        }
-        uint32_t* q = &b.qnodes[size_t(n) * dw];
-        uint32_t exps = 0;
-        for (int ax = 0; ax < 3; ++ax) {
-            float p;
-            int eb;
-            uint8_t qlo[8], qhi[8];
-            quantAxis(&f[(2 * ax) * W], &f[(2 * ax + 1) * W], live, W, p, eb, qlo, qhi);
-            std::memcpy(&q[ax], &p, 4);
-            exps |= uint32_t(eb) << (8 * ax);
-            for (int k = 0; k < W; ++k) {
-                q[4 + (2 * ax) * q4 + (k >> 2)] |= uint32_t(qlo[k]) << (8 * (k & 3));
-                q[4 + (2 * ax + 1) * q4 + (k >> 2)] |= uint32_t(qhi[k]) << (8 * (k & 3));
-            }
-        }
-        q[3] = exps;
-        for (int k = 0; k < W; ++k) q[qRefOffset(W) + k] = ref[k];
-    }
+#pragma omp parallel for schedule(static)
+    for (long long n = 0; n < (long long)b.nodeCount; ++n)
+        quantizeNode(reinterpret_cast<const float*>(&b.nodes[size_t(n) * per]), W, &b.qnodes[size_t(n) * dw]);
     b.quantized = true;
 }
 

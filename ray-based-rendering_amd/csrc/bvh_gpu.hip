@@ -1,0 +1,545 @@
+// Device build of the wide BVH: the same node and primitive layouts as the host builder
+// (bvh.h), so every traversal kernel, the render arena and the host traversal use its output
+// unchanged.  The reference builds a binary SAH tree serially on the host (bvh.cpp:47-209);
+// this build trades some tree quality for build time:
+//   1. primitive boxes, world and centroid bounds (one pass, block reductions + atomics)
+//   2. 30-bit Morton codes of the centroids, sorted with their primitive index (radix sort)
+//   3. the binary radix tree over the sorted codes, every internal node in parallel (Karras
+//      2012, "Maximizing parallelism in the construction of BVHs, octrees and k-d trees");
+//      equal codes are ordered by index
+//   4. internal boxes bottom-up: one thread per leaf climbs while it is the second child to
+//      arrive at a node (per-node arrival counter)
+//   5. collapse to W-wide nodes, one level per launch: a node takes its binary node's two
+//      children and keeps opening the largest-area child until it has W (the host collapse's
+//      rule); a subtree of at most leaf_max primitives becomes a leaf (its primitives are
+//      contiguous in Morton order); child boxes are padded as the host pads them
+//   6. leaf-ordered primitive records (bvh.h), and for quantized layouts the 8-bit nodes by
+//      the host's own quantizer (bvh_quant.h), so both produce the same bits from the same
+//      float nodes
+// Nodes, records and counts are then copied to the host structure as well (host traversal,
+// statistics, the render arena).
+#include <hip/hip_runtime.h>
+#include <hipcub/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bvh.h"
+#include "bvh_quant.h"
+#include "context.h"
+
+namespace rbr {
+
+namespace {
+
+struct GGeom {
+    const float* tris;
+    const uint32_t* triIds;
+    const float* sphs;
+    const uint32_t* sphIds;
+    const float* quads;
+    const uint32_t* quadIds;
+    uint32_t ntri, nsph, nquad;
+};
+
+constexpr uint32_t kLeafTag = 0x80000000u;   // binary child code: leaf (sorted position)
+
+__device__ __forceinline__ float mn(float a, float b) { return b < a ? b : a; }   // std::min
+__device__ __forceinline__ float mx(float a, float b) { return a < b ? b : a; }   // std::max
+
+// Primitive box with the host builder's arithmetic (bvh_build.cpp).
+__device__ void primBox(const GGeom& g, uint32_t i, float lo[3], float hi[3]) {
+    if (i < g.ntri) {
+        const float* t = g.tris + 9 * size_t(i);
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = mn(mn(t[k], t[3 + k]), t[6 + k]);
+            hi[k] = mx(mx(t[k], t[3 + k]), t[6 + k]);
+        }
+    } else if (i < g.ntri + g.nsph) {
+        const float* s = g.sphs + 4 * size_t(i - g.ntri);
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = s[k] - s[3];
+            hi[k] = s[k] + s[3];
+        }
+    } else {
+        const float* s = g.quads + 12 * size_t(i - g.ntri - g.nsph);
+        for (int k = 0; k < 3; ++k) {
+            const float c1 = s[k] + s[3 + k], c2 = s[k] + s[6 + k], c3 = s[k] + s[3 + k] + s[6 + k];
+            lo[k] = mn(mn(mn(s[k], c1), c2), c3);
+            hi[k] = mx(mx(mx(s[k], c1), c2), c3);
+        }
+    }
+}
+
+// Order-preserving float <-> uint32 (for atomicMin / atomicMax on floats).
+__device__ __forceinline__ uint32_t fkey(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unkey(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+// red[0..2] world lo, [3..5] world hi, [6..8] centroid lo, [9..11] centroid hi (keys).
+__global__ __launch_bounds__(256) void gbBoundsKernel(GGeom g, uint32_t n, float4* __restrict__ boxLo,
+                                                      float4* __restrict__ boxHi, uint32_t* red) {
+    __shared__ uint32_t sh[12];
+    if (threadIdx.x < 12) sh[threadIdx.x] = (threadIdx.x % 6) < 3 ? 0xFFFFFFFFu : 0u;
+    __syncthreads();
+    uint32_t loc[12];
+    for (int k = 0; k < 12; ++k) loc[k] = (k % 6) < 3 ? 0xFFFFFFFFu : 0u;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        float lo[3], hi[3];
+        primBox(g, i, lo, hi);
+        boxLo[i] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+        boxHi[i] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+        for (int k = 0; k < 3; ++k) {
+            const float c = 0.5f * (lo[k] + hi[k]);
+            loc[k] = min(loc[k], fkey(lo[k]));
+            loc[3 + k] = max(loc[3 + k], fkey(hi[k]));
+            loc[6 + k] = min(loc[6 + k], fkey(c));
+            loc[9 + k] = max(loc[9 + k], fkey(c));
+        }
+    }
+    for (int k = 0; k < 12; ++k) {
+        if ((k % 6) < 3) atomicMin(&sh[k], loc[k]);
+        else atomicMax(&sh[k], loc[k]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        if ((threadIdx.x % 6) < 3) atomicMin(&red[threadIdx.x], sh[threadIdx.x]);
+        else atomicMax(&red[threadIdx.x], sh[threadIdx.x]);
+    }
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {   // 10 bits -> every third bit
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void gbMortonKernel(uint32_t n, const float4* __restrict__ boxLo,
+                                                      const float4* __restrict__ boxHi, const uint32_t* red,
+                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 lo = boxLo[i], hi = boxHi[i];
+    const float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        const float a = unkey(red[6 + k]), b = unkey(red[9 + k]);
+        const float ext = b - a;
+        float u = ext > 0.0f ? (c[k] - a) / ext : 0.5f;
+        u = fminf(fmaxf(u, 0.0f), 1.0f);
+        q[k] = min(uint32_t(u * 1024.0f), 1023u);
+    }
+    keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    vals[i] = i;
+}
+
+// Common-prefix length of sorted keys i and j (j out of range: -1); equal keys compare
+// their positions.
+__device__ __forceinline__ int delta(const uint32_t* keys, uint32_t n, int i, int j) {
+    if (j < 0 || j >= int(n)) return -1;
+    const uint32_t a = keys[i], b = keys[j];
+    if (a != b) return __clz(a ^ b);
+    return 32 + __clz(uint32_t(i) ^ uint32_t(j));
+}
+
+// Internal node i of the radix tree over n sorted keys: its range [first, last], children
+// (internal index, or kLeafTag | sorted position) and the parent links of both.
+__global__ __launch_bounds__(256) void gbKarrasKernel(uint32_t n, const uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ left, uint32_t* __restrict__ right,
+                                                      uint32_t* __restrict__ first, uint32_t* __restrict__ last,
+                                                      uint32_t* __restrict__ parentInt,
+                                                      uint32_t* __restrict__ parentLeaf) {
+    const int i = int(blockIdx.x * 256 + threadIdx.x);
+    if (i >= int(n) - 1) return;
+    const int d = delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(keys, n, i, j);
+    int s = 0;
+    for (int t = (l + 1) >> 1;; t = (t + 1) >> 1) {
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t == 1) break;
+    }
+    const int gamma = i + s * d + (d < 0 ? -1 : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const uint32_t lc = lo == gamma ? (kLeafTag | uint32_t(gamma)) : uint32_t(gamma);
+    const uint32_t rc = hi == gamma + 1 ? (kLeafTag | uint32_t(gamma + 1)) : uint32_t(gamma + 1);
+    left[i] = lc;
+    right[i] = rc;
+    first[i] = uint32_t(lo);
+    last[i] = uint32_t(hi);
+    if (lc & kLeafTag) parentLeaf[lc & ~kLeafTag] = uint32_t(i);
+    else parentInt[lc] = uint32_t(i);
+    if (rc & kLeafTag) parentLeaf[rc & ~kLeafTag] = uint32_t(i);
+    else parentInt[rc] = uint32_t(i);
+}
+
+__device__ __forceinline__ float ld(const float* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void st(float* p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Box of a binary child: a leaf is its primitive's box (sorted position -> primitive).
+__device__ __forceinline__ void childBox(uint32_t c, const uint32_t* vals, const float4* boxLo, const float4* boxHi,
+                                         const float* nodeBox, float lo[3], float hi[3]) {
+    if (c & kLeafTag) {
+        const uint32_t p = vals[c & ~kLeafTag];
+        const float4 a = boxLo[p], b = boxHi[p];
+        lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
+        hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
+    } else {
+        const float* q = nodeBox + 6 * size_t(c);
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = ld(q + k);
+            hi[k] = ld(q + 3 + k);
+        }
+    }
+}
+
+// Internal boxes bottom-up.  The arrival counter is taken with acquire-release order at
+// device scope, the boxes are read and written with device-scope atomics, so the second
+// arrival at a node sees the box its sibling's thread wrote before arriving.
+__global__ __launch_bounds__(256) void gbRefitKernel(uint32_t n, const uint32_t* __restrict__ vals,
+                                                     const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
+                                                     const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
+                                                     const uint32_t* __restrict__ parentInt,
+                                                     const uint32_t* __restrict__ parentLeaf, uint32_t* arrivals,
+                                                     float* nodeBox) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n || n < 2) return;
+    uint32_t p = parentLeaf[i];
+    while (true) {
+        const uint32_t prior = __hip_atomic_fetch_add(&arrivals[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prior == 0u) return;   // the sibling's thread finishes this node
+        float alo[3], ahi[3], blo[3], bhi[3];
+        childBox(left[p], vals, boxLo, boxHi, nodeBox, alo, ahi);
+        childBox(right[p], vals, boxLo, boxHi, nodeBox, blo, bhi);
+        float* q = nodeBox + 6 * size_t(p);
+        for (int k = 0; k < 3; ++k) {
+            st(q + k, mn(alo[k], blo[k]));
+            st(q + 3 + k, mx(ahi[k], bhi[k]));
+        }
+        if (p == 0u) return;
+        p = parentInt[p];
+    }
+}
+
+struct CollapseArgs {
+    uint32_t n, W, leafMax;
+    const uint32_t *vals, *left, *right, *first, *last;
+    const float4 *boxLo, *boxHi;
+    const float* nodeBox;
+    const uint32_t* red;
+    float* nodes;           // float wide nodes (bvh.h), f4PerNode * 4 floats each
+    uint32_t* counters;     // [0] wide nodes, [1] leaves, [2] next-level items
+};
+
+__device__ __forceinline__ uint32_t codeCount(const CollapseArgs& a, uint32_t c) {
+    return (c & kLeafTag) ? 1u : a.last[c] - a.first[c] + 1u;
+}
+__device__ __forceinline__ uint32_t codeFirst(const CollapseArgs& a, uint32_t c) {
+    return (c & kLeafTag) ? (c & ~kLeafTag) : a.first[c];
+}
+
+// One level of the collapse: item k = (binary node code, wide node index).
+__global__ __launch_bounds__(256) void gbCollapseKernel(CollapseArgs a, const uint2* __restrict__ in, uint32_t nIn,
+                                                        uint2* __restrict__ out) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nIn) return;
+    const uint2 item = in[k];
+    const uint32_t W = a.W;
+    uint32_t kids[8];
+    uint32_t nk = 0;
+    if ((item.x & kLeafTag) || codeCount(a, item.x) <= a.leafMax) {
+        kids[nk++] = item.x;   // a root small enough to be one leaf
+    } else {
+        kids[nk++] = a.left[item.x];
+        kids[nk++] = a.right[item.x];
+        while (nk < W) {
+            int best = -1;
+            float bestA = -1.0f;
+            for (uint32_t i = 0; i < nk; ++i) {
+                const uint32_t c = kids[i];
+                if ((c & kLeafTag) || codeCount(a, c) <= a.leafMax) continue;
+                float lo[3], hi[3];
+                childBox(c, a.vals, a.boxLo, a.boxHi, a.nodeBox, lo, hi);
+                const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+                const float ar = 2.0f * (dx * dy + dy * dz + dz * dx);
+                if (ar > bestA) {
+                    bestA = ar;
+                    best = int(i);
+                }
+            }
+            if (best < 0) break;
+            const uint32_t open = kids[best];
+            kids[best] = a.left[open];
+            kids[nk++] = a.right[open];
+        }
+    }
+    // padding of the host collapse: pad + |x| * 1e-6 outward, pad = 1e-5 * extent + 1e-6
+    float ext = 0.0f;
+    for (int c = 0; c < 3; ++c) ext = fmaxf(ext, unkey(a.red[3 + c]) - unkey(a.red[c]));
+    const float pad = 1e-5f * ext + 1e-6f;
+    float* f = a.nodes + size_t(item.y) * (W == 4 ? 32 : 64);
+    for (uint32_t i = 0; i < W; ++i) {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        uint32_t ref = kEmpty;
+        if (i < nk) {
+            const uint32_t c = kids[i];
+            float blo[3], bhi[3];
+            childBox(c, a.vals, a.boxLo, a.boxHi, a.nodeBox, blo, bhi);
+            for (int q = 0; q < 3; ++q) {
+                lo[q] = blo[q] - (pad + fabsf(blo[q]) * 1e-6f);
+                hi[q] = bhi[q] + (pad + fabsf(bhi[q]) * 1e-6f);
+            }
+            const uint32_t cnt = codeCount(a, c);
+            if ((c & kLeafTag) || cnt <= a.leafMax) {
+                ref = kLeafBit | ((cnt - 1u) << kCountShift) | codeFirst(a, c);   // bvh.h leaf encoding
+                atomicAdd(&a.counters[1], 1u);
+            } else {
+                ref = atomicAdd(&a.counters[0], 1u);
+                out[atomicAdd(&a.counters[2], 1u)] = make_uint2(c, ref);
+            }
+        }
+        f[0 * W + i] = lo[0];
+        f[1 * W + i] = hi[0];
+        f[2 * W + i] = lo[1];
+        f[3 * W + i] = hi[1];
+        f[4 * W + i] = lo[2];
+        f[5 * W + i] = hi[2];
+        f[6 * W + i] = __uint_as_float(ref);
+    }
+    for (uint32_t i = 7 * W; i < (W == 4 ? 32u : 64u); ++i) f[i] = 0.0f;
+}
+
+// Leaf-ordered primitive records (bvh.h), as the host builder writes them.
+__global__ __launch_bounds__(256) void gbRecordsKernel(GGeom g, uint32_t n, const uint32_t* __restrict__ vals,
+                                                       float4* __restrict__ rec) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = vals[i];
+    float4* r = rec + 3 * size_t(i);
+    if (p < g.ntri) {
+        const float* t = g.tris + 9 * size_t(p);
+        r[0] = make_float4(t[0], t[1], t[2], __uint_as_float(g.triIds[p]));
+        r[1] = make_float4(t[3] - t[0], t[4] - t[1], t[5] - t[2], __uint_as_float(KIND_TRI));
+        r[2] = make_float4(t[6] - t[0], t[7] - t[1], t[8] - t[2], 0.0f);
+    } else if (p < g.ntri + g.nsph) {
+        const uint32_t q = p - g.ntri;
+        const float* s = g.sphs + 4 * size_t(q);
+        r[0] = make_float4(s[0], s[1], s[2], __uint_as_float(g.sphIds[q]));
+        r[1] = make_float4(s[3], 0.0f, 0.0f, __uint_as_float(KIND_SPHERE));
+        r[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+        const uint32_t q = p - g.ntri - g.nsph;
+        const float* s = g.quads + 12 * size_t(q);
+        r[0] = make_float4(s[0], s[1], s[2], __uint_as_float(g.quadIds[q]));
+        r[1] = make_float4(s[3], s[4], s[5], __uint_as_float(KIND_QUAD));
+        r[2] = make_float4(s[6], s[7], s[8], __uint_as_float(q));
+    }
+}
+
+__global__ __launch_bounds__(256) void gbQuantizeKernel(uint32_t count, uint32_t W, const float* __restrict__ nodes,
+                                                        uint32_t* __restrict__ q) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t dw = W == 4 ? 16u : 32u;
+    uint32_t tmp[32];
+    for (uint32_t k = 0; k < dw; ++k) tmp[k] = 0u;
+    quantizeNode(nodes + size_t(i) * (W == 4 ? 32 : 64), int(W), tmp);
+    for (uint32_t k = 0; k < dw; ++k) q[size_t(i) * dw + k] = tmp[k];
+}
+
+// Reduction seeds, counters (wide node 0 is the root) and the root item: binary node 0, or
+// the single primitive when there is only one.
+__global__ void gbInitKernel(uint32_t* red, uint32_t* counters, uint2* items, uint32_t rootCode) {
+    const uint32_t t = threadIdx.x;
+    if (t < 12) red[t] = (t % 6) < 3 ? 0xFFFFFFFFu : 0u;
+    if (t < 3) counters[t] = t == 0 ? 1u : 0u;
+    if (t == 0) items[0] = make_uint2(rootCode, 0u);
+}
+
+__global__ __launch_bounds__(256) void gbZeroKernel(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+float hostUnkey(uint32_t k) {
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+}  // namespace
+
+int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool quant) {
+    using Clock = std::chrono::steady_clock;
+    const auto t0 = Clock::now();
+    const HostGeometry& geo = s->geom;
+    WideBVH& out = s->bvh;
+    out = WideBVH();
+    out.width = width;
+    const uint32_t ntri = uint32_t(geo.triIds.size()), nsph = uint32_t(geo.sphIds.size()),
+                   nquad = uint32_t(geo.quadIds.size());
+    const uint32_t n = ntri + nsph + nquad;
+    out.primCount = n;
+    int rc = RBR_OK;
+    if (n == 0) return rc;
+    if (n >= kStartMask) return fail(ctx, RBR_ERR_UNSUPPORTED, "too many primitives for the leaf encoding");
+    hipStream_t st0 = ctx->stream;
+    // Device temporaries, freed on every exit.
+    DevBuf dTri, dTriId, dSph, dSphId, dQuad, dQuadId, dLo, dHi, dRed, dKeys, dVals, dKeys2, dVals2, dSortTmp, dLeft,
+        dRight, dFirst, dLast, dPInt, dPLeaf, dArr, dNodeBox, dItemsA, dItemsB, dCnt, dFloatNodes;
+    auto freeAll = [&]() {
+        for (DevBuf* b : {&dTri, &dTriId, &dSph, &dSphId, &dQuad, &dQuadId, &dLo, &dHi, &dRed, &dKeys, &dVals, &dKeys2,
+                          &dVals2, &dSortTmp, &dLeft, &dRight, &dFirst, &dLast, &dPInt, &dPLeaf, &dArr, &dNodeBox,
+                          &dItemsA, &dItemsB, &dCnt, &dFloatNodes})
+            freeBuffer(*b);
+    };
+    auto chk = [&](int r) {
+        if (r && !rc) rc = r;
+        return rc == RBR_OK;
+    };
+    const uint32_t ni = std::max(1u, n - 1);
+    const int W = width;
+    const size_t f4n = size_t(W == 4 ? 8 : 16);
+    const size_t maxNodes = std::max<size_t>(1, n);
+    if (!chk(uploadBuffer(ctx, dTri, geo.tris.data(), geo.tris.size() * 4)) ||
+        !chk(uploadBuffer(ctx, dTriId, geo.triIds.data(), geo.triIds.size() * 4)) ||
+        !chk(uploadBuffer(ctx, dSph, geo.sphs.data(), geo.sphs.size() * 4)) ||
+        !chk(uploadBuffer(ctx, dSphId, geo.sphIds.data(), geo.sphIds.size() * 4)) ||
+        !chk(uploadBuffer(ctx, dQuad, geo.quads.data(), geo.quads.size() * 4)) ||
+        !chk(uploadBuffer(ctx, dQuadId, geo.quadIds.data(), geo.quadIds.size() * 4)) ||
+        !chk(ensureBuffer(ctx, dLo, size_t(n) * 16)) || !chk(ensureBuffer(ctx, dHi, size_t(n) * 16)) ||
+        !chk(ensureBuffer(ctx, dRed, 12 * 4)) || !chk(ensureBuffer(ctx, dKeys, size_t(n) * 4)) ||
+        !chk(ensureBuffer(ctx, dVals, size_t(n) * 4)) || !chk(ensureBuffer(ctx, dKeys2, size_t(n) * 4)) ||
+        !chk(ensureBuffer(ctx, dVals2, size_t(n) * 4)) || !chk(ensureBuffer(ctx, dLeft, size_t(ni) * 4)) ||
+        !chk(ensureBuffer(ctx, dRight, size_t(ni) * 4)) || !chk(ensureBuffer(ctx, dFirst, size_t(ni) * 4)) ||
+        !chk(ensureBuffer(ctx, dLast, size_t(ni) * 4)) || !chk(ensureBuffer(ctx, dPInt, size_t(ni) * 4)) ||
+        !chk(ensureBuffer(ctx, dPLeaf, size_t(n) * 4)) || !chk(ensureBuffer(ctx, dArr, size_t(ni) * 4)) ||
+        !chk(ensureBuffer(ctx, dNodeBox, size_t(ni) * 24)) || !chk(ensureBuffer(ctx, dItemsA, maxNodes * 8)) ||
+        !chk(ensureBuffer(ctx, dItemsB, maxNodes * 8)) || !chk(ensureBuffer(ctx, dCnt, 16)) ||
+        !chk(ensureBuffer(ctx, dFloatNodes, maxNodes * f4n * 16)) ||
+        !chk(ensureBuffer(ctx, s->dPrims, size_t(n) * 48))) {
+        freeAll();
+        return rc;
+    }
+    GGeom g{static_cast<const float*>(dTri.ptr), static_cast<const uint32_t*>(dTriId.ptr),
+            static_cast<const float*>(dSph.ptr), static_cast<const uint32_t*>(dSphId.ptr),
+            static_cast<const float*>(dQuad.ptr), static_cast<const uint32_t*>(dQuadId.ptr), ntri, nsph, nquad};
+    auto u32 = [](DevBuf& b) { return static_cast<uint32_t*>(b.ptr); };
+    auto f4 = [](DevBuf& b) { return static_cast<float4*>(b.ptr); };
+    const unsigned blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(gbInitKernel, dim3(1), dim3(64), 0, st0, u32(dRed), u32(dCnt), static_cast<uint2*>(dItemsA.ptr),
+                       n == 1 ? kLeafTag : 0u);
+    hipLaunchKernelGGL(gbBoundsKernel, dim3(std::min(blocks, 2048u)), dim3(256), 0, st0, g, n, f4(dLo), f4(dHi),
+                       u32(dRed));
+    hipLaunchKernelGGL(gbMortonKernel, dim3(blocks), dim3(256), 0, st0, n, f4(dLo), f4(dHi), u32(dRed), u32(dKeys),
+                       u32(dVals));
+    size_t tmpBytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, u32(dKeys), u32(dKeys2), u32(dVals),
+                                                      u32(dVals2), int(n), 0, 30, st0);
+    if (e != hipSuccess || !chk(ensureBuffer(ctx, dSortTmp, std::max<size_t>(tmpBytes, 16)))) {
+        freeAll();
+        return rc ? rc : failHip(ctx, e, "radix sort size");
+    }
+    e = hipcub::DeviceRadixSort::SortPairs(dSortTmp.ptr, tmpBytes, u32(dKeys), u32(dKeys2), u32(dVals), u32(dVals2),
+                                           int(n), 0, 30, st0);
+    if (e != hipSuccess) {
+        freeAll();
+        return failHip(ctx, e, "radix sort");
+    }
+    const uint32_t* keys = u32(dKeys2);
+    const uint32_t* vals = u32(dVals2);
+    if (n >= 2) {
+        hipLaunchKernelGGL(gbKarrasKernel, dim3((n - 1 + 255) / 256), dim3(256), 0, st0, n, keys, u32(dLeft), u32(dRight),
+                           u32(dFirst), u32(dLast), u32(dPInt), u32(dPLeaf));
+        hipLaunchKernelGGL(gbZeroKernel, dim3((n - 1 + 255) / 256), dim3(256), 0, st0, u32(dArr), n - 1);
+        hipLaunchKernelGGL(gbRefitKernel, dim3(blocks), dim3(256), 0, st0, n, vals, f4(dLo), f4(dHi), u32(dLeft),
+                           u32(dRight), u32(dPInt), u32(dPLeaf), u32(dArr), static_cast<float*>(dNodeBox.ptr));
+    }
+    CollapseArgs ca{n, uint32_t(W), uint32_t(leafMax), vals, u32(dLeft), u32(dRight), u32(dFirst), u32(dLast),
+                    f4(dLo), f4(dHi), static_cast<const float*>(dNodeBox.ptr), u32(dRed),
+                    static_cast<float*>(dFloatNodes.ptr), u32(dCnt)};
+    uint32_t nIn = 1, depth = 0;
+    uint2* itIn = static_cast<uint2*>(dItemsA.ptr);
+    uint2* itOut = static_cast<uint2*>(dItemsB.ptr);
+    uint32_t cnt[3] = {1, 0, 0};
+    while (nIn > 0) {
+        ++depth;
+        hipLaunchKernelGGL(gbCollapseKernel, dim3((nIn + 255) / 256), dim3(256), 0, st0, ca, itIn, nIn, itOut);
+        e = hipMemcpyAsync(cnt, dCnt.ptr, 12, hipMemcpyDeviceToHost, st0);
+        if (e == hipSuccess) e = hipStreamSynchronize(st0);
+        if (e != hipSuccess) {
+            freeAll();
+            return failHip(ctx, e, "collapse level");
+        }
+        nIn = cnt[2];
+        hipLaunchKernelGGL(gbZeroKernel, dim3(1), dim3(256), 0, st0, u32(dCnt) + 2, 1u);
+        std::swap(itIn, itOut);
+    }
+    out.nodeCount = cnt[0];
+    out.leafCount = cnt[1];
+    out.maxDepth = depth;
+    hipLaunchKernelGGL(gbRecordsKernel, dim3(blocks), dim3(256), 0, st0, g, n, vals, static_cast<float4*>(s->dPrims.ptr));
+    const size_t floatBytes = size_t(out.nodeCount) * f4n * 16;
+    if (quant) {
+        const size_t qBytes = size_t(out.nodeCount) * qNodeDwords(W) * 4;
+        if (!chk(ensureBuffer(ctx, s->dNodes, qBytes))) {
+            freeAll();
+            return rc;
+        }
+        hipLaunchKernelGGL(gbQuantizeKernel, dim3((out.nodeCount + 255) / 256), dim3(256), 0, st0, out.nodeCount,
+                           uint32_t(W), static_cast<const float*>(dFloatNodes.ptr), u32(s->dNodes));
+    } else {
+        if (!chk(ensureBuffer(ctx, s->dNodes, floatBytes))) {
+            freeAll();
+            return rc;
+        }
+        e = hipMemcpyAsync(s->dNodes.ptr, dFloatNodes.ptr, floatBytes, hipMemcpyDeviceToDevice, st0);
+    }
+    // Host copies: float nodes (host traversal), device nodes, records, bounds.
+    out.nodes.resize(size_t(out.nodeCount) * f4n);
+    out.prims.resize(size_t(n) * 3);
+    uint32_t red[12];
+    if (e == hipSuccess) e = hipMemcpyAsync(out.nodes.data(), dFloatNodes.ptr, floatBytes, hipMemcpyDeviceToHost, st0);
+    if (e == hipSuccess && quant) {
+        out.qnodes.resize(size_t(out.nodeCount) * qNodeDwords(W));
+        e = hipMemcpyAsync(out.qnodes.data(), s->dNodes.ptr, out.qnodes.size() * 4, hipMemcpyDeviceToHost, st0);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out.prims.data(), s->dPrims.ptr, out.prims.size() * 16, hipMemcpyDeviceToHost, st0);
+    if (e == hipSuccess) e = hipMemcpyAsync(red, dRed.ptr, 48, hipMemcpyDeviceToHost, st0);
+    if (e == hipSuccess) e = hipStreamSynchronize(st0);
+    if (e == hipSuccess) e = hipGetLastError();
+    freeAll();
+    if (e != hipSuccess) return failHip(ctx, e, "device BVH build");
+    out.quantized = quant;
+    for (int k = 0; k < 3; ++k) {
+        out.worldLo[k] = hostUnkey(red[k]);
+        out.worldHi[k] = hostUnkey(red[3 + k]);
+    }
+    out.quadNormals.resize(nquad);
+    for (uint32_t i = 0; i < nquad; ++i) {
+        const float* q = &geo.quads[12 * size_t(i)];
+        out.quadNormals[i] = {q[9], q[10], q[11], 0.0f};
+    }
+    out.sahCost = 0.0;   // the binary tree stays on the device
+    out.buildMs = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    return RBR_OK;
+}
+
+}  // namespace rbr

@@ -78,6 +78,10 @@ int launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack, boo
 // launches themselves allocate nothing (stream capture).
 int prepareTraversal(rbr_ctx* ctx, uint32_t maxStack);
 
+// Device build of the scene's wide BVH (bvh_gpu.hip): fills s->bvh (host copies) and the
+// scene's device node and primitive buffers.
+int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool quant);
+
 // Gives the scene a new version number (after any change to it).
 void bumpVersion(rbr_scene* s);
 void freeShading(rbr_scene* s);          // render.hip

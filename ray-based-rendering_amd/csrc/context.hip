@@ -335,12 +335,46 @@ int rbr_scene_build_bvh(rbr_scene* s, int width, int leaf_max) {
     return RBR_OK;
 }
 
+int rbr_scene_build_bvh_device(rbr_scene* s, int width, int leaf_max) {
+    if (!s) return RBR_ERR_INVALID;
+    rbr_ctx* ctx = s->ctx;
+    const int W = width & ~kQuantFlag;
+    if ((W != 4 && W != 8) || (width & ~(kQuantFlag | 0xFF)))
+        return fail(ctx, RBR_ERR_INVALID, "BVH width must be 4 or 8, optionally | RBR_BVH_QUANTIZED");
+    if (leaf_max < 1 || leaf_max > 8) return fail(ctx, RBR_ERR_INVALID, "leaf_max must be in 1..8");
+    if (ctx->device < 0) return fail(ctx, RBR_ERR_STATE, "the device build needs a GPU context");
+    if (int rc = useDevice(ctx)) return rc;
+    s->built = false;
+    bumpVersion(s);
+    if (int rc = buildWideBVHDevice(ctx, s, W, leaf_max, (width & kQuantFlag) != 0)) return rc;
+    if (!s->bvh.empty() && uint64_t(W - 1) * s->bvh.maxDepth + 1 > uint64_t(kMaxStack))
+        return fail(ctx, RBR_ERR_UNSUPPORTED, "BVH too deep for the traversal stack");
+    int rc;
+    if ((rc = uploadBuffer(ctx, s->dQuadN, s->bvh.quadNormals.data(), s->bvh.quadNormals.size() * sizeof(F4)))) return rc;
+    if ((rc = uploadBuffer(ctx, s->dPlanes, s->geom.planes.data(), s->geom.planes.size() * sizeof(float)))) return rc;
+    if ((rc = uploadBuffer(ctx, s->dPlaneIds, s->geom.planeIds.data(), s->geom.planeIds.size() * sizeof(uint32_t)))) return rc;
+    s->built = true;
+    return RBR_OK;
+}
+
 int rbr_scene_device_nodes(const rbr_scene* s, void* dst, size_t capacity, size_t* bytes) {
     if (!s) return RBR_ERR_INVALID;
     if (!s->built) return fail(s->ctx, RBR_ERR_STATE, "BVH not built");
     const size_t n = s->bvh.deviceNodeBytes();
     if (bytes) *bytes = n;
     if (dst && capacity) std::memcpy(dst, s->bvh.deviceNodes(), std::min(capacity, n));
+    return RBR_OK;
+}
+
+int rbr_scene_bvh_arrays(const rbr_scene* s, void* nodes, size_t node_capacity, void* prims, size_t prim_capacity,
+                         size_t* node_bytes, size_t* prim_bytes) {
+    if (!s) return RBR_ERR_INVALID;
+    if (!s->built) return fail(s->ctx, RBR_ERR_STATE, "BVH not built");
+    const size_t nb = s->bvh.nodes.size() * sizeof(F4), pb = s->bvh.prims.size() * sizeof(F4);
+    if (node_bytes) *node_bytes = nb;
+    if (prim_bytes) *prim_bytes = pb;
+    if (nodes && node_capacity) std::memcpy(nodes, s->bvh.nodes.data(), std::min(node_capacity, nb));
+    if (prims && prim_capacity) std::memcpy(prims, s->bvh.prims.data(), std::min(prim_capacity, pb));
     return RBR_OK;
 }
 

@@ -74,8 +74,10 @@ _sig(_lib.rbr_scene_create, [_vp, C.POINTER(_vp)])
 _sig(_lib.rbr_scene_destroy, [_vp], None)
 _sig(_lib.rbr_scene_set_geometry, [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp])
 _sig(_lib.rbr_scene_build_bvh, [_vp, C.c_int, C.c_int])
+_sig(_lib.rbr_scene_build_bvh_device, [_vp, C.c_int, C.c_int])
 _sig(_lib.rbr_scene_bvh_stats, [_vp, C.POINTER(BvhStats)])
 _sig(_lib.rbr_scene_device_nodes, [_vp, _vp, _sz, C.POINTER(_sz)])
+_sig(_lib.rbr_scene_bvh_arrays, [_vp, _vp, _sz, _vp, _sz, C.POINTER(_sz), C.POINTER(_sz)])
 BVH_QUANTIZED = 0x100
 _sig(_lib.rbr_scene_world_bounds, [_vp, _vp, _vp])
 _sig(_lib.rbr_trace_closest, [_vp, _vp, _sz, _vp])
@@ -239,8 +241,10 @@ class Scene:
         self.ctx.check(_lib.rbr_scene_set_geometry(self._h, _ptr(t), n(t), _ptr(ti), _ptr(s), n(s), _ptr(si),
                                                    _ptr(q), n(q), _ptr(qi), _ptr(p), n(p), _ptr(pi)))
 
-    def build_bvh(self, width: int = 4, leaf_max: int = 4) -> None:
-        self.ctx.check(_lib.rbr_scene_build_bvh(self._h, width, leaf_max))
+    def build_bvh(self, width: int = 4, leaf_max: int = 4, device: bool = False) -> None:
+        """Host binned-SAH build, or (device=True) the GPU Morton-order build."""
+        fn = _lib.rbr_scene_build_bvh_device if device else _lib.rbr_scene_build_bvh
+        self.ctx.check(fn(self._h, width, leaf_max))
 
     def device_nodes(self) -> np.ndarray:
         """The node array as the kernels read it, as uint32 words."""
@@ -249,6 +253,22 @@ class Scene:
         out = np.zeros(n.value // 4, np.uint32)
         self.ctx.check(_lib.rbr_scene_device_nodes(self._h, out.ctypes.data, n.value, C.byref(n)))
         return out
+
+    def float_nodes(self) -> np.ndarray:
+        """The float wide nodes (bvh.h layout) as uint32 words."""
+        nb, pb = _sz(0), _sz(0)
+        self.ctx.check(_lib.rbr_scene_bvh_arrays(self._h, None, 0, None, 0, C.byref(nb), C.byref(pb)))
+        out = np.zeros(nb.value // 4, np.uint32)
+        self.ctx.check(_lib.rbr_scene_bvh_arrays(self._h, out.ctypes.data, nb.value, None, 0, C.byref(nb), C.byref(pb)))
+        return out
+
+    def prim_records(self) -> np.ndarray:
+        """Leaf-ordered primitive records as float32 (n, 3, 4); word 3 of row 0 holds the id bits."""
+        nb, pb = _sz(0), _sz(0)
+        self.ctx.check(_lib.rbr_scene_bvh_arrays(self._h, None, 0, None, 0, C.byref(nb), C.byref(pb)))
+        out = np.zeros(pb.value // 4, np.float32)
+        self.ctx.check(_lib.rbr_scene_bvh_arrays(self._h, None, 0, out.ctypes.data, pb.value, C.byref(nb), C.byref(pb)))
+        return out.reshape(-1, 3, 4)
 
     def stats(self) -> dict:
         st = BvhStats()
@@ -464,13 +484,13 @@ def _job_config(j: dict) -> RbrRenderConfig:
                     j.get("mis", True), j.get("sample_offset", 0), j.get("guided", 0), j.get("guide_depth", 0))
 
 
-def scene_from_desc(ctx: Context, desc, width: int = 4 | 0x100, leaf: int = 4) -> Scene:
-    """Uploads a scenes.SceneDesc: geometry, wide BVH (quantized BVH4 by default) and
-    shading tables."""
+def scene_from_desc(ctx: Context, desc, width: int = 4 | 0x100, leaf: int = 4, device_build: bool = False) -> Scene:
+    """Uploads a scenes.SceneDesc: geometry, wide BVH (quantized BVH4 by default; host SAH
+    build, or the GPU build with device_build) and shading tables."""
     sc = Scene(ctx)
     sc.set_geometry(tris=desc.tris if len(desc.tris) else None, spheres=desc.spheres if len(desc.spheres) else None,
                     quads=desc.quads if len(desc.quads) else None, planes=desc.planes if len(desc.planes) else None)
-    sc.build_bvh(width, leaf)
+    sc.build_bvh(width, leaf, device=device_build)
     pm = np.ascontiguousarray(desc.prim_material, np.int32)
     mats, lights, med = c_materials(desc.materials), c_lights(desc.lights), c_medium(desc.medium)
     bg = np.asarray(desc.background, np.float32)
