@@ -314,6 +314,16 @@ int rbr_guide_collected(rbr_ctx* ctx, uint64_t* count, double* weight_sum);
 int rbr_guide_train_collected(rbr_ctx* ctx, int steps, uint32_t batch, float lr, double* loss);
 int rbr_guide_clear_collected(rbr_ctx* ctx);
 
+/* ---- Scene ingestion (host only) ---------------------------------------------------------- */
+/* Triangles of Wavefront OBJ text as ObjImporter::parse builds them (obj_importer.cpp:59-150):
+ * v / vt / vn / f records, 1-based and negative indices, polygons fanned around their first
+ * corner; 9 floats per triangle (positions only: the device path shades with geometric
+ * normals).  tris == NULL returns the count only; *normals / *uvs report the vn / vt records.
+ * RBR_ERR_INVALID when a face names a missing vertex or no triangle results (the reference's
+ * failures). */
+int rbr_obj_parse(const char* text, size_t length, float* tris, size_t capacity, size_t* count, size_t* normals,
+                  size_t* uvs);
+
 /* ---- Film output (host only; films are float RGB, row-major from the top row) ----------- */
 /* 8-bit conversion of a mean-radiance film, as Film::toRGB8 / Film::toRGB8ToneMapped
  * (film.cpp:27-63): per channel v (times exposure, then v / (v + 1) when tone_map),

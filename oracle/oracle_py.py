@@ -68,6 +68,7 @@ def _load_ref():
         _sig(lib.ref_brute_any, [_vp, _vp, _sz, _vp, C.c_int])
         _sig(lib.ref_max_threads, [], C.c_int)
         _sig(lib.ref_film_to_rgb8, [_vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, _vp])
+        _sig(lib.ref_obj_parse, [C.c_char_p, _sz, _vp, _sz], C.c_long)
         _ref = lib
     return _ref
 
@@ -78,6 +79,18 @@ def ref_film_to_rgb8(film, gamma: float = 2.2, tone_map: bool = False, exposure:
     f = np.ascontiguousarray(film, np.float32)
     out = np.empty(f.shape, np.uint8)
     _load_ref().ref_film_to_rgb8(f.ctypes.data, f.shape[1], f.shape[0], gamma, int(tone_map), exposure, out.ctypes.data)
+    return out
+
+
+def ref_obj_parse(text: bytes):
+    """The reference's ObjImporter::parse: (n, 9) float32 triangles, or None when it fails."""
+    import numpy as np
+    lib = _load_ref()
+    n = lib.ref_obj_parse(text, len(text), None, 0)
+    if n < 0:
+        return None
+    out = np.zeros((n, 9), np.float32)
+    lib.ref_obj_parse(text, len(text), out.ctypes.data, n)
     return out
 
 

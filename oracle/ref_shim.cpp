@@ -5,6 +5,7 @@
 // only instantiates the reference's types through their public headers.
 #include "render/acceleration/bvh.h"
 #include "render/core/film.h"
+#include "render/io/obj_importer.h"
 #include "render/geometry/plane.h"
 #include "render/geometry/quad.h"
 #include "render/geometry/sphere.h"
@@ -161,6 +162,22 @@ void ref_film_to_rgb8(const float* rgb, int w, int h, float gamma, int tone_map,
         for (int x = 0; x < w; ++x) film.addSample(x, y, v3(rgb + (size_t(y) * w + x) * 3), 1.0f);
     const std::vector<unsigned char> v = tone_map ? film.toRGB8ToneMapped(exposure, gamma) : film.toRGB8(gamma);
     for (size_t i = 0; i < v.size(); ++i) out[i] = v[i];
+}
+
+// ObjImporter::parse of `text`; the triangles as 9 floats each (positions[indices]).  Returns
+// the triangle count, or -1 when the importer fails.
+long ref_obj_parse(const char* text, size_t length, float* tris, size_t capacity) {
+    TriangleMesh mesh;
+    if (!ObjImporter::parse(std::string(text, length), mesh)) return -1;
+    const size_t n = mesh.triangleCount();
+    for (size_t t = 0; t < n && t < capacity; ++t)
+        for (int c = 0; c < 3; ++c) {
+            const Vec3& p = mesh.positions[size_t(mesh.indices[3 * t + c])];
+            tris[9 * t + 3 * c] = p.x;
+            tris[9 * t + 3 * c + 1] = p.y;
+            tris[9 * t + 3 * c + 2] = p.z;
+        }
+    return long(n);
 }
 
 }  // extern "C"

@@ -37,7 +37,7 @@ CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-fn
              "-I", INC, "-I", CSRC, "-Wall", "-Wno-unused-parameter"]
 
 HIP_SOURCES = ["context.hip", "trace.hip", "render.hip", "guide.hip", "bvh_gpu.hip"]
-CXX_SOURCES = ["bvh_build.cpp", "film_io.cpp"]
+CXX_SOURCES = ["bvh_build.cpp", "film_io.cpp", "obj_io.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -89,6 +89,7 @@ _sig(_lib.rbr_time_trace, [_vp, C.c_int, _vp, _sz, _vp, C.c_int, C.POINTER(C.c_f
 _sig(_lib.rbr_host_trace_closest, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
 _sig(_lib.rbr_host_trace_any, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
 
+_sig(_lib.rbr_obj_parse, [C.c_char_p, _sz, _vp, _sz, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz)])
 _sig(_lib.rbr_film_to_rgb8, [_vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, _vp])
 _sig(_lib.rbr_film_write_png, [C.c_char_p, _vp, C.c_int, C.c_int])
 _sig(_lib.rbr_film_write_pfm, [C.c_char_p, _vp, C.c_int, C.c_int])
@@ -322,6 +323,22 @@ class Scene:
 
 
 # ---------------------------------------------------------------- workloads
+
+def parse_obj(text: bytes) -> np.ndarray:
+    """Triangles (n, 9) of Wavefront OBJ text, as the reference's ObjImporter builds them."""
+    n, nn, nu = _sz(0), _sz(0), _sz(0)
+    if _lib.rbr_obj_parse(text, len(text), None, 0, C.byref(n), C.byref(nn), C.byref(nu)):
+        raise ValueError("OBJ import failed: a face names a missing vertex, or no triangle")
+    out = np.zeros((n.value, 9), np.float32)
+    if _lib.rbr_obj_parse(text, len(text), out.ctypes.data, n.value, C.byref(n), C.byref(nn), C.byref(nu)):
+        raise ValueError("OBJ import failed")
+    return out
+
+
+def load_obj(path: str) -> np.ndarray:
+    with open(path, "rb") as f:
+        return parse_obj(f.read())
+
 
 def _film(film) -> np.ndarray:
     f = np.ascontiguousarray(film, np.float32)
