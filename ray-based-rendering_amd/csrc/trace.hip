@@ -374,11 +374,13 @@ __global__ __launch_bounds__(64) void traceKernelPersistent(TraceArgs a, Persist
 }
 
 // Persistent waves with per-lane ray fetch: a lane whose ray is done goes idle, and once at
-// least p.refill lanes of the wave are idle the wave takes that many new rays from the device
-// counter (one atomic) and hands them to its idle lanes, so a wave no longer waits for its
-// slowest ray before taking work (incoherent rays vary several-fold in node count).  Between
-// refills each live lane runs one traversal round.  Every wave exits once the counter has
-// passed the ray count and all of its lanes are idle.
+// least p.refill lanes of the wave are idle the wave hands its idle lanes new rays, so a
+// wave no longer waits for its slowest ray before taking work (incoherent rays vary
+// several-fold in node count, and half of them may miss everything).  Rays come from a
+// 64-ray range the wave reserves with one atomic on the device counter (the rate of the
+// batch kernel's atomics); when the range runs dry mid-refill the rest comes from the next
+// range.  Between refills each live lane runs one traversal round.  Every wave exits once the
+// counter has passed the ray count and all of its lanes are idle.
 template <int W, bool Q, int LDS, bool ANY, bool MIXED>
 __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArgs p) {
     __shared__ uint2 ldsStack[LDS * 64];
@@ -389,18 +391,22 @@ __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArg
     const uint32_t lane = threadIdx.x;
     bool alive = false, more = true, occ = false;
     uint32_t i = 0, ref = kEmptyD, leaf = kEmptyD, bestId = RBR_MISS_ID;
+    uint32_t qBase = 0u, qEnd = 0u;   // reserved, not yet assigned rays (uniform)
     float bestT = 0.0f;
     DRay r{};
     while (true) {
         const uint64_t idle = __ballot(alive ? 0 : 1);
         const uint32_t nIdle = uint32_t(__popcll(idle));
         if (more && nIdle >= p.refill) {
-            uint32_t base = 0u;
-            if (lane == 0) base = atomicAdd(p.counter, nIdle);
-            base = __shfl(base, 0);
-            if (base + nIdle >= n) more = false;
+            const uint32_t avail = qEnd - qBase;
+            uint32_t base2 = 0u;
+            if (avail < nIdle) {
+                if (lane == 0) base2 = atomicAdd(p.counter, 64u);
+                base2 = __shfl(base2, 0);
+            }
             if (!alive) {
-                const uint32_t idx = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+                const uint32_t rank = uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+                const uint32_t idx = rank < avail ? qBase + rank : base2 + (rank - avail);
                 if (idx < n) {
                     i = idx;
                     r = loadRay(a.rays, idx);
@@ -412,6 +418,13 @@ __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArg
                     leaf = kEmptyD;
                     alive = true;
                 }
+            }
+            if (avail < nIdle) {
+                qBase = base2 + (nIdle - avail);
+                qEnd = base2 + 64u;
+                if (base2 >= n) more = false;
+            } else {
+                qBase += nIdle;
             }
         }
         if (!__any(alive ? 1 : 0)) {
