@@ -114,7 +114,17 @@ typedef struct rbr_camera {
     int32_t width, height;
 } rbr_camera;
 
-enum { RBR_INTEGRATOR_PATH = 0, RBR_INTEGRATOR_VOLPATH = 1 };
+/* PATH / VOLPATH: PathTracingIntegrator / VolumetricPathIntegrator.  NORMAL, DEPTH, AO: the
+ * reference's debug integrators (debug_integrators.cpp:5-60): shading normal mapped to
+ * [0, 1], linear depth (white near), ambient occlusion by cosine-weighted rays weighted by
+ * 1 - t / distance.  One rbr_render call takes either debug jobs only or path jobs only. */
+enum {
+    RBR_INTEGRATOR_PATH = 0,
+    RBR_INTEGRATOR_VOLPATH = 1,
+    RBR_INTEGRATOR_NORMAL = 2,
+    RBR_INTEGRATOR_DEPTH = 3,
+    RBR_INTEGRATOR_AO = 4
+};
 
 /* IntegratorConfig (integrator.h:22-28) plus the GPU stream seed. */
 typedef struct rbr_render_config {
@@ -127,6 +137,11 @@ typedef struct rbr_render_config {
     uint64_t seed;
     uint32_t sample_offset; /* first sample index (progressive)  */
     uint32_t guide_depth;   /* guided jobs: bounces 0..guide_depth-1 draw from the guide (0 = all) */
+    /* debug integrators (0 = the reference's defaults, debug_integrators.h:24-47) */
+    int32_t ao_samples;     /* AO rays per camera sample (16)        */
+    float ao_distance;      /* AO ray length (2.0)                   */
+    float depth_near;       /* DEPTH near plane (0.1)                */
+    float depth_far;        /* DEPTH far plane (100.0)               */
 } rbr_render_config;
 
 typedef struct rbr_ctx rbr_ctx;

@@ -3,6 +3,7 @@
 // INFRASTRUCTURE ONLY: it renders the same scene description the oracle and the GPU path
 // take, through the reference's own Scene / PathTracingIntegrator / VolumetricPathIntegrator,
 // so the oracle restatement can be compared against the reference itself.
+#include "render/renderers/debug_integrators.h"
 #include "render/camera/pinhole_camera.h"
 #include "render/geometry/plane.h"
 #include "render/geometry/quad.h"
@@ -121,7 +122,16 @@ double ref_render(void* h, const rbr_camera* cam, const rbr_render_config* cfg, 
     ic.useMIS = cfg->use_mis != 0;
     std::unique_ptr<Integrator> integ;
     if (cfg->integrator == RBR_INTEGRATOR_VOLPATH) integ = std::make_unique<VolumetricPathIntegrator>(ic, VolumeConfig{});
+    else if (cfg->integrator == RBR_INTEGRATOR_NORMAL) integ = std::make_unique<NormalIntegrator>();
+    else if (cfg->integrator == RBR_INTEGRATOR_DEPTH)
+        integ = (cfg->depth_near != 0.0f || cfg->depth_far != 0.0f)
+                    ? std::make_unique<DepthIntegrator>(cfg->depth_near, cfg->depth_far)
+                    : std::make_unique<DepthIntegrator>();
+    else if (cfg->integrator == RBR_INTEGRATOR_AO)
+        integ = std::make_unique<AOIntegrator>(cfg->ao_samples ? cfg->ao_samples : 16,
+                                               cfg->ao_distance > 0.0f ? cfg->ao_distance : 2.0f);
     else integ = std::make_unique<PathTracingIntegrator>(ic);
+    integ->config = ic;   // render() reads samplesPerPixel from it
     Film film(cam->width, cam->height);
     if (nthreads > 0) omp_set_num_threads(nthreads);
     double t0 = omp_get_wtime();

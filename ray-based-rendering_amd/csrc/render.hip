@@ -995,6 +995,103 @@ __global__ __launch_bounds__(256) void guideFinalizeKernel(float4* __restrict__ 
 
 __global__ void guideFinalizeEndKernel(uint32_t* counts, uint32_t cap) { counts[1] = min(counts[0], cap); }
 
+// ---------------------------------------------------------------- debug integrators
+
+// Hit point and ray-facing normal of extension-queue entry i (the surface part of shadeEntry:
+// SurfaceInteraction's normal after setFaceNormal).
+__device__ __forceinline__ bool debugHit(const DJob& jb, const Queues& q, int cur, uint32_t i, F3& P, F3& n, float& t) {
+    const uint2 h = q.hit[i];
+    if (h.x == RBR_MISS_ID) return false;
+    const float4 r0 = q.ray[cur][2 * size_t(i)], r1 = q.ray[cur][2 * size_t(i) + 1];
+    const F3 ro = mk(r0.x, r0.y, r0.z), rd = mk(r0.w, r1.x, r1.y);
+    t = __uint_as_float(h.y);
+    P = ro + rd * t;
+    const float4 s0 = jb.primShade[2 * size_t(h.x)], s1 = jb.primShade[2 * size_t(h.x) + 1];
+    const F3 outward = __float_as_uint(s0.w) == 1u ? (P - mk(s0.x, s0.y, s0.z)) * (1.0f / s1.x) : mk(s0.x, s0.y, s0.z);
+    n = dot(rd, outward) < 0.0f ? outward : -outward;
+    return true;
+}
+
+// NormalIntegrator / DepthIntegrator / AOIntegrator::Li (debug_integrators.cpp:5-60) for the
+// camera rays of debug jobs.  Normal and depth are final here; AO sends its rays (cosine-
+// weighted about the normal, spawned as spawnRay does, length aoDistance) through the
+// shadow queue, traced for the closest hit, and debugGatherKernel weighs them.
+__global__ __launch_bounds__(256) void debugShadeKernel(const DJob* __restrict__ jobs, int njobs, uint32_t maxSlots,
+                                                        int cur, Queues q) {
+    const uint32_t count = q.counts[cur];
+    for (uint32_t b = blockIdx.x * 256u; b < count; b += gridDim.x * 256u) {
+        const uint32_t i = b + threadIdx.x;
+        const bool active = i < count;
+        uint32_t p = 0;
+        int j = 0;
+        bool hit = false;
+        F3 P = mk(0, 0, 0), n = mk(0, 0, 1), L = mk(0, 0, 0);
+        float t = 0.0f;
+        Rng rng{0};
+        if (active) {
+            p = q.path[cur][i];
+            j = findJob(jobs, njobs, p);
+            const DJob& jb = jobs[j];
+            const uint32_t local = p - jb.pathBase;
+            rng = pathRng(jb.seed, local % jb.pixels, jb.sampleBase + local / jb.pixels, jb.pixels);
+            hit = debugHit(jb, q, cur, i, P, n, t);
+            if (jb.integrator == RBR_INTEGRATOR_NORMAL) {
+                if (hit) L = (n + mk(1.0f, 1.0f, 1.0f)) * 0.5f;
+            } else if (jb.integrator == RBR_INTEGRATOR_DEPTH) {
+                float v = 1.0f;
+                if (hit) {
+                    float d = (t - jb.depthNear) / (jb.depthFar - jb.depthNear);
+                    d = fmaxf(0.0f, fminf(1.0f, d));
+                    v = 1.0f - d;
+                }
+                L = mk(v, v, v);
+            } else if (!hit) {
+                L = mk(1.0f, 1.0f, 1.0f);   // AO miss
+            }
+            q.L[p] = make_float4(L.x, L.y, L.z, 0.0f);
+            for (uint32_t k = 0; k < maxSlots; ++k) q.sindex[size_t(p) * maxSlots + k] = -1;
+        }
+        const DJob& jb = jobs[j];
+        const bool ao = active && hit && jb.integrator == RBR_INTEGRATOR_AO;
+        const Frame frame(n);
+        for (uint32_t k = 0; k < maxSlots; ++k) {
+            const bool want = ao && k < jb.aoSamples;
+            F3 so = mk(0, 0, 0), sd = mk(0, 0, 1);
+            if (want) {   // sampling::cosineHemisphere (sampler.h:73-81) about the normal
+                const float u1 = rng.u(16 + 2 * k), u2 = rng.u(17 + 2 * k);
+                const float rr = sqrtf(u1);
+                float sn, cs;
+                sincosf(kTwoPi * u2, &sn, &cs);
+                spawn(P, n, frame.toWorld(mk(rr * cs, rr * sn, sqrtf(fmaxf(0.0f, 1.0f - u1)))), so, sd);
+            }
+            const uint32_t slot = waveAppend(&q.counts[2], want);
+            if (want) {
+                putRay(q.sray, slot, so, sd, kEpsD, jb.aoDistance);
+                q.sroot[slot] = jb.root;
+                q.sindex[size_t(p) * maxSlots + k] = int32_t(slot);
+            }
+        }
+    }
+}
+
+// AO: 1 - (sum over the path's rays that hit something of 1 - t / distance) / samples.
+__global__ __launch_bounds__(256) void debugGatherKernel(const DJob* __restrict__ jobs, int njobs, uint32_t maxSlots,
+                                                         int cur, Queues q, const uint2* __restrict__ shit) {
+    const uint32_t count = q.counts[cur];
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) {
+        const uint32_t p = q.path[cur][i];
+        const DJob& jb = jobs[findJob(jobs, njobs, p)];
+        if (jb.integrator != RBR_INTEGRATOR_AO || q.sindex[size_t(p) * maxSlots] < 0) continue;
+        float occlusion = 0.0f;
+        for (uint32_t k = 0; k < jb.aoSamples; ++k) {
+            const uint2 h = shit[q.sindex[size_t(p) * maxSlots + k]];
+            if (h.x != RBR_MISS_ID) occlusion += 1.0f - (__uint_as_float(h.y) / jb.aoDistance);
+        }
+        const float ao = 1.0f - (occlusion / float(jb.aoSamples));
+        q.L[p] = make_float4(ao, ao, ao, 0.0f);
+    }
+}
+
 __global__ __launch_bounds__(256) void resetCountsKernel(uint32_t* counts, int nxt) {
     counts[nxt] = 0u;
     counts[2] = 0u;
@@ -1116,7 +1213,7 @@ DMaterial makeMaterial(const rbr_material& in) {
 
 // Per-context render state: the path pool and queues, grown on demand.
 struct RenderState {
-    DevBuf ray[2], path[2], root[2], hit, sray, sroot, scontrib, socc, sindex, T, L, counts, jobs, guideOut, recOf;
+    DevBuf ray[2], path[2], root[2], hit, sray, sroot, scontrib, socc, sindex, T, L, counts, jobs, guideOut, recOf, shit;
     DevBuf arenaNodes, arenaPrims, arenaQuadN, arenaShade, arenaMats, arenaLights;
     std::vector<DevBuf> films, momS, momQ;
     double lastTotalMs = 0.0;
@@ -1281,7 +1378,8 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     std::map<const rbr_scene*, int> sceneIdx;
     int width = 0;
     uint32_t maxStack = 1, maxDepthAll = 0, maxLights = 0;
-    bool anyMixed = false, anyPlanes = false, anyGuided = false, anyRecord = false;
+    bool anyMixed = false, anyPlanes = false, anyGuided = false, anyRecord = false, anyDebug = false, anyAO = false;
+    uint32_t aoSlots = 0;
     for (int j = 0; j < njobs; ++j) {
         const rbr_job& jb = jobs[j];
         if (!jb.scene || jb.scene->ctx != ctx) return fail(ctx, RBR_ERR_INVALID, "job scene missing or from another context");
@@ -1289,8 +1387,18 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         if (!jb.film_rgb) return fail(ctx, RBR_ERR_INVALID, "job film is null");
         if (jb.camera.width < 1 || jb.camera.height < 1 || jb.config.spp < 1 || jb.config.max_depth < 1)
             return fail(ctx, RBR_ERR_INVALID, "job needs width, height, spp and max_depth >= 1");
-        if (jb.config.integrator != RBR_INTEGRATOR_PATH && jb.config.integrator != RBR_INTEGRATOR_VOLPATH)
+        if (jb.config.integrator < RBR_INTEGRATOR_PATH || jb.config.integrator > RBR_INTEGRATOR_AO)
             return fail(ctx, RBR_ERR_INVALID, "unknown integrator");
+        const bool dbg = jb.config.integrator >= RBR_INTEGRATOR_NORMAL;
+        if (j > 0 && dbg != anyDebug)
+            return fail(ctx, RBR_ERR_UNSUPPORTED, "debug integrators cannot share a launch with path integrators");
+        anyDebug = dbg;
+        anyAO |= jb.config.integrator == RBR_INTEGRATOR_AO;
+        if (jb.config.integrator == RBR_INTEGRATOR_AO) {
+            if (jb.config.ao_samples < 0 || jb.config.ao_samples > 256 || jb.config.ao_distance < 0.0f)
+                return fail(ctx, RBR_ERR_INVALID, "ao_samples must be in 0..256 and ao_distance >= 0");
+            aoSlots = std::max(aoSlots, uint32_t(jb.config.ao_samples ? jb.config.ao_samples : 16));
+        }
         if (jb.config.guided < 0 || jb.config.guided > 3) return fail(ctx, RBR_ERR_INVALID, "guided must be 0..3");
         if (jb.config.guided) {
             if (jb.config.integrator != RBR_INTEGRATOR_PATH)
@@ -1315,6 +1423,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         maxDepthAll = std::max(maxDepthAll, uint32_t(jb.config.max_depth));
         maxLights = std::max(maxLights, uint32_t(s->lights.size()));
     }
+    if (anyDebug) maxLights = aoSlots;   // debug launches use the shadow queue for AO rays only
     if (width == 0) width = 4;
     if (anyPlanes && scenes.size() > 1) return fail(ctx, RBR_ERR_UNSUPPORTED, "infinite planes in a multi-scene render");
     if (!ctx->render) ctx->render = new RenderState();
@@ -1457,6 +1566,10 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         d.useMIS = jb.config.use_mis ? 1u : 0u;
         d.guided = uint32_t(jb.config.guided);
         d.guideDepth = jb.config.guide_depth ? jb.config.guide_depth : 0xFFFFFFFFu;
+        d.aoSamples = uint32_t(jb.config.ao_samples ? jb.config.ao_samples : 16);
+        d.aoDistance = jb.config.ao_distance > 0.0f ? jb.config.ao_distance : 2.0f;
+        d.depthNear = jb.config.depth_near != 0.0f || jb.config.depth_far != 0.0f ? jb.config.depth_near : 0.1f;
+        d.depthFar = jb.config.depth_near != 0.0f || jb.config.depth_far != 0.0f ? jb.config.depth_far : 100.0f;
         d.root = jb.scene->bvh.empty() ? kEmpty : sl.nodeOff;
         d.pixels = uint32_t(jb.camera.width) * uint32_t(jb.camera.height);
         d.seed = jb.config.seed;
@@ -1489,6 +1602,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     for (auto& a : allocs)
         if ((rc = ensureBuffer(ctx, *a.b, a.bytes))) return rc;
     if (anyGuided && (rc = ensureBuffer(ctx, R.guideOut, size_t(P) * kGuideRecBytes))) return rc;
+    if (anyAO && (rc = ensureBuffer(ctx, R.shit, size_t(P) * ML * sizeof(uint2)))) return rc;
     if (anyRecord) {
         if ((rc = ensureBuffer(ctx, R.recOf, size_t(P) * 4))) return rc;
         if ((rc = guideEnsureRecords(ctx, *ctx->guide))) return rc;
@@ -1514,7 +1628,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     q.recCount = anyRecord ? static_cast<uint32_t*>(ctx->guide->dRecCount.ptr) : nullptr;
     q.recCap = anyRecord ? ctx->guide->recCap : 0u;
     q.perm = q.sortKey = q.bins = nullptr;
-    if (ctx->renderSort) {
+    if (ctx->renderSort && !anyDebug) {
         if ((rc = ensureBuffer(ctx, R.perm, size_t(P) * 4)) || (rc = ensureBuffer(ctx, R.sortKey, size_t(P) * 4)) ||
             (rc = ensureBuffer(ctx, R.bins, kSortBins * 4)))
             return rc;
@@ -1555,7 +1669,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
             dj[j].pathCount = take[j] * dj[j].pixels;
             dj[j].sampleBase = jobs[j].config.sample_offset + done[j];
             base += dj[j].pathCount;
-            if (take[j]) depthRound = std::max(depthRound, uint32_t(jobs[j].config.max_depth));
+            if (take[j]) depthRound = std::max(depthRound, anyDebug ? 1u : uint32_t(jobs[j].config.max_depth));
         }
         e = hipMemcpyAsync(R.jobs.ptr, dj.data(), sizeof(DJob) * njobs, hipMemcpyHostToDevice, ctx->stream);
         if (e != hipSuccess) return failHip(ctx, e, "job upload");
@@ -1579,6 +1693,23 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
                 a.hits = q.hit;
                 a.occ = nullptr;
                 if (int r2 = launchTraversal(ctx, a, width, maxStack, false)) return r2;
+                if (anyDebug) {
+                    hipLaunchKernelGGL(debugShadeKernel, dim3(gridPersist), dim3(256), 0, ctx->stream, djd, njobs, ML, cur, q);
+                    if (anyAO) {
+                        TraceArgs b = ta;
+                        b.rays = q.sray;
+                        b.n = used * ML;
+                        b.nDev = q.counts + 2;
+                        b.rootOf = q.sroot;
+                        b.hits = static_cast<uint2*>(R.shit.ptr);
+                        b.occ = nullptr;
+                        if (int r2 = launchTraversal(ctx, b, width, maxStack, false)) return r2;
+                        hipLaunchKernelGGL(debugGatherKernel, dim3(gridPersist), dim3(256), 0, ctx->stream, djd, njobs, ML,
+                                           cur, q, static_cast<const uint2*>(R.shit.ptr));
+                    }
+                    cur ^= 1;
+                    continue;
+                }
                 if (anyGuided) {
                     GuideInferArgs gi{};
                     gi.g = guideDev(*ctx->guide);
@@ -1640,7 +1771,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
             auto putp = [&](const void* ptr) { key.push_back(uint64_t(reinterpret_cast<uintptr_t>(ptr))); };
             auto putf = [&](float f) { uint32_t u; std::memcpy(&u, &f, 4); key.push_back(u); };
             put(used); put(depthRound); put(maxLights); put(uint64_t(width)); put(maxStack); put(uint64_t(njobs));
-            put(anyGuided); put(anyRecord);
+            put(anyGuided); put(anyRecord); put(anyDebug); put(anyAO); putp(R.shit.ptr);
             for (int j = 0; j < njobs; ++j) { put(take[j]); put(dj[j].pixels); }
             putp(ta.nodes); putp(ta.prims); putp(ta.quadN); putp(ta.planes); putp(ta.planeIds); put(ta.nplanes); put(ta.mixed);
             for (int k = 0; k < 2; ++k) { putp(q.ray[k]); putp(q.path[k]); putp(q.root[k]); }

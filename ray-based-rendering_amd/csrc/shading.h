@@ -75,6 +75,9 @@ struct DJob {
     uint32_t integrator, maxDepth, useNEE, useMIS;
     uint32_t guided;             // bit 0: guided direction sampling, bit 1: record training samples
     uint32_t guideDepth;         // bounces below this draw from the guide
+    uint32_t aoSamples;          // AO: rays per camera sample
+    float aoDistance;            // AO: ray length
+    float depthNear, depthFar;   // DEPTH: mapping range
     uint32_t root;               // BVH root node in the node arena
     uint32_t pathBase, pathCount;   // pool slots of this job in the current round
     uint32_t sampleBase;         // sample index of the job's first pool slot this round

@@ -3,6 +3,7 @@ integrators, test_oracle_render.py).  The two use different random streams, so p
 statistical: per-pixel z-scores of the mean radiance over independent samples
     z = (m_gpu - m_ref) / sqrt(var_gpu / n_gpu + var_ref / n_ref)
 must look standard normal (no bias), and whole-image means must agree."""
+import ctypes as C
 import sys
 
 import numpy as np
@@ -169,3 +170,66 @@ def test_trace_fetch_modes_do_not_change_images(ctx, mode):
         ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
     for k in range(len(jobs)):
         assert np.array_equal(out[k], base[k]), k
+
+
+@pytest.mark.parametrize("integrator", [R.NORMAL, R.DEPTH, R.AO])
+def test_debug_integrators_match_reference(ctx, integrator):
+    """NormalIntegrator / DepthIntegrator / AOIntegrator (debug_integrators.cpp) on the GPU
+    against the reference's own integrators (compiled from its sources): per-pixel z-test
+    (the reference's variance taken as the GPU's, both estimate the same per-sample law),
+    and for the normal image every pixel whose samples all hit one flat surface equals the
+    reference's film value bit for bit (same float summation of equal values)."""
+    say(f"case: debug integrator {integrator}")
+    desc = S.cornell_c0(48, 36)
+    cfg = dict(depth_near=600.0, depth_far=1600.0) if integrator == R.DEPTH else {}
+    if integrator == R.AO:
+        cfg = dict(ao_samples=8, ao_distance=120.0)
+    sc = R.scene_from_desc(ctx, desc)
+    n = 256
+    _, g1, g2 = R.render_moments(ctx, dict(scene=sc, spp=n, depth=1, seed=13, integrator=integrator, **cfg))
+    ref = O.RefRender(desc)
+    rc = ref._R.c_config(n, 1, 0, integrator, True, True, 0, 0, 0, cfg.get("ao_samples", 0), cfg.get("ao_distance", 0.0),
+                         cfg.get("depth_near", 0.0), cfg.get("depth_far", 0.0))
+    rfilm = np.zeros((36, 48, 3), np.float32)
+    rcam = ref._R.c_camera(desc.camera)   # kept alive across the call
+    ref._lib.ref_render(ref._h, C.addressof(rcam), C.addressof(rc), rfilm.ctypes.data, 0)
+    ref.close()
+    mg = (g1 / n).reshape(-1, 3)
+    vg = np.maximum(g2 / n - (g1 / n) ** 2, 0.0).reshape(-1, 3)
+    mr = rfilm.reshape(-1, 3).astype(np.float64)
+    se = np.sqrt(2.0 * vg / n)
+    ok = se > 0
+    z = np.zeros_like(mg)
+    z[ok] = (mg[ok] - mr[ok]) / se[ok]
+    # the GPU variance stands in for the reference's, so pixels where rare events (an AO ray
+    # that only now and then finds a surface) give a tiny GPU variance can show large |z|:
+    # the checks are the median z, the tail fraction and the z of the whole-image mean
+    zi = float((mg.sum() - mr.sum()) / np.sqrt(max((2.0 * vg / n).sum(), 1e-30)))
+    med = float(np.median(z[ok])) if ok.any() else 0.0
+    say(f"  median z {med:+.3f}  mean z {z[ok].mean() if ok.any() else 0:+.3f}  "
+        f"|z|>5 {(np.abs(z[ok]) > 5).mean() if ok.any() else 0:.4f}  image z {zi:+.2f}  "
+        f"means {mg.mean():.5f} vs {mr.mean():.5f}")
+    if ok.any():
+        assert abs(med) < 0.15 and (np.abs(z[ok]) > 5).mean() < 0.01 and abs(zi) < 5.0
+    assert abs(mg.mean() - mr.mean()) < 0.01 * max(mr.mean(), 1e-3) + 2e-3
+    if integrator == R.NORMAL:
+        # both films sum the samples of a pixel in float, in sample order, then scale by
+        # 1/spp, so a pixel whose samples are all equal has the same bits in both
+        film = R.render(ctx, [dict(scene=sc, spp=n, depth=1, seed=13, integrator=integrator)])[0].reshape(-1, 3)
+        flat = (vg == 0).all(axis=1)
+        same = (film == rfilm.reshape(-1, 3)).all(axis=1)
+        say(f"  constant pixels {flat.mean():.3f}, equal among them {same[flat].mean():.4f}")
+        assert flat.mean() > 0.7 and same[flat].mean() > 0.99
+
+
+def test_debug_integrators_launch_rules(ctx):
+    sc = R.scene_from_desc(ctx, S.cornell_c0(16, 16))
+    with pytest.raises(R.RbrError) as e:
+        R.render(ctx, [dict(scene=sc, spp=1, depth=1, integrator=R.AO), dict(scene=sc, spp=1, depth=4)])
+    assert e.value.code == 5
+    with pytest.raises(R.RbrError):
+        R.render(ctx, [dict(scene=sc, spp=1, depth=1, integrator=R.AO, ao_samples=1000)])
+    # several debug jobs share one launch
+    a, b = R.render(ctx, [dict(scene=sc, spp=4, depth=1, integrator=R.NORMAL),
+                          dict(scene=sc, spp=4, depth=1, integrator=R.AO, ao_samples=4, ao_distance=50.0)])
+    assert a.shape == b.shape == (16, 16, 3) and np.isfinite(a).all() and (b >= 0).all() and (b <= 1).all()
