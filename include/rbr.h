@@ -236,7 +236,9 @@ int rbr_time_trace(rbr_scene* scene, int kind, const rbr_ray* d_rays, size_t n, 
 
 /* Traversal launch tuning (the autotuner's knobs): mode 0 = persistent waves that fetch
  * 64-ray batches from a device counter (default), 1 = one ray per lane, 2 / 3 / 4 =
- * persistent waves whose idle lanes take new rays once 8 / 16 / 32 of them are idle; lds_depth = LDS
+ * persistent waves whose idle lanes take new rays once 8 / 16 / 32 of them are idle,
+ * 5 / 6 / 7 / 8 = persistent waves that advance every lane by one node and one primitive per
+ * iteration and refill once 4 / 8 / 16 / 32 lanes are idle; lds_depth = LDS
  * stack entries per lane (8, 12, 16, 24, 32; deeper entries spill to a per-wave global
  * buffer); waves_per_cu = persistent waves launched per compute unit (1..32). */
 int rbr_set_trace_tuning(rbr_ctx* ctx, int mode, int lds_depth, int waves_per_cu);

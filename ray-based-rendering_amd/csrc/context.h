@@ -32,7 +32,7 @@ struct rbr_ctx {
     rbr::RenderState* render = nullptr;
     rbr::GuideState* guide = nullptr;
     // Traversal tuning (rbr_set_trace_tuning) and persistent-launch state.
-    int traceMode = 0;                 // 0 persistent waves, 1 one ray per lane, 2-4 per-lane fetch
+    int traceMode = 0;                 // 0 persistent waves, 1 one ray per lane, 2-4 per-lane fetch, 5-8 steps
     int ldsDepth = rbr::kLdsDepthDefault;
     int wavesPerCu = rbr::kWavesPerCuDefault;
     rbr::DevBuf dCounter, dSpill;

@@ -106,11 +106,14 @@ int rbr::launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack
     }
     const unsigned grid = unsigned(std::max(1, ctx->cuCount) * ctx->wavesPerCu);
     const uint32_t spillDepth = std::max<uint32_t>(1u, maxStack > uint32_t(ctx->ldsDepth) ? maxStack - ctx->ldsDepth : 1u);
-    if (int rc = ensureBuffer(ctx, ctx->dCounter, sizeof(uint32_t))) return rc;
+    if (int rc = ensureBuffer(ctx, ctx->dCounter, kCounterBytes)) return rc;
     if (int rc = ensureBuffer(ctx, ctx->dSpill, size_t(grid) * spillDepth * 64 * sizeof(uint2))) return rc;
-    // modes 2, 3, 4: per-lane fetch once 8, 16, 32 lanes of a wave are idle
-    const uint32_t refill = ctx->traceMode >= 2 ? (4u << (ctx->traceMode - 1)) : 0u;
-    PersistArgs p{static_cast<uint32_t*>(ctx->dCounter.ptr), static_cast<uint2*>(ctx->dSpill.ptr), spillDepth, refill};
+    // modes 2, 3, 4: per-lane fetch once 8, 16, 32 lanes of a wave are idle (while-while rounds);
+    // modes 5, 6, 7, 8: per-lane fetch once 4, 8, 16, 32 lanes are idle (single steps)
+    const int m = ctx->traceMode;
+    const uint32_t refill = m >= 5 ? (2u << (m - 4)) : m >= 2 ? (4u << (m - 1)) : 0u;
+    PersistArgs p{static_cast<uint32_t*>(ctx->dCounter.ptr), static_cast<uint2*>(ctx->dSpill.ptr), spillDepth, refill,
+                  m >= 5 ? 1u : 0u};
     e = launchTrace(a, width, any, ctx->ldsDepth, &p, grid, ctx->stream);
     if (e != hipSuccess) return failHip(ctx, e, "trace launch");
     return RBR_OK;
@@ -119,7 +122,7 @@ int rbr::launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack
 int rbr::prepareTraversal(rbr_ctx* ctx, uint32_t maxStack) {
     const unsigned grid = unsigned(std::max(1, ctx->cuCount) * ctx->wavesPerCu);
     const uint32_t spillDepth = std::max<uint32_t>(1u, maxStack > uint32_t(ctx->ldsDepth) ? maxStack - ctx->ldsDepth : 1u);
-    if (int rc = ensureBuffer(ctx, ctx->dCounter, sizeof(uint32_t))) return rc;
+    if (int rc = ensureBuffer(ctx, ctx->dCounter, kCounterBytes)) return rc;
     return ensureBuffer(ctx, ctx->dSpill, size_t(grid) * spillDepth * 64 * sizeof(uint2));
 }
 
@@ -243,7 +246,8 @@ int rbr_calib_copy(rbr_ctx* ctx, void* dst, const void* src, size_t bytes) {
 
 int rbr_set_trace_tuning(rbr_ctx* ctx, int mode, int lds_depth, int waves_per_cu) {
     if (!ctx) return RBR_ERR_INVALID;
-    if (mode < 0 || mode > 4) return fail(ctx, RBR_ERR_INVALID, "mode must be 0 (persistent), 1 (lane) or 2-4 (per-lane fetch)");
+    if (mode < 0 || mode > 8)
+        return fail(ctx, RBR_ERR_INVALID, "mode must be 0 (persistent), 1 (lane), 2-4 (per-lane fetch) or 5-8 (steps)");
     if (lds_depth != 8 && lds_depth != 12 && lds_depth != 16 && lds_depth != 24 && lds_depth != 32)
         return fail(ctx, RBR_ERR_INVALID, "lds_depth must be 8, 12, 16, 24 or 32");
     if (waves_per_cu < 1 || waves_per_cu > 32) return fail(ctx, RBR_ERR_INVALID, "waves_per_cu must be in 1..32");

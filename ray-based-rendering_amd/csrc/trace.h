@@ -35,11 +35,17 @@ struct TraceArgs {
 
 // Persistent-wave launch state: a batch counter (zeroed by a one-lane kernel launched just
 // before the traversal on the same stream) and the per-wave spill rows of the stack.
+// Batch counters of the persistent kernels: kSlices counters kSliceStride words apart.
+constexpr uint32_t kSlices = 16;
+constexpr uint32_t kSliceStride = 64;
+constexpr size_t kCounterBytes = size_t(kSlices) * kSliceStride * sizeof(uint32_t);
+
 struct PersistArgs {
-    uint32_t* counter;
+    uint32_t* counter;        // kSlices batch counters
     uint2* spill;             // [grid][spillDepth][64]
     uint32_t spillDepth;
     uint32_t refill;          // 0: waves take 64-ray batches; else per-lane fetch once this many lanes idle
+    uint32_t step;            // with refill: single node / primitive steps instead of while-while rounds
 };
 
 // ldsDepth in {8, 12, 16, 24, 32}; p == nullptr selects the one-ray-per-lane kernel (which

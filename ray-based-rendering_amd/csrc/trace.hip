@@ -27,6 +27,7 @@ namespace {
 
 __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ float4 u4f(uint4 v) { return make_float4(u2f(v.x), u2f(v.y), u2f(v.z), u2f(v.w)); }
 
 struct DRay {
     V o, d;
@@ -41,14 +42,8 @@ __device__ __forceinline__ float safeInv(float x) {
     return __builtin_amdgcn_rcpf(y);
 }
 
-__device__ __forceinline__ DRay loadRay(const float4* rays, uint32_t i) {
-    float4 a = rays[2 * size_t(i)];
-    float4 b = rays[2 * size_t(i) + 1];
-    DRay r;
-    r.o = V{a.x, a.y, a.z};
-    r.d = V{a.w, b.x, b.y};
-    r.tmin = b.z;
-    r.tmax = b.w;
+// The reciprocal direction and o/d the slab tests use, from o and d.
+__device__ __forceinline__ void setupRay(DRay& r) {
     r.ix = safeInv(r.d.x);
     r.iy = safeInv(r.d.y);
     r.iz = safeInv(r.d.z);
@@ -58,6 +53,17 @@ __device__ __forceinline__ DRay loadRay(const float4* rays, uint32_t i) {
     r.nx = r.ix < 0.0f;
     r.ny = r.iy < 0.0f;
     r.nz = r.iz < 0.0f;
+}
+
+// A ray from its two 16-byte rows (o.xyz d.x | d.yz tmin tmax).
+__device__ __forceinline__ DRay loadRay(const float4* rays, uint32_t i) {
+    const float4 a = rays[2 * size_t(i)], b = rays[2 * size_t(i) + 1];
+    DRay r;
+    r.o = V{a.x, a.y, a.z};
+    r.d = V{a.w, b.x, b.y};
+    r.tmin = b.z;
+    r.tmax = b.w;
+    setupRay(r);
     return r;
 }
 
@@ -77,23 +83,37 @@ __device__ __forceinline__ float slab(const DRay& r, float lx, float hx, float l
     return t0;
 }
 
+// The 16-byte rows of a node the child tests read: quantized BVH4 64 B, quantized BVH8 the
+// first 96 B of its 128, float BVH4 128 B, float BVH8 256 B (bvh.h layouts).
+template <int W, bool Q>
+struct NodeRows {
+    static constexpr int kRows = Q ? (W == 4 ? 4 : 6) : (W == 4 ? 8 : 16);
+    static constexpr int kStride = Q ? (W == 4 ? 4 : 8) : (W == 4 ? 8 : 16);   // rows per node
+    uint4 v[kRows];
+};
+
+template <int W, bool Q>
+__device__ __forceinline__ void loadNode(const float4* nodes, uint32_t ref, NodeRows<W, Q>& nr) {
+    const uint4* nd = reinterpret_cast<const uint4*>(nodes) + size_t(ref) * NodeRows<W, Q>::kStride;
+#pragma unroll
+    for (int i = 0; i < NodeRows<W, Q>::kRows; ++i) nr.v[i] = nd[i];
+}
+
 // Children of a quantized node (bvh.h): the ray is moved into the node's grid once,
 // t(q) = q * (s * 1/d) + (p - o) * 1/d, so each of the 6 planes of a child costs one FMA on
 // its byte.  Fills entry distances (tc) and refs (rc) like the float path.
 template <int W>
-__device__ __forceinline__ void childrenQ(const float4* nodes, uint32_t ref, const DRay& r, float tfar, float* tc,
+__device__ __forceinline__ void childrenQ(const NodeRows<W, true>& nr, const DRay& r, float tfar, float* tc,
                                           uint32_t* rc) {
     constexpr int Q4 = W / 4;
-    constexpr int NV = W == 4 ? 4 : 6;   // uint4 rows that hold data (64 B / 96 B)
-    const uint4* nd = reinterpret_cast<const uint4*>(nodes) + size_t(ref) * (W == 4 ? 4 : 8);
+    constexpr int NV = NodeRows<W, true>::kRows;
     uint32_t w[4 * NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-        const uint4 v = nd[i];
-        w[4 * i] = v.x;
-        w[4 * i + 1] = v.y;
-        w[4 * i + 2] = v.z;
-        w[4 * i + 3] = v.w;
+        w[4 * i] = nr.v[i].x;
+        w[4 * i + 1] = nr.v[i].y;
+        w[4 * i + 2] = nr.v[i].z;
+        w[4 * i + 3] = nr.v[i].w;
     }
     const float ax = u2f((w[3] & 0xFFu) << 23) * r.ix;
     const float ay = u2f(((w[3] >> 8) & 0xFFu) << 23) * r.iy;
@@ -132,15 +152,14 @@ __device__ __forceinline__ void childrenQ(const float4* nodes, uint32_t ref, con
 
 // Children of a float node (bvh.h), same outputs as childrenQ.
 template <int W>
-__device__ __forceinline__ void childrenF(const float4* nodes, uint32_t ref, const DRay& r, float tfar, float* tc,
+__device__ __forceinline__ void childrenF(const NodeRows<W, false>& nr, const DRay& r, float tfar, float* tc,
                                           uint32_t* rc) {
-    const float4* nd = nodes + size_t(ref) * (W == 4 ? 8 : 16);
+    constexpr int q = W / 4;
 #pragma unroll
-    for (int half = 0; half < W / 4; ++half) {
-        const int q = W / 4;
-        float4 lx = nd[0 * q + half], hx = nd[1 * q + half], ly = nd[2 * q + half];
-        float4 hy = nd[3 * q + half], lz = nd[4 * q + half], hz = nd[5 * q + half];
-        uint4 ch = *reinterpret_cast<const uint4*>(nd + 6 * q + half);
+    for (int half = 0; half < q; ++half) {
+        const float4 lx = u4f(nr.v[0 * q + half]), hx = u4f(nr.v[1 * q + half]), ly = u4f(nr.v[2 * q + half]);
+        const float4 hy = u4f(nr.v[3 * q + half]), lz = u4f(nr.v[4 * q + half]), hz = u4f(nr.v[5 * q + half]);
+        const uint4 ch = nr.v[6 * q + half];
         const float* LX = &lx.x; const float* HX = &hx.x; const float* LY = &ly.x;
         const float* HY = &hy.x; const float* LZ = &lz.x; const float* HZ = &hz.x;
         const uint32_t* CH = &ch.x;
@@ -154,10 +173,19 @@ __device__ __forceinline__ void childrenF(const float4* nodes, uint32_t ref, con
     }
 }
 
+template <int W, bool Q>
+__device__ __forceinline__ void children(const NodeRows<W, Q>& nr, const DRay& r, float tfar, float* tc,
+                                         uint32_t* rc) {
+    if constexpr (Q) childrenQ<W>(nr, r, tfar, tc, rc);
+    else childrenF<W>(nr, r, tfar, tc, rc);
+}
+
+// One primitive record (3 rows: v0 / centre / corner + id, e1 / radius + kind, e2 + quad
+// normal index) against the ray.
 template <bool MIXED>
-__device__ __forceinline__ bool testPrim(const TraceArgs& a, const float4* rec, const DRay& r, float tmax, float& t,
-                                         uint32_t& id) {
-    float4 p0 = rec[0], p1 = rec[1], p2 = rec[2];
+__device__ __forceinline__ bool testPrimRows(const TraceArgs& a, const float4* rec, const DRay& r, float tmax, float& t,
+                                             uint32_t& id) {
+    const float4 p0 = rec[0], p1 = rec[1], p2 = rec[2];
     id = f2u(p0.w);
     if (!MIXED) return triTest(r.o, r.d, V{p0.x, p0.y, p0.z}, V{p1.x, p1.y, p1.z}, V{p2.x, p2.y, p2.z}, r.tmin, tmax, t);
     uint32_t kind = f2u(p1.w);
@@ -167,6 +195,12 @@ __device__ __forceinline__ bool testPrim(const TraceArgs& a, const float4* rec, 
     float4 n = a.quadN[f2u(p2.w)];
     return quadTest(r.o, r.d, V{p0.x, p0.y, p0.z}, V{p1.x, p1.y, p1.z}, V{p2.x, p2.y, p2.z}, V{n.x, n.y, n.z}, r.tmin,
                     tmax, t);
+}
+
+template <bool MIXED>
+__device__ __forceinline__ bool testPrim(const TraceArgs& a, const float4* rec, const DRay& r, float tmax, float& t,
+                                         uint32_t& id) {
+    return testPrimRows<MIXED>(a, rec, r, tmax, t, id);
 }
 
 template <typename T>
@@ -183,20 +217,30 @@ __device__ __forceinline__ void cswap(float& ta, T& ra, float& tb, T& rb) {
 // Traversal stack: LDS rows [depth][lane] for the first LDS entries of each lane; deeper
 // entries spill to a per-wave region of a global buffer (persistent kernel) or to private
 // memory (one-ray-per-lane kernel).
+// Entries are (ref, entry distance) packed in 64 bits.  The row pointers carry their address
+// space: with plain pointers the compiler merges the LDS and global accesses of push/pop into
+// one select of pointers and a flat access, which waits on both memory counters and costs
+// several instructions to form the address.
+typedef __attribute__((address_space(3))) unsigned long long LdsU64;
+typedef __attribute__((address_space(1))) unsigned long long GlobalU64;
+
 template <int LDS>
 struct GlobalSpillStack {
-    uint2* lds;     // this wave's LDS rows
-    uint2* spill;   // this wave's spill rows [depth - LDS][64]
+    LdsU64* lds;        // this wave's LDS rows
+    GlobalU64* spill;   // this wave's spill rows [depth - LDS][64]
     int sp = 0;
     __device__ __forceinline__ void push(uint32_t ref, float t) {
-        uint2 e = make_uint2(ref, f2u(t));
+        const unsigned long long e = (static_cast<unsigned long long>(f2u(t)) << 32) | ref;
         if (sp < LDS) lds[sp * 64 + threadIdx.x] = e;
         else spill[(sp - LDS) * 64 + threadIdx.x] = e;
         ++sp;
     }
     __device__ __forceinline__ uint2 pop() {
         --sp;
-        return sp < LDS ? lds[sp * 64 + threadIdx.x] : spill[(sp - LDS) * 64 + threadIdx.x];
+        unsigned long long e;
+        if (sp < LDS) e = lds[sp * 64 + threadIdx.x];
+        else e = spill[(sp - LDS) * 64 + threadIdx.x];
+        return make_uint2(static_cast<uint32_t>(e), static_cast<uint32_t>(e >> 32));
     }
 };
 
@@ -217,6 +261,58 @@ struct PrivateSpillStack {
     }
 };
 
+// Next stack entry that can still hold a hit (closest hit: entry distance <= best t), or
+// kEmptyD.
+template <bool ANY, typename Stack>
+__device__ __forceinline__ uint32_t popNext(Stack& st, float bestT) {
+    while (st.sp > 0) {
+        const uint2 e = st.pop();
+        if (ANY || u2f(e.y) <= bestT) return e.x;
+    }
+    return kEmptyD;
+}
+
+// The hit children of a node (entry distances tc, refs rc; misses are +inf / kEmptyD): for
+// closest hit they are ordered nearest first and all but the nearest are pushed; returns the
+// child to visit next, or kEmptyD when no child is hit.
+template <int W, bool ANY, typename Stack>
+__device__ __forceinline__ uint32_t descend(float* tc, uint32_t* rc, Stack& st) {
+    if (!ANY) {
+        // Sorting network, ascending entry distance (misses sort last as +inf).
+        if (W == 4) {
+            cswap(tc[0], rc[0], tc[1], rc[1]); cswap(tc[2], rc[2], tc[3], rc[3]);
+            cswap(tc[0], rc[0], tc[2], rc[2]); cswap(tc[1], rc[1], tc[3], rc[3]);
+            cswap(tc[1], rc[1], tc[2], rc[2]);
+        } else {
+            // Optimal 8-input network (19 comparators, depth 6).
+            constexpr int net[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6},
+                                        {3, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}, {2, 4}, {3, 5},
+                                        {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+            for (int c = 0; c < 19; ++c)
+                cswap(tc[net[c][0]], rc[net[c][0]], tc[net[c][1]], rc[net[c][1]]);
+        }
+    }
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < W; ++k) cnt += rc[k] != kEmptyD ? 1 : 0;
+    if (cnt == 0) return kEmptyD;
+    if (ANY) {
+        uint32_t first = kEmptyD;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            if (rc[k] == kEmptyD) continue;
+            if (first == kEmptyD) first = rc[k];
+            else st.push(rc[k], tc[k]);
+        }
+        return first;
+    }
+#pragma unroll
+    for (int k = W - 1; k >= 1; --k)
+        if (k < cnt) st.push(rc[k], tc[k]);
+    return rc[0];
+}
+
 // One round of the traversal loop of one ray ("while-while" with postponed leaves, after Aila
 // & Laine 2009): a lane walks interior nodes until it meets a leaf, parks it, and keeps
 // walking until every lane of the wave that still traverses holds a parked leaf; then the
@@ -228,58 +324,17 @@ struct PrivateSpillStack {
 template <int W, bool Q, bool ANY, bool MIXED, typename Stack>
 __device__ __forceinline__ void traceRound(const TraceArgs& a, const DRay& r, Stack& st, uint32_t& ref, uint32_t& leaf,
                                            float& bestT, uint32_t& bestId, bool& occluded) {
-    // next stack entry that can still hold a hit (closest hit: entry distance <= best t)
-    auto popNext = [&]() -> uint32_t {
-        while (st.sp > 0) {
-            const uint2 e = st.pop();
-            if (ANY || u2f(e.y) <= bestT) return e.x;
-        }
-        return kEmptyD;
-    };
     while (ref != kEmptyD && !isLeafD(ref)) {
         float tc[W];
         uint32_t rc[W];
-        if (Q) childrenQ<W>(a.nodes, ref, r, bestT, tc, rc);
-        else childrenF<W>(a.nodes, ref, r, bestT, tc, rc);
-        if (!ANY) {
-            // Sorting network, ascending entry distance (misses sort last as +inf).
-            if (W == 4) {
-                cswap(tc[0], rc[0], tc[1], rc[1]); cswap(tc[2], rc[2], tc[3], rc[3]);
-                cswap(tc[0], rc[0], tc[2], rc[2]); cswap(tc[1], rc[1], tc[3], rc[3]);
-                cswap(tc[1], rc[1], tc[2], rc[2]);
-            } else {
-                // Optimal 8-input network (19 comparators, depth 6).
-                constexpr int net[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6},
-                                            {3, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}, {2, 4}, {3, 5},
-                                            {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
-#pragma unroll
-                for (int c = 0; c < 19; ++c)
-                    cswap(tc[net[c][0]], rc[net[c][0]], tc[net[c][1]], rc[net[c][1]]);
-            }
-        }
-        int cnt = 0;
-#pragma unroll
-        for (int k = 0; k < W; ++k) cnt += rc[k] != kEmptyD ? 1 : 0;
-        if (cnt == 0) {
-            ref = popNext();
-        } else if (ANY) {
-            uint32_t first = kEmptyD;
-#pragma unroll
-            for (int k = 0; k < W; ++k) {
-                if (rc[k] == kEmptyD) continue;
-                if (first == kEmptyD) first = rc[k];
-                else st.push(rc[k], tc[k]);
-            }
-            ref = first;
-        } else {
-#pragma unroll
-            for (int k = W - 1; k >= 1; --k)
-                if (k < cnt) st.push(rc[k], tc[k]);
-            ref = rc[0];
-        }
+        NodeRows<W, Q> nr;
+        loadNode<W, Q>(a.nodes, ref, nr);
+        children<W, Q>(nr, r, bestT, tc, rc);
+        ref = descend<W, ANY>(tc, rc, st);
+        if (ref == kEmptyD) ref = popNext<ANY>(st, bestT);
         if (ref != kEmptyD && isLeafD(ref) && leaf == kEmptyD) {
             leaf = ref;
-            ref = popNext();
+            ref = popNext<ANY>(st, bestT);
         }
         if (__all(leaf != kEmptyD ? 1 : 0)) break;
     }
@@ -298,7 +353,7 @@ __device__ __forceinline__ void traceRound(const TraceArgs& a, const DRay& r, St
         leaf = kEmptyD;
         if (ref != kEmptyD && isLeafD(ref)) {
             leaf = ref;
-            ref = popNext();
+            ref = popNext<ANY>(st, bestT);
         }
     }
 }
@@ -341,24 +396,53 @@ __device__ __forceinline__ void storeResult(const TraceArgs& a, uint32_t i, floa
     else a.hits[i] = make_uint2(bestId, bestId == RBR_MISS_ID ? 0u : f2u(bestT));
 }
 
-// Persistent waves: each 64-lane workgroup takes batches of 64 consecutive rays from a
-// device counter until the batch is exhausted, so slow rays do not idle a whole launch
-// slot and every resident wave stays busy to the end.  The counter is zeroed by
-// resetCounterKernel right before the launch on the same stream; the ray count can come
-// from device memory (queue lengths written by an earlier kernel).
+// Work distribution of the persistent kernels: the 64-ray batches are split into kSlices
+// equal slices, each with its own counter (kSliceStride words apart), and a wave drains the
+// slice of its workgroup index, then moves on to the following ones until every slice is
+// drained.  Atomics on one address are serialized (about 11 ns apart on MI355X, so one
+// counter caps a launch at ~90 M batch grabs/s, and that was the limit of the traversal of
+// 2^22 rays); over 16 counters the grabs no longer queue.  The counters are zeroed by
+// resetCounterKernel right before the launch on the same stream.
+struct SliceWalk {
+    uint32_t s;      // current slice
+    uint32_t left;   // slices not yet found drained
+    uint32_t per;    // rays per slice (a multiple of 64)
+};
+
+__device__ __forceinline__ SliceWalk sliceWalk(uint32_t n) {
+    const uint32_t batches = (n + 63u) / 64u;
+    return SliceWalk{blockIdx.x % kSlices, kSlices, ((batches + kSlices - 1u) / kSlices) * 64u};
+}
+
+// First ray of the next 64-ray batch, or n once every slice is drained.  Called by the whole
+// wave with every lane active (the result is uniform); lane 0 does the atomic.
+__device__ __forceinline__ uint32_t nextBatch(uint32_t* counters, uint32_t n, SliceWalk& w) {
+    while (w.left > 0u) {
+        uint32_t b = 0u;
+        if (__lane_id() == 0) b = atomicAdd(counters + size_t(w.s) * kSliceStride, 64u);
+        b = __builtin_amdgcn_readfirstlane(b);
+        const uint32_t start = w.s * w.per + b;
+        if (b < w.per && start < n) return start;
+        --w.left;
+        w.s = w.s + 1u == kSlices ? 0u : w.s + 1u;
+    }
+    return n;
+}
+
+// Persistent waves: each 64-lane workgroup takes batches of 64 consecutive rays (nextBatch)
+// until every batch is taken, so slow rays do not idle a whole launch slot and every
+// resident wave stays busy to the end.  The ray count can come from device memory (queue
+// lengths written by an earlier kernel).
 template <int W, bool Q, int LDS, bool ANY, bool MIXED>
 __global__ __launch_bounds__(64) void traceKernelPersistent(TraceArgs a, PersistArgs p) {
     __shared__ uint2 ldsStack[LDS * 64];
-    __shared__ uint32_t batch;
     GlobalSpillStack<LDS> st;
-    st.lds = ldsStack;
-    st.spill = p.spill + size_t(blockIdx.x) * size_t(p.spillDepth) * 64;
+    st.lds = (LdsU64*)ldsStack;
+    st.spill = (GlobalU64*)(p.spill + size_t(blockIdx.x) * size_t(p.spillDepth) * 64);
     const uint32_t n = a.nDev ? *a.nDev : a.n;
+    SliceWalk walk = sliceWalk(n);
     while (true) {
-        if (threadIdx.x == 0) batch = atomicAdd(p.counter, 64u);
-        __syncthreads();
-        const uint32_t base = batch;
-        __syncthreads();
+        const uint32_t base = nextBatch(p.counter, n, walk);
         if (base >= n) break;
         const uint32_t i = base + threadIdx.x;
         if (i < n) {
@@ -377,18 +461,19 @@ __global__ __launch_bounds__(64) void traceKernelPersistent(TraceArgs a, Persist
 // least p.refill lanes of the wave are idle the wave hands its idle lanes new rays, so a
 // wave no longer waits for its slowest ray before taking work (incoherent rays vary
 // several-fold in node count, and half of them may miss everything).  Rays come from a
-// 64-ray range the wave reserves with one atomic on the device counter (the rate of the
-// batch kernel's atomics); when the range runs dry mid-refill the rest comes from the next
-// range.  Between refills each live lane runs one traversal round.  Every wave exits once the
-// counter has passed the ray count and all of its lanes are idle.
+// 64-ray range the wave reserves with nextBatch (the batch kernel's slices, one atomic per
+// range); when the range runs dry mid-refill the rest comes from the next range.  Between
+// refills each live lane runs one traversal round.  Every wave exits once every slice is
+// drained and all of its lanes are idle.
 template <int W, bool Q, int LDS, bool ANY, bool MIXED>
 __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArgs p) {
     __shared__ uint2 ldsStack[LDS * 64];
     GlobalSpillStack<LDS> st;
-    st.lds = ldsStack;
-    st.spill = p.spill + size_t(blockIdx.x) * size_t(p.spillDepth) * 64;
+    st.lds = (LdsU64*)ldsStack;
+    st.spill = (GlobalU64*)(p.spill + size_t(blockIdx.x) * size_t(p.spillDepth) * 64);
     const uint32_t n = a.nDev ? *a.nDev : a.n;
     const uint32_t lane = threadIdx.x;
+    SliceWalk walk = sliceWalk(n);
     bool alive = false, more = true, occ = false;
     uint32_t i = 0, ref = kEmptyD, leaf = kEmptyD, bestId = RBR_MISS_ID;
     uint32_t qBase = 0u, qEnd = 0u;   // reserved, not yet assigned rays (uniform)
@@ -400,10 +485,7 @@ __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArg
         if (more && nIdle >= p.refill) {
             const uint32_t avail = qEnd - qBase;
             uint32_t base2 = 0u;
-            if (avail < nIdle) {
-                if (lane == 0) base2 = atomicAdd(p.counter, 64u);
-                base2 = __shfl(base2, 0);
-            }
+            if (avail < nIdle) base2 = nextBatch(p.counter, n, walk);
             if (!alive) {
                 const uint32_t rank = uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
                 const uint32_t idx = rank < avail ? qBase + rank : base2 + (rank - avail);
@@ -422,7 +504,7 @@ __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArg
             if (avail < nIdle) {
                 qBase = base2 + (nIdle - avail);
                 qEnd = base2 + 64u;
-                if (base2 >= n) more = false;
+                if (base2 >= n) more = false;   // every slice drained
             } else {
                 qBase += nIdle;
             }
@@ -442,7 +524,110 @@ __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArg
     }
 }
 
-__global__ void resetCounterKernel(uint32_t* c) { *c = 0u; }
+// Persistent waves in single steps with per-lane ray fetch.  Each iteration of a wave, every
+// lane with an interior node to visit tests that node's children, and every lane with a leaf
+// in progress tests one primitive of it, so no lane waits for the others to reach a leaf (the
+// while-while rounds above make a lane wait until every lane holds one) and a lane whose ray
+// is done is handed a new ray as soon as p.refill lanes are idle.  The node fetch is issued at
+// the top of the iteration, before the primitive step, so the two loads of an iteration are
+// in flight together.  A lane holds at most one leaf in progress; a leaf met while one is in
+// progress stays in ref until the current one is finished.  Every entry that can still hold a
+// closer hit is visited and every primitive of every visited leaf is tested, so the hits are
+// those of the other kernels.  Rays come from 64-ray ranges reserved with nextBatch, as in
+// traceKernelDynamic, and every wave exits once every slice is drained and all of its lanes
+// are idle.
+template <int W, bool Q, int LDS, bool ANY, bool MIXED>
+__global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p) {
+    __shared__ uint2 ldsStack[LDS * 64];
+    GlobalSpillStack<LDS> st;
+    st.lds = (LdsU64*)ldsStack;
+    st.spill = (GlobalU64*)(p.spill + size_t(blockIdx.x) * size_t(p.spillDepth) * 64);
+    const uint32_t n = a.nDev ? *a.nDev : a.n;
+    const uint32_t lane = threadIdx.x;
+    SliceWalk walk = sliceWalk(n);
+    bool alive = false, more = true, occ = false;
+    uint32_t i = 0, ref = kEmptyD, prim = 0u, primLeft = 0u, bestId = RBR_MISS_ID;
+    uint32_t qBase = 0u, qEnd = 0u;   // reserved, not yet assigned rays (uniform)
+    float bestT = 0.0f;
+    DRay r{};
+    while (true) {
+        const uint64_t idle = __ballot(alive ? 0 : 1);
+        const uint32_t nIdle = uint32_t(__popcll(idle));
+        if (more && nIdle >= p.refill) {
+            const uint32_t avail = qEnd - qBase;
+            uint32_t base2 = 0u;
+            if (avail < nIdle) base2 = nextBatch(p.counter, n, walk);
+            if (!alive) {
+                const uint32_t rank = uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+                const uint32_t idx = rank < avail ? qBase + rank : base2 + (rank - avail);
+                if (idx < n) {
+                    i = idx;
+                    r = loadRay(a.rays, idx);
+                    bestT = r.tmax;
+                    bestId = RBR_MISS_ID;
+                    occ = false;
+                    st.sp = 0;
+                    ref = a.nodes ? (a.rootOf ? a.rootOf[idx] : 0u) : kEmptyD;
+                    primLeft = 0u;
+                    alive = true;
+                }
+            }
+            if (avail < nIdle) {
+                qBase = base2 + (nIdle - avail);
+                qEnd = base2 + 64u;
+                if (base2 >= n) more = false;   // every slice drained
+            } else {
+                qBase += nIdle;
+            }
+        }
+        if (!__any(alive ? 1 : 0)) {
+            if (!more) break;
+            continue;
+        }
+        // One primitive of the leaf in progress, then one interior node.
+        if (primLeft != 0u) {
+            float t;
+            uint32_t id;
+            if (testPrim<MIXED>(a, a.prims + size_t(prim) * 3, r, bestT, t, id)) {
+                if (ANY) occ = true;
+                else if (t < bestT || id < bestId) { bestT = t; bestId = id; }
+            }
+            ++prim;
+            --primLeft;
+            if (ANY && occ) {
+                primLeft = 0u;
+                ref = kEmptyD;
+            } else if (primLeft == 0u && ref != kEmptyD && isLeafD(ref)) {
+                prim = leafStartD(ref);
+                primLeft = leafCountD(ref);
+                ref = popNext<ANY>(st, bestT);
+            }
+        }
+        if (alive && ref != kEmptyD && !isLeafD(ref)) {
+            float tc[W];
+            uint32_t rc[W];
+            NodeRows<W, Q> nr;
+            loadNode<W, Q>(a.nodes, ref, nr);
+            children<W, Q>(nr, r, bestT, tc, rc);
+            ref = descend<W, ANY>(tc, rc, st);
+            if (ref == kEmptyD) ref = popNext<ANY>(st, bestT);
+            if (ref != kEmptyD && isLeafD(ref) && primLeft == 0u) {
+                prim = leafStartD(ref);
+                primLeft = leafCountD(ref);
+                ref = popNext<ANY>(st, bestT);
+            }
+        }
+        if (alive && ref == kEmptyD && primLeft == 0u) {
+            tracePlanes<ANY>(a, r, bestT, bestId, occ);
+            storeResult<ANY>(a, i, bestT, bestId, occ);
+            alive = false;
+        }
+    }
+}
+
+__global__ void resetCounterKernel(uint32_t* c) {
+    if (threadIdx.x < kSlices) c[size_t(threadIdx.x) * kSliceStride] = 0u;
+}
 
 // One ray per lane, stack overflow in private memory (kept for comparison and for callers
 // that cannot provide the spill buffer).
@@ -463,12 +648,16 @@ __global__ __launch_bounds__(64) void traceKernelLane(TraceArgs a) {
 
 template <int W, bool Q, int LDS, bool ANY>
 hipError_t launchT(const TraceArgs& a, const PersistArgs* p, unsigned grid, hipStream_t s) {
-    if (p && p->refill) {
-        hipLaunchKernelGGL(resetCounterKernel, dim3(1), dim3(1), 0, s, p->counter);
+    if (p && p->refill && p->step) {
+        hipLaunchKernelGGL(resetCounterKernel, dim3(1), dim3(64), 0, s, p->counter);
+        if (a.mixed) hipLaunchKernelGGL((traceKernelStep<W, Q, LDS, ANY, true>), dim3(grid), dim3(64), 0, s, a, *p);
+        else hipLaunchKernelGGL((traceKernelStep<W, Q, LDS, ANY, false>), dim3(grid), dim3(64), 0, s, a, *p);
+    } else if (p && p->refill) {
+        hipLaunchKernelGGL(resetCounterKernel, dim3(1), dim3(64), 0, s, p->counter);
         if (a.mixed) hipLaunchKernelGGL((traceKernelDynamic<W, Q, LDS, ANY, true>), dim3(grid), dim3(64), 0, s, a, *p);
         else hipLaunchKernelGGL((traceKernelDynamic<W, Q, LDS, ANY, false>), dim3(grid), dim3(64), 0, s, a, *p);
     } else if (p) {
-        hipLaunchKernelGGL(resetCounterKernel, dim3(1), dim3(1), 0, s, p->counter);
+        hipLaunchKernelGGL(resetCounterKernel, dim3(1), dim3(64), 0, s, p->counter);
         if (a.mixed) hipLaunchKernelGGL((traceKernelPersistent<W, Q, LDS, ANY, true>), dim3(grid), dim3(64), 0, s, a, *p);
         else hipLaunchKernelGGL((traceKernelPersistent<W, Q, LDS, ANY, false>), dim3(grid), dim3(64), 0, s, a, *p);
     } else {

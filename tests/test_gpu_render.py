@@ -154,10 +154,10 @@ def test_render_tuning_does_not_change_images(ctx, pool, graphs, sort):
         R.set_render_tuning(ctx, 1000, True, False)
 
 
-@pytest.mark.parametrize("mode", [2, 3])
+@pytest.mark.parametrize("mode", [2, 3, 5, 7])
 def test_trace_fetch_modes_do_not_change_images(ctx, mode):
-    """The renderer's traversals run on the device-sized queues; with per-lane ray fetch they
-    return the same hits, so every pixel is bit-identical."""
+    """The renderer's traversals run on the device-sized queues; with per-lane ray fetch (in
+    rounds or in single steps) they return the same hits, so every pixel is bit-identical."""
     say(f"case: render with trace mode {mode}")
     a = R.scene_from_desc(ctx, S.cornell_c0(40, 30))
     f = R.scene_from_desc(ctx, S.fog_jobs_c3(scale=0.02)[9]["scene"])

@@ -156,11 +156,12 @@ _ORACLE: dict = {}
 
 
 @pytest.mark.parametrize("mode,depth,waves", [(0, 8, 1), (0, 12, 32), (0, 32, 16), (1, 8, 16), (1, 24, 16),
-                                               (2, 8, 32), (3, 8, 1), (3, 12, 16), (4, 32, 16)])
+                                               (2, 8, 32), (3, 8, 1), (3, 12, 16), (4, 32, 16),
+                                               (5, 8, 32), (6, 8, 1), (6, 12, 16), (7, 32, 16), (8, 8, 24)])
 def test_launch_tunings_identical(ctx, mode, depth, waves):
     """Every launch configuration (persistent waves with tiny/large LDS stacks and so
-    deep global spills, one ray per lane, or persistent waves whose idle lanes fetch new
-    rays) returns the same bit-exact hits."""
+    deep global spills, one ray per lane, persistent waves whose idle lanes fetch new rays,
+    in rounds or in single node / primitive steps) returns the same bit-exact hits."""
     say(f"case: tuning mode {mode} lds {depth} waves {waves}")
     tris = R.icosphere(48, 100.0, 0.1, 0xC1)
     lo, hi = R.mesh_bounds(tris)
@@ -174,5 +175,26 @@ def test_launch_tunings_identical(ctx, mode, depth, waves):
             sc = gpu_scene(ctx, width, 4, tris=tris)
             assert_same(sc.trace_closest(rays), ref)
             assert np.array_equal(sc.trace_any(rays), o.any(rays))
+    finally:
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
+
+
+@pytest.mark.parametrize("mode", [5, 8])
+def test_step_modes_mixed_primitives(ctx, mode):
+    """The single-step kernel on a world of spheres, triangles and an unbounded plane, for
+    every layout: the same hits as brute force (closest with lowest-id ties, and any hit)."""
+    say(f"case: step mode {mode} mixed world")
+    rng = np.random.default_rng(500 + mode)
+    w = worlds.make_world(rng, 60, 600, True)
+    rays = worlds.make_rays(rng, 20000)
+    kw = worlds.geometry_kwargs(w)
+    o = O.OracleScene(**kw)
+    ref_c, ref_a = o.closest(rays, brute=True), o.any(rays, brute=True)
+    ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, 8, 32))
+    try:
+        for width in WIDTHS:
+            sc = gpu_scene(ctx, width, 4, **kw)
+            assert_same(sc.trace_closest(rays), ref_c)
+            assert np.array_equal(sc.trace_any(rays), ref_a)
     finally:
         ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
