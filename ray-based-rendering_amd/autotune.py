@@ -86,13 +86,13 @@ def apply(ctx: R.Context, cfg: dict | None) -> None:
 
 # ---------------------------------------------------------------- render tuning
 
-def render_sweep(ctx: R.Context, rj: list, paths: int, pools, sorts, graphs, waves, reps: int = 2,
+def render_sweep(ctx: R.Context, rj: list, paths: int, pools, sorts, graphs, waves, modes=(0,), reps: int = 2,
                  log=print) -> list[dict]:
     results = []
     ref = None
-    for pool, sort, graph, wpc in itertools.product(pools, sorts, graphs, waves):
+    for pool, sort, graph, wpc, mode in itertools.product(pools, sorts, graphs, waves, modes):
         R.set_render_tuning(ctx, pool, graph, sort)
-        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, wpc))
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, 8, wpc))
         films = R.render(ctx, rj)   # warm-up; records the round graphs when on
         times = []
         for _ in range(reps):
@@ -104,9 +104,9 @@ def render_sweep(ctx: R.Context, rj: list, paths: int, pools, sorts, graphs, wav
         else:
             for a, b in zip(films, ref):
                 if not np.allclose(a, b, rtol=2e-6, atol=1e-7):
-                    raise RuntimeError(f"pool={pool} sort={sort} graphs={graph} waves={wpc} changed the image")
+                    raise RuntimeError(f"pool={pool} sort={sort} graphs={graph} waves={wpc} mode={mode} changed the image")
         rec = {"pool_paths": pool, "sort_materials": int(sort), "use_graphs": int(graph), "waves_per_cu": wpc,
-               "ms": ms, "mpaths_s": paths / ms / 1e3}
+               "trace_mode": mode, "ms": ms, "mpaths_s": paths / ms / 1e3}
         log(json.dumps(rec))
         results.append(rec)
     R.set_render_tuning(ctx)
@@ -125,7 +125,7 @@ def apply_render(ctx: R.Context, cfg: dict | None) -> None:
     """Applies a render_config() entry (None: library defaults)."""
     if cfg:
         R.set_render_tuning(ctx, cfg["pool_paths"], bool(cfg["use_graphs"]), bool(cfg["sort_materials"]))
-        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, cfg["waves_per_cu"]))
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, cfg.get("trace_mode", 0), 8, cfg["waves_per_cu"]))
     else:
         R.set_render_tuning(ctx)
         ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
@@ -145,10 +145,10 @@ def main_render(argv) -> int:
         with open(args.out) as f:
             out["workloads"] = json.load(f).get("workloads", {})
     if args.quick:
-        grid = dict(pools=[1 << 23, 1 << 24], sorts=[False, True], graphs=[True], waves=[32])
+        grid = dict(pools=[1 << 24, 1 << 25], sorts=[False, True], graphs=[True], waves=[16, 32], modes=[0, 6])
     else:
         grid = dict(pools=[1 << 22, 1 << 23, 1 << 24, 1 << 25], sorts=[False, True], graphs=[False, True],
-                    waves=[16, 32])
+                    waves=[16, 32], modes=[0, 2, 6, 8])
     for key in args.only.split(","):
         jobs = B.workload_jobs(key)
         scs = [R.scene_from_desc(ctx, j["desc"]) for j in jobs]
@@ -185,10 +185,10 @@ def main() -> int:
     ctx = R.Context(0)
     name, cus = ctx.device_info()
     if args.quick:
-        grid = dict(widths=[4, 8, 4 | R.BVH_QUANTIZED, 8 | R.BVH_QUANTIZED], leaves=[4], modes=[0, 2, 3, 4],
-                    depths=[8, 16], waves=[16, 24, 32])
+        grid = dict(widths=[4, 8, 4 | R.BVH_QUANTIZED, 8 | R.BVH_QUANTIZED], leaves=[4], modes=[0, 2, 5, 6, 7, 8],
+                    depths=[8, 12], waves=[16, 24, 32])
     else:
-        grid = dict(widths=[4, 8, 4 | R.BVH_QUANTIZED, 8 | R.BVH_QUANTIZED], leaves=[2, 4, 8], modes=[0, 1, 2, 3, 4],
+        grid = dict(widths=[4, 8, 4 | R.BVH_QUANTIZED, 8 | R.BVH_QUANTIZED], leaves=[2, 4, 8], modes=list(range(9)),
                     depths=[8, 12, 16, 24, 32], waves=[4, 8, 16, 24, 32])
     out = {"device": name, "cus": cus, "workload": f"icosphere n={args.n_sub} ({len(tris)} tris), {args.rays} rays",
            "time": time.strftime("%Y-%m-%dT%H:%M:%S")}

@@ -598,6 +598,68 @@ __device__ __forceinline__ uint32_t waveAppend(uint32_t* counter, bool want) {
     return base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
 }
 
+// Block-wide appends of the shading kernel.  Atomics on one address are serialized (about
+// 11 ns apart on MI355X), and with one atomic per wave and queue the shading of tens of
+// millions of paths queued behind its own counters; a block of kShadeBlock threads now
+// issues one atomic per call (blockAppend), or one per kShadowChunk slots (blockAppendChunk).
+// Every thread of the block must make the same sequence of calls.  `sh` is block-shared
+// scratch of kApWords words; a call's two barriers order the reuse of `sh` by the next one.
+constexpr uint32_t kApWaves = kShadeBlock / 64;
+constexpr uint32_t kApWords = kApWaves + 3;
+constexpr uint32_t kShadowChunk = 1024;   // >= kShadeBlock: one chunk always holds a call's slots
+
+__device__ __forceinline__ uint32_t blockAppend(uint32_t* counter, bool want, uint32_t* sh) {
+    const unsigned long long m = __ballot(want ? 1 : 0);
+    const uint32_t wave = threadIdx.x >> 6, lane = __lane_id();
+    if (lane == 0) sh[wave] = uint32_t(__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0u;
+        for (uint32_t w = 0; w < kApWaves; ++w) {
+            const uint32_t c = sh[w];
+            sh[w] = tot;
+            tot += c;
+        }
+        sh[kApWaves] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    return sh[kApWaves] + sh[wave] + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+}
+
+// Same, with the block's slots taken from a chunk of kShadowChunk slots it reserved earlier
+// (chunk[0..1] = next free slot, end; block-shared), reserving the next chunk when the current
+// one runs out (a call's slots may straddle the two).  The slots left in the block's last
+// chunk are never written by shading; the caller marks them.
+__device__ __forceinline__ uint32_t blockAppendChunk(uint32_t* counter, bool want, uint32_t* sh, uint32_t* chunk) {
+    const unsigned long long m = __ballot(want ? 1 : 0);
+    const uint32_t wave = threadIdx.x >> 6, lane = __lane_id();
+    if (lane == 0) sh[wave] = uint32_t(__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0u;
+        for (uint32_t w = 0; w < kApWaves; ++w) {
+            const uint32_t c = sh[w];
+            sh[w] = tot;
+            tot += c;
+        }
+        const uint32_t cur = chunk[0], avail = chunk[1] - cur;
+        uint32_t nb = 0u;
+        if (tot > avail) {
+            nb = atomicAdd(counter, kShadowChunk);
+            chunk[0] = nb + (tot - avail);
+            chunk[1] = nb + kShadowChunk;
+        } else {
+            chunk[0] = cur + tot;
+        }
+        sh[kApWaves] = cur;
+        sh[kApWaves + 1] = avail;
+        sh[kApWaves + 2] = nb;
+    }
+    __syncthreads();
+    const uint32_t r = sh[wave] + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+    return r < sh[kApWaves + 1] ? sh[kApWaves] + r : sh[kApWaves + 2] + (r - sh[kApWaves + 1]);
+}
+
 __device__ __forceinline__ int findJob(const DJob* jobs, int njobs, uint32_t p) {
     int j = 0;
     while (j + 1 < njobs && p >= jobs[j + 1].pathBase) ++j;
@@ -647,7 +709,8 @@ __global__ __launch_bounds__(256) void cameraKernel(const DJob* __restrict__ job
 // One bounce of shading for queue entry i (lanes with i >= count only take part in the wave
 // collectives).  mixLds: this block's LDS for the per-vertex mixture weights.
 __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int njobs, uint32_t depth, uint32_t maxLights,
-                                           int cur, const Queues& q, uint32_t i, uint32_t count, float* mixLds) {
+                                           int cur, const Queues& q, uint32_t i, uint32_t count, float* mixLds,
+                                           uint32_t* apSh, uint32_t* shadowChunk) {
     const bool active = i < count;
     const int nxt = cur ^ 1;
 
@@ -716,7 +779,7 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
         }
     }
 
-    // Next-event estimation: one shadow ray per light, appended wave by wave.
+    // Next-event estimation: one shadow ray per light, appended block by block.
     const DJob& jb = jobs[j];
     const bool volInt = active && jb.integrator == RBR_INTEGRATOR_VOLPATH;
     Frame frame(mode == 1 ? outward : mk(0, 0, 1));
@@ -760,7 +823,7 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
                 }
             }
         }
-        const uint32_t slot = waveAppend(&q.counts[2], want);
+        const uint32_t slot = blockAppendChunk(&q.counts[2], want, apSh, shadowChunk);
         if (want) {
             putRay(q.sray, slot, so, sd, kEpsD, smax);
             q.sroot[slot] = jb.root;
@@ -837,7 +900,7 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
     if (q.recCount) {
         // Training record of this vertex: the radiance that arrives later along wi is
         // L_final - L(after this vertex's NEE), the gather kernel stores the latter.
-        const uint32_t r = waveAppend(q.recCount, record);
+        const uint32_t r = blockAppend(q.recCount, record, apSh);
         const bool keep = record && r < q.recCap;
         if (active) q.recOf[p] = keep ? r : 0xFFFFFFFFu;
         if (keep) {
@@ -850,7 +913,7 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
             o[5] = make_float4(L.x, L.y, L.z, 0.0f);
         }
     }
-    const uint32_t slot = waveAppend(&q.counts[nxt], cont);
+    const uint32_t slot = blockAppend(&q.counts[nxt], cont, apSh);
     if (cont) {
         putRay(q.ray[nxt], slot, no, nd, kEpsD, INFINITY);
         q.path[nxt][slot] = p;
@@ -863,9 +926,20 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
 __global__ __launch_bounds__(256, 4) void shadeKernel(const DJob* __restrict__ jobs, int njobs, uint32_t depth,
                                                    uint32_t maxLights, int cur, Queues q) {
     __shared__ float mixLds[2 * 6 * kShadeBlock];
+    __shared__ uint32_t apSh[kApWords];
+    __shared__ uint32_t shadowChunk[2];
+    if (threadIdx.x == 0) shadowChunk[0] = shadowChunk[1] = 0u;
+    __syncthreads();
     const uint32_t count = q.counts[cur];
     for (uint32_t b = blockIdx.x * 256u; b < count; b += gridDim.x * 256u)
-        shadeEntry(jobs, njobs, depth, maxLights, cur, q, b + threadIdx.x, count, mixLds);
+        shadeEntry(jobs, njobs, depth, maxLights, cur, q, b + threadIdx.x, count, mixLds, apSh, shadowChunk);
+    // Shadow slots of the block's last chunk that no shading point took: rays that test
+    // nothing (no BVH root, empty [tmin, tmax]); no path's shadow index refers to them.
+    __syncthreads();
+    for (uint32_t t = shadowChunk[0] + threadIdx.x; t < shadowChunk[1]; t += 256u) {
+        putRay(q.sray, t, mk(0, 0, 0), mk(0, 0, 1), 1.0f, 0.0f);
+        q.sroot[t] = kEmptyD;
+    }
 }
 
 // Material-ordered shading: bin of each extension-queue entry (its hit's material; misses and
@@ -1593,11 +1667,14 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     for (const DJob& d : dj) maxPixels = std::max(maxPixels, d.pixels);
     const uint32_t P = uint32_t(std::min<uint64_t>(std::max<uint64_t>(ctx->renderPool, maxPixels), totalPaths));
     const uint32_t ML = std::max(1u, maxLights);
+    // shadow queue: one slot per path and light, plus the unused tail of each shading block's
+    // last slot chunk (at most one chunk per persistent block)
+    const size_t SQ = size_t(P) * ML + size_t(std::max(1, ctx->cuCount)) * 8u * kShadowChunk;
     struct { DevBuf* b; size_t bytes; } allocs[] = {
         {&R.ray[0], size_t(P) * 32}, {&R.ray[1], size_t(P) * 32}, {&R.path[0], size_t(P) * 4}, {&R.path[1], size_t(P) * 4},
         {&R.root[0], size_t(P) * 4}, {&R.root[1], size_t(P) * 4}, {&R.hit, size_t(P) * 8},
-        {&R.sray, size_t(P) * ML * 32}, {&R.sroot, size_t(P) * ML * 4}, {&R.scontrib, size_t(P) * ML * 16},
-        {&R.socc, size_t(P) * ML}, {&R.sindex, size_t(P) * ML * 4}, {&R.T, size_t(P) * 16}, {&R.L, size_t(P) * 16},
+        {&R.sray, SQ * 32}, {&R.sroot, SQ * 4}, {&R.scontrib, SQ * 16},
+        {&R.socc, SQ}, {&R.sindex, size_t(P) * ML * 4}, {&R.T, size_t(P) * 16}, {&R.L, size_t(P) * 16},
         {&R.counts, 16}, {&R.jobs, sizeof(DJob) * size_t(njobs)}};
     for (auto& a : allocs)
         if ((rc = ensureBuffer(ctx, *a.b, a.bytes))) return rc;
