@@ -453,9 +453,17 @@ __global__ __launch_bounds__(512) void guideFwdBwdKernel(GuideTrainArgs a) {
         trainBack<2>(dz, acc, mask, lwt, a, grp, idLo, idHi, lane);
         trainBack<1>(dz, acc, mask, lwt, a, grp, idLo, idHi, lane);
     }
-    // Loss: one atomic per wave.
+    // Loss: one atomic per workgroup (atomics on one address are serialized, and one per wave
+    // made the end of the launch queue behind them).
+    __shared__ float lossSh[kTrainWaves];
     for (int off = 32; off > 0; off >>= 1) lossAcc += __shfl_down(lossAcc, off);
-    if (lane == 0) atomicAdd(a.loss, double(lossAcc));
+    if (lane == 0) lossSh[wave] = lossAcc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kTrainWaves; ++w) t += double(lossSh[w]);
+        atomicAdd(a.loss, t);
+    }
 }
 
 // One (layer, 64-column) tile of dW_l[n][k] = sum_s dZ_l[n][s] A_l[k][s] over one chunk of

@@ -980,9 +980,11 @@ __global__ __launch_bounds__(1024) void sortScanKernel(uint32_t* bins) {
     bins[t] = t ? sh[t - 1] : 0u;
 }
 
-// Scatter in chunks of 1024 entries per block: ranks within the chunk from an LDS histogram,
-// one global atomic per (chunk, bin) reserves the chunk's range in the bin.
-constexpr uint32_t kSortChunk = 1024;
+// Scatter in chunks of 4096 entries per block: ranks within the chunk from an LDS histogram,
+// one global atomic per (chunk, bin) reserves the chunk's range in the bin.  Atomics on one
+// bin are serialized, so with few materials the chunk size sets the scatter's floor (1024-entry
+// chunks: ~32 K atomics per hot bin and bounce of a 32 Mi-path round).
+constexpr uint32_t kSortChunk = 4096;
 
 __global__ __launch_bounds__(256) void sortScatterKernel(int cur, Queues q) {
     __shared__ uint32_t cnt[kSortBins];
