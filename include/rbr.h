@@ -194,6 +194,18 @@ int rbr_scene_set_geometry(rbr_scene* scene,
                            const float* spheres /* 4*n: center radius */, uint32_t nsph, const uint32_t* sph_ids,
                            const float* quads /* 12*n: origin u v normal */, uint32_t nquad, const uint32_t* quad_ids,
                            const float* planes /* 4*n: normal d */, uint32_t nplane, const uint32_t* plane_ids);
+/* Vertex normals of the triangles (Triangle's constructor with n0, n1, n2, triangle.h:29-33):
+ * 9 floats per triangle of the last rbr_scene_set_geometry, in its order, each normalized here
+ * as the constructor does.  Such triangles shade with the normal interpolated at the hit and
+ * turned towards the geometric normal (Triangle::intersect, triangle.cpp:40-48); their hits are
+ * unchanged.  A triangle whose nine values are all zero keeps its geometric normal (a mesh
+ * without normals in a scene with others).  NULL or 0 removes them; rbr_scene_set_geometry
+ * removes them too. */
+int rbr_scene_set_tri_normals(rbr_scene* scene, const float* normals, size_t ntri);
+/* The shading normal Triangle::intersect gives a triangle (v0 v1 v2, 9 floats) with vertex
+ * normals (n0 n1 n2, 9 floats, normalized here) where the ray o + t d meets it (the ray must
+ * hit it); what the renderer uses for such triangles. */
+int rbr_triangle_shading_normal(const float* verts, const float* normals, const float* o, const float* d, float* out);
 
 /* Builds the wide BVH (width 4 or 8) over the bounded primitives, leaves of at most
  * leaf_max primitives (1..8), and uploads it.  width | RBR_BVH_QUANTIZED stores the device
@@ -334,12 +346,16 @@ int rbr_guide_clear_collected(rbr_ctx* ctx);
 /* ---- Scene ingestion (host only) ---------------------------------------------------------- */
 /* Triangles of Wavefront OBJ text as ObjImporter::parse builds them (obj_importer.cpp:59-150):
  * v / vt / vn / f records, 1-based and negative indices, polygons fanned around their first
- * corner; 9 floats per triangle (positions only: the device path shades with geometric
- * normals).  tris == NULL returns the count only; *normals / *uvs report the vn / vt records.
- * RBR_ERR_INVALID when a face names a missing vertex or no triangle results (the reference's
- * failures). */
+ * corner; 9 floats per triangle (positions).  tris == NULL returns the count only; *normals /
+ * *uvs report the vn / vt records.  RBR_ERR_INVALID when a face names a missing vertex or no
+ * triangle results (the reference's failures). */
 int rbr_obj_parse(const char* text, size_t length, float* tris, size_t capacity, size_t* count, size_t* normals,
                   size_t* uvs);
+/* The vertex normals of the same triangles, 9 floats per triangle (its corners' "vn" vectors as
+ * written), when the reference's importer keeps them: some corner names a normal and every
+ * corner's normal is nonzero (obj_importer.cpp:135-143); *count = 0 when it does not.  For
+ * rbr_scene_set_tri_normals.  normals == NULL returns the count only. */
+int rbr_obj_parse_normals(const char* text, size_t length, float* normals, size_t capacity, size_t* count);
 
 /* ---- Film output (host only; films are float RGB, row-major from the top row) ----------- */
 /* 8-bit conversion of a mean-radiance film, as Film::toRGB8 / Film::toRGB8ToneMapped

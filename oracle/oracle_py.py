@@ -69,6 +69,8 @@ def _load_ref():
         _sig(lib.ref_max_threads, [], C.c_int)
         _sig(lib.ref_film_to_rgb8, [_vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, _vp])
         _sig(lib.ref_obj_parse, [C.c_char_p, _sz, _vp, _sz], C.c_long)
+        _sig(lib.ref_obj_parse_normals, [C.c_char_p, _sz, _vp, _sz], C.c_long)
+        _sig(lib.ref_triangle_normal, [_vp, _vp, _vp, _vp, _vp, _vp, _vp], C.c_int)
         _ref = lib
     return _ref
 
@@ -92,6 +94,35 @@ def ref_obj_parse(text: bytes):
     out = np.zeros((n, 9), np.float32)
     lib.ref_obj_parse(text, len(text), out.ctypes.data, n)
     return out
+
+
+def ref_obj_parse_normals(text: bytes):
+    """The vertex normals the reference's ObjImporter keeps, (n, 9) per triangle corner, None
+    when it keeps none; raises ValueError when the import fails."""
+    import numpy as np
+    lib = _load_ref()
+    n = lib.ref_obj_parse_normals(text, len(text), None, 0)
+    if n < 0:
+        raise ValueError("reference import failed")
+    if n == 0:
+        return None
+    out = np.zeros((n, 9), np.float32)
+    lib.ref_obj_parse_normals(text, len(text), out.ctypes.data, n)
+    return out
+
+
+def ref_triangle_normal(verts, normals, o, d):
+    """The reference's Triangle::intersect with vertex normals: (normal faced towards the ray,
+    frontFace, the normalized direction it used), or None on a miss."""
+    import numpy as np
+    v, n = np.ascontiguousarray(verts, np.float32).reshape(9), np.ascontiguousarray(normals, np.float32).reshape(9)
+    oo, dd = np.ascontiguousarray(o, np.float32).reshape(3), np.ascontiguousarray(d, np.float32).reshape(3)
+    dn, out = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    front = C.c_int(0)
+    if not _load_ref().ref_triangle_normal(v.ctypes.data, n.ctypes.data, oo.ctypes.data, dd.ctypes.data, dn.ctypes.data,
+                                           out.ctypes.data, C.byref(front)):
+        return None
+    return out, bool(front.value), dn
 
 
 def _arr(a, cols):
@@ -218,7 +249,7 @@ def _render_structs():
 
 
 _sig(_orc.orc_rscene_create, [_vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp,
-                              _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp], _vp)
+                              _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp], _vp)
 _sig(_orc.orc_rscene_destroy, [_vp])
 _sig(_orc.orc_render, [_vp, _vp, _vp, _vp, C.c_int])
 _sig(_orc.orc_render_moments, [_vp, _vp, _vp, _vp, _sz, _vp, _vp, C.c_int])
@@ -241,12 +272,15 @@ class OracleRender:
         pm = np.ascontiguousarray(desc.prim_material, np.int32)
         mats, lights, med = R.c_materials(desc.materials), R.c_lights(desc.lights), R.c_medium(desc.medium)
         bg = np.asarray(desc.background, np.float32)
-        self._keep = (t, s, q, p, pm, mats, lights, med, bg)
+        tn = getattr(desc, "tri_normals", None)
+        tn = None if tn is None else np.ascontiguousarray(tn, np.float32).reshape(-1, 9)
+        self._keep = (t, s, q, p, pm, mats, lights, med, bg, tn)
         self.desc = desc
         self._R = R
         self._h = _orc.orc_rscene_create(_p(t), nt, None, _p(s), ns, None, _p(q), nq, None, _p(p), np_, None,
                                          C.addressof(mats), len(desc.materials), pm.ctypes.data, len(pm),
-                                         C.addressof(lights), len(desc.lights), C.addressof(med), bg.ctypes.data)
+                                         C.addressof(lights), len(desc.lights), C.addressof(med), bg.ctypes.data,
+                                         _p(tn))
 
     def close(self):
         if self._h:
@@ -286,7 +320,8 @@ class RefRender:
     def __init__(self, desc):
         lib = _load_ref()
         if not hasattr(lib, "_render_sig"):
-            _sig(lib.ref_scene_create, [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _u32, _vp, _vp], _vp)
+            _sig(lib.ref_scene_create, [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _vp],
+                 _vp)
             _sig(lib.ref_scene_destroy, [_vp])
             _sig(lib.ref_render, [_vp, _vp, _vp, _vp, C.c_int], C.c_double)
             lib._render_sig = True
@@ -295,11 +330,13 @@ class RefRender:
         pm = np.ascontiguousarray(desc.prim_material, np.int32)
         mats, lights, med = R.c_materials(desc.materials), R.c_lights(desc.lights), R.c_medium(desc.medium)
         bg = np.asarray(desc.background, np.float32)
-        self._keep = (t, s, q, p, pm, mats, lights, med, bg)
+        tn = getattr(desc, "tri_normals", None)
+        tn = None if tn is None else np.ascontiguousarray(tn, np.float32).reshape(-1, 9)
+        self._keep = (t, s, q, p, pm, mats, lights, med, bg, tn)
         self._lib, self._R, self.desc = lib, R, desc
         self._h = lib.ref_scene_create(_p(t), nt, _p(s), ns, _p(q), nq, _p(p), np_, C.addressof(mats),
                                        len(desc.materials), pm.ctypes.data, C.addressof(lights), len(desc.lights),
-                                       C.addressof(med), bg.ctypes.data)
+                                       C.addressof(med), bg.ctypes.data, _p(tn))
 
     def close(self):
         if self._h:

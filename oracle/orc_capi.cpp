@@ -151,9 +151,22 @@ void* orc_rscene_create(const float* tris, uint32_t ntri, const uint32_t* tri_id
                         const uint32_t* sph_ids, const float* quads, uint32_t nquad, const uint32_t* quad_ids,
                         const float* planes, uint32_t nplane, const uint32_t* plane_ids, const rbr_material* mats,
                         uint32_t nmat, const int32_t* primMat, uint32_t nPrimIds, const rbr_light* lights,
-                        uint32_t nlights, const rbr_medium* medium, const float* background) {
+                        uint32_t nlights, const rbr_medium* medium, const float* background, const float* tri_normals) {
     orc_scene* g = orc_scene_create(tris, ntri, tri_ids, spheres, nsph, sph_ids, quads, nquad, quad_ids, planes, nplane,
                                     plane_ids);
+    // vertex normals (9 per triangle; an all-zero row keeps the triangle flat), normalized as
+    // Triangle's constructor does (triangle.h:29-33)
+    for (uint32_t i = 0; tri_normals && i < ntri; ++i) {
+        const float* n = tri_normals + 9 * size_t(i);
+        bool any = false;
+        for (int k = 0; k < 9; ++k) any = any || n[k] != 0.0f;
+        if (!any) continue;
+        orc::Tri& tr = g->s.tris[i];
+        tr.smooth = true;
+        tr.n0 = f3(n).normalized();
+        tr.n1 = f3(n + 3).normalized();
+        tr.n2 = f3(n + 6).normalized();
+    }
     auto* r = new RS();
     r->s.geo = std::move(g->s);
     delete g;

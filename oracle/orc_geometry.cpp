@@ -365,12 +365,28 @@ bool Scene::fill(uint32_t id, const Ray& r, float t, Isect& is) const {
     is.wo = -r.d;
     is.id = id;
     switch (p.type) {
-        case PT_TRI: {  // triangle.cpp:30-47 (no vertex normals)
+        case PT_TRI: {  // triangle.cpp:30-51
             const Tri& tr = tris[p.index];
             is.p = r.at(t);
             V3 e1 = tr.v1 - tr.v0, e2 = tr.v2 - tr.v0;
             V3 gn = cross(e1, e2).normalized();
-            is.setFaceNormal(r, gn);
+            if (!tr.smooth) {
+                is.setFaceNormal(r, gn);
+                return true;
+            }
+            // barycentrics of the Moller-Trumbore terms (triangle.cpp:7-28), then the
+            // interpolated normal turned into the geometric hemisphere (triangle.cpp:40-48)
+            V3 h = cross(r.d, e2);
+            float a = dot(e1, h);
+            float f = 1.0f / a;
+            V3 sv = r.o - tr.v0;
+            float u = f * dot(sv, h);
+            V3 q = cross(sv, e1);
+            float v = f * dot(r.d, q);
+            float w = 1.0f - u - v;
+            V3 n = (tr.n0 * w + tr.n1 * u + tr.n2 * v).normalized();
+            if (dot(n, gn) < 0.0f) n = -n;
+            is.setFaceNormal(r, n);
             return true;
         }
         case PT_SPHERE: {  // sphere.cpp:24-35

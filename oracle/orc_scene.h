@@ -1,6 +1,6 @@
 // CPU restatement of the reference's primitives and BVH.  TEST INFRASTRUCTURE ONLY.
 //
-//   Triangle::intersect (Moller-Trumbore)   src/geometry/triangle.cpp:6-65
+//   Triangle::intersect (Moller-Trumbore)   src/geometry/triangle.cpp:6-65 (vertex normals 40-48)
 //   Sphere::intersect                        src/geometry/sphere.cpp:6-43
 //   Quad::intersect                          src/geometry/quad.cpp:12-42
 //   Plane::intersect                         src/geometry/plane.cpp:12-35
@@ -24,7 +24,12 @@ namespace orc {
 
 enum PrimType : uint32_t { PT_TRI = 0, PT_SPHERE = 1, PT_QUAD = 2, PT_PLANE = 3 };
 
-struct Tri { V3 v0, v1, v2; uint32_t id; };
+struct Tri {
+    V3 v0, v1, v2;
+    uint32_t id;
+    bool smooth = false;   // vertex normals n0..n2 (unit, as Triangle's constructor stores them)
+    V3 n0, n1, n2;
+};
 struct Sph { V3 c; float r; uint32_t id; };
 struct Quad { V3 origin, u, v, n; uint32_t id; };
 struct Plane { V3 n; float d; uint32_t id; };

@@ -180,4 +180,41 @@ long ref_obj_parse(const char* text, size_t length, float* tris, size_t capacity
     return long(n);
 }
 
+// The vertex normals ObjImporter::parse keeps (mesh.normals at each triangle corner, 9 floats per
+// triangle).  Returns the triangle count when the mesh keeps normals, 0 when it does not, -1
+// when the importer fails.
+long ref_obj_parse_normals(const char* text, size_t length, float* normals, size_t capacity) {
+    TriangleMesh mesh;
+    if (!ObjImporter::parse(std::string(text, length), mesh)) return -1;
+    if (mesh.normals.size() != mesh.positions.size()) return 0;
+    const size_t n = mesh.triangleCount();
+    for (size_t t = 0; t < n && t < capacity; ++t)
+        for (int c = 0; c < 3; ++c) {
+            const Vec3& v = mesh.normals[size_t(mesh.indices[3 * t + c])];
+            normals[9 * t + 3 * c] = v.x;
+            normals[9 * t + 3 * c + 1] = v.y;
+            normals[9 * t + 3 * c + 2] = v.z;
+        }
+    return long(n);
+}
+
+// Triangle::intersect for a triangle with vertex normals (v, n: 9 floats each) and the ray
+// Ray(o, d) (which normalizes d; dn receives the direction used).  On a hit writes the
+// interaction's normal (faced towards the ray by setFaceNormal) and frontFace; returns 1.
+int ref_triangle_normal(const float* v, const float* n, const float* o, const float* d, float* dn, float* normal,
+                        int* front) {
+    Triangle tri(v3(v), v3(v + 3), v3(v + 6), v3(n), v3(n + 3), v3(n + 6));
+    Ray ray(v3(o), v3(d));
+    dn[0] = ray.direction.x;
+    dn[1] = ray.direction.y;
+    dn[2] = ray.direction.z;
+    SurfaceInteraction isect;
+    if (!tri.intersect(ray, isect)) return 0;
+    normal[0] = isect.normal.x;
+    normal[1] = isect.normal.y;
+    normal[2] = isect.normal.z;
+    *front = isect.frontFace ? 1 : 0;
+    return 1;
+}
+
 }  // extern "C"

@@ -39,11 +39,12 @@ struct RefScene {
 extern "C" {
 
 // Primitive ids follow the order tris, spheres, quads, planes (the default numbering of the
-// C ABI); prim_material is indexed by that id.
+// C ABI); prim_material is indexed by that id.  triNormals (nullable): 9 floats per triangle,
+// vertex normals of the triangles whose row is not all zero.
 void* ref_scene_create(const float* tris, uint32_t ntri, const float* spheres, uint32_t nsph, const float* quads,
                        uint32_t nquad, const float* planes, uint32_t nplane, const rbr_material* mats, uint32_t nmat,
                        const int32_t* primMat, const rbr_light* lights, uint32_t nlights, const rbr_medium* medium,
-                       const float* background) {
+                       const float* background, const float* triNormals) {
     auto* r = new RefScene();
     for (uint32_t i = 0; i < nmat; ++i) {
         auto m = std::make_unique<Material>();
@@ -62,7 +63,14 @@ void* ref_scene_create(const float* tris, uint32_t ntri, const float* spheres, u
     };
     for (uint32_t i = 0; i < ntri; ++i, ++id) {
         const float* t = tris + 9 * size_t(i);
-        r->scene.addPrimitive(std::make_unique<Triangle>(v3(t), v3(t + 3), v3(t + 6), matOf(id)));
+        const float* n = triNormals ? triNormals + 9 * size_t(i) : nullptr;
+        bool smooth = false;
+        for (int k = 0; n && k < 9; ++k) smooth = smooth || n[k] != 0.0f;
+        if (smooth)   // TriangleMesh::toPrimitives with normals (mesh.cpp:29-32)
+            r->scene.addPrimitive(std::make_unique<Triangle>(v3(t), v3(t + 3), v3(t + 6), v3(n), v3(n + 3), v3(n + 6),
+                                                             matOf(id)));
+        else
+            r->scene.addPrimitive(std::make_unique<Triangle>(v3(t), v3(t + 3), v3(t + 6), matOf(id)));
     }
     for (uint32_t i = 0; i < nsph; ++i, ++id) {
         const float* t = spheres + 4 * size_t(i);

@@ -45,6 +45,7 @@ struct rbr_ctx {
 struct rbr_scene {
     rbr_ctx* ctx = nullptr;
     rbr::HostGeometry geom;
+    std::vector<float> triNormals;   // unit corner normals, 9 per triangle of geom.tris, or empty
     rbr::WideBVH bvh;
     bool built = false;
     rbr::DevBuf dNodes, dPrims, dQuadN, dPlanes, dPlaneIds;

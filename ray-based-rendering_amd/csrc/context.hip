@@ -10,6 +10,7 @@
 #include <string>
 
 #include "context.h"
+#include "prim_tests.h"
 #include "trace.h"
 
 using namespace rbr;
@@ -52,7 +53,7 @@ void rbr::freeBuffer(DevBuf& b) {
 int rbr::uploadBuffer(rbr_ctx* ctx, DevBuf& b, const void* src, size_t bytes) {
     int rc = ensureBuffer(ctx, b, bytes);
     if (rc) return rc;
-    if (bytes == 0) return RBR_OK;
+    if (bytes == 0 || !src) return RBR_OK;   // no source (an empty table): allocation only
     hipError_t e = hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) return failHip(ctx, e, "hipMemcpyAsync H2D");
     e = hipStreamSynchronize(ctx->stream);
@@ -308,9 +309,45 @@ int rbr_scene_set_geometry(rbr_scene* s, const float* tris, uint32_t ntri, const
         !ids(g.planeIds, plane_ids, nplane))
         return fail(ctx, RBR_ERR_INVALID, "primitive id 0xFFFFFFFF is reserved for a miss");
     s->geom = std::move(g);
+    s->triNormals.clear();
     s->built = false;
     s->bvh = WideBVH();
     bumpVersion(s);
+    return RBR_OK;
+}
+
+int rbr_scene_set_tri_normals(rbr_scene* s, const float* normals, size_t ntri) {
+    if (!s) return RBR_ERR_INVALID;
+    if (ntri == 0 || !normals) {
+        s->triNormals.clear();
+        bumpVersion(s);
+        return RBR_OK;
+    }
+    if (ntri != s->geom.tris.size() / 9)
+        return fail(s->ctx, RBR_ERR_INVALID, "vertex normals must cover every triangle of the geometry");
+    s->triNormals.resize(ntri * 9);
+    for (size_t k = 0; k < ntri * 3; ++k) {
+        const V n = vnormalized(V{normals[3 * k], normals[3 * k + 1], normals[3 * k + 2]});
+        s->triNormals[3 * k] = n.x;
+        s->triNormals[3 * k + 1] = n.y;
+        s->triNormals[3 * k + 2] = n.z;
+    }
+    bumpVersion(s);
+    return RBR_OK;
+}
+
+int rbr_triangle_shading_normal(const float* verts, const float* normals, const float* o, const float* d, float* out) {
+    if (!verts || !normals || !o || !d || !out) return RBR_ERR_INVALID;
+    const V v0{verts[0], verts[1], verts[2]};
+    const V e1 = vsub(V{verts[3], verts[4], verts[5]}, v0), e2 = vsub(V{verts[6], verts[7], verts[8]}, v0);
+    const V geo = vnormalized(vcross(e1, e2));
+    const V n = smoothNormal(V{o[0], o[1], o[2]}, V{d[0], d[1], d[2]}, v0, e1, e2,
+                             vnormalized(V{normals[0], normals[1], normals[2]}),
+                             vnormalized(V{normals[3], normals[4], normals[5]}),
+                             vnormalized(V{normals[6], normals[7], normals[8]}), geo);
+    out[0] = n.x;
+    out[1] = n.y;
+    out[2] = n.z;
     return RBR_OK;
 }
 

@@ -126,22 +126,13 @@ __device__ __forceinline__ bool guideQuery(const GuideInferArgs& a, uint32_t i, 
     if (!(jb.guided & 1u) || a.depth >= jb.guideDepth) return false;
     const uint2 h = a.hits[i];
     if (h.x == 0xFFFFFFFFu) return false;
-    const float4 s0 = jb.primShade[2 * size_t(h.x)], s1 = jb.primShade[2 * size_t(h.x) + 1];
+    const float4 s1 = jb.primShade[2 * size_t(h.x) + 1];
     if (__float_as_int(s1.w) < 0) return false;
     const float4 r0 = a.rays[2 * size_t(i)], r1 = a.rays[2 * size_t(i) + 1];
     const float t = __uint_as_float(h.y);
     const float P[3] = {r0.x + r0.w * t, r0.y + r1.x * t, r0.z + r1.y * t};
-    float n[3];
-    if (__float_as_uint(s0.w) == 1u) {
-        const float inv = 1.0f / s1.x;
-        n[0] = (P[0] - s0.x) * inv;
-        n[1] = (P[1] - s0.y) * inv;
-        n[2] = (P[2] - s0.z) * inv;
-    } else {
-        n[0] = s0.x;
-        n[1] = s0.y;
-        n[2] = s0.z;
-    }
+    const V ow = hitOutward(jb, h.x, V{r0.x, r0.y, r0.z}, V{r0.w, r1.x, r1.y}, V{P[0], P[1], P[2]});
+    const float n[3] = {ow.x, ow.y, ow.z};
     const float wo[3] = {-r0.w, -r1.x, -r1.y};
     scaleQuery(a.g, P, n, wo, s9);
     return true;

@@ -26,6 +26,34 @@ RBR_HD V vsub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 RBR_HD float vdot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 RBR_HD V vcross(V a, V b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 
+// Vec3::normalized (vector.h:209-215): v * (1 / |v|), or 0 for a zero vector.
+RBR_HD V vnormalized(V a) {
+    const float len = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    if (len > 0.0f) {
+        const float inv = 1.0f / len;
+        return {a.x * inv, a.y * inv, a.z * inv};
+    }
+    return {0.0f, 0.0f, 0.0f};
+}
+
+// Shading normal of a triangle with vertex normals where the ray meets it (Triangle::intersect,
+// triangle.cpp:6-48): the barycentrics u, v from the same Moller-Trumbore terms as the hit,
+// normalized(n0 w + n1 u + n2 v) with w = 1 - u - v, turned into the hemisphere of the geometric
+// normal geo.  n0..n2 are the unit corner normals (Triangle's constructor normalizes them).
+RBR_HD V smoothNormal(V o, V d, V v0, V e1, V e2, V n0, V n1, V n2, V geo) {
+    const V h = vcross(d, e2);
+    const float a = vdot(e1, h);
+    const float f = 1.0f / a;
+    const V s = vsub(o, v0);
+    const float u = f * vdot(s, h);
+    const V q = vcross(s, e1);
+    const float v = f * vdot(d, q);
+    const float w = 1.0f - u - v;
+    V n = vnormalized({n0.x * w + n1.x * u + n2.x * v, n0.y * w + n1.y * u + n2.y * v, n0.z * w + n1.z * u + n2.z * v});
+    if (vdot(n, geo) < 0.0f) n = {-n.x, -n.y, -n.z};
+    return n;
+}
+
 // Returns true and t when the triangle (v0, e1, e2) is hit with t in [tmin, tmax].
 RBR_HD bool triTest(V o, V d, V v0, V e1, V e2, float tmin, float tmax, float& tout) {
     V h = vcross(d, e2);

@@ -761,12 +761,12 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
             if (!hitSurf) {
                 L = L + T * ld3(jb.bg);
             } else {
-                const float4 s0 = jb.primShade[2 * size_t(h.x)], s1 = jb.primShade[2 * size_t(h.x) + 1];
+                const float4 s1 = jb.primShade[2 * size_t(h.x) + 1];
                 mat = __float_as_int(s1.w);
                 if (mat >= 0) {
                     P = ro + rd * th;
-                    const uint32_t kind = __float_as_uint(s0.w);
-                    outward = kind == 1u ? (P - mk(s0.x, s0.y, s0.z)) * (1.0f / s1.x) : mk(s0.x, s0.y, s0.z);
+                    const V ow = hitOutward(jb, h.x, V{ro.x, ro.y, ro.z}, V{rd.x, rd.y, rd.z}, V{P.x, P.y, P.z});
+                    outward = mk(ow.x, ow.y, ow.z);
                     const bool front = dot(rd, outward) < 0.0f;
                     n = front ? outward : -outward;
                     const DMaterial& m = jb.mats[mat];
@@ -1082,8 +1082,8 @@ __device__ __forceinline__ bool debugHit(const DJob& jb, const Queues& q, int cu
     const F3 ro = mk(r0.x, r0.y, r0.z), rd = mk(r0.w, r1.x, r1.y);
     t = __uint_as_float(h.y);
     P = ro + rd * t;
-    const float4 s0 = jb.primShade[2 * size_t(h.x)], s1 = jb.primShade[2 * size_t(h.x) + 1];
-    const F3 outward = __float_as_uint(s0.w) == 1u ? (P - mk(s0.x, s0.y, s0.z)) * (1.0f / s1.x) : mk(s0.x, s0.y, s0.z);
+    const V ow = hitOutward(jb, h.x, V{ro.x, ro.y, ro.z}, V{rd.x, rd.y, rd.z}, V{P.x, P.y, P.z});
+    const F3 outward = mk(ow.x, ow.y, ow.z);
     n = dot(rd, outward) < 0.0f ? outward : -outward;
     return true;
 }
@@ -1290,7 +1290,7 @@ DMaterial makeMaterial(const rbr_material& in) {
 // Per-context render state: the path pool and queues, grown on demand.
 struct RenderState {
     DevBuf ray[2], path[2], root[2], hit, sray, sroot, scontrib, socc, sindex, T, L, counts, jobs, guideOut, recOf, shit;
-    DevBuf arenaNodes, arenaPrims, arenaQuadN, arenaShade, arenaMats, arenaLights;
+    DevBuf arenaNodes, arenaPrims, arenaQuadN, arenaShade, arenaSmooth, arenaMats, arenaLights;
     std::vector<DevBuf> films, momS, momQ;
     double lastTotalMs = 0.0;
     double stageMs[4] = {0, 0, 0, 0};
@@ -1328,7 +1328,8 @@ void rbr::destroyRenderState(rbr_ctx* ctx) {
     for (DevBuf* b : {&r->ray[0], &r->ray[1], &r->path[0], &r->path[1], &r->root[0], &r->root[1], &r->hit, &r->sray,
                       &r->sroot, &r->scontrib, &r->socc, &r->sindex, &r->T, &r->L, &r->counts, &r->jobs,
                       &r->guideOut, &r->recOf,
-                      &r->arenaNodes, &r->arenaPrims, &r->arenaQuadN, &r->arenaShade, &r->arenaMats, &r->arenaLights})
+                      &r->arenaNodes, &r->arenaPrims, &r->arenaQuadN, &r->arenaShade, &r->arenaSmooth, &r->arenaMats,
+                      &r->arenaLights})
         freeBuffer(*b);
     for (auto* v : {&r->films, &r->momS, &r->momQ})
         for (DevBuf& b : *v) freeBuffer(b);
@@ -1371,8 +1372,10 @@ int rbr_scene_set_shading(rbr_scene* s, const rbr_material* materials, uint32_t 
 
 namespace {
 
-// Shading records by primitive id: (outward normal or sphere centre, kind) (radius, -, -, material).
-void buildShade(const rbr_scene* s, uint32_t idBase, int32_t matBase, std::vector<F4>& out) {
+// Shading records by primitive id: (outward normal or sphere centre, kind) (radius, smooth
+// index, -, material); triangles with vertex normals (kind 4) also get a record of 6 F4 in
+// `smooth` (v0, e1, e2 as the traversal's primitive records hold them, unit n0, n1, n2).
+void buildShade(const rbr_scene* s, uint32_t idBase, int32_t matBase, std::vector<F4>& out, std::vector<F4>& smooth) {
     const HostGeometry& g = s->geom;
     const uint32_t n = g.anyId ? g.maxId + 1 : 0;
     const size_t o = out.size();
@@ -1394,8 +1397,22 @@ void buildShade(const rbr_scene* s, uint32_t idBase, int32_t matBase, std::vecto
         float l = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
         float inv = l > 0.0f ? 1.0f / l : 0.0f;
         const uint32_t id = g.triIds[i];
-        out[o + 2 * id] = {c[0] * inv, c[1] * inv, c[2] * inv, asF(0u)};
-        out[o + 2 * id + 1] = {0, 0, 0, mat(id)};
+        const bool flat = s->triNormals.empty() ||
+                          std::all_of(&s->triNormals[9 * i], &s->triNormals[9 * i] + 9, [](float v) { return v == 0.0f; });
+        if (flat) {
+            out[o + 2 * id] = {c[0] * inv, c[1] * inv, c[2] * inv, asF(0u)};
+            out[o + 2 * id + 1] = {0, 0, 0, mat(id)};
+        } else {
+            const float* n = &s->triNormals[9 * i];
+            out[o + 2 * id] = {c[0] * inv, c[1] * inv, c[2] * inv, asF(4u)};
+            out[o + 2 * id + 1] = {0, asF(uint32_t(smooth.size() / 6)), 0, mat(id)};
+            smooth.push_back({t[0], t[1], t[2], 0});
+            smooth.push_back({e1[0], e1[1], e1[2], 0});
+            smooth.push_back({e2[0], e2[1], e2[2], 0});
+            smooth.push_back({n[0], n[1], n[2], 0});
+            smooth.push_back({n[3], n[4], n[5], 0});
+            smooth.push_back({n[6], n[7], n[8], 0});
+        }
     }
     for (size_t i = 0; i < g.sphIds.size(); ++i) {
         const float* sp = &g.sphs[4 * i];
@@ -1510,7 +1527,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
 
     // ---- node/primitive arena (packed only when several scenes share the launch); reused
     // while the same scenes, unchanged, are rendered again
-    std::vector<F4> shade;
+    std::vector<F4> shade, smooth;
     std::vector<DMaterial> mats;
     std::vector<DLight> lights;
     TraceArgs ta{};
@@ -1534,7 +1551,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         sl.idOff = idOff;
         sl.matOff = uint32_t(mats.size());
         sl.lightOff = uint32_t(lights.size());
-        buildShade(sl.s, idOff, int32_t(sl.matOff), shade);
+        buildShade(sl.s, idOff, int32_t(sl.matOff), shade, smooth);
         for (const rbr_material& m : sl.s->materials) mats.push_back(makeMaterial(m));
         for (const rbr_light& l : sl.s->lights) lights.push_back(makeLight(l));
         nodeOff += sl.s->bvh.nodeCount;
@@ -1590,6 +1607,8 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     if (!arenaValid) {
         R.arenaKey.clear();   // stays invalid if an upload below fails
         if ((rc = uploadBuffer(ctx, R.arenaShade, shade.data(), std::max<size_t>(1, shade.size()) * sizeof(F4)))) return rc;
+        if ((rc = uploadBuffer(ctx, R.arenaSmooth, smooth.data(), std::max<size_t>(1, smooth.size()) * sizeof(F4))))
+            return rc;
         if ((rc = uploadBuffer(ctx, R.arenaMats, mats.data(), std::max<size_t>(1, mats.size()) * sizeof(DMaterial)))) return rc;
         if ((rc = uploadBuffer(ctx, R.arenaLights, lights.data(), std::max<size_t>(1, lights.size()) * sizeof(DLight)))) return rc;
         R.arenaKey = key;
@@ -1633,6 +1652,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         d.medium.base = jb.scene->medium.base_height;
         d.medium.g = jb.scene->medium.g;
         d.primShade = static_cast<const float4*>(R.arenaShade.ptr);   // ids are arena-global
+        d.smooth = static_cast<const float4*>(R.arenaSmooth.ptr);
         d.mats = static_cast<const DMaterial*>(R.arenaMats.ptr);
         d.lights = static_cast<const DLight*>(R.arenaLights.ptr) + sl.lightOff;
         d.nLights = uint32_t(jb.scene->lights.size());

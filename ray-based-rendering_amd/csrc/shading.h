@@ -11,6 +11,8 @@
 
 #include <cstdint>
 
+#include "prim_tests.h"
+
 namespace rbr {
 
 enum DLobeType : uint32_t { DL_LAMBERT = 0, DL_SPEC_REFL = 1, DL_SPEC_TRANS = 2, DL_MF_REFL = 3, DL_MF_TRANS = 4 };
@@ -68,7 +70,8 @@ struct DCamera {
 struct DJob {
     DCamera cam;
     DMedium medium;
-    const float4* primShade;     // 3 float4 per global primitive id
+    const float4* primShade;     // 2 float4 per global primitive id
+    const float4* smooth;        // 6 float4 per triangle with vertex normals: v0, e1, e2, n0, n1, n2
     const DMaterial* mats;
     const DLight* lights;
     uint32_t nLights;
@@ -88,5 +91,26 @@ struct DJob {
     double* momSum;              // optional: 3 doubles per pixel
     double* momSq;
 };
+
+// Outward surface normal at a hit from the shading records (2 float4 per primitive id):
+// (normal, kind) (radius, smooth index, -, material).  Kinds 0 triangle, 2 quad and 3 plane
+// hold the geometric normal; a sphere's (kind 1) is (P - centre) / radius; a triangle with
+// vertex normals (kind 4) interpolates them at the hit (smoothNormal, Triangle::intersect,
+// triangle.cpp:40-48), for the same ray o, d that found it.
+__device__ __forceinline__ V hitOutward(const DJob& jb, uint32_t id, V o, V d, V P) {
+    const float4 s0 = jb.primShade[2 * size_t(id)], s1 = jb.primShade[2 * size_t(id) + 1];
+    const uint32_t kind = __float_as_uint(s0.w);
+    if (kind == 1u) {
+        const float inv = 1.0f / s1.x;
+        return V{(P.x - s0.x) * inv, (P.y - s0.y) * inv, (P.z - s0.z) * inv};
+    }
+    if (kind == 4u) {
+        const float4* r = jb.smooth + 6 * size_t(__float_as_uint(s1.y));
+        const float4 a = r[0], b = r[1], c = r[2], n0 = r[3], n1 = r[4], n2 = r[5];
+        return smoothNormal(o, d, V{a.x, a.y, a.z}, V{b.x, b.y, b.z}, V{c.x, c.y, c.z}, V{n0.x, n0.y, n0.z},
+                            V{n1.x, n1.y, n1.z}, V{n2.x, n2.y, n2.z}, V{s0.x, s0.y, s0.z});
+    }
+    return V{s0.x, s0.y, s0.z};
+}
 
 }  // namespace rbr

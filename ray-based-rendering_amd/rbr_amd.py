@@ -90,6 +90,9 @@ _sig(_lib.rbr_host_trace_closest, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.
 _sig(_lib.rbr_host_trace_any, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
 
 _sig(_lib.rbr_obj_parse, [C.c_char_p, _sz, _vp, _sz, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz)])
+_sig(_lib.rbr_obj_parse_normals, [C.c_char_p, _sz, _vp, _sz, C.POINTER(_sz)])
+_sig(_lib.rbr_scene_set_tri_normals, [_vp, _vp, _sz])
+_sig(_lib.rbr_triangle_shading_normal, [_vp, _vp, _vp, _vp, _vp])
 _sig(_lib.rbr_film_to_rgb8, [_vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, _vp])
 _sig(_lib.rbr_film_write_png, [C.c_char_p, _vp, C.c_int, C.c_int])
 _sig(_lib.rbr_film_write_pfm, [C.c_char_p, _vp, C.c_int, C.c_int])
@@ -242,6 +245,12 @@ class Scene:
         self.ctx.check(_lib.rbr_scene_set_geometry(self._h, _ptr(t), n(t), _ptr(ti), _ptr(s), n(s), _ptr(si),
                                                    _ptr(q), n(q), _ptr(qi), _ptr(p), n(p), _ptr(pi)))
 
+    def set_tri_normals(self, normals) -> None:
+        """Vertex normals (n, 9) for the triangles of set_geometry (None removes them)."""
+        nm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 9)
+        self._keep_normals = nm
+        self.ctx.check(_lib.rbr_scene_set_tri_normals(self._h, _ptr(nm), 0 if nm is None else len(nm)))
+
     def build_bvh(self, width: int = 4, leaf_max: int = 4, device: bool = False) -> None:
         """Host binned-SAH build, or (device=True) the GPU Morton-order build."""
         fn = _lib.rbr_scene_build_bvh_device if device else _lib.rbr_scene_build_bvh
@@ -335,9 +344,35 @@ def parse_obj(text: bytes) -> np.ndarray:
     return out
 
 
-def load_obj(path: str) -> np.ndarray:
+def parse_obj_normals(text: bytes) -> np.ndarray | None:
+    """Vertex normals (n, 9) of the triangles parse_obj returns, as written in the file, or
+    None when the reference's importer would drop them (some corner without a normal)."""
+    n = _sz(0)
+    if _lib.rbr_obj_parse_normals(text, len(text), None, 0, C.byref(n)):
+        raise ValueError("OBJ import failed: a face names a missing vertex, or no triangle")
+    if n.value == 0:
+        return None
+    out = np.zeros((n.value, 9), np.float32)
+    if _lib.rbr_obj_parse_normals(text, len(text), out.ctypes.data, n.value, C.byref(n)):
+        raise ValueError("OBJ import failed")
+    return out
+
+
+def load_obj(path: str, with_normals: bool = False):
+    """Triangles of an OBJ file; with_normals: (triangles, vertex normals or None)."""
     with open(path, "rb") as f:
-        return parse_obj(f.read())
+        text = f.read()
+    return (parse_obj(text), parse_obj_normals(text)) if with_normals else parse_obj(text)
+
+
+def triangle_shading_normal(verts, normals, o, d) -> np.ndarray:
+    """Triangle::intersect's shading normal for a triangle with vertex normals (host)."""
+    v, n = np.ascontiguousarray(verts, np.float32).reshape(9), np.ascontiguousarray(normals, np.float32).reshape(9)
+    oo, dd = np.ascontiguousarray(o, np.float32).reshape(3), np.ascontiguousarray(d, np.float32).reshape(3)
+    out = np.zeros(3, np.float32)
+    if _lib.rbr_triangle_shading_normal(v.ctypes.data, n.ctypes.data, oo.ctypes.data, dd.ctypes.data, out.ctypes.data):
+        raise ValueError("invalid argument")
+    return out
 
 
 def _film(film) -> np.ndarray:
@@ -511,6 +546,8 @@ def scene_from_desc(ctx: Context, desc, width: int = 4 | 0x100, leaf: int = 4, d
     sc = Scene(ctx)
     sc.set_geometry(tris=desc.tris if len(desc.tris) else None, spheres=desc.spheres if len(desc.spheres) else None,
                     quads=desc.quads if len(desc.quads) else None, planes=desc.planes if len(desc.planes) else None)
+    if getattr(desc, "tri_normals", None) is not None:
+        sc.set_tri_normals(desc.tri_normals)
     sc.build_bvh(width, leaf, device=device_build)
     pm = np.ascontiguousarray(desc.prim_material, np.int32)
     mats, lights, med = c_materials(desc.materials), c_lights(desc.lights), c_medium(desc.medium)

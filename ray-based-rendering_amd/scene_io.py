@@ -14,7 +14,7 @@ Scene file:
     "materials":  {"white": {"baseColor": [.73, .73, .73], "metallic": 0, "roughness": 1,
                              "ior": 1.5, "transmission": 0, "emission": [0, 0, 0]}, ...},
     "objects":    [{"type": "mesh", "file": "bunny.obj", "material": "white",
-                    "scale": 1.0, "translate": [0, 0, 0]},
+                    "scale": 1.0, "translate": [0, 0, 0], "smooth": true},
                    {"type": "sphere", "center": [x, y, z], "radius": r, "material": ...},
                    {"type": "quad", "origin": [..], "u": [..], "v": [..], "material": ...},
                    {"type": "box", "size": [..], "rotateY": deg, "translate": [..], "material": ...},
@@ -28,7 +28,9 @@ Scene file:
   }
 Material keys follow Material's parameters (material.h); "diffuse": [r, g, b] and
 "emissive": [r, g, b] are the reference's Material::diffuse / Material::emissive
-shorthands.  Relative OBJ paths are taken from the scene file's folder.  An area light of a
+shorthands.  Relative OBJ paths are taken from the scene file's folder; a mesh whose file
+has vertex normals for every corner shades with them (the reference's importer keeps them
+then) unless "smooth" is false; scale and translate do not change them.  An area light of a
 visible emitter also needs the emitting quad as an object, as in the reference's scenes.
 """
 from __future__ import annotations
@@ -115,9 +117,9 @@ def scene_from_dict(d: dict, base_dir: str = ".", width: int | None = None, heig
         t = o.get("type")
         if t == "mesh":
             path = o["file"] if os.path.isabs(o["file"]) else os.path.join(base_dir, o["file"])
-            tris = R.load_obj(path).reshape(-1, 3, 3)
-            tris = tris * F(o.get("scale", 1.0)) + np.asarray(_v3(o.get("translate")), F)
-            b.tri_mesh(tris.reshape(-1, 9).astype(F), mat_of(o))
+            tris, normals = R.load_obj(path, with_normals=True)
+            tris = tris.reshape(-1, 3, 3) * F(o.get("scale", 1.0)) + np.asarray(_v3(o.get("translate")), F)
+            b.tri_mesh(tris.reshape(-1, 9).astype(F), mat_of(o), normals if o.get("smooth", True) else None)
         elif t == "sphere":
             b.sphere(_v3(o["center"]), float(o["radius"]), mat_of(o))
         elif t == "quad":

@@ -114,6 +114,7 @@ class SceneDesc:
     medium: dict | None = None
     background: tuple = (0.0, 0.0, 0.0)
     name: str = ""
+    tri_normals: np.ndarray | None = None   # (n, 9) vertex normals; all-zero rows shade flat
 
     @property
     def prim_material(self) -> np.ndarray:
@@ -128,6 +129,7 @@ class _Builder:
     def __init__(self):
         self.tris, self.spheres, self.quads, self.planes = [], [], [], []
         self.tm, self.sm, self.qm, self.pm = [], [], [], []
+        self.tn, self.any_normals = [], False
         self.materials, self.lights = [], []
 
     def mat(self, m) -> int:
@@ -155,9 +157,14 @@ class _Builder:
         for o, u, v in corners.values():
             self.quad(rot(o) + off, rot(u), rot(v), m)
 
-    def tri_mesh(self, tris, m):
+    def tri_mesh(self, tris, m, normals=None):
         self.tris.append(np.asarray(tris, F).reshape(-1, 9))
         self.tm.append(np.full(len(self.tris[-1]), m, np.int32))
+        nrm = np.zeros((len(self.tris[-1]), 9), F) if normals is None else np.asarray(normals, F).reshape(-1, 9)
+        if len(nrm) != len(self.tris[-1]):
+            raise ValueError("one row of vertex normals per triangle")
+        self.tn.append(nrm)
+        self.any_normals = self.any_normals or normals is not None
 
     def sphere(self, c, r, m):
         self.spheres.append(np.array([c[0], c[1], c[2], r], F))
@@ -173,7 +180,8 @@ class _Builder:
             tri_mat=np.concatenate(self.tm).astype(np.int32) if self.tm else np.zeros(0, np.int32),
             sphere_mat=np.asarray(self.sm, np.int32), quad_mat=np.asarray(self.qm, np.int32),
             plane_mat=np.asarray(self.pm, np.int32), materials=self.materials, lights=self.lights,
-            camera=camera, medium=medium, background=tuple(background), name=name)
+            camera=camera, medium=medium, background=tuple(background), name=name,
+            tri_normals=np.concatenate(self.tn).astype(F) if self.any_normals else None)
 
 
 def cornell_c0(width: int = 256, height: int = 256) -> SceneDesc:
@@ -196,6 +204,25 @@ def cornell_c0(width: int = 256, height: int = 256) -> SceneDesc:
     b.lights.append(area_light((278, 554, 279.5), (130, 0, 0), (0, 0, 105), (17, 12, 4), 1.0))
     cam = look_at((278, 278, -800), (278, 278, 0), (0, 1, 0), width, height, 40.0)
     return b.build(cam, name="cornell555")
+
+
+def smooth_ball(w: int = 40, h: int = 30, smooth: bool = True) -> SceneDesc:
+    """A coarse icosphere (r = 1, 320 triangles) whose corner normals are the radial
+    directions, on a floor under an area light: the smooth-shading (vertex normal) case of
+    Triangle::intersect.  smooth=False: the same triangles shaded flat."""
+    import rbr_amd as R   # the icosphere generator lives in the host library
+    b = _Builder()
+    white = b.mat(diffuse((0.7, 0.7, 0.7)))
+    ball = b.mat(material((0.8, 0.5, 0.3), 0.0, 0.4, 1.5, 0.0, (0.0, 0.0, 0.0)))
+    light = b.mat(emissive((8.0, 8.0, 8.0)))
+    tris = R.icosphere(2, 1.0, 0.0, 3).astype(F)
+    v = tris.reshape(-1, 3, 3).astype(np.float64)
+    nrm = (v / np.linalg.norm(v, axis=2, keepdims=True)).astype(F).reshape(-1, 9)
+    b.tri_mesh(tris + np.tile(F([0, 1, 0]), 3), ball, nrm if smooth else None)
+    b.quad((-4, 0, -4), (8, 0, 0), (0, 0, 8), white)
+    b.quad((-1, 3.5, -1), (2, 0, 0), (0, 0, 2), light)
+    b.lights.append(area_light((0, 3.5, 0), (2, 0, 0), (0, 0, 2), (8, 8, 8), 1.0))
+    return b.build(look_at((0.0, 1.6, 4.0), (0.0, 1.0, 0.0), (0, 1, 0), w, h, 35.0), name="smooth ball")
 
 
 def cornell_indirect(width: int = 256, height: int = 256, emission=(255.0, 180.0, 60.0)) -> SceneDesc:

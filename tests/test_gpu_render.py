@@ -233,3 +233,54 @@ def test_debug_integrators_launch_rules(ctx):
     a, b = R.render(ctx, [dict(scene=sc, spp=4, depth=1, integrator=R.NORMAL),
                           dict(scene=sc, spp=4, depth=1, integrator=R.AO, ao_samples=4, ao_distance=50.0)])
     assert a.shape == b.shape == (16, 16, 3) and np.isfinite(a).all() and (b >= 0).all() and (b <= 1).all()
+
+
+
+def test_smooth_mesh_path_unbiased_vs_oracle(ctx):
+    """Triangles with vertex normals (a coarse icosphere with radial corner normals) shade with
+    the normal interpolated at the hit, as Triangle::intersect does: the GPU path tracer
+    against the oracle (bit-exact with the reference's own integrator on this scene,
+    test_oracle_render.py), per-pixel z-test; the flat-shaded GPU image fails the same test."""
+    say("case: smooth mesh, path tracing")
+    desc = S.smooth_ball(40, 30)
+    gpu_vs_oracle(ctx, desc, 1024, 1024, 4)
+    sc = R.scene_from_desc(ctx, S.smooth_ball(40, 30, smooth=False))
+    _, g1, g2 = R.render_moments(ctx, dict(scene=sc, spp=1024, depth=4, seed=11))
+    r1, r2 = O.OracleRender(desc).moments(1024, 4, seed=5)
+    z, mg, mr, bz, zi = zscores(g1, g2, 1024, r1, r2, 1024)
+    say(f"  flat vs smooth oracle: |z|>4.5 {(np.abs(z) > 4.5).mean():.4f} image z {zi:+.2f}")
+    assert (np.abs(z) > 4.5).mean() > 0.05
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference library not built")
+def test_smooth_mesh_normals_match_reference(ctx):
+    """The NormalIntegrator image of the smooth mesh against the reference's own
+    NormalIntegrator on the same smooth triangles (z-test with the GPU variance standing in for
+    the reference's); the flat-shaded image fails it."""
+    say("case: smooth mesh, normal integrator")
+    n = 256
+    desc = S.smooth_ball(40, 30)
+    sc = R.scene_from_desc(ctx, desc)
+    _, g1, g2 = R.render_moments(ctx, dict(scene=sc, spp=n, depth=1, seed=21, integrator=R.NORMAL))
+    flat = R.render(ctx, [dict(scene=R.scene_from_desc(ctx, S.smooth_ball(40, 30, smooth=False)), spp=n, depth=1,
+                               seed=21, integrator=R.NORMAL)])[0]
+    ref = O.RefRender(desc)
+    rfilm, _ = ref.render(n, 1, integrator=R.NORMAL)
+    ref.close()
+    mg = (g1 / n).reshape(-1, 3)
+    vg = np.maximum(g2 / n - (g1 / n) ** 2, 0.0).reshape(-1, 3)
+    mr = rfilm.reshape(-1, 3).astype(np.float64)
+    se = np.sqrt(2.0 * vg / n)
+    ok = se > 0
+
+    def stats(m):
+        z = np.zeros_like(m)
+        z[ok] = (m[ok] - mr[ok]) / se[ok]
+        zi = float((m.sum() - mr.sum()) / np.sqrt(max((2.0 * vg / n).sum(), 1e-30)))
+        return float(np.median(z[ok])), float((np.abs(z[ok]) > 5).mean()), zi
+
+    med, tail, zi = stats(mg)
+    _, ftail, fzi = stats(flat.reshape(-1, 3).astype(np.float64))
+    say(f"  smooth: median z {med:+.3f} |z|>5 {tail:.4f} image z {zi:+.2f}; flat: |z|>5 {ftail:.4f} image z {fzi:+.2f}")
+    assert abs(med) < 0.15 and tail < 0.01 and abs(zi) < 5.0
+    assert ftail > 0.05 or abs(fzi) > 10.0

@@ -25,6 +25,19 @@ def test_cornell_path_bit_exact(nee, mis):
     assert o.mean() > 0.01
 
 
+@pytest.mark.parametrize("smooth", [True, False])
+def test_smooth_mesh_path_bit_exact(smooth):
+    """Triangles with vertex normals: the oracle's interpolated shading normal and the
+    reference's Triangle::intersect (triangle.cpp:40-48) give the same image bit for bit,
+    except where a path meets a shared edge of two triangles (both report the same t; the
+    reference keeps the one it tested last, the oracle the lowest id) or the ball's contact
+    point with the floor: about 1 pixel in 1,000, smooth or flat alike."""
+    o, r, exact = compare(S.smooth_ball(40, 30, smooth), 8, 4)
+    assert exact >= 0.998, exact
+    assert abs(float(o.mean()) - float(r.mean())) < 3e-3 * float(r.mean())
+    assert o.mean() > 0.01
+
+
 def test_materials_path_bit_exact():
     # Spheres rest on the floor: a path through the exact contact point is a tie between
     # two primitives, which the reference resolves by test order and the oracle by lowest

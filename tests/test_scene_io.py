@@ -178,3 +178,75 @@ def test_render_config_on_gpu(tmp_path):
     assert film.shape == (48, 64, 3) and np.array_equal(film, ref)
     assert np.array_equal(R.read_pfm(str(tmp_path / "out" / "cornell.pfm")), film)
     assert (tmp_path / "out" / "cornell.png").stat().st_size > 64 * 48 * 3
+
+
+OBJ_NORMAL_CASES = {
+    "all_corners": b"v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 2\nvn 0 0.6 0.8\nvn 1 1 1\nf 1//1 2//2 3//3\n",
+    "quad_fan_shared": (b"v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nvt 0 0\nvn 0 0 1\nvn 0.1 0 1\n"
+                        b"f 1/1/1 2/1/2 3/1/1 4/1/2\n"),
+    "negative_indices": b"v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nvn 1 0 0\nf -3//-2 -2//-1 -1//-2\n",
+    "one_corner_without": b"v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nf 1//1 2//1 3\n",
+    "zero_normal": b"v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nvn 0 0 0\nf 1//1 2//2 3//1\n",
+    "out_of_range": b"v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nf 1//1 2//5 3//1\n",
+    "no_normals": b"v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n",
+    "degenerate_face_without": b"v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nf 1//1 2//1 3//1\nf 1 2\n",
+    "tiny_normal": b"v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1e-30\nf 1//1 2//1 3//1\n",
+}
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference library not built")
+@pytest.mark.parametrize("name", sorted(OBJ_NORMAL_CASES))
+def test_obj_normals_match_reference_importer(name):
+    """Vertex normals kept (or dropped) exactly as the reference's importer does, per corner."""
+    text = OBJ_NORMAL_CASES[name]
+    ref = O.ref_obj_parse_normals(text)
+    mine = R.parse_obj_normals(text)
+    assert (ref is None) == (mine is None), (ref, mine)
+    if ref is not None:
+        assert np.array_equal(mine.view(np.uint32), ref.view(np.uint32))
+    assert np.array_equal(R.parse_obj(text), O.ref_obj_parse(text))
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference library not built")
+def test_shading_normal_matches_reference_triangle():
+    """rbr_triangle_shading_normal (the function the renderer evaluates for triangles with
+    vertex normals) against the reference's Triangle::intersect on random triangles, corner
+    normals (unnormalized, some facing away) and rays: the same bits after the reference's
+    setFaceNormal (faced towards the ray)."""
+    rng = np.random.default_rng(17)
+    checked = 0
+    for _ in range(3000):
+        v = rng.normal(size=9).astype(np.float32) * np.float32(3)
+        n = rng.normal(size=9).astype(np.float32)
+        bary = rng.dirichlet([1, 1, 1])
+        target = (bary[0] * v[0:3] + bary[1] * v[3:6] + bary[2] * v[6:9]).astype(np.float32)
+        o = (target + rng.normal(size=3) * 4).astype(np.float32)
+        d = (target - o).astype(np.float32)
+        hit = O.ref_triangle_normal(v, n, o, d)
+        if hit is None:
+            continue
+        rn, front, dn = hit
+        mine = R.triangle_shading_normal(v, n, o, dn)
+        # the reference's frontFace test: dot(ray.direction, outward) < 0 in float
+        f32 = np.float32
+        dotv = f32(f32(dn[0] * mine[0]) + f32(dn[1] * mine[1])) + f32(dn[2] * mine[2])
+        faced = mine if dotv < 0 else -mine
+        assert bool(dotv < 0) == front
+        assert np.array_equal(faced.view(np.uint32), rn.view(np.uint32)), (v, n, o, dn, faced, rn)
+        checked += 1
+    assert checked > 2000
+
+
+def test_json_scene_smooth_mesh(tmp_path):
+    text = OBJ_NORMAL_CASES["all_corners"]
+    (tmp_path / "m.obj").write_bytes(text)
+    d = {"materials": {"m": {"diffuse": [0.5, 0.5, 0.5]}},
+         "objects": [{"type": "mesh", "file": "m.obj", "material": "m", "scale": 2.0},
+                     {"type": "mesh", "file": "m.obj", "material": "m", "smooth": False},
+                     {"type": "sphere", "center": [0, 5, 0], "radius": 1, "material": "m"}],
+         "camera": {"width": 8, "height": 8}}
+    desc = scene_io.scene_from_dict(d, str(tmp_path))
+    assert desc.tri_normals is not None and desc.tri_normals.shape == (2, 9)
+    assert np.array_equal(desc.tri_normals[0], R.parse_obj_normals(text)[0]) and not desc.tri_normals[1].any()
+    flat = scene_io.scene_from_dict({**d, "objects": d["objects"][1:]}, str(tmp_path))
+    assert flat.tri_normals is None
