@@ -123,7 +123,8 @@ enum {
     RBR_INTEGRATOR_VOLPATH = 1,
     RBR_INTEGRATOR_NORMAL = 2,
     RBR_INTEGRATOR_DEPTH = 3,
-    RBR_INTEGRATOR_AO = 4
+    RBR_INTEGRATOR_AO = 4,
+    RBR_INTEGRATOR_WHITTED = 5   /* WhittedIntegrator (whitted_integrator.cpp) */
 };
 
 /* IntegratorConfig (integrator.h:22-28) plus the GPU stream seed. */
@@ -142,6 +143,9 @@ typedef struct rbr_render_config {
     float ao_distance;      /* AO ray length (2.0)                   */
     float depth_near;       /* DEPTH near plane (0.1)                */
     float depth_far;        /* DEPTH far plane (100.0)               */
+    /* WHITTED: the scene's ambient light (Scene::ambientLight, scene.h:28; the reference's
+     * default is 0.1 in every channel, here it is what the caller sets) */
+    float ambient[3];
 } rbr_render_config;
 
 typedef struct rbr_ctx rbr_ctx;

@@ -14,6 +14,7 @@
 #include "render/lighting/point_light.h"
 #include "render/renderers/path_tracing_integrator.h"
 #include "render/renderers/volumetric_path_integrator.h"
+#include "render/renderers/whitted_integrator.h"
 #include "render/scene/scene.h"
 
 #include <omp.h>
@@ -135,7 +136,10 @@ double ref_render(void* h, const rbr_camera* cam, const rbr_render_config* cfg, 
         integ = (cfg->depth_near != 0.0f || cfg->depth_far != 0.0f)
                     ? std::make_unique<DepthIntegrator>(cfg->depth_near, cfg->depth_far)
                     : std::make_unique<DepthIntegrator>();
-    else if (cfg->integrator == RBR_INTEGRATOR_AO)
+    else if (cfg->integrator == RBR_INTEGRATOR_WHITTED) {
+        integ = std::make_unique<WhittedIntegrator>(ic);
+        r->scene.ambientLight = v3(cfg->ambient);
+    } else if (cfg->integrator == RBR_INTEGRATOR_AO)
         integ = std::make_unique<AOIntegrator>(cfg->ao_samples ? cfg->ao_samples : 16,
                                                cfg->ao_distance > 0.0f ? cfg->ao_distance : 2.0f);
     else integ = std::make_unique<PathTracingIntegrator>(ic);

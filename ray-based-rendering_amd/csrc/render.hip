@@ -31,6 +31,7 @@
 #include "guide_dev.h"
 #include "shading.h"
 #include "trace.h"
+#include "trace_dev.h"
 
 // Geometry that decides occlusion (hit points, light sample points, shadow-ray origins and
 // lengths) must round exactly as the reference does: an emitter quad coplanar with its
@@ -1174,6 +1175,198 @@ __global__ __launch_bounds__(256) void resetCountsKernel(uint32_t* counts, int n
 }
 
 // Adds this round's samples of one job to its film (and optional moments), in sample order.
+// ---------------------------------------------------------------- Whitted
+
+constexpr uint32_t kWhittedMaxDepth = 16;
+constexpr uint32_t kWhittedStack = kWhittedMaxDepth + 2;   // pending rays of a depth-first walk
+
+// refract of whitted_integrator.cpp:102-110 (eta = etaI / etaT, normal facing the incident ray)
+__device__ __forceinline__ bool whittedRefract(F3 incident, F3 n, float eta, F3& out) {
+    const float cosI = -dot(incident, n);
+    const float sin2T = eta * eta * (1.0f - cosI * cosI);
+    if (sin2T > 1.0f) return false;
+    const float cosT = sqrtf(1.0f - sin2T);
+    out = normalize(incident * eta + n * (eta * cosI - cosT));
+    return true;
+}
+
+// WhittedIntegrator::Li (whitted_integrator.cpp:7-110) for one camera sample per thread.  The
+// reference's recursion is a tree whose value is linear in its leaves, so a thread walks it
+// depth first with an explicit stack of (ray, weight, depth): each surface adds its classic
+// local shading (classic_shading.h: ambient, then per light a shadow ray and Lambert + Blinn-
+// Phong terms) times the weight of the path to it, a smooth conductor pushes its mirror ray
+// (weight x baseColor x metallic), a transmissive dielectric pushes the reflected ray
+// (x (1-m) t F) and, below total internal reflection, the refracted ray
+// (x (1-m) t (1-F) baseColor); misses, surfaces without material and rays deeper than the
+// maximum depth return the background, emitters their emission.  Rays are traced inline with
+// the traversal of trace_dev.h (closest hit; any hit for shadow rays).
+template <int W, bool Q, bool MIXED>
+__global__ __launch_bounds__(256) void whittedKernel(const DJob* __restrict__ jobs, int njobs, uint32_t total,
+                                                     TraceArgs a, Queues q) {
+    __shared__ uint2 ldsStack[kLdsDepthDefault * 256];
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= total) return;
+    const int j = findJob(jobs, njobs, i);
+    const DJob& jb = jobs[j];
+    const uint32_t local = i - jb.pathBase;
+    const uint32_t pixel = local % jb.pixels;
+    const uint32_t sample = jb.sampleBase + local / jb.pixels;
+    const Rng rng = pathRng(jb.seed, pixel, sample, jb.pixels);
+    const DCamera& c = jb.cam;
+    const float px = float(pixel % c.width) + rng.u(0);
+    const float py = float(pixel / c.width) + rng.u(1);
+    const F3 local3 = mk((px - c.cx) / c.fx, (c.cy - py) / c.fy, 1.0f);
+    const F3 d0 = normalize(normalize(ld3(c.right) * local3.x + ld3(c.up) * local3.y + ld3(c.fwd) * local3.z));
+
+    tdev::PrivateSpillStack<kLdsDepthDefault, 256> st;
+    st.lds = ldsStack;
+    F3 so[kWhittedStack], sd[kWhittedStack], sw[kWhittedStack];
+    uint32_t sdep[kWhittedStack];
+    int n = 0;
+    so[0] = ld3(c.pos);
+    sd[0] = d0;
+    sw[0] = mk(1.0f, 1.0f, 1.0f);
+    sdep[0] = 0;
+    n = 1;
+    F3 L = mk(0, 0, 0);
+    const F3 bg = ld3(jb.bg);
+    uint32_t vtx = 0;   // shaded vertices so far: random dimensions of their light samples
+    while (n > 0) {
+        --n;
+        const F3 o = so[n], d = sd[n], w = sw[n];
+        const uint32_t depth = sdep[n];
+        if (depth > jb.maxDepth) {
+            L = L + w * bg;
+            continue;
+        }
+        tdev::DRay r;
+        r.o = V{o.x, o.y, o.z};
+        r.d = V{d.x, d.y, d.z};
+        r.tmin = kEpsD;
+        r.tmax = INFINITY;
+        tdev::setupRay(r);
+        float t;
+        uint32_t id;
+        bool occ;
+        st.sp = 0;
+        tdev::traceRay<W, Q, false, MIXED>(a, r, jb.root, st, t, id, occ);
+        if (id == RBR_MISS_ID) {
+            L = L + w * bg;
+            continue;
+        }
+        const int32_t mi = __float_as_int(jb.primShade[2 * size_t(id) + 1].w);
+        if (mi < 0) {
+            L = L + w * bg;
+            continue;
+        }
+        const DMaterial& m = jb.mats[mi];
+        if (m.emissive) {
+            L = L + w * ld3(m.emission);
+            continue;
+        }
+        const F3 P = o + d * t;
+        const V ow = hitOutward(jb, id, r.o, r.d, V{P.x, P.y, P.z});
+        const F3 outward = mk(ow.x, ow.y, ow.z);
+        const bool front = dot(d, outward) < 0.0f;
+        const F3 nn = front ? outward : -outward;
+        const F3 wo = -d;
+        const F3 base = ld3(m.base);
+        const float mm = fminf(fmaxf(m.metallic, 0.0f), 1.0f);
+        const float tt = fminf(fmaxf(m.transmission, 0.0f), 1.0f);
+        const bool smooth = m.roughness < 0.03f;
+        const float dw = (1.0f - mm) * (1.0f - tt);
+        const float cw = smooth ? 0.0f : mm;
+        if (dw > 0.0f || cw > 0.0f) {   // classic::shade (classic_shading.h:27-58)
+            F3 res = base * (0.1f * dw) * ld3(jb.ambient);
+            const float al = fmaxf(m.roughness * m.roughness, 1e-2f);
+            const float shin = fmaxf(2.0f / (al * al) - 2.0f, 1.0f);
+            const float f0 = fabsf(m.ior - 1.0f) > 1e-3f ? ((m.ior - 1.0f) / (m.ior + 1.0f)) * ((m.ior - 1.0f) / (m.ior + 1.0f))
+                                                         : 0.0f;
+            const F3 spec = mk(f0 * dw, f0 * dw, f0 * dw) + base * cw;
+            const float specNorm = (shin + 8.0f) / (8.0f * 3.14159265358979323846f);
+            for (uint32_t l = 0; l < jb.nLights; ++l) {
+                const uint32_t dim = 2u + 2u * (vtx * kMaxLights + l);
+                const LSample ls = sampleLight(jb.lights[l], P, rng.u(dim), rng.u(dim + 1));
+                if (ls.pdf <= 0.0f) continue;
+                F3 sro, srd;
+                spawn(P, nn, ls.wi, sro, srd);
+                tdev::DRay sr;
+                sr.o = V{sro.x, sro.y, sro.z};
+                sr.d = V{srd.x, srd.y, srd.z};
+                sr.tmin = kEpsD;
+                sr.tmax = ls.dist - kEpsD;
+                tdev::setupRay(sr);
+                float st2;
+                uint32_t sid;
+                bool blocked;
+                st.sp = 0;
+                tdev::traceRay<W, Q, true, MIXED>(a, sr, jb.root, st, st2, sid, blocked);
+                if (blocked) continue;
+                const float ndl = fmaxf(0.0f, dot(nn, ls.wi));
+                if (ndl <= 0.0f) continue;
+                F3 shaded = base * (0.7f * dw * ndl);
+                const F3 hv = normalize(ls.wi + wo);
+                const float ndh = fmaxf(0.0f, dot(nn, hv));
+                shaded = shaded + spec * (specNorm * powf(ndh, shin) * ndl);
+                res = res + shaded * ls.Li;
+            }
+            L = L + w * res;
+        }
+        ++vtx;
+        const F3 rdir = d - nn * (2.0f * dot(d, nn));   // reflect (vector.h:277-279)
+        if (smooth && mm > 0.0f && n < int(kWhittedStack)) {   // traceReflection
+            spawn(P, nn, rdir, so[n], sd[n]);
+            sw[n] = w * base * mm;
+            sdep[n] = depth + 1;
+            ++n;
+        }
+        if (tt > 0.0f && mm < 1.0f) {   // traceRefraction
+            const float cosI = fabsf(dot(d, nn));
+            float etaI = 1.0f, etaT = m.ior;
+            if (!front) {
+                etaI = m.ior;
+                etaT = 1.0f;
+            }
+            const float kr = fresnelDielectric(cosI, etaI, etaT);
+            const float share = (1.0f - mm) * tt;
+            if (n < int(kWhittedStack)) {
+                spawn(P, nn, rdir, so[n], sd[n]);
+                sw[n] = w * (share * kr);
+                sdep[n] = depth + 1;
+                ++n;
+            }
+            F3 tdir;
+            if (kr < 1.0f && whittedRefract(d, nn, etaI / etaT, tdir) && n < int(kWhittedStack)) {
+                so[n] = P - nn * kEpsD;
+                sd[n] = normalize(tdir);
+                sw[n] = w * base * (share * (1.0f - kr));
+                sdep[n] = depth + 1;
+                ++n;
+            }
+        }
+    }
+    q.L[i] = make_float4(L.x, L.y, L.z, 0.0f);
+}
+
+template <int W, bool Q>
+hipError_t launchWhittedW(const DJob* jobs, int njobs, uint32_t total, const TraceArgs& a, const Queues& q, hipStream_t s) {
+    const dim3 g((total + 255) / 256);
+    if (a.mixed) hipLaunchKernelGGL((whittedKernel<W, Q, true>), g, dim3(256), 0, s, jobs, njobs, total, a, q);
+    else hipLaunchKernelGGL((whittedKernel<W, Q, false>), g, dim3(256), 0, s, jobs, njobs, total, a, q);
+    return hipGetLastError();
+}
+
+hipError_t launchWhitted(int width, const DJob* jobs, int njobs, uint32_t total, const TraceArgs& a, const Queues& q,
+                         hipStream_t s) {
+    switch (width) {
+        case 4: return launchWhittedW<4, false>(jobs, njobs, total, a, q, s);
+        case 8: return launchWhittedW<8, false>(jobs, njobs, total, a, q, s);
+        case 4 | kQuantFlag: return launchWhittedW<4, true>(jobs, njobs, total, a, q, s);
+        case 8 | kQuantFlag: return launchWhittedW<8, true>(jobs, njobs, total, a, q, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 __global__ __launch_bounds__(256) void accumulateKernel(const DJob* __restrict__ jobs, int j, uint32_t samples,
                                                         const float4* __restrict__ L) {
     const DJob& jb = jobs[j];
@@ -1215,6 +1408,11 @@ __global__ void zeroKernelD(double* p, size_t n) {
 
 DMaterial makeMaterial(const rbr_material& in) {
     DMaterial m{};
+    for (int k = 0; k < 3; ++k) m.base[k] = in.base_color[k];
+    m.metallic = in.metallic;
+    m.roughness = in.roughness;
+    m.ior = in.ior;
+    m.transmission = in.transmission;
     const float metallic = std::min(std::max(in.metallic, 0.0f), 1.0f);
     const float t = std::min(std::max(in.transmission, 0.0f), 1.0f);
     const float dielectric = 1.0f - metallic;
@@ -1471,7 +1669,8 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     std::map<const rbr_scene*, int> sceneIdx;
     int width = 0;
     uint32_t maxStack = 1, maxDepthAll = 0, maxLights = 0;
-    bool anyMixed = false, anyPlanes = false, anyGuided = false, anyRecord = false, anyDebug = false, anyAO = false;
+    bool anyMixed = false, anyPlanes = false, anyGuided = false, anyRecord = false, anyDebug = false, anyAO = false,
+         anyWhitted = false;
     uint32_t aoSlots = 0;
     for (int j = 0; j < njobs; ++j) {
         const rbr_job& jb = jobs[j];
@@ -1480,12 +1679,17 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         if (!jb.film_rgb) return fail(ctx, RBR_ERR_INVALID, "job film is null");
         if (jb.camera.width < 1 || jb.camera.height < 1 || jb.config.spp < 1 || jb.config.max_depth < 1)
             return fail(ctx, RBR_ERR_INVALID, "job needs width, height, spp and max_depth >= 1");
-        if (jb.config.integrator < RBR_INTEGRATOR_PATH || jb.config.integrator > RBR_INTEGRATOR_AO)
+        if (jb.config.integrator < RBR_INTEGRATOR_PATH || jb.config.integrator > RBR_INTEGRATOR_WHITTED)
             return fail(ctx, RBR_ERR_INVALID, "unknown integrator");
-        const bool dbg = jb.config.integrator >= RBR_INTEGRATOR_NORMAL;
-        if (j > 0 && dbg != anyDebug)
-            return fail(ctx, RBR_ERR_UNSUPPORTED, "debug integrators cannot share a launch with path integrators");
+        const bool dbg = jb.config.integrator >= RBR_INTEGRATOR_NORMAL && jb.config.integrator <= RBR_INTEGRATOR_AO;
+        const bool wht = jb.config.integrator == RBR_INTEGRATOR_WHITTED;
+        if (j > 0 && (dbg != anyDebug || wht != anyWhitted))
+            return fail(ctx, RBR_ERR_UNSUPPORTED,
+                        "path, debug and Whitted integrators cannot share a launch (one kind per rbr_render)");
         anyDebug = dbg;
+        anyWhitted = wht;
+        if (wht && jb.config.max_depth > int32_t(kWhittedMaxDepth))
+            return fail(ctx, RBR_ERR_UNSUPPORTED, "Whitted max_depth above 16");
         anyAO |= jb.config.integrator == RBR_INTEGRATOR_AO;
         if (jb.config.integrator == RBR_INTEGRATOR_AO) {
             if (jb.config.ao_samples < 0 || jb.config.ao_samples > 256 || jb.config.ao_distance < 0.0f)
@@ -1519,6 +1723,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     if (anyDebug) maxLights = aoSlots;   // debug launches use the shadow queue for AO rays only
     if (width == 0) width = 4;
     if (anyPlanes && scenes.size() > 1) return fail(ctx, RBR_ERR_UNSUPPORTED, "infinite planes in a multi-scene render");
+    if (anyWhitted && maxStack > uint32_t(kMaxStack)) return fail(ctx, RBR_ERR_UNSUPPORTED, "BVH too deep for the Whitted stack");
     if (!ctx->render) ctx->render = new RenderState();
     RenderState& R = *ctx->render;
     // node layout of the launch (all scenes share it): dwords per node, child-ref offset
@@ -1666,6 +1871,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         d.aoDistance = jb.config.ao_distance > 0.0f ? jb.config.ao_distance : 2.0f;
         d.depthNear = jb.config.depth_near != 0.0f || jb.config.depth_far != 0.0f ? jb.config.depth_near : 0.1f;
         d.depthFar = jb.config.depth_near != 0.0f || jb.config.depth_far != 0.0f ? jb.config.depth_far : 100.0f;
+        for (int k = 0; k < 3; ++k) d.ambient[k] = jb.config.ambient[k];
         d.root = jb.scene->bvh.empty() ? kEmpty : sl.nodeOff;
         d.pixels = uint32_t(jb.camera.width) * uint32_t(jb.camera.height);
         d.seed = jb.config.seed;
@@ -1780,6 +1986,16 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         const unsigned gridSort = std::min((used + kSortChunk - 1) / kSortChunk, cus * 4u);
         // The round's launch sequence; it allocates nothing, so it can be captured.
         auto launchRound = [&]() -> int {
+            if (anyWhitted) {
+                if (hipError_t we = launchWhitted(width, djd, njobs, used, ta, q, ctx->stream); we != hipSuccess)
+                    return failHip(ctx, we, "Whitted launch");
+                for (int j = 0; j < njobs; ++j) {
+                    if (!take[j]) continue;
+                    hipLaunchKernelGGL(accumulateKernel, dim3((dj[j].pixels + 255) / 256), dim3(256), 0, ctx->stream,
+                                       djd, j, take[j], q.L);
+                }
+                return RBR_OK;
+            }
             hipLaunchKernelGGL(cameraKernel, dim3(grid), dim3(256), 0, ctx->stream, djd, njobs, used, q);
             int cur = 0;
             for (uint32_t depth = 0; depth < depthRound; ++depth) {

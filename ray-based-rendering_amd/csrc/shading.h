@@ -38,7 +38,8 @@ struct DMaterial {       // 6 lobes + header
     uint32_t sfKind;
     float sfEtaI, sfEtaT;
     float emission[3];
-    float pad[4];
+    // the material's own parameters, for the classic shading of the Whitted integrator
+    float base[3], metallic, roughness, ior, transmission;
 };
 
 struct DLight {
@@ -81,6 +82,7 @@ struct DJob {
     uint32_t aoSamples;          // AO: rays per camera sample
     float aoDistance;            // AO: ray length
     float depthNear, depthFar;   // DEPTH: mapping range
+    float ambient[3];            // WHITTED: Scene::ambientLight
     uint32_t root;               // BVH root node in the node arena
     uint32_t pathBase, pathCount;   // pool slots of this job in the current round
     uint32_t sampleBase;         // sample index of the job's first pool slot this round

@@ -473,7 +473,7 @@ class RbrRenderConfig(C.Structure):
     _fields_ = [("integrator", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32), ("use_nee", C.c_int32),
                 ("use_mis", C.c_int32), ("guided", C.c_int32), ("seed", C.c_uint64), ("sample_offset", C.c_uint32),
                 ("guide_depth", C.c_uint32), ("ao_samples", C.c_int32), ("ao_distance", C.c_float),
-                ("depth_near", C.c_float), ("depth_far", C.c_float)]
+                ("depth_near", C.c_float), ("depth_far", C.c_float), ("ambient", C.c_float * 3)]
 
 
 class RbrJob(C.Structure):
@@ -481,7 +481,7 @@ class RbrJob(C.Structure):
                 ("film_rgb", C.POINTER(C.c_float))]
 
 
-PATH, VOLPATH, NORMAL, DEPTH, AO = 0, 1, 2, 3, 4
+PATH, VOLPATH, NORMAL, DEPTH, AO, WHITTED = 0, 1, 2, 3, 4, 5
 
 _sig(_lib.rbr_scene_set_shading, [_vp, C.POINTER(RbrMaterial), _u32, _vp, _u32, C.POINTER(RbrLight), _u32,
                                   C.POINTER(RbrMedium), _vp])
@@ -529,15 +529,17 @@ def c_camera(cam) -> RbrCamera:
 
 def c_config(spp: int, depth: int, seed: int = 1, integrator: int = PATH, nee: bool = True, mis: bool = True,
              sample_offset: int = 0, guided: int = 0, guide_depth: int = 0, ao_samples: int = 0, ao_distance: float = 0.0,
-             depth_near: float = 0.0, depth_far: float = 0.0) -> RbrRenderConfig:
+             depth_near: float = 0.0, depth_far: float = 0.0, ambient=(0.1, 0.1, 0.1)) -> RbrRenderConfig:
+    """ambient: the Whitted integrator's Scene::ambientLight (the reference's default 0.1)."""
     return RbrRenderConfig(integrator, spp, depth, int(nee), int(mis), int(guided), seed, sample_offset, guide_depth,
-                           ao_samples, ao_distance, depth_near, depth_far)
+                           ao_samples, ao_distance, depth_near, depth_far, (C.c_float * 3)(*map(float, ambient)))
 
 
 def _job_config(j: dict) -> RbrRenderConfig:
     return c_config(j["spp"], j["depth"], j.get("seed", 1), j.get("integrator", PATH), j.get("nee", True),
                     j.get("mis", True), j.get("sample_offset", 0), j.get("guided", 0), j.get("guide_depth", 0),
-                    j.get("ao_samples", 0), j.get("ao_distance", 0.0), j.get("depth_near", 0.0), j.get("depth_far", 0.0))
+                    j.get("ao_samples", 0), j.get("ao_distance", 0.0), j.get("depth_near", 0.0), j.get("depth_far", 0.0),
+                    j.get("ambient", (0.1, 0.1, 0.1)))
 
 
 def scene_from_desc(ctx: Context, desc, width: int = 4 | 0x100, leaf: int = 4, device_build: bool = False) -> Scene:

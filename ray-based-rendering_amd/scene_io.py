@@ -163,20 +163,26 @@ def load_scene(path: str, **kw) -> S.SceneDesc:
 
 
 # Render configuration (the reference's configs/*.json) -> render settings.  Integrator types
-# with a device path: "pathtracing" and "volumepath"; cameras: "perspective" (pinhole; an
-# aperture > 0 is refused, there is no thin lens on the device).
+# with a device path (config.cpp:45-52): "pathtracing" / "pt", "volumepath" / "vpt",
+# "whitted", "normal", "depth", "ao"; cameras: "perspective" (pinhole; an aperture > 0 is
+# refused, there is no thin lens on the device).  "ambient" is the scene's ambient light, which
+# the Whitted integrator uses (the reference's default 0.1).
+_INTEGRATORS = {"pathtracing": R.PATH, "pt": R.PATH, "volumepath": R.VOLPATH, "vpt": R.VOLPATH,
+                "whitted": R.WHITTED, "normal": R.NORMAL, "depth": R.DEPTH, "ao": R.AO}
+
+
 def load_config(path: str) -> dict:
     c = read_json(path)
     integ = c.get("integrator", {})
     typ = integ.get("type", "pathtracing")
-    if typ not in ("pathtracing", "volumepath"):
-        raise ValueError(f"integrator {typ!r} has no device path (pathtracing, volumepath)")
+    if typ not in _INTEGRATORS:
+        raise ValueError(f"integrator {typ!r} has no device path ({', '.join(sorted(_INTEGRATORS))})")
     cam = c.get("camera", {})
     if cam.get("type", "perspective") != "perspective" or float(cam.get("aperture", 0.0)) > 0.0:
         raise ValueError("only the pinhole perspective camera has a device path")
     img = c.get("image", {})
     out = {
-        "integrator": R.PATH if typ == "pathtracing" else R.VOLPATH,
+        "integrator": _INTEGRATORS[typ],
         "depth": int(integ.get("maxDepth", 10)),
         "spp": int(integ.get("samplesPerPixel", 128)),
         "nee": bool(integ.get("nee", True)),
@@ -191,6 +197,7 @@ def load_config(path: str) -> dict:
                    "up": cam.get("up", [0.0, 1.0, 0.0]), "fov": cam.get("fov", 60.0)},
         "scene": c.get("scene", ""),
         "background": c.get("background"),
+        "ambient": tuple(float(a) for a in c.get("ambient", (0.1, 0.1, 0.1))),
     }
     return out
 
@@ -212,7 +219,7 @@ def render_config(ctx: R.Context, config_path: str, scene_path: str | None = Non
     desc = scene_from_dict(d, os.path.dirname(os.path.abspath(sp)), cfg["width"], cfg["height"], cfg["camera"])
     sc = R.scene_from_desc(ctx, desc)
     film = R.render(ctx, [dict(scene=sc, spp=cfg["spp"], depth=cfg["depth"], seed=seed, integrator=cfg["integrator"],
-                               nee=cfg["nee"], mis=cfg["mis"])])[0]
+                               nee=cfg["nee"], mis=cfg["mis"], ambient=cfg["ambient"])])[0]
     sc.close()
     if write:
         out = cfg["output"]

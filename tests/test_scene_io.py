@@ -157,9 +157,13 @@ def test_config_mapping(tmp_path):
     (tmp_path / "d.json").write_text(bad)
     with pytest.raises(ValueError):
         scene_io.load_config(str(tmp_path / "d.json"))
-    (tmp_path / "e.json").write_text(CONFIG_TEXT.replace('"pathtracing",', '"whitted",'))
+    (tmp_path / "e.json").write_text(CONFIG_TEXT.replace('"pathtracing",', '"simple",'))
     with pytest.raises(ValueError):
         scene_io.load_config(str(tmp_path / "e.json"))
+    (tmp_path / "f.json").write_text(CONFIG_TEXT.replace('"pathtracing",', '"whitted",').replace(
+        '"scene": "cornell.json",', '"scene": "cornell.json", "ambient": [0.0, 0.0, 0.0],'))
+    w = scene_io.load_config(str(tmp_path / "f.json"))
+    assert w["integrator"] == R.WHITTED and w["ambient"] == (0.0, 0.0, 0.0) and cfg["ambient"] == (0.1, 0.1, 0.1)
 
 
 @pytest.mark.gpu
