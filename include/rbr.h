@@ -212,10 +212,14 @@ int rbr_scene_set_tri_normals(rbr_scene* scene, const float* normals, size_t ntr
 int rbr_triangle_shading_normal(const float* verts, const float* normals, const float* o, const float* d, float* out);
 
 /* Builds the wide BVH (width 4 or 8) over the bounded primitives, leaves of at most
- * leaf_max primitives (1..8), and uploads it.  width | RBR_BVH_QUANTIZED stores the device
- * nodes with 8-bit child boxes relative to each node (64 B per 4-wide node, 128 B per 8-wide
- * node instead of 128 / 256 B); the boxes are rounded outward, so hits are unchanged. */
+ * leaf_max primitives (1..8), and uploads it (replaces BVH::build, bvh.cpp:10-31, 47-209).
+ * The binary SAH tree is collapsed to `width` by the SAH-optimal dynamic program, which also
+ * chooses the leaf sizes; width | RBR_BVH_GREEDY collapses it greedily instead (largest-area
+ * child opened first, leaves fixed by the binary build).  width | RBR_BVH_QUANTIZED stores the
+ * device nodes with 8-bit child boxes relative to each node (64 B per 4-wide node, 128 B per
+ * 8-wide node instead of 128 / 256 B); the boxes are rounded outward, so hits are unchanged. */
 #define RBR_BVH_QUANTIZED 0x100
+#define RBR_BVH_GREEDY 0x200
 int rbr_scene_build_bvh(rbr_scene* scene, int width, int leaf_max);
 /* The same layouts built on the GPU (no reference counterpart; the reference builds serially
  * on the host, bvh.cpp:47-209): Morton-ordered radix tree (LBVH), collapsed to `width` with the
@@ -254,7 +258,9 @@ int rbr_time_trace(rbr_scene* scene, int kind, const rbr_ray* d_rays, size_t n, 
  * 64-ray batches from a device counter (default), 1 = one ray per lane, 2 / 3 / 4 =
  * persistent waves whose idle lanes take new rays once 8 / 16 / 32 of them are idle,
  * 5 / 6 / 7 / 8 = persistent waves that advance every lane by one node and one primitive per
- * iteration and refill once 4 / 8 / 16 / 32 lanes are idle; lds_depth = LDS
+ * iteration and refill once 4 / 8 / 16 / 32 lanes are idle, 9 / 10 / 11 / 12 = the same with
+ * the primitive step deferred until 32 lanes hold a leaf or fewer than 48 lanes have a node
+ * to test; lds_depth = LDS
  * stack entries per lane (8, 12, 16, 24, 32; deeper entries spill to a per-wave global
  * buffer); waves_per_cu = persistent waves launched per compute unit (1..32). */
 int rbr_set_trace_tuning(rbr_ctx* ctx, int mode, int lds_depth, int waves_per_cu);

@@ -99,8 +99,11 @@ struct WideBVH {
 void quantizeWideBVH(WideBVH& b);
 
 // Builds the binary binned-SAH tree and collapses it to `width` (4 or 8); leaves hold at
-// most leaf_max (1..8) primitives.
-void buildWideBVH(const HostGeometry& g, int width, int leafMax, WideBVH& out);
+// most leaf_max (1..8) primitives.  optimal: the SAH-optimal collapse with one primitive test
+// costing primCost node visits; else the greedy largest-area collapse.
+constexpr float kCollapsePrimCost = 0.8f;
+void buildWideBVH(const HostGeometry& g, int width, int leafMax, WideBVH& out, bool optimal = true,
+                  float primCost = kCollapsePrimCost);
 
 // Host traversal of the same flat layout (the per-ray path of the Accelerator adapter and
 // the node/primitive visit counter).  Same acceptance rule as the kernels.

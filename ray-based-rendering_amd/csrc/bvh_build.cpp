@@ -1,6 +1,7 @@
 // Host construction of the wide BVH: binned SAH over all three axes (32 bins), built in
-// parallel with OpenMP tasks, then collapsed greedily (open the largest-area interior
-// child until the node has `width` children).  Replaces BVH::build / buildRecursive /
+// parallel with OpenMP tasks, then collapsed to `width` children per node, either by the
+// SAH-optimal dynamic program (default) or greedily (open the largest-area interior child
+// until the node has `width` children).  Replaces BVH::build / buildRecursive /
 // findBestSplit (bvh.cpp:10-31, 47-209): same input (every bounded primitive, id = caller's
 // index), different tree; results do not depend on the tree because every traversal
 // returns the lowest-id closest hit.
@@ -220,6 +221,136 @@ struct Collapser {
     }
 };
 
+// SAH-optimal collapse of the binary tree into `width`-wide nodes (after Ylitie, Karras &
+// Laine 2017, "Efficient incoherent ray traversal on GPUs through compressed wide BVHs").
+// Bottom-up over the binary nodes, cost(n, i) is the least SAH cost of the binary subtree n
+// when it may fill at most i child slots of its wide parent.  A slot is either a leaf (any
+// binary subtree of at most leafMax primitives: its primitives are contiguous) or a wide node
+// whose slots are drawn from n's two children:
+//   cost(n, 1) = min(leaf(n), A(n) c_node + dist(n, W)),   leaf(n) = A(n) count(n) c_prim
+//   cost(n, i) = min(dist(n, i), cost(n, i - 1)),           dist(n, j) = min_k cost(l, k) + cost(r, j - k)
+// so leaf sizes and which binary levels become wide nodes are chosen together.  The greedy
+// collapse above fixes the leaves first and fills each node by area alone.
+struct OptimalCollapser {
+    static constexpr int8_t kLeaf = -1, kWide = -2, kFewer = -3, kDist = 0;
+    const std::vector<BNode>& bn;
+    WideBVH& out;
+    int W, leafMax;
+    float pad, cNode, cPrim;
+    std::vector<float> cost;     // [node * W + i - 1]
+    std::vector<int8_t> how;     // same index: kLeaf / kWide (i = 1), kDist / kFewer (i > 1)
+    std::vector<int8_t> split;   // [node * W + j - 1]: slots given to the left child in dist(n, j)
+
+    void solve() {
+        const size_t n = bn.size();
+        cost.assign(n * W, INFINITY);
+        how.assign(n * W, kFewer);
+        split.assign(n * W, 1);
+        // Children are allocated after their parent, so descending indices visit them first.
+        for (size_t m = n; m-- > 0;) {
+            const BNode& b = bn[m];
+            const float a = area(b.lo, b.hi);
+            const float leafC = b.count <= uint32_t(leafMax) ? a * float(b.count) * cPrim : INFINITY;
+            float* c = &cost[m * W];
+            int8_t* h = &how[m * W];
+            if (b.leaf()) {
+                c[0] = leafC;
+                h[0] = kLeaf;
+                for (int i = 1; i < W; ++i) c[i] = c[0];
+                continue;
+            }
+            const float* cl = &cost[size_t(b.left) * W];
+            const float* cr = &cost[size_t(b.right) * W];
+            float dist[9];
+            for (int j = 2; j <= W; ++j) {
+                float best = INFINITY;
+                int bk = 1;
+                for (int k = 1; k < j; ++k) {
+                    const float v = cl[k - 1] + cr[j - k - 1];
+                    if (v < best) { best = v; bk = k; }
+                }
+                dist[j] = best;
+                split[m * W + j - 1] = int8_t(bk);
+            }
+            const float wideC = a * cNode + dist[W];
+            if (leafC <= wideC) { c[0] = leafC; h[0] = kLeaf; }
+            else { c[0] = wideC; h[0] = kWide; }
+            for (int i = 2; i <= W; ++i) {
+                if (dist[i] < c[i - 2]) { c[i - 1] = dist[i]; h[i - 1] = kDist; }
+                else { c[i - 1] = c[i - 2]; h[i - 1] = kFewer; }
+            }
+        }
+    }
+
+    // Slots of binary subtree m given at most i of them: (node, is leaf).
+    void gather(int32_t m, int i, int32_t* kids, bool* leaf, int& nk) const {
+        while (how[size_t(m) * W + i - 1] == kFewer) --i;
+        const int8_t h = how[size_t(m) * W + i - 1];
+        if (i == 1) {
+            kids[nk] = m;
+            leaf[nk++] = h == kLeaf;
+            return;
+        }
+        const int k = split[size_t(m) * W + i - 1];
+        gather(bn[m].left, k, kids, leaf, nk);
+        gather(bn[m].right, i - k, kids, leaf, nk);
+    }
+
+    void padBox(const float lo[3], const float hi[3], float plo[3], float phi[3]) const {
+        for (int k = 0; k < 3; ++k) {
+            plo[k] = lo[k] - (pad + std::fabs(lo[k]) * 1e-6f);
+            phi[k] = hi[k] + (pad + std::fabs(hi[k]) * 1e-6f);
+        }
+    }
+
+    // Emits the wide node whose slots are drawn from binary node b and returns its index;
+    // a root that is best kept as one leaf becomes a node with that single leaf.
+    uint32_t emit(int32_t b, uint32_t depth) {
+        out.maxDepth = std::max(out.maxDepth, depth);
+        int32_t kids[8];
+        bool leaf[8];
+        int nk = 0;
+        if (bn[b].leaf() || how[size_t(b) * W] == kLeaf) {
+            kids[nk] = b;
+            leaf[nk++] = true;
+        } else {
+            const int k = split[size_t(b) * W + W - 1];
+            gather(bn[b].left, k, kids, leaf, nk);
+            gather(bn[b].right, W - k, kids, leaf, nk);
+        }
+        const uint32_t idx = out.nodeCount++;
+        const int per = out.f4PerNode();
+        out.nodes.resize(size_t(out.nodeCount) * per);
+        uint32_t child[8];
+        float blo[8][3], bhi[8][3];
+        for (int i = 0; i < W; ++i) {
+            child[i] = kEmpty;
+            for (int k = 0; k < 3; ++k) { blo[i][k] = INFINITY; bhi[i][k] = -INFINITY; }
+        }
+        for (int i = 0; i < nk; ++i) {
+            const BNode& c = bn[kids[i]];
+            padBox(c.lo, c.hi, blo[i], bhi[i]);
+            if (leaf[i]) {
+                child[i] = makeLeaf(c.start, c.count);
+                out.leafCount++;
+            } else {
+                child[i] = emit(kids[i], depth + 1);
+            }
+        }
+        float* f = reinterpret_cast<float*>(&out.nodes[size_t(idx) * per]);
+        for (int i = 0; i < W; ++i) {
+            f[0 * W + i] = blo[i][0];
+            f[1 * W + i] = bhi[i][0];
+            f[2 * W + i] = blo[i][1];
+            f[3 * W + i] = bhi[i][1];
+            f[4 * W + i] = blo[i][2];
+            f[5 * W + i] = bhi[i][2];
+            std::memcpy(&f[6 * W + i], &child[i], 4);
+        }
+        return idx;
+    }
+};
+
 inline void minmax3(const float* a, const float* b, const float* c, float lo[3], float hi[3]) {
     for (int k = 0; k < 3; ++k) {
         lo[k] = std::min(std::min(a[k], b[k]), c[k]);
@@ -229,7 +360,7 @@ inline void minmax3(const float* a, const float* b, const float* c, float lo[3],
 
 }  // namespace
 
-void buildWideBVH(const HostGeometry& g, int width, int leafMax, WideBVH& out) {
+void buildWideBVH(const HostGeometry& g, int width, int leafMax, WideBVH& out, bool optimal, float primCost) {
     auto t0 = std::chrono::steady_clock::now();
     out = WideBVH();
     out.width = width;
@@ -276,7 +407,9 @@ void buildWideBVH(const HostGeometry& g, int width, int leafMax, WideBVH& out) {
 
     std::vector<BNode> bn(size_t(2) * n + 1);
     std::atomic<int32_t> next{1};
-    Builder b{p, bn, next, std::max(1, std::min(leafMax, int(kMaxLeaf)))};
+    // The optimal collapse picks the leaves itself, so its binary tree goes down to single
+    // primitives.
+    Builder b{p, bn, next, optimal ? 1 : std::max(1, std::min(leafMax, int(kMaxLeaf)))};
 #pragma omp parallel
 #pragma omp single
     b.build(0, 0, n, 0);
@@ -289,9 +422,16 @@ void buildWideBVH(const HostGeometry& g, int width, int leafMax, WideBVH& out) {
     out.sahCost = sah;
 
     const float ext = std::max({whi[0] - wlo[0], whi[1] - wlo[1], whi[2] - wlo[2]});
-    Collapser c{bn, out, width, 1e-5f * ext + 1e-6f};
     out.nodes.reserve(size_t(n / 2 + 8) * out.f4PerNode());
-    c.emit(0, 1);
+    if (optimal) {
+        OptimalCollapser c{bn, out, width, std::max(1, std::min(leafMax, int(kMaxLeaf))), 1e-5f * ext + 1e-6f, 1.0f,
+                           primCost, {}, {}, {}};
+        c.solve();
+        c.emit(0, 1);
+    } else {
+        Collapser c{bn, out, width, 1e-5f * ext + 1e-6f};
+        c.emit(0, 1);
+    }
 
     // Leaf-ordered primitive records.
     out.prims.resize(size_t(n) * 3);

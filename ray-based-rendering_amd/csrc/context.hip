@@ -110,11 +110,14 @@ int rbr::launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack
     if (int rc = ensureBuffer(ctx, ctx->dCounter, kCounterBytes)) return rc;
     if (int rc = ensureBuffer(ctx, ctx->dSpill, size_t(grid) * spillDepth * 64 * sizeof(uint2))) return rc;
     // modes 2, 3, 4: per-lane fetch once 8, 16, 32 lanes of a wave are idle (while-while rounds);
-    // modes 5, 6, 7, 8: per-lane fetch once 4, 8, 16, 32 lanes are idle (single steps)
+    // modes 5, 6, 7, 8: per-lane fetch once 4, 8, 16, 32 lanes are idle (single steps);
+    // modes 9, 10, 11, 12: as 5-8 with primitive steps deferred until 32 lanes hold a leaf or
+    // fewer than 48 lanes have an interior node
     const int m = ctx->traceMode;
-    const uint32_t refill = m >= 5 ? (2u << (m - 4)) : m >= 2 ? (4u << (m - 1)) : 0u;
+    const int ms = m >= 9 ? m - 4 : m;
+    const uint32_t refill = ms >= 5 ? (2u << (ms - 4)) : ms >= 2 ? (4u << (ms - 1)) : 0u;
     PersistArgs p{static_cast<uint32_t*>(ctx->dCounter.ptr), static_cast<uint2*>(ctx->dSpill.ptr), spillDepth, refill,
-                  m >= 5 ? 1u : 0u};
+                  ms >= 5 ? 1u : 0u, m >= 9 ? kDeferPrimMin : 0u, kDeferNodeMin};
     e = launchTrace(a, width, any, ctx->ldsDepth, &p, grid, ctx->stream);
     if (e != hipSuccess) return failHip(ctx, e, "trace launch");
     return RBR_OK;
@@ -247,8 +250,8 @@ int rbr_calib_copy(rbr_ctx* ctx, void* dst, const void* src, size_t bytes) {
 
 int rbr_set_trace_tuning(rbr_ctx* ctx, int mode, int lds_depth, int waves_per_cu) {
     if (!ctx) return RBR_ERR_INVALID;
-    if (mode < 0 || mode > 8)
-        return fail(ctx, RBR_ERR_INVALID, "mode must be 0 (persistent), 1 (lane), 2-4 (per-lane fetch) or 5-8 (steps)");
+    if (mode < 0 || mode > 12)
+        return fail(ctx, RBR_ERR_INVALID, "mode must be 0 (persistent), 1 (lane), 2-4 (per-lane fetch), 5-8 (steps) or 9-12 (deferred steps)");
     if (lds_depth != 8 && lds_depth != 12 && lds_depth != 16 && lds_depth != 24 && lds_depth != 32)
         return fail(ctx, RBR_ERR_INVALID, "lds_depth must be 8, 12, 16, 24 or 32");
     if (waves_per_cu < 1 || waves_per_cu > 32) return fail(ctx, RBR_ERR_INVALID, "waves_per_cu must be in 1..32");
@@ -354,13 +357,13 @@ int rbr_triangle_shading_normal(const float* verts, const float* normals, const 
 int rbr_scene_build_bvh(rbr_scene* s, int width, int leaf_max) {
     if (!s) return RBR_ERR_INVALID;
     rbr_ctx* ctx = s->ctx;
-    const int W = width & ~kQuantFlag;
-    if ((W != 4 && W != 8) || (width & ~(kQuantFlag | 0xFF)))
-        return fail(ctx, RBR_ERR_INVALID, "BVH width must be 4 or 8, optionally | RBR_BVH_QUANTIZED");
+    const int W = width & 0xFF;
+    if ((W != 4 && W != 8) || (width & ~(RBR_BVH_QUANTIZED | RBR_BVH_GREEDY | 0xFF)))
+        return fail(ctx, RBR_ERR_INVALID, "BVH width must be 4 or 8, optionally | RBR_BVH_QUANTIZED | RBR_BVH_GREEDY");
     if (leaf_max < 1 || leaf_max > 8) return fail(ctx, RBR_ERR_INVALID, "leaf_max must be in 1..8");
     s->built = false;
     bumpVersion(s);
-    buildWideBVH(s->geom, W, leaf_max, s->bvh);
+    buildWideBVH(s->geom, W, leaf_max, s->bvh, (width & RBR_BVH_GREEDY) == 0);
     if (width & kQuantFlag) quantizeWideBVH(s->bvh);
     if (!s->bvh.empty() && uint64_t(W - 1) * s->bvh.maxDepth + 1 > uint64_t(kMaxStack))
         return fail(ctx, RBR_ERR_UNSUPPORTED, "BVH too deep for the traversal stack");

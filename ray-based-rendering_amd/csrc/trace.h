@@ -40,12 +40,19 @@ constexpr uint32_t kSlices = 16;
 constexpr uint32_t kSliceStride = 64;
 constexpr size_t kCounterBytes = size_t(kSlices) * kSliceStride * sizeof(uint32_t);
 
+// Step kernel with deferred primitive steps (trace modes 9-12): primitives are tested once this
+// many lanes of a wave hold a leaf, or once fewer than kDeferNodeMin lanes have a node to test.
+constexpr uint32_t kDeferPrimMin = 32;
+constexpr uint32_t kDeferNodeMin = 48;
+
 struct PersistArgs {
     uint32_t* counter;        // kSlices batch counters
     uint2* spill;             // [grid][spillDepth][64]
     uint32_t spillDepth;
     uint32_t refill;          // 0: waves take 64-ray batches; else per-lane fetch once this many lanes idle
     uint32_t step;            // with refill: single node / primitive steps instead of while-while rounds
+    uint32_t primMin;         // step kernel: 0 tests primitives every iteration; else only once this many
+    uint32_t nodeMin;         //   lanes hold a leaf or fewer than nodeMin lanes have an interior node
 };
 
 // ldsDepth in {8, 12, 16, 24, 32}; p == nullptr selects the one-ray-per-lane kernel (which

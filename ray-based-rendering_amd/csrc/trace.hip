@@ -150,15 +150,18 @@ __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArg
 // lane with an interior node to visit tests that node's children, and every lane with a leaf
 // in progress tests one primitive of it, so no lane waits for the others to reach a leaf (the
 // while-while rounds above make a lane wait until every lane holds one) and a lane whose ray
-// is done is handed a new ray as soon as p.refill lanes are idle.  The node fetch is issued at
-// the top of the iteration, before the primitive step, so the two loads of an iteration are
-// in flight together.  A lane holds at most one leaf in progress; a leaf met while one is in
-// progress stays in ref until the current one is finished.  Every entry that can still hold a
+// is done is handed a new ray as soon as p.refill lanes are idle.  The two steps run one
+// after the other (overlapping their loads doubled the registers held across the iteration).
+// A lane holds at most one leaf in progress; a leaf met while one is in progress stays in ref
+// until the current one is finished.  With DEFER the primitive step is skipped in iterations
+// where fewer than p.primMin lanes hold a leaf and at least p.nodeMin lanes have an interior
+// node, so the primitive tests run with more lanes at once; a ray waits for its leaves, never
+// the other way round, so every wave still drains.  Every entry that can still hold a
 // closer hit is visited and every primitive of every visited leaf is tested, so the hits are
 // those of the other kernels.  Rays come from 64-ray ranges reserved with nextBatch, as in
 // traceKernelDynamic, and every wave exits once every slice is drained and all of its lanes
 // are idle.
-template <int W, bool Q, int LDS, bool ANY, bool MIXED>
+template <int W, bool Q, int LDS, bool ANY, bool MIXED, bool DEFER>
 __global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p) {
     __shared__ uint2 ldsStack[LDS * 64];
     GlobalSpillStack<LDS> st;
@@ -206,8 +209,15 @@ __global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p
             if (!more) break;
             continue;
         }
-        // One primitive of the leaf in progress, then one interior node.
-        if (primLeft != 0u) {
+        // One primitive of the leaf in progress (DEFER: only once primMin lanes hold one, or
+        // fewer than nodeMin lanes have an interior node to test), then one interior node.
+        bool doPrim = true;
+        if (DEFER) {
+            const uint32_t nPrim = uint32_t(__popcll(__ballot(primLeft != 0u ? 1 : 0)));
+            const uint32_t nNode = uint32_t(__popcll(__ballot(alive && ref != kEmptyD && !isLeafD(ref) ? 1 : 0)));
+            doPrim = nPrim >= p.primMin || nNode < p.nodeMin;
+        }
+        if (doPrim && primLeft != 0u) {
             float t;
             uint32_t id;
             if (testPrim<MIXED>(a, a.prims + size_t(prim) * 3, r, bestT, t, id)) {
@@ -219,24 +229,31 @@ __global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p
             if (ANY && occ) {
                 primLeft = 0u;
                 ref = kEmptyD;
-            } else if (primLeft == 0u && ref != kEmptyD && isLeafD(ref)) {
-                prim = leafStartD(ref);
-                primLeft = leafCountD(ref);
-                ref = popNext<ANY>(st, bestT);
+                st.sp = 0;
             }
         }
         if (alive && ref != kEmptyD && !isLeafD(ref)) {
             float tc[W];
             uint32_t rc[W];
+            bool hit[W];
             NodeRows<W, Q> nr;
             loadNode<W, Q>(a.nodes, ref, nr);
-            children<W, Q>(nr, r, bestT, tc, rc);
-            ref = descend<W, ANY>(tc, rc, st);
-            if (ref == kEmptyD) ref = popNext<ANY>(st, bestT);
-            if (ref != kEmptyD && isLeafD(ref) && primLeft == 0u) {
-                prim = leafStartD(ref);
-                primLeft = leafCountD(ref);
-                ref = popNext<ANY>(st, bestT);
+            children<W, Q>(nr, r, bestT, tc, rc, hit);
+            ref = descend<W, ANY>(tc, rc, hit, st);
+        }
+        // Settle: a leaf in ref starts when no leaf is in progress; an empty ref is refilled
+        // from the stack (entries farther than the closest hit are dropped).  One loop serves
+        // both, so a wave pays for the longest chain of pops of one lane, once per iteration.
+        if (alive) {
+            while (true) {
+                if (primLeft == 0u && ref != kEmptyD && isLeafD(ref)) {
+                    prim = leafStartD(ref);
+                    primLeft = leafCountD(ref);
+                    ref = kEmptyD;
+                }
+                if (ref != kEmptyD || st.sp == 0) break;
+                const uint2 e = st.pop();
+                if (ANY || u2f(e.y) <= bestT) ref = e.x;
             }
         }
         if (alive && ref == kEmptyD && primLeft == 0u) {
@@ -272,8 +289,13 @@ template <int W, bool Q, int LDS, bool ANY>
 hipError_t launchT(const TraceArgs& a, const PersistArgs* p, unsigned grid, hipStream_t s) {
     if (p && p->refill && p->step) {
         hipLaunchKernelGGL(resetCounterKernel, dim3(1), dim3(64), 0, s, p->counter);
-        if (a.mixed) hipLaunchKernelGGL((traceKernelStep<W, Q, LDS, ANY, true>), dim3(grid), dim3(64), 0, s, a, *p);
-        else hipLaunchKernelGGL((traceKernelStep<W, Q, LDS, ANY, false>), dim3(grid), dim3(64), 0, s, a, *p);
+        if (p->primMin) {
+            if (a.mixed) hipLaunchKernelGGL((traceKernelStep<W, Q, LDS, ANY, true, true>), dim3(grid), dim3(64), 0, s, a, *p);
+            else hipLaunchKernelGGL((traceKernelStep<W, Q, LDS, ANY, false, true>), dim3(grid), dim3(64), 0, s, a, *p);
+        } else {
+            if (a.mixed) hipLaunchKernelGGL((traceKernelStep<W, Q, LDS, ANY, true, false>), dim3(grid), dim3(64), 0, s, a, *p);
+            else hipLaunchKernelGGL((traceKernelStep<W, Q, LDS, ANY, false, false>), dim3(grid), dim3(64), 0, s, a, *p);
+        }
     } else if (p && p->refill) {
         hipLaunchKernelGGL(resetCounterKernel, dim3(1), dim3(64), 0, s, p->counter);
         if (a.mixed) hipLaunchKernelGGL((traceKernelDynamic<W, Q, LDS, ANY, true>), dim3(grid), dim3(64), 0, s, a, *p);

@@ -102,10 +102,10 @@ __device__ __forceinline__ void loadNode(const float4* nodes, uint32_t ref, Node
 
 // Children of a quantized node (bvh.h): the ray is moved into the node's grid once,
 // t(q) = q * (s * 1/d) + (p - o) * 1/d, so each of the 6 planes of a child costs one FMA on
-// its byte.  Fills entry distances (tc) and refs (rc) like the float path.
+// its byte.  Fills entry distances (tc), refs (rc) and hit flags like the float path.
 template <int W>
 __device__ __forceinline__ void childrenQ(const NodeRows<W, true>& nr, const DRay& r, float tfar, float* tc,
-                                          uint32_t* rc) {
+                                          uint32_t* rc, bool* hit) {
     constexpr int Q4 = W / 4;
     constexpr int NV = NodeRows<W, true>::kRows;
     uint32_t w[4 * NV];
@@ -147,14 +147,15 @@ __device__ __forceinline__ void childrenQ(const NodeRows<W, true>& nr, const DRa
         const uint32_t ch = w[4 + 6 * Q4 + k];
         const bool h = t0 <= t1 && ch != kEmptyD;
         tc[k] = h ? t0 : INFINITY;
-        rc[k] = h ? ch : kEmptyD;
+        rc[k] = ch;
+        hit[k] = h;
     }
 }
 
 // Children of a float node (bvh.h), same outputs as childrenQ.
 template <int W>
 __device__ __forceinline__ void childrenF(const NodeRows<W, false>& nr, const DRay& r, float tfar, float* tc,
-                                          uint32_t* rc) {
+                                          uint32_t* rc, bool* hit) {
     constexpr int q = W / 4;
 #pragma unroll
     for (int half = 0; half < q; ++half) {
@@ -168,17 +169,21 @@ __device__ __forceinline__ void childrenF(const NodeRows<W, false>& nr, const DR
         for (int k = 0; k < 4; ++k) {
             bool h;
             float t = slab(r, LX[k], HX[k], LY[k], HY[k], LZ[k], HZ[k], tfar, h);
+            h = h && CH[k] != kEmptyD;
             tc[half * 4 + k] = h ? t : INFINITY;
-            rc[half * 4 + k] = h ? CH[k] : kEmptyD;
+            rc[half * 4 + k] = CH[k];
+            hit[half * 4 + k] = h;
         }
     }
 }
 
+// Entry distance (+inf when missed), ref (as stored, also for a missed child) and hit flag of
+// every child slot; empty slots never hit.
 template <int W, bool Q>
 __device__ __forceinline__ void children(const NodeRows<W, Q>& nr, const DRay& r, float tfar, float* tc,
-                                         uint32_t* rc) {
-    if constexpr (Q) childrenQ<W>(nr, r, tfar, tc, rc);
-    else childrenF<W>(nr, r, tfar, tc, rc);
+                                         uint32_t* rc, bool* hit) {
+    if constexpr (Q) childrenQ<W>(nr, r, tfar, tc, rc, hit);
+    else childrenF<W>(nr, r, tfar, tc, rc, hit);
 }
 
 // One primitive record (3 rows: v0 / centre / corner + id, e1 / radius + kind, e2 + quad
@@ -274,11 +279,16 @@ __device__ __forceinline__ uint32_t popNext(Stack& st, float bestT) {
     return kEmptyD;
 }
 
-// The hit children of a node (entry distances tc, refs rc; misses are +inf / kEmptyD): for
+// The hit children of a node (entry distances tc, +inf when missed; refs rc; hit flags): for
 // closest hit they are ordered nearest first and all but the nearest are pushed; returns the
-// child to visit next, or kEmptyD when no child is hit.
+// child to visit next, or kEmptyD when no child is hit.  The hits are counted before the
+// sort, which moves them (finite distances) ahead of the misses.
 template <int W, bool ANY, typename Stack>
-__device__ __forceinline__ uint32_t descend(float* tc, uint32_t* rc, Stack& st) {
+__device__ __forceinline__ uint32_t descend(float* tc, uint32_t* rc, const bool* hit, Stack& st) {
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < W; ++k) cnt += hit[k] ? 1 : 0;
+    if (cnt == 0) return kEmptyD;
     if (!ANY) {
         // Sorting network, ascending entry distance (misses sort last as +inf).
         if (W == 4) {
@@ -295,15 +305,11 @@ __device__ __forceinline__ uint32_t descend(float* tc, uint32_t* rc, Stack& st) 
                 cswap(tc[net[c][0]], rc[net[c][0]], tc[net[c][1]], rc[net[c][1]]);
         }
     }
-    int cnt = 0;
-#pragma unroll
-    for (int k = 0; k < W; ++k) cnt += rc[k] != kEmptyD ? 1 : 0;
-    if (cnt == 0) return kEmptyD;
     if (ANY) {
         uint32_t first = kEmptyD;
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            if (rc[k] == kEmptyD) continue;
+            if (!hit[k]) continue;
             if (first == kEmptyD) first = rc[k];
             else st.push(rc[k], tc[k]);
         }
@@ -329,10 +335,11 @@ __device__ __forceinline__ void traceRound(const TraceArgs& a, const DRay& r, St
     while (ref != kEmptyD && !isLeafD(ref)) {
         float tc[W];
         uint32_t rc[W];
+        bool hit[W];
         NodeRows<W, Q> nr;
         loadNode<W, Q>(a.nodes, ref, nr);
-        children<W, Q>(nr, r, bestT, tc, rc);
-        ref = descend<W, ANY>(tc, rc, st);
+        children<W, Q>(nr, r, bestT, tc, rc, hit);
+        ref = descend<W, ANY>(tc, rc, hit, st);
         if (ref == kEmptyD) ref = popNext<ANY>(st, bestT);
         if (ref != kEmptyD && isLeafD(ref) && leaf == kEmptyD) {
             leaf = ref;

@@ -157,7 +157,8 @@ _ORACLE: dict = {}
 
 @pytest.mark.parametrize("mode,depth,waves", [(0, 8, 1), (0, 12, 32), (0, 32, 16), (1, 8, 16), (1, 24, 16),
                                                (2, 8, 32), (3, 8, 1), (3, 12, 16), (4, 32, 16),
-                                               (5, 8, 32), (6, 8, 1), (6, 12, 16), (7, 32, 16), (8, 8, 24)])
+                                               (5, 8, 32), (6, 8, 1), (6, 12, 16), (7, 32, 16), (8, 8, 24),
+                                               (9, 8, 32), (10, 12, 1), (11, 8, 32), (12, 32, 16)])
 def test_launch_tunings_identical(ctx, mode, depth, waves):
     """Every launch configuration (persistent waves with tiny/large LDS stacks and so
     deep global spills, one ray per lane, persistent waves whose idle lanes fetch new rays,
@@ -179,10 +180,11 @@ def test_launch_tunings_identical(ctx, mode, depth, waves):
         ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
 
 
-@pytest.mark.parametrize("mode", [5, 8])
+@pytest.mark.parametrize("mode", [5, 8, 9, 11])
 def test_step_modes_mixed_primitives(ctx, mode):
-    """The single-step kernel on a world of spheres, triangles and an unbounded plane, for
-    every layout: the same hits as brute force (closest with lowest-id ties, and any hit)."""
+    """The single-step kernel (primitive steps every iteration or deferred) on a world of
+    spheres, triangles and an unbounded plane, for every layout: the same hits as brute force
+    (closest with lowest-id ties, and any hit)."""
     say(f"case: step mode {mode} mixed world")
     rng = np.random.default_rng(500 + mode)
     w = worlds.make_world(rng, 60, 600, True)
