@@ -83,7 +83,7 @@ RBR_HD inline void quantizeNode(const float* f, int W, uint32_t* q) {
         ref[k] = u;
         live[k] = u != 0xFFFFFFFFu;
     }
-    uint32_t exps = 0;
+    uint32_t exps = 0, scales[3];
     for (int ax = 0; ax < 3; ++ax) {
         float p;
         int eb;
@@ -93,12 +93,19 @@ RBR_HD inline void quantizeNode(const float* f, int W, uint32_t* q) {
         __builtin_memcpy(&pu, &p, 4);
         q[ax] = pu;
         exps |= uint32_t(eb) << (8 * ax);
+        scales[ax] = uint32_t(eb) << 23;   // 2^(eb - 127) as a float (eb in 1..254)
         for (int k = 0; k < W; ++k) {
             q[4 + (2 * ax) * q4 + (k >> 2)] |= uint32_t(qlo[k]) << (8 * (k & 3));
             q[4 + (2 * ax + 1) * q4 + (k >> 2)] |= uint32_t(qhi[k]) << (8 * (k & 3));
         }
     }
-    q[3] = exps;
+    if (W == 4) {
+        q[3] = scales[0];
+        q[14] = scales[1];
+        q[15] = scales[2];
+    } else {
+        q[3] = exps;
+    }
     for (int k = 0; k < W; ++k) q[4 + 6 * q4 + k] = ref[k];
 }
 

@@ -116,9 +116,13 @@ __device__ __forceinline__ void childrenQ(const NodeRows<W, true>& nr, const DRa
         w[4 * i + 2] = nr.v[i].z;
         w[4 * i + 3] = nr.v[i].w;
     }
-    const float ax = u2f((w[3] & 0xFFu) << 23) * r.ix;
-    const float ay = u2f(((w[3] >> 8) & 0xFFu) << 23) * r.iy;
-    const float az = u2f(((w[3] >> 16) & 0xFFu) << 23) * r.iz;
+    // per-axis scales: floats in dwords 3, 14, 15 (W = 4) or exponent bytes of dword 3 (W = 8)
+    const float sx = W == 4 ? u2f(w[3]) : u2f((w[3] & 0xFFu) << 23);
+    const float sy = W == 4 ? u2f(w[14]) : u2f(((w[3] >> 8) & 0xFFu) << 23);
+    const float sz = W == 4 ? u2f(w[15]) : u2f(((w[3] >> 16) & 0xFFu) << 23);
+    const float ax = sx * r.ix;
+    const float ay = sy * r.iy;
+    const float az = sz * r.iz;
     const float bx = __builtin_fmaf(u2f(w[0]), r.ix, -r.oix);
     const float by = __builtin_fmaf(u2f(w[1]), r.iy, -r.oiy);
     const float bz = __builtin_fmaf(u2f(w[2]), r.iz, -r.oiz);

@@ -88,7 +88,12 @@ def test_structure_and_quantization(ctx, width):
     live = refs != 0xFFFFFFFF
     p = nq[:, 0:3].view(np.float32)
     for ax in range(3):
-        e = ((nq[:, 3] >> (8 * ax)) & 0xFF).astype(np.int64) - 127
+        if width == 4:   # scales stored as floats (powers of two) in dwords 3, 14, 15
+            word = nq[:, (3, 14, 15)[ax]]
+            assert np.all(word & 0x807FFFFF == 0)
+            e = ((word >> 23) & 0xFF).astype(np.int64) - 127
+        else:            # biased exponent bytes in dword 3
+            e = ((nq[:, 3] >> (8 * ax)) & 0xFF).astype(np.int64) - 127
         s = np.ldexp(np.float32(1.0), e).astype(np.float32)
         for k in range(width):
             wlo = nq[:, 4 + (2 * ax) * q4 + (k >> 2)]

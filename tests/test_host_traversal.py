@@ -173,7 +173,11 @@ def test_quantized_nodes_contain_float_boxes(width):
     for n in range(len(Q)):
         q = Q[n]
         p = q[:3].view(np.float32)
-        e = [(int(q[3]) >> (8 * a)) & 0xFF for a in range(3)]
+        if width == 4:   # scales stored as floats (powers of two) in dwords 3, 14, 15
+            assert all(int(q[d]) & 0x807FFFFF == 0 for d in (3, 14, 15))
+            e = [(int(q[d]) >> 23) & 0xFF for d in (3, 14, 15)]
+        else:            # biased exponent bytes in dword 3
+            e = [(int(q[3]) >> (8 * a)) & 0xFF for a in range(3)]
         s = [np.float32(2.0 ** (ea - 127)) for ea in e]
         for k in range(width):
             ref_f = Fu[n, 6 * width + k]
