@@ -185,7 +185,7 @@ def main() -> int:
     ctx = R.Context(0)
     name, cus = ctx.device_info()
     if args.quick:
-        grid = dict(widths=[4, 8, 4 | R.BVH_QUANTIZED, 8 | R.BVH_QUANTIZED], leaves=[4], modes=[0, 2, 5, 6, 7, 8, 10, 11, 12],
+        grid = dict(widths=[4, 8, 4 | R.BVH_QUANTIZED, 8 | R.BVH_QUANTIZED], leaves=[4], modes=[0, 2, 6, 7, 8, 11, 12],
                     depths=[8, 12], waves=[16, 24, 32])
     else:
         grid = dict(widths=[4, 8, 4 | R.BVH_QUANTIZED, 8 | R.BVH_QUANTIZED], leaves=[2, 4, 8], modes=list(range(13)),
