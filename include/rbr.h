@@ -222,9 +222,9 @@ int rbr_triangle_shading_normal(const float* verts, const float* normals, const 
 #define RBR_BVH_GREEDY 0x200
 int rbr_scene_build_bvh(rbr_scene* scene, int width, int leaf_max);
 /* The same layouts built on the GPU (no reference counterpart; the reference builds serially
- * on the host, bvh.cpp:47-209): Morton-ordered radix tree (LBVH), collapsed to `width` with the
- * host's largest-area rule, leaves of up to leaf_max Morton-contiguous primitives, quantized by
- * the host's quantizer.  Builds in milliseconds where the host SAH build takes hundreds; the
+ * on the host, bvh.cpp:47-209): Morton-ordered radix tree (LBVH), collapsed to `width` by the
+ * host's SAH-optimal program (| RBR_BVH_GREEDY: the host's largest-area rule), leaves of up to
+ * leaf_max Morton-contiguous primitives, quantized by the host's quantizer.  Builds in milliseconds where the host SAH build takes hundreds; the
  * tree costs more node visits per ray.  Hits are the same (they do not depend on the tree).
  * Needs a GPU context; the host copies (host traversal, statistics) are filled as well. */
 int rbr_scene_build_bvh_device(rbr_scene* scene, int width, int leaf_max);

@@ -9,10 +9,12 @@
 //      equal codes are ordered by index
 //   4. internal boxes bottom-up: one thread per leaf climbs while it is the second child to
 //      arrive at a node (per-node arrival counter)
-//   5. collapse to W-wide nodes, one level per launch: a node takes its binary node's two
-//      children and keeps opening the largest-area child until it has W (the host collapse's
-//      rule); a subtree of at most leaf_max primitives becomes a leaf (its primitives are
-//      contiguous in Morton order); child boxes are padded as the host pads them
+//   5. collapse to W-wide nodes, one level per launch, by the host's SAH-optimal dynamic
+//      program (its cost tables are filled in step 4's climb): a node's slots and which of
+//      them are leaves (subtrees of at most leaf_max primitives, contiguous in Morton order)
+//      come from the tables; or, with RBR_BVH_GREEDY, a node keeps opening its largest-area
+//      child until it has W and every subtree of at most leaf_max primitives is a leaf; child
+//      boxes are padded as the host pads them
 //   6. leaf-ordered primitive records (bvh.h), and for quantized layouts the 8-bit nodes by
 //      the host's own quantizer (bvh_quant.h), so both produce the same bits from the same
 //      float nodes
@@ -209,15 +211,49 @@ __device__ __forceinline__ void childBox(uint32_t c, const uint32_t* vals, const
     }
 }
 
-// Internal boxes bottom-up.  The arrival counter is taken with acquire-release order at
-// device scope, the boxes are read and written with device-scope atomics, so the second
-// arrival at a node sees the box its sibling's thread wrote before arriving.
+__device__ __forceinline__ float boxArea(const float lo[3], const float hi[3]) {
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    if (dx < 0.0f || dy < 0.0f || dz < 0.0f) return 0.0f;
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+// SAH-optimal collapse (the host builder's dynamic program, bvh_build.cpp), evaluated in the
+// same bottom-up pass as the boxes.  Per internal node: cost[W] (least SAH cost of the
+// subtree when it may fill at most i slots of its wide parent) and one decision word:
+//   bit i-1 (i = 1): 1 = one leaf, 0 = one wide node;  (i > 1): 1 = split over both
+//   children, 0 = use at most i - 1 slots;  bits 8 + 3 (j - 2): slots of the left child
+//   when j slots are split.
+// A binary leaf (one primitive) costs A c_prim for any number of slots.
+struct DpArgs {
+    uint32_t W, leafMax;
+    float cPrim;
+    const uint32_t *first, *last;
+    float* cost;   // [internal node][W]
+    float* dec;    // [internal node]: decision word (bits in a float, stored like the boxes)
+};
+
+__device__ __forceinline__ void childCosts(uint32_t c, const uint32_t* vals, const float4* boxLo,
+                                           const float4* boxHi, const float* nodeBox, const DpArgs& dp, float* out) {
+    if (c & kLeafTag) {
+        float lo[3], hi[3];
+        childBox(c, vals, boxLo, boxHi, nodeBox, lo, hi);
+        const float v = boxArea(lo, hi) * dp.cPrim;
+        for (uint32_t i = 0; i < dp.W; ++i) out[i] = v;
+    } else {
+        for (uint32_t i = 0; i < dp.W; ++i) out[i] = ld(dp.cost + size_t(c) * dp.W + i);
+    }
+}
+
+// Internal boxes (and the collapse's cost tables) bottom-up.  The arrival counter is taken
+// with acquire-release order at device scope, boxes and tables are read and written with
+// device-scope atomics, so the second arrival at a node sees what its sibling's thread wrote
+// before arriving.
 __global__ __launch_bounds__(256) void gbRefitKernel(uint32_t n, const uint32_t* __restrict__ vals,
                                                      const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
                                                      const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
                                                      const uint32_t* __restrict__ parentInt,
                                                      const uint32_t* __restrict__ parentLeaf, uint32_t* arrivals,
-                                                     float* nodeBox) {
+                                                     float* nodeBox, DpArgs dp) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n || n < 2) return;
     uint32_t p = parentLeaf[i];
@@ -228,10 +264,40 @@ __global__ __launch_bounds__(256) void gbRefitKernel(uint32_t n, const uint32_t*
         childBox(left[p], vals, boxLo, boxHi, nodeBox, alo, ahi);
         childBox(right[p], vals, boxLo, boxHi, nodeBox, blo, bhi);
         float* q = nodeBox + 6 * size_t(p);
+        float lo[3], hi[3];
         for (int k = 0; k < 3; ++k) {
-            st(q + k, mn(alo[k], blo[k]));
-            st(q + 3 + k, mx(ahi[k], bhi[k]));
+            lo[k] = mn(alo[k], blo[k]);
+            hi[k] = mx(ahi[k], bhi[k]);
+            st(q + k, lo[k]);
+            st(q + 3 + k, hi[k]);
         }
+        float cl[8], cr[8], dist[9];
+        childCosts(left[p], vals, boxLo, boxHi, nodeBox, dp, cl);
+        childCosts(right[p], vals, boxLo, boxHi, nodeBox, dp, cr);
+        uint32_t dec = 0u;
+        for (uint32_t j = 2; j <= dp.W; ++j) {
+            float best = INFINITY;
+            uint32_t bk = 1;
+            for (uint32_t k = 1; k < j; ++k) {
+                const float v = cl[k - 1] + cr[j - k - 1];
+                if (v < best) { best = v; bk = k; }
+            }
+            dist[j] = best;
+            dec |= bk << (8 + 3 * (j - 2));
+        }
+        const float a = boxArea(lo, hi);
+        const uint32_t cnt = dp.last[p] - dp.first[p] + 1u;
+        const float leafC = cnt <= dp.leafMax ? a * float(cnt) * dp.cPrim : INFINITY;
+        const float wideC = a + dist[dp.W];   // one node visit costs 1
+        float c = leafC <= wideC ? leafC : wideC;
+        if (leafC <= wideC) dec |= 1u;
+        float* co = dp.cost + size_t(p) * dp.W;
+        st(co, c);
+        for (uint32_t j = 2; j <= dp.W; ++j) {
+            if (dist[j] < c) { c = dist[j]; dec |= 1u << (j - 1); }
+            st(co + j - 1, c);
+        }
+        st(dp.dec + p, __uint_as_float(dec));
         if (p == 0u) return;
         p = parentInt[p];
     }
@@ -242,6 +308,7 @@ struct CollapseArgs {
     const uint32_t *vals, *left, *right, *first, *last;
     const float4 *boxLo, *boxHi;
     const float* nodeBox;
+    const float* dec;       // decision words of the optimal collapse (nullptr: greedy)
     const uint32_t* red;
     float* nodes;           // float wide nodes (bvh.h), f4PerNode * 4 floats each
     uint32_t* counters;     // [0] wide nodes, [1] leaves, [2] next-level items
@@ -262,9 +329,46 @@ __global__ __launch_bounds__(256) void gbCollapseKernel(CollapseArgs a, const ui
     const uint2 item = in[k];
     const uint32_t W = a.W;
     uint32_t kids[8];
+    bool kidLeaf[8];
     uint32_t nk = 0;
-    if ((item.x & kLeafTag) || codeCount(a, item.x) <= a.leafMax) {
-        kids[nk++] = item.x;   // a root small enough to be one leaf
+    const auto decOf = [&](uint32_t c) { return __float_as_uint(a.dec[c]); };
+    if (item.x & kLeafTag) {
+        kids[nk] = item.x;   // a single primitive
+        kidLeaf[nk++] = true;
+    } else if (a.dec) {
+        // slots from the decision words: (binary node, slots) pairs, at most W pending
+        if (decOf(item.x) & 1u) {
+            kids[nk] = item.x;   // a root best kept as one leaf
+            kidLeaf[nk++] = true;
+        } else {
+            uint32_t sc[8], ss[8], ns = 0;
+            const uint32_t k0 = (decOf(item.x) >> (8 + 3 * (W - 2))) & 7u;
+            sc[ns] = a.left[item.x]; ss[ns++] = k0;
+            sc[ns] = a.right[item.x]; ss[ns++] = W - k0;
+            while (ns > 0 && nk < W) {
+                --ns;
+                const uint32_t c = sc[ns];
+                uint32_t slots = ss[ns];
+                if (c & kLeafTag) {
+                    kids[nk] = c;
+                    kidLeaf[nk++] = true;
+                    continue;
+                }
+                const uint32_t d = decOf(c);
+                while (slots > 1u && !((d >> (slots - 1u)) & 1u)) --slots;
+                if (slots == 1u) {
+                    kids[nk] = c;
+                    kidLeaf[nk++] = (d & 1u) != 0u;
+                    continue;
+                }
+                const uint32_t k = (d >> (8 + 3 * (slots - 2u))) & 7u;
+                sc[ns] = a.left[c]; ss[ns++] = k;
+                sc[ns] = a.right[c]; ss[ns++] = slots - k;
+            }
+        }
+    } else if (codeCount(a, item.x) <= a.leafMax) {
+        kids[nk] = item.x;   // a root small enough to be one leaf
+        kidLeaf[nk++] = true;
     } else {
         kids[nk++] = a.left[item.x];
         kids[nk++] = a.right[item.x];
@@ -288,6 +392,7 @@ __global__ __launch_bounds__(256) void gbCollapseKernel(CollapseArgs a, const ui
             kids[best] = a.left[open];
             kids[nk++] = a.right[open];
         }
+        for (uint32_t i = 0; i < nk; ++i) kidLeaf[i] = (kids[i] & kLeafTag) || codeCount(a, kids[i]) <= a.leafMax;
     }
     // padding of the host collapse: pad + |x| * 1e-6 outward, pad = 1e-5 * extent + 1e-6
     float ext = 0.0f;
@@ -306,7 +411,7 @@ __global__ __launch_bounds__(256) void gbCollapseKernel(CollapseArgs a, const ui
                 hi[q] = bhi[q] + (pad + fabsf(bhi[q]) * 1e-6f);
             }
             const uint32_t cnt = codeCount(a, c);
-            if ((c & kLeafTag) || cnt <= a.leafMax) {
+            if (kidLeaf[i]) {
                 ref = kLeafBit | ((cnt - 1u) << kCountShift) | codeFirst(a, c);   // bvh.h leaf encoding
                 atomicAdd(&a.counters[1], 1u);
             } else {
@@ -386,7 +491,7 @@ float hostUnkey(uint32_t k) {
 
 }  // namespace
 
-int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool quant) {
+int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool quant, bool optimal) {
     using Clock = std::chrono::steady_clock;
     const auto t0 = Clock::now();
     const HostGeometry& geo = s->geom;
@@ -403,11 +508,11 @@ int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool 
     hipStream_t st0 = ctx->stream;
     // Device temporaries, freed on every exit.
     DevBuf dTri, dTriId, dSph, dSphId, dQuad, dQuadId, dLo, dHi, dRed, dKeys, dVals, dKeys2, dVals2, dSortTmp, dLeft,
-        dRight, dFirst, dLast, dPInt, dPLeaf, dArr, dNodeBox, dItemsA, dItemsB, dCnt, dFloatNodes;
+        dRight, dFirst, dLast, dPInt, dPLeaf, dArr, dNodeBox, dItemsA, dItemsB, dCnt, dFloatNodes, dCost, dDec;
     auto freeAll = [&]() {
         for (DevBuf* b : {&dTri, &dTriId, &dSph, &dSphId, &dQuad, &dQuadId, &dLo, &dHi, &dRed, &dKeys, &dVals, &dKeys2,
                           &dVals2, &dSortTmp, &dLeft, &dRight, &dFirst, &dLast, &dPInt, &dPLeaf, &dArr, &dNodeBox,
-                          &dItemsA, &dItemsB, &dCnt, &dFloatNodes})
+                          &dItemsA, &dItemsB, &dCnt, &dFloatNodes, &dCost, &dDec})
             freeBuffer(*b);
     };
     auto chk = [&](int r) {
@@ -434,6 +539,7 @@ int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool 
         !chk(ensureBuffer(ctx, dNodeBox, size_t(ni) * 24)) || !chk(ensureBuffer(ctx, dItemsA, maxNodes * 8)) ||
         !chk(ensureBuffer(ctx, dItemsB, maxNodes * 8)) || !chk(ensureBuffer(ctx, dCnt, 16)) ||
         !chk(ensureBuffer(ctx, dFloatNodes, maxNodes * f4n * 16)) ||
+        !chk(ensureBuffer(ctx, dCost, size_t(ni) * W * 4)) || !chk(ensureBuffer(ctx, dDec, size_t(ni) * 4)) ||
         !chk(ensureBuffer(ctx, s->dPrims, size_t(n) * 48))) {
         freeAll();
         return rc;
@@ -469,11 +575,14 @@ int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool 
         hipLaunchKernelGGL(gbKarrasKernel, dim3((n - 1 + 255) / 256), dim3(256), 0, st0, n, keys, u32(dLeft), u32(dRight),
                            u32(dFirst), u32(dLast), u32(dPInt), u32(dPLeaf));
         hipLaunchKernelGGL(gbZeroKernel, dim3((n - 1 + 255) / 256), dim3(256), 0, st0, u32(dArr), n - 1);
+        const DpArgs dp{uint32_t(W), uint32_t(leafMax), kCollapsePrimCost, u32(dFirst), u32(dLast),
+                        static_cast<float*>(dCost.ptr), static_cast<float*>(dDec.ptr)};
         hipLaunchKernelGGL(gbRefitKernel, dim3(blocks), dim3(256), 0, st0, n, vals, f4(dLo), f4(dHi), u32(dLeft),
-                           u32(dRight), u32(dPInt), u32(dPLeaf), u32(dArr), static_cast<float*>(dNodeBox.ptr));
+                           u32(dRight), u32(dPInt), u32(dPLeaf), u32(dArr), static_cast<float*>(dNodeBox.ptr), dp);
     }
     CollapseArgs ca{n, uint32_t(W), uint32_t(leafMax), vals, u32(dLeft), u32(dRight), u32(dFirst), u32(dLast),
-                    f4(dLo), f4(dHi), static_cast<const float*>(dNodeBox.ptr), u32(dRed),
+                    f4(dLo), f4(dHi), static_cast<const float*>(dNodeBox.ptr),
+                    optimal && n >= 2 ? static_cast<const float*>(dDec.ptr) : nullptr, u32(dRed),
                     static_cast<float*>(dFloatNodes.ptr), u32(dCnt)};
     uint32_t nIn = 1, depth = 0;
     uint2* itIn = static_cast<uint2*>(dItemsA.ptr);

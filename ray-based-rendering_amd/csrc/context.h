@@ -81,7 +81,7 @@ int prepareTraversal(rbr_ctx* ctx, uint32_t maxStack);
 
 // Device build of the scene's wide BVH (bvh_gpu.hip): fills s->bvh (host copies) and the
 // scene's device node and primitive buffers.
-int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool quant);
+int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool quant, bool optimal);
 
 // Gives the scene a new version number (after any change to it).
 void bumpVersion(rbr_scene* s);

@@ -36,18 +36,20 @@ def dev_scene(ctx, width=4, leaf=4, **kw):
     return sc
 
 
+@pytest.mark.parametrize("collapse", [0, R.BVH_GREEDY])
 @pytest.mark.parametrize("width", WIDTHS)
 @pytest.mark.parametrize("cfg", [(30, 30, False, 3000), (400, 1500, True, 6000), (0, 2, False, 500),
                                  (1, 0, False, 200)])
-def test_device_build_hits_match_brute_force(ctx, width, cfg):
+def test_device_build_hits_match_brute_force(ctx, width, cfg, collapse):
+    """Device-built trees (SAH-optimal or greedy collapse) give the brute-force hits."""
     ns, nt, plane, nr = cfg
-    say(f"case: device build world {cfg} width {width}")
+    say(f"case: device build world {cfg} width {width} collapse {collapse}")
     rng = np.random.default_rng(77 + ns + nt)
     w = worlds.make_world(rng, ns, nt, plane)
     rays = worlds.make_rays(rng, nr)
     kw = worlds.geometry_kwargs(w)
     o = O.OracleScene(**kw)
-    sc = dev_scene(ctx, width, 4, **kw)
+    sc = dev_scene(ctx, width | collapse, 4, **kw)
     assert_same(sc.trace_closest(rays), o.closest(rays, brute=True))
     assert np.array_equal(sc.trace_any(rays), o.any(rays, brute=True))
     # the host traversal runs on the downloaded copy of the same tree
@@ -67,15 +69,16 @@ def test_duplicates_and_leaf_sizes(ctx):
             assert h["id"][0] == 100 and h["t"][0] == np.float32(1.0), (leaf, width)
 
 
+@pytest.mark.parametrize("collapse", [0, R.BVH_GREEDY])
 @pytest.mark.parametrize("width", [4, 8])
-def test_structure_and_quantization(ctx, width):
+def test_structure_and_quantization(ctx, width, collapse):
     """Quantized nodes carry the float nodes' refs and decode (fma(q, 2^e, p), one rounding,
     as the kernels evaluate them) to boxes that contain the float child boxes; every primitive
     sits in exactly one leaf of at most leaf_max primitives, inside its padded child box."""
-    say(f"case: device build structure width {width}")
+    say(f"case: device build structure width {width} collapse {collapse}")
     tris = R.icosphere(48, 100.0, 0.1, 0xC1)
     leaf = 4
-    sc = dev_scene(ctx, width | R.BVH_QUANTIZED, leaf, tris=tris)
+    sc = dev_scene(ctx, width | R.BVH_QUANTIZED | collapse, leaf, tris=tris)
     st = sc.stats()
     assert st["prim_count"] == len(tris) and st["node_count"] >= 1
     q4 = width // 4
