@@ -145,10 +145,10 @@ def main_render(argv) -> int:
         with open(args.out) as f:
             out["workloads"] = json.load(f).get("workloads", {})
     if args.quick:
-        grid = dict(pools=[1 << 24, 1 << 25], sorts=[False, True], graphs=[True], waves=[16, 32], modes=[0, 6])
+        grid = dict(pools=[1 << 24, 1 << 25], sorts=[False, True], graphs=[True], waves=[16, 32], modes=[0, 11])
     else:
         grid = dict(pools=[1 << 22, 1 << 23, 1 << 24, 1 << 25], sorts=[False, True], graphs=[False, True],
-                    waves=[16, 32], modes=[0, 2, 6, 8])
+                    waves=[16, 32], modes=[0, 6, 8, 11])
     for key in args.only.split(","):
         jobs = B.workload_jobs(key)
         scs = [R.scene_from_desc(ctx, j["desc"]) for j in jobs]
