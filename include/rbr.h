@@ -260,7 +260,8 @@ int rbr_time_trace(rbr_scene* scene, int kind, const rbr_ray* d_rays, size_t n, 
  * 5 / 6 / 7 / 8 = persistent waves that advance every lane by one node and one primitive per
  * iteration and refill once 4 / 8 / 16 / 32 lanes are idle, 9 / 10 / 11 / 12 = the same with
  * the primitive step deferred until 32 lanes hold a leaf or fewer than 48 lanes have a node
- * to test; lds_depth = LDS
+ * to test; mode | (m + 1) << 8 runs any-hit launches (shadow rays) in mode m and closest-hit
+ * launches in mode & 0xFF; lds_depth = LDS
  * stack entries per lane (8, 12, 16, 24, 32; deeper entries spill to a per-wave global
  * buffer); waves_per_cu = persistent waves launched per compute unit (1..32). */
 int rbr_set_trace_tuning(rbr_ctx* ctx, int mode, int lds_depth, int waves_per_cu);

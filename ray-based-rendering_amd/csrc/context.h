@@ -32,7 +32,8 @@ struct rbr_ctx {
     rbr::RenderState* render = nullptr;
     rbr::GuideState* guide = nullptr;
     // Traversal tuning (rbr_set_trace_tuning) and persistent-launch state.
-    int traceMode = 0;                 // 0 persistent waves, 1 one ray per lane, 2-4 per-lane fetch, 5-8 steps
+    int traceMode = 0;                 // 0 persistent waves, 1 one ray per lane, 2-4 per-lane fetch, 5-12 steps
+    int traceModeAny = -1;             // mode of any-hit launches (-1: traceMode)
     int ldsDepth = rbr::kLdsDepthDefault;
     int wavesPerCu = rbr::kWavesPerCuDefault;
     rbr::DevBuf dCounter, dSpill;

@@ -100,7 +100,8 @@ TraceArgs rbr::sceneTraceArgs(const rbr_scene* s) { return traceArgs(s); }
 int rbr::launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack, bool any) {
     if (a.n == 0 && !a.nDev) return RBR_OK;
     hipError_t e;
-    if (ctx->traceMode == 1 && !a.nDev) {
+    const int m = any && ctx->traceModeAny >= 0 ? ctx->traceModeAny : ctx->traceMode;
+    if (m == 1 && !a.nDev) {
         if (maxStack > uint32_t(kMaxStack)) return fail(ctx, RBR_ERR_UNSUPPORTED, "BVH too deep for the lane stack");
         e = launchTrace(a, width, any, ctx->ldsDepth, nullptr, 0, ctx->stream);
         return e == hipSuccess ? RBR_OK : failHip(ctx, e, "trace launch");
@@ -113,7 +114,6 @@ int rbr::launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack
     // modes 5, 6, 7, 8: per-lane fetch once 4, 8, 16, 32 lanes are idle (single steps);
     // modes 9, 10, 11, 12: as 5-8 with primitive steps deferred until 32 lanes hold a leaf or
     // fewer than 48 lanes have an interior node
-    const int m = ctx->traceMode;
     const int ms = m >= 9 ? m - 4 : m;
     const uint32_t refill = ms >= 5 ? (2u << (ms - 4)) : ms >= 2 ? (4u << (ms - 1)) : 0u;
     PersistArgs p{static_cast<uint32_t*>(ctx->dCounter.ptr), static_cast<uint2*>(ctx->dSpill.ptr), spillDepth, refill,
@@ -250,12 +250,16 @@ int rbr_calib_copy(rbr_ctx* ctx, void* dst, const void* src, size_t bytes) {
 
 int rbr_set_trace_tuning(rbr_ctx* ctx, int mode, int lds_depth, int waves_per_cu) {
     if (!ctx) return RBR_ERR_INVALID;
-    if (mode < 0 || mode > 12)
-        return fail(ctx, RBR_ERR_INVALID, "mode must be 0 (persistent), 1 (lane), 2-4 (per-lane fetch), 5-8 (steps) or 9-12 (deferred steps)");
+    const int closest = mode & 0xFF, anyCode = (mode >> 8) & 0xFF;
+    if (mode < 0 || (mode >> 16) != 0 || closest > 12 || anyCode > 13)
+        return fail(ctx, RBR_ERR_INVALID,
+                    "mode must be 0 (persistent), 1 (lane), 2-4 (per-lane fetch), 5-8 (steps) or 9-12 (deferred steps), "
+                    "optionally | (any-hit mode + 1) << 8");
     if (lds_depth != 8 && lds_depth != 12 && lds_depth != 16 && lds_depth != 24 && lds_depth != 32)
         return fail(ctx, RBR_ERR_INVALID, "lds_depth must be 8, 12, 16, 24 or 32");
     if (waves_per_cu < 1 || waves_per_cu > 32) return fail(ctx, RBR_ERR_INVALID, "waves_per_cu must be in 1..32");
-    ctx->traceMode = mode;
+    ctx->traceMode = closest;
+    ctx->traceModeAny = anyCode ? anyCode - 1 : -1;
     ctx->ldsDepth = lds_depth;
     ctx->wavesPerCu = waves_per_cu;
     return RBR_OK;

@@ -2099,7 +2099,8 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
                 for (int k = 0; k < 3; ++k) { putf(gd.lo[k]); putf(gd.inv[k]); }
             }
             if (anyRecord) { putp(ctx->guide->dRecCount.ptr); putp(ctx->guide->dCsum.ptr); }
-            put(uint64_t(ctx->traceMode)); put(uint64_t(ctx->ldsDepth)); put(uint64_t(ctx->wavesPerCu));
+            put(uint64_t(ctx->traceMode)); put(uint64_t(uint32_t(ctx->traceModeAny))); put(uint64_t(ctx->ldsDepth));
+            put(uint64_t(ctx->wavesPerCu));
             putp(ctx->dCounter.ptr); putp(ctx->dSpill.ptr); put(ctx->dSpill.bytes);
             size_t hit = R.graphs.size();
             for (size_t k = 0; k < R.graphs.size(); ++k)

@@ -158,11 +158,13 @@ _ORACLE: dict = {}
 @pytest.mark.parametrize("mode,depth,waves", [(0, 8, 1), (0, 12, 32), (0, 32, 16), (1, 8, 16), (1, 24, 16),
                                                (2, 8, 32), (3, 8, 1), (3, 12, 16), (4, 32, 16),
                                                (5, 8, 32), (6, 8, 1), (6, 12, 16), (7, 32, 16), (8, 8, 24),
-                                               (9, 8, 32), (10, 12, 1), (11, 8, 32), (12, 32, 16)])
+                                               (9, 8, 32), (10, 12, 1), (11, 8, 32), (12, 32, 16),
+                                               (0 | (11 + 1) << 8, 8, 32), (11 | (0 + 1) << 8, 12, 16)])
 def test_launch_tunings_identical(ctx, mode, depth, waves):
     """Every launch configuration (persistent waves with tiny/large LDS stacks and so
     deep global spills, one ray per lane, persistent waves whose idle lanes fetch new rays,
-    in rounds or in single node / primitive steps) returns the same bit-exact hits."""
+    in rounds or in single node / primitive steps, and a different mode for any-hit
+    launches) returns the same bit-exact hits."""
     say(f"case: tuning mode {mode} lds {depth} waves {waves}")
     tris = R.icosphere(48, 100.0, 0.1, 0xC1)
     lo, hi = R.mesh_bounds(tris)
