@@ -262,7 +262,7 @@ int rbr_time_trace(rbr_scene* scene, int kind, const rbr_ray* d_rays, size_t n, 
  * the primitive step deferred until 32 lanes hold a leaf or fewer than 48 lanes have a node
  * to test; mode | (m + 1) << 8 runs any-hit launches (shadow rays) in mode m and closest-hit
  * launches in mode & 0xFF; lds_depth = LDS
- * stack entries per lane (8, 12, 16, 24, 32; deeper entries spill to a per-wave global
+ * stack entries per lane (8, 10, 12, 16, 24, 32; deeper entries spill to a per-wave global
  * buffer); waves_per_cu = persistent waves launched per compute unit (1..32). */
 int rbr_set_trace_tuning(rbr_ctx* ctx, int mode, int lds_depth, int waves_per_cu);
 

@@ -186,10 +186,10 @@ def main() -> int:
     name, cus = ctx.device_info()
     if args.quick:
         grid = dict(widths=[4, 8, 4 | R.BVH_QUANTIZED, 8 | R.BVH_QUANTIZED], leaves=[4], modes=[0, 2, 6, 7, 8, 11, 12],
-                    depths=[8, 12], waves=[16, 24, 32])
+                    depths=[8, 10, 12], waves=[16, 24, 32])
     else:
         grid = dict(widths=[4, 8, 4 | R.BVH_QUANTIZED, 8 | R.BVH_QUANTIZED], leaves=[2, 4, 8], modes=list(range(13)),
-                    depths=[8, 12, 16, 24, 32], waves=[4, 8, 16, 24, 32])
+                    depths=[8, 10, 12, 16, 24, 32], waves=[4, 8, 16, 24, 32])
     out = {"device": name, "cus": cus, "workload": f"icosphere n={args.n_sub} ({len(tris)} tris), {args.rays} rays",
            "time": time.strftime("%Y-%m-%dT%H:%M:%S")}
     for kind, r in (("closest", rays), ("any", shadow)):

@@ -255,8 +255,8 @@ int rbr_set_trace_tuning(rbr_ctx* ctx, int mode, int lds_depth, int waves_per_cu
         return fail(ctx, RBR_ERR_INVALID,
                     "mode must be 0 (persistent), 1 (lane), 2-4 (per-lane fetch), 5-8 (steps) or 9-12 (deferred steps), "
                     "optionally | (any-hit mode + 1) << 8");
-    if (lds_depth != 8 && lds_depth != 12 && lds_depth != 16 && lds_depth != 24 && lds_depth != 32)
-        return fail(ctx, RBR_ERR_INVALID, "lds_depth must be 8, 12, 16, 24 or 32");
+    if (lds_depth != 8 && lds_depth != 10 && lds_depth != 12 && lds_depth != 16 && lds_depth != 24 && lds_depth != 32)
+        return fail(ctx, RBR_ERR_INVALID, "lds_depth must be 8, 10, 12, 16, 24 or 32");
     if (waves_per_cu < 1 || waves_per_cu > 32) return fail(ctx, RBR_ERR_INVALID, "waves_per_cu must be in 1..32");
     ctx->traceMode = closest;
     ctx->traceModeAny = anyCode ? anyCode - 1 : -1;

@@ -55,7 +55,7 @@ struct PersistArgs {
     uint32_t nodeMin;         //   lanes hold a leaf or fewer than nodeMin lanes have an interior node
 };
 
-// ldsDepth in {8, 12, 16, 24, 32}; p == nullptr selects the one-ray-per-lane kernel (which
+// ldsDepth in {8, 10, 12, 16, 24, 32}; p == nullptr selects the one-ray-per-lane kernel (which
 // needs a.nDev == nullptr).
 hipError_t launchTrace(const TraceArgs& a, int width, bool any, int ldsDepth, const PersistArgs* p, unsigned grid,
                        hipStream_t s);

@@ -337,6 +337,7 @@ hipError_t launchTrace(const TraceArgs& a, int width, bool any, int ldsDepth, co
     if (!p && a.nDev) return hipErrorInvalidValue;
     switch (ldsDepth) {
         case 8: return launchL<8>(a, p, grid, width, any, s);
+        case 10: return launchL<10>(a, p, grid, width, any, s);
         case 12: return launchL<12>(a, p, grid, width, any, s);
         case 24: return launchL<24>(a, p, grid, width, any, s);
         case 32: return launchL<32>(a, p, grid, width, any, s);
