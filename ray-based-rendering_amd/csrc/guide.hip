@@ -5,9 +5,11 @@
 //                        computed transposed (H^T = W X^T, 16x16x32 MFMA with the weights as
 //                        the A operand), so a layer's accumulators are already the next
 //                        layer's B operand; weights are read from LDS in fragment order (one
-//                        16-byte read per lane per fragment, shared by the wave's 4 tiles of 16
-//                        queries); outputs go through a small LDS stage to be turned into 8
-//                        vMF lobes per query (FP16 record)
+//                        16-byte read per lane per fragment, shared by the wave's 2 tiles of 16
+//                        queries); one 1,024-thread workgroup per CU shares one LDS copy of the
+//                        weights, so 4 waves per SIMD fit (register file and LDS both); outputs
+//                        go through a small LDS stage to be turned into 8 vMF lobes per query
+//                        (FP16 record)
 //   guideFwdBwdKernel    training, 64 samples per wave: the same register-resident transposed
 //                        forward, the loss -c*log p(dir) and its gradient w.r.t. the 40 outputs
 //                        (through a small LDS stage), and the backward pass in the same
@@ -43,7 +45,7 @@ namespace {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int kWaves = 4;                     // waves per inference workgroup
+constexpr int kWaves = 16;                    // waves per inference workgroup (one weight copy in LDS)
 constexpr int kTile = 16;                     // queries / samples per wave tile
 
 // Orders a wave's LDS writes before its later LDS reads by other lanes (no block barrier: the
@@ -62,7 +64,7 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kIT = 4;                         // 16-query tiles per wave
+constexpr int kIT = 2;                         // 16-query tiles per wave
 constexpr int kInferStage = kTile * kGOutPad;  // floats of LDS output stage per wave
 
 __device__ __forceinline__ v4f mfma32(u32x4 a, u32x4 b, v4f c) {
@@ -110,7 +112,7 @@ __device__ __forceinline__ void hiddenLayer(v4f (&acc)[kIT][4], const u32x4* lwf
 // Queries come either from an explicit array (9 floats: position, normal, wo) or from the
 // render's extension queue (hit point, outward normal, -ray direction); each wave takes
 // kIT * 16 queries at a time.  Queries that are not guided are skipped (no output).
-__global__ __launch_bounds__(256, 2) void guideInferKernel(GuideInferArgs a) {
+__global__ __launch_bounds__(64 * kWaves, 1) void guideInferKernel(GuideInferArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     u32x4* lwf = reinterpret_cast<u32x4*>(smem);
     float* lb = reinterpret_cast<float*>(smem + kGWeights * 2);
@@ -783,8 +785,8 @@ int guideSetAttributes(rbr_ctx* ctx) {
 int launchGuideInfer(rbr_ctx* ctx, const GuideInferArgs& a, uint32_t maxN) {
     if (int rc = guideSetAttributes(ctx)) return rc;
     const unsigned groups = (maxN + kTile * kIT - 1) / (kTile * kIT);
-    const unsigned grid = std::max(1u, std::min<unsigned>((groups + kWaves - 1) / kWaves, unsigned(ctx->cuCount) * 2));
-    hipLaunchKernelGGL(guideInferKernel, dim3(grid), dim3(256), inferSmem(), ctx->stream, a);
+    const unsigned grid = std::max(1u, std::min<unsigned>((groups + kWaves - 1) / kWaves, unsigned(ctx->cuCount)));
+    hipLaunchKernelGGL(guideInferKernel, dim3(grid), dim3(64 * kWaves), inferSmem(), ctx->stream, a);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? RBR_OK : failHip(ctx, e, "guide inference launch");
 }
