@@ -159,7 +159,8 @@ _ORACLE: dict = {}
                                                (2, 8, 32), (3, 8, 1), (3, 12, 16), (4, 32, 16),
                                                (5, 8, 32), (6, 8, 1), (6, 12, 16), (7, 32, 16), (8, 8, 24),
                                                (9, 8, 32), (10, 12, 1), (11, 8, 32), (12, 32, 16),
-                                               (0 | (11 + 1) << 8, 8, 32), (11 | (0 + 1) << 8, 12, 16)])
+                                               (0 | (11 + 1) << 8, 8, 32), (11 | (0 + 1) << 8, 12, 16),
+                                               (7, 10, 32), (11, 10, 24), (3, 10, 16)])
 def test_launch_tunings_identical(ctx, mode, depth, waves):
     """Every launch configuration (persistent waves with tiny/large LDS stacks and so
     deep global spills, one ray per lane, persistent waves whose idle lanes fetch new rays,
