@@ -386,12 +386,8 @@ void buildWideBVH(const HostGeometry& g, int width, int leafMax, WideBVH& out, b
         q.index = i;
     }
     for (uint32_t i = 0; i < nquad; ++i) {
-        const float* s = &g.quads[12 * size_t(i)];
         BPrim& q = p[ntri + nsph + i];
-        float c1[3], c2[3], c3[3];
-        for (int k = 0; k < 3; ++k) { c1[k] = s[k] + s[3 + k]; c2[k] = s[k] + s[6 + k]; c3[k] = s[k] + s[3 + k] + s[6 + k]; }
-        minmax3(s, c1, c2, q.lo, q.hi);
-        for (int k = 0; k < 3; ++k) { q.lo[k] = std::min(q.lo[k], c3[k]); q.hi[k] = std::max(q.hi[k], c3[k]); }
+        quadBounds(&g.quads[12 * size_t(i)], q.lo, q.hi);
         q.kind = KIND_QUAD;
         q.index = i;
     }

@@ -32,6 +32,7 @@
 #include "bvh.h"
 #include "bvh_quant.h"
 #include "context.h"
+#include "prim_tests.h"
 
 namespace rbr {
 
@@ -67,12 +68,7 @@ __device__ void primBox(const GGeom& g, uint32_t i, float lo[3], float hi[3]) {
             hi[k] = s[k] + s[3];
         }
     } else {
-        const float* s = g.quads + 12 * size_t(i - g.ntri - g.nsph);
-        for (int k = 0; k < 3; ++k) {
-            const float c1 = s[k] + s[3 + k], c2 = s[k] + s[6 + k], c3 = s[k] + s[3 + k] + s[6 + k];
-            lo[k] = mn(mn(mn(s[k], c1), c2), c3);
-            hi[k] = mx(mx(mx(s[k], c1), c2), c3);
-        }
+        quadBounds(g.quads + 12 * size_t(i - g.ntri - g.nsph), lo, hi);
     }
 }
 

@@ -105,6 +105,36 @@ RBR_HD bool quadTest(V o, V d, V org, V u, V vv, V n, float tmin, float tmax, fl
     return true;
 }
 
+// Box of every point quadTest can accept for the quad (origin, u, v) = s[0..8].  The test
+// projects the hit onto u and onto v separately, so when u is not orthogonal to v it accepts
+// the parallelogram whose corners p = origin + a u + b v solve
+//   [u.u u.v; u.v v.v] [a b]^T = [i u.u, j v.v]^T,  i, j in {0, 1},
+// not the one spanned by u and v (Quad::getBounds, quad.cpp:44-51, bounds only the latter, so
+// the reference's BVH misses hits its brute-force loop finds on such quads).  The box covers
+// both parallelograms; for rectangles they coincide.  A degenerate quad (u parallel to v)
+// keeps the box of its own corners.
+RBR_HD void quadBounds(const float* s, float lo[3], float hi[3]) {
+    for (int k = 0; k < 3; ++k) {
+        const float c1 = s[k] + s[3 + k], c2 = s[k] + s[6 + k], c3 = s[k] + s[3 + k] + s[6 + k];
+        lo[k] = fminf(fminf(fminf(s[k], c1), c2), c3);
+        hi[k] = fmaxf(fmaxf(fmaxf(s[k], c1), c2), c3);
+    }
+    const double uu = double(s[3]) * s[3] + double(s[4]) * s[4] + double(s[5]) * s[5];
+    const double vv = double(s[6]) * s[6] + double(s[7]) * s[7] + double(s[8]) * s[8];
+    const double uv = double(s[3]) * s[6] + double(s[4]) * s[7] + double(s[5]) * s[8];
+    const double det = uu * vv - uv * uv;
+    if (!(det > 1e-12 * uu * vv)) return;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            const double a = vv * (i * uu - j * uv) / det, b = uu * (j * vv - i * uv) / det;
+            for (int k = 0; k < 3; ++k) {
+                const float p = float(double(s[k]) + a * s[3 + k] + b * s[6 + k]);
+                lo[k] = fminf(lo[k], p);
+                hi[k] = fmaxf(hi[k], p);
+            }
+        }
+}
+
 RBR_HD bool planeTest(V o, V d, V n, float pd, float tmin, float tmax, float& tout) {
     float denom = vdot(n, d);
     if (fabsf(denom) < kEps) return false;

@@ -203,3 +203,32 @@ def test_step_modes_mixed_primitives(ctx, mode):
             assert np.array_equal(sc.trace_any(rays), ref_a)
     finally:
         ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
+
+
+@pytest.mark.parametrize("width", WIDTHS)
+def test_quads_bit_exact(ctx, width):
+    """Quad::intersect (quad.cpp:12-42) through the traversal kernels: Cornell-style walls
+    sharing edges, rotated box faces and random parallelograms, with rays aimed at edges and
+    corners and near-grazing rays (|n.d| around the 1e-4 cut).  Ids and t equal the oracle's
+    lowest-id brute force, for the default kernel and the tuned step kernel."""
+    say(f"case: quad world width {width}")
+    rng = np.random.default_rng(77)
+    q = worlds.make_quad_world(rng, 300)
+    rays = worlds.make_quad_rays(rng, q, 20000)
+    o = O.OracleScene(quads=q)
+    ref_c, ref_a = o.closest(rays, brute=True), o.any(rays, brute=True)
+    # with triangles of the same world in front: mixed leaves
+    tris = worlds.make_world(rng, 0, 200, False)["tris"]
+    mixed_kw = dict(tris=tris, tri_ids=np.arange(len(q), len(q) + len(tris), dtype=np.uint32), quads=q,
+                    quad_ids=np.arange(len(q), dtype=np.uint32))
+    om = O.OracleScene(**mixed_kw)
+    ref_mc = om.closest(rays, brute=True)
+    for mode in (0, 11):
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, 8, 32))
+        try:
+            sc = gpu_scene(ctx, width, 4, quads=q)
+            assert_same(sc.trace_closest(rays), ref_c)
+            assert np.array_equal(sc.trace_any(rays), ref_a)
+            assert_same(gpu_scene(ctx, width, 4, **mixed_kw).trace_closest(rays), ref_mc)
+        finally:
+            ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))

@@ -211,3 +211,22 @@ def test_quantized_build_keeps_host_hits(width):
     ha, _, _ = a.host_trace_closest(rays)
     hb, _, _ = b.host_trace_closest(rays)
     assert np.array_equal(ha, hb)
+
+
+@pytest.mark.parametrize("width", [4, 8, 4 | R.BVH_QUANTIZED])
+def test_host_quads_equal_brute_force(width):
+    """Quads (rectangles and general parallelograms): the wide BVH boxes every point the quad
+    test accepts, so the host traversal equals the brute force of Quad::intersect."""
+    rng = np.random.default_rng(31)
+    q = worlds.make_quad_world(rng, 300)
+    rays = worlds.make_quad_rays(rng, q, 20000)
+    ref = O.OracleScene(quads=q).closest(rays, brute=True)
+    ctx = R.Context(-1)
+    sc = R.Scene(ctx)
+    sc.set_geometry(quads=q)
+    sc.build_bvh(width, 4)
+    h, _, _ = sc.host_trace_closest(rays)
+    bad = np.nonzero((h["id"] != ref["id"]) | (h["t"] != ref["t"]))[0]
+    assert len(bad) == 0, (len(bad), h[bad[:4]], ref[bad[:4]])
+    occ, _, _ = sc.host_trace_any(rays)
+    assert np.array_equal(occ, O.OracleScene(quads=q).any(rays, brute=True))

@@ -73,3 +73,68 @@ def geometry_kwargs(w: dict) -> dict:
         kw["planes"] = w["planes"]
     kw.update(world_ids(w))
     return kw
+
+
+def _quad_row(origin, u, v) -> np.ndarray:
+    """(origin, u, v, normal) with Quad's normal = cross(u, v).normalized() (quad.cpp:6-10),
+    in FP32 without contraction."""
+    origin, u, v = (np.asarray(a, np.float32) for a in (origin, u, v))
+    n = np.array([u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]], np.float32)
+    return np.concatenate([origin, u, v, normalize32(n)]).astype(np.float32)
+
+
+def make_quad_world(rng: np.random.Generator, n_random: int) -> np.ndarray:
+    """Quads as Quad (quad.cpp:12-42) meets them in the reference's scenes: a closed box of
+    axis-aligned walls that share edges (Cornell-style), two small rotated boxes whose faces
+    meet at edges, and randomly oriented quads of random extent (some long and thin)."""
+    rows = []
+    s = np.float32(12.0)
+    for o, u, v in (((-s, -s, -s), (2 * s, 0, 0), (0, 0, 2 * s)), ((-s, s, -s), (2 * s, 0, 0), (0, 0, 2 * s)),
+                    ((-s, -s, s), (2 * s, 0, 0), (0, 2 * s, 0)), ((-s, -s, -s), (0, 2 * s, 0), (0, 0, 2 * s)),
+                    ((s, -s, -s), (0, 0, 2 * s), (0, 2 * s, 0))):
+        rows.append(_quad_row(o, u, v))
+    for size, ang, off in (((3.3, 3.3, 3.3), -18.0, (-4.0, -12.0, -2.0)), ((3.3, 6.6, 3.3), 15.0, (2.0, -12.0, 3.0))):
+        a = np.radians(ang)
+        c, sn = np.float32(np.cos(a)), np.float32(np.sin(a))
+        rot = lambda p: np.array([c * p[0] + sn * p[2], p[1], -sn * p[0] + c * p[2]], np.float32)
+        sx, sy, sz = size
+        off = np.asarray(off, np.float32)
+        for o, u, v in (((0, 0, 0), (sx, 0, 0), (0, sy, 0)), ((0, 0, sz), (sx, 0, 0), (0, sy, 0)),
+                        ((0, 0, 0), (0, 0, sz), (0, sy, 0)), ((sx, 0, 0), (0, 0, sz), (0, sy, 0)),
+                        ((0, 0, 0), (sx, 0, 0), (0, 0, sz)), ((0, sy, 0), (sx, 0, 0), (0, 0, sz))):
+            rows.append(_quad_row(rot(o) + off, rot(u), rot(v)))
+    for k in range(n_random):
+        o = rng.uniform(-10.0, 10.0, 3)
+        u = rng.normal(size=3) * rng.uniform(0.2, 4.0)
+        v = rng.normal(size=3) * rng.uniform(0.2, 4.0) * (0.05 if k % 9 == 0 else 1.0)
+        rows.append(_quad_row(o, u, v))
+    return np.asarray(rows, np.float32)
+
+
+def make_quad_rays(rng: np.random.Generator, quads: np.ndarray, n: int) -> np.ndarray:
+    """Rays aimed at quad points with (u, v) coordinates on the edges and corners (0, 1),
+    just inside and outside them, and random; plus near-grazing rays whose |normal . d| is
+    around the reference's EPSILON = 1e-4 cut, and the random set of make_rays."""
+    n_aim = n // 2
+    q = quads[rng.integers(0, len(quads), n_aim)]
+    grid = np.array([0.0, 1.0, 0.5, 1e-7, 1 - 1e-7, -1e-6, 1 + 1e-6], np.float32)
+    cu = np.where(rng.random(n_aim) < 0.6, grid[rng.integers(0, len(grid), n_aim)], rng.random(n_aim)).astype(np.float32)
+    cv = np.where(rng.random(n_aim) < 0.6, grid[rng.integers(0, len(grid), n_aim)], rng.random(n_aim)).astype(np.float32)
+    target = q[:, 0:3] + cu[:, None] * q[:, 3:6] + cv[:, None] * q[:, 6:9]
+    o = rng.uniform(-11.5, 11.5, (n_aim, 3)).astype(np.float32)
+    d = (target - o).astype(np.float32)
+    graze = np.arange(n_aim) % 5 == 0
+    # Tilt the direction into the quad's plane: keep only a tiny normal component.
+    nrm = q[:, 9:12]
+    dn = (d * nrm).sum(1, keepdims=True)
+    tang = d - dn * nrm
+    eps = rng.choice(np.array([0.5e-4, 0.99e-4, 1.01e-4, 2e-4, 1e-3], np.float32), n_aim)
+    tl = np.linalg.norm(tang, axis=1, keepdims=True) + 1e-12
+    d_g = tang / tl + (np.sign(dn) + (dn == 0)) * eps[:, None] * nrm
+    d[graze] = d_g[graze]
+    ok = np.linalg.norm(d.astype(np.float64), axis=1) > 1e-3
+    r = np.empty(int(ok.sum()), RAY_DTYPE)
+    r["o"], r["d"] = o[ok], normalize32(d[ok])
+    r["tmin"] = np.float32(1e-4)
+    r["tmax"] = np.inf
+    return np.concatenate([r, make_rays(rng, n - n_aim)])
