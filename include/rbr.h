@@ -249,6 +249,16 @@ int rbr_trace_any(rbr_scene* scene, const rbr_ray* rays, size_t n, uint8_t* occl
 /* Device-pointer variants: asynchronous on the context stream. */
 int rbr_trace_closest_device(rbr_scene* scene, const rbr_ray* d_rays, size_t n, rbr_hit* d_hits);
 int rbr_trace_any_device(rbr_scene* scene, const rbr_ray* d_rays, size_t n, uint8_t* d_occluded);
+/* rbr_trace_closest_device without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same hits, bit for bit.  Its batch counters and
+ * stack spill rows live in a fixed 64 MiB scratch array that each GPU the library runs on
+ * holds until the process ends; the context's waves per CU are lowered to what that scratch
+ * holds for the BVH's depth (RBR_ERR_UNSUPPORTED if not even one wave per compute unit
+ * fits), n is limited to 2^31 as in the other entry points.  All calls on one GPU share that
+ * scratch, so calls that overlap on one GPU (other streams, other contexts, other host
+ * threads) race; serialize them. */
+int rbr_trace_closest_device_noalloc(rbr_scene* scene, const rbr_ray* d_rays, size_t n, rbr_hit* d_hits);
 /* Launches `iters` back-to-back traversals (kind 0 closest, 1 any) over device-resident rays
  * and returns the mean kernel time measured with HIP events on the launch stream. */
 int rbr_time_trace(rbr_scene* scene, int kind, const rbr_ray* d_rays, size_t n, void* d_out,

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -130,6 +131,36 @@ int rbr::prepareTraversal(rbr_ctx* ctx, uint32_t maxStack) {
     return ensureBuffer(ctx, ctx->dSpill, size_t(grid) * spillDepth * 64 * sizeof(uint2));
 }
 
+// ---------------------------------------------------------------- no-allocation scratch
+// Device memory of rbr_trace_closest_device_noalloc: the batch counters (first kCounterBytes,
+// rounded to 256 B), then the global stack spill rows of the persistent waves.  One copy per
+// GPU, held until the process ends.  In host code `scratch` is only a stand-in for the symbol:
+// its device address comes from hipGetSymbolAddress, once per device (noAllocScratch).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+constexpr size_t kNoAllocCounterBytes = (kCounterBytes + 255) / 256 * 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMu;
+void* gScratchPtr[kMaxDevices] = {};
+
+// Device address of `scratch` on the context's device (which must be current); looked up on
+// the first call for a device and kept.
+int noAllocScratch(rbr_ctx* ctx, void** out) {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices) return fail(ctx, RBR_ERR_UNSUPPORTED, "device index too large");
+    std::lock_guard<std::mutex> lock(gScratchMu);
+    if (!gScratchPtr[ctx->device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return failHip(ctx, e == hipSuccess ? hipErrorInvalidSymbol : e, "hipGetSymbolAddress(scratch)");
+        gScratchPtr[ctx->device] = p;
+    }
+    *out = gScratchPtr[ctx->device];
+    return RBR_OK;
+}
+}  // namespace
+
 // ---------------------------------------------------------------- context
 
 extern "C" {
@@ -161,6 +192,15 @@ int rbr_ctx_create(int device, rbr_ctx** out) {
         ctx->name = prop.gcnArchName;
         ctx->name += " / ";
         ctx->name += prop.name;
+    }
+    // Set-up of the no-allocation entry point: the scratch array's address on this device.
+    void* scr = nullptr;
+    if (device < kMaxDevices && noAllocScratch(ctx, &scr) != RBR_OK) {
+        (void)hipEventDestroy(ctx->ev0);
+        (void)hipEventDestroy(ctx->ev1);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return RBR_ERR_HIP;
     }
     *out = ctx;
     return RBR_OK;
@@ -505,6 +545,47 @@ int rbr_trace_any_device(rbr_scene* s, const rbr_ray* d_rays, size_t n, uint8_t*
     if (!s) return RBR_ERR_INVALID;
     if (int rc = useDevice(s->ctx)) return rc;
     return traceDevice(s, d_rays, n, d_occluded, true);
+}
+
+int rbr_trace_closest_device_noalloc(rbr_scene* s, const rbr_ray* d_rays, size_t n, rbr_hit* d_hits) {
+    if (!s) return RBR_ERR_INVALID;
+    rbr_ctx* ctx = s->ctx;
+    if (int rc = useDevice(ctx)) return rc;
+    if (!s->built) return fail(ctx, RBR_ERR_STATE, "trace before rbr_scene_build_bvh");
+    if (n > 0x80000000ull) return fail(ctx, RBR_ERR_INVALID, "ray batch too large (limit 2^31)");
+    if (n && (!d_rays || !d_hits)) return fail(ctx, RBR_ERR_INVALID, "null ray or result pointer");
+    if (n == 0) return RBR_OK;
+    void* base = nullptr;
+    if (int rc = noAllocScratch(ctx, &base)) return rc;
+    TraceArgs a = traceArgs(s);
+    a.rays = reinterpret_cast<const float4*>(d_rays);
+    a.n = uint32_t(n);
+    a.hits = reinterpret_cast<uint2*>(d_hits);
+    const uint32_t maxStack = s->bvh.empty() ? 1u : uint32_t(s->bvh.width - 1) * s->bvh.maxDepth + 1u;
+    const int m = ctx->traceMode;
+    hipError_t e;
+    if (m == 1) {   // one ray per lane: private stack, no scratch
+        if (maxStack > uint32_t(kMaxStack)) return fail(ctx, RBR_ERR_UNSUPPORTED, "BVH too deep for the lane stack");
+        e = launchTrace(a, s->bvh.code(), false, ctx->ldsDepth, nullptr, 0, ctx->stream);
+        return e == hipSuccess ? RBR_OK : failHip(ctx, e, "trace launch");
+    }
+    // As many persistent waves as the tuning asks for and the spill rows of the scratch hold.
+    const uint32_t spillDepth = std::max<uint32_t>(1u, maxStack > uint32_t(ctx->ldsDepth) ? maxStack - ctx->ldsDepth : 1u);
+    const size_t perWave = size_t(spillDepth) * 64 * sizeof(uint2);
+    const size_t fit = (kNoAllocScratchBytes - kNoAllocCounterBytes) / perWave;
+    const size_t cus = size_t(std::max(1, ctx->cuCount));
+    const size_t grid = std::min(cus * size_t(ctx->wavesPerCu), fit);
+    if (grid < cus)
+        return fail(ctx, RBR_ERR_UNSUPPORTED,
+                    "BVH too deep for the no-allocation scratch: " + std::to_string(perWave) + " B of stack per wave, " +
+                        std::to_string(kNoAllocScratchBytes) + " B in all, fewer than one wave per compute unit fits");
+    const int ms = m >= 9 ? m - 4 : m;
+    const uint32_t refill = ms >= 5 ? (2u << (ms - 4)) : ms >= 2 ? (4u << (ms - 1)) : 0u;
+    unsigned char* b = static_cast<unsigned char*>(base);
+    PersistArgs p{reinterpret_cast<uint32_t*>(b), reinterpret_cast<uint2*>(b + kNoAllocCounterBytes), spillDepth, refill,
+                  ms >= 5 ? 1u : 0u, m >= 9 ? kDeferPrimMin : 0u, kDeferNodeMin};
+    e = launchTrace(a, s->bvh.code(), false, ctx->ldsDepth, &p, unsigned(grid), ctx->stream);
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "trace launch");
 }
 
 int rbr_host_trace_closest(const rbr_scene* s, const rbr_ray* rays, size_t n, rbr_hit* hits, uint64_t* nodeVisits,
