@@ -62,14 +62,27 @@ def cpu_baseline(tris, rays, gpu_hits, seconds: float) -> dict:
     _, sec = run(rays[:probe])
     n = int(min(len(rays), max(probe, probe * seconds / max(sec, 1e-6))))
     h, sec = run(rays[:n])
-    ids_equal = int((h["id"] == gpu_hits["id"][:n]).sum())
-    hit = h["id"] != 0xFFFFFFFF
-    t_equal = bool(np.array_equal(h["t"][hit], gpu_hits["t"][:n][hit]))
+    g = gpu_hits[:n]
+    ids_equal = int((h["id"] == g["id"]).sum())
+    miss_c, miss_g = h["id"] == 0xFFFFFFFF, g["id"] == 0xFFFFFFFF
+    hit = ~miss_c
+    t_equal = bool(np.array_equal(miss_c, miss_g) and np.array_equal(h["t"][hit], g["t"][hit]))
+    # Every id difference must be an exact tie: the GPU's (lower) id and the CPU's id both
+    # give the same t under the Moller-Trumbore test of the oracle.
+    diff = np.nonzero(h["id"] != g["id"])[0]
+    ties = 0
+    for i in diff[:1000]:
+        if miss_c[i] or miss_g[i]:
+            continue
+        tg = O.triangle_t(tris[g["id"][i]], rays[i])
+        tc = O.triangle_t(tris[h["id"][i]], rays[i])
+        ties += int(tg is not None and tc is not None and tg == tc == float(g["t"][i]) and g["id"][i] < h["id"][i])
     return {
         "value": n / sec / 1e6, "unit": "Mrays/s", "cores": threads, "kind": kind,
         "sample": f"first {n} of the {len(rays)} closest-hit rays, same mesh; BVH build {build_ms:.0f} ms (serial)",
         "cpu": platform.processor() or platform.machine(),
-        "parity_on_sample": {"rays": n, "ids_equal": ids_equal, "t_bit_equal_on_hits": t_equal},
+        "parity_on_sample": {"rays": n, "ids_equal": ids_equal, "hit_sets_equal_and_t_bit_equal": t_equal,
+                             "id_differences": int(len(diff)), "id_differences_that_are_exact_ties": ties},
     }
 
 
@@ -112,6 +125,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 256 MiB streaming copy kernel 3x (byte-counter calibration under rocprofv3 --pmc)")
+    ap.add_argument("--no-paths", action="store_true", help="skip the path-tracing lines (C0, C2, C3)")
+    ap.add_argument("--no-c4", action="store_true", help="skip guided vs plain time-to-relMSE (C4)")
+    ap.add_argument("--path-cpu-seconds", type=float, default=5.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {args.gpus} is not supported "
@@ -180,9 +196,13 @@ def main() -> int:
     node_b = st["node_bytes"] / max(1, st["node_count"])   # 64/128 B (quantized) or 128/256 B
     nodes_per_ray, prims_per_ray = nv / len(sample), pt / len(sample)
     bytes_compulsory = 32 + 8
-    bytes_per_ray = bytes_compulsory + nodes_per_ray * node_b + prims_per_ray * 48
+    # A triangle test reads its three vertices (36 B); the leaf record the kernel loads is 48 B
+    # (v0, e1, e2 and the id/kind words), reported as the layout figure.
+    bytes_per_ray = bytes_compulsory + nodes_per_ray * node_b + prims_per_ray * 36
+    bytes_per_ray_layout = bytes_compulsory + nodes_per_ray * node_b + prims_per_ray * 48
     rays_per_s = N_RAYS / (ms_closest * 1e-3)
     achieved = rays_per_s * bytes_per_ray / 1e9
+    traffic = measured_traffic(args.width)
 
     rec = {
         "metric": "closest-hit Mrays/s (1.0M-tri displaced icosphere, 2^22 incoherent rays)",
@@ -213,8 +233,12 @@ def main() -> int:
         "bvh_build_ms": st["build_ms"],
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.width),
-            "bytes_per_ray": bytes_per_ray, "nodes_per_ray": nodes_per_ray, "prims_per_ray": prims_per_ray,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_frac": (traffic / (ms_closest * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
+            "limiter": "instruction issue (VALU), from the SQ counters in profiles/",
+            "bytes_per_ray": bytes_per_ray, "bytes_per_ray_layout": bytes_per_ray_layout,
+            "frac_layout_bytes": rays_per_s * bytes_per_ray_layout / 1e9 / HBM_PEAK_GBS,
+            "nodes_per_ray": nodes_per_ray, "prims_per_ray": prims_per_ray,
             "frac_compulsory_only": rays_per_s * bytes_compulsory / 1e9 / HBM_PEAK_GBS,
         },
         "cpu_baseline": None,
@@ -222,9 +246,24 @@ def main() -> int:
     if not args.no_cpu:
         log("bench: CPU baseline")
         rec["cpu_baseline"] = cpu_baseline(tris, rays, hits, args.cpu_seconds)
-    print(json.dumps(rec), flush=True)
     for b in (d_rays, d_shadow, d_hits, d_occ):
         b.free()
+    sc.close()
+    # The other workloads of the metric, outside the timed region of the closest-hit steps.
+    if not args.no_paths:
+        import bench_paths as BP
+        rec["paths"] = {}
+        for key in ("c0", "c2", "c3"):
+            log(f"bench: path tracing {key}")
+            rec["paths"][key] = BP.measure(ctx, key, cpu_seconds=0.0 if args.no_cpu else args.path_cpu_seconds, log=log)
+    if not args.no_c4:
+        import bench_guiding as BG
+        log("bench: c4 guided vs plain time to relMSE 0.01")
+        rec["c4"] = BG.c4_summary(ctx)
+        rec["time_to_relmse_0.01_plain_s"] = rec["c4"]["plain_s"]
+        rec["time_to_relmse_0.01_guided_s"] = rec["c4"]["guided_s"]
+        rec["time_to_relmse_0.01_speedup"] = rec["c4"]["speedup"]
+    print(json.dumps(rec), flush=True)
     ctx.close()
     return 0
 

@@ -320,6 +320,27 @@ int rbr_set_render_tuning(rbr_ctx* ctx, uint32_t pool_paths, int use_graphs, int
 /* Time of the last rbr_render call: total ms and per-stage ms (extend, shade, shadow). */
 int rbr_last_render_timing(rbr_ctx* ctx, double* total_ms, double* stage_ms /* 4 */);
 
+/* Traversal census of path-tracing renders (profiling helper, no counterpart in the
+ * reference): with rays_per_queue > 0, every later rbr_render / rbr_render_moments of PATH or
+ * VOLPATH jobs also copies an evenly strided sample of up to rays_per_queue rays of each
+ * bounce's extension queue and shadow queue (with their BVH roots) and, once the render is
+ * done, traces the samples with the host traversal of the same wide BVH to count node visits
+ * and primitive tests; hipGraph replay is off while it is on.  Images are unchanged.  0 turns
+ * it off (the default).  rbr_last_render_stats returns the counts of the last such render. */
+typedef struct rbr_render_stats {
+    uint64_t extension_rays;      /* closest-hit rays traced: camera rays and continuations */
+    uint64_t shadow_rays;         /* any-hit rays traced: NEE shadow rays                  */
+    uint64_t shadow_empty;        /* shadow-queue slots left empty by the chunked appends  */
+    uint64_t sampled_extension;   /* rays traced by the host census                        */
+    uint64_t sampled_shadow;
+    double ext_nodes_per_ray;     /* wide-node visits per extension ray (count-weighted)   */
+    double ext_prims_per_ray;     /* primitive tests per extension ray                     */
+    double shadow_nodes_per_ray;  /* per non-empty shadow ray                              */
+    double shadow_prims_per_ray;
+} rbr_render_stats;
+int rbr_set_render_census(rbr_ctx* ctx, uint32_t rays_per_queue);
+int rbr_last_render_stats(rbr_ctx* ctx, rbr_render_stats* out);
+
 /* ---------------------------------------------------------------- path guiding
  * Extension without a counterpart in the reference, whose integrators sample the BSDF only
  * (path_tracing_integrator.cpp:40-60).  A per-context MLP maps a shading point (position

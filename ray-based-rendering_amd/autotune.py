@@ -34,6 +34,15 @@ CACHE = os.path.join(PKG, "tuning", "traversal.json")
 RENDER_CACHE = os.path.join(PKG, "tuning", "render.json")
 
 
+LDS_PER_CU = 160 * 1024
+
+
+def resident_waves(lds_depth: int, waves_per_cu: int) -> int:
+    """Persistent traversal waves that fit on one compute unit: each holds an LDS stack of
+    lds_depth entries x 64 lanes x 8 B."""
+    return min(waves_per_cu, LDS_PER_CU // (lds_depth * 64 * 8))
+
+
 def sweep(ctx: R.Context, tris: np.ndarray, rays: np.ndarray, kind: int, widths, leaves, modes, depths, waves,
           iters: int = 10, log=print) -> list[dict]:
     n = len(rays)
@@ -49,6 +58,8 @@ def sweep(ctx: R.Context, tris: np.ndarray, rays: np.ndarray, kind: int, widths,
         for mode, depth, wpc in itertools.product(modes, depths, waves):
             if mode == 1 and wpc != waves[0]:
                 continue  # waves per CU only matters for the persistent kernel
+            if mode != 1 and resident_waves(depth, wpc) < wpc:
+                continue  # more persistent waves than the LDS stacks let reside on a CU
             ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, depth, wpc))
             sc.time_trace(kind, d_rays, n, d_out, 2)
             ms = sc.time_trace(kind, d_rays, n, d_out, iters)
@@ -60,6 +71,7 @@ def sweep(ctx: R.Context, tris: np.ndarray, rays: np.ndarray, kind: int, widths,
             else:
                 same = bool(np.array_equal(out, ref))
             rec = {"width": width, "leaf": leaf, "mode": mode, "lds_depth": depth, "waves_per_cu": wpc,
+                   "resident_waves_per_cu": resident_waves(depth, wpc) if mode != 1 else None,
                    "ms": ms, "mrays_s": n / ms / 1e3, "identical": same}
             log(json.dumps(rec))
             if not same:
@@ -158,7 +170,7 @@ def main_render(argv) -> int:
         print(f"workload {key}: {B.WORKLOADS[key]}", flush=True)
         res = render_sweep(ctx, rj, paths, **grid, log=lambda s: print("  " + s, flush=True))
         best = min(res, key=lambda x: x["ms"])
-        out["workloads"][key] = {"name": B.WORKLOADS[key], "best": best, "all": res}
+        out["workloads"][key] = {"name": B.WORKLOADS[key], "best": best}
         print(f"  best: {best}", flush=True)
         for s in scs:
             s.close()
@@ -196,7 +208,6 @@ def main() -> int:
         res = sweep(ctx, tris, r, 0 if kind == "closest" else 1, **grid, log=lambda s: print(s, flush=True))
         best = min(res, key=lambda x: x["ms"])
         out[kind] = best
-        out[kind + "_all"] = res
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     with open(args.out, "w") as f:
         json.dump(out, f, indent=1)

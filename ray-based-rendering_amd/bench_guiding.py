@@ -138,6 +138,36 @@ def run_guided(ctx, sc, ref, args, sched, log):
     return p, losses
 
 
+def c4_summary(ctx, width: int = 1920, height: int = 1080, ref_spp: int = 1024, max_spp: int = 64,
+               spp_per_call: int = 4, target: float = 0.01, schedule: dict | None = None, batch: int = 1 << 18,
+               reserve: int = 1 << 24, log=None) -> dict:
+    """Time to relMSE <= target on the C2 terrain for the plain GPU NEE/MIS tracer and the
+    guided tracer (one schedule), against a ref_spp-sample plain reference accumulated in FP64,
+    for the benchmark's JSON line."""
+    args = argparse.Namespace(max_spp=max_spp, spp_per_call=spp_per_call, batch=batch, lr=1e-3, reserve=reserve)
+    sched = {**DEFAULT_SCHEDULE, **(schedule or {})}
+    sc = R.scene_from_desc(ctx, S.terrain_c2(width, height))
+    R.render(ctx, [dict(scene=sc, spp=1, depth=8, seed=1)])
+    _, s1, s2 = R.render_moments(ctx, dict(scene=sc, spp=ref_spp, depth=8, seed=0x5EED))
+    ref = s1 / ref_spp
+    var = np.maximum(s2 / ref_spp - ref * ref, 0.0)
+    ref_err = float(np.mean(var / ref_spp / (ref * ref + EPS)))
+    plain = run_plain(ctx, sc, ref, args, log)
+    g0 = R.Guide(ctx, *sc.world_bounds(), seed=1)   # warm-up of the guided code paths
+    R.render(ctx, [dict(scene=sc, spp=1, depth=8, seed=7, guided=3, guide_depth=sched["guide_depth"])])
+    g0.train_collected(1, batch, 1e-3)
+    g0.clear_collected()
+    guided, losses = run_guided(ctx, sc, ref, args, sched, log)
+    tp, tg = time_to(plain.curve, target), time_to(guided.curve, target)
+    sc.close()
+    return {"workload": f"c4 guiding, terrain {width}x{height}, depth 8", "target_relmse": target,
+            "plain_s": tp, "guided_s": tg, "speedup": (tp / tg) if (tp and tg) else None,
+            "reference": {"spp": ref_spp, "accumulation": "fp64", "ref_relmse": ref_err},
+            "schedule": sched, "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None,
+            "final": {"spp": max_spp, "plain_relmse": plain.curve[-1][1], "guided_relmse": guided.curve[-1][1],
+                      "plain_s": plain.curve[-1][0], "guided_s": guided.curve[-1][0]}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--scene", choices=sorted(SCENES), default="terrain")

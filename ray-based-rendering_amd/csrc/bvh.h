@@ -115,5 +115,8 @@ uint32_t hostClosest(const WideBVH& b, const HostGeometry& g, const float o[3], 
                      float tmin, float tmax, float* tHit, TraceCounters* cnt);
 bool hostAny(const WideBVH& b, const HostGeometry& g, const float o[3], const float d[3], float tmin,
              float tmax, TraceCounters* cnt);
+// Node visits and primitive tests summed over a batch of rays (32 B each: o, d, tmin, tmax)
+// traced by the host traversal, closest hit or any hit; OpenMP over the rays.
+TraceCounters hostCount(const WideBVH& b, const HostGeometry& g, const float* rays, size_t n, bool any);
 
 }  // namespace rbr

@@ -632,4 +632,22 @@ void quantizeWideBVH(WideBVH& b) {
     b.quantized = true;
 }
 
+TraceCounters hostCount(const WideBVH& b, const HostGeometry& g, const float* rays, size_t n, bool any) {
+    uint64_t nodes = 0, prims = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : nodes, prims)
+    for (long long i = 0; i < (long long)n; ++i) {
+        const float* r = rays + 8 * size_t(i);
+        TraceCounters c;
+        float t;
+        if (any) hostAny(b, g, r, r + 3, r[6], r[7], &c);
+        else hostClosest(b, g, r, r + 3, r[6], r[7], &t, &c);
+        nodes += c.nodes;
+        prims += c.prims;
+    }
+    TraceCounters out;
+    out.nodes = nodes;
+    out.prims = prims;
+    return out;
+}
+
 }  // namespace rbr

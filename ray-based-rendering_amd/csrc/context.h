@@ -41,6 +41,7 @@ struct rbr_ctx {
     uint32_t renderPool = 1u << 24;    // paths in flight (queue capacity)
     int renderGraphs = 1;              // replay each round's launch sequence as a hipGraph
     int renderSort = 1;                // shade the extension queue in material order
+    uint32_t renderCensus = 0;         // rays sampled per queue and bounce for rbr_last_render_stats (0: off)
 };
 
 struct rbr_scene {

@@ -1174,6 +1174,23 @@ __global__ __launch_bounds__(256) void resetCountsKernel(uint32_t* counts, int n
     counts[2] = 0u;
 }
 
+// Traversal census: an evenly strided sample of up to k entries of a queue of *count rays
+// (entries j * max(1, count / k), j < min(k, count)), with their BVH roots, and the count.
+__global__ __launch_bounds__(256) void censusKernel(const float4* __restrict__ rays, const uint32_t* __restrict__ roots,
+                                                    const uint32_t* __restrict__ count, uint32_t k, float4* outRays,
+                                                    uint32_t* outRoots, uint32_t* outCount) {
+    const uint32_t n = *count;
+    const uint32_t stride = n > k ? n / k : 1u;
+    const uint32_t m = n < k ? n : k;
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j == 0) *outCount = n;
+    if (j >= m) return;
+    const size_t src = size_t(j) * stride;
+    outRays[2 * size_t(j)] = rays[2 * src];
+    outRays[2 * size_t(j) + 1] = rays[2 * src + 1];
+    outRoots[j] = roots[src];
+}
+
 // Adds this round's samples of one job to its film (and optional moments), in sample order.
 // ---------------------------------------------------------------- Whitted
 
@@ -1506,7 +1523,14 @@ struct RenderState {
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
+    // traversal census (rbr_set_render_census): sampled rays and roots per (bounce, queue)
+    // slot, the queue length of each slot, and the result of the last census render
+    DevBuf censusRays, censusRoots, censusCounts;
+    rbr_render_stats stats{};
+    bool statsValid = false;
 };
+
+constexpr uint32_t kCensusSlots = 512;   // (round, bounce, queue) samples per render
 
 constexpr size_t kGraphCache = 8;
 
@@ -1532,7 +1556,7 @@ void rbr::destroyRenderState(rbr_ctx* ctx) {
     for (auto* v : {&r->films, &r->momS, &r->momQ})
         for (DevBuf& b : *v) freeBuffer(b);
     for (hipEvent_t e : r->events) (void)hipEventDestroy(e);
-    for (DevBuf* b : {&r->perm, &r->sortKey, &r->bins}) freeBuffer(*b);
+    for (DevBuf* b : {&r->perm, &r->sortKey, &r->bins, &r->censusRays, &r->censusRoots, &r->censusCounts}) freeBuffer(*b);
     for (auto& g : r->graphs) (void)hipGraphExecDestroy(g.exec);
     delete r;
     ctx->render = nullptr;
@@ -1658,6 +1682,61 @@ struct SceneSlot {
 }  // namespace
 
 namespace {
+
+// Host half of the traversal census: downloads the sampled rays and counts node visits and
+// primitive tests with the host traversal of each ray's scene (found by its BVH root), each
+// slot's sample weighted by queue length / sample size.
+int censusStats(rbr_ctx* ctx, RenderState& R, const std::vector<SceneSlot>& scenes, const std::vector<uint8_t>& kind,
+                uint32_t k) {
+    const size_t slots = kind.size();
+    std::vector<uint32_t> counts(slots), roots(slots * k);
+    std::vector<float> rays(slots * k * 8);
+    hipError_t e = hipSuccess;
+    if (slots) {
+        e = hipMemcpyAsync(counts.data(), R.censusCounts.ptr, slots * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(roots.data(), R.censusRoots.ptr, slots * k * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(rays.data(), R.censusRays.ptr, slots * k * 32, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return failHip(ctx, e, "census download");
+    }
+    double n[2] = {0, 0}, nodes[2] = {0, 0}, prims[2] = {0, 0}, empty = 0;
+    uint64_t sampled[2] = {0, 0};
+    std::vector<float> group;
+    for (size_t sl = 0; sl < slots; ++sl) {
+        const uint32_t cnt = counts[sl], m = std::min(cnt, k);
+        if (!m) continue;
+        const double w = double(cnt) / double(m);
+        const int kd = kind[sl];
+        n[kd] += cnt;
+        sampled[kd] += m;
+        for (const SceneSlot& sc : scenes) {
+            group.clear();
+            for (uint32_t j = 0; j < m; ++j)
+                if (roots[sl * k + j] == sc.nodeOff)
+                    group.insert(group.end(), &rays[(sl * k + j) * 8], &rays[(sl * k + j) * 8] + 8);
+            if (group.empty() || sc.s->bvh.empty()) continue;
+            const TraceCounters c = hostCount(sc.s->bvh, sc.s->geom, group.data(), group.size() / 8, kd == 1);
+            nodes[kd] += w * double(c.nodes);
+            prims[kd] += w * double(c.prims);
+        }
+        if (kd == 1)
+            for (uint32_t j = 0; j < m; ++j) empty += roots[sl * k + j] == kEmptyD ? w : 0.0;
+    }
+    rbr_render_stats& st = R.stats;
+    st = rbr_render_stats{};
+    st.shadow_empty = uint64_t(empty + 0.5);
+    st.extension_rays = uint64_t(n[0] + 0.5);
+    st.shadow_rays = uint64_t(std::max(0.0, n[1] - empty) + 0.5);
+    st.sampled_extension = sampled[0];
+    st.sampled_shadow = sampled[1];
+    st.ext_nodes_per_ray = n[0] > 0 ? nodes[0] / n[0] : 0.0;
+    st.ext_prims_per_ray = n[0] > 0 ? prims[0] / n[0] : 0.0;
+    const double ns = n[1] - empty;
+    st.shadow_nodes_per_ray = ns > 0 ? nodes[1] / ns : 0.0;
+    st.shadow_prims_per_ray = ns > 0 ? prims[1] / ns : 0.0;
+    R.statsValid = true;
+    return RBR_OK;
+}
 
 int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momSum, double* const* momSq) {
     if (!ctx || !jobs || njobs < 1) return RBR_ERR_INVALID;
@@ -1944,6 +2023,27 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     if ((rc = prepareTraversal(ctx, maxStack))) return rc;
     if (anyGuided && (rc = guideSetAttributes(ctx))) return rc;
 
+    // ---- traversal census (profiling): sample buffers; graphs are not used meanwhile
+    const bool census = ctx->renderCensus > 0 && !anyDebug && !anyWhitted;
+    const uint32_t censusK = ctx->renderCensus;
+    std::vector<uint8_t> censusKind;   // per slot: 0 extension queue, 1 shadow queue
+    R.statsValid = false;
+    if (census) {
+        if ((rc = ensureBuffer(ctx, R.censusRays, size_t(kCensusSlots) * censusK * 32)) ||
+            (rc = ensureBuffer(ctx, R.censusRoots, size_t(kCensusSlots) * censusK * 4)) ||
+            (rc = ensureBuffer(ctx, R.censusCounts, size_t(kCensusSlots) * 4)))
+            return rc;
+    }
+    auto censusSample = [&](const float4* rays, const uint32_t* roots, const uint32_t* count, uint8_t kind) {
+        if (!census || censusKind.size() >= kCensusSlots) return;
+        const size_t slot = censusKind.size();
+        censusKind.push_back(kind);
+        hipLaunchKernelGGL(censusKernel, dim3((censusK + 255) / 256), dim3(256), 0, ctx->stream, rays, roots, count,
+                           censusK, static_cast<float4*>(R.censusRays.ptr) + slot * censusK * 2,
+                           static_cast<uint32_t*>(R.censusRoots.ptr) + slot * censusK,
+                           static_cast<uint32_t*>(R.censusCounts.ptr) + slot);
+    };
+
     // ---- rounds: give every unfinished job as many whole samples as the pool holds
     std::vector<uint32_t> done(njobs, 0);
     hipError_t e = hipSuccess;
@@ -2008,6 +2108,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
                 a.hits = q.hit;
                 a.occ = nullptr;
                 if (int r2 = launchTraversal(ctx, a, width, maxStack, false)) return r2;
+                censusSample(q.ray[cur], q.root[cur], q.counts + cur, 0);
                 if (anyDebug) {
                     hipLaunchKernelGGL(debugShadeKernel, dim3(gridPersist), dim3(256), 0, ctx->stream, djd, njobs, ML, cur, q);
                     if (anyAO) {
@@ -2056,6 +2157,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
                     b.hits = nullptr;
                     b.occ = q.socc;
                     if (int r2 = launchTraversal(ctx, b, width, maxStack, true)) return r2;
+                    censusSample(q.sray, q.sroot, q.counts + 2, 1);
                     hipLaunchKernelGGL(gatherKernel, dim3(gridPersist), dim3(256), 0, ctx->stream, maxLights, cur, q);
                 }
                 cur ^= 1;
@@ -2076,7 +2178,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
             hipError_t le = hipGetLastError();
             return le == hipSuccess ? RBR_OK : failHip(ctx, le, "render launch");
         };
-        if (!ctx->renderGraphs) {
+        if (!ctx->renderGraphs || census) {
             if ((rc = launchRound())) return rc;
         } else {
             // Every value a launch of the round depends on; the captured graph is replayed while
@@ -2150,6 +2252,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
             if (e != hipSuccess) return failHip(ctx, e, "moment download");
         }
     }
+    if (census && (rc = censusStats(ctx, R, scenes, censusKind, censusK))) return rc;
     float ms = 0.0f;
     if ((e = hipEventElapsedTime(&ms, ev[0], ev[1])) != hipSuccess) return failHip(ctx, e, "hipEventElapsedTime");
     R.lastTotalMs = ms;
@@ -2178,6 +2281,20 @@ int rbr_set_render_tuning(rbr_ctx* ctx, uint32_t pool_paths, int use_graphs, int
     ctx->renderPool = pool_paths;
     ctx->renderGraphs = use_graphs;
     ctx->renderSort = sort_materials;
+    return RBR_OK;
+}
+
+int rbr_set_render_census(rbr_ctx* ctx, uint32_t rays_per_queue) {
+    if (!ctx) return RBR_ERR_INVALID;
+    if (rays_per_queue > (1u << 20)) return fail(ctx, RBR_ERR_INVALID, "rays_per_queue must be at most 2^20");
+    ctx->renderCensus = rays_per_queue;
+    return RBR_OK;
+}
+
+int rbr_last_render_stats(rbr_ctx* ctx, rbr_render_stats* out) {
+    if (!ctx || !out) return RBR_ERR_INVALID;
+    if (!ctx->render || !ctx->render->statsValid) return fail(ctx, RBR_ERR_STATE, "no census render yet");
+    *out = ctx->render->stats;
     return RBR_OK;
 }
 

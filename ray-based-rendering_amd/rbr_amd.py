@@ -589,6 +589,29 @@ def render_moments(ctx: Context, job: dict):
     return film, s1, s2
 
 
+class RbrRenderStats(C.Structure):
+    _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("shadow_empty", C.c_uint64),
+                ("sampled_extension", C.c_uint64), ("sampled_shadow", C.c_uint64),
+                ("ext_nodes_per_ray", C.c_double), ("ext_prims_per_ray", C.c_double),
+                ("shadow_nodes_per_ray", C.c_double), ("shadow_prims_per_ray", C.c_double)]
+
+
+_sig(_lib.rbr_set_render_census, [_vp, C.c_uint32])
+_sig(_lib.rbr_last_render_stats, [_vp, C.POINTER(RbrRenderStats)])
+
+
+def set_render_census(ctx: Context, rays_per_queue: int) -> None:
+    """Traversal census of later path renders (0: off): sampled rays of every bounce's queues
+    are traced by the host traversal to count node visits and primitive tests."""
+    ctx.check(_lib.rbr_set_render_census(ctx.handle, rays_per_queue))
+
+
+def last_render_stats(ctx: Context) -> dict:
+    st = RbrRenderStats()
+    ctx.check(_lib.rbr_last_render_stats(ctx.handle, C.byref(st)))
+    return {f: getattr(st, f) for f, _ in RbrRenderStats._fields_}
+
+
 def last_render_ms(ctx: Context) -> float:
     t = C.c_double(0)
     st = (C.c_double * 4)()

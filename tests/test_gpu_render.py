@@ -342,3 +342,29 @@ def test_whitted_launch_rules(ctx):
     a, b = R.render(ctx, [dict(scene=sc, spp=4, depth=4, integrator=R.WHITTED, seed=2),
                           dict(scene=sc, spp=4, depth=4, integrator=R.WHITTED, seed=2, ambient=(0.0, 0.0, 0.0))])
     assert a.shape == b.shape == (16, 16, 3) and np.isfinite(a).all() and (a >= b - 1e-6).all() and a.mean() > b.mean()
+
+
+def test_traversal_census(ctx):
+    """rbr_set_render_census: the census render's image equals the plain render's bit for bit;
+    a depth-1 render traces exactly one camera ray per path sample; node visits and primitive
+    tests per ray are those of the host traversal (at least one node per non-empty ray)."""
+    say("case: traversal census")
+    sc = R.scene_from_desc(ctx, S.cornell_c0(64, 48))
+    job = dict(scene=sc, spp=4, depth=8, seed=3)
+    plain = R.render(ctx, [job])[0]
+    R.set_render_census(ctx, 1024)
+    try:
+        cen = R.render(ctx, [job])[0]
+        st = R.last_render_stats(ctx)
+        one = R.render(ctx, [dict(job, depth=1)])[0]
+        st1 = R.last_render_stats(ctx)
+    finally:
+        R.set_render_census(ctx, 0)
+    say(f"  {st}")
+    assert np.array_equal(plain, cen)
+    assert st1["extension_rays"] == 64 * 48 * 4
+    assert st["extension_rays"] > 64 * 48 * 4 and st["shadow_rays"] > 0
+    assert st["shadow_rays"] <= st["extension_rays"]   # one light: at most one shadow ray per vertex
+    assert st["ext_nodes_per_ray"] >= 1.0 and st["ext_prims_per_ray"] > 0
+    assert st["shadow_nodes_per_ray"] >= 1.0
+    assert np.array_equal(one, R.render(ctx, [dict(job, depth=1)])[0])
