@@ -222,7 +222,9 @@ def main() -> int:
                         "dirs uniform on S^2, tmin 1e-4, tmax inf" % len(tris),
             "bvh": f"{layout_name(args.width).upper()}, leaf<={args.leaf}, {node_b:.0f} B nodes", "nodes": st["node_count"],
             "max_depth": st["max_depth"],
-            "launch": ({k: tuning[k] for k in ("mode", "lds_depth", "waves_per_cu")} if tuning
+            "launch": ({**{k: tuning[k] for k in ("mode", "lds_depth", "waves_per_cu")},
+                        "resident_waves_per_cu": min(tuning["waves_per_cu"], 160 * 1024 // (tuning["lds_depth"] * 512))}
+                       if tuning
                        else "library defaults (persistent, LDS 8, 32 waves/CU)"),
             "seed": SEED,
         },

@@ -273,7 +273,10 @@ int rbr_time_trace(rbr_scene* scene, int kind, const rbr_ray* d_rays, size_t n, 
  * to test; mode | (m + 1) << 8 runs any-hit launches (shadow rays) in mode m and closest-hit
  * launches in mode & 0xFF; lds_depth = LDS
  * stack entries per lane (8, 10, 12, 16, 24, 32; deeper entries spill to a per-wave global
- * buffer); waves_per_cu = persistent waves launched per compute unit (1..32). */
+ * buffer); waves_per_cu = persistent waves launched per compute unit (1..32; a CU holds at
+ * most 160 KiB / (lds_depth * 512 B) of them at once, a larger grid runs its extra waves as
+ * others end).  Mode 1 needs the ray count on the host: launches whose count stays on the
+ * device (the rays of rbr_render) run mode 0 instead, for closest and for any hit. */
 int rbr_set_trace_tuning(rbr_ctx* ctx, int mode, int lds_depth, int waves_per_cu);
 
 /* Per-ray host traversal of the same flat wide BVH with the same acceptance rule — the path
@@ -302,6 +305,14 @@ int rbr_scene_set_shading(rbr_scene* scene,
  * (path_tracing_integrator.cpp:5-68) or VolumetricPathIntegrator::Li
  * (volumetric_path_integrator.cpp:6-89), for one or many jobs sharing one device ray pool. */
 int rbr_render(rbr_ctx* ctx, const rbr_job* jobs, int njobs);
+/* rbr_render with progress reports, for Integrator::render's progress callback
+ * (integrator.cpp:39-45; the reference calls it from its worker threads): progress(fraction,
+ * user) is called after each round of the launch (a round renders as many whole samples of
+ * every job as the ray pool holds) with the fraction of all path samples done, from a HIP
+ * runtime thread, in increasing order, the last time with 1; every call has returned when
+ * rbr_render_progress returns.  The function must not call into this library. */
+typedef void (*rbr_progress_fn)(float fraction, void* user);
+int rbr_render_progress(rbr_ctx* ctx, const rbr_job* jobs, int njobs, rbr_progress_fn progress, void* user);
 
 /* Per-pixel sums for variance estimates: adds each sample's radiance and its square to
  * caller arrays (3 doubles per pixel each); used by the convergence tooling. */

@@ -188,7 +188,10 @@ def _loss_and_dO(O: np.ndarray, dirs: np.ndarray, c: np.ndarray):
     r = np.exp(a - logp[:, None])                       # responsibilities
     cc = c[:, None]
     x = 2.0 * kap
-    ratio = np.where(x < 1e-4, 1.0 - 0.5 * x, x / np.expm1(np.maximum(x, 1e-300)))
+    # x / (e^x - 1), written as x e^-x / (1 - e^-x) so that large x underflows to 0 instead of
+    # overflowing expm1; the series 1 - x/2 below 1e-4.
+    xs = np.maximum(x, 1e-300)
+    ratio = np.where(x < 1e-4, 1.0 - 0.5 * x, xs * np.exp(-xs) / -np.expm1(-xs))
     dl = -cc * r * (1.0 - ratio + kap * (dcos - 1.0))
     dl = np.where((lraw < LOG_KAPPA_MIN) | (lraw > LOG_KAPPA_MAX), 0.0, dl)
     gm = (-cc * r * kap / ln[:, :, 0])[:, :, None] * (dirs[:, None, :] - dcos[:, :, None] * mu)

@@ -1,19 +1,30 @@
 // Checks of the drop-in adapters through the reference's own interfaces, in the style of the
 // reference's tests (tests/test_accelerators.cpp: a local check() counting failures, the exit
-// code is the result).  Usage: rbr_adapter_tests [device]   (device -1: host only)
+// code is the result).  Usage: rbr_adapter_tests [device [accel]]   (device -1: host only;
+// accel: the accelerator checks only)
 //
 //  * GpuBVH must return exactly what a brute-force loop returns (hit/miss and t; here also
 //    the primitive, with the lowest-id rule at exact ties), on the reference test's worlds
 //    (random spheres + triangles + axis-aligned flat triangles + an infinite plane) and ray
 //    mix (a third axis-aligned, a quarter with finite tMax); empty, single-sphere (t = 4) and
 //    rebuild cases; batched GPU queries equal the per-ray ones.
-//  * GpuPathTracingIntegrator renders a Cornell scene through Integrator::render; its image
-//    mean matches the CPU PathTracingIntegrator's within the Monte Carlo error.
+//  * GpuBVH's per-ray queries from 8 threads at once equal the single-threaded answers
+//    (integrator.cpp:12-16 calls them from OpenMP workers; std::threads here so that a
+//    ThreadSanitizer build of this program sees every access of the adapter code).
+//  * GpuPathTracingIntegrator renders through Integrator::render: on a Cornell scene with a
+//    glass sphere and on a ball of triangles with vertex normals, the mean of every 16x16
+//    block of the image matches the CPU PathTracingIntegrator's within the Monte Carlo error;
+//    a second render of the same scene reuses the device BVH (a material change is picked up
+//    without a rebuild); the progress callback reports increasing fractions ending at 1.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gpu_bvh.h"
@@ -23,7 +34,9 @@
 #include "render/geometry/quad.h"
 #include "render/geometry/sphere.h"
 #include "render/geometry/triangle.h"
+#include "render/geometry/mesh.h"
 #include "render/lighting/area_light.h"
+#include "render/lighting/point_light.h"
 #include "render/scene/scene_builder.h"
 
 using namespace render;
@@ -153,10 +166,98 @@ double filmMean(const Film& f) {
     return s / (3.0 * f.width * f.height);
 }
 
+// Block means of two films of one scene: every 16x16 block whose mean exceeds 0.02 agrees
+// within `tol` relative (the blocks average 256 pixels x spp samples).
+void compareBlocks(const Film& a, const Film& b, double tol, const std::string& label) {
+    const int bs = 16;
+    int bad = 0, used = 0;
+    double worst = 0.0;
+    for (int by = 0; by + bs <= a.height; by += bs)
+        for (int bx = 0; bx + bs <= a.width; bx += bs) {
+            double sa = 0.0, sb = 0.0;
+            for (int y = by; y < by + bs; ++y)
+                for (int x = bx; x < bx + bs; ++x) {
+                    Color ca = a.getPixel(x, y), cb = b.getPixel(x, y);
+                    sa += ca.x + ca.y + ca.z;
+                    sb += cb.x + cb.y + cb.z;
+                }
+            sa /= 3.0 * bs * bs;
+            sb /= 3.0 * bs * bs;
+            if (sb < 0.02) continue;
+            ++used;
+            const double rel = std::fabs(sa - sb) / sb;
+            worst = std::max(worst, rel);
+            if (rel > tol) ++bad;
+        }
+    std::printf("  %s: %d blocks compared, worst relative difference %.4f\n", label.c_str(), used, worst);
+    check(used > 0 && bad == 0, label + ": block means differ (" + std::to_string(bad) + " blocks)");
+}
+
+// A ball of triangles with vertex normals (the sphere's normals) on a floor, lit by a point
+// light: TriangleMesh::toPrimitives keeps the normals (mesh.cpp:29-32).
+std::unique_ptr<Scene> smoothBall(SceneBuilder& b) {
+    TriangleMesh m;
+    const int nu = 24, nv = 12;
+    for (int j = 0; j <= nv; ++j)
+        for (int i = 0; i <= nu; ++i) {
+            const float th = 3.14159265f * float(j) / nv, ph = 2.0f * 3.14159265f * float(i) / nu;
+            const Vec3 n(std::sin(th) * std::cos(ph), std::cos(th), std::sin(th) * std::sin(ph));
+            m.positions.push_back(Vec3(0.0f, 1.0f, 0.0f) + n);
+            m.normals.push_back(n);
+        }
+    for (int j = 0; j < nv; ++j)
+        for (int i = 0; i < nu; ++i) {
+            const int a = j * (nu + 1) + i, c = a + nu + 1;
+            m.indices.insert(m.indices.end(), {a, c, a + 1, a + 1, c, c + 1});
+        }
+    b.mesh(m, b.diffuse(Color(0.8f, 0.5f, 0.3f)));
+    b.quad(Vec3(-5, 0, -5), Vec3(10, 0, 0), Vec3(0, 0, 10), b.diffuse(Color(0.6f)));
+    b.pointLight(Vec3(2, 5, -3), Color(1.0f), 40.0f);
+    return b.build();
+}
+
+// Per-ray queries of one GpuBVH from 8 threads against the single-threaded answers.
+void concurrentQueries(GpuBVH& accel, const World& world, std::mt19937& rng) {
+    accel.build(world.prims);
+    std::uniform_real_distribution<float> pos(-14.0f, 14.0f), dir(-1.0f, 1.0f);
+    std::vector<Ray> rays;
+    for (int i = 0; i < 4000; ++i) {
+        Vec3 d(dir(rng), dir(rng), dir(rng));
+        if (d.length() < 1e-3f) continue;
+        rays.emplace_back(Vec3(pos(rng), pos(rng), pos(rng)), d);
+    }
+    std::vector<float> t0(rays.size());
+    std::vector<int> h0(rays.size()), p0(rays.size());
+    for (size_t i = 0; i < rays.size(); ++i) {
+        Ray r = rays[i];
+        SurfaceInteraction is;
+        h0[i] = accel.intersect(r, is) ? 1 : 0;
+        t0[i] = h0[i] ? is.t : 0.0f;
+        p0[i] = accel.intersectP(rays[i]) ? 1 : 0;
+    }
+    std::atomic<int> mism{0};
+    std::vector<std::thread> th;
+    for (int k = 0; k < 8; ++k)
+        th.emplace_back([&, k]() {
+            for (size_t i = k; i < rays.size(); i += 8) {
+                Ray r = rays[i];
+                SurfaceInteraction is;
+                const int h = accel.intersect(r, is) ? 1 : 0;
+                if (h != h0[i] || (h && is.t != t0[i]) || (accel.intersectP(rays[i]) ? 1 : 0) != p0[i]) ++mism;
+            }
+        });
+    for (auto& t : th) t.join();
+    std::printf("  concurrent per-ray queries: %zu rays x 8 threads, %d mismatches\n", rays.size(), mism.load());
+    check(mism.load() == 0, "gpu_bvh concurrent per-ray queries");
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
     const int device = argc > 1 ? std::atoi(argv[1]) : 0;
+    // "accel": the accelerator checks only (the ThreadSanitizer build runs these; its
+    // renders would run single-threaded and take minutes).
+    const bool accelOnly = argc > 2 && std::string(argv[2]) == "accel";
     std::mt19937 rng(12345);
     {
         GpuBVH empty(device);
@@ -194,24 +295,63 @@ int main(int argc, char** argv) {
         check(hit == refHit && (!hit || isect.t == tRef), "gpu_bvh rebuild");
     }
 
-    // Integrator drop-in: same image mean as the CPU integrator within the noise.
+    concurrentQueries(accel, big, rng);
+    if (accelOnly) {
+        if (failures == 0) std::printf("All adapter tests passed\n");
+        return failures == 0 ? 0 : 1;
+    }
+
+    // Integrator drop-in: the CPU integrator's image within the noise, block by block.
     SceneBuilder b1, b2;
     auto scene = cornell(b1);
     PinholeCamera cam(64, 64, Vec3(278, 278, -800), Vec3(278, 278, 0), Vec3(0, 1, 0), 40.0f);
     IntegratorConfig ic;
-    ic.samplesPerPixel = 64;
+    ic.samplesPerPixel = 1024;   // the glass sphere's caustics are heavy-tailed
     ic.maxDepth = 8;
     GpuPathTracingIntegrator gpu(ic, device, 7);
     PathTracingIntegrator cpu(ic);
     Film fg(64, 64), fc(64, 64);
-    gpu.render(*scene, cam, fg);
+    std::mutex pm;
+    std::vector<float> fractions;
+    gpu.render(*scene, cam, fg, [&](float f) {
+        std::lock_guard<std::mutex> lock(pm);
+        fractions.push_back(f);
+    });
     cpu.render(*scene, cam, fc);
     const double mg = filmMean(fg), mc = filmMean(fc);
     std::printf("integrator: %s, mean %.5f vs CPU %.5f%s%s\n", gpu.lastRenderOnGpu() ? "GPU" : "CPU fallback", mg, mc,
                 gpu.lastRenderOnGpu() ? "" : ", reason: ", gpu.lastFallbackReason().c_str());
     check(std::fabs(mg - mc) < 0.02 * mc, "gpu_pathtracing image mean");
-    if (device >= 0) check(gpu.lastRenderOnGpu(), "gpu_pathtracing ran on the GPU");
-    else check(!gpu.lastRenderOnGpu(), "host-only run falls back to the CPU integrator");
+    compareBlocks(fg, fc, 0.08, "cornell blocks");
+    check(!fractions.empty() && fractions.back() == 1.0f && std::is_sorted(fractions.begin(), fractions.end()),
+          "progress fractions increase to 1");
+    if (device >= 0) {
+        check(gpu.lastRenderOnGpu(), "gpu_pathtracing ran on the GPU");
+        // Same scene again: no rebuild.  A changed material is picked up without one.
+        Film f2(64, 64);
+        gpu.render(*scene, cam, f2);
+        check(gpu.bvhBuilds() == 1, "second render of the same scene reuses the BVH");
+        scene->materials()[0]->baseColor = Color(0.2f);
+        Film f3(64, 64);
+        gpu.render(*scene, cam, f3);
+        check(gpu.bvhBuilds() == 1 && filmMean(f3) < 0.9 * filmMean(f2), "material change without a rebuild");
+        scene->materials()[0]->baseColor = Color(0.73f);
+    } else {
+        check(!gpu.lastRenderOnGpu(), "host-only run falls back to the CPU integrator");
+    }
+
+    // Triangles with vertex normals render on the device with the interpolated normal.
+    SceneBuilder b3;
+    auto ball = smoothBall(b3);
+    PinholeCamera cam2(64, 64, Vec3(0, 2.5f, -5), Vec3(0, 0.8f, 0), Vec3(0, 1, 0), 45.0f);
+    GpuPathTracingIntegrator gpu2(ic, device, 9);
+    Film fs(64, 64), fsc(64, 64);
+    gpu2.render(*ball, cam2, fs);
+    cpu.render(*ball, cam2, fsc);
+    std::printf("smooth ball: %s, mean %.5f vs CPU %.5f\n", gpu2.lastRenderOnGpu() ? "GPU" : "CPU fallback",
+                filmMean(fs), filmMean(fsc));
+    if (device >= 0) check(gpu2.lastRenderOnGpu(), "vertex-normal triangles render on the GPU");
+    compareBlocks(fs, fsc, 0.08, "smooth ball blocks");
 
     if (failures == 0) std::printf("All adapter tests passed\n");
     return failures == 0 ? 0 : 1;

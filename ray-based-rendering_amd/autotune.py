@@ -38,8 +38,10 @@ LDS_PER_CU = 160 * 1024
 
 
 def resident_waves(lds_depth: int, waves_per_cu: int) -> int:
-    """Persistent traversal waves that fit on one compute unit: each holds an LDS stack of
-    lds_depth entries x 64 lanes x 8 B."""
+    """Persistent traversal waves that fit on one compute unit at once: each holds an LDS
+    stack of lds_depth entries x 64 lanes x 8 B.  A grid of more waves per CU than that is
+    still correct (the extra waves start as others end and take the batches left), so the
+    sweep keeps such points and records how many waves were resident."""
     return min(waves_per_cu, LDS_PER_CU // (lds_depth * 64 * 8))
 
 
@@ -58,8 +60,6 @@ def sweep(ctx: R.Context, tris: np.ndarray, rays: np.ndarray, kind: int, widths,
         for mode, depth, wpc in itertools.product(modes, depths, waves):
             if mode == 1 and wpc != waves[0]:
                 continue  # waves per CU only matters for the persistent kernel
-            if mode != 1 and resident_waves(depth, wpc) < wpc:
-                continue  # more persistent waves than the LDS stacks let reside on a CU
             ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, depth, wpc))
             sc.time_trace(kind, d_rays, n, d_out, 2)
             ms = sc.time_trace(kind, d_rays, n, d_out, iters)

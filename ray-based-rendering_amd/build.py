@@ -2,8 +2,10 @@
 
   lib/librbr.so            HIP kernels for gfx950 + the C ABI of include/rbr.h
   lib/librbr_workloads.so  seeded synthetic inputs (include/rbr_workloads.h)
-  lib/librbr_adapter.so, lib/rbr_adapter_tests
-                           GpuBVH / GpuPathTracingIntegrator against the reference headers
+  lib/librbr_adapter.so, lib/rbr_adapter_tests(_tsan), lib/librbr_reference.so
+                           GpuBVH / GpuPathTracingIntegrator against the reference headers and
+                           the reference's render_lib; lib/rbr_reference_*: the reference
+                           patched to select them (adapter/Makefile)
   ../oracle/_build, _ref   the CPU oracle (test infrastructure; built by oracle/Makefile)
 
 Run:  python ray-based-rendering_amd/build.py [--no-oracle]
@@ -72,7 +74,7 @@ def build(oracle: bool = True, jobs: int = 8) -> None:
     if oracle:
         _run([make, "-s", "-j", str(jobs), "-C", os.path.join(ROOT, "oracle"), "all"])
         # Drop-in adapters against the reference's headers (kept prebuilt where it is absent).
-        _run([make, "-s", "-C", os.path.join(PKG, "adapter"), "all"])
+        _run([make, "-s", "-j", str(jobs), "-C", os.path.join(PKG, "adapter"), "all"])
 
 
 if __name__ == "__main__":

@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <vector>
 
@@ -1738,7 +1739,31 @@ int censusStats(rbr_ctx* ctx, RenderState& R, const std::vector<SceneSlot>& scen
     return RBR_OK;
 }
 
-int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momSum, double* const* momSq) {
+// Progress reports of one render: the fraction of its path samples whose round has finished,
+// handed to the caller's function by a host callback the stream runs after each round (on a
+// runtime thread).  The items outlive every callback: renderImpl waits for the stream before
+// they go (the guard also covers early error returns).
+struct ProgressItem {
+    rbr_progress_fn fn;
+    void* user;
+    float fraction;
+};
+
+void runProgress(void* data) {
+    const ProgressItem* it = static_cast<const ProgressItem*>(data);
+    it->fn(it->fraction, it->user);
+}
+
+struct ProgressQueue {
+    hipStream_t stream = nullptr;
+    std::deque<ProgressItem> items;
+    ~ProgressQueue() {
+        if (!items.empty()) (void)hipStreamSynchronize(stream);
+    }
+};
+
+int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momSum, double* const* momSq,
+               rbr_progress_fn progress = nullptr, void* progressUser = nullptr) {
     if (!ctx || !jobs || njobs < 1) return RBR_ERR_INVALID;
     if (njobs > 64) return fail(ctx, RBR_ERR_UNSUPPORTED, "at most 64 jobs per render");
     if (int rc = useDevice(ctx)) return rc;
@@ -2055,6 +2080,8 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     }
     ev[0] = R.events[0];
     ev[1] = R.events[1];
+    ProgressQueue progressQ;
+    progressQ.stream = ctx->stream;
     if ((e = hipEventRecord(ev[0], ctx->stream)) != hipSuccess) return failHip(ctx, e, "hipEventRecord");
     while (true) {
         uint32_t used = 0;
@@ -2233,6 +2260,13 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
             if ((e = hipGraphLaunch(R.graphs.front().exec, ctx->stream)) != hipSuccess) return failHip(ctx, e, "hipGraphLaunch");
         }
         for (int j = 0; j < njobs; ++j) done[j] += take[j];
+        if (progress) {
+            uint64_t finished = 0;
+            for (int j = 0; j < njobs; ++j) finished += uint64_t(done[j]) * dj[j].pixels;
+            progressQ.items.push_back({progress, progressUser, float(double(finished) / double(totalPaths))});
+            if ((e = hipLaunchHostFunc(ctx->stream, runProgress, &progressQ.items.back())) != hipSuccess)
+                return failHip(ctx, e, "hipLaunchHostFunc");
+        }
     }
     if ((e = hipEventRecord(ev[1], ctx->stream)) != hipSuccess) return failHip(ctx, e, "hipEventRecord");
     // ---- films
@@ -2265,6 +2299,10 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
 extern "C" {
 
 int rbr_render(rbr_ctx* ctx, const rbr_job* jobs, int njobs) { return renderImpl(ctx, jobs, njobs, nullptr, nullptr); }
+
+int rbr_render_progress(rbr_ctx* ctx, const rbr_job* jobs, int njobs, rbr_progress_fn progress, void* user) {
+    return renderImpl(ctx, jobs, njobs, nullptr, nullptr, progress, user);
+}
 
 int rbr_render_moments(rbr_ctx* ctx, const rbr_job* job, double* sum_rgb, double* sumsq_rgb) {
     if (!ctx || !job || !sum_rgb || !sumsq_rgb) return RBR_ERR_INVALID;

@@ -561,11 +561,16 @@ def scene_from_desc(ctx: Context, desc, width: int = 4 | 0x100, leaf: int = 4, d
     return sc
 
 
-def render(ctx: Context, jobs: list) -> list[np.ndarray]:
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_float, C.c_void_p)
+_sig(_lib.rbr_render_progress, [_vp, C.POINTER(RbrJob), C.c_int, PROGRESS_FN, _vp])
+
+
+def render(ctx: Context, jobs: list, progress=None) -> list[np.ndarray]:
     """Integrator::render for one or many jobs in one launch sequence.  Each job is a dict
     with scene (a Scene from scene_from_desc), spp, depth and optional seed, integrator, nee,
     mis, camera, sample_offset, guided (1: sample the context's guide, 3: also record training
-    samples, 2: record training samples from plain BSDF-sampled paths), guide_depth.  Returns one (H, W, 3) float32 film per job (per-pixel mean radiance)."""
+    samples, 2: record training samples from plain BSDF-sampled paths), guide_depth.  Returns one (H, W, 3) float32 film per job (per-pixel mean radiance).
+    progress(fraction): called after each round of the launch (rbr_render_progress)."""
     arr = (RbrJob * len(jobs))()
     films = []
     for i, j in enumerate(jobs):
@@ -574,7 +579,11 @@ def render(ctx: Context, jobs: list) -> list[np.ndarray]:
         films.append(film)
         arr[i] = RbrJob(j["scene"].handle.value, c_camera(cam), _job_config(j),
                         film.ctypes.data_as(C.POINTER(C.c_float)))
-    ctx.check(_lib.rbr_render(ctx.handle, arr, len(jobs)))
+    if progress is None:
+        ctx.check(_lib.rbr_render(ctx.handle, arr, len(jobs)))
+    else:
+        cb = PROGRESS_FN(lambda f, _user: progress(f))
+        ctx.check(_lib.rbr_render_progress(ctx.handle, arr, len(jobs), cb, None))
     return films
 
 

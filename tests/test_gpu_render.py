@@ -175,6 +175,31 @@ def test_trace_fetch_modes_do_not_change_images(ctx, mode):
         assert np.array_equal(out[k], base[k]), k
 
 
+def test_split_any_hit_modes_with_graphs(ctx):
+    """Shadow rays in their own traversal mode (mode | (m + 1) << 8) with hipGraph replay and
+    material sort on: the images equal the eager single-mode render bit for bit, and changing
+    ONLY the any-hit mode between two renders on one context does not replay the graph
+    captured under the other mode (every replay equals its eager render)."""
+    say("case: split any-hit modes, graphs on")
+    a = R.scene_from_desc(ctx, S.materials_scene(40, 30))
+    f = R.scene_from_desc(ctx, S.fog_jobs_c3(scale=0.02)[3]["scene"])
+    jobs = [dict(scene=a, spp=16, depth=8, seed=3), dict(scene=f, spp=4, depth=5, seed=5, integrator=R.VOLPATH)]
+    R.set_render_tuning(ctx, 1 << 23, False, False)
+    ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
+    base = R.render(ctx, jobs)
+    try:
+        R.set_render_tuning(ctx, 1 << 23, True, True)
+        for mode in (11 | (0 + 1) << 8, 0 | (11 + 1) << 8, 11 | (0 + 1) << 8, 11 | (7 + 1) << 8, 0 | (11 + 1) << 8):
+            ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, 12, 32))
+            for rep in range(2):   # capture, then replay
+                out = R.render(ctx, jobs)
+                for k in range(len(jobs)):
+                    assert np.array_equal(out[k], base[k]), (mode, rep, k)
+    finally:
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
+        R.set_render_tuning(ctx)
+
+
 @pytest.mark.parametrize("integrator", [R.NORMAL, R.DEPTH, R.AO])
 def test_debug_integrators_match_reference(ctx, integrator):
     """NormalIntegrator / DepthIntegrator / AOIntegrator (debug_integrators.cpp) on the GPU
@@ -368,3 +393,21 @@ def test_traversal_census(ctx):
     assert st["ext_nodes_per_ray"] >= 1.0 and st["ext_prims_per_ray"] > 0
     assert st["shadow_nodes_per_ray"] >= 1.0
     assert np.array_equal(one, R.render(ctx, [dict(job, depth=1)])[0])
+
+
+def test_progress_reports(ctx):
+    """rbr_render_progress: one report per round of the launch (a small pool splits the job into
+    several rounds), increasing, the last exactly 1, and the film equals rbr_render's."""
+    say("case: progress reports")
+    sc = R.scene_from_desc(ctx, S.cornell_c0(64, 64))
+    job = dict(scene=sc, spp=64, depth=8, seed=3)
+    R.set_render_tuning(ctx, 1 << 16, False, False)
+    try:
+        seen = []
+        film = R.render(ctx, [job], progress=seen.append)
+        plain = R.render(ctx, [job])[0]
+    finally:
+        R.set_render_tuning(ctx)
+    say(f"  {len(seen)} reports: {seen[:4]} ... {seen[-1]}")
+    assert np.array_equal(film[0], plain)
+    assert len(seen) >= 4 and seen == sorted(seen) and seen[-1] == 1.0 and seen[0] > 0.0
