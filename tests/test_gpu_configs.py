@@ -223,3 +223,40 @@ def test_c4_guided_terrain_unbiased(ctx):
     z, mg, mr, bz, zi = zscores(g1.reshape(-1, 3)[px], g2.reshape(-1, 3)[px], 512, r1[px], r2[px], 512)
     say(f"  guided vs oracle subset: mean z {z.mean():+.4f}  |z|>4.5 {(np.abs(z) > 4.5).mean():.5f}  image z {zi:+.2f}")
     check_unbiased(z, mg, mr, bz, zi, frac_tail=0.01, mean_tol=0.03)
+
+
+def relvar(s1, s2, n):
+    """Mean over pixels of the luminance variance of one sample over (mean^2 + 1e-2)."""
+    m = s1.reshape(-1, 3) / n
+    v = np.maximum(s2.reshape(-1, 3) / n - m * m, 0.0)
+    lum = np.array([0.2126, 0.7152, 0.0722])
+    return float(np.mean((v @ lum) / ((m @ lum) ** 2 + 1e-2)))
+
+
+def test_c4_variance_decomposition_bounds_guiding(ctx):
+    """What guiding can gain on the C2 scene.  A guide changes only how continuation
+    directions are drawn; the camera vertex and its NEE (a depth-1 render: emission seen
+    directly, the area lights and the sun sampled at the first hit) are the same random
+    variables with or without it.  The share of the per-sample variance they carry bounds
+    the variance reduction of any guide at equal spp: relvar(full) / relvar(depth 1) is
+    about 2.3-2.6 here (97.7 % of the image is that depth-1 light), so with the guide's inference and training on top, a 2x sooner
+    time-to-relMSE is out of reach on this scene.  A trained guide (first bounce, the best
+    measured schedule) is checked to stay above that floor."""
+    say("case: C4 variance decomposition, terrain 480x270")
+    sc = R.scene_from_desc(ctx, S.terrain_c2(480, 270))
+    _, d1, d2 = R.render_moments(ctx, dict(scene=sc, spp=256, depth=1, seed=71))
+    _, f1, f2 = R.render_moments(ctx, dict(scene=sc, spp=256, depth=8, seed=72))
+    lo, hi = sc.world_bounds()
+    g = R.Guide(ctx, lo, hi, seed=0xC4)
+    g.reserve(1 << 22)
+    for it in range(8):
+        R.render(ctx, [dict(scene=sc, spp=2, depth=8, seed=900 + it, guided=2)])
+        g.train_collected(16, 1 << 16, 1e-3)
+        g.clear_collected()
+    _, g1, g2 = R.render_moments(ctx, dict(scene=sc, spp=256, depth=8, seed=73, guided=1, guide_depth=1))
+    vd, vf, vg = relvar(d1, d2, 256), relvar(f1, f2, 256), relvar(g1, g2, 256)
+    mean_d, mean_f = float(d1.mean() / 256), float(f1.mean() / 256)
+    say(f"  relvar: depth-1 {vd:.4g}, full {vf:.4g} (bound on the gain {vf / vd:.3f}), guided {vg:.4g}; "
+        f"depth-1 share of the image mean {mean_d / mean_f:.3f}")
+    assert 1.0 < vf / vd < 3.0
+    assert vg > 0.9 * vd
