@@ -38,6 +38,8 @@ def main() -> int:
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     ap.add_argument("--l2")
+    ap.add_argument("--sq", help="counter CSV of an SQ_* run (instruction counts, wave cycles)")
+    ap.add_argument("--rays", type=float, default=0.0, help="rays per launch of --kernel (per-64-ray figures)")
     ap.add_argument("--name", required=True)
     ap.add_argument("--command", required=True)
     ap.add_argument("--kernel", default="traceKernelPersistent<4, true, 8, false")
@@ -85,6 +87,26 @@ def main() -> int:
                 lines.append(f"| `{short(k)[:90]}` | {h:.0f} | {m:.0f} | {h / (h + m):.3f} |")
                 if args.kernel in k:
                     out["l2_hit_rate"] = h / (h + m)
+    if args.sq:
+        sq = counters(args.sq)
+        names = sorted({c for k, c in sq if args.kernel in k})
+        kern = [k for k, _ in sq if args.kernel in k]
+        if kern:
+            k0 = kern[0]
+            lines += ["", f"SQ counters of `{short(k0)[:90]}` (per launch):", "", "| counter | value |", "|---|---|"]
+            for c in names:
+                lines.append(f"| {c} | {sq[(k0, c)]:.4g} |")
+                out[c] = sq[(k0, c)]
+            if args.rays and (k0, "SQ_INSTS_VALU") in sq:
+                per = 64.0 / args.rays
+                out["valu_per_64_rays"] = sq[(k0, "SQ_INSTS_VALU")] * per
+                out["salu_per_64_rays"] = sq.get((k0, "SQ_INSTS_SALU"), 0.0) * per
+                act = sq.get((k0, "SQ_ACTIVE_INST_VALU"), 0.0)
+                wait = sq.get((k0, "SQ_WAIT_INST_ANY"), 0.0)
+                cyc = sq.get((k0, "SQ_WAVE_CYCLES"), 0.0)
+                lines += ["", f"VALU wave-instructions per 64 rays: {out['valu_per_64_rays']:.0f} (SALU "
+                          f"{out['salu_per_64_rays']:.0f}); waiting on data: {wait / max(cyc, 1):.0%} of wave residency; "
+                          f"VALU active cycles / wave cycles: {act * 4 / max(cyc, 1):.0%}."]
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
     with open(os.path.join(ROOT, "profiles", args.name + ".md"), "w") as fh:
         fh.write("\n".join(lines) + "\n")
