@@ -25,16 +25,19 @@
 // Moller-Trumbore test accepts the ray.
 //
 // Quantized nodes (device layout when built with kQuantFlag; W = 4: 64 B, W = 8: 128 B):
-//   dword 0-2  node origin p (the children's union box corner, float)
+//   dword 0-2  node origin p (biased, see below; float)
 //   dword 3    W = 8: biased exponents ex | ey << 8 | ez << 16: per-axis scale s = 2^(e - 127);
 //              W = 4: the x scale s_x as a float (bits ex << 23)
-//   then for axis x, y, z: W bytes of q_lo (child k in byte k), then W bytes of q_hi
+//   then for axis x, y, z: W/2 dwords, dword g holding the plane codes of children 2g and
+//   2g + 1 as bytes (q_lo[2g], q_lo[2g+1], q_hi[2g], q_hi[2g+1])
 //   then W child refs (same encodings as above), zero padding to the node size, except that
 //   W = 4 keeps s_y and s_z as floats in dwords 14 and 15 (the traversal then multiplies
 //   them with 1/d directly instead of decoding exponent bytes on every node visit)
-// The decoded box lo = fma(q_lo, s, p), hi = fma(q_hi, s, p) (one rounding, as the device
-// evaluates it) contains the padded float box of the child; empty children have q_lo = 255,
-// q_hi = 0 and ref kEmpty.  The host keeps the float nodes for its own traversal.
+// The stored origin is biased (bvh_quant.h): p = the children's lower corner - 1024 s,
+// rounded down, and a code q stands for the plane p + (1024 + q) s.  The decoded box lo =
+// fma(1024 + q_lo, s, p), hi = fma(1024 + q_hi, s, p) (one rounding, as the device evaluates
+// it) contains the padded float box of the child; empty children have q_lo = 255, q_hi = 0
+// and ref kEmpty.  The host keeps the float nodes for its own traversal.
 #pragma once
 
 #include <cstdint>

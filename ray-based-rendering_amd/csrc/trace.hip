@@ -118,7 +118,7 @@ __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArg
                     bestId = RBR_MISS_ID;
                     occ = false;
                     st.sp = 0;
-                    ref = a.nodes ? (a.rootOf ? a.rootOf[idx] : 0u) : kEmptyD;
+                    ref = a.rootOf ? a.rootOf[idx] : 0u;
                     leaf = kEmptyD;
                     alive = true;
                 }
@@ -146,12 +146,51 @@ __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArg
     }
 }
 
+// One step's fetches and tests: the primitive of the leaf in progress (lanes with primNow)
+// and one interior node (lanes with nodeNow).  Both fetches are issued before either test
+// waits for its data, so one memory latency covers the primitive and the node.  Every lane
+// fetches (lanes with nothing to test read primitive 0 and node 0, one cache line per wave), so
+// the fetched rows are defined in every lane and need no registers kept from the previous
+// step.  (Skipping a fetch that no lane of the wave needs, behind a wave-uniform branch, was
+// measured slower.)
+template <int W, bool Q, bool ANY, bool MIXED, typename Stack>
+__device__ __forceinline__ void fetchTest(const TraceArgs& a, const DRay& r, Stack& st, bool primNow, bool nodeNow,
+                                          uint32_t& prim, uint32_t& primLeft, uint32_t& ref, float& bestT,
+                                          uint32_t& bestId, bool& occ) {
+    const float4* rec = a.prims + size_t(primNow ? prim : 0u) * 3;
+    const float4 p0 = rec[0], p1 = rec[1], p2 = rec[2];
+    NodeRows<W, Q> nr;
+    loadNode<W, Q>(a.nodes, nodeNow ? ref : 0u, nr);
+    if (primNow) {
+        float t;
+        uint32_t id;
+        if (testPrimLoaded<MIXED>(a, p0, p1, p2, r, bestT, t, id)) {
+            if (ANY) occ = true;
+            else if (t < bestT || id < bestId) { bestT = t; bestId = id; }
+        }
+        ++prim;
+        --primLeft;
+        if (ANY && occ) {
+            primLeft = 0u;
+            ref = kEmptyD;
+            st.sp = 0;
+        }
+    }
+    if (nodeNow && !(ANY && occ)) {
+        float tc[W];
+        uint32_t rc[W];
+        bool hit[W];
+        children<W, Q>(nr, r, bestT, tc, rc, hit);
+        ref = descend<W, ANY>(tc, rc, hit, st);
+    }
+}
+
 // Persistent waves in single steps with per-lane ray fetch.  Each iteration of a wave, every
 // lane with an interior node to visit tests that node's children, and every lane with a leaf
 // in progress tests one primitive of it, so no lane waits for the others to reach a leaf (the
 // while-while rounds above make a lane wait until every lane holds one) and a lane whose ray
-// is done is handed a new ray as soon as p.refill lanes are idle.  The two steps run one
-// after the other (overlapping their loads doubled the registers held across the iteration).
+// is done is handed a new ray as soon as p.refill lanes are idle.  The fetches of the two steps
+// are issued together (fetchTest).
 // A lane holds at most one leaf in progress; a leaf met while one is in progress stays in ref
 // until the current one is finished.  With DEFER the primitive step is skipped in iterations
 // where fewer than p.primMin lanes hold a leaf and at least p.nodeMin lanes have an interior
@@ -170,6 +209,19 @@ __global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p
     const uint32_t n = a.nDev ? *a.nDev : a.n;
     const uint32_t lane = threadIdx.x;
     SliceWalk walk = sliceWalk(n);
+    if (!a.nodes) {   // no BVH: only the unbounded primitives, one ray per lane
+        for (uint32_t base = nextBatch(p.counter, n, walk); base < n; base = nextBatch(p.counter, n, walk)) {
+            const uint32_t idx = base + lane;
+            if (idx >= n) continue;
+            const DRay rr = loadRay(a.rays, idx);
+            float bt = rr.tmax;
+            uint32_t bid = RBR_MISS_ID;
+            bool oc = false;
+            tracePlanes<ANY>(a, rr, bt, bid, oc);
+            storeResult<ANY>(a, idx, bt, bid, oc);
+        }
+        return;
+    }
     bool alive = false, more = true, occ = false;
     uint32_t i = 0, ref = kEmptyD, prim = 0u, primLeft = 0u, bestId = RBR_MISS_ID;
     uint32_t qBase = 0u, qEnd = 0u;   // reserved, not yet assigned rays (uniform)
@@ -192,7 +244,7 @@ __global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p
                     bestId = RBR_MISS_ID;
                     occ = false;
                     st.sp = 0;
-                    ref = a.nodes ? (a.rootOf ? a.rootOf[idx] : 0u) : kEmptyD;
+                    ref = a.rootOf ? a.rootOf[idx] : 0u;
                     primLeft = 0u;
                     alive = true;
                 }
@@ -217,30 +269,9 @@ __global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p
             const uint32_t nNode = uint32_t(__popcll(__ballot(alive && ref != kEmptyD && !isLeafD(ref) ? 1 : 0)));
             doPrim = nPrim >= p.primMin || nNode < p.nodeMin;
         }
-        if (doPrim && primLeft != 0u) {
-            float t;
-            uint32_t id;
-            if (testPrim<MIXED>(a, a.prims + size_t(prim) * 3, r, bestT, t, id)) {
-                if (ANY) occ = true;
-                else if (t < bestT || id < bestId) { bestT = t; bestId = id; }
-            }
-            ++prim;
-            --primLeft;
-            if (ANY && occ) {
-                primLeft = 0u;
-                ref = kEmptyD;
-                st.sp = 0;
-            }
-        }
-        if (alive && ref != kEmptyD && !isLeafD(ref)) {
-            float tc[W];
-            uint32_t rc[W];
-            bool hit[W];
-            NodeRows<W, Q> nr;
-            loadNode<W, Q>(a.nodes, ref, nr);
-            children<W, Q>(nr, r, bestT, tc, rc, hit);
-            ref = descend<W, ANY>(tc, rc, hit, st);
-        }
+        const bool primNow = doPrim && primLeft != 0u;
+        const bool nodeNow = alive && ref != kEmptyD && !isLeafD(ref);
+        fetchTest<W, Q, ANY, MIXED>(a, r, st, primNow, nodeNow, prim, primLeft, ref, bestT, bestId, occ);
         // Settle: a leaf in ref starts when no leaf is in progress; an empty ref is refilled
         // from the stack (entries farther than the closest hit are dropped).  One loop serves
         // both, so a wave pays for the longest chain of pops of one lane, once per iteration.

@@ -35,8 +35,18 @@ struct DRay {
     float ix, iy, iz;      // 1/d with zero components replaced by +-1e-30
     float oix, oiy, oiz;   // o * inv
     bool nx, ny, nz;       // sign of inv
+    uint32_t selx, sely, selz;   // v_perm_b32 selectors of the near plane codes (childrenQ)
     float tmin, tmax;
 };
+
+// Selectors of v_perm_b32(plane dword, 0x64646464, sel) for the plane codes of a child pair
+// (dword bytes q_lo[2g], q_lo[2g+1], q_hi[2g], q_hi[2g+1]; bvh.h): the result holds two
+// half-precision numbers 1024 + q (byte q under the exponent byte 0x64), the lo codes for
+// kSelLo and the hi codes for kSelHi = kSelLo ^ kSelSwap.
+constexpr uint32_t kSelLo = 0x00050004u;
+constexpr uint32_t kSelHi = 0x00070006u;
+constexpr uint32_t kSelSwap = 0x00020002u;
+constexpr uint32_t kHalfExp = 0x64646464u;
 
 __device__ __forceinline__ float safeInv(float x) {
     float y = fabsf(x) < 1e-30f ? copysignf(1e-30f, x) : x;
@@ -54,6 +64,9 @@ __device__ __forceinline__ void setupRay(DRay& r) {
     r.nx = r.ix < 0.0f;
     r.ny = r.iy < 0.0f;
     r.nz = r.iz < 0.0f;
+    r.selx = r.nx ? kSelHi : kSelLo;
+    r.sely = r.ny ? kSelHi : kSelLo;
+    r.selz = r.nz ? kSelHi : kSelLo;
 }
 
 // A ray from its two 16-byte rows (o.xyz d.x | d.yz tmin tmax).
@@ -100,13 +113,22 @@ __device__ __forceinline__ void loadNode(const float4* nodes, uint32_t ref, Node
     for (int i = 0; i < NodeRows<W, Q>::kRows; ++i) nr.v[i] = nd[i];
 }
 
+// Half-precision number j (0: low, 1: high) of a dword, widened to float.  Used as an FMA
+// operand it becomes one v_fma_mix_f32 with op_sel.
+__device__ __forceinline__ float halfOf(uint32_t w, int j) {
+    return float(__builtin_bit_cast(_Float16, static_cast<unsigned short>(j ? w >> 16 : w & 0xFFFFu)));
+}
+
 // Children of a quantized node (bvh.h): the ray is moved into the node's grid once,
-// t(q) = q * (s * 1/d) + (p - o) * 1/d, so each of the 6 planes of a child costs one FMA on
-// its byte.  Fills entry distances (tc), refs (rc) and hit flags like the float path.
+// t(c) = c * (s * 1/d) + (p - o) * 1/d with c = 1024 + q, so each of the 6 planes of a child
+// costs one FMA.  One v_perm_b32 per axis and child pair picks the near (or far) plane codes
+// by the ray's direction sign and widens them to the half-precision numbers 1024 + q, which
+// v_fma_mix_f32 reads directly.  Fills entry distances (tc), refs (rc) and hit flags like the
+// float path.
 template <int W>
 __device__ __forceinline__ void childrenQ(const NodeRows<W, true>& nr, const DRay& r, float tfar, float* tc,
                                           uint32_t* rc, bool* hit) {
-    constexpr int Q4 = W / 4;
+    constexpr int H = W / 2;
     constexpr int NV = NodeRows<W, true>::kRows;
     uint32_t w[4 * NV];
 #pragma unroll
@@ -126,33 +148,32 @@ __device__ __forceinline__ void childrenQ(const NodeRows<W, true>& nr, const DRa
     const float bx = __builtin_fmaf(u2f(w[0]), r.ix, -r.oix);
     const float by = __builtin_fmaf(u2f(w[1]), r.iy, -r.oiy);
     const float bz = __builtin_fmaf(u2f(w[2]), r.iz, -r.oiz);
-    // near / far plane words per axis: one select covers 4 children
-    uint32_t nw[3][Q4], fw[3][Q4];
+    const uint32_t fselx = r.selx ^ kSelSwap, fsely = r.sely ^ kSelSwap, fselz = r.selz ^ kSelSwap;
 #pragma unroll
-    for (int g = 0; g < Q4; ++g) {
-        nw[0][g] = r.nx ? w[4 + 1 * Q4 + g] : w[4 + 0 * Q4 + g];
-        fw[0][g] = r.nx ? w[4 + 0 * Q4 + g] : w[4 + 1 * Q4 + g];
-        nw[1][g] = r.ny ? w[4 + 3 * Q4 + g] : w[4 + 2 * Q4 + g];
-        fw[1][g] = r.ny ? w[4 + 2 * Q4 + g] : w[4 + 3 * Q4 + g];
-        nw[2][g] = r.nz ? w[4 + 5 * Q4 + g] : w[4 + 4 * Q4 + g];
-        fw[2][g] = r.nz ? w[4 + 4 * Q4 + g] : w[4 + 5 * Q4 + g];
-    }
+    for (int g = 0; g < H; ++g) {
+        const uint32_t nxw = __builtin_amdgcn_perm(w[4 + g], kHalfExp, r.selx);
+        const uint32_t fxw = __builtin_amdgcn_perm(w[4 + g], kHalfExp, fselx);
+        const uint32_t nyw = __builtin_amdgcn_perm(w[4 + H + g], kHalfExp, r.sely);
+        const uint32_t fyw = __builtin_amdgcn_perm(w[4 + H + g], kHalfExp, fsely);
+        const uint32_t nzw = __builtin_amdgcn_perm(w[4 + 2 * H + g], kHalfExp, r.selz);
+        const uint32_t fzw = __builtin_amdgcn_perm(w[4 + 2 * H + g], kHalfExp, fselz);
 #pragma unroll
-    for (int k = 0; k < W; ++k) {
-        const int g = k >> 2, sh = 8 * (k & 3);
-        const float t0x = __builtin_fmaf(float((nw[0][g] >> sh) & 0xFFu), ax, bx);
-        const float t1x = __builtin_fmaf(float((fw[0][g] >> sh) & 0xFFu), ax, bx);
-        const float t0y = __builtin_fmaf(float((nw[1][g] >> sh) & 0xFFu), ay, by);
-        const float t1y = __builtin_fmaf(float((fw[1][g] >> sh) & 0xFFu), ay, by);
-        const float t0z = __builtin_fmaf(float((nw[2][g] >> sh) & 0xFFu), az, bz);
-        const float t1z = __builtin_fmaf(float((fw[2][g] >> sh) & 0xFFu), az, bz);
-        const float t0 = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, 0.0f));
-        const float t1 = fminf(fminf(t1x, t1y), fminf(t1z, tfar));
-        const uint32_t ch = w[4 + 6 * Q4 + k];
-        const bool h = t0 <= t1 && ch != kEmptyD;
-        tc[k] = h ? t0 : INFINITY;
-        rc[k] = ch;
-        hit[k] = h;
+        for (int j = 0; j < 2; ++j) {
+            const int k = 2 * g + j;
+            const float t0x = __builtin_fmaf(halfOf(nxw, j), ax, bx);
+            const float t1x = __builtin_fmaf(halfOf(fxw, j), ax, bx);
+            const float t0y = __builtin_fmaf(halfOf(nyw, j), ay, by);
+            const float t1y = __builtin_fmaf(halfOf(fyw, j), ay, by);
+            const float t0z = __builtin_fmaf(halfOf(nzw, j), az, bz);
+            const float t1z = __builtin_fmaf(halfOf(fzw, j), az, bz);
+            const float t0 = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, 0.0f));
+            const float t1 = fminf(fminf(t1x, t1y), fminf(t1z, tfar));
+            const uint32_t ch = w[4 + 3 * H + k];
+            const bool h = t0 <= t1 && ch != kEmptyD;
+            tc[k] = h ? t0 : INFINITY;
+            rc[k] = ch;
+            hit[k] = h;
+        }
     }
 }
 
@@ -191,11 +212,10 @@ __device__ __forceinline__ void children(const NodeRows<W, Q>& nr, const DRay& r
 }
 
 // One primitive record (3 rows: v0 / centre / corner + id, e1 / radius + kind, e2 + quad
-// normal index) against the ray.
+// normal index), already loaded, against the ray.
 template <bool MIXED>
-__device__ __forceinline__ bool testPrimRows(const TraceArgs& a, const float4* rec, const DRay& r, float tmax, float& t,
-                                             uint32_t& id) {
-    const float4 p0 = rec[0], p1 = rec[1], p2 = rec[2];
+__device__ __forceinline__ bool testPrimLoaded(const TraceArgs& a, const float4& p0, const float4& p1,
+                                               const float4& p2, const DRay& r, float tmax, float& t, uint32_t& id) {
     id = f2u(p0.w);
     if (!MIXED) return triTest(r.o, r.d, V{p0.x, p0.y, p0.z}, V{p1.x, p1.y, p1.z}, V{p2.x, p2.y, p2.z}, r.tmin, tmax, t);
     uint32_t kind = f2u(p1.w);
@@ -210,18 +230,8 @@ __device__ __forceinline__ bool testPrimRows(const TraceArgs& a, const float4* r
 template <bool MIXED>
 __device__ __forceinline__ bool testPrim(const TraceArgs& a, const float4* rec, const DRay& r, float tmax, float& t,
                                          uint32_t& id) {
-    return testPrimRows<MIXED>(a, rec, r, tmax, t, id);
-}
-
-template <typename T>
-__device__ __forceinline__ void cswap(float& ta, T& ra, float& tb, T& rb) {
-    bool s = tb < ta;
-    float tt = s ? tb : ta;
-    tb = s ? ta : tb;
-    ta = tt;
-    T rr = s ? rb : ra;
-    rb = s ? ra : rb;
-    ra = rr;
+    const float4 p0 = rec[0], p1 = rec[1], p2 = rec[2];
+    return testPrimLoaded<MIXED>(a, p0, p1, p2, r, tmax, t, id);
 }
 
 // Traversal stack: LDS rows [depth][lane] for the first LDS entries of each lane; deeper
@@ -283,32 +293,28 @@ __device__ __forceinline__ uint32_t popNext(Stack& st, float bestT) {
     return kEmptyD;
 }
 
-// The hit children of a node (entry distances tc, +inf when missed; refs rc; hit flags): for
-// closest hit they are ordered nearest first and all but the nearest are pushed; returns the
-// child to visit next, or kEmptyD when no child is hit.  The hits are counted before the
-// sort, which moves them (finite distances) ahead of the misses.
+template <typename T>
+__device__ __forceinline__ void cswapKey(uint32_t& ka, T& ra, uint32_t& kb, T& rb) {
+    bool s = kb < ka;
+    uint32_t kk = s ? kb : ka;
+    kb = s ? ka : kb;
+    ka = kk;
+    T rr = s ? rb : ra;
+    rb = s ? ra : rb;
+    ra = rr;
+}
+
+// The hit children of a node (entry distances tc; refs rc; hit flags): for closest hit they
+// are ordered nearest first and all but the nearest are pushed; returns the child to visit
+// next, or kEmptyD when no child is hit.  The closest-hit order sorts keys: the bits of the
+// entry distance (non-negative, so integer order is float order) for a hit, 0xFFFFFFFF for a
+// miss, so misses sort last and a key tells by itself whether its child is pushed.
 template <int W, bool ANY, typename Stack>
 __device__ __forceinline__ uint32_t descend(float* tc, uint32_t* rc, const bool* hit, Stack& st) {
-    int cnt = 0;
+    bool anyHit = false;
 #pragma unroll
-    for (int k = 0; k < W; ++k) cnt += hit[k] ? 1 : 0;
-    if (cnt == 0) return kEmptyD;
-    if (!ANY) {
-        // Sorting network, ascending entry distance (misses sort last as +inf).
-        if (W == 4) {
-            cswap(tc[0], rc[0], tc[1], rc[1]); cswap(tc[2], rc[2], tc[3], rc[3]);
-            cswap(tc[0], rc[0], tc[2], rc[2]); cswap(tc[1], rc[1], tc[3], rc[3]);
-            cswap(tc[1], rc[1], tc[2], rc[2]);
-        } else {
-            // Optimal 8-input network (19 comparators, depth 6).
-            constexpr int net[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6},
-                                        {3, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}, {2, 4}, {3, 5},
-                                        {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
-#pragma unroll
-            for (int c = 0; c < 19; ++c)
-                cswap(tc[net[c][0]], rc[net[c][0]], tc[net[c][1]], rc[net[c][1]]);
-        }
-    }
+    for (int k = 0; k < W; ++k) anyHit = anyHit || hit[k];
+    if (!anyHit) return kEmptyD;
     if (ANY) {
         uint32_t first = kEmptyD;
 #pragma unroll
@@ -319,9 +325,25 @@ __device__ __forceinline__ uint32_t descend(float* tc, uint32_t* rc, const bool*
         }
         return first;
     }
+    uint32_t key[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) key[k] = hit[k] ? f2u(tc[k]) : 0xFFFFFFFFu;
+    // Sorting network, ascending key.
+    if (W == 4) {
+        cswapKey(key[0], rc[0], key[1], rc[1]); cswapKey(key[2], rc[2], key[3], rc[3]);
+        cswapKey(key[0], rc[0], key[2], rc[2]); cswapKey(key[1], rc[1], key[3], rc[3]);
+        cswapKey(key[1], rc[1], key[2], rc[2]);
+    } else {
+        // Optimal 8-input network (19 comparators, depth 6).
+        constexpr int net[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6},
+                                    {3, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}, {2, 4}, {3, 5},
+                                    {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+        for (int c = 0; c < 19; ++c) cswapKey(key[net[c][0]], rc[net[c][0]], key[net[c][1]], rc[net[c][1]]);
+    }
 #pragma unroll
     for (int k = W - 1; k >= 1; --k)
-        if (k < cnt) st.push(rc[k], tc[k]);
+        if (key[k] != 0xFFFFFFFFu) st.push(rc[k], u2f(key[k]));
     return rc[0];
 }
 

@@ -72,8 +72,8 @@ def test_duplicates_and_leaf_sizes(ctx):
 @pytest.mark.parametrize("collapse", [0, R.BVH_GREEDY])
 @pytest.mark.parametrize("width", [4, 8])
 def test_structure_and_quantization(ctx, width, collapse):
-    """Quantized nodes carry the float nodes' refs and decode (fma(q, 2^e, p), one rounding,
-    as the kernels evaluate them) to boxes that contain the float child boxes; every primitive
+    """Quantized nodes carry the float nodes' refs and decode (fma(1024 + q, 2^e, p), one
+    rounding, as the kernels evaluate them) to boxes that contain the float child boxes; every primitive
     sits in exactly one leaf of at most leaf_max primitives, inside its padded child box."""
     say(f"case: device build structure width {width} collapse {collapse}")
     tris = R.icosphere(48, 100.0, 0.1, 0xC1)
@@ -99,12 +99,12 @@ def test_structure_and_quantization(ctx, width, collapse):
             e = ((nq[:, 3] >> (8 * ax)) & 0xFF).astype(np.int64) - 127
         s = np.ldexp(np.float32(1.0), e).astype(np.float32)
         for k in range(width):
-            wlo = nq[:, 4 + (2 * ax) * q4 + (k >> 2)]
-            whi = nq[:, 4 + (2 * ax + 1) * q4 + (k >> 2)]
-            qlo = ((wlo >> (8 * (k & 3))) & 0xFF).astype(np.float32)
-            qhi = ((whi >> (8 * (k & 3))) & 0xFF).astype(np.float32)
-            dlo = (qlo * s + p[:, ax]).astype(np.float32)   # q * 2^e is exact: one rounding
-            dhi = (qhi * s + p[:, ax]).astype(np.float32)
+            word = nq[:, 4 + ax * (width // 2) + (k >> 1)]   # lo, lo, hi, hi of children 2g, 2g+1
+            qlo = 1024 + ((word >> (8 * (k & 1))) & 0xFF).astype(np.float64)
+            qhi = 1024 + ((word >> (8 * (2 + (k & 1)))) & 0xFF).astype(np.float64)
+            # (1024 + q) * 2^e is exact; float64 sum then float32 rounding stands for the fma
+            dlo = (qlo * s + p[:, ax].astype(np.float64)).astype(np.float32)
+            dhi = (qhi * s + p[:, ax].astype(np.float64)).astype(np.float32)
             m = live[:, k]
             assert (dlo[m] <= fb[m, 2 * ax, k]).all() and (dhi[m] >= fb[m, 2 * ax + 1, k]).all()
     lo_t = np.minimum(np.minimum(tris[:, 0:3], tris[:, 3:6]), tris[:, 6:9])

@@ -183,6 +183,27 @@ def test_launch_tunings_identical(ctx, mode, depth, waves):
         ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
 
 
+@pytest.mark.parametrize("mode", [0, 5, 11])
+def test_planes_only_every_kernel(ctx, mode):
+    """A scene of unbounded planes only (no BVH, so the kernels take their no-node path) in
+    the persistent and single-step kernels: the same hits as brute force."""
+    say(f"case: planes only mode {mode}")
+    rng = np.random.default_rng(77 + mode)
+    planes = np.concatenate([rng.normal(size=(5, 3)), rng.uniform(-20, 20, (5, 1))], 1).astype(np.float32)
+    planes[:, :3] /= np.linalg.norm(planes[:, :3], axis=1, keepdims=True)
+    rays = worlds.make_rays(rng, 5000)
+    o = O.OracleScene(planes=planes)
+    ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, 8, 32))
+    try:
+        sc = gpu_scene(ctx, 4 | R.BVH_QUANTIZED, 4, planes=planes)
+        h = sc.trace_closest(rays)
+        assert_same(h, o.closest(rays, brute=True))
+        assert np.array_equal(sc.trace_any(rays), o.any(rays, brute=True))
+        assert (h["id"] != R.MISS).any()
+    finally:
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
+
+
 @pytest.mark.parametrize("mode", [5, 8, 9, 11])
 def test_step_modes_mixed_primitives(ctx, mode):
     """The single-step kernel (primitive steps every iteration or deferred) on a world of

@@ -150,8 +150,12 @@ def _fma32(a, b, c):
 
 @pytest.mark.parametrize("width", [4, 8])
 def test_quantized_nodes_contain_float_boxes(width):
-    """Every child box of a quantized node, decoded as lo = fma(q_lo, 2^(e-127), p),
-    contains the child's padded float box, and the child refs are the float node's."""
+    """Every child box of a quantized node, decoded as lo = fma(1024 + q_lo, 2^(e-127), p)
+    (codes of children 2g, 2g+1 in one dword per axis: lo, lo, hi, hi), contains the child's
+    padded float box, and the child refs are the float node's.  The traversal reads a code as
+    the half-precision number with bits 0x6400 | q, which is 1024 + q."""
+    codes = np.arange(256, dtype=np.uint16)
+    assert np.array_equal((codes | 0x6400).view(np.float16).astype(np.float32), 1024 + codes.astype(np.float32))
     ctx = R.Context(-1)
     rng = np.random.default_rng(width)
     tris = R.icosphere(24, 100.0, 0.1, 7)
@@ -187,9 +191,10 @@ def test_quantized_nodes_contain_float_boxes(width):
                 continue
             for a in range(3):
                 lo_f, hi_f = F[n, (2 * a) * width + k], F[n, (2 * a + 1) * width + k]
-                qlo = (int(q[4 + (2 * a) * q4 + k // 4]) >> (8 * (k % 4))) & 0xFF
-                qhi = (int(q[4 + (2 * a + 1) * q4 + k // 4]) >> (8 * (k % 4))) & 0xFF
-                lo_q, hi_q = _fma32(qlo, s[a], p[a]), _fma32(qhi, s[a], p[a])
+                word = int(q[4 + a * (width // 2) + k // 2])
+                qlo = (word >> (8 * (k % 2))) & 0xFF
+                qhi = (word >> (8 * (2 + k % 2))) & 0xFF
+                lo_q, hi_q = _fma32(1024 + qlo, s[a], p[a]), _fma32(1024 + qhi, s[a], p[a])
                 assert lo_q <= lo_f and hi_q >= hi_f, (n, k, a, lo_q, lo_f, hi_q, hi_f)
                 checked += 1
     assert checked > 1000
