@@ -237,7 +237,9 @@ def main() -> int:
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "traffic_frac": (traffic / (ms_closest * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
-            "limiter": "instruction issue (VALU), from the SQ counters in profiles/",
+            "limiter": ("vector memory pipeline: about one L1 miss per node or primitive fetch at a ~430-cycle "
+                        "L2 latency (texture data unit busy 84 %, address unit 72 %, VALU issue 56 % of SIMD "
+                        "quad-cycles), from the TA/TD/TCP/SQ counters in profiles/"),
             "bytes_per_ray": bytes_per_ray, "bytes_per_ray_layout": bytes_per_ray_layout,
             "frac_layout_bytes": rays_per_s * bytes_per_ray_layout / 1e9 / HBM_PEAK_GBS,
             "nodes_per_ray": nodes_per_ray, "prims_per_ray": prims_per_ray,

@@ -2,8 +2,8 @@
 
   python ray-based-rendering_amd/profile_summary.py --stats DIR/x_kernel_stats.csv \
       [--fetch DIR/x_counter_collection.csv] [--write DIR/x_counter_collection.csv] \
-      [--l2 DIR/x_counter_collection.csv] --name NAME --command "..." [--kernel SUBSTR] \
-      [--copy-bytes 268435456]
+      [--l2 DIR/x_counter_collection.csv] [--sq CSV] [--mem CSV] [--rays N] --name NAME \
+      --command "..." [--kernel SUBSTR] [--copy-bytes 268435456]
 
 Writes profiles/NAME.md (kernel table, counters) and, for traversal, profiles/NAME_pmc.json
 with the calibrated HBM bytes per launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are
@@ -44,6 +44,9 @@ def main() -> int:
     ap.add_argument("--command", required=True)
     ap.add_argument("--kernel", default="traceKernelPersistent<4, true, 8, false")
     ap.add_argument("--copy-bytes", type=float, default=256 * 1024 * 1024)
+    ap.add_argument("--mem", help="counter CSV of a TA/TD/TCP/TCC/GRBM run (vector memory pipeline)")
+    ap.add_argument("--xcds", type=float, default=8.0, help="GRBM_GUI_ACTIVE is summed over this many XCDs")
+    ap.add_argument("--simds", type=float, default=1024.0, help="SIMDs of the device (4 per CU)")
     args = ap.parse_args()
     rows = list(csv.DictReader(open(args.stats)))
     lines = [f"# {args.name}", "", "Command (from the repository root, on one MI355X):", "", "```", args.command, "```", "",
@@ -101,12 +104,54 @@ def main() -> int:
                 per = 64.0 / args.rays
                 out["valu_per_64_rays"] = sq[(k0, "SQ_INSTS_VALU")] * per
                 out["salu_per_64_rays"] = sq.get((k0, "SQ_INSTS_SALU"), 0.0) * per
-                act = sq.get((k0, "SQ_ACTIVE_INST_VALU"), 0.0)
-                wait = sq.get((k0, "SQ_WAIT_INST_ANY"), 0.0)
-                cyc = sq.get((k0, "SQ_WAVE_CYCLES"), 0.0)
-                lines += ["", f"VALU wave-instructions per 64 rays: {out['valu_per_64_rays']:.0f} (SALU "
-                          f"{out['salu_per_64_rays']:.0f}); waiting on data: {wait / max(cyc, 1):.0%} of wave residency; "
-                          f"VALU active cycles / wave cycles: {act * 4 / max(cyc, 1):.0%}."]
+                cyc = max(sq.get((k0, "SQ_WAVE_CYCLES"), 0.0), 1.0)   # wave quad-cycles
+                txt = [f"VALU wave-instructions per 64 rays: {out['valu_per_64_rays']:.0f} (SALU "
+                       f"{out['salu_per_64_rays']:.0f})"]
+                for c, what in (("SQ_WAIT_ANY", "waiting (dependencies, memory)"),
+                                ("SQ_WAIT_INST_ANY", "ready but waiting for issue"),
+                                ("SQ_ACTIVE_INST_ANY", "issuing")):
+                    if (k0, c) in sq:
+                        out[c + "_frac"] = sq[(k0, c)] / cyc
+                        txt.append(f"{what}: {out[c + '_frac']:.0%} of wave residency")
+                grbm = sq.get((k0, "GRBM_GUI_ACTIVE"), 0.0)
+                if grbm and (k0, "SQ_ACTIVE_INST_VALU2") in sq:
+                    quads = grbm / args.xcds / 4.0 * args.simds   # SIMD quad-cycles of the launch
+                    issue = sq[(k0, "SQ_INSTS_VALU")] - sq[(k0, "SQ_ACTIVE_INST_VALU2")]
+                    out["valu_issue_frac"] = issue / quads
+                    txt.append(f"quad-cycles of a SIMD with a VALU issue: {out['valu_issue_frac']:.0%} "
+                               "(two VALU issued together counted once)")
+                lines += ["", "; ".join(txt) + "."]
+    if args.mem:
+        mem = counters(args.mem)
+        kern = [k for k, _ in mem if args.kernel in k]
+        if kern:
+            k0 = kern[0]
+            g = lambda c: mem.get((k0, c), 0.0)
+            cyc = g("GRBM_GUI_ACTIVE") / args.xcds   # clock cycles of the launch
+            cus = args.simds / 4
+            lines += ["", f"Vector memory pipeline of `{short(k0)[:90]}` (per launch; per-CU figures over "
+                      f"{cyc:.3g} cycles):", "", "| counter | value | per CU, of the launch |", "|---|---|---|"]
+            for c in ("TA_TA_BUSY_sum", "TA_ADDR_STALLED_BY_TC_CYCLES_sum", "TD_TD_BUSY_sum", "TD_TC_STALL_sum",
+                      "TCP_PENDING_STALL_CYCLES_sum"):
+                if (k0, c) in mem:
+                    out[c + "_frac"] = g(c) / cus / max(cyc, 1.0)
+                    lines.append(f"| {c} | {g(c):.4g} | {out[c + '_frac']:.0%} |")
+            req, lat = g("TCP_TCC_READ_REQ_sum"), g("TCP_TCC_READ_REQ_LATENCY_sum")
+            acc = g("TCP_TOTAL_CACHE_ACCESSES_sum")
+            if req:
+                out["l1_to_l2_latency_cycles"] = lat / req
+                out["l1_misses_in_flight_per_cu"] = lat / cus / max(cyc, 1.0)
+                lines.append(f"| TCP_TCC_READ_REQ_sum | {req:.4g} | L1->L2 latency {lat / req:.0f} cycles, "
+                             f"{out['l1_misses_in_flight_per_cu']:.0f} in flight |")
+                if args.rays:
+                    out["l1_misses_per_ray"] = req / args.rays
+            if acc:
+                out["l1_hit_rate"] = 1.0 - req / acc
+                lines.append(f"| TCP_TOTAL_CACHE_ACCESSES_sum | {acc:.4g} | L1 hit rate {out['l1_hit_rate']:.2f} |")
+            h, m = g("TCC_HIT_sum"), g("TCC_MISS_sum")
+            if h + m:
+                out["l2_hit_rate_mem_run"] = h / (h + m)
+                lines.append(f"| TCC_HIT_sum / TCC_MISS_sum | {h:.4g} / {m:.4g} | L2 hit rate {h / (h + m):.2f} |")
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
     with open(os.path.join(ROOT, "profiles", args.name + ".md"), "w") as fh:
         fh.write("\n".join(lines) + "\n")
