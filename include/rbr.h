@@ -27,7 +27,10 @@ enum {
     RBR_ERR_HIP = 2,       /* a HIP runtime call failed; rbr_last_error() has the text   */
     RBR_ERR_STATE = 3,     /* call out of order (e.g. trace before a BVH was built)      */
     RBR_ERR_NOMEM = 4,     /* device or host allocation failed                           */
-    RBR_ERR_UNSUPPORTED = 5/* feature not handled by the GPU path (caller falls back)    */
+    RBR_ERR_UNSUPPORTED = 5,/* feature not handled by the GPU path (caller falls back)   */
+    RBR_ERR_WATCHDOG = 6   /* a ray exceeded the traversal step cap (rbr_set_trace_step_cap):
+                              the BVH is corrupt or the cap below its size; the results of the
+                              launches since the last check are incomplete                  */
 };
 
 #define RBR_MISS 0xFFFFFFFFu
@@ -278,6 +281,19 @@ int rbr_time_trace(rbr_scene* scene, int kind, const rbr_ray* d_rays, size_t n, 
  * others end).  Mode 1 needs the ray count on the host: launches whose count stays on the
  * device (the rays of rbr_render) run mode 0 instead, for closest and for any hit. */
 int rbr_set_trace_tuning(rbr_ctx* ctx, int mode, int lds_depth, int waves_per_cu);
+
+/* Watchdog of the traversal kernels (no counterpart in the reference, whose loops cannot
+ * fail but for ExponentialMedium::sampleDistance, medium.cpp:68-84).  A ray's node visits
+ * cannot exceed its BVH's node count (each node has one parent, so it is entered at most
+ * once, and a leaf holds at most 8 primitives), so every kernel counts them per ray and gives
+ * up on a ray past `cap` (checked every 16-32 loop iterations): the kernel still ends
+ * normally, the ray keeps the closest hit found so far, and a device flag is raised.  The
+ * synchronous entry points (rbr_trace_closest / _any, rbr_render, rbr_time_trace,
+ * rbr_synchronize) read and clear that flag and then return RBR_ERR_WATCHDOG; device-pointer
+ * launches report at the next rbr_synchronize.  cap = 0 (the default) uses that bound, the
+ * node count + 1 of the BVH (or node arena) being traversed; a smaller cap forces the error
+ * path (tests). */
+int rbr_set_trace_step_cap(rbr_ctx* ctx, uint32_t cap);
 
 /* Per-ray host traversal of the same flat wide BVH with the same acceptance rule — the path
  * the Accelerator adapter's per-ray intersect()/intersectP() take, since those are called

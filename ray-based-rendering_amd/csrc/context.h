@@ -37,6 +37,10 @@ struct rbr_ctx {
     int ldsDepth = rbr::kLdsDepthDefault;
     int wavesPerCu = rbr::kWavesPerCuDefault;
     rbr::DevBuf dCounter, dSpill;
+    // Traversal watchdog (rbr_set_trace_step_cap): device flag, pinned host copy, cap override.
+    rbr::DevBuf dFault;
+    uint32_t* hFault = nullptr;
+    uint32_t stepCap = 0;
     // Render tuning (rbr_set_render_tuning).
     uint32_t renderPool = 1u << 24;    // paths in flight (queue capacity)
     int renderGraphs = 1;              // replay each round's launch sequence as a hipGraph
@@ -77,6 +81,13 @@ TraceArgs sceneTraceArgs(const rbr_scene* s);
 // Launches closest (any=false) or any-hit traversal of `a` (rays/outputs filled in) for a
 // scene whose wide BVH has the given width and stack need, using the context's tuning.
 int launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack, bool any);
+// Fills in the watchdog fields of `a`: the context's cap override (else a.stepCap as the
+// caller set it, the bound of the BVH traversed) and its fault flag.
+void armWatchdog(rbr_ctx* ctx, TraceArgs& a);
+// Enqueues the read of the fault flag on the context stream; after the stream is
+// synchronized, checkFault returns RBR_ERR_WATCHDOG (and clears the flag) if it was raised.
+int queueFaultRead(rbr_ctx* ctx);
+int checkFault(rbr_ctx* ctx);
 // Allocates what launchTraversal needs for a BVH of this width and stack need, so that the
 // launches themselves allocate nothing (stream capture).
 int prepareTraversal(rbr_ctx* ctx, uint32_t maxStack);

@@ -86,6 +86,8 @@ static void freeSceneDevice(rbr_scene* s) {
 
 static TraceArgs traceArgs(const rbr_scene* s) {
     TraceArgs a{};
+    // Watchdog bound: a ray enters each node at most once.
+    a.stepCap = s->bvh.nodeCount + 1u;
     a.nodes = s->bvh.empty() ? nullptr : static_cast<const float4*>(s->dNodes.ptr);
     a.prims = static_cast<const float4*>(s->dPrims.ptr);
     a.quadN = static_cast<const float4*>(s->dQuadN.ptr);
@@ -98,8 +100,30 @@ static TraceArgs traceArgs(const rbr_scene* s) {
 
 TraceArgs rbr::sceneTraceArgs(const rbr_scene* s) { return traceArgs(s); }
 
+void rbr::armWatchdog(rbr_ctx* ctx, TraceArgs& a) {
+    if (ctx->stepCap) a.stepCap = ctx->stepCap;
+    a.fault = static_cast<uint32_t*>(ctx->dFault.ptr);
+}
+
+int rbr::queueFaultRead(rbr_ctx* ctx) {
+    if (!ctx->hFault) return RBR_OK;
+    hipError_t e = hipMemcpyAsync(ctx->hFault, ctx->dFault.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "watchdog flag read");
+}
+
+int rbr::checkFault(rbr_ctx* ctx) {
+    if (!ctx->hFault || *ctx->hFault == 0u) return RBR_OK;
+    *ctx->hFault = 0u;
+    hipError_t e = hipMemcpy(ctx->dFault.ptr, ctx->hFault, sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return failHip(ctx, e, "watchdog flag clear");
+    return fail(ctx, RBR_ERR_WATCHDOG,
+                "traversal watchdog: a ray exceeded the step cap (corrupt BVH or rbr_set_trace_step_cap below the "
+                "tree's node and primitive count); results since the last check are incomplete");
+}
+
 int rbr::launchTraversal(rbr_ctx* ctx, TraceArgs a, int width, uint32_t maxStack, bool any) {
     if (a.n == 0 && !a.nDev) return RBR_OK;
+    armWatchdog(ctx, a);
     hipError_t e;
     const int m = any && ctx->traceModeAny >= 0 ? ctx->traceModeAny : ctx->traceMode;
     if (m == 1 && !a.nDev) {
@@ -193,15 +217,23 @@ int rbr_ctx_create(int device, rbr_ctx** out) {
         ctx->name += " / ";
         ctx->name += prop.name;
     }
-    // Set-up of the no-allocation entry point: the scratch array's address on this device.
+    // Set-up of the no-allocation entry point: the scratch array's address on this device;
+    // the traversal watchdog's flag (zeroed here, so no launch path allocates or clears it).
     void* scr = nullptr;
-    if (device < kMaxDevices && noAllocScratch(ctx, &scr) != RBR_OK) {
+    bool ok = !(device < kMaxDevices && noAllocScratch(ctx, &scr) != RBR_OK);
+    ok = ok && ensureBuffer(ctx, ctx->dFault, 256) == RBR_OK &&
+         hipMemset(ctx->dFault.ptr, 0, 256) == hipSuccess &&
+         hipHostMalloc(reinterpret_cast<void**>(&ctx->hFault), sizeof(uint32_t), hipHostMallocDefault) == hipSuccess;
+    if (!ok) {
+        freeBuffer(ctx->dFault);
+        if (ctx->hFault) (void)hipHostFree(ctx->hFault);
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return RBR_ERR_HIP;
     }
+    *ctx->hFault = 0u;
     *out = ctx;
     return RBR_OK;
 }
@@ -215,6 +247,8 @@ void rbr_ctx_destroy(rbr_ctx* ctx) {
     destroyGuideState(ctx);
     freeBuffer(ctx->dCounter);
     freeBuffer(ctx->dSpill);
+    freeBuffer(ctx->dFault);
+    if (ctx->hFault) (void)hipHostFree(ctx->hFault);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     (void)hipStreamDestroy(ctx->stream);
@@ -267,8 +301,16 @@ int rbr_copy_to_host(rbr_ctx* ctx, void* dst, const void* src, size_t bytes) {
 int rbr_synchronize(rbr_ctx* ctx) {
     if (!ctx) return RBR_ERR_INVALID;
     if (int rc = useDevice(ctx)) return rc;
+    if (int rc = queueFaultRead(ctx)) return rc;
     hipError_t e = hipStreamSynchronize(ctx->stream);
-    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "hipStreamSynchronize");
+    if (e != hipSuccess) return failHip(ctx, e, "hipStreamSynchronize");
+    return checkFault(ctx);
+}
+
+int rbr_set_trace_step_cap(rbr_ctx* ctx, uint32_t cap) {
+    if (!ctx) return RBR_ERR_INVALID;
+    ctx->stepCap = cap;
+    return RBR_OK;
 }
 
 __global__ __launch_bounds__(256) void calibCopyKernel(float4* __restrict__ dst, const float4* __restrict__ src,
@@ -523,8 +565,10 @@ static int traceHost(rbr_scene* s, const rbr_ray* rays, size_t n, void* out, boo
     if ((rc = traceDevice(s, static_cast<const rbr_ray*>(s->dRays.ptr), n, s->dOut.ptr, any))) return rc;
     e = hipMemcpyAsync(out, s->dOut.ptr, outBytes, hipMemcpyDeviceToHost, ctx->stream);
     if (e != hipSuccess) return failHip(ctx, e, "result download");
+    if ((rc = queueFaultRead(ctx))) return rc;
     e = hipStreamSynchronize(ctx->stream);
-    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "trace");
+    if (e != hipSuccess) return failHip(ctx, e, "trace");
+    return checkFault(ctx);
 }
 
 int rbr_trace_closest(rbr_scene* s, const rbr_ray* rays, size_t n, rbr_hit* hits) {
@@ -558,6 +602,7 @@ int rbr_trace_closest_device_noalloc(rbr_scene* s, const rbr_ray* d_rays, size_t
     void* base = nullptr;
     if (int rc = noAllocScratch(ctx, &base)) return rc;
     TraceArgs a = traceArgs(s);
+    armWatchdog(ctx, a);
     a.rays = reinterpret_cast<const float4*>(d_rays);
     a.n = uint32_t(n);
     a.hits = reinterpret_cast<uint2*>(d_hits);
@@ -635,13 +680,14 @@ int rbr_time_trace(rbr_scene* s, int kind, const rbr_ray* d_rays, size_t n, void
         if (int rc = traceDevice(s, d_rays, n, d_out, kind == 1)) return rc;
     }
     e = hipEventRecord(ctx->ev1, ctx->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    if (e == hipSuccess && queueFaultRead(ctx) != RBR_OK) return RBR_ERR_HIP;
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return failHip(ctx, e, "timing");
     float ms = 0.0f;
     e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     if (e != hipSuccess) return failHip(ctx, e, "hipEventElapsedTime");
     *mean_ms = ms / float(iters);
-    return RBR_OK;
+    return checkFault(ctx);
 }
 
 }  // extern "C"

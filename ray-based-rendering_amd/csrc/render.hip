@@ -1912,6 +1912,9 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         ta.quadN = static_cast<const float4*>(R.arenaQuadN.ptr);
         ta.nplanes = 0;
         ta.mixed = anyMixed ? 1u : 0u;
+        // watchdog bound of the arena: a ray stays in its job's BVH, which has fewer nodes than
+        // the arena
+        ta.stepCap = nodeOff + 1u;
     }
     if (!arenaValid) {
         R.arenaKey.clear();   // stays invalid if an upload below fails
@@ -2111,6 +2114,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
         const unsigned cus = unsigned(std::max(1, ctx->cuCount));
         const unsigned gridPersist = std::min(grid, cus * 8u);
         const unsigned gridSort = std::min((used + kSortChunk - 1) / kSortChunk, cus * 4u);
+        armWatchdog(ctx, ta);   // the Whitted kernel traces inline with ta
         // The round's launch sequence; it allocates nothing, so it can be captured.
         auto launchRound = [&]() -> int {
             if (anyWhitted) {
@@ -2231,6 +2235,7 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
             put(uint64_t(ctx->traceMode)); put(uint64_t(uint32_t(ctx->traceModeAny))); put(uint64_t(ctx->ldsDepth));
             put(uint64_t(ctx->wavesPerCu));
             putp(ctx->dCounter.ptr); putp(ctx->dSpill.ptr); put(ctx->dSpill.bytes);
+            put(ta.stepCap); putp(ta.fault);
             size_t hit = R.graphs.size();
             for (size_t k = 0; k < R.graphs.size(); ++k)
                 if (R.graphs[k].key == key) hit = k;
@@ -2291,7 +2296,9 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     if ((e = hipEventElapsedTime(&ms, ev[0], ev[1])) != hipSuccess) return failHip(ctx, e, "hipEventElapsedTime");
     R.lastTotalMs = ms;
     R.stageMs[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return RBR_OK;
+    if ((rc = queueFaultRead(ctx))) return rc;
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return failHip(ctx, e, "hipStreamSynchronize");
+    return checkFault(ctx);
 }
 
 }  // namespace

@@ -31,6 +31,10 @@ struct TraceArgs {
     const uint32_t* rootOf;   // optional: per-ray root node (several BVHs in one node arena)
     uint2* hits;              // closest: (id, t bits)
     uint8_t* occ;             // any
+    // Watchdog: a ray gives up after stepCap node visits plus primitive tests (a correct tree
+    // needs at most its node and primitive counts) and sets *fault to 1.
+    uint32_t stepCap;
+    uint32_t* fault;
 };
 
 // Persistent-wave launch state: a batch counter (zeroed by a one-lane kernel launched just

@@ -97,7 +97,7 @@ __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArg
     const uint32_t lane = threadIdx.x;
     SliceWalk walk = sliceWalk(n);
     bool alive = false, more = true, occ = false;
-    uint32_t i = 0, ref = kEmptyD, leaf = kEmptyD, bestId = RBR_MISS_ID;
+    uint32_t i = 0, ref = kEmptyD, leaf = kEmptyD, bestId = RBR_MISS_ID, steps = 0u;
     uint32_t qBase = 0u, qEnd = 0u;   // reserved, not yet assigned rays (uniform)
     float bestT = 0.0f;
     DRay r{};
@@ -118,8 +118,9 @@ __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArg
                     bestId = RBR_MISS_ID;
                     occ = false;
                     st.sp = 0;
-                    ref = a.rootOf ? a.rootOf[idx] : 0u;
+                    ref = a.nodes ? (a.rootOf ? a.rootOf[idx] : 0u) : kEmptyD;
                     leaf = kEmptyD;
+                    steps = 0u;
                     alive = true;
                 }
             }
@@ -136,7 +137,8 @@ __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArg
             continue;
         }
         if (alive) {
-            if (ref != kEmptyD || leaf != kEmptyD) traceRound<W, Q, ANY, MIXED>(a, r, st, ref, leaf, bestT, bestId, occ);
+            if (ref != kEmptyD || leaf != kEmptyD)
+                traceRound<W, Q, ANY, MIXED>(a, r, st, ref, leaf, bestT, bestId, occ, steps);
             if ((ANY && occ) || (ref == kEmptyD && leaf == kEmptyD)) {
                 tracePlanes<ANY>(a, r, bestT, bestId, occ);
                 storeResult<ANY>(a, i, bestT, bestId, occ);
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArg
 template <int W, bool Q, bool ANY, bool MIXED, typename Stack>
 __device__ __forceinline__ void fetchTest(const TraceArgs& a, const DRay& r, Stack& st, bool primNow, bool nodeNow,
                                           uint32_t& prim, uint32_t& primLeft, uint32_t& ref, float& bestT,
-                                          uint32_t& bestId, bool& occ) {
+                                          uint32_t& bestId, bool& occ, uint32_t& steps) {
     const float4* rec = a.prims + size_t(primNow ? prim : 0u) * 3;
     const float4 p0 = rec[0], p1 = rec[1], p2 = rec[2];
     NodeRows<W, Q> nr;
@@ -177,6 +179,7 @@ __device__ __forceinline__ void fetchTest(const TraceArgs& a, const DRay& r, Sta
         }
     }
     if (nodeNow && !(ANY && occ)) {
+        ++steps;   // node visits of the ray (watchdog)
         float tc[W];
         uint32_t rc[W];
         bool hit[W];
@@ -223,7 +226,7 @@ __global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p
         return;
     }
     bool alive = false, more = true, occ = false;
-    uint32_t i = 0, ref = kEmptyD, prim = 0u, primLeft = 0u, bestId = RBR_MISS_ID;
+    uint32_t i = 0, ref = kEmptyD, prim = 0u, primLeft = 0u, bestId = RBR_MISS_ID, steps = 0u, iter = 0u;
     uint32_t qBase = 0u, qEnd = 0u;   // reserved, not yet assigned rays (uniform)
     float bestT = 0.0f;
     DRay r{};
@@ -246,6 +249,7 @@ __global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p
                     st.sp = 0;
                     ref = a.rootOf ? a.rootOf[idx] : 0u;
                     primLeft = 0u;
+                    steps = 0u;
                     alive = true;
                 }
             }
@@ -271,7 +275,18 @@ __global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p
         }
         const bool primNow = doPrim && primLeft != 0u;
         const bool nodeNow = alive && ref != kEmptyD && !isLeafD(ref);
-        fetchTest<W, Q, ANY, MIXED>(a, r, st, primNow, nodeNow, prim, primLeft, ref, bestT, bestId, occ);
+        fetchTest<W, Q, ANY, MIXED>(a, r, st, primNow, nodeNow, prim, primLeft, ref, bestT, bestId, occ, steps);
+        // Watchdog: a correct tree takes at most its node count of node visits per ray; past
+        // the cap the ray is abandoned (checked every 32 iterations of the wave).
+        // The ballot inside the wave-uniform test keeps this a branch taken once in 32 iterations
+        // (without it the compiler turns the check into selects on every iteration).
+        if ((++iter & 31u) == 0u && __any(alive && steps > a.stepCap ? 1 : 0)) {
+            if (alive && steps > a.stepCap) {
+                uint32_t none = kEmptyD;
+                abandonRay(a, st, ref, none);
+                primLeft = 0u;
+            }
+        }
         // Settle: a leaf in ref starts when no leaf is in progress; an empty ref is refilled
         // from the stack (entries farther than the closest hit are dropped).  One loop serves
         // both, so a wave pays for the longest chain of pops of one lane, once per iteration.
@@ -362,10 +377,12 @@ hipError_t launchL(const TraceArgs& a, const PersistArgs* p, unsigned grid, int 
 
 }  // namespace
 
-hipError_t launchTrace(const TraceArgs& a, int width, bool any, int ldsDepth, const PersistArgs* p, unsigned grid,
+hipError_t launchTrace(const TraceArgs& args, int width, bool any, int ldsDepth, const PersistArgs* p, unsigned grid,
                        hipStream_t s) {
-    if (a.n == 0 && !a.nDev) return hipSuccess;
-    if (!p && a.nDev) return hipErrorInvalidValue;
+    if (args.n == 0 && !args.nDev) return hipSuccess;
+    if (!p && args.nDev) return hipErrorInvalidValue;
+    TraceArgs a = args;
+    if (a.stepCap == 0u) a.stepCap = 0xFFFFFFFFu;   // no cap given: unbounded
     switch (ldsDepth) {
         case 8: return launchL<8>(a, p, grid, width, any, s);
         case 10: return launchL<10>(a, p, grid, width, any, s);

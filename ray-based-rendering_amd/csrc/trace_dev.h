@@ -347,6 +347,16 @@ __device__ __forceinline__ uint32_t descend(float* tc, uint32_t* rc, const bool*
     return rc[0];
 }
 
+// Watchdog: gives up on the ray (it keeps the closest hit found so far) and raises the fault
+// flag the host reads at its next synchronizing call.
+template <typename Stack>
+__device__ __forceinline__ void abandonRay(const TraceArgs& a, Stack& st, uint32_t& ref, uint32_t& leaf) {
+    if (a.fault) atomicOr(a.fault, 1u);
+    ref = kEmptyD;
+    leaf = kEmptyD;
+    st.sp = 0;
+}
+
 // One round of the traversal loop of one ray ("while-while" with postponed leaves, after Aila
 // & Laine 2009): a lane walks interior nodes until it meets a leaf, parks it, and keeps
 // walking until every lane of the wave that still traverses holds a parked leaf; then the
@@ -354,11 +364,21 @@ __device__ __forceinline__ uint32_t descend(float* tc, uint32_t* rc, const bool*
 // work.  Parking a leaf can only add node visits (culling by the closest hit comes later),
 // never change the result: the closest hit is the minimum over every primitive tested, and
 // every entry that can still hold a hit at least as close is visited.  On return, ref ==
-// kEmptyD (or an any-hit occlusion) means the ray is done.
+// kEmptyD (or an any-hit occlusion) means the ray is done.  `steps` counts the ray's node
+// visits across rounds; past a.stepCap the ray is abandoned (abandonRay), checked every 16
+// iterations of the node loop and once at the end of the round.
 template <int W, bool Q, bool ANY, bool MIXED, typename Stack>
 __device__ __forceinline__ void traceRound(const TraceArgs& a, const DRay& r, Stack& st, uint32_t& ref, uint32_t& leaf,
-                                           float& bestT, uint32_t& bestId, bool& occluded) {
+                                           float& bestT, uint32_t& bestId, bool& occluded, uint32_t& steps) {
+    uint32_t it = 0u;
     while (ref != kEmptyD && !isLeafD(ref)) {
+        ++steps;
+        if ((++it & 15u) == 0u && __any(steps > a.stepCap ? 1 : 0)) {
+            if (steps > a.stepCap) {
+                abandonRay(a, st, ref, leaf);
+                return;
+            }
+        }
         float tc[W];
         uint32_t rc[W];
         bool hit[W];
@@ -391,6 +411,7 @@ __device__ __forceinline__ void traceRound(const TraceArgs& a, const DRay& r, St
             ref = popNext<ANY>(st, bestT);
         }
     }
+    if (steps > a.stepCap) abandonRay(a, st, ref, leaf);
 }
 
 // Unbounded primitives (infinite planes), tested linearly after the BVH.
@@ -417,9 +438,9 @@ __device__ __forceinline__ void traceRay(const TraceArgs& a, const DRay& r, uint
     bestId = RBR_MISS_ID;
     occluded = false;
     if (a.nodes && root != kEmptyD) {
-        uint32_t ref = root, leaf = kEmptyD;
+        uint32_t ref = root, leaf = kEmptyD, steps = 0;
         do {
-            traceRound<W, Q, ANY, MIXED>(a, r, st, ref, leaf, bestT, bestId, occluded);
+            traceRound<W, Q, ANY, MIXED>(a, r, st, ref, leaf, bestT, bestId, occluded, steps);
         } while (!((ANY && occluded) || ref == kEmptyD));
     }
     tracePlanes<ANY>(a, r, bestT, bestId, occluded);

@@ -20,7 +20,9 @@ RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("tmin", "<f4"), ("tmax"
 HIT_DTYPE = np.dtype([("id", "<u4"), ("t", "<f4")])
 MISS = 0xFFFFFFFF
 
-ERRORS = {1: "invalid argument", 2: "HIP error", 3: "bad state", 4: "out of memory", 5: "unsupported"}
+ERRORS = {1: "invalid argument", 2: "HIP error", 3: "bad state", 4: "out of memory", 5: "unsupported",
+          6: "traversal watchdog"}
+ERR_WATCHDOG = 6
 
 
 class RbrError(RuntimeError):
@@ -86,6 +88,7 @@ _sig(_lib.rbr_trace_any, [_vp, _vp, _sz, _vp])
 _sig(_lib.rbr_trace_closest_device, [_vp, _vp, _sz, _vp])
 _sig(_lib.rbr_trace_any_device, [_vp, _vp, _sz, _vp])
 _sig(_lib.rbr_set_trace_tuning, [_vp, C.c_int, C.c_int, C.c_int])
+_sig(_lib.rbr_set_trace_step_cap, [_vp, C.c_uint32])
 _sig(_lib.rbr_time_trace, [_vp, C.c_int, _vp, _sz, _vp, C.c_int, C.POINTER(C.c_float)])
 _sig(_lib.rbr_host_trace_closest, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
 _sig(_lib.rbr_host_trace_any, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
@@ -171,6 +174,10 @@ class Context:
 
     def synchronize(self) -> None:
         self.check(_lib.rbr_synchronize(self._h))
+
+    def set_trace_step_cap(self, cap: int) -> None:
+        """Traversal watchdog cap (node visits + primitive tests per ray); 0 = the BVH's own bound."""
+        self.check(_lib.rbr_set_trace_step_cap(self._h, int(cap)))
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -411,3 +411,25 @@ def test_progress_reports(ctx):
     say(f"  {len(seen)} reports: {seen[:4]} ... {seen[-1]}")
     assert np.array_equal(film[0], plain)
     assert len(seen) >= 4 and seen == sorted(seen) and seen[-1] == 1.0 and seen[0] > 0.0
+
+
+@pytest.mark.parametrize("integrator", [0, R.WHITTED])
+def test_render_watchdog_trips_on_tiny_cap_and_recovers(ctx, integrator):
+    """With a traversal step cap far below the Cornell box's BVH bound the render's traversal
+    launches (wavefront queues through the persistent kernels, or the Whitted kernel's inline
+    traversal) give up on their rays, the kernels end normally and rbr_render reports the
+    traversal watchdog; at the default cap the same render is bit-identical to one that never
+    saw the cap (graphs captured with the tiny cap are not replayed)."""
+    say(f"case: render watchdog integrator {integrator}")
+    sc = R.scene_from_desc(ctx, S.cornell_c0(32, 32))
+    job = dict(scene=sc, spp=4, depth=4, seed=11, integrator=integrator)
+    clean = R.render(ctx, [job])[0]
+    try:
+        ctx.set_trace_step_cap(2)
+        with pytest.raises(R.RbrError) as e:
+            R.render(ctx, [job])
+        assert e.value.code == R.ERR_WATCHDOG
+    finally:
+        ctx.set_trace_step_cap(0)
+    again = R.render(ctx, [job])[0]
+    assert np.array_equal(clean, again)

@@ -183,7 +183,7 @@ def test_launch_tunings_identical(ctx, mode, depth, waves):
         ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
 
 
-@pytest.mark.parametrize("mode", [0, 5, 11])
+@pytest.mark.parametrize("mode", [0, 3, 5, 11])
 def test_planes_only_every_kernel(ctx, mode):
     """A scene of unbounded planes only (no BVH, so the kernels take their no-node path) in
     the persistent and single-step kernels: the same hits as brute force."""
@@ -201,6 +201,46 @@ def test_planes_only_every_kernel(ctx, mode):
         assert np.array_equal(sc.trace_any(rays), o.any(rays, brute=True))
         assert (h["id"] != R.MISS).any()
     finally:
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 3, 11])
+def test_watchdog_trips_on_tiny_cap_and_recovers(ctx, mode):
+    """A step cap far below what a ray needs makes every kernel family give up on its rays
+    (the kernels still end normally) and the synchronous calls report the traversal watchdog;
+    the device-pointer launch reports it at the next synchronize.  Back at the default cap
+    (the BVH's own node + primitive bound, which a correct traversal never exceeds) the hits
+    are bit-exact again."""
+    say(f"case: watchdog mode {mode}")
+    tris = R.icosphere(24, 100.0, 0.1, 0xC1)
+    lo, hi = R.mesh_bounds(tris)
+    rays = R.rays_uniform(lo, hi, 20000, 5)
+    o = O.OracleScene(tris=tris)
+    ref = o.closest(rays, brute=True)
+    ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, 8, 32))
+    sc = gpu_scene(ctx, 4 | R.BVH_QUANTIZED, 4, tris=tris)
+    try:
+        ctx.set_trace_step_cap(3)
+        for call in (sc.trace_closest, sc.trace_any):
+            with pytest.raises(R.RbrError) as e:
+                call(rays)
+            assert e.value.code == R.ERR_WATCHDOG
+        d_rays, d_hits = ctx.alloc(rays.nbytes), ctx.alloc(len(rays) * 8)
+        d_rays.upload(rays)
+        sc.trace_closest_device(d_rays, len(rays), d_hits)
+        with pytest.raises(R.RbrError) as e:
+            ctx.synchronize()
+        assert e.value.code == R.ERR_WATCHDOG
+        ctx.synchronize()   # the flag was cleared by the report
+        ctx.set_trace_step_cap(0)
+        assert_same(sc.trace_closest(rays), ref)
+        assert np.array_equal(sc.trace_any(rays), o.any(rays, brute=True))
+        sc.trace_closest_device(d_rays, len(rays), d_hits)
+        ctx.synchronize()
+        d_rays.free()
+        d_hits.free()
+    finally:
+        ctx.set_trace_step_cap(0)
         ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
 
 
