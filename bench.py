@@ -223,7 +223,7 @@ def main() -> int:
             "bvh": f"{layout_name(args.width).upper()}, leaf<={args.leaf}, {node_b:.0f} B nodes", "nodes": st["node_count"],
             "max_depth": st["max_depth"],
             "launch": ({**{k: tuning[k] for k in ("mode", "lds_depth", "waves_per_cu")},
-                        "resident_waves_per_cu": min(tuning["waves_per_cu"], 160 * 1024 // (tuning["lds_depth"] * 512))}
+                        "resident_waves_per_cu": sc.resident_waves(False)}
                        if tuning
                        else "library defaults (persistent, LDS 8, 32 waves/CU)"),
             "seed": SEED,

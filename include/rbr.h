@@ -276,11 +276,18 @@ int rbr_time_trace(rbr_scene* scene, int kind, const rbr_ray* d_rays, size_t n, 
  * to test; mode | (m + 1) << 8 runs any-hit launches (shadow rays) in mode m and closest-hit
  * launches in mode & 0xFF; lds_depth = LDS
  * stack entries per lane (8, 10, 12, 16, 24, 32; deeper entries spill to a per-wave global
- * buffer); waves_per_cu = persistent waves launched per compute unit (1..32; a CU holds at
- * most 160 KiB / (lds_depth * 512 B) of them at once, a larger grid runs its extra waves as
- * others end).  Mode 1 needs the ray count on the host: launches whose count stays on the
+ * buffer); waves_per_cu = persistent waves launched per compute unit (1..32; a CU holds as
+ * many at once as the kernel's LDS (lds_depth * 512 B of stack, plus 2 KiB of result staging
+ * in modes 5-12) and registers allow, rbr_trace_resident_waves; a larger grid runs its extra
+ * waves as others end).  Mode 1 needs the ray count on the host: launches whose count stays on the
  * device (the rays of rbr_render) run mode 0 instead, for closest and for any hit. */
 int rbr_set_trace_tuning(rbr_ctx* ctx, int mode, int lds_depth, int waves_per_cu);
+
+/* Waves per compute unit that the traversal launch of the current tuning (closest hit, or any
+ * hit when `any`) keeps resident at once on this scene's BVH layout: the persistent grid of
+ * waves_per_cu capped by what the kernel's registers and LDS allow (hipOccupancy...), or the
+ * per-CU limit alone for mode 1.  A tuning query with no counterpart in the reference. */
+int rbr_trace_resident_waves(rbr_scene* scene, int any, int* waves_per_cu);
 
 /* Watchdog of the traversal kernels (no counterpart in the reference, whose loops cannot
  * fail but for ExponentialMedium::sampleDistance, medium.cpp:68-84).  A ray's node visits

@@ -34,15 +34,6 @@ CACHE = os.path.join(PKG, "tuning", "traversal.json")
 RENDER_CACHE = os.path.join(PKG, "tuning", "render.json")
 
 
-LDS_PER_CU = 160 * 1024
-
-
-def resident_waves(lds_depth: int, waves_per_cu: int) -> int:
-    """Persistent traversal waves that fit on one compute unit at once: each holds an LDS
-    stack of lds_depth entries x 64 lanes x 8 B.  A grid of more waves per CU than that is
-    still correct (the extra waves start as others end and take the batches left), so the
-    sweep keeps such points and records how many waves were resident."""
-    return min(waves_per_cu, LDS_PER_CU // (lds_depth * 64 * 8))
 
 
 def sweep(ctx: R.Context, tris: np.ndarray, rays: np.ndarray, kind: int, widths, leaves, modes, depths, waves,
@@ -62,7 +53,8 @@ def sweep(ctx: R.Context, tris: np.ndarray, rays: np.ndarray, kind: int, widths,
                 continue  # waves per CU only matters for the persistent kernel
             ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, depth, wpc))
             sc.time_trace(kind, d_rays, n, d_out, 2)
-            ms = sc.time_trace(kind, d_rays, n, d_out, iters)
+            # best of three timed runs: single runs of a few launches vary by 2-3 %
+            ms = min(sc.time_trace(kind, d_rays, n, d_out, iters) for _ in range(3))
             out = np.empty(n, R.HIT_DTYPE if kind == 0 else np.uint8)
             d_out.download(out)
             same = True
@@ -70,8 +62,10 @@ def sweep(ctx: R.Context, tris: np.ndarray, rays: np.ndarray, kind: int, widths,
                 ref = out.copy()
             else:
                 same = bool(np.array_equal(out, ref))
+            # Waves resident at once (registers and LDS of the kernel); a larger grid is still
+            # correct (its extra waves start as others end), so such points stay in the sweep.
             rec = {"width": width, "leaf": leaf, "mode": mode, "lds_depth": depth, "waves_per_cu": wpc,
-                   "resident_waves_per_cu": resident_waves(depth, wpc) if mode != 1 else None,
+                   "resident_waves_per_cu": sc.resident_waves(kind == 1) if mode != 1 else None,
                    "ms": ms, "mrays_s": n / ms / 1e3, "identical": same}
             log(json.dumps(rec))
             if not same:

@@ -313,6 +313,20 @@ int rbr_set_trace_step_cap(rbr_ctx* ctx, uint32_t cap) {
     return RBR_OK;
 }
 
+int rbr_trace_resident_waves(rbr_scene* s, int any, int* waves_per_cu) {
+    if (!s || !waves_per_cu) return RBR_ERR_INVALID;
+    rbr_ctx* ctx = s->ctx;
+    if (int rc = useDevice(ctx)) return rc;
+    if (!s->built) return fail(ctx, RBR_ERR_STATE, "rbr_trace_resident_waves before rbr_scene_build_bvh");
+    const int m = any && ctx->traceModeAny >= 0 ? ctx->traceModeAny : ctx->traceMode;
+    const bool mixed = (s->geom.sphIds.size() + s->geom.quadIds.size()) > 0;
+    int blocks = 0;
+    hipError_t e = traceResidentWaves(s->bvh.code(), any != 0, mixed, ctx->ldsDepth, m, &blocks);
+    if (e != hipSuccess) return failHip(ctx, e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    *waves_per_cu = m == 1 ? blocks : std::min(blocks, ctx->wavesPerCu);
+    return RBR_OK;
+}
+
 __global__ __launch_bounds__(256) void calibCopyKernel(float4* __restrict__ dst, const float4* __restrict__ src,
                                                        size_t n) {
     for (size_t i = blockIdx.x * size_t(256) + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) dst[i] = src[i];

@@ -64,4 +64,8 @@ struct PersistArgs {
 hipError_t launchTrace(const TraceArgs& a, int width, bool any, int ldsDepth, const PersistArgs* p, unsigned grid,
                        hipStream_t s);
 
+// Waves of the kernel that launchTrace would run for (width, any, mixed, ldsDepth, tuning mode)
+// that one compute unit holds at once (its registers and LDS; one wave per workgroup).
+hipError_t traceResidentWaves(int width, bool any, bool mixed, int ldsDepth, int mode, int* wavesPerCu);
+
 }  // namespace rbr

@@ -18,6 +18,9 @@ namespace {
 
 using namespace tdev;
 
+// Result staging of the step kernel: 64-ray ranges held in LDS at once per wave (512 B each).
+constexpr uint32_t kStageSlots = 4;
+
 // Work distribution of the persistent kernels: the 64-ray batches are split into kSlices
 // equal slices, each with its own counter (kSliceStride words apart), and a wave drains the
 // slice of its workgroup index, then moves on to the following ones until every slice is
@@ -150,19 +153,23 @@ __global__ __launch_bounds__(64) void traceKernelDynamic(TraceArgs a, PersistArg
 
 // One step's fetches and tests: the primitive of the leaf in progress (lanes with primNow)
 // and one interior node (lanes with nodeNow).  Both fetches are issued before either test
-// waits for its data, so one memory latency covers the primitive and the node.  Every lane
-// fetches (lanes with nothing to test read primitive 0 and node 0, one cache line per wave), so
-// the fetched rows are defined in every lane and need no registers kept from the previous
-// step.  (Skipping a fetch that no lane of the wave needs, behind a wave-uniform branch, was
-// measured slower.)
+// waits for its data, so one memory latency covers the primitive and the node.  Only the lanes
+// that test something fetch (the loads run under the lane mask): the address and data units
+// then process those lanes alone, which measured 2-8 % faster than every lane fetching
+// primitive 0 and node 0 in its place.
 template <int W, bool Q, bool ANY, bool MIXED, typename Stack>
 __device__ __forceinline__ void fetchTest(const TraceArgs& a, const DRay& r, Stack& st, bool primNow, bool nodeNow,
                                           uint32_t& prim, uint32_t& primLeft, uint32_t& ref, float& bestT,
                                           uint32_t& bestId, bool& occ, uint32_t& steps) {
-    const float4* rec = a.prims + size_t(primNow ? prim : 0u) * 3;
-    const float4 p0 = rec[0], p1 = rec[1], p2 = rec[2];
+    float4 p0, p1, p2;
+    if (primNow) {
+        const float4* rec = a.prims + size_t(prim) * 3;
+        p0 = rec[0];
+        p1 = rec[1];
+        p2 = rec[2];
+    }
     NodeRows<W, Q> nr;
-    loadNode<W, Q>(a.nodes, nodeNow ? ref : 0u, nr);
+    if (nodeNow) loadNode<W, Q>(a.nodes, ref, nr);
     if (primNow) {
         float t;
         uint32_t id;
@@ -185,6 +192,25 @@ __device__ __forceinline__ void fetchTest(const TraceArgs& a, const DRay& r, Sta
         bool hit[W];
         children<W, Q>(nr, r, bestT, tc, rc, hit);
         ref = descend<W, ANY>(tc, rc, hit, st);
+    }
+}
+
+// Settle: a leaf in ref starts when no leaf is in progress; an empty ref is refilled from the
+// stack (entries farther than the closest hit are dropped).  One loop serves both, so a wave
+// pays for the longest chain of pops of one lane, once per iteration.
+template <bool ANY, typename Stack>
+__device__ __forceinline__ void settle(Stack& st, bool alive, uint32_t& ref, uint32_t& prim, uint32_t& primLeft,
+                                       float bestT) {
+    if (!alive) return;
+    while (true) {
+        if (primLeft == 0u && ref != kEmptyD && isLeafD(ref)) {
+            prim = leafStartD(ref);
+            primLeft = leafCountD(ref);
+            ref = kEmptyD;
+        }
+        if (ref != kEmptyD || st.sp == 0) break;
+        const uint2 e = st.pop();
+        if (ANY || u2f(e.y) <= bestT) ref = e.x;
     }
 }
 
@@ -230,18 +256,33 @@ __global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p
     uint32_t qBase = 0u, qEnd = 0u;   // reserved, not yet assigned rays (uniform)
     float bestT = 0.0f;
     DRay r{};
+    // Results staged in LDS per 64-ray range (kStageSlots ranges at a time) and written out
+    // together once the range's last ray is done: the lanes of a wave finish rays of a range
+    // at scattered times, and 8-byte (or 1-byte) writes spread over that time left the result
+    // lines partly written in L2 and written back several times (4.5x the result bytes for
+    // closest hit, 20x for any hit).  stageLeft[s] counts the rays of slot s not yet done; 0
+    // means free.  A range that finds its slot still busy writes its results directly, so no
+    // lane ever waits for a slot.
+    __shared__ uint2 stage[kStageSlots * 64];
+    __shared__ uint32_t stageLeft[kStageSlots];
+    if (lane < kStageSlots) stageLeft[lane] = 0u;
+    uint32_t slot = 0u, curSlot = 0u, rangeSeq = 0u;   // slot of the lane's ray, of the current range
     while (true) {
         const uint64_t idle = __ballot(alive ? 0 : 1);
         const uint32_t nIdle = uint32_t(__popcll(idle));
         if (more && nIdle >= p.refill) {
             const uint32_t avail = qEnd - qBase;
             uint32_t base2 = 0u;
+            const uint32_t ns = rangeSeq % kStageSlots;
+            const bool nsFree = __builtin_amdgcn_readfirstlane(stageLeft[ns]) == 0u;
+            const uint32_t newSlot = nsFree ? ns : kStageSlots;   // kStageSlots: written directly
             if (avail < nIdle) base2 = nextBatch(p.counter, n, walk);
             if (!alive) {
                 const uint32_t rank = uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
                 const uint32_t idx = rank < avail ? qBase + rank : base2 + (rank - avail);
                 if (idx < n) {
                     i = idx;
+                    slot = rank < avail ? curSlot : newSlot;
                     r = loadRay(a.rays, idx);
                     bestT = r.tmax;
                     bestId = RBR_MISS_ID;
@@ -256,7 +297,15 @@ __global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p
             if (avail < nIdle) {
                 qBase = base2 + (nIdle - avail);
                 qEnd = base2 + 64u;
-                if (base2 >= n) more = false;   // every slice drained
+                if (base2 >= n) {
+                    more = false;   // every slice drained
+                } else {
+                    // Ranges start at multiples of 64 (slices hold whole batches), so a ray's
+                    // row in its slot is i & 63.
+                    if (lane == 0 && newSlot < kStageSlots) stageLeft[ns] = min(64u, n - base2);
+                    curSlot = newSlot;
+                    ++rangeSeq;
+                }
             } else {
                 qBase += nIdle;
             }
@@ -287,25 +336,30 @@ __global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p
                 primLeft = 0u;
             }
         }
-        // Settle: a leaf in ref starts when no leaf is in progress; an empty ref is refilled
-        // from the stack (entries farther than the closest hit are dropped).  One loop serves
-        // both, so a wave pays for the longest chain of pops of one lane, once per iteration.
-        if (alive) {
-            while (true) {
-                if (primLeft == 0u && ref != kEmptyD && isLeafD(ref)) {
-                    prim = leafStartD(ref);
-                    primLeft = leafCountD(ref);
-                    ref = kEmptyD;
-                }
-                if (ref != kEmptyD || st.sp == 0) break;
-                const uint2 e = st.pop();
-                if (ANY || u2f(e.y) <= bestT) ref = e.x;
-            }
-        }
+        settle<ANY>(st, alive, ref, prim, primLeft, bestT);
+        bool last = false;
         if (alive && ref == kEmptyD && primLeft == 0u) {
             tracePlanes<ANY>(a, r, bestT, bestId, occ);
-            storeResult<ANY>(a, i, bestT, bestId, occ);
+            if (slot < kStageSlots) {
+                stage[slot * 64u + (i & 63u)] = ANY ? make_uint2(occ ? 1u : 0u, 0u)
+                                                    : make_uint2(bestId, bestId == RBR_MISS_ID ? 0u : f2u(bestT));
+                last = atomicSub(&stageLeft[slot], 1u) == 1u;
+            } else {
+                storeResult<ANY>(a, i, bestT, bestId, occ);
+            }
             alive = false;
+        }
+        // Write out every range whose last ray finished in this iteration (64 lanes, one row each).
+        for (uint64_t fl = __ballot(last ? 1 : 0); fl; fl &= fl - 1ull) {
+            const int l = __builtin_ctzll(fl);
+            const uint32_t s = __builtin_amdgcn_readlane(slot, l);
+            const uint32_t b = __builtin_amdgcn_readlane(i, l) & ~63u;
+            __syncthreads();   // the workgroup is this one wave: orders the rows written above
+            const uint2 e = stage[s * 64u + lane];
+            if (b + lane < n) {
+                if (ANY) a.occ[b + lane] = uint8_t(e.x);
+                else a.hits[b + lane] = e;
+            }
         }
     }
 }
@@ -375,7 +429,44 @@ hipError_t launchL(const TraceArgs& a, const PersistArgs* p, unsigned grid, int 
     }
 }
 
+template <int W, bool Q, int LDS, bool ANY, bool MIXED>
+hipError_t residentT(int mode, int* blocks) {
+    const void* f;
+    if (mode >= 9) f = reinterpret_cast<const void*>(&traceKernelStep<W, Q, LDS, ANY, MIXED, true>);
+    else if (mode >= 5) f = reinterpret_cast<const void*>(&traceKernelStep<W, Q, LDS, ANY, MIXED, false>);
+    else if (mode >= 2) f = reinterpret_cast<const void*>(&traceKernelDynamic<W, Q, LDS, ANY, MIXED>);
+    else if (mode == 0) f = reinterpret_cast<const void*>(&traceKernelPersistent<W, Q, LDS, ANY, MIXED>);
+    else f = reinterpret_cast<const void*>(&traceKernelLane<W, Q, LDS, ANY, MIXED>);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, f, 64, 0);
+}
+
+template <int LDS>
+hipError_t residentL(int width, bool any, bool mixed, int mode, int* blocks) {
+#define RBR_RES(W, Q)                                                                             \
+    return any ? (mixed ? residentT<W, Q, LDS, true, true>(mode, blocks) : residentT<W, Q, LDS, true, false>(mode, blocks)) \
+               : (mixed ? residentT<W, Q, LDS, false, true>(mode, blocks) : residentT<W, Q, LDS, false, false>(mode, blocks))
+    switch (width) {
+        case 4: RBR_RES(4, false);
+        case 8: RBR_RES(8, false);
+        case 4 | kQuantFlag: RBR_RES(4, true);
+        case 8 | kQuantFlag: RBR_RES(8, true);
+        default: return hipErrorInvalidValue;
+    }
+#undef RBR_RES
+}
+
 }  // namespace
+
+hipError_t traceResidentWaves(int width, bool any, bool mixed, int ldsDepth, int mode, int* wavesPerCu) {
+    switch (ldsDepth) {
+        case 8: return residentL<8>(width, any, mixed, mode, wavesPerCu);
+        case 10: return residentL<10>(width, any, mixed, mode, wavesPerCu);
+        case 12: return residentL<12>(width, any, mixed, mode, wavesPerCu);
+        case 24: return residentL<24>(width, any, mixed, mode, wavesPerCu);
+        case 32: return residentL<32>(width, any, mixed, mode, wavesPerCu);
+        default: return residentL<16>(width, any, mixed, mode, wavesPerCu);
+    }
+}
 
 hipError_t launchTrace(const TraceArgs& args, int width, bool any, int ldsDepth, const PersistArgs* p, unsigned grid,
                        hipStream_t s) {

@@ -89,6 +89,7 @@ _sig(_lib.rbr_trace_closest_device, [_vp, _vp, _sz, _vp])
 _sig(_lib.rbr_trace_any_device, [_vp, _vp, _sz, _vp])
 _sig(_lib.rbr_set_trace_tuning, [_vp, C.c_int, C.c_int, C.c_int])
 _sig(_lib.rbr_set_trace_step_cap, [_vp, C.c_uint32])
+_sig(_lib.rbr_trace_resident_waves, [_vp, C.c_int, C.POINTER(C.c_int)])
 _sig(_lib.rbr_time_trace, [_vp, C.c_int, _vp, _sz, _vp, C.c_int, C.POINTER(C.c_float)])
 _sig(_lib.rbr_host_trace_closest, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
 _sig(_lib.rbr_host_trace_any, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
@@ -321,6 +322,12 @@ class Scene:
         ms = C.c_float(0.0)
         self.ctx.check(_lib.rbr_time_trace(self._h, kind, d_rays.ptr, n, d_out.ptr, iters, C.byref(ms)))
         return ms.value
+
+    def resident_waves(self, any_hit: bool = False) -> int:
+        """Traversal waves per CU resident at once under the context's current tuning."""
+        w = C.c_int(0)
+        self.ctx.check(_lib.rbr_trace_resident_waves(self._h, 1 if any_hit else 0, C.byref(w)))
+        return w.value
 
     def host_trace_closest(self, rays: np.ndarray):
         rays = np.ascontiguousarray(rays, RAY_DTYPE)

@@ -293,3 +293,62 @@ def test_quads_bit_exact(ctx, width):
             assert_same(gpu_scene(ctx, width, 4, **mixed_kw).trace_closest(rays), ref_mc)
         finally:
             ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
+
+
+@pytest.mark.parametrize("mode", [7, 11])
+def test_step_kernel_staged_results_every_count(ctx, mode):
+    """The step kernels stage each 64-ray range's results in LDS and write the range out when
+    its last ray is done (a range that finds its slot busy writes directly).  Ray counts below,
+    at and around multiples of 64, a device-memory count, and a grid of one wave (every range
+    of the launch through one wave's slots): every result is written, bit-exact, and nothing
+    past the count is touched."""
+    say(f"case: staged results mode {mode}")
+    tris = R.icosphere(24, 100.0, 0.1, 0xC1)
+    lo, hi = R.mesh_bounds(tris)
+    rays = R.rays_uniform(lo, hi, 4097, 9)
+    o = O.OracleScene(tris=tris)
+    ref_c, ref_a = o.closest(rays, brute=True), o.any(rays, brute=True)
+    sc = gpu_scene(ctx, 4 | R.BVH_QUANTIZED, 4, tris=tris)
+    try:
+        for waves in (32, 1):
+            ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, 8, waves))
+            for n in (1, 5, 63, 64, 65, 127, 128, 1000, 4097):
+                assert_same(sc.trace_closest(rays[:n]), ref_c[:n])
+                assert np.array_equal(sc.trace_any(rays[:n]), ref_a[:n])
+            # device pointers with a guard region after the count
+            n = 1000
+            d_rays, d_hits = ctx.alloc(rays.nbytes), ctx.alloc((n + 64) * 8)
+            d_rays.upload(rays)
+            guard = np.full((n + 64) * 8, 0xAB, np.uint8)
+            d_hits.upload(guard)
+            sc.trace_closest_device(d_rays, n, d_hits)
+            ctx.synchronize()
+            out = np.empty((n + 64) * 8, np.uint8)
+            d_hits.download(out)
+            assert_same(out[: n * 8].view(R.HIT_DTYPE), ref_c[:n])
+            assert (out[n * 8:] == 0xAB).all()
+            d_rays.free()
+            d_hits.free()
+    finally:
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
+
+
+def test_resident_waves_query(ctx):
+    """rbr_trace_resident_waves reports the waves per CU the launch keeps resident (the HIP
+    occupancy calculator on the kernel the tuning selects): at most the grid asked for, at
+    least one, and no more for the step kernel with a deep LDS stack (16 KiB of stack and 2 KiB
+    of staged results per wave leave room for 8 of them in 160 KiB) than with a shallow one."""
+    tris = R.icosphere(16, 100.0, 0.1, 0xC1)
+    sc = gpu_scene(ctx, 4 | R.BVH_QUANTIZED, 4, tris=tris)
+    try:
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 11, 8, 32))
+        w8 = sc.resident_waves(False)
+        say(f"resident waves: mode 11, lds 8: {w8}")
+        assert 1 <= w8 <= 32
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 11, 32, 32))
+        w32 = sc.resident_waves(False)
+        assert 1 <= w32 <= min(w8, 160 * 1024 // (32 * 512 + 2048))
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 11, 8, 4))
+        assert sc.resident_waves(True) == 4
+    finally:
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
