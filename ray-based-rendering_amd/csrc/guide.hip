@@ -74,10 +74,10 @@ __device__ __forceinline__ v4f mfma32(u32x4 a, u32x4 b, v4f c) {
 // ReLU + BF16 of output blocks 2kb, 2kb+1 -> the next layer's B operand for k-block kb.
 __device__ __forceinline__ u32x4 reluPack(const v4f& lo, const v4f& hi) {
     u32x4 r;
-    r[0] = packBf2(fmaxf(lo[0], 0.0f), fmaxf(lo[1], 0.0f));
-    r[1] = packBf2(fmaxf(lo[2], 0.0f), fmaxf(lo[3], 0.0f));
-    r[2] = packBf2(fmaxf(hi[0], 0.0f), fmaxf(hi[1], 0.0f));
-    r[3] = packBf2(fmaxf(hi[2], 0.0f), fmaxf(hi[3], 0.0f));
+    r[0] = reluBf2(packBf2(lo[0], lo[1]));
+    r[1] = reluBf2(packBf2(lo[2], lo[3]));
+    r[2] = reluBf2(packBf2(hi[0], hi[1]));
+    r[3] = reluBf2(packBf2(hi[2], hi[3]));
     return r;
 }
 
@@ -272,32 +272,52 @@ __device__ __forceinline__ void storeBlock(uint16_t* dst, int F, uint32_t grp, i
     for (int t = 0; t < kTT; ++t) storeRows(base, 32 * kb + 16, t, op[t], idHi, lane);
 }
 
-// Bit i set when operand slot i is not +-0 (the ReLU mask of a layer input, slot order).
-__device__ __forceinline__ uint32_t nonzeroBits(u32x4 h) {
-    uint32_t m = 0u;
+// ReLU masks of the layer inputs, kept as two words per tile: for layer L and k-block kb, bit
+// o + w (o = 4 (2 (L - 1) + kb)) of `lo` / `hi` is set when the low / high BF16 half of operand
+// word w is not zero (reluPack's slot order: word w holds slots 2w, 2w+1).
+struct ReluMask {
+    uint32_t lo, hi;
+};
+
+// Adds the flags of a ReLU output operand at offset o: min(bits, 1) per half is the flag
+// (ReLU outputs are +0 or positive, so nonzero bits are nonzero numbers), at bits w and 16 + w.
+// The v_pk_min_u16 is written as an instruction of its own, with (1, 1) in a register (an
+// inline constant of a packed instruction feeds its high half 0): left to the compiler,
+// min(max(x, 0), 1) and the shifts after it became compares whose masks the gradient test
+// against the FP64 oracle found wrong.
+__device__ __forceinline__ uint32_t minOne16(uint32_t x, uint32_t ones) {
+    uint32_t y;
+    asm("v_pk_min_u16 %0, %1, %2" : "=v"(y) : "v"(x), "v"(ones));
+    return y;
+}
+
+__device__ __forceinline__ void addMask(ReluMask& m, u32x4 h, int o) {
+    uint32_t f = 0u;
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
-        m |= ((h[w] & 0x7FFFu) != 0u ? 1u : 0u) << (2 * w);
-        m |= ((h[w] & 0x7FFF0000u) != 0u ? 1u : 0u) << (2 * w + 1);
-    }
-    return m;
+    for (int w = 0; w < 4; ++w) f |= minOne16(h[w], 0x00010001u) << w;
+    m.lo |= (f & 0xFu) << o;
+    m.hi |= (f >> 16) << o;
 }
 
 // Masked BF16 pack of output blocks 2kb, 2kb+1 into a k-block operand (reluPack's slot order):
-// slot i keeps its value where bit i of `bits` is set.
-__device__ __forceinline__ u32x4 maskPack(const v4f& lo, const v4f& hi, uint32_t bits) {
+// each word is packed and ANDed with the halves its flags keep (a one-bit field sign-extended
+// to 0 or ~0 per half, merged with v_bfi).
+__device__ __forceinline__ u32x4 maskPack(const v4f& lo, const v4f& hi, const ReluMask& m, int o) {
+    const float v[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
     u32x4 r;
-    r[0] = packBf2((bits & 1u) ? lo[0] : 0.0f, (bits & 2u) ? lo[1] : 0.0f);
-    r[1] = packBf2((bits & 4u) ? lo[2] : 0.0f, (bits & 8u) ? lo[3] : 0.0f);
-    r[2] = packBf2((bits & 16u) ? hi[0] : 0.0f, (bits & 32u) ? hi[1] : 0.0f);
-    r[3] = packBf2((bits & 64u) ? hi[2] : 0.0f, (bits & 128u) ? hi[3] : 0.0f);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const uint32_t keepLo = uint32_t(int32_t(m.lo << (31 - (o + w))) >> 31);
+        const uint32_t keepHi = uint32_t(int32_t(m.hi << (31 - (o + w))) >> 31);
+        r[w] = packBf2(v[2 * w], v[2 * w + 1]) & ((keepLo & 0xFFFFu) | (keepHi & 0xFFFF0000u));
+    }
     return r;
 }
 
 // Layer L (1..4) of the training forward: acc holds layer L-1's pre-activations on entry and
-// layer L's on return.  Stores A_L, records its ReLU mask (bits 16 (L-1) + 8 kb + slot).
+// layer L's on return.  Stores A_L, records its ReLU mask.
 template <int L, int MB>
-__device__ __forceinline__ void trainLayer(v4f (&acc)[kTT][4], uint64_t (&mask)[kTT], const u32x4* lwf, const float* lb,
+__device__ __forceinline__ void trainLayer(v4f (&acc)[kTT][4], ReluMask (&mask)[kTT], const u32x4* lwf, const float* lb,
                                            const GuideTrainArgs& a, uint32_t grp, u32x4 idLo, u32x4 idHi, int lane) {
     const int g = lane >> 4;
     u32x4 h[2][kTT];
@@ -305,7 +325,8 @@ __device__ __forceinline__ void trainLayer(v4f (&acc)[kTT][4], uint64_t (&mask)[
     for (int t = 0; t < kTT; ++t) {
         h[0][t] = reluPack(acc[t][0], acc[t][1]);
         h[1][t] = reluPack(acc[t][2], acc[t][3]);
-        mask[t] |= uint64_t(nonzeroBits(h[0][t]) | (nonzeroBits(h[1][t]) << 8)) << (16 * (L - 1));
+        addMask(mask[t], h[0][t], 8 * (L - 1));
+        addMask(mask[t], h[1][t], 8 * (L - 1) + 4);
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb) acc[t][mb] = biasInit(lb, 64 * L, mb, g);
     }
@@ -324,7 +345,7 @@ __device__ __forceinline__ void trainLayer(v4f (&acc)[kTT][4], uint64_t (&mask)[
 // Backward through layer L (4..1): dz holds dZ_L as k-block operands; on return it holds
 // dZ_{L-1} = BF16(W_L^T dZ_L masked by A_L > 0), which is also stored.
 template <int L>
-__device__ __forceinline__ void trainBack(u32x4 (&dz)[2][kTT], v4f (&acc)[kTT][4], const uint64_t (&mask)[kTT],
+__device__ __forceinline__ void trainBack(u32x4 (&dz)[2][kTT], v4f (&acc)[kTT][4], const ReluMask (&mask)[kTT],
                                           const u32x4* lwt, const GuideTrainArgs& a, uint32_t grp, u32x4 idLo,
                                           u32x4 idHi, int lane) {
 #pragma unroll
@@ -341,9 +362,8 @@ __device__ __forceinline__ void trainBack(u32x4 (&dz)[2][kTT], v4f (&acc)[kTT][4
         }
 #pragma unroll
     for (int t = 0; t < kTT; ++t) {
-        const uint32_t bits = uint32_t(mask[t] >> (16 * (L - 1)));
-        dz[0][t] = maskPack(acc[t][0], acc[t][1], bits & 0xFFu);
-        dz[1][t] = maskPack(acc[t][2], acc[t][3], (bits >> 8) & 0xFFu);
+        dz[0][t] = maskPack(acc[t][0], acc[t][1], mask[t], 8 * (L - 1));
+        dz[1][t] = maskPack(acc[t][2], acc[t][3], mask[t], 8 * (L - 1) + 4);
     }
     storeBlock(a.dz[L - 1], kGHid, grp, 0, dz[0], idLo, idHi, lane);
     storeBlock(a.dz[L - 1], kGHid, grp, 1, dz[1], idLo, idHi, lane);
@@ -384,11 +404,11 @@ __global__ __launch_bounds__(512) void guideFwdBwdKernel(GuideTrainArgs a) {
         // layer 0: features are produced one k-block at a time and stored as A_0
         v4f acc[kTT][4];
         float2 st[kTT];
-        uint64_t mask[kTT];
+        ReluMask mask[kTT];
 #pragma unroll
         for (int t = 0; t < kTT; ++t) {
             st[t] = make_float2(0.0f, 1.0f);
-            mask[t] = 0ull;
+            mask[t] = ReluMask{0u, 0u};
 #pragma unroll
             for (int mb = 0; mb < 4; ++mb) acc[t][mb] = biasInit(lb, 0, mb, g);
         }

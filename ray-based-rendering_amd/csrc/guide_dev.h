@@ -30,13 +30,25 @@ __device__ __forceinline__ uint32_t packBf2(float lo, float hi) {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{lo, hi}, bf16x2));
 }
 
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+// ReLU of a BF16 pair on its bits: a BF16 number read as a 16-bit integer has the sign of the
+// number, so max(bits, 0) keeps the non-negative halves and turns the negative ones (and -0)
+// into +0 (one v_pk_max_i16).  Equal to packing fmaxf(x, 0) but for NaN inputs, and it needs
+// none of the canonicalising v_max_f32 that fmaxf of a matrix-core result costs.
+__device__ __forceinline__ uint32_t reluBf2(uint32_t w) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, w), s16x2{0, 0}));
+}
+
 // Features of one 32-wide k-block for lane group g (0..3) of a query with scalars s9: word w
 // is the BF16 (sin, cos) pair of local pair q = 4 kb + w (guidePair's order).  Scalar j = q / 3
 // runs over frequencies 3g, 3g+1, 3g+2: the first is evaluated directly as
 // sin/cos(2 pi frac(2^(k-1) x)) (scaling by a power of two and the fraction are exact; the
 // hardware sine takes revolutions), the next two by angle doubling,
-//   sin 2a = 2 sin a cos a,  cos 2a = (cos a - sin a)(cos a + sin a),
-// carrying (sin, cos) in st between calls.  kb must be a compile-time constant after unrolling.
+//   sin 2a = (2 sin a) cos a,  cos 2a = 1 - (2 sin a) sin a   (three instructions),
+// carrying (sin, cos) in st between calls.  The absolute error of cos 2a stays within a few
+// FP32 ulps of 1, far below the BF16 rounding of the features.  kb must be a compile-time
+// constant after unrolling.
 __device__ __forceinline__ u32x4 guideEncodeBlock(const float* s9, int g, int kb, float2& st) {
     u32x4 out;
 #pragma unroll
@@ -52,8 +64,9 @@ __device__ __forceinline__ u32x4 guideEncodeBlock(const float* s9, int g, int kb
             st.x = __builtin_amdgcn_sinf(f);
             st.y = __builtin_amdgcn_cosf(f);
         } else {
-            const float sn = 2.0f * st.x * st.y;
-            const float cs = (st.y - st.x) * (st.y + st.x);
+            const float t2 = st.x + st.x;
+            const float sn = t2 * st.y;
+            const float cs = __builtin_fmaf(-t2, st.x, 1.0f);
             st.x = sn;
             st.y = cs;
         }
@@ -181,47 +194,57 @@ __device__ __forceinline__ void decodeLobe(const float* o, int k, GLobe& lobe) {
     lobe.w = expf(o[5 * k + 4] - mx) / se;
 }
 
+// Max / sum over the four lanes 16 apart that share a sample (lane = 16 part + sample).
+__device__ __forceinline__ float partMax(float v) {
+    v = fmaxf(v, __shfl_xor(v, 16));
+    return fmaxf(v, __shfl_xor(v, 32));
+}
+__device__ __forceinline__ float partSum(float v) {
+    v += __shfl_xor(v, 16);
+    return v + __shfl_xor(v, 32);
+}
+
 // Loss -c*log p(dir) of one sample and its gradient w.r.t. the 10 outputs of lobes
-// 2*part and 2*part+1 (layout per lobe: mu 3, log kappa, logit).  Returns the loss.  Hardware
-// exp / log / rsq where only relative accuracy matters (the gradient is rounded to BF16);
-// log(1 - e^-x) keeps expm1 / log1p.
+// 2*part and 2*part+1 (layout per lobe: mu 3, log kappa, logit).  Returns the loss.  Called
+// by the whole wave: the four lanes of a sample (lane = 16 part + sample) each evaluate their
+// two lobes and combine the log-sum-exps of the mixture weights and of the lobe densities
+// over the eight lobes with cross-lane max / sum.  Hardware exp / log / rsq where only
+// relative accuracy matters (the gradient is rounded to BF16); log(1 - e^-x) keeps expm1 /
+// log1p.
 __device__ __forceinline__ float outputGrad(const float* o, const float* dir, float c, int part, float* grad) {
-    float a[kGLobes], wlog[kGLobes];
-    float mx = -INFINITY;
-    for (int k = 0; k < kGLobes; ++k) mx = fmaxf(mx, o[5 * k + 4]);
-    float se = 0.0f;
-    for (int k = 0; k < kGLobes; ++k) se += __expf(o[5 * k + 4] - mx);
-    const float lse = mx + __logf(se);
-    float amax = -INFINITY;
-    for (int k = 0; k < kGLobes; ++k) {
-        const float il = __builtin_amdgcn_rsqf(
-            fmaxf(o[5 * k] * o[5 * k] + o[5 * k + 1] * o[5 * k + 1] + o[5 * k + 2] * o[5 * k + 2], 1e-12f));
-        const float d = (o[5 * k] * dir[0] + o[5 * k + 1] * dir[1] + o[5 * k + 2] * dir[2]) * il;
-        const float l = fminf(fmaxf(o[5 * k + 3], kGLogKappaMin), kGLogKappaMax);
-        const float kap = __expf(l);
-        wlog[k] = o[5 * k + 4] - lse;
-        a[k] = wlog[k] + l - kLog2Pi - log1mexp(2.0f * kap) + kap * (d - 1.0f);
-        amax = fmaxf(amax, a[k]);
-    }
-    float sa = 0.0f;
-    for (int k = 0; k < kGLobes; ++k) sa += __expf(a[k] - amax);
-    const float logp = amax + __logf(sa);
+    float lg[2], pre[2], il[2], dd[2], kap[2];
+    bool clamped[2];
+#pragma unroll
     for (int j = 0; j < 2; ++j) {
         const int k = 2 * part + j;
-        const float r = __expf(a[k] - logp);
-        const float w = __expf(wlog[k]);
-        const float il = __builtin_amdgcn_rsqf(
+        lg[j] = o[5 * k + 4];
+        il[j] = __builtin_amdgcn_rsqf(
             fmaxf(o[5 * k] * o[5 * k] + o[5 * k + 1] * o[5 * k + 1] + o[5 * k + 2] * o[5 * k + 2], 1e-12f));
-        const float mu[3] = {o[5 * k] * il, o[5 * k + 1] * il, o[5 * k + 2] * il};
-        const float d = mu[0] * dir[0] + mu[1] * dir[1] + mu[2] * dir[2];
+        dd[j] = (o[5 * k] * dir[0] + o[5 * k + 1] * dir[1] + o[5 * k + 2] * dir[2]) * il[j];
         const float lraw = o[5 * k + 3];
-        const bool clamped = lraw < kGLogKappaMin || lraw > kGLogKappaMax;
-        const float kap = __expf(fminf(fmaxf(lraw, kGLogKappaMin), kGLogKappaMax));
-        const float x = 2.0f * kap;
+        clamped[j] = lraw < kGLogKappaMin || lraw > kGLogKappaMax;
+        const float l = fminf(fmaxf(lraw, kGLogKappaMin), kGLogKappaMax);
+        kap[j] = __expf(l);
+        pre[j] = l - kLog2Pi - log1mexp(2.0f * kap[j]) + kap[j] * (dd[j] - 1.0f);
+    }
+    const float mx = partMax(fmaxf(lg[0], lg[1]));
+    const float lse = mx + __logf(partSum(__expf(lg[0] - mx) + __expf(lg[1] - mx)));
+    float a[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) a[j] = (lg[j] - lse) + pre[j];
+    const float amax = partMax(fmaxf(a[0], a[1]));
+    const float logp = amax + __logf(partSum(__expf(a[0] - amax) + __expf(a[1] - amax)));
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int k = 2 * part + j;
+        const float r = __expf(a[j] - logp);
+        const float w = __expf(lg[j] - lse);
+        const float mu[3] = {o[5 * k] * il[j], o[5 * k + 1] * il[j], o[5 * k + 2] * il[j]};
+        const float x = 2.0f * kap[j];
         const float ratio = x < 1e-4f ? 1.0f - 0.5f * x : x / expm1f(x);   // 2k e^{-2k} / (1 - e^{-2k})
-        const float dl = clamped ? 0.0f : -c * r * (1.0f - ratio + kap * (d - 1.0f));
-        const float gm = -c * r * kap * il;
-        for (int q = 0; q < 3; ++q) grad[5 * j + q] = gm * (dir[q] - d * mu[q]);
+        const float dl = clamped[j] ? 0.0f : -c * r * (1.0f - ratio + kap[j] * (dd[j] - 1.0f));
+        const float gm = -c * r * kap[j] * il[j];
+        for (int q = 0; q < 3; ++q) grad[5 * j + q] = gm * (dir[q] - dd[j] * mu[q]);
         grad[5 * j + 3] = dl;
         grad[5 * j + 4] = c * (w - r);
     }
