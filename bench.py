@@ -102,15 +102,27 @@ def tuned_layout() -> tuple[int, int, dict | None]:
     return int(cfg["width"]), int(cfg["leaf"]), cfg
 
 
-def measured_traffic(width: int):
-    """HBM bytes per closest-hit launch from the committed rocprofv3 --pmc summary of this
-    workload (FETCH_SIZE + WRITE_SIZE, scaled by the same counters' reading for a copy kernel
-    of known size), or None when no such summary exists for this BVH layout."""
+def profile_summary(width: int) -> dict:
+    """The committed rocprofv3 --pmc summary of this workload for this BVH layout (empty when
+    there is none): calibrated HBM bytes per closest-hit launch (FETCH_SIZE + WRITE_SIZE, scaled
+    by the same counters' reading for a copy kernel of known size) and the pipeline counters."""
     path = os.path.join(ROOT, "profiles", f"traversal_closest_c1_{layout_name(width)}_pmc.json")
     if not os.path.exists(path):
-        return None
+        return {}
     with open(path) as f:
-        return json.load(f).get("traffic_bytes_per_launch")
+        return json.load(f)
+
+
+def limiter_text(pm: dict) -> str | None:
+    """What the counters say limits the traversal, from the same summary."""
+    need = ("l1_to_l2_latency_cycles", "TD_TD_BUSY_sum_frac", "TA_TA_BUSY_sum_frac", "valu_issue_frac")
+    if not all(k in pm for k in need):
+        return None
+    return (f"vector memory pipeline: about one L1 miss per node or primitive fetch at a "
+            f"~{pm['l1_to_l2_latency_cycles']:.0f}-cycle L2 latency (texture data unit busy "
+            f"{100 * pm['TD_TD_BUSY_sum_frac']:.0f} %, address unit {100 * pm['TA_TA_BUSY_sum_frac']:.0f} %, "
+            f"VALU issue {100 * pm['valu_issue_frac']:.0f} % of SIMD quad-cycles), from the TA/TD/TCP/SQ "
+            f"counters in profiles/{pm.get('name', '')}.md")
 
 
 def main() -> int:
@@ -202,7 +214,8 @@ def main() -> int:
     bytes_per_ray_layout = bytes_compulsory + nodes_per_ray * node_b + prims_per_ray * 48
     rays_per_s = N_RAYS / (ms_closest * 1e-3)
     achieved = rays_per_s * bytes_per_ray / 1e9
-    traffic = measured_traffic(args.width)
+    pm = profile_summary(args.width)
+    traffic = pm.get("traffic_bytes_per_launch")
 
     rec = {
         "metric": "closest-hit Mrays/s (1.0M-tri displaced icosphere, 2^22 incoherent rays)",
@@ -237,9 +250,7 @@ def main() -> int:
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "traffic_frac": (traffic / (ms_closest * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
-            "limiter": ("vector memory pipeline: about one L1 miss per node or primitive fetch at a ~430-cycle "
-                        "L2 latency (texture data unit busy 84 %, address unit 72 %, VALU issue 56 % of SIMD "
-                        "quad-cycles), from the TA/TD/TCP/SQ counters in profiles/"),
+            "limiter": limiter_text(pm),
             "bytes_per_ray": bytes_per_ray, "bytes_per_ray_layout": bytes_per_ray_layout,
             "frac_layout_bytes": rays_per_s * bytes_per_ray_layout / 1e9 / HBM_PEAK_GBS,
             "nodes_per_ray": nodes_per_ray, "prims_per_ray": prims_per_ray,

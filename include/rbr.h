@@ -277,7 +277,7 @@ int rbr_time_trace(rbr_scene* scene, int kind, const rbr_ray* d_rays, size_t n, 
  * launches in mode & 0xFF; lds_depth = LDS
  * stack entries per lane (8, 10, 12, 16, 24, 32; deeper entries spill to a per-wave global
  * buffer); waves_per_cu = persistent waves launched per compute unit (1..32; a CU holds as
- * many at once as the kernel's LDS (lds_depth * 512 B of stack, plus 2 KiB of result staging
+ * many at once as the kernel's LDS (lds_depth * 512 B of stack, plus 3 KiB of result staging
  * in modes 5-12) and registers allow, rbr_trace_resident_waves; a larger grid runs its extra
  * waves as others end).  Mode 1 needs the ray count on the host: launches whose count stays on the
  * device (the rays of rbr_render) run mode 0 instead, for closest and for any hit. */

@@ -19,7 +19,10 @@ namespace {
 using namespace tdev;
 
 // Result staging of the step kernel: 64-ray ranges held in LDS at once per wave (512 B each).
-constexpr uint32_t kStageSlots = 4;
+// With 6 slots 79 % of the closest-hit result lines reach HBM whole (WRITE_SIZE 1.28x the
+// result bytes on the 2^22-ray batch; 1.68x with 4 slots, 4.5x without staging) at the same
+// rate as 4 slots.
+constexpr uint32_t kStageSlots = 6;
 
 // Work distribution of the persistent kernels: the 64-ray batches are split into kSlices
 // equal slices, each with its own counter (kSliceStride words apart), and a wave drains the
@@ -261,22 +264,29 @@ __global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p
     // at scattered times, and 8-byte (or 1-byte) writes spread over that time left the result
     // lines partly written in L2 and written back several times (4.5x the result bytes for
     // closest hit, 20x for any hit).  stageLeft[s] counts the rays of slot s not yet done; 0
-    // means free.  A range that finds its slot still busy writes its results directly, so no
-    // lane ever waits for a slot.
+    // means free.  A range that finds no free slot writes its results directly, so no lane
+    // ever waits for a slot.
     __shared__ uint2 stage[kStageSlots * 64];
     __shared__ uint32_t stageLeft[kStageSlots];
     if (lane < kStageSlots) stageLeft[lane] = 0u;
-    uint32_t slot = 0u, curSlot = 0u, rangeSeq = 0u;   // slot of the lane's ray, of the current range
+    uint32_t slot = 0u, curSlot = 0u;   // slot of the lane's ray, of the current range
     while (true) {
         const uint64_t idle = __ballot(alive ? 0 : 1);
         const uint32_t nIdle = uint32_t(__popcll(idle));
         if (more && nIdle >= p.refill) {
             const uint32_t avail = qEnd - qBase;
             uint32_t base2 = 0u;
-            const uint32_t ns = rangeSeq % kStageSlots;
-            const bool nsFree = __builtin_amdgcn_readfirstlane(stageLeft[ns]) == 0u;
-            const uint32_t newSlot = nsFree ? ns : kStageSlots;   // kStageSlots: written directly
-            if (avail < nIdle) base2 = nextBatch(p.counter, n, walk);
+            // A new range takes the first free slot (the current range's is busy until its last
+            // ray is done), or none: its rays then write their results directly.
+            uint32_t newSlot = kStageSlots;
+            if (avail < nIdle) {
+                for (uint32_t s2 = 0; s2 < kStageSlots; ++s2)
+                    if (__builtin_amdgcn_readfirstlane(stageLeft[s2]) == 0u) {
+                        newSlot = s2;
+                        break;
+                    }
+                base2 = nextBatch(p.counter, n, walk);
+            }
             if (!alive) {
                 const uint32_t rank = uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
                 const uint32_t idx = rank < avail ? qBase + rank : base2 + (rank - avail);
@@ -302,9 +312,8 @@ __global__ __launch_bounds__(64) void traceKernelStep(TraceArgs a, PersistArgs p
                 } else {
                     // Ranges start at multiples of 64 (slices hold whole batches), so a ray's
                     // row in its slot is i & 63.
-                    if (lane == 0 && newSlot < kStageSlots) stageLeft[ns] = min(64u, n - base2);
+                    if (lane == 0 && newSlot < kStageSlots) stageLeft[newSlot] = min(64u, n - base2);
                     curSlot = newSlot;
-                    ++rangeSeq;
                 }
             } else {
                 qBase += nIdle;
