@@ -381,22 +381,27 @@ int rbr_last_render_stats(rbr_ctx* ctx, rbr_render_stats* out);
  * scaled by the box lo..hi, outward normal, wo) to a mixture of 8 von Mises-Fisher lobes over
  * incident directions.  Layout of the network and its parameters: rbr_guide_param_count()
  * floats, W0[64][224] W1[64][64] W2[64][64] W3[64][64] W4[48][64] b0[64] b1[64] b2[64] b3[64]
- * b4[48] (W[out][in]; input features 216..223 and outputs 40..47 are padding).  Inference
- * and training run on the matrix cores in BF16 with FP32 accumulation. */
+ * b4[48] (W[out][in]; input features 216..223 and outputs 41..47 are padding; output 40 is the
+ * logit of the probability s with which a guided vertex samples the guide rather than the
+ * BSDF, zero at initialisation).  Inference and training run on the matrix cores in BF16 with
+ * FP32 accumulation. */
 int rbr_guide_param_count(void);
 int rbr_guide_init(rbr_ctx* ctx, uint64_t seed, const float lo[3], const float hi[3]);
 int rbr_guide_get_params(rbr_ctx* ctx, float* params, size_t n);
 int rbr_guide_set_params(rbr_ctx* ctx, const float* params, size_t n);
 /* queries: 9 floats each (position, normal, wo); lobes: 40 floats each, 8 x (mu 3, kappa, weight),
- * at the FP16 precision the renderer samples them with. */
+ * at the FP16 precision the renderer samples them with; the weights are the mixture weights
+ * times s and sum to s. */
 int rbr_guide_eval(rbr_ctx* ctx, const float* queries, size_t n, float* lobes);
 /* Draws per_query directions from each query's mixture with the renderer's sampler (u: 3
  * uniforms per direction; lobe choice, polar angle, azimuth) and returns them with their
  * density: dirs 3 floats and pdfs 1 float per direction, query-major. */
 int rbr_guide_sample(rbr_ctx* ctx, const float* queries, size_t nq, size_t per_query, const float* u, float* dirs,
                      float* pdfs);
-/* samples: 13 floats each (position 3, normal 3, wo 3, direction 3, weight c >= 0).
- * loss = (1/n) sum -c log p(direction); grad = its gradient (rbr_guide_param_count floats). */
+/* samples: 14 floats each (position 3, normal 3, wo 3, direction 3, weight c >= 0, BSDF density
+ * pb of the direction, <= 0 for none).  loss = (1/n) sum -c log q(direction) with
+ * q = s p + (1 - s) pb where pb > 0 and q = p elsewhere (p the guide's mixture density);
+ * grad = its gradient (rbr_guide_param_count floats). */
 int rbr_guide_loss_grad(rbr_ctx* ctx, const float* samples, size_t n, double* loss, float* grad);
 /* One Adam step (beta 0.9 / 0.999) on the loss above; loss (optional) is the pre-step value. */
 int rbr_guide_train_step(rbr_ctx* ctx, const float* samples, size_t n, float lr, double* loss);

@@ -19,10 +19,20 @@ Network (all shapes as the device stores them):
            feature_pairs(), plus 4 zero pairs: 224 features
   layers : W0[64][224] W1..W3[64][64] W4[48][64], biases b0..b3[64], b4[48]; ReLU after
            layers 0..3; the padding features and outputs 40..47 carry no information
-  output : 8 lobes x (mean 3, log kappa, logit); mean normalised, log kappa clamped to
-           [-5, 8] (zero gradient outside), weights = softmax(logits)
+  output : 8 lobes x (mean 3, log kappa, logit) in outputs 0..39; mean normalised, log kappa
+           clamped to [-5, 8] (zero gradient outside), weights = softmax(logits); output 40
+           sets the selection probability s = 0.1 + 0.8 sigmoid(o_40) with which the
+           renderer draws a direction from the guide rather than the BSDF (one-sample MIS)
   density: p(w) = sum_k w_k kappa_k / (2 pi (1 - e^{-2 kappa_k})) e^{kappa_k (mu_k . w - 1)}
-  loss   : (1/n) sum_i -c_i log p_i(d_i)
+  record : what inference hands the renderer: mu, kappa and the weights times s, so the
+           stored weights sum to s
+  loss   : (1/n) sum_i -c_i log q_i(d_i) with q = s p + (1 - s) p_bsdf when the sample carries
+           the BSDF density p_bsdf > 0 of its direction, q = p when it does not
+  gradient: of the surrogate (1/n) sum_i -c_i [log p_i(d_i) + log(s_i p^_i + (1 - s_i) p_bsdf_i)]
+           with p^ = p held constant: the lobes learn the guide's own cross-entropy (as without
+           a selection term, so a small s does not starve them of gradient), and the selection
+           row alone learns the mixture's, d/d o_40 = -c (p - p_bsdf) / q * ds/d o_40; the
+           backward pass leaves row 40 of W4 out of the hidden layers' gradient
 
 quant=True reproduces the device's rounding points: encoded inputs, weights, hidden
 activations and back-propagated activation gradients are rounded to bfloat16 (round to
@@ -34,6 +44,8 @@ from __future__ import annotations
 import numpy as np
 
 N_IN, N_IN_PAD, N_HID, N_OUT, N_OUT_PAD, LOBES, FREQ = 216, 224, 64, 40, 48, 8, 12
+SEL = 40   # output index of the selection logit
+SEL_MIN, SEL_MAX = 0.1, 0.9   # s = SEL_MIN + (SEL_MAX - SEL_MIN) sigmoid(o_40)
 W_SHAPES = [(64, 224), (64, 64), (64, 64), (64, 64), (48, 64)]
 B_SIZES = [64, 64, 64, 64, 48]
 N_WEIGHTS = sum(a * b for a, b in W_SHAPES)
@@ -146,6 +158,17 @@ def decode(O: np.ndarray):
     return mu, kappa, w
 
 
+def selection(O: np.ndarray) -> np.ndarray:
+    """(n, 48) outputs -> selection probability s = SEL_MIN + (SEL_MAX - SEL_MIN) sigmoid(o_40)."""
+    return SEL_MIN + (SEL_MAX - SEL_MIN) / (1.0 + np.exp(-np.asarray(O, np.float64)[:, SEL]))
+
+
+def decode_record(O: np.ndarray):
+    """What inference stores: mu, kappa and the weights times the selection probability."""
+    mu, kappa, w = decode(O)
+    return mu, kappa, w * selection(O)[:, None]
+
+
 def log1mexp(x):
     """log(1 - e^{-x}), x > 0."""
     x = np.asarray(x, np.float64)
@@ -167,8 +190,9 @@ def mixture_pdf(mu, kappa, w, dirs) -> np.ndarray:
     return np.exp(lp).sum(axis=2)
 
 
-def _loss_and_dO(O: np.ndarray, dirs: np.ndarray, c: np.ndarray):
-    """Per-sample loss -c log p and its gradient w.r.t. the 48 outputs."""
+def _loss_and_dO(O: np.ndarray, dirs: np.ndarray, c: np.ndarray, pb: np.ndarray | None = None):
+    """Per-sample loss -c log q and its gradient w.r.t. the 48 outputs: q = s p + (1 - s) pb
+    where pb > 0, else q = p (module docstring)."""
     n = O.shape[0]
     o = O[:, :N_OUT].reshape(n, LOBES, 5)
     m = o[:, :, :3]
@@ -186,6 +210,18 @@ def _loss_and_dO(O: np.ndarray, dirs: np.ndarray, c: np.ndarray):
     amax = a.max(axis=1, keepdims=True)
     logp = (amax + np.log(np.exp(a - amax).sum(axis=1, keepdims=True)))[:, 0]
     r = np.exp(a - logp[:, None])                       # responsibilities
+    # Selection (module docstring): the guide terms below are those of -log p; the selection
+    # logit gets d(-log q)/d o_40 = -(p - pb) / q * s (1 - s) with p held constant.
+    if pb is None:
+        pb = np.full(n, -1.0)
+    pb = np.asarray(pb, np.float64)
+    sel = pb > 0.0
+    sg = 1.0 / (1.0 + np.exp(-O[:, SEL]))
+    s_ = SEL_MIN + (SEL_MAX - SEL_MIN) * sg
+    log_pb = np.log(np.where(sel, pb, 1.0))
+    logq = np.where(sel, np.logaddexp(np.log(s_) + logp, np.log(1.0 - s_) + log_pb), logp)
+    dsel = np.where(sel, -c * (np.exp(logp - logq) - np.exp(log_pb - logq)) * (SEL_MAX - SEL_MIN) * sg * (1.0 - sg),
+                    0.0)
     cc = c[:, None]
     x = 2.0 * kap
     # x / (e^x - 1), written as x e^-x / (1 - e^-x) so that large x underflows to 0 instead of
@@ -199,39 +235,71 @@ def _loss_and_dO(O: np.ndarray, dirs: np.ndarray, c: np.ndarray):
     dO = np.zeros((n, N_OUT_PAD))
     g = np.concatenate([gm, dl[:, :, None], dz[:, :, None]], axis=2).reshape(n, N_OUT)
     dO[:, :N_OUT] = g
-    return -c * logp, dO
+    dO[:, SEL] = dsel
+    return -c * logq, dO
 
 
 def split_samples(samples: np.ndarray, lo, hi):
-    s = np.asarray(samples, np.float64).reshape(-1, 13)
+    """(n, 13) or (n, 14) samples: position, normal, wo, direction, weight c and, in column 13,
+    the BSDF density of the direction (<= 0 or absent: none)."""
+    s = np.asarray(samples, np.float64)
+    s = s.reshape(-1, s.shape[-1] if s.ndim == 2 else 13)
     s9 = scale_queries(s[:, :9], lo, hi)
     dirs = s[:, 9:12]
     c = np.maximum(np.where(np.isfinite(s[:, 12]), s[:, 12], 0.0), 0.0)
-    return s9, dirs, c
+    pb = s[:, 13] if s.shape[1] > 13 else np.full(len(c), -1.0)
+    pb = np.where(np.isfinite(pb), pb, -1.0)
+    return s9, dirs, c, pb
 
 
 def loss(params: np.ndarray, samples: np.ndarray, lo, hi, quant: bool = False) -> float:
-    s9, dirs, c = split_samples(samples, lo, hi)
+    s9, dirs, c, pb = split_samples(samples, lo, hi)
     _, O = forward(params, encode(s9), quant)
-    l, _ = _loss_and_dO(O, dirs, c)
+    l, _ = _loss_and_dO(O, dirs, c, pb)
     return float(l.sum() / len(c))
 
 
+def surrogate(params: np.ndarray, samples: np.ndarray, lo, hi, at: np.ndarray) -> float:
+    """The function loss_grad differentiates, evaluated at params with p^ and the hidden
+    features under the selection row taken at `at`: (1/n) sum -c [log p + log(s p^ + (1 - s)
+    pb)] (the second term only where pb > 0).  Its derivative at params = at is loss_grad's
+    gradient (finite-difference checks)."""
+    s9, dirs, c, pb = split_samples(samples, lo, hi)
+    X = encode(s9)
+    _, O = forward(params, X)
+    Aa, Oa = forward(at, X)
+    mu, kappa, w = decode(O)
+    p = mixture_pdf(mu, kappa, w, dirs)[:, 0]
+    mua, kappaa, wa = decode(Oa)
+    pa = mixture_pdf(mua, kappaa, wa, dirs)[:, 0]
+    # the selection row on the hidden features at `at` (they get no gradient from it)
+    Ws, bs = unpack(params)
+    Osel = Oa.copy()
+    Osel[:, SEL] = Aa[4] @ Ws[4][SEL] + bs[4][SEL]
+    s_ = selection(Osel)
+    mix = np.where(pb > 0.0, np.log(s_ * pa + (1.0 - s_) * np.where(pb > 0.0, pb, 1.0)), 0.0)
+    return float(np.sum(-c * (np.log(p) + mix)) / len(c))
+
+
 def loss_grad(params: np.ndarray, samples: np.ndarray, lo, hi, quant: bool = False):
-    """Mean loss and its gradient w.r.t. params (float64), by hand-written backpropagation."""
-    s9, dirs, c = split_samples(samples, lo, hi)
+    """Mean loss and the gradient (of surrogate(); module docstring) w.r.t. params (float64),
+    by hand-written backpropagation."""
+    s9, dirs, c, pb = split_samples(samples, lo, hi)
     n = len(c)
     Ws, bs = unpack(params)
     q = bf16 if quant else (lambda v: v)
     A, O = forward(params, encode(s9), quant)
-    l, dO = _loss_and_dO(O, dirs, c / n)
+    l, dO = _loss_and_dO(O, dirs, c / n, pb)
     gW, gb = [None] * 5, [None] * 5
     dz = q(dO)
     for layer in range(4, -1, -1):
         gW[layer] = dz.T @ A[layer]
         gb[layer] = dz.sum(axis=0)
         if layer > 0:
-            da = dz @ q(Ws[layer])
+            Wb = Ws[layer].copy()
+            if layer == 4:
+                Wb[N_OUT:] = 0.0   # the selection row stays out of the hidden layers' gradient
+            da = dz @ q(Wb)
             dz = q(np.where(A[layer] > 0.0, da, 0.0))
     return float(l.sum()), pack(gW, gb)
 
@@ -240,6 +308,13 @@ def eval_lobes(params: np.ndarray, queries: np.ndarray, lo, hi, quant: bool = Fa
     s9 = scale_queries(queries, lo, hi)
     _, O = forward(params, encode(s9), quant)
     return decode(O)
+
+
+def eval_record(params: np.ndarray, queries: np.ndarray, lo, hi, quant: bool = False):
+    """The lobes as inference stores them (weights times the selection probability)."""
+    s9 = scale_queries(queries, lo, hi)
+    _, O = forward(params, encode(s9), quant)
+    return decode_record(O)
 
 
 def init_params(seed: int) -> np.ndarray:

@@ -6,7 +6,8 @@
 //            each encoded as sin/cos(2^k * pi * x) for k = 0..11  -> 216 features (padded to
 //            224, in the order of guidePair)
 //   hidden : 4 layers of 64 ReLU units
-//   output : 40 = 8 lobes x (mean 3, log kappa 1, logit 1)
+//   output : 40 = 8 lobes x (mean 3, log kappa 1, logit 1), and output 40 the logit of the
+//            probability of sampling the guide rather than the BSDF (kGSel)
 // Weights live as W[out][in] in BF16 for the matrix cores (with an FP32 master copy and
 // FP32 Adam moments); all products accumulate in FP32.
 #pragma once
@@ -27,6 +28,14 @@ constexpr int kGParams = kGWeights + kGBias;
 constexpr int kGOffW[5] = {0, kGW0, kGW0 + kGW1, kGW0 + 2 * kGW1, kGW0 + 3 * kGW1};
 constexpr int kGOffB[5] = {kGWeights, kGWeights + 64, kGWeights + 128, kGWeights + 192, kGWeights + 256};
 constexpr float kGLogKappaMin = -5.0f, kGLogKappaMax = 8.0f;
+// Output kGSel sets the selection probability s = kGSelMin + (kGSelMax - kGSelMin) sigmoid(o)
+// with which the renderer draws a guided vertex's direction from the guide rather than from
+// the BSDF (one of layer 4's padding rows, zero at initialisation: s starts at 1/2).  The
+// range keeps either strategy at 10 % or more, which bounds the weight of a direction the
+// other one covers and the learned one misses.  Only the row's own weights and bias learn it:
+// the backward pass leaves it out of the gradient of the hidden layers, which the lobes train.
+constexpr int kGSel = kGOut;
+constexpr float kGSelMin = 0.1f, kGSelMax = 0.9f;
 
 // Input feature order.  Pair p (0..111) is features 2p = sin(2^k pi s_j), 2p+1 = cos(...);
 // guidePair gives (j, k) for it, or returns false for the 4 zero padding pairs.  The order

@@ -198,7 +198,10 @@ __global__ __launch_bounds__(64 * kWaves, 1) void guideInferKernel(GuideInferArg
                 float se = 0.0f;
 #pragma unroll
                 for (int k = 0; k < kGLobes; ++k) se += __expf(ov[5 * k + 4] - mx);
-                const float rse = __builtin_amdgcn_rcpf(se);
+                // The stored weights carry the selection probability s (kGSel): they sum to s,
+                // which the renderer reads back as the guide's share of the samples.
+                const float rse = __builtin_amdgcn_rcpf(se) *
+                                  (kGSelMin + (kGSelMax - kGSelMin) * __builtin_amdgcn_rcpf(1.0f + __expf(-ov[kGSel])));
                 uint32_t words[5];
 #pragma unroll
                 for (int a5 = 0; a5 < 5; ++a5) words[a5] = 0u;
@@ -388,18 +391,19 @@ __global__ __launch_bounds__(512) void guideFwdBwdKernel(GuideTrainArgs a) {
     float lossAcc = 0.0f;
     for (uint32_t grp = blockIdx.x * kTrainWaves + wave; grp < groups; grp += gridDim.x * kTrainWaves) {
         const uint32_t s0 = grp * kTile * kTT;
-        float s9[kTT][9], dir[kTT][3], c[kTT];
+        float s9[kTT][9], dir[kTT][3], c[kTT], pb[kTT];
         bool valid[kTT];
 #pragma unroll
         for (int t = 0; t < kTT; ++t) {
             const uint32_t si = s0 + kTile * t + r;
             valid[t] = si < n;
             c[t] = 0.0f;
+            pb[t] = 0.0f;
             dir[t][0] = 0.0f;
             dir[t][1] = 0.0f;
             dir[t][2] = 1.0f;
             for (int j = 0; j < 9; ++j) s9[t][j] = 0.0f;
-            if (valid[t]) guideSample(a, si, s9[t], dir[t], c[t]);
+            if (valid[t]) guideSample(a, si, s9[t], dir[t], c[t], pb[t]);
         }
         // layer 0: features are produced one k-block at a time and stored as A_0
         v4f acc[kTT][4];
@@ -439,14 +443,14 @@ __global__ __launch_bounds__(512) void guideFwdBwdKernel(GuideTrainArgs a) {
                 *reinterpret_cast<float4*>(stage + r * kGOutPad + mb * 16 + 4 * g) =
                     make_float4(acc[t][mb][0], acc[t][mb][1], acc[t][mb][2], acc[t][mb][3]);
             waveSync();
-            float grad[10];
-            const float l = outputGrad(stage + r * kGOutPad, dir[t], c[t] * a.scale, g, grad);
+            float grad[10], gsel;
+            const float l = outputGrad(stage + r * kGOutPad, dir[t], c[t] * a.scale, pb[t], g, grad, gsel);
             if (g == 0 && valid[t]) lossAcc += l;
             waveSync();
 #pragma unroll
             for (int q = 0; q < 10; ++q) stage[r * kGOutPad + 10 * g + q] = grad[q];
             if (g == 0)
-                for (int q = kGOut; q < kGOutPad; ++q) stage[r * kGOutPad + q] = 0.0f;
+                for (int q = kGOut; q < kGOutPad; ++q) stage[r * kGOutPad + q] = q == kGSel ? gsel : 0.0f;
             waveSync();
 #pragma unroll
             for (int kb = 0; kb < 2; ++kb) {
@@ -582,11 +586,14 @@ __host__ __device__ inline int fragIndexOf(int i) {
     return guideFragPos(l, o / K, o % K);
 }
 
-// Element index of weight i of layers 1-4 in the transposed fragment copy (-1 for layer 0).
+// Element index of weight i of layers 1-4 in the transposed fragment copy (-1 for layer 0 and
+// for layer 4's rows from kGOut on: the selection row's gradient stays out of the hidden
+// layers, and the padding rows are zero).
 __host__ __device__ inline int tfragIndexOf(int i) {
     if (i < kGOffW[1]) return -1;
     const int l = i < kGOffW[2] ? 1 : i < kGOffW[3] ? 2 : i < kGOffW[4] ? 3 : 4;
     const int o = i - kGOffW[l];
+    if (l == 4 && o / kGHid >= kGOut) return -1;
     return guideTFragPos(l, o / kGHid, o % kGHid);
 }
 
@@ -662,7 +669,7 @@ __global__ __launch_bounds__(256) void guideSynthKernel(float* out, uint32_t n, 
         s[0] = make_float4(p[0], p[1], p[2], synthU(i, 9));
         s[1] = make_float4(nn[0], nn[1], nn[2], 0.0f);
         s[2] = make_float4(wo[0], wo[1], wo[2], 0.0f);
-        s[3] = make_float4(d[0], d[1], d[2], 0.0f);
+        s[3] = make_float4(d[0], d[1], d[2], 0.05f + synthU(i, 10));   // a BSDF density: the selection term runs
     }
 }
 
@@ -1055,11 +1062,11 @@ int rbr_guide_sample(rbr_ctx* ctx, const float* queries, size_t nq, size_t per_q
 static int uploadSamples(rbr_ctx* ctx, GuideState& g, const float* samples, size_t n) {
     std::vector<float4> packed(n * 4);
     for (size_t i = 0; i < n; ++i) {
-        const float* s = samples + 13 * i;
+        const float* s = samples + 14 * i;
         packed[4 * i + 0] = make_float4(s[0], s[1], s[2], s[12]);
         packed[4 * i + 1] = make_float4(s[3], s[4], s[5], 0.0f);
         packed[4 * i + 2] = make_float4(s[6], s[7], s[8], 0.0f);
-        packed[4 * i + 3] = make_float4(s[9], s[10], s[11], 0.0f);
+        packed[4 * i + 3] = make_float4(s[9], s[10], s[11], s[13]);
     }
     return uploadBuffer(ctx, g.dQ, packed.data(), packed.size() * sizeof(float4));
 }

@@ -151,7 +151,8 @@ __device__ __forceinline__ bool guideQuery(const GuideInferArgs& a, uint32_t i, 
     return true;
 }
 
-__device__ __forceinline__ void guideSample(const GuideTrainArgs& a, uint32_t i, float* s9, float* dir, float& c) {
+__device__ __forceinline__ void guideSample(const GuideTrainArgs& a, uint32_t i, float* s9, float* dir, float& c,
+                                            float& pb) {
     uint32_t r = i;
     if (a.total) {
         // i * permMul < 2^53 is exact in FP64; the quotient from the reciprocal is off by at
@@ -171,6 +172,7 @@ __device__ __forceinline__ void guideSample(const GuideTrainArgs& a, uint32_t i,
     dir[1] = d.y;
     dir[2] = d.z;
     c = isfinite(p.w) ? fminf(fmaxf(p.w, 0.0f), a.cmax) : 0.0f;
+    pb = isfinite(d.w) ? d.w : 0.0f;   // BSDF density of the direction; <= 0: none
 }
 
 // log(1 - exp(-x)) for x > 0
@@ -204,14 +206,19 @@ __device__ __forceinline__ float partSum(float v) {
     return v + __shfl_xor(v, 32);
 }
 
-// Loss -c*log p(dir) of one sample and its gradient w.r.t. the 10 outputs of lobes
-// 2*part and 2*part+1 (layout per lobe: mu 3, log kappa, logit).  Returns the loss.  Called
-// by the whole wave: the four lanes of a sample (lane = 16 part + sample) each evaluate their
-// two lobes and combine the log-sum-exps of the mixture weights and of the lobe densities
-// over the eight lobes with cross-lane max / sum.  Hardware exp / log / rsq where only
-// relative accuracy matters (the gradient is rounded to BF16); log(1 - e^-x) keeps expm1 /
-// log1p.
-__device__ __forceinline__ float outputGrad(const float* o, const float* dir, float c, int part, float* grad) {
+// Loss -c*log q(dir) of one sample, q = s p + (1 - s) pb with s the selection probability of
+// o[kGSel] (guide.h) when the sample carries the BSDF density pb > 0 of its direction (the
+// renderer draws from the guide with probability s, from the BSDF otherwise), else q = p.  Gradients: the 10 outputs of
+// lobes 2*part and 2*part+1 (layout per lobe: mu 3, log kappa, logit) get those of -c log p,
+// the guide's own cross-entropy, so that a small s does not starve the lobes; gsel, for the
+// selection logit, is d(-c log q)/d o[kGSel] with p held constant (0 without pb).
+// Called by the whole wave: the four lanes of a sample (lane = 16 part + sample) each evaluate
+// their two lobes and combine the log-sum-exps of the mixture weights and of the lobe
+// densities over the eight lobes with cross-lane max / sum.  Hardware exp / log / rsq where
+// only relative accuracy matters (the gradient is rounded to BF16); log(1 - e^-x) keeps
+// expm1 / log1p.
+__device__ __forceinline__ float outputGrad(const float* o, const float* dir, float c, float pb, int part, float* grad,
+                                            float& gsel) {
     float lg[2], pre[2], il[2], dd[2], kap[2];
     bool clamped[2];
 #pragma unroll
@@ -234,6 +241,19 @@ __device__ __forceinline__ float outputGrad(const float* o, const float* dir, fl
     for (int j = 0; j < 2; ++j) a[j] = (lg[j] - lse) + pre[j];
     const float amax = partMax(fmaxf(a[0], a[1]));
     const float logp = amax + __logf(partSum(__expf(a[0] - amax) + __expf(a[1] - amax)));
+    // Selection: d(-log q)/d o_sel = -(p - pb) / q * ds/do_sel, ds/do_sel = (kGSelMax -
+    // kGSelMin) sigma (1 - sigma); q in logs so that a density far below the other neither
+    // overflows nor loses the sum.
+    float logq = logp;
+    gsel = 0.0f;
+    if (pb > 0.0f) {
+        const float sg = __builtin_amdgcn_rcpf(1.0f + __expf(-o[kGSel]));
+        const float s = kGSelMin + (kGSelMax - kGSelMin) * sg;
+        const float x1 = __logf(s) + logp, x2 = __logf(1.0f - s) + __logf(pb);
+        const float xm = fmaxf(x1, x2);
+        logq = xm + __logf(__expf(x1 - xm) + __expf(x2 - xm));
+        gsel = -c * (__expf(logp - logq) - __expf(__logf(pb) - logq)) * (kGSelMax - kGSelMin) * sg * (1.0f - sg);
+    }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const int k = 2 * part + j;
@@ -248,12 +268,11 @@ __device__ __forceinline__ float outputGrad(const float* o, const float* dir, fl
         grad[5 * j + 3] = dl;
         grad[5 * j + 4] = c * (w - r);
     }
-    return isfinite(logp) ? -c * logp : 0.0f;
+    return isfinite(logq) ? -c * logq : 0.0f;
 }
 
 // ---------------------------------------------------------------- sampling the mixture
 
-constexpr float kGuideFrac = 0.5f;     // one-sample MIS: probability of drawing from the guide
 constexpr uint32_t kRecStride = 6;     // float4 per recorded training sample
 constexpr double kTargetClamp = 50.0;  // recorded targets above this multiple of their mean are clamped
 
@@ -294,6 +313,17 @@ __device__ __forceinline__ float guideMixPdf(const uint4* rec, const float* w) {
         if (wk > 0.0f) p += wk * vmfDensity(mu, kap, w);
     }
     return tot > 0.0f ? p / tot : 0.0f;
+}
+
+// Probability of drawing a guided vertex's direction from the guide (one-sample MIS with the
+// BSDF): the sum of the stored weights, which inference scales by the network's selection
+// probability.  The same value picks the strategy and weights the combined density, so the
+// estimator stays unbiased whatever FP16 made of it.
+__device__ __forceinline__ float guideSelectProb(const uint4* rec) {
+    float tot = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kGLobes; ++k) tot += guideVal(rec, 4, k);
+    return fminf(fmaxf(tot, 0.0f), 1.0f);
 }
 
 // Draws a unit direction from the mixture: u0 picks the lobe, (u1, u2) the direction within
@@ -344,7 +374,8 @@ struct GuideState {
     DevBuf act[5], dz[5], dPartial;
     size_t cap = 0, partialChunks = 0;
     // Samples recorded by the renderer, kRecStride float4 each:
-    //   (position, c) (outward normal, path slot) (wo, -) (direction, -) (throughput, -) (L, -)
+    //   (position, c) (outward normal, path slot) (wo, -) (direction, BSDF density of it)
+    //   (throughput, -) (L, -)
     // dRecCount holds {appended, finalised}; dCsum the sum of c over finalised records.
     DevBuf dRec, dRecCount, dCsum;
     uint32_t recCap = 0, recReserve = 1u << 22;

@@ -837,35 +837,40 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
     // Continuation: phase or BSDF sample, Russian roulette, spawn.
     bool cont = false, record = false;
     F3 no = mk(0, 0, 0), nd = mk(0, 0, 0), wiRec = mk(0, 0, 0), Tin = T;
+    float pdfRec = 0.0f;   // BSDF density of the recorded direction
     if (mode == 2) {
         nd = normalize(hgSample(jb.medium.g, rd, rng.u(base + 4), rng.u(base + 5)));
         no = P;
         cont = depth + 1 < jb.maxDepth;
     } else if (mode == 1) {
         const DMaterial& m = jb.mats[mat];
-        // Guided jobs: one-sample MIS between the guide's vMF mixture (probability
-        // kGuideFrac) and the BSDF; the direction's density is the mixture of both, a delta
-        // BSDF sample keeps its discrete probability.  All-delta materials are never guided.
-        // Recording jobs write a training sample per guidable vertex, whichever density drew
-        // the direction (guided = 2 records plain BSDF-sampled paths).
+        // Guided jobs: one-sample MIS between the guide's vMF mixture (probability s, which
+        // the network learns: the record's weights sum to it) and the BSDF; the direction's
+        // density is the mixture of both, a delta BSDF sample keeps its discrete probability.
+        // All-delta materials are never guided.  Recording jobs write a training sample per
+        // guidable vertex, whichever density drew the direction (guided = 2 records plain
+        // BSDF-sampled paths), with the BSDF density of that direction.
         const bool guidable = !m.allDelta && depth < jb.guideDepth;
         const bool guided = (jb.guided & 1u) && guidable;
         uint4 gl[5];
-        if (guided)
+        float sel = 0.0f;
+        if (guided) {
             for (int k = 0; k < 5; ++k) gl[k] = q.guide[size_t(src) * 5 + k];
-        const bool fromGuide = guided && rng.u(base + 6) < kGuideFrac;
+            sel = guideSelectProb(gl);
+        }
+        const bool fromGuide = guided && rng.u(base + 6) < sel;
         BSample bs{mk(0, 0, 0), mk(0, 0, 0), 0.0f, false};
         F3 wi = mk(0, 0, 0);
-        float pdf = 0.0f;
+        float pdf = 0.0f, pdfBsdf = 0.0f;
         if (!fromGuide) {
             bs = mixSample(m, mx, woL, rng.u(base + 0), rng.u(base + 1));
             if (bs.pdf > 0.0f && !black(bs.f)) {
                 wi = frame.toWorld(bs.wi);
                 pdf = bs.pdf;
+                pdfBsdf = bs.pdf;
                 if (guided) {
                     const float wa[3] = {wi.x, wi.y, wi.z};
-                    pdf = bs.delta ? (1.0f - kGuideFrac) * bs.pdf
-                                   : kGuideFrac * guideMixPdf(gl, wa) + (1.0f - kGuideFrac) * bs.pdf;
+                    pdf = bs.delta ? (1.0f - sel) * bs.pdf : sel * guideMixPdf(gl, wa) + (1.0f - sel) * bs.pdf;
                 }
             }
         } else {
@@ -875,7 +880,8 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
             const F3 wiL = frame.toLocal(wi);
             bs.f = mixEval(m, mx, woL, wiL);
             const float wa[3] = {wi.x, wi.y, wi.z};
-            pdf = kGuideFrac * guideMixPdf(gl, wa) + (1.0f - kGuideFrac) * mixPdf(m, mx, woL, wiL);
+            pdfBsdf = mixPdf(m, mx, woL, wiL);
+            pdf = sel * guideMixPdf(gl, wa) + (1.0f - sel) * pdfBsdf;
             if (black(bs.f)) pdf = 0.0f;
         }
         if (pdf > 0.0f) {
@@ -892,6 +898,7 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
                 cont = depth + 1 < jb.maxDepth;
                 record = guidable && (jb.guided & 2u) && !bs.delta && cont;
                 wiRec = wi;
+                pdfRec = pdfBsdf;
             }
         }
     }
@@ -910,7 +917,7 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
             o[0] = make_float4(P.x, P.y, P.z, 0.0f);
             o[1] = make_float4(outward.x, outward.y, outward.z, __uint_as_float(p));
             o[2] = make_float4(-rd.x, -rd.y, -rd.z, 0.0f);
-            o[3] = make_float4(wiRec.x, wiRec.y, wiRec.z, 0.0f);
+            o[3] = make_float4(wiRec.x, wiRec.y, wiRec.z, pdfRec);
             o[4] = make_float4(Tin.x, Tin.y, Tin.z, 0.0f);
             o[5] = make_float4(L.x, L.y, L.z, 0.0f);
         }

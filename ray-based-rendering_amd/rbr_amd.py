@@ -703,16 +703,28 @@ class Guide:
                                              dirs.ctypes.data, pdfs.ctypes.data))
         return dirs, pdfs
 
+    @staticmethod
+    def _samples14(samples) -> np.ndarray:
+        """(n, 14) float32 samples; (n, 13) ones get a BSDF density of -1 (none)."""
+        s = np.asarray(samples, np.float32)
+        s = s.reshape(-1, s.shape[-1] if s.ndim == 2 else 14)
+        if s.shape[1] == 13:
+            s = np.concatenate([s, np.full((len(s), 1), -1.0, np.float32)], axis=1)
+        if s.shape[1] != 14:
+            raise ValueError("guide samples have 13 or 14 columns")
+        return np.ascontiguousarray(s)
+
     def loss_grad(self, samples) -> tuple[float, np.ndarray]:
-        """samples (n, 13): position, normal, wo, direction, weight -> (mean loss, gradient)."""
-        s = np.ascontiguousarray(samples, np.float32).reshape(-1, 13)
+        """samples (n, 14): position, normal, wo, direction, weight, BSDF density of the
+        direction (<= 0: none; 13 columns: none) -> (mean loss, gradient)."""
+        s = self._samples14(samples)
         g = np.zeros(GUIDE_PARAMS, np.float32)
         loss = C.c_double(0)
         self.ctx.check(_lib.rbr_guide_loss_grad(self.ctx.handle, s.ctypes.data, len(s), C.byref(loss), g.ctypes.data))
         return loss.value, g
 
     def train_step(self, samples, lr: float = 1e-3) -> float:
-        s = np.ascontiguousarray(samples, np.float32).reshape(-1, 13)
+        s = self._samples14(samples)
         loss = C.c_double(0)
         self.ctx.check(_lib.rbr_guide_train_step(self.ctx.handle, s.ctypes.data, len(s), lr, C.byref(loss)))
         return loss.value

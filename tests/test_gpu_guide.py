@@ -46,7 +46,8 @@ def test_eval_matches_oracle(ctx, n):
     g.params = params
     say(f"guide eval n={n}")
     out = g.eval(samples[:, :9]).astype(np.float64)
-    mu, kappa, w = G.eval_lobes(params.astype(np.float64), samples[:, :9], lo, hi, quant=True)
+    # the record's weights carry the selection probability (they sum to it)
+    mu, kappa, w = G.eval_record(params.astype(np.float64), samples[:, :9], lo, hi, quant=True)
     assert np.all(np.isfinite(out))
     # The record is FP16 (2^-11 relative).  Beyond that, most queries agree to float32
     # rounding; the hardware sine and angle doubling (error ~1e-6) sometimes put an encoded
@@ -62,7 +63,7 @@ def test_eval_matches_oracle(ctx, n):
     # a flipped query's unit mean moves further when its raw mean vector is short
     assert e_q.max() < 0.2
     # against the unquantised network: BF16 error budget
-    mu_x, kappa_x, w_x = G.eval_lobes(params.astype(np.float64), samples[:, :9], lo, hi, quant=False)
+    mu_x, kappa_x, w_x = G.eval_record(params.astype(np.float64), samples[:, :9], lo, hi, quant=False)
     assert np.median(np.abs(out[:, :, :3] - mu_x)) < 2e-2
     assert np.median(np.abs(out[:, :, 4] - w_x)) < 5e-3
     # the mixture density, the quantity the sampler uses

@@ -62,7 +62,8 @@ def test_guided_cornell_unbiased(ctx, trained):
 
 def test_guided_materials_unbiased(ctx):
     """Delta lobes (mirror, glass), rough transmission and mixed lobes: delta samples keep
-    their discrete probability (1 - 0.5) x p_bsdf, all-delta vertices are not guided."""
+    their discrete probability (1 - s) x p_bsdf with s the learned selection probability,
+    all-delta vertices are not guided."""
     say("case: guided materials 40x30")
     sc = R.scene_from_desc(ctx, S.materials_scene(40, 30))
     lo, hi = sc.world_bounds()

@@ -16,7 +16,7 @@ def random_params(seed, scale=1.0):
     for b in bs:
         b[:] = rng.normal(0.0, 0.1 * scale, b.shape)
     Ws[4][:] = rng.normal(0.0, 0.3 * scale, Ws[4].shape)
-    Ws[4][40:] = 0.0
+    Ws[4][41:] = 0.0   # row 40 is the selection logit
     Ws[0][:, G.padding_features()] = 0.0
     return G.pack(Ws, bs)
 
@@ -33,15 +33,18 @@ def random_samples(seed, n, lo=(-2.0, -1.0, 0.0), hi=(3.0, 2.0, 5.0)):
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     c = rng.exponential(1.0, n)
     c[rng.random(n) < 0.1] = 0.0
+    # BSDF density of the direction; a quarter of the samples carry none (the guide alone)
+    pb = rng.exponential(0.3, n)
+    pb[rng.random(n) < 0.25] = -1.0
     # the device takes float32 samples: keep values representable
-    samples = np.concatenate([pos, nrm, wo, d, c[:, None]], axis=1).astype(np.float32).astype(np.float64)
+    samples = np.concatenate([pos, nrm, wo, d, c[:, None], pb[:, None]], axis=1).astype(np.float32).astype(np.float64)
     return samples, lo.astype(np.float32), hi.astype(np.float32)
 
 
 def torch_loss(params, samples, lo, hi):
     """Independent float64 restatement with torch layers; autograd gives the gradient."""
     p = torch.tensor(params, dtype=torch.float64, requires_grad=True)
-    s9, dirs, c = G.split_samples(samples, lo, hi)
+    s9, dirs, c, pb = G.split_samples(samples, lo, hi)
     x = torch.tensor(s9)
     feats = []
     for jk in G.feature_pairs():
@@ -62,6 +65,8 @@ def torch_loss(params, samples, lo, hi):
         o += nb
     h = X
     for l in range(5):
+        if l == 4:
+            h4 = h
         h = torch.nn.functional.linear(h, Ws[l], bs[l])
         if l < 4:
             h = torch.relu(h)
@@ -73,8 +78,18 @@ def torch_loss(params, samples, lo, hi):
     cos = (mu * d[:, None, :]).sum(dim=2)
     lognorm = torch.log(kap) - np.log(2 * np.pi) - torch.log(1.0 - torch.exp(-2.0 * kap))
     logp = torch.logsumexp(logw + lognorm + kap * (cos - 1.0), dim=1)
-    loss = -(torch.tensor(c) * logp).sum() / len(c)
-    loss.backward()
+    # one-sample MIS with the BSDF: q = s p + (1 - s) pb where the sample carries pb > 0; the
+    # lobes learn -log p, the selection logit -log q with p held constant
+    # the selection row reads the hidden features but sends them no gradient
+    sel = G.SEL_MIN + (G.SEL_MAX - G.SEL_MIN) * torch.sigmoid(h4.detach() @ Ws[4][40] + bs[4][40])
+    pbt = torch.tensor(pb)
+    ct = torch.tensor(c)
+    mix = torch.log(sel * torch.exp(logp.detach()) + (1.0 - sel) * torch.clamp(pbt, min=1e-300))
+    surrogate = -(ct * (logp + torch.where(pbt > 0, mix, torch.zeros_like(mix)))).sum() / len(c)
+    surrogate.backward()
+    with torch.no_grad():
+        logq = torch.where(pbt > 0, torch.log(sel * torch.exp(logp) + (1.0 - sel) * torch.clamp(pbt, min=1e-300)), logp)
+        loss = -(ct * logq).sum() / len(c)
     return loss.item(), p.grad.numpy()
 
 
@@ -124,18 +139,81 @@ def test_gradient_matches_finite_differences():
     for l, (r, c) in enumerate(G.W_SHAPES):
         real = np.setdiff1d(np.arange(c), G.padding_features()) if l == 0 else np.arange(c)
         for _ in range(4):
-            idx.append(offs[l] + rng.integers(r if l < 4 else 40) * c + rng.choice(real))
+            idx.append(offs[l] + rng.integers(r if l < 4 else 41) * c + rng.choice(real))
+    # the selection logit's row and bias
+    idx += [offs[4] + 40 * 64 + int(rng.integers(64)) for _ in range(3)]
     b0 = G.N_WEIGHTS
     for nb in G.B_SIZES:
         idx.append(b0 + rng.integers(min(nb, 40)))
         b0 += nb
+    idx.append(G.N_WEIGHTS + 4 * 64 + 40)
     h = 1e-6
     for i in idx:
         p1, p2 = params.copy(), params.copy()
         p1[i] += h
         p2[i] -= h
-        fd = (G.loss(p1, samples, lo, hi) - G.loss(p2, samples, lo, hi)) / (2 * h)
+        fd = (G.surrogate(p1, samples, lo, hi, params) - G.surrogate(p2, samples, lo, hi, params)) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(g[i]), (i, fd, g[i])
+
+
+def test_selection_term_off_without_bsdf_density():
+    """Samples without a BSDF density (column 13 <= 0, or 13 columns) train the guide alone:
+    the loss is -c log p and the selection logit gets no gradient.  With a density, the lobes'
+    gradient is unchanged (they learn -c log p either way) and only the selection row moves."""
+    params = random_params(25)
+    samples, lo, hi = random_samples(26, 64)
+    none = samples.copy()
+    none[:, 13] = -1.0
+    l13, g13 = G.loss_grad(params, samples[:, :13], lo, hi)
+    l14, g14 = G.loss_grad(params, none, lo, hi)
+    assert l13 == l14 and np.array_equal(g13, g14)
+    offs = np.cumsum([0] + [a * b for a, b in G.W_SHAPES])
+    assert not g14[offs[4] + 40 * 64:offs[4] + 41 * 64].any() and g14[G.N_WEIGHTS + 4 * 64 + 40] == 0.0
+    s9, dirs, c, _ = G.split_samples(samples, lo, hi)
+    _, O = G.forward(params, G.encode(s9))
+    mu, kappa, w = G.decode(O)
+    p = G.mixture_pdf(mu, kappa, w, dirs)[:, 0]
+    assert abs(l13 - float(np.sum(-c * np.log(p)) / len(c))) <= 1e-12 * abs(l13)
+    # with the densities the gradient differs from g14 in the selection row and bias only: the
+    # lobes learn -c log p either way and the hidden layers get nothing from the selection term
+    _, gs = G.loss_grad(params, samples, lo, hi)
+    sel_idx = set(range(offs[4] + 40 * 64, offs[4] + 41 * 64)) | {G.N_WEIGHTS + 4 * 64 + 40}
+    assert set(np.nonzero(gs != g14)[0].tolist()) <= sel_idx
+    assert gs[offs[4] + 40 * 64:offs[4] + 41 * 64].any()
+
+
+def test_selection_optimum_matches_closed_form():
+    """For one query whose samples all share the guide density p and the BSDF density pb, the
+    loss sum_i -c_i log(s p_i + (1 - s) pb_i) is minimised where its derivative in s vanishes;
+    the oracle's gradient w.r.t. the selection bias is zero at the s that a 1-D root finder
+    puts there (and has the sign of the derivative on either side)."""
+    from scipy.optimize import brentq
+    params = random_params(27)
+    samples, lo, hi = random_samples(28, 40)
+    samples[:, :9] = samples[0, :9]          # one query: same network outputs for all
+    s9, dirs, c, _ = G.split_samples(samples, lo, hi)
+    _, O = G.forward(params, G.encode(s9))
+    mu, kappa, w = G.decode(O)
+    p = G.mixture_pdf(mu, kappa, w, dirs)[:, 0]
+    # half the samples favour the guide, half the BSDF, so the optimum is inside (0, 1)
+    samples[:, 13] = (p * np.where(np.arange(len(p)) % 2 == 0, 0.25, 4.0)).astype(np.float32)
+    s9, dirs, c, pb = G.split_samples(samples, lo, hi)
+    dL = lambda s: float(np.sum(-c * (p - pb) / (s * p + (1 - s) * pb)))
+    lo_s, hi_s = G.SEL_MIN + 1e-9, G.SEL_MAX - 1e-9
+    assert dL(lo_s) < 0 < dL(hi_s)
+    s_star = brentq(dL, lo_s, hi_s, xtol=1e-14)
+    Ws, bs = G.unpack(params)
+    z_now = O[0, 40]
+    ib = G.N_WEIGHTS + 4 * 64 + 40
+    logit = lambda s: np.log((s - G.SEL_MIN) / (G.SEL_MAX - s))
+    for s_t, sign in ((s_star, 0), (0.5 * (lo_s + s_star), -1), (0.5 * (s_star + hi_s), 1)):
+        bs2 = [b.copy() for b in bs]
+        bs2[4][40] += logit(s_t) - z_now
+        _, g = G.loss_grad(G.pack(Ws, bs2), samples, lo, hi)
+        if sign == 0:
+            assert abs(g[ib]) < 1e-9
+        else:
+            assert np.sign(g[ib]) == sign
 
 
 def fibonacci_sphere(m):
