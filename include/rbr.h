@@ -399,9 +399,11 @@ int rbr_guide_eval(rbr_ctx* ctx, const float* queries, size_t n, float* lobes);
 int rbr_guide_sample(rbr_ctx* ctx, const float* queries, size_t nq, size_t per_query, const float* u, float* dirs,
                      float* pdfs);
 /* samples: 14 floats each (position 3, normal 3, wo 3, direction 3, weight c >= 0, BSDF density
- * pb of the direction, <= 0 for none).  loss = (1/n) sum -c log q(direction) with
- * q = s p + (1 - s) pb where pb > 0 and q = p elsewhere (p the guide's mixture density);
- * grad = its gradient (rbr_guide_param_count floats). */
+ * pb of the direction, <= 0 for none; the direction counts as drawn from the BSDF).
+ * loss = (1/n) sum -c log q(direction) with q = s p + (1 - s) pb where pb > 0 and q = p
+ * elsewhere (p the guide's mixture density).  grad (rbr_guide_param_count floats): the lobe
+ * outputs learn -c log p, the selection row (output 40) the second moment c^2 pb / q of the
+ * one-sample MIS estimator, which the hidden layers do not see. */
 int rbr_guide_loss_grad(rbr_ctx* ctx, const float* samples, size_t n, double* loss, float* grad);
 /* One Adam step (beta 0.9 / 0.999) on the loss above; loss (optional) is the pre-step value. */
 int rbr_guide_train_step(rbr_ctx* ctx, const float* samples, size_t n, float lr, double* loss);

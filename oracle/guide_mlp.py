@@ -28,11 +28,15 @@ Network (all shapes as the device stores them):
            stored weights sum to s
   loss   : (1/n) sum_i -c_i log q_i(d_i) with q = s p + (1 - s) p_bsdf when the sample carries
            the BSDF density p_bsdf > 0 of its direction, q = p when it does not
-  gradient: of the surrogate (1/n) sum_i -c_i [log p_i(d_i) + log(s_i p^_i + (1 - s_i) p_bsdf_i)]
-           with p^ = p held constant: the lobes learn the guide's own cross-entropy (as without
-           a selection term, so a small s does not starve them of gradient), and the selection
-           row alone learns the mixture's, d/d o_40 = -c (p - p_bsdf) / q * ds/d o_40; the
-           backward pass leaves row 40 of W4 out of the hidden layers' gradient
+  gradient: of the surrogate (1/n) sum_i [-c_i log p_i(d_i) + c_i^2 g_i / q^_i] with
+           q^ = s p^ + (1 - s) p_bsdf, p^ = p held constant and g the density the direction was
+           drawn with (the BSDF's for these samples): the lobes learn the guide's own
+           cross-entropy (as without a selection term, so a small s does not starve them of
+           gradient), and the selection row alone learns to lower the second moment of the
+           one-sample MIS estimator, int f^2 / q, estimated per sample as c^2 g / q (c = f / g
+           for a direction drawn with density g); its derivative is
+           d/d o_40 = -c^2 g (p - p_bsdf) / q^2 * ds/d o_40; the backward pass leaves row 40 of
+           W4 out of the hidden layers' gradient
 
 quant=True reproduces the device's rounding points: encoded inputs, weights, hidden
 activations and back-propagated activation gradients are rounded to bfloat16 (round to
@@ -190,7 +194,7 @@ def mixture_pdf(mu, kappa, w, dirs) -> np.ndarray:
     return np.exp(lp).sum(axis=2)
 
 
-def _loss_and_dO(O: np.ndarray, dirs: np.ndarray, c: np.ndarray, pb: np.ndarray | None = None):
+def _loss_and_dO(O: np.ndarray, dirs: np.ndarray, c: np.ndarray, pb: np.ndarray | None = None, n_mean: int = 1):
     """Per-sample loss -c log q and its gradient w.r.t. the 48 outputs: q = s p + (1 - s) pb
     where pb > 0, else q = p (module docstring)."""
     n = O.shape[0]
@@ -220,8 +224,11 @@ def _loss_and_dO(O: np.ndarray, dirs: np.ndarray, c: np.ndarray, pb: np.ndarray 
     s_ = SEL_MIN + (SEL_MAX - SEL_MIN) * sg
     log_pb = np.log(np.where(sel, pb, 1.0))
     logq = np.where(sel, np.logaddexp(np.log(s_) + logp, np.log(1.0 - s_) + log_pb), logp)
-    dsel = np.where(sel, -c * (np.exp(logp - logq) - np.exp(log_pb - logq)) * (SEL_MAX - SEL_MIN) * sg * (1.0 - sg),
-                    0.0)
+    # second-moment gradient: c here is c_i / n_mean, so c^2 g / q^2 per mean sample is
+    # c * c * n_mean * g / q^2 with g = pb (the directions were drawn from the BSDF)
+    q_ = np.exp(logq)
+    dsel = np.where(sel, -(c * c * n_mean) * np.where(sel, pb, 0.0) * (np.exp(logp) - np.where(sel, pb, 0.0)) / (q_ * q_)
+                    * (SEL_MAX - SEL_MIN) * sg * (1.0 - sg), 0.0)
     cc = c[:, None]
     x = 2.0 * kap
     # x / (e^x - 1), written as x e^-x / (1 - e^-x) so that large x underflows to 0 instead of
@@ -261,9 +268,9 @@ def loss(params: np.ndarray, samples: np.ndarray, lo, hi, quant: bool = False) -
 
 def surrogate(params: np.ndarray, samples: np.ndarray, lo, hi, at: np.ndarray) -> float:
     """The function loss_grad differentiates, evaluated at params with p^ and the hidden
-    features under the selection row taken at `at`: (1/n) sum -c [log p + log(s p^ + (1 - s)
-    pb)] (the second term only where pb > 0).  Its derivative at params = at is loss_grad's
-    gradient (finite-difference checks)."""
+    features under the selection row taken at `at`: (1/n) sum [-c log p + c^2 pb / (s p^ +
+    (1 - s) pb)] (the second term only where pb > 0).  Its derivative at params = at is
+    loss_grad's gradient (finite-difference checks)."""
     s9, dirs, c, pb = split_samples(samples, lo, hi)
     X = encode(s9)
     _, O = forward(params, X)
@@ -277,8 +284,9 @@ def surrogate(params: np.ndarray, samples: np.ndarray, lo, hi, at: np.ndarray) -
     Osel = Oa.copy()
     Osel[:, SEL] = Aa[4] @ Ws[4][SEL] + bs[4][SEL]
     s_ = selection(Osel)
-    mix = np.where(pb > 0.0, np.log(s_ * pa + (1.0 - s_) * np.where(pb > 0.0, pb, 1.0)), 0.0)
-    return float(np.sum(-c * (np.log(p) + mix)) / len(c))
+    pbs = np.where(pb > 0.0, pb, 1.0)
+    second = np.where(pb > 0.0, c * c * pbs / (s_ * pa + (1.0 - s_) * pbs), 0.0)
+    return float(np.sum(-c * np.log(p) + second) / len(c))
 
 
 def loss_grad(params: np.ndarray, samples: np.ndarray, lo, hi, quant: bool = False):
@@ -289,7 +297,7 @@ def loss_grad(params: np.ndarray, samples: np.ndarray, lo, hi, quant: bool = Fal
     Ws, bs = unpack(params)
     q = bf16 if quant else (lambda v: v)
     A, O = forward(params, encode(s9), quant)
-    l, dO = _loss_and_dO(O, dirs, c / n, pb)
+    l, dO = _loss_and_dO(O, dirs, c / n, pb, n)
     gW, gb = [None] * 5, [None] * 5
     dz = q(dO)
     for layer in range(4, -1, -1):

@@ -391,7 +391,7 @@ __global__ __launch_bounds__(512) void guideFwdBwdKernel(GuideTrainArgs a) {
     float lossAcc = 0.0f;
     for (uint32_t grp = blockIdx.x * kTrainWaves + wave; grp < groups; grp += gridDim.x * kTrainWaves) {
         const uint32_t s0 = grp * kTile * kTT;
-        float s9[kTT][9], dir[kTT][3], c[kTT], pb[kTT];
+        float s9[kTT][9], dir[kTT][3], c[kTT], pb[kTT], pd[kTT];
         bool valid[kTT];
 #pragma unroll
         for (int t = 0; t < kTT; ++t) {
@@ -399,11 +399,12 @@ __global__ __launch_bounds__(512) void guideFwdBwdKernel(GuideTrainArgs a) {
             valid[t] = si < n;
             c[t] = 0.0f;
             pb[t] = 0.0f;
+            pd[t] = 0.0f;
             dir[t][0] = 0.0f;
             dir[t][1] = 0.0f;
             dir[t][2] = 1.0f;
             for (int j = 0; j < 9; ++j) s9[t][j] = 0.0f;
-            if (valid[t]) guideSample(a, si, s9[t], dir[t], c[t], pb[t]);
+            if (valid[t]) guideSample(a, si, s9[t], dir[t], c[t], pb[t], pd[t]);
         }
         // layer 0: features are produced one k-block at a time and stored as A_0
         v4f acc[kTT][4];
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(512) void guideFwdBwdKernel(GuideTrainArgs a) {
                     make_float4(acc[t][mb][0], acc[t][mb][1], acc[t][mb][2], acc[t][mb][3]);
             waveSync();
             float grad[10], gsel;
-            const float l = outputGrad(stage + r * kGOutPad, dir[t], c[t] * a.scale, pb[t], g, grad, gsel);
+            const float l = outputGrad(stage + r * kGOutPad, dir[t], c[t] * a.scale, c[t], pb[t], pd[t], g, grad, gsel);
             if (g == 0 && valid[t]) lossAcc += l;
             waveSync();
 #pragma unroll
@@ -668,8 +669,9 @@ __global__ __launch_bounds__(256) void guideSynthKernel(float* out, uint32_t n, 
         float4* s = reinterpret_cast<float4*>(out) + size_t(i) * 4;
         s[0] = make_float4(p[0], p[1], p[2], synthU(i, 9));
         s[1] = make_float4(nn[0], nn[1], nn[2], 0.0f);
-        s[2] = make_float4(wo[0], wo[1], wo[2], 0.0f);
-        s[3] = make_float4(d[0], d[1], d[2], 0.05f + synthU(i, 10));   // a BSDF density: the selection term runs
+        const float pbs = 0.05f + synthU(i, 10);   // a BSDF density: the selection term runs
+        s[2] = make_float4(wo[0], wo[1], wo[2], pbs);
+        s[3] = make_float4(d[0], d[1], d[2], pbs);
     }
 }
 
@@ -1065,7 +1067,7 @@ static int uploadSamples(rbr_ctx* ctx, GuideState& g, const float* samples, size
         const float* s = samples + 14 * i;
         packed[4 * i + 0] = make_float4(s[0], s[1], s[2], s[12]);
         packed[4 * i + 1] = make_float4(s[3], s[4], s[5], 0.0f);
-        packed[4 * i + 2] = make_float4(s[6], s[7], s[8], 0.0f);
+        packed[4 * i + 2] = make_float4(s[6], s[7], s[8], s[13]);   // drawn from the BSDF: density pb
         packed[4 * i + 3] = make_float4(s[9], s[10], s[11], s[13]);
     }
     return uploadBuffer(ctx, g.dQ, packed.data(), packed.size() * sizeof(float4));

@@ -152,7 +152,7 @@ __device__ __forceinline__ bool guideQuery(const GuideInferArgs& a, uint32_t i, 
 }
 
 __device__ __forceinline__ void guideSample(const GuideTrainArgs& a, uint32_t i, float* s9, float* dir, float& c,
-                                            float& pb) {
+                                            float& pb, float& pdraw) {
     uint32_t r = i;
     if (a.total) {
         // i * permMul < 2^53 is exact in FP64; the quotient from the reciprocal is off by at
@@ -172,7 +172,8 @@ __device__ __forceinline__ void guideSample(const GuideTrainArgs& a, uint32_t i,
     dir[1] = d.y;
     dir[2] = d.z;
     c = isfinite(p.w) ? fminf(fmaxf(p.w, 0.0f), a.cmax) : 0.0f;
-    pb = isfinite(d.w) ? d.w : 0.0f;   // BSDF density of the direction; <= 0: none
+    pb = isfinite(d.w) ? d.w : 0.0f;          // BSDF density of the direction; <= 0: none
+    pdraw = isfinite(wo.w) ? wo.w : 0.0f;     // density the direction was drawn with
 }
 
 // log(1 - exp(-x)) for x > 0
@@ -208,17 +209,19 @@ __device__ __forceinline__ float partSum(float v) {
 
 // Loss -c*log q(dir) of one sample, q = s p + (1 - s) pb with s the selection probability of
 // o[kGSel] (guide.h) when the sample carries the BSDF density pb > 0 of its direction (the
-// renderer draws from the guide with probability s, from the BSDF otherwise), else q = p.  Gradients: the 10 outputs of
-// lobes 2*part and 2*part+1 (layout per lobe: mu 3, log kappa, logit) get those of -c log p,
-// the guide's own cross-entropy, so that a small s does not starve the lobes; gsel, for the
-// selection logit, is d(-c log q)/d o[kGSel] with p held constant (0 without pb).
+// renderer draws from the guide with probability s, from the BSDF otherwise), else q = p.
+// Gradients: the 10 outputs of lobes 2*part and 2*part+1 (layout per lobe: mu 3, log kappa,
+// logit) get those of -c log p, the guide's own cross-entropy, so that a small s does not
+// starve the lobes; gsel, for the selection logit, is the derivative of the sample's share of
+// the one-sample MIS estimator's second moment, c c0 g / q with p held constant (c0 = c
+// before the batch scale, g the density the direction was drawn with; 0 without pb).
 // Called by the whole wave: the four lanes of a sample (lane = 16 part + sample) each evaluate
 // their two lobes and combine the log-sum-exps of the mixture weights and of the lobe
 // densities over the eight lobes with cross-lane max / sum.  Hardware exp / log / rsq where
 // only relative accuracy matters (the gradient is rounded to BF16); log(1 - e^-x) keeps
 // expm1 / log1p.
-__device__ __forceinline__ float outputGrad(const float* o, const float* dir, float c, float pb, int part, float* grad,
-                                            float& gsel) {
+__device__ __forceinline__ float outputGrad(const float* o, const float* dir, float c, float c0, float pb, float g,
+                                            int part, float* grad, float& gsel) {
     float lg[2], pre[2], il[2], dd[2], kap[2];
     bool clamped[2];
 #pragma unroll
@@ -241,9 +244,9 @@ __device__ __forceinline__ float outputGrad(const float* o, const float* dir, fl
     for (int j = 0; j < 2; ++j) a[j] = (lg[j] - lse) + pre[j];
     const float amax = partMax(fmaxf(a[0], a[1]));
     const float logp = amax + __logf(partSum(__expf(a[0] - amax) + __expf(a[1] - amax)));
-    // Selection: d(-log q)/d o_sel = -(p - pb) / q * ds/do_sel, ds/do_sel = (kGSelMax -
-    // kGSelMin) sigma (1 - sigma); q in logs so that a density far below the other neither
-    // overflows nor loses the sum.
+    // Selection: d(c c0 g / q)/d o_sel = -c c0 g (p - pb) / q^2 * ds/do_sel, ds/do_sel =
+    // (kGSelMax - kGSelMin) sigma (1 - sigma); q in logs so that a density far below the other
+    // neither overflows nor loses the sum.
     float logq = logp;
     gsel = 0.0f;
     if (pb > 0.0f) {
@@ -252,7 +255,9 @@ __device__ __forceinline__ float outputGrad(const float* o, const float* dir, fl
         const float x1 = __logf(s) + logp, x2 = __logf(1.0f - s) + __logf(pb);
         const float xm = fmaxf(x1, x2);
         logq = xm + __logf(__expf(x1 - xm) + __expf(x2 - xm));
-        gsel = -c * (__expf(logp - logq) - __expf(__logf(pb) - logq)) * (kGSelMax - kGSelMin) * sg * (1.0f - sg);
+        const float rq = __expf(-logq);
+        gsel = -(c * c0) * g * (__expf(logp) - pb) * rq * rq * (kGSelMax - kGSelMin) * sg * (1.0f - sg);
+        if (!isfinite(gsel)) gsel = 0.0f;
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -374,8 +379,8 @@ struct GuideState {
     DevBuf act[5], dz[5], dPartial;
     size_t cap = 0, partialChunks = 0;
     // Samples recorded by the renderer, kRecStride float4 each:
-    //   (position, c) (outward normal, path slot) (wo, -) (direction, BSDF density of it)
-    //   (throughput, -) (L, -)
+    //   (position, c) (outward normal, path slot) (wo, density the direction was drawn with)
+    //   (direction, BSDF density of it) (throughput, -) (L, -)
     // dRecCount holds {appended, finalised}; dCsum the sum of c over finalised records.
     DevBuf dRec, dRecCount, dCsum;
     uint32_t recCap = 0, recReserve = 1u << 22;

@@ -837,7 +837,7 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
     // Continuation: phase or BSDF sample, Russian roulette, spawn.
     bool cont = false, record = false;
     F3 no = mk(0, 0, 0), nd = mk(0, 0, 0), wiRec = mk(0, 0, 0), Tin = T;
-    float pdfRec = 0.0f;   // BSDF density of the recorded direction
+    float pdfRec = 0.0f, pdfDraw = 0.0f;   // BSDF density of the recorded direction, density it was drawn with
     if (mode == 2) {
         nd = normalize(hgSample(jb.medium.g, rd, rng.u(base + 4), rng.u(base + 5)));
         no = P;
@@ -899,6 +899,7 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
                 record = guidable && (jb.guided & 2u) && !bs.delta && cont;
                 wiRec = wi;
                 pdfRec = pdfBsdf;
+                pdfDraw = pdf;
             }
         }
     }
@@ -916,7 +917,7 @@ __device__ __forceinline__ void shadeEntry(const DJob* __restrict__ jobs, int nj
             float4* o = q.rec + size_t(r) * kRecStride;
             o[0] = make_float4(P.x, P.y, P.z, 0.0f);
             o[1] = make_float4(outward.x, outward.y, outward.z, __uint_as_float(p));
-            o[2] = make_float4(-rd.x, -rd.y, -rd.z, 0.0f);
+            o[2] = make_float4(-rd.x, -rd.y, -rd.z, pdfDraw);
             o[3] = make_float4(wiRec.x, wiRec.y, wiRec.z, pdfRec);
             o[4] = make_float4(Tin.x, Tin.y, Tin.z, 0.0f);
             o[5] = make_float4(L.x, L.y, L.z, 0.0f);

@@ -84,8 +84,9 @@ def torch_loss(params, samples, lo, hi):
     sel = G.SEL_MIN + (G.SEL_MAX - G.SEL_MIN) * torch.sigmoid(h4.detach() @ Ws[4][40] + bs[4][40])
     pbt = torch.tensor(pb)
     ct = torch.tensor(c)
-    mix = torch.log(sel * torch.exp(logp.detach()) + (1.0 - sel) * torch.clamp(pbt, min=1e-300))
-    surrogate = -(ct * (logp + torch.where(pbt > 0, mix, torch.zeros_like(mix)))).sum() / len(c)
+    pbs = torch.where(pbt > 0, pbt, torch.ones_like(pbt))
+    second = ct * ct * pbs / (sel * torch.exp(logp.detach()) + (1.0 - sel) * pbs)
+    surrogate = (-(ct * logp) + torch.where(pbt > 0, second, torch.zeros_like(second))).sum() / len(c)
     surrogate.backward()
     with torch.no_grad():
         logq = torch.where(pbt > 0, torch.log(sel * torch.exp(logp) + (1.0 - sel) * torch.clamp(pbt, min=1e-300)), logp)
@@ -183,10 +184,11 @@ def test_selection_term_off_without_bsdf_density():
 
 
 def test_selection_optimum_matches_closed_form():
-    """For one query whose samples all share the guide density p and the BSDF density pb, the
-    loss sum_i -c_i log(s p_i + (1 - s) pb_i) is minimised where its derivative in s vanishes;
-    the oracle's gradient w.r.t. the selection bias is zero at the s that a 1-D root finder
-    puts there (and has the sign of the derivative on either side)."""
+    """For one query (all samples share the network outputs), the selection row minimises
+    sum_i c_i^2 pb_i / (s p_i + (1 - s) pb_i), the second moment of the one-sample MIS
+    estimator for directions drawn from the BSDF; its derivative in s vanishes at the optimum:
+    the oracle's gradient w.r.t. the selection bias is zero at the s that a 1-D root finder puts
+    there (and has the sign of the derivative on either side)."""
     from scipy.optimize import brentq
     params = random_params(27)
     samples, lo, hi = random_samples(28, 40)
@@ -196,9 +198,9 @@ def test_selection_optimum_matches_closed_form():
     mu, kappa, w = G.decode(O)
     p = G.mixture_pdf(mu, kappa, w, dirs)[:, 0]
     # half the samples favour the guide, half the BSDF, so the optimum is inside (0, 1)
-    samples[:, 13] = (p * np.where(np.arange(len(p)) % 2 == 0, 0.25, 4.0)).astype(np.float32)
+    samples[:, 13] = (p * np.where(np.arange(len(p)) % 2 == 0, 0.01, 1.5)).astype(np.float32)
     s9, dirs, c, pb = G.split_samples(samples, lo, hi)
-    dL = lambda s: float(np.sum(-c * (p - pb) / (s * p + (1 - s) * pb)))
+    dL = lambda s: float(np.sum(-c * c * pb * (p - pb) / (s * p + (1 - s) * pb) ** 2))
     lo_s, hi_s = G.SEL_MIN + 1e-9, G.SEL_MAX - 1e-9
     assert dL(lo_s) < 0 < dL(hi_s)
     s_star = brentq(dL, lo_s, hi_s, xtol=1e-14)
