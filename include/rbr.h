@@ -28,9 +28,11 @@ enum {
     RBR_ERR_STATE = 3,     /* call out of order (e.g. trace before a BVH was built)      */
     RBR_ERR_NOMEM = 4,     /* device or host allocation failed                           */
     RBR_ERR_UNSUPPORTED = 5,/* feature not handled by the GPU path (caller falls back)   */
-    RBR_ERR_WATCHDOG = 6   /* a ray exceeded the traversal step cap (rbr_set_trace_step_cap):
+    RBR_ERR_WATCHDOG = 6,  /* a ray exceeded the traversal step cap (rbr_set_trace_step_cap):
                               the BVH is corrupt or the cap below its size; the results of the
                               launches since the last check are incomplete                  */
+    RBR_ERR_IO = 7         /* a file could not be opened, read or written, or is not of the
+                              expected format (rbr_guide_save / rbr_guide_load)             */
 };
 
 #define RBR_MISS 0xFFFFFFFFu
@@ -249,7 +251,13 @@ int rbr_scene_world_bounds(const rbr_scene* scene, float lo[3], float hi[3]);
  * Host-array variants copy in, trace and copy out synchronously. */
 int rbr_trace_closest(rbr_scene* scene, const rbr_ray* rays, size_t n, rbr_hit* hits);
 int rbr_trace_any(rbr_scene* scene, const rbr_ray* rays, size_t n, uint8_t* occluded);
-/* Device-pointer variants: asynchronous on the context stream. */
+/* Device-pointer variants: asynchronous on the context stream.  rbr_ctx_set_stream makes a
+ * caller's hipStream_t (passed as void*) the context stream: the device-pointer entry points are
+ * then the asynchronous variants on that stream (the synchronous ones wait on it).  The call
+ * first waits for the work already queued; NULL returns to the context's own stream, which the
+ * caller does before destroying its stream.  No counterpart in the reference (its traversal is
+ * synchronous host code). */
+int rbr_ctx_set_stream(rbr_ctx* ctx, void* stream);
 int rbr_trace_closest_device(rbr_scene* scene, const rbr_ray* d_rays, size_t n, rbr_hit* d_hits);
 int rbr_trace_any_device(rbr_scene* scene, const rbr_ray* d_rays, size_t n, uint8_t* d_occluded);
 /* rbr_trace_closest_device without any allocation.  No counterpart in the reference: for
@@ -389,6 +397,12 @@ int rbr_guide_param_count(void);
 int rbr_guide_init(rbr_ctx* ctx, uint64_t seed, const float lo[3], const float hi[3]);
 int rbr_guide_get_params(rbr_ctx* ctx, float* params, size_t n);
 int rbr_guide_set_params(rbr_ctx* ctx, const float* params, size_t n);
+/* The trained guide as a file: FP32 parameters, scene box, and the Adam moments and step when
+ * training has started, so that rbr_guide_load (on any context of this library) continues
+ * training exactly where the saved one stood.  RBR_ERR_IO for a file that cannot be written or
+ * read or is not a guide file of this network. */
+int rbr_guide_save(rbr_ctx* ctx, const char* path);
+int rbr_guide_load(rbr_ctx* ctx, const char* path);
 /* queries: 9 floats each (position, normal, wo); lobes: 40 floats each, 8 x (mu 3, kappa, weight),
  * at the FP16 precision the renderer samples them with; the weights are the mixture weights
  * times s and sum to s. */

@@ -26,7 +26,8 @@ struct rbr_ctx {
     int device = 0;
     int cuCount = 0;
     std::string name;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // the stream every launch goes to (ownStream or the caller's)
+    hipStream_t ownStream = nullptr;   // created and destroyed by the context
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     rbr::RenderState* render = nullptr;

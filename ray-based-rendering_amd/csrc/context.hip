@@ -234,7 +234,21 @@ int rbr_ctx_create(int device, rbr_ctx** out) {
         return RBR_ERR_HIP;
     }
     *ctx->hFault = 0u;
+    ctx->ownStream = ctx->stream;
     *out = ctx;
+    return RBR_OK;
+}
+
+int rbr_ctx_set_stream(rbr_ctx* ctx, void* stream) {
+    if (!ctx) return RBR_ERR_INVALID;
+    if (ctx->device < 0) return fail(ctx, RBR_ERR_STATE, "a host-only context has no stream");
+    if (int rc = useDevice(ctx)) return rc;
+    // Work already queued finishes before later launches go to the other stream, so the
+    // context's buffers (batch counters, spill rows, render queues) are never used by two
+    // streams at once.
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return failHip(ctx, e, "hipStreamSynchronize");
+    ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->ownStream;
     return RBR_OK;
 }
 
@@ -251,7 +265,7 @@ void rbr_ctx_destroy(rbr_ctx* ctx) {
     if (ctx->hFault) (void)hipHostFree(ctx->hFault);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
-    (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->ownStream);
     delete ctx;
 }
 

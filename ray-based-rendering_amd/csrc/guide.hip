@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -990,6 +991,93 @@ int rbr_guide_set_params(rbr_ctx* ctx, const float* params, size_t n) {
 }
 
 int rbr_guide_param_count(void) { return kGParams; }
+
+// Guide file: "RBRGUID1", uint32 version 1, parameter count, flags (bit 0: Adam moments
+// present), Adam step; float scene box lo[3] and 1 / extent[3]; uint64 batch-permutation
+// state; FP32 master parameters, then the Adam moments m and v when present.  Little-endian,
+// as the device that wrote it.
+namespace {
+constexpr char kGuideMagic[8] = {'R', 'B', 'R', 'G', 'U', 'I', 'D', '1'};
+struct GuideFileHead {
+    char magic[8];
+    uint32_t version, nparams, flags, step;
+    float lo[3], inv[3];
+    uint64_t rng;
+};
+}  // namespace
+
+int rbr_guide_save(rbr_ctx* ctx, const char* path) {
+    if (!ctx || !path) return RBR_ERR_INVALID;
+    GuideState& g = state(ctx);
+    if (!g.ready) return fail(ctx, RBR_ERR_STATE, "guide not initialised");
+    if (int rc = useDevice(ctx)) return rc;
+    std::vector<float> buf(size_t(kGParams) * 3, 0.0f);
+    hipError_t e = hipMemcpyAsync(buf.data(), g.dMaster.ptr, size_t(kGParams) * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && g.adamInit) {
+        e = hipMemcpyAsync(buf.data() + kGParams, g.dM.ptr, size_t(kGParams) * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(buf.data() + 2 * size_t(kGParams), g.dV.ptr, size_t(kGParams) * 4, hipMemcpyDeviceToHost,
+                               ctx->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return failHip(ctx, e, "guide state download");
+    GuideFileHead h{};
+    std::memcpy(h.magic, kGuideMagic, 8);
+    h.version = 1;
+    h.nparams = kGParams;
+    h.flags = g.adamInit ? 1u : 0u;
+    h.step = uint32_t(g.step);
+    for (int k = 0; k < 3; ++k) {
+        h.lo[k] = g.lo[k];
+        h.inv[k] = g.inv[k];
+    }
+    h.rng = g.trainRng;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(ctx, RBR_ERR_IO, std::string("cannot write ") + path);
+    const size_t words = size_t(kGParams) * (g.adamInit ? 3 : 1);
+    const bool ok = std::fwrite(&h, sizeof h, 1, f) == 1 && std::fwrite(buf.data(), 4, words, f) == words;
+    if (std::fclose(f) != 0 || !ok) return fail(ctx, RBR_ERR_IO, std::string("write failed: ") + path);
+    return RBR_OK;
+}
+
+int rbr_guide_load(rbr_ctx* ctx, const char* path) {
+    if (!ctx || !path) return RBR_ERR_INVALID;
+    if (int rc = useDevice(ctx)) return rc;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(ctx, RBR_ERR_IO, std::string("cannot read ") + path);
+    GuideFileHead h{};
+    std::vector<float> buf;
+    bool ok = std::fread(&h, sizeof h, 1, f) == 1 && std::memcmp(h.magic, kGuideMagic, 8) == 0 && h.version == 1 &&
+              h.nparams == uint32_t(kGParams) && (h.flags & ~1u) == 0u;
+    if (ok) {
+        const size_t words = size_t(kGParams) * ((h.flags & 1u) ? 3 : 1);
+        buf.resize(words);
+        ok = std::fread(buf.data(), 4, words, f) == words && std::fgetc(f) == EOF;
+    }
+    std::fclose(f);
+    if (!ok) return fail(ctx, RBR_ERR_IO, std::string("not a guide file of this network: ") + path);
+    GuideState& g = state(ctx);
+    for (int k = 0; k < 3; ++k) {
+        g.lo[k] = h.lo[k];
+        g.inv[k] = h.inv[k];
+    }
+    g.master.assign(buf.begin(), buf.begin() + kGParams);
+    g.trainRng = h.rng;
+    g.ready = true;
+    if (int rc = refreshBf16(ctx, g)) return rc;
+    g.adamInit = false;
+    g.step = 0;
+    if (h.flags & 1u) {
+        int rc;
+        if ((rc = ensureBuffer(ctx, g.dM, size_t(kGParams) * 4))) return rc;
+        if ((rc = ensureBuffer(ctx, g.dV, size_t(kGParams) * 4))) return rc;
+        if ((rc = uploadBuffer(ctx, g.dM, buf.data() + kGParams, size_t(kGParams) * 4))) return rc;
+        if ((rc = uploadBuffer(ctx, g.dV, buf.data() + 2 * size_t(kGParams), size_t(kGParams) * 4))) return rc;
+        g.adamInit = true;
+        g.step = int(h.step);
+    }
+    return RBR_OK;
+}
 
 int rbr_guide_eval(rbr_ctx* ctx, const float* queries, size_t n, float* lobes) {
     if (!ctx || (n && (!queries || !lobes))) return RBR_ERR_INVALID;

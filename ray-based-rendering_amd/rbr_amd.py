@@ -21,8 +21,9 @@ HIT_DTYPE = np.dtype([("id", "<u4"), ("t", "<f4")])
 MISS = 0xFFFFFFFF
 
 ERRORS = {1: "invalid argument", 2: "HIP error", 3: "bad state", 4: "out of memory", 5: "unsupported",
-          6: "traversal watchdog"}
+          6: "traversal watchdog", 7: "file error"}
 ERR_WATCHDOG = 6
+ERR_IO = 7
 
 
 class RbrError(RuntimeError):
@@ -89,6 +90,7 @@ _sig(_lib.rbr_trace_closest_device, [_vp, _vp, _sz, _vp])
 _sig(_lib.rbr_trace_any_device, [_vp, _vp, _sz, _vp])
 _sig(_lib.rbr_set_trace_tuning, [_vp, C.c_int, C.c_int, C.c_int])
 _sig(_lib.rbr_set_trace_step_cap, [_vp, C.c_uint32])
+_sig(_lib.rbr_ctx_set_stream, [_vp, _vp])
 _sig(_lib.rbr_trace_resident_waves, [_vp, C.c_int, C.POINTER(C.c_int)])
 _sig(_lib.rbr_time_trace, [_vp, C.c_int, _vp, _sz, _vp, C.c_int, C.POINTER(C.c_float)])
 _sig(_lib.rbr_host_trace_closest, [_vp, _vp, _sz, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)])
@@ -175,6 +177,11 @@ class Context:
 
     def synchronize(self) -> None:
         self.check(_lib.rbr_synchronize(self._h))
+
+    def set_stream(self, stream) -> None:
+        """Makes a caller's hipStream_t (an int or c_void_p handle) the context stream; None
+        returns to the context's own stream (rbr_ctx_set_stream)."""
+        self.check(_lib.rbr_ctx_set_stream(self._h, None if stream is None else C.c_void_p(int(stream))))
 
     def set_trace_step_cap(self, cap: int) -> None:
         """Traversal watchdog cap (node visits + primitive tests per ray); 0 = the BVH's own bound."""
@@ -648,6 +655,8 @@ _sig(_lib.rbr_guide_param_count, [])
 _sig(_lib.rbr_guide_init, [_vp, C.c_uint64, _vp, _vp])
 _sig(_lib.rbr_guide_get_params, [_vp, _vp, _sz])
 _sig(_lib.rbr_guide_set_params, [_vp, _vp, _sz])
+_sig(_lib.rbr_guide_save, [_vp, C.c_char_p])
+_sig(_lib.rbr_guide_load, [_vp, C.c_char_p])
 _sig(_lib.rbr_guide_eval, [_vp, _vp, _sz, _vp])
 _sig(_lib.rbr_guide_loss_grad, [_vp, _vp, _sz, C.POINTER(C.c_double), _vp])
 _sig(_lib.rbr_guide_train_step, [_vp, _vp, _sz, C.c_float, C.POINTER(C.c_double)])
@@ -671,6 +680,18 @@ class Guide:
         hi = np.ascontiguousarray(hi, np.float32)
         ctx.check(_lib.rbr_guide_init(ctx.handle, seed, lo.ctypes.data, hi.ctypes.data))
         self.lo, self.hi = lo, hi
+
+    @classmethod
+    def load(cls, ctx: Context, path: str) -> "Guide":
+        """The guide saved at `path` (rbr_guide_load): parameters, scene box, Adam state."""
+        g = cls.__new__(cls)
+        g.ctx = ctx
+        ctx.check(_lib.rbr_guide_load(ctx.handle, os.fsencode(path)))
+        g.lo = g.hi = None
+        return g
+
+    def save(self, path: str) -> None:
+        self.ctx.check(_lib.rbr_guide_save(self.ctx.handle, os.fsencode(path)))
 
     @property
     def params(self) -> np.ndarray:

@@ -212,3 +212,35 @@ def test_training_fits_a_known_distribution(ctx):
     say(f"  after 600 steps: {l1:.4f}")
     assert l1 < l0 - 0.5 * (l0 - entropy)
     assert l1 < entropy + 0.5
+
+
+def test_save_load_resumes_training(ctx, tmp_path):
+    """rbr_guide_save / rbr_guide_load: a loaded guide has the saved parameters and scene box
+    (identical lobes) and, with the Adam moments and step restored, the same next training steps
+    as the guide that kept going; a file of another kind is refused with RBR_ERR_IO."""
+    samples, lo, hi = random_samples(91, 4096)
+    g = R.Guide(ctx, lo, hi, seed=3)
+    for _ in range(3):
+        g.train_step(samples, lr=1e-3)
+    path = str(tmp_path / "guide.bin")
+    g.save(path)
+    q = samples[:64, :9]
+    before = g.eval(q)
+    with R.Context(0) as c2:
+        h = R.Guide.load(c2, path)
+        np.testing.assert_array_equal(h.params, g.params)
+        np.testing.assert_array_equal(h.eval(q), before)
+        la = [g.train_step(samples, lr=1e-3) for _ in range(2)]
+        lb = [h.train_step(samples, lr=1e-3) for _ in range(2)]
+        np.testing.assert_allclose(la, lb, rtol=1e-12)   # losses are summed with atomics
+        np.testing.assert_array_equal(h.params, g.params)
+        bad = tmp_path / "bad.bin"
+        bad.write_bytes(b"not a guide file" * 8)
+        with pytest.raises(R.RbrError) as e:
+            R.Guide.load(c2, str(bad))
+        assert e.value.code == R.ERR_IO
+    # a guide saved before any training step loads without Adam state
+    g2 = R.Guide(ctx, lo, hi, seed=4)
+    g2.save(path)
+    with R.Context(0) as c3:
+        np.testing.assert_array_equal(R.Guide.load(c3, path).params, g2.params)

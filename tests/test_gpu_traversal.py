@@ -352,3 +352,45 @@ def test_resident_waves_query(ctx):
         assert sc.resident_waves(True) == 4
     finally:
         ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
+
+
+def test_caller_stream(ctx):
+    """rbr_ctx_set_stream: the device-pointer launches run on a stream the caller created (the
+    asynchronous variant), the synchronous calls wait on it, and NULL returns to the context's
+    own stream before the caller destroys its stream; results are those of the own stream."""
+    import ctypes as C
+    # the HIP runtime librbr.so runs on (a second copy of the runtime would have no device)
+    with open("/proc/self/maps") as maps:
+        libs = sorted({ln.split()[-1] for ln in maps if "libamdhip64.so" in ln})
+    assert libs, "libamdhip64 not loaded"
+    hip = C.CDLL(libs[0])
+    hip.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    hip.hipStreamSynchronize.argtypes = [C.c_void_p]
+    tris = R.icosphere(24, 100.0, 0.1, 0xC1)
+    lo, hi = R.mesh_bounds(tris)
+    rays = R.rays_uniform(lo, hi, 20000, 13)
+    o = O.OracleScene(tris=tris)
+    ref = o.closest(rays, brute=True)
+    sc = gpu_scene(ctx, 4 | R.BVH_QUANTIZED, 4, tris=tris)
+    stream = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(stream)) == 0
+    try:
+        ctx.set_stream(stream.value)
+        d_rays, d_hits = ctx.alloc(rays.nbytes), ctx.alloc(len(rays) * 8)
+        d_rays.upload(rays)
+        for mode in (0, 11):
+            ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, mode, 8, 32))
+            sc.trace_closest_device(d_rays, len(rays), d_hits)
+            assert hip.hipStreamSynchronize(stream) == 0   # the caller's own wait
+            h = np.empty(len(rays), R.HIT_DTYPE)
+            d_hits.download(h)
+            assert_same(h, ref)
+            assert_same(sc.trace_closest(rays), ref)
+        d_rays.free()
+        d_hits.free()
+    finally:
+        ctx.check(R._lib.rbr_set_trace_tuning(ctx.handle, 0, 8, 32))
+        ctx.set_stream(None)
+        assert hip.hipStreamDestroy(stream) == 0
+    assert_same(sc.trace_closest(rays), ref)
