@@ -2,7 +2,8 @@
 
   python ray-based-rendering_amd/bench_guide_kernels.py
 
-One JSON line per measurement (inputs resident in HBM, HIP events on the context stream):
+One JSON line per measurement (inputs resident in HBM, HIP events on the context stream; the
+best of three timed runs of 20 launches, 10 for training steps):
   inference  n queries:  encoding + 5 layers + lobe decode (guideInferKernel)
   training   n samples:  forward + loss gradient + backward + weight/bias gradients + Adam
 Algorithmic FLOP (real layer sizes, no padding):
@@ -31,12 +32,12 @@ def main() -> int:
     ctx = R.Context(0)
     g = R.Guide(ctx, [0, 0, 0], [1, 1, 1], seed=1)
     for n in (1 << 18, 1 << 20, 1 << 22):
-        ms = g.time(0, n, 20)
+        ms = min(g.time(0, n, 20) for _ in range(3))
         fl = FWD_FLOP * n / (ms * 1e-3)
         print(json.dumps({"kernel": "guide inference", "n": n, "ms": ms, "queries_s": n / (ms * 1e-3),
                           "tflop_s": fl / 1e12, "frac_bf16_peak": fl / PEAK}), flush=True)
     for n in (1 << 16, 1 << 18):
-        ms = g.time(1, n, 10)
+        ms = min(g.time(1, n, 10) for _ in range(3))
         fl = TRAIN_FLOP * n / (ms * 1e-3)
         print(json.dumps({"kernel": "guide training step", "n": n, "ms": ms, "samples_s": n / (ms * 1e-3),
                           "tflop_s": fl / 1e12, "frac_bf16_peak": fl / PEAK}), flush=True)

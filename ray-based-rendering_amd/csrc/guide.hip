@@ -66,7 +66,11 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kIT = 2;                         // 16-query tiles per wave
-constexpr int kInferStage = kTile * kGOutPad;  // floats of LDS output stage per wave
+// Floats per query row of a wave's LDS output stage: 52, not kGOutPad = 48, so that the 16 rows
+// of a tile start in 16 different banks (with 48 a column of the tile fell in only 4 of the
+// 64 banks, and every lane-parallel read of one output across the tile's queries conflicted).
+constexpr int kStageRow = 52;
+constexpr int kInferStage = kTile * kStageRow;  // floats of LDS output stage per wave
 
 __device__ __forceinline__ v4f mfma32(u32x4 a, u32x4 b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
@@ -187,41 +191,47 @@ __global__ __launch_bounds__(64 * kWaves, 1) void guideInferKernel(GuideInferArg
         for (int t = 0; t < kIT; ++t) {
 #pragma unroll
             for (int mb = 0; mb < 3; ++mb)
-                *reinterpret_cast<float4*>(stage + r * kGOutPad + mb * 16 + 4 * g) =
+                *reinterpret_cast<float4*>(stage + r * kStageRow + mb * 16 + 4 * g) =
                     make_float4(o[t][mb][0], o[t][mb][1], o[t][mb][2], o[t][mb][3]);
             waveSync();
-            if (valid[t]) {
-                const float* ov = stage + r * kGOutPad;
-                // hardware exp / rsq / rcp: the lobes are stored in FP16
-                float mx = -INFINITY;
+            {
+                // Lane (r, g) decodes lobes 2g, 2g+1 of query r (ten consecutive outputs).  The
+                // softmax max and sum over the query's 8 weights are taken across its four lanes
+                // (r, r + 16, r + 32, r + 48) with two exchanges each, so each weight is
+                // exponentiated once.  Every lane takes part: the four lanes of a query agree on
+                // valid[t].  Hardware exp / rsq / rcp: the lobes are stored in FP16.
+                const float* ov = stage + r * kStageRow;
+                float v[10];
 #pragma unroll
-                for (int k = 0; k < kGLobes; ++k) mx = fmaxf(mx, ov[5 * k + 4]);
-                float se = 0.0f;
-#pragma unroll
-                for (int k = 0; k < kGLobes; ++k) se += __expf(ov[5 * k + 4] - mx);
+                for (int q = 0; q < 10; ++q) v[q] = ov[10 * g + q];
+                float mx = fmaxf(v[4], v[9]);
+                mx = fmaxf(mx, __shfl_xor(mx, 16));
+                mx = fmaxf(mx, __shfl_xor(mx, 32));
+                const float e[2] = {__expf(v[4] - mx), __expf(v[9] - mx)};
+                float se = e[0] + e[1];
+                se += __shfl_xor(se, 16);
+                se += __shfl_xor(se, 32);
                 // The stored weights carry the selection probability s (kGSel): they sum to s,
                 // which the renderer reads back as the guide's share of the samples.
                 const float rse = __builtin_amdgcn_rcpf(se) *
                                   (kGSelMin + (kGSelMax - kGSelMin) * __builtin_amdgcn_rcpf(1.0f + __expf(-ov[kGSel])));
-                uint32_t words[5];
-#pragma unroll
-                for (int a5 = 0; a5 < 5; ++a5) words[a5] = 0u;
+                float vals[2][5];
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    const int k = 2 * g + j;
-                    const float m0 = ov[5 * k], m1 = ov[5 * k + 1], m2 = ov[5 * k + 2];
+                    const float m0 = v[5 * j], m1 = v[5 * j + 1], m2 = v[5 * j + 2];
                     const float il = __builtin_amdgcn_rsqf(fmaxf(m0 * m0 + m1 * m1 + m2 * m2, 1e-12f));
-                    const float vals[5] = {m0 * il, m1 * il, m2 * il,
-                                           __expf(fminf(fmaxf(ov[5 * k + 3], kGLogKappaMin), kGLogKappaMax)),
-                                           __expf(ov[5 * k + 4] - mx) * rse};
-#pragma unroll
-                    for (int a5 = 0; a5 < 5; ++a5)
-                        words[a5] |= uint32_t(__half_as_ushort(__float2half_rn(vals[a5]))) << (16 * j);
+                    vals[j][0] = m0 * il;
+                    vals[j][1] = m1 * il;
+                    vals[j][2] = m2 * il;
+                    vals[j][3] = __expf(fminf(fmaxf(v[5 * j + 3], kGLogKappaMin), kGLogKappaMax));
+                    vals[j][4] = e[j] * rse;
                 }
-                const uint32_t qi = (grp * kIT + t) * kTile + r;
-                uint32_t* dst = reinterpret_cast<uint32_t*>(a.out + size_t(qi) * 5);
+                if (valid[t]) {
+                    const uint32_t qi = (grp * kIT + t) * kTile + r;
+                    uint32_t* dst = reinterpret_cast<uint32_t*>(a.out + size_t(qi) * 5);
 #pragma unroll
-                for (int a5 = 0; a5 < 5; ++a5) dst[4 * a5 + g] = words[a5];
+                    for (int a5 = 0; a5 < 5; ++a5) dst[4 * a5 + g] = packH2(vals[0][a5], vals[1][a5]);
+                }
             }
             waveSync();
         }
@@ -442,17 +452,17 @@ __global__ __launch_bounds__(512) void guideFwdBwdKernel(GuideTrainArgs a) {
         for (int t = 0; t < kTT; ++t) {
 #pragma unroll
             for (int mb = 0; mb < 3; ++mb)
-                *reinterpret_cast<float4*>(stage + r * kGOutPad + mb * 16 + 4 * g) =
+                *reinterpret_cast<float4*>(stage + r * kStageRow + mb * 16 + 4 * g) =
                     make_float4(acc[t][mb][0], acc[t][mb][1], acc[t][mb][2], acc[t][mb][3]);
             waveSync();
             float grad[10], gsel;
-            const float l = outputGrad(stage + r * kGOutPad, dir[t], c[t] * a.scale, c[t], pb[t], pd[t], g, grad, gsel);
+            const float l = outputGrad(stage + r * kStageRow, dir[t], c[t] * a.scale, c[t], pb[t], pd[t], g, grad, gsel);
             if (g == 0 && valid[t]) lossAcc += l;
             waveSync();
 #pragma unroll
-            for (int q = 0; q < 10; ++q) stage[r * kGOutPad + 10 * g + q] = grad[q];
+            for (int q = 0; q < 10; ++q) stage[r * kStageRow + 10 * g + q] = grad[q];
             if (g == 0)
-                for (int q = kGOut; q < kGOutPad; ++q) stage[r * kGOutPad + q] = q == kGSel ? gsel : 0.0f;
+                for (int q = kGOut; q < kGOutPad; ++q) stage[r * kStageRow + q] = q == kGSel ? gsel : 0.0f;
             waveSync();
 #pragma unroll
             for (int kb = 0; kb < 2; ++kb) {
@@ -460,7 +470,7 @@ __global__ __launch_bounds__(512) void guideFwdBwdKernel(GuideTrainArgs a) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     const int idx = 32 * kb + 4 * g + (i & 3) + 16 * (i >> 2);
-                    v[i] = idx < kGOutPad ? stage[r * kGOutPad + idx] : 0.0f;
+                    v[i] = idx < kGOutPad ? stage[r * kStageRow + idx] : 0.0f;
                 }
                 dz[kb][t] = u32x4{packBf2(v[0], v[1]), packBf2(v[2], v[3]), packBf2(v[4], v[5]), packBf2(v[6], v[7])};
             }

@@ -30,6 +30,13 @@ __device__ __forceinline__ uint32_t packBf2(float lo, float hi) {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{lo, hi}, bf16x2));
 }
 
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+// Two FP32 values -> FP16 pair (lo in bits 0-15), round to nearest even, one v_cvt_pk_f16_f32.
+__device__ __forceinline__ uint32_t packH2(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{lo, hi}, f16x2));
+}
+
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 
 // ReLU of a BF16 pair on its bits: a BF16 number read as a 16-bit integer has the sign of the
@@ -60,7 +67,9 @@ __device__ __forceinline__ u32x4 guideEncodeBlock(const float* s9, int g, int kb
         }
         if (q % 3 == 0) {
             const float r = s9[q / 3] * __builtin_ldexpf(1.0f, 3 * g - 1);
-            const float f = r - floorf(r);
+            // r - floor(r) in one v_fract_f32 (which returns the largest float below 1 where
+            // that difference rounds to 1, for r just below an integer)
+            const float f = __builtin_amdgcn_fractf(r);
             st.x = __builtin_amdgcn_sinf(f);
             st.y = __builtin_amdgcn_cosf(f);
         } else {
