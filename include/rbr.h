@@ -231,7 +231,10 @@ int rbr_scene_build_bvh(rbr_scene* scene, int width, int leaf_max);
  * host's SAH-optimal program (| RBR_BVH_GREEDY: the host's largest-area rule), leaves of up to
  * leaf_max Morton-contiguous primitives, quantized by the host's quantizer.  Builds in milliseconds where the host SAH build takes hundreds; the
  * tree costs more node visits per ray.  Hits are the same (they do not depend on the tree).
- * Needs a GPU context; the host copies (host traversal, statistics) are filled as well. */
+ * Needs a GPU context.  The build's device temporaries stay with the context (grown as needed,
+ * released by rbr_ctx_destroy), so rebuilding allocates nothing once they fit; the host copies of
+ * the nodes and records (host traversal, rbr_scene_device_nodes / rbr_scene_bvh_arrays, renders
+ * of several scenes at once) are made on their first use, which may come from several threads. */
 int rbr_scene_build_bvh_device(rbr_scene* scene, int width, int leaf_max);
 /* Copies the node array as the kernels read it (layout of bvh.h; up to capacity bytes) and
  * reports its size: for tooling and layout tests.  Works on host-only contexts. */

@@ -99,6 +99,9 @@ struct WideBVH {
     int deviceRefOffset() const { return quantized ? qRefOffset(width) : 6 * width; }
     const void* deviceNodes() const { return quantized ? static_cast<const void*>(qnodes.data()) : nodes.data(); }
     size_t deviceNodeBytes() const { return quantized ? qnodes.size() * 4 : nodes.size() * sizeof(F4); }
+    // The same sizes from the counts (also while the host arrays of a device build are empty).
+    size_t nodeBytes() const { return size_t(nodeCount) * size_t(deviceNodeDwords()) * 4; }
+    size_t primBytes() const { return size_t(primCount) * 3 * sizeof(F4); }
 };
 
 // Fills b.qnodes from the float nodes (conservative 8-bit child boxes) and sets b.quantized.

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -47,6 +49,9 @@ struct rbr_ctx {
     int renderGraphs = 1;              // replay each round's launch sequence as a hipGraph
     int renderSort = 1;                // shade the extension queue in material order
     uint32_t renderCensus = 0;         // rays sampled per queue and bounce for rbr_last_render_stats (0: off)
+    // Temporaries of the device BVH build, kept (and grown) between builds so that a rebuild
+    // allocates nothing once they fit.
+    std::vector<rbr::DevBuf> buildTmp;
 };
 
 struct rbr_scene {
@@ -57,6 +62,11 @@ struct rbr_scene {
     bool built = false;
     rbr::DevBuf dNodes, dPrims, dQuadN, dPlanes, dPlaneIds;
     rbr::DevBuf dRays, dOut;   // staging for the host-array trace entry points
+    // Device build: the float nodes stay on the device, and the host copies of the nodes and
+    // records (host traversal, render arena, exports) are made on first use (ensureHostBvh).
+    rbr::DevBuf dFloatNodes;
+    std::atomic<bool> hostStale{false};
+    std::mutex hostMu;
     // Shading tables (render.hip).
     bool shaded = false;
     std::vector<rbr_material> materials;
@@ -93,9 +103,15 @@ int checkFault(rbr_ctx* ctx);
 // launches themselves allocate nothing (stream capture).
 int prepareTraversal(rbr_ctx* ctx, uint32_t maxStack);
 
-// Device build of the scene's wide BVH (bvh_gpu.hip): fills s->bvh (host copies) and the
-// scene's device node and primitive buffers.
+// Device build of the scene's wide BVH (bvh_gpu.hip): fills the scene's device node, float
+// node and primitive buffers and s->bvh's counts and bounds; the host arrays of s->bvh stay
+// empty until ensureHostBvh.
 int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool quant, bool optimal);
+
+// Makes s->bvh's host arrays (float nodes, device-layout nodes, records) of a device-built
+// tree from its device buffers, once; a no-op for host-built trees.  Safe to call from
+// several threads at once.
+int ensureHostBvh(const rbr_scene* s);
 
 // Gives the scene a new version number (after any change to it).
 void bumpVersion(rbr_scene* s);

@@ -62,6 +62,30 @@ int rbr::uploadBuffer(rbr_ctx* ctx, DevBuf& b, const void* src, size_t bytes) {
     return RBR_OK;
 }
 
+int rbr::ensureHostBvh(const rbr_scene* cs) {
+    rbr_scene* s = const_cast<rbr_scene*>(cs);   // the host copies are a cache of the device tree
+    if (!s->hostStale.load(std::memory_order_acquire)) return RBR_OK;
+    std::lock_guard<std::mutex> lock(s->hostMu);
+    if (!s->hostStale.load(std::memory_order_relaxed)) return RBR_OK;
+    rbr_ctx* ctx = s->ctx;
+    if (int rc = useDevice(ctx)) return rc;
+    WideBVH& b = s->bvh;
+    b.nodes.resize(size_t(b.nodeCount) * size_t(b.f4PerNode()));
+    b.prims.resize(size_t(b.primCount) * 3);
+    // float nodes: the scene's float copy of a quantized tree, else its device nodes
+    const void* floatSrc = b.quantized ? s->dFloatNodes.ptr : s->dNodes.ptr;
+    hipError_t e = hipMemcpyAsync(b.nodes.data(), floatSrc, b.nodes.size() * sizeof(F4), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && b.quantized) {
+        b.qnodes.resize(size_t(b.nodeCount) * size_t(qNodeDwords(b.width)));
+        e = hipMemcpyAsync(b.qnodes.data(), s->dNodes.ptr, b.qnodes.size() * 4, hipMemcpyDeviceToHost, ctx->stream);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(b.prims.data(), s->dPrims.ptr, b.primBytes(), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return failHip(ctx, e, "BVH host copy");
+    s->hostStale.store(false, std::memory_order_release);
+    return RBR_OK;
+}
+
 void rbr::bumpVersion(rbr_scene* s) {
     static std::atomic<uint64_t> counter{0};
     s->version = ++counter;
@@ -74,6 +98,7 @@ int rbr::useDevice(rbr_ctx* ctx) {
 }
 
 static void freeSceneDevice(rbr_scene* s) {
+    freeBuffer(s->dFloatNodes);
     freeBuffer(s->dNodes);
     freeBuffer(s->dPrims);
     freeBuffer(s->dQuadN);
@@ -262,6 +287,7 @@ void rbr_ctx_destroy(rbr_ctx* ctx) {
     freeBuffer(ctx->dCounter);
     freeBuffer(ctx->dSpill);
     freeBuffer(ctx->dFault);
+    for (DevBuf& b : ctx->buildTmp) freeBuffer(b);
     if (ctx->hFault) (void)hipHostFree(ctx->hFault);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
@@ -477,6 +503,8 @@ int rbr_scene_build_bvh(rbr_scene* s, int width, int leaf_max) {
     if (leaf_max < 1 || leaf_max > 8) return fail(ctx, RBR_ERR_INVALID, "leaf_max must be in 1..8");
     s->built = false;
     bumpVersion(s);
+    s->hostStale = false;
+    freeBuffer(s->dFloatNodes);
     buildWideBVH(s->geom, W, leaf_max, s->bvh, (width & RBR_BVH_GREEDY) == 0);
     if (width & kQuantFlag) quantizeWideBVH(s->bvh);
     if (!s->bvh.empty() && uint64_t(W - 1) * s->bvh.maxDepth + 1 > uint64_t(kMaxStack))
@@ -504,8 +532,10 @@ int rbr_scene_build_bvh_device(rbr_scene* s, int width, int leaf_max) {
     if (int rc = useDevice(ctx)) return rc;
     s->built = false;
     bumpVersion(s);
+    s->hostStale = false;
     if (int rc = buildWideBVHDevice(ctx, s, W, leaf_max, (width & kQuantFlag) != 0, (width & RBR_BVH_GREEDY) == 0))
         return rc;
+    s->hostStale = true;
     if (!s->bvh.empty() && uint64_t(W - 1) * s->bvh.maxDepth + 1 > uint64_t(kMaxStack))
         return fail(ctx, RBR_ERR_UNSUPPORTED, "BVH too deep for the traversal stack");
     int rc;
@@ -519,9 +549,12 @@ int rbr_scene_build_bvh_device(rbr_scene* s, int width, int leaf_max) {
 int rbr_scene_device_nodes(const rbr_scene* s, void* dst, size_t capacity, size_t* bytes) {
     if (!s) return RBR_ERR_INVALID;
     if (!s->built) return fail(s->ctx, RBR_ERR_STATE, "BVH not built");
-    const size_t n = s->bvh.deviceNodeBytes();
+    const size_t n = s->bvh.nodeBytes();
     if (bytes) *bytes = n;
-    if (dst && capacity) std::memcpy(dst, s->bvh.deviceNodes(), std::min(capacity, n));
+    if (dst && capacity) {
+        if (int rc = ensureHostBvh(s)) return rc;
+        std::memcpy(dst, s->bvh.deviceNodes(), std::min(capacity, n));
+    }
     return RBR_OK;
 }
 
@@ -529,9 +562,11 @@ int rbr_scene_bvh_arrays(const rbr_scene* s, void* nodes, size_t node_capacity, 
                          size_t* node_bytes, size_t* prim_bytes) {
     if (!s) return RBR_ERR_INVALID;
     if (!s->built) return fail(s->ctx, RBR_ERR_STATE, "BVH not built");
-    const size_t nb = s->bvh.nodes.size() * sizeof(F4), pb = s->bvh.prims.size() * sizeof(F4);
+    const size_t nb = size_t(s->bvh.nodeCount) * size_t(s->bvh.f4PerNode()) * sizeof(F4), pb = s->bvh.primBytes();
     if (node_bytes) *node_bytes = nb;
     if (prim_bytes) *prim_bytes = pb;
+    if ((nodes && node_capacity) || (prims && prim_capacity))
+        if (int rc = ensureHostBvh(s)) return rc;
     if (nodes && node_capacity) std::memcpy(nodes, s->bvh.nodes.data(), std::min(node_capacity, nb));
     if (prims && prim_capacity) std::memcpy(prims, s->bvh.prims.data(), std::min(prim_capacity, pb));
     return RBR_OK;
@@ -546,8 +581,8 @@ int rbr_scene_bvh_stats(const rbr_scene* s, rbr_bvh_stats* out) {
     out->prim_count = s->bvh.primCount;
     out->unbounded = uint32_t(s->geom.planeIds.size());
     out->max_depth = s->bvh.maxDepth;
-    out->node_bytes = uint64_t(s->bvh.deviceNodeBytes());
-    out->prim_bytes = uint64_t(s->bvh.prims.size()) * sizeof(F4);
+    out->node_bytes = uint64_t(s->bvh.nodeBytes());
+    out->prim_bytes = uint64_t(s->bvh.primBytes());
     out->sah_cost = s->bvh.sahCost;
     out->build_ms = s->bvh.buildMs;
     return RBR_OK;
@@ -666,6 +701,7 @@ int rbr_host_trace_closest(const rbr_scene* s, const rbr_ray* rays, size_t n, rb
     if (!s) return RBR_ERR_INVALID;
     if (!s->built) return fail(s->ctx, RBR_ERR_STATE, "trace before rbr_scene_build_bvh");
     if (n && (!rays || !hits)) return fail(s->ctx, RBR_ERR_INVALID, "null ray or result array");
+    if (int rc = ensureHostBvh(s)) return rc;
     TraceCounters total;
     for (size_t i = 0; i < n; ++i) {
         TraceCounters c;
@@ -686,6 +722,7 @@ int rbr_host_trace_any(const rbr_scene* s, const rbr_ray* rays, size_t n, uint8_
     if (!s) return RBR_ERR_INVALID;
     if (!s->built) return fail(s->ctx, RBR_ERR_STATE, "trace before rbr_scene_build_bvh");
     if (n && (!rays || !occ)) return fail(s->ctx, RBR_ERR_INVALID, "null ray or result array");
+    if (int rc = ensureHostBvh(s)) return rc;
     TraceCounters total;
     for (size_t i = 0; i < n; ++i) {
         TraceCounters c;

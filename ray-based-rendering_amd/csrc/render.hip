@@ -1724,6 +1724,7 @@ int censusStats(rbr_ctx* ctx, RenderState& R, const std::vector<SceneSlot>& scen
                 if (roots[sl * k + j] == sc.nodeOff)
                     group.insert(group.end(), &rays[(sl * k + j) * 8], &rays[(sl * k + j) * 8] + 8);
             if (group.empty() || sc.s->bvh.empty()) continue;
+            if (int rc = ensureHostBvh(sc.s)) return rc;
             const TraceCounters c = hostCount(sc.s->bvh, sc.s->geom, group.data(), group.size() / 8, kd == 1);
             nodes[kd] += w * double(c.nodes);
             prims[kd] += w * double(c.prims);
@@ -1882,6 +1883,8 @@ int renderImpl(rbr_ctx* ctx, const rbr_job* jobs, int njobs, double* const* momS
     } else if (scenes.size() == 1) {
         ta = sceneTraceArgs(scenes[0].s);
     } else {
+        for (const SceneSlot& sl : scenes)
+            if ((rc = ensureHostBvh(sl.s))) return rc;
         std::vector<uint32_t> nodes(size_t(nodeOff) * nodeDw);
         std::vector<F4> prims(size_t(primOff) * 3), quadN(quadOff);
         for (const SceneSlot& sl : scenes) {

@@ -2,8 +2,10 @@
 brute force / the host-built tree (hits do not depend on the tree), its structure obeys the
 layout's invariants (every primitive in exactly one leaf, padded child boxes that contain
 their primitives, leaves of at most leaf_max primitives), its quantized nodes decode to boxes
-that contain its float boxes, and rendering on it gives the host tree's images."""
+that contain its float boxes, rendering on it gives the host tree's images, and the host copies
+of its arrays (made on first host use) serve concurrent host traversals and repeated builds."""
 import sys
+import threading
 
 import numpy as np
 import pytest
@@ -147,3 +149,44 @@ def test_render_on_device_built_tree(ctx):
     fa = R.render(ctx, [dict(scene=a, spp=8, depth=8, seed=3)])[0]
     fb = R.render(ctx, [dict(scene=b, spp=8, depth=8, seed=3)])[0]
     assert np.array_equal(fa, fb)
+
+
+def test_host_copies_on_first_use_concurrent_and_rebuilt(ctx):
+    """A device build leaves the host arrays to be copied on first host use: eight threads that
+    start host traversals at once on a fresh tree all get the device's hits; a rebuild of the same
+    scene (temporaries reused from the context) with another layout, and a two-scene render
+    (whose node arena is assembled on the host), still agree with the host-built tree."""
+    say("case: host copies of a device-built tree, concurrent first use")
+    rng = np.random.default_rng(91)
+    w = worlds.make_world(rng, 200, 3000, False)
+    rays = worlds.make_rays(rng, 4096)
+    kw = worlds.geometry_kwargs(w)
+    sc = dev_scene(ctx, 4 | R.BVH_QUANTIZED, 4, **kw)
+    st = sc.stats()
+    ref = sc.trace_closest(rays)
+    out = [None] * 8
+
+    def work(k):
+        out[k] = sc.host_trace_closest(rays)[0]
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for o in out:
+        assert_same(o, ref)
+    assert st["node_bytes"] == sc.device_nodes().nbytes
+    say("case: rebuild on the device with another layout")
+    sc.build_bvh(8, 2, device=True)
+    assert_same(sc.trace_closest(rays), ref)
+    assert_same(sc.host_trace_closest(rays)[0], ref)
+    say("case: two-scene render on device-built trees")
+    desc = S.cornell_c0(32, 24)
+    a1, a2 = R.scene_from_desc(ctx, desc), R.scene_from_desc(ctx, S.cornell_c0(24, 32))
+    b1, b2 = R.scene_from_desc(ctx, desc, device_build=True), R.scene_from_desc(ctx, S.cornell_c0(24, 32), device_build=True)
+    jobs = lambda x, y: [dict(scene=x, spp=4, depth=6, seed=5), dict(scene=y, spp=4, depth=6, seed=6)]
+    fa = R.render(ctx, jobs(a1, a2))
+    fb = R.render(ctx, jobs(b1, b2))
+    for x, y in zip(fa, fb):
+        assert np.array_equal(x, y)
