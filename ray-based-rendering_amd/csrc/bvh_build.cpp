@@ -16,6 +16,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <utility>
+#include <vector>
 
 namespace rbr {
 
@@ -28,10 +31,34 @@ struct BPrim {
 
 struct BNode {
     float lo[3], hi[3];
-    int32_t left = -1, right = -1;
-    uint32_t start = 0, count = 0;
+    int32_t left, right;   // -1: leaf
+    uint32_t start, count;
     bool leaf() const { return left < 0; }
 };
+
+// Vectors whose elements are left uninitialised on allocation: the builder fills its large
+// arrays in parallel loops (which also spreads their first touch over the threads) instead of
+// having them zeroed serially first.
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <typename U>
+    NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+    template <typename U>
+    void construct(U* q) noexcept {
+        ::new (static_cast<void*>(q)) U;
+    }
+    template <typename U, typename... A>
+    void construct(U* q, A&&... a) {
+        ::new (static_cast<void*>(q)) U(std::forward<A>(a)...);
+    }
+};
+using PrimVec = std::vector<BPrim, NoInitAlloc<BPrim>>;
+using NodeVec = std::vector<BNode, NoInitAlloc<BNode>>;
 
 inline float area(const float lo[3], const float hi[3]) {
     float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
@@ -39,10 +66,13 @@ inline float area(const float lo[3], const float hi[3]) {
     return 2.0f * (dx * dy + dy * dz + dz * dx);
 }
 
+// Node indices are laid out by range, so no counter is shared between threads: the node of a
+// range of k primitives owns a block of 2k - 1 indices (itself first, then its left child's
+// block, then its right child's), which holds the subtree down to single primitives; where a
+// leaf keeps several primitives the rest of its block stays unused (an empty leaf).
 struct Builder {
-    std::vector<BPrim>& p;
-    std::vector<BNode>& nodes;
-    std::atomic<int32_t>& nextNode;
+    PrimVec& p;
+    NodeVec& nodes;
     int leafMax;
     static constexpr int kBins = 32;
 
@@ -52,41 +82,139 @@ struct Builder {
         nd.count = e - s;
     }
 
-    void build(int32_t ni, uint32_t s, uint32_t e, int depth) {
-        BNode& nd = nodes[ni];
-        float clo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, chi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
-        for (int k = 0; k < 3; ++k) { nd.lo[k] = 3.4e38f; nd.hi[k] = -3.4e38f; }
-        for (uint32_t i = s; i < e; ++i) {
+    // Bounds of a range: node box and centroid box.
+    struct Bounds {
+        float lo[3], hi[3], clo[3], chi[3];
+        void clear() {
+            for (int k = 0; k < 3; ++k) { lo[k] = clo[k] = 3.4e38f; hi[k] = chi[k] = -3.4e38f; }
+        }
+        void add(const BPrim& q) {
             for (int k = 0; k < 3; ++k) {
-                nd.lo[k] = std::min(nd.lo[k], p[i].lo[k]);
-                nd.hi[k] = std::max(nd.hi[k], p[i].hi[k]);
-                clo[k] = std::min(clo[k], p[i].c[k]);
-                chi[k] = std::max(chi[k], p[i].c[k]);
+                lo[k] = std::min(lo[k], q.lo[k]);
+                hi[k] = std::max(hi[k], q.hi[k]);
+                clo[k] = std::min(clo[k], q.c[k]);
+                chi[k] = std::max(chi[k], q.c[k]);
             }
         }
+        void merge(const Bounds& o) {
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], o.lo[k]);
+                hi[k] = std::max(hi[k], o.hi[k]);
+                clo[k] = std::min(clo[k], o.clo[k]);
+                chi[k] = std::max(chi[k], o.chi[k]);
+            }
+        }
+    };
+    // The 32 bins of the three axes, filled in one pass over a range.
+    struct Bins {
+        int cnt[3][kBins];
+        float lo[3][kBins][3], hi[3][kBins][3];
+        void clear() {
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < kBins; ++b) {
+                    cnt[a][b] = 0;
+                    for (int k = 0; k < 3; ++k) { lo[a][b][k] = 3.4e38f; hi[a][b][k] = -3.4e38f; }
+                }
+        }
+        void merge(const Bins& o) {
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < kBins; ++b) {
+                    cnt[a][b] += o.cnt[a][b];
+                    for (int k = 0; k < 3; ++k) {
+                        lo[a][b][k] = std::min(lo[a][b][k], o.lo[a][b][k]);
+                        hi[a][b][k] = std::max(hi[a][b][k], o.hi[a][b][k]);
+                    }
+                }
+        }
+    };
+    // Ranges above this size are scanned in parallel chunks (min, max and counts merge exactly,
+    // so the split found does not depend on the chunking).
+    static constexpr uint32_t kParallelScan = 1u << 17, kScanChunk = 1u << 15;
+
+    void bounds(uint32_t s, uint32_t e, Bounds& out) const {
+        out.clear();
+        if (e - s <= kParallelScan) {
+            for (uint32_t i = s; i < e; ++i) out.add(p[i]);
+            return;
+        }
+        const uint32_t chunks = (e - s + kScanChunk - 1) / kScanChunk;
+        std::vector<Bounds> part(chunks);
+#pragma omp taskloop default(shared) grainsize(1)
+        for (uint32_t c = 0; c < chunks; ++c) {
+            part[c].clear();
+            const uint32_t a = s + c * kScanChunk, b = std::min(e, a + kScanChunk);
+            for (uint32_t i = a; i < b; ++i) part[c].add(p[i]);
+        }
+        for (const Bounds& b : part) out.merge(b);
+    }
+
+    void binRange(uint32_t s, uint32_t e, const float cmin[3], const float scale[3], const bool use[3], Bins& out) const {
+        for (uint32_t i = s; i < e; ++i) {
+            const BPrim& q = p[i];
+            for (int ax = 0; ax < 3; ++ax) {
+                if (!use[ax]) continue;
+                const int b = std::min(kBins - 1, int((q.c[ax] - cmin[ax]) * scale[ax]));
+                out.cnt[ax][b]++;
+                for (int k = 0; k < 3; ++k) {
+                    out.lo[ax][b][k] = std::min(out.lo[ax][b][k], q.lo[k]);
+                    out.hi[ax][b][k] = std::max(out.hi[ax][b][k], q.hi[k]);
+                }
+            }
+        }
+    }
+
+    void bins(uint32_t s, uint32_t e, const float cmin[3], const float scale[3], const bool use[3], Bins& out) const {
+        out.clear();
+        if (e - s <= kParallelScan) {
+            binRange(s, e, cmin, scale, use, out);
+            return;
+        }
+        const uint32_t chunks = (e - s + kScanChunk - 1) / kScanChunk;
+        std::vector<Bins> part(chunks);
+#pragma omp taskloop default(shared) grainsize(1)
+        for (uint32_t c = 0; c < chunks; ++c) {
+            part[c].clear();
+            const uint32_t a = s + c * kScanChunk;
+            binRange(a, std::min(e, a + kScanChunk), cmin, scale, use, part[c]);
+        }
+        for (const Bins& b : part) out.merge(b);
+    }
+
+    void build(int32_t ni, uint32_t s, uint32_t e, int depth) {
+        BNode& nd = nodes[ni];
+        Bounds bd;
+        bounds(s, e, bd);
+        const float* clo = bd.clo;
+        const float* chi = bd.chi;
+        for (int k = 0; k < 3; ++k) { nd.lo[k] = bd.lo[k]; nd.hi[k] = bd.hi[k]; }
         const uint32_t n = e - s;
         if (n <= 1 || depth > 60) { leaf(nd, s, e); if (n > (uint32_t)leafMax) forceSplit(ni, s, e, depth); return; }
 
-        // Binned SAH on every axis.
+        // Binned SAH on every axis (one pass fills the bins of all three).
         float bestCost = 3.4e38f;
         int bestAxis = -1, bestBin = -1;
         const float parentArea = area(nd.lo, nd.hi);
+        float scales[3];
+        bool use[3];
         for (int ax = 0; ax < 3; ++ax) {
-            float ext = chi[ax] - clo[ax];
-            if (!(ext > 0.0f)) continue;
-            float scale = float(kBins) * (1.0f - 1e-6f) / ext;
-            int cnt[kBins] = {0};
-            float blo[kBins][3], bhi[kBins][3];
-            for (int b = 0; b < kBins; ++b)
-                for (int k = 0; k < 3; ++k) { blo[b][k] = 3.4e38f; bhi[b][k] = -3.4e38f; }
-            for (uint32_t i = s; i < e; ++i) {
-                int b = std::min(kBins - 1, int((p[i].c[ax] - clo[ax]) * scale));
-                cnt[b]++;
-                for (int k = 0; k < 3; ++k) {
-                    blo[b][k] = std::min(blo[b][k], p[i].lo[k]);
-                    bhi[b][k] = std::max(bhi[b][k], p[i].hi[k]);
-                }
+            const float ext = chi[ax] - clo[ax];
+            use[ax] = ext > 0.0f;
+            scales[ax] = use[ax] ? float(kBins) * (1.0f - 1e-6f) / ext : 0.0f;
+        }
+        std::unique_ptr<Bins> dense;
+        if (n > kSparse) {
+            dense.reset(new Bins);
+            bins(s, e, clo, scales, use, *dense);
+        }
+        for (int ax = 0; ax < 3; ++ax) {
+            if (!use[ax]) continue;
+            if (n <= kSparse) {
+                sparseBins(s, e, ax, clo[ax], scales[ax], parentArea, bestCost, bestAxis, bestBin);
+                continue;
             }
+            const int* cnt = dense->cnt[ax];
+            const float(*blo)[3] = dense->lo[ax];
+            const float(*bhi)[3] = dense->hi[ax];
             float rightArea[kBins];
             int rightCnt[kBins];
             float rlo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, rhi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
@@ -125,31 +253,86 @@ struct Builder {
         split(ni, s, m, e, depth);
     }
 
+    // The binned SAH sweep of one axis over the occupied bins only, for small ranges: the same
+    // candidates in the same order with the same float operations as the 32-bin sweep (an
+    // empty bin changes neither box nor count, and of equal costs the first is kept), so it
+    // picks the same split; it just skips the empty bins, which dominate below a few dozen
+    // primitives.
+    static constexpr uint32_t kSparse = 24;
+    void sparseBins(uint32_t s, uint32_t e, int ax, float cmin, float scale, float parentArea, float& bestCost,
+                    int& bestAxis, int& bestBin) const {
+        const uint32_t n = e - s;
+        uint8_t bin[kSparse];
+        uint8_t ord[kSparse];
+        for (uint32_t i = 0; i < n; ++i) {
+            bin[i] = uint8_t(std::min(kBins - 1, int((p[s + i].c[ax] - cmin) * scale)));
+            uint32_t j = i;   // insertion sort of the primitives by bin
+            while (j > 0 && bin[ord[j - 1]] > bin[i]) { ord[j] = ord[j - 1]; --j; }
+            ord[j] = uint8_t(i);
+        }
+        // occupied bins in ascending order: bin, count, box
+        int ob[kSparse], oc[kSparse], m = 0;
+        float olo[kSparse][3], ohi[kSparse][3];
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t i = ord[k];
+            if (m == 0 || ob[m - 1] != bin[i]) {
+                ob[m] = bin[i];
+                oc[m] = 0;
+                for (int q = 0; q < 3; ++q) { olo[m][q] = 3.4e38f; ohi[m][q] = -3.4e38f; }
+                ++m;
+            }
+            ++oc[m - 1];
+            for (int q = 0; q < 3; ++q) {
+                olo[m - 1][q] = std::min(olo[m - 1][q], p[s + i].lo[q]);
+                ohi[m - 1][q] = std::max(ohi[m - 1][q], p[s + i].hi[q]);
+            }
+        }
+        float rightArea[kSparse];
+        int rightCnt[kSparse];
+        float rlo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, rhi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+        int rc = 0;
+        for (int j = m - 1; j > 0; --j) {
+            rc += oc[j];
+            for (int q = 0; q < 3; ++q) { rlo[q] = std::min(rlo[q], olo[j][q]); rhi[q] = std::max(rhi[q], ohi[j][q]); }
+            rightArea[j] = area(rlo, rhi);
+            rightCnt[j] = rc;
+        }
+        float llo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, lhi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+        int lc = 0;
+        for (int j = 0; j + 1 < m; ++j) {
+            lc += oc[j];
+            for (int q = 0; q < 3; ++q) { llo[q] = std::min(llo[q], olo[j][q]); lhi[q] = std::max(lhi[q], ohi[j][q]); }
+            if (ob[j] > kBins - 2) continue;
+            float cost = 1.0f + (area(llo, lhi) * lc + rightArea[j + 1] * rightCnt[j + 1]) / std::max(parentArea, 1e-30f);
+            if (cost < bestCost) { bestCost = cost; bestAxis = ax; bestBin = ob[j]; }
+        }
+    }
+
     void forceSplit(int32_t ni, uint32_t s, uint32_t e, int depth) {
         split(ni, s, s + (e - s) / 2, e, depth);
     }
 
     void split(int32_t ni, uint32_t s, uint32_t m, uint32_t e, int depth) {
-        int32_t l = nextNode.fetch_add(2);
+        const int32_t l = ni + 1, r = ni + 2 * int32_t(m - s);
         nodes[ni].left = l;
-        nodes[ni].right = l + 1;
+        nodes[ni].right = r;
         nodes[ni].start = s;
         nodes[ni].count = e - s;
         if (e - s > 16384) {
 #pragma omp task default(shared) firstprivate(l, s, m, depth)
             build(l, s, m, depth + 1);
-#pragma omp task default(shared) firstprivate(l, m, e, depth)
-            build(l + 1, m, e, depth + 1);
+#pragma omp task default(shared) firstprivate(r, m, e, depth)
+            build(r, m, e, depth + 1);
 #pragma omp taskwait
         } else {
             build(l, s, m, depth + 1);
-            build(l + 1, m, e, depth + 1);
+            build(r, m, e, depth + 1);
         }
     }
 };
 
 struct Collapser {
-    const std::vector<BNode>& bn;
+    const NodeVec& bn;
     WideBVH& out;
     int W;
     float pad;
@@ -233,7 +416,7 @@ struct Collapser {
 // collapse above fixes the leaves first and fills each node by area alone.
 struct OptimalCollapser {
     static constexpr int8_t kLeaf = -1, kWide = -2, kFewer = -3, kDist = 0;
-    const std::vector<BNode>& bn;
+    const NodeVec& bn;
     WideBVH& out;
     int W, leafMax;
     float pad, cNode, cPrim;
@@ -246,39 +429,63 @@ struct OptimalCollapser {
         cost.assign(n * W, INFINITY);
         how.assign(n * W, kFewer);
         split.assign(n * W, 1);
-        // Children are allocated after their parent, so descending indices visit them first.
-        for (size_t m = n; m-- > 0;) {
-            const BNode& b = bn[m];
-            const float a = area(b.lo, b.hi);
-            const float leafC = b.count <= uint32_t(leafMax) ? a * float(b.count) * cPrim : INFINITY;
-            float* c = &cost[m * W];
-            int8_t* h = &how[m * W];
-            if (b.leaf()) {
-                c[0] = leafC;
-                h[0] = kLeaf;
-                for (int i = 1; i < W; ++i) c[i] = c[0];
-                continue;
+#pragma omp parallel
+#pragma omp single
+        solveSubtree(0);
+    }
+
+    // Post-order over binary subtree m (a node's table needs its children's); subtrees of more
+    // than kTaskPrims primitives run as tasks.  Each table is a function of the children's
+    // alone, so the result does not depend on the schedule.
+    static constexpr uint32_t kTaskPrims = 16384;
+    void solveSubtree(int32_t m) {
+        const BNode& b = bn[m];
+        if (!b.leaf()) {
+            if (b.count > kTaskPrims) {
+#pragma omp task default(shared)
+                solveSubtree(b.left);
+#pragma omp task default(shared)
+                solveSubtree(b.right);
+#pragma omp taskwait
+            } else {
+                solveSubtree(b.left);
+                solveSubtree(b.right);
             }
-            const float* cl = &cost[size_t(b.left) * W];
-            const float* cr = &cost[size_t(b.right) * W];
-            float dist[9];
-            for (int j = 2; j <= W; ++j) {
-                float best = INFINITY;
-                int bk = 1;
-                for (int k = 1; k < j; ++k) {
-                    const float v = cl[k - 1] + cr[j - k - 1];
-                    if (v < best) { best = v; bk = k; }
-                }
-                dist[j] = best;
-                split[m * W + j - 1] = int8_t(bk);
+        }
+        solveNode(size_t(m));
+    }
+
+    void solveNode(size_t m) {
+        const BNode& b = bn[m];
+        const float a = area(b.lo, b.hi);
+        const float leafC = b.count <= uint32_t(leafMax) ? a * float(b.count) * cPrim : INFINITY;
+        float* c = &cost[m * W];
+        int8_t* h = &how[m * W];
+        if (b.leaf()) {
+            c[0] = leafC;
+            h[0] = kLeaf;
+            for (int i = 1; i < W; ++i) c[i] = c[0];
+            return;
+        }
+        const float* cl = &cost[size_t(b.left) * W];
+        const float* cr = &cost[size_t(b.right) * W];
+        float dist[9];
+        for (int j = 2; j <= W; ++j) {
+            float best = INFINITY;
+            int bk = 1;
+            for (int k = 1; k < j; ++k) {
+                const float v = cl[k - 1] + cr[j - k - 1];
+                if (v < best) { best = v; bk = k; }
             }
-            const float wideC = a * cNode + dist[W];
-            if (leafC <= wideC) { c[0] = leafC; h[0] = kLeaf; }
-            else { c[0] = wideC; h[0] = kWide; }
-            for (int i = 2; i <= W; ++i) {
-                if (dist[i] < c[i - 2]) { c[i - 1] = dist[i]; h[i - 1] = kDist; }
-                else { c[i - 1] = c[i - 2]; h[i - 1] = kFewer; }
-            }
+            dist[j] = best;
+            split[m * W + j - 1] = int8_t(bk);
+        }
+        const float wideC = a * cNode + dist[W];
+        if (leafC <= wideC) { c[0] = leafC; h[0] = kLeaf; }
+        else { c[0] = wideC; h[0] = kWide; }
+        for (int i = 2; i <= W; ++i) {
+            if (dist[i] < c[i - 2]) { c[i - 1] = dist[i]; h[i - 1] = kDist; }
+            else { c[i - 1] = c[i - 2]; h[i - 1] = kFewer; }
         }
     }
 
@@ -303,13 +510,10 @@ struct OptimalCollapser {
         }
     }
 
-    // Emits the wide node whose slots are drawn from binary node b and returns its index;
-    // a root that is best kept as one leaf becomes a node with that single leaf.
-    uint32_t emit(int32_t b, uint32_t depth) {
-        out.maxDepth = std::max(out.maxDepth, depth);
-        int32_t kids[8];
-        bool leaf[8];
-        int nk = 0;
+    // The slots of the wide node drawn from binary node b; a root that is best kept as one
+    // leaf becomes a node with that single leaf.
+    void slots(int32_t b, int32_t* kids, bool* leaf, int& nk) const {
+        nk = 0;
         if (bn[b].leaf() || how[size_t(b) * W] == kLeaf) {
             kids[nk] = b;
             leaf[nk++] = true;
@@ -318,25 +522,76 @@ struct OptimalCollapser {
             gather(bn[b].left, k, kids, leaf, nk);
             gather(bn[b].right, W - k, kids, leaf, nk);
         }
-        const uint32_t idx = out.nodeCount++;
+    }
+
+    // Wide nodes in the subtree of the wide node drawn from b (itself included), kept per
+    // binary node for the emission.
+    std::vector<uint32_t> wideSize;
+    uint32_t sizeSubtree(int32_t b) {
+        int32_t kids[8];
+        bool leaf[8];
+        int nk;
+        slots(b, kids, leaf, nk);
+        uint32_t part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const bool par = bn[b].count > kTaskPrims;
+        for (int i = 0; i < nk; ++i) {
+            if (leaf[i]) continue;
+            if (par) {
+#pragma omp task default(shared) firstprivate(i)
+                part[i] = sizeSubtree(kids[i]);
+            } else {
+                part[i] = sizeSubtree(kids[i]);
+            }
+        }
+        if (par) {
+#pragma omp taskwait
+        }
+        uint32_t total = 1;
+        for (int i = 0; i < nk; ++i) total += part[i];
+        wideSize[b] = total;
+        return total;
+    }
+
+    // Emits the wide nodes in depth-first pre-order: the node drawn from b at index idx, its
+    // interior children at idx + 1 + the sizes of the subtrees before them, so subtrees can be
+    // written in parallel and still land where a serial depth-first emission puts them.
+    std::atomic<uint32_t> leaves{0}, deepest{0};
+    void emitAt(int32_t b, uint32_t idx, uint32_t depth) {
+        uint32_t d = deepest.load(std::memory_order_relaxed);
+        while (depth > d && !deepest.compare_exchange_weak(d, depth, std::memory_order_relaxed)) {}
+        int32_t kids[8];
+        bool leaf[8];
+        int nk;
+        slots(b, kids, leaf, nk);
         const int per = out.f4PerNode();
-        out.nodes.resize(size_t(out.nodeCount) * per);
         uint32_t child[8];
         float blo[8][3], bhi[8][3];
         for (int i = 0; i < W; ++i) {
             child[i] = kEmpty;
             for (int k = 0; k < 3; ++k) { blo[i][k] = INFINITY; bhi[i][k] = -INFINITY; }
         }
+        const bool par = bn[b].count > kTaskPrims;
+        uint32_t next = idx + 1, nleaf = 0;
         for (int i = 0; i < nk; ++i) {
             const BNode& c = bn[kids[i]];
             padBox(c.lo, c.hi, blo[i], bhi[i]);
             if (leaf[i]) {
                 child[i] = makeLeaf(c.start, c.count);
-                out.leafCount++;
+                ++nleaf;
+                continue;
+            }
+            child[i] = next;
+            const int32_t kb = kids[i];
+            const uint32_t at = next;
+            next += wideSize[kb];
+            if (par) {
+#pragma omp task default(shared) firstprivate(kb, at)
+                emitAt(kb, at, depth + 1);
             } else {
-                child[i] = emit(kids[i], depth + 1);
+                emitAt(kb, at, depth + 1);
             }
         }
+        leaves.fetch_add(nleaf, std::memory_order_relaxed);
         float* f = reinterpret_cast<float*>(&out.nodes[size_t(idx) * per]);
         for (int i = 0; i < W; ++i) {
             f[0 * W + i] = blo[i][0];
@@ -347,7 +602,25 @@ struct OptimalCollapser {
             f[5 * W + i] = bhi[i][2];
             std::memcpy(&f[6 * W + i], &child[i], 4);
         }
-        return idx;
+        if (par) {
+#pragma omp taskwait
+        }
+    }
+
+    // The whole wide tree from binary root 0.
+    void emitAll() {
+        wideSize.assign(bn.size(), 0u);
+        uint32_t total = 0;
+#pragma omp parallel
+#pragma omp single
+        total = sizeSubtree(0);
+        out.nodeCount = total;
+        out.nodes.assign(size_t(total) * out.f4PerNode(), F4{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma omp parallel
+#pragma omp single
+        emitAt(0, 0, 1);
+        out.leafCount = leaves.load();
+        out.maxDepth = deepest.load();
     }
 };
 
@@ -369,7 +642,7 @@ void buildWideBVH(const HostGeometry& g, int width, int leafMax, WideBVH& out, b
     out.primCount = n;
     if (n == 0) return;
 
-    std::vector<BPrim> p(n);
+    PrimVec p(n);
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < (long long)ntri; ++i) {
         const float* t = &g.tris[9 * size_t(i)];
@@ -392,7 +665,8 @@ void buildWideBVH(const HostGeometry& g, int width, int leafMax, WideBVH& out, b
         q.index = i;
     }
     float wlo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, whi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
-    for (uint32_t i = 0; i < n; ++i) {
+#pragma omp parallel for schedule(static) reduction(min : wlo[:3]) reduction(max : whi[:3])
+    for (long long i = 0; i < (long long)n; ++i) {
         for (int k = 0; k < 3; ++k) {
             p[i].c[k] = 0.5f * (p[i].lo[k] + p[i].hi[k]);
             wlo[k] = std::min(wlo[k], p[i].lo[k]);
@@ -401,15 +675,20 @@ void buildWideBVH(const HostGeometry& g, int width, int leafMax, WideBVH& out, b
     }
     for (int k = 0; k < 3; ++k) { out.worldLo[k] = wlo[k]; out.worldHi[k] = whi[k]; }
 
-    std::vector<BNode> bn(size_t(2) * n + 1);
-    std::atomic<int32_t> next{1};
+    NodeVec bn(size_t(2) * n - 1);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)bn.size(); ++i) {   // unused blocks read as empty leaves
+        BNode& x = bn[i];
+        for (int k = 0; k < 3; ++k) x.lo[k] = x.hi[k] = 0.0f;
+        x.left = x.right = -1;
+        x.start = x.count = 0;
+    }
     // The optimal collapse picks the leaves itself, so its binary tree goes down to single
     // primitives.
-    Builder b{p, bn, next, optimal ? 1 : std::max(1, std::min(leafMax, int(kMaxLeaf)))};
+    Builder b{p, bn, optimal ? 1 : std::max(1, std::min(leafMax, int(kMaxLeaf)))};
 #pragma omp parallel
 #pragma omp single
     b.build(0, 0, n, 0);
-    bn.resize(next.load());
 
     // SAH cost of the binary tree (Ct = 1, Ci = 1), relative to the root area.
     const float rootA = std::max(area(bn[0].lo, bn[0].hi), 1e-30f);
@@ -423,7 +702,7 @@ void buildWideBVH(const HostGeometry& g, int width, int leafMax, WideBVH& out, b
         OptimalCollapser c{bn, out, width, std::max(1, std::min(leafMax, int(kMaxLeaf))), 1e-5f * ext + 1e-6f, 1.0f,
                            primCost, {}, {}, {}};
         c.solve();
-        c.emit(0, 1);
+        c.emitAll();
     } else {
         Collapser c{bn, out, width, 1e-5f * ext + 1e-6f};
         c.emit(0, 1);
@@ -431,7 +710,8 @@ void buildWideBVH(const HostGeometry& g, int width, int leafMax, WideBVH& out, b
 
     // Leaf-ordered primitive records.
     out.prims.resize(size_t(n) * 3);
-    for (uint32_t i = 0; i < n; ++i) {
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i) {
         const BPrim& q = p[i];
         F4* r = &out.prims[size_t(i) * 3];
         float idf, kf;

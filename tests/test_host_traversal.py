@@ -235,3 +235,39 @@ def test_host_quads_equal_brute_force(width):
     assert len(bad) == 0, (len(bad), h[bad[:4]], ref[bad[:4]])
     occ, _, _ = sc.host_trace_any(rays)
     assert np.array_equal(occ, O.OracleScene(quads=q).any(rays, brute=True))
+
+
+_BUILD_HASH = r"""
+import hashlib, sys
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+import rbr_amd as R
+tris = R.icosphere(96, 100.0, 0.1, 0xC1)
+ctx = R.Context(-1)
+out = []
+for width, leaf in ((4 | R.BVH_QUANTIZED, 4), (8 | R.BVH_GREEDY, 3)):
+    sc = R.Scene(ctx)
+    sc.set_geometry(tris=tris)
+    sc.build_bvh(width, leaf)
+    out.append(hashlib.md5(sc.device_nodes().tobytes() + sc.prim_records().tobytes()).hexdigest())
+print(" ".join(out))
+"""
+
+
+def test_host_build_is_deterministic_across_thread_counts():
+    """The host build splits its work over OpenMP tasks (parallel scans of large ranges, node
+    indices laid out by range, subtrees solved and emitted in parallel); the tree must not
+    depend on the thread count: 184,320 triangles (above the sizes where the parallel paths
+    start), optimal and greedy collapse, built with 1 and 4 threads."""
+    import subprocess
+    import sys
+
+    pkg = os.path.join(ROOT, "ray-based-rendering_amd")
+    hashes = set()
+    for threads in ("1", "4"):
+        env = dict(os.environ, OMP_NUM_THREADS=threads)
+        res = subprocess.run([sys.executable, "-c", _BUILD_HASH, pkg], env=env, capture_output=True, text=True,
+                             timeout=300)
+        assert res.returncode == 0, res.stderr
+        hashes.add(res.stdout.strip())
+    assert len(hashes) == 1, hashes
