@@ -222,7 +222,8 @@ def main() -> int:
             "metric": "time-to-relMSE guided vs plain GPU PT",
             "reference": {"spp": args.ref_spp, "accumulation": "fp64", "ref_relmse": ref_err},
             "schedule": sched,
-            "training": {"batch": args.batch, "lr": args.lr, "guide_fraction": 0.5,
+            "training": {"batch": args.batch, "lr": args.lr,
+                         "guide_selection": "learned per vertex, s in [0.1, 0.9]",
                          "loss_first": losses[0] if losses else None,
                          "loss_last": losses[-1] if losses else None},
             "max_spp": args.max_spp,
