@@ -225,6 +225,9 @@ int rbr_triangle_shading_normal(const float* verts, const float* normals, const 
  * 8-wide node instead of 128 / 256 B); the boxes are rounded outward, so hits are unchanged. */
 #define RBR_BVH_QUANTIZED 0x100
 #define RBR_BVH_GREEDY 0x200
+/* Device build only: the binary tree by locally-ordered clustering (mutual nearest neighbours
+ * within 16 positions of the Morton order, merged round by round) instead of the radix tree. */
+#define RBR_BVH_PLOC 0x400
 int rbr_scene_build_bvh(rbr_scene* scene, int width, int leaf_max);
 /* The same layouts built on the GPU (no reference counterpart; the reference builds serially
  * on the host, bvh.cpp:47-209): Morton-ordered radix tree (LBVH), collapsed to `width` by the

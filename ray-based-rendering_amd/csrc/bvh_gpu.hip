@@ -18,10 +18,11 @@
 //   6. leaf-ordered primitive records (bvh.h), and for quantized layouts the 8-bit nodes by
 //      the host's own quantizer (bvh_quant.h), so both produce the same bits from the same
 //      float nodes
-// Nodes, records and counts are then copied to the host structure as well (host traversal,
-// statistics, the render arena).
+// With RBR_BVH_PLOC, step 3 is locally-ordered clustering instead (gbPloc* below).  The host
+// copies of nodes and records are made on their first use (ensureHostBvh).
 #include <hip/hip_runtime.h>
 #include <hipcub/device/device_radix_sort.hpp>
+#include <hipcub/device/device_scan.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -185,6 +186,191 @@ __global__ __launch_bounds__(256) void gbKarrasKernel(uint32_t n, const uint32_t
     else parentInt[lc] = uint32_t(i);
     if (rc & kLeafTag) parentLeaf[rc & ~kLeafTag] = uint32_t(i);
     else parentInt[rc] = uint32_t(i);
+}
+
+// ---- Locally-ordered clustering (after Meister & Bittner 2018, "Parallel Locally-Ordered
+// Clustering for Bounding Volume Hierarchy Construction"): the sorted primitives start as
+// clusters; every round, each cluster finds the cluster within kPlocRadius positions whose
+// union box has the least surface area, mutual nearest neighbours merge (the left one keeps
+// the place of the pair) and the array is compacted, until one cluster is left.  The result
+// replaces the radix tree: the same arrays (children, parent links, leaf ranges) in the same
+// encoding, with leaves renumbered to the depth-first order of the new tree so that every
+// subtree still covers a contiguous range of positions.
+constexpr int kPlocRadius = 16;
+
+__device__ __forceinline__ float unionArea(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+    const float dx = mx(ahi.x, bhi.x) - mn(alo.x, blo.x);
+    const float dy = mx(ahi.y, bhi.y) - mn(alo.y, blo.y);
+    const float dz = mx(ahi.z, bhi.z) - mn(alo.z, blo.z);
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// Nearest neighbour of every cluster by the key (union area, lower index, higher index): a
+// strict order on pairs, so the least pair of a round is mutual and every round merges.
+__global__ __launch_bounds__(256) void gbPlocNearestKernel(uint32_t m, const float4* __restrict__ lo,
+                                                           const float4* __restrict__ hi, int32_t* __restrict__ nn) {
+    __shared__ float4 slo[256 + 2 * kPlocRadius], shi[256 + 2 * kPlocRadius];
+    const int base = int(blockIdx.x * 256) - kPlocRadius;
+    for (int t = threadIdx.x; t < 256 + 2 * kPlocRadius; t += 256) {
+        const int k = base + t;
+        if (k >= 0 && k < int(m)) {
+            slo[t] = lo[k];
+            shi[t] = hi[k];
+        }
+    }
+    __syncthreads();
+    const int i = int(blockIdx.x * 256 + threadIdx.x);
+    if (i >= int(m)) return;
+    const int c = int(threadIdx.x) + kPlocRadius;
+    const float4 a = slo[c], b = shi[c];
+    float best = INFINITY;
+    int bj = -1;
+    for (int d = -kPlocRadius; d <= kPlocRadius; ++d) {
+        const int j = i + d;
+        if (d == 0 || j < 0 || j >= int(m)) continue;
+        const float ar = unionArea(a, b, slo[c + d], shi[c + d]);
+        // for equal areas the pair with the lower (then higher) index wins; i is common to both
+        const bool better = bj < 0 || ar < best ||
+                            (ar == best && (min(i, j) < min(i, bj) || (min(i, j) == min(i, bj) && max(i, j) < max(i, bj))));
+        if (better) {
+            best = ar;
+            bj = j;
+        }
+    }
+    nn[i] = bj;
+}
+
+// Pairs (2k, 2k + 1): used for a round in which no pair was mutual (only possible with
+// non-finite boxes), so the clustering always ends.
+__global__ __launch_bounds__(256) void gbPlocPairKernel(uint32_t m, int32_t* __restrict__ nn) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t j = i ^ 1u;
+    nn[i] = j < m ? int32_t(j) : -1;
+}
+
+__device__ __forceinline__ uint32_t clusterCount(uint32_t code, const uint32_t* count) {
+    return (code & kLeafTag) ? 1u : count[code];
+}
+
+// A mutual pair (i < j) becomes internal node (n - 2 - merge number), so the last merge, the
+// root, is node 0; keep[i] says whether position i survives the round.
+__global__ __launch_bounds__(256) void gbPlocMergeKernel(uint32_t m, uint32_t n, const uint32_t* __restrict__ C,
+                                                         const float4* __restrict__ lo, const float4* __restrict__ hi,
+                                                         const int32_t* __restrict__ nn, uint32_t* merges,
+                                                         uint32_t* __restrict__ left, uint32_t* __restrict__ right,
+                                                         uint32_t* __restrict__ parentInt, uint32_t* __restrict__ parentLeaf,
+                                                         uint32_t* __restrict__ count, uint32_t* __restrict__ outC,
+                                                         float4* __restrict__ outLo, float4* __restrict__ outHi,
+                                                         uint32_t* __restrict__ keep) {
+    const int i = int(blockIdx.x * 256 + threadIdx.x);
+    if (i >= int(m)) return;
+    const int j = nn[i];
+    const bool mutual = j >= 0 && nn[j] == i;
+    if (mutual && i > j) {
+        keep[i] = 0u;
+        return;
+    }
+    keep[i] = 1u;
+    if (!mutual) {
+        outC[i] = C[i];
+        outLo[i] = lo[i];
+        outHi[i] = hi[i];
+        return;
+    }
+    const uint32_t a = C[i], b = C[j];
+    const uint32_t id = (n - 2u) - atomicAdd(merges, 1u);
+    left[id] = a;
+    right[id] = b;
+    if (a & kLeafTag) parentLeaf[a & ~kLeafTag] = id;
+    else parentInt[a] = id;
+    if (b & kLeafTag) parentLeaf[b & ~kLeafTag] = id;
+    else parentInt[b] = id;
+    count[id] = clusterCount(a, count) + clusterCount(b, count);
+    outC[i] = id;
+    const float4 la = lo[i], lb = lo[j], ha = hi[i], hb = hi[j];
+    outLo[i] = make_float4(mn(la.x, lb.x), mn(la.y, lb.y), mn(la.z, lb.z), 0.0f);
+    outHi[i] = make_float4(mx(ha.x, hb.x), mx(ha.y, hb.y), mx(ha.z, hb.z), 0.0f);
+}
+
+// Compaction of the survivors (pos: exclusive prefix sum of keep); the new count goes to
+// *mOut.
+__global__ __launch_bounds__(256) void gbPlocCompactKernel(uint32_t m, const uint32_t* __restrict__ keep,
+                                                           const uint32_t* __restrict__ pos, const uint32_t* __restrict__ inC,
+                                                           const float4* __restrict__ inLo, const float4* __restrict__ inHi,
+                                                           uint32_t* __restrict__ C, float4* __restrict__ lo,
+                                                           float4* __restrict__ hi, uint32_t* mOut) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    if (i == m - 1u) *mOut = pos[i] + keep[i];
+    if (!keep[i]) return;
+    const uint32_t k = pos[i];
+    C[k] = inC[i];
+    lo[k] = inLo[i];
+    hi[k] = inHi[i];
+}
+
+// Depth-first position of the first leaf under a node: the sum, over the ancestors where the
+// path goes right, of the left sibling's primitive count.
+__device__ __forceinline__ uint32_t dfsOffset(uint32_t code, uint32_t p, const uint32_t* left, const uint32_t* right,
+                                              const uint32_t* parentInt, const uint32_t* count) {
+    uint32_t off = 0u;
+    while (true) {
+        if (right[p] == code) off += clusterCount(left[p], count);
+        if (p == 0u) break;
+        code = p;
+        p = parentInt[p];
+    }
+    return off;
+}
+
+// Leaves: depth-first position of every sorted position; internal nodes: their leaf range.
+__global__ __launch_bounds__(256) void gbPlocRangesKernel(uint32_t n, const uint32_t* __restrict__ left,
+                                                          const uint32_t* __restrict__ right,
+                                                          const uint32_t* __restrict__ parentInt,
+                                                          const uint32_t* __restrict__ parentLeaf,
+                                                          const uint32_t* __restrict__ count, uint32_t* __restrict__ leafPos,
+                                                          uint32_t* __restrict__ first, uint32_t* __restrict__ last) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) leafPos[k] = dfsOffset(kLeafTag | k, parentLeaf[k], left, right, parentInt, count);
+    if (k < n - 1u) {
+        const uint32_t off = k == 0u ? 0u : dfsOffset(k, parentInt[k], left, right, parentInt, count);
+        first[k] = off;
+        last[k] = off + count[k] - 1u;
+    }
+}
+
+// Renumbers the leaves to their depth-first positions: the primitive of each position, the
+// parent links of the leaves and the leaf codes of the children.
+__global__ __launch_bounds__(256) void gbPlocRenumberKernel(uint32_t n, const uint32_t* __restrict__ leafPos,
+                                                            const uint32_t* __restrict__ valsIn,
+                                                            const uint32_t* __restrict__ parentLeafIn,
+                                                            uint32_t* __restrict__ valsOut,
+                                                            uint32_t* __restrict__ parentLeafOut, uint32_t* left,
+                                                            uint32_t* right) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) {
+        const uint32_t q = leafPos[k];
+        valsOut[q] = valsIn[k];
+        parentLeafOut[q] = parentLeafIn[k];
+    }
+    if (k < n - 1u) {
+        const uint32_t a = left[k], b = right[k];
+        if (a & kLeafTag) left[k] = kLeafTag | leafPos[a & ~kLeafTag];
+        if (b & kLeafTag) right[k] = kLeafTag | leafPos[b & ~kLeafTag];
+    }
+}
+
+// Cluster boxes of the sorted primitives.
+__global__ __launch_bounds__(256) void gbPlocInitKernel(uint32_t n, const uint32_t* __restrict__ vals,
+                                                        const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
+                                                        uint32_t* __restrict__ C, float4* __restrict__ lo,
+                                                        float4* __restrict__ hi) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    C[k] = kLeafTag | k;
+    lo[k] = boxLo[vals[k]];
+    hi[k] = boxHi[vals[k]];
 }
 
 __device__ __forceinline__ float ld(const float* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
@@ -497,7 +683,70 @@ float hostUnkey(uint32_t k) {
 
 }  // namespace
 
-int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool quant, bool optimal) {
+// The binary tree by locally-ordered clustering (gbPloc* kernels) over the n >= 2 sorted
+// primitives (vals: sorted position -> primitive): fills the same arrays as the radix tree
+// (children, leaf ranges, parent links, with leaves at their depth-first positions) and
+// dVals3, the primitive of each depth-first position.  One round per launch group; the host
+// reads the cluster count after each.
+static int buildClusterTree(rbr_ctx* ctx, uint32_t n, const uint32_t* vals, const float4* boxLo, const float4* boxHi,
+                            DevBuf& dC0, DevBuf& dC1, DevBuf& dL0, DevBuf& dL1, DevBuf& dH0, DevBuf& dH1, DevBuf& dNn,
+                            DevBuf& dKeep, DevBuf& dPos, DevBuf& dCount, DevBuf& dScanTmp, DevBuf& dLeafPos,
+                            DevBuf& dVals3, DevBuf& dPLeaf2, DevBuf& dCnt, DevBuf& dLeft, DevBuf& dRight, DevBuf& dFirst,
+                            DevBuf& dLast, DevBuf& dPInt, DevBuf& dPLeaf) {
+    hipStream_t st = ctx->stream;
+    int rc = RBR_OK;
+    auto chk = [&](int r) {
+        if (r && !rc) rc = r;
+        return rc == RBR_OK;
+    };
+    size_t scanBytes = 0;
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, static_cast<uint32_t*>(nullptr),
+                                                    static_cast<uint32_t*>(nullptr), int(n), st);
+    if (e != hipSuccess) return failHip(ctx, e, "cluster scan size");
+    const size_t u = size_t(n) * 4, f = size_t(n) * 16;
+    if (!chk(ensureBuffer(ctx, dC0, u)) || !chk(ensureBuffer(ctx, dC1, u)) || !chk(ensureBuffer(ctx, dL0, f)) ||
+        !chk(ensureBuffer(ctx, dL1, f)) || !chk(ensureBuffer(ctx, dH0, f)) || !chk(ensureBuffer(ctx, dH1, f)) ||
+        !chk(ensureBuffer(ctx, dNn, u)) || !chk(ensureBuffer(ctx, dKeep, u)) || !chk(ensureBuffer(ctx, dPos, u)) ||
+        !chk(ensureBuffer(ctx, dCount, u)) || !chk(ensureBuffer(ctx, dScanTmp, std::max<size_t>(scanBytes, 16))) ||
+        !chk(ensureBuffer(ctx, dLeafPos, u)) || !chk(ensureBuffer(ctx, dVals3, u)) || !chk(ensureBuffer(ctx, dPLeaf2, u)))
+        return rc;
+    auto U = [](DevBuf& b) { return static_cast<uint32_t*>(b.ptr); };
+    auto F = [](DevBuf& b) { return static_cast<float4*>(b.ptr); };
+    const unsigned blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(gbPlocInitKernel, dim3(blocks), dim3(256), 0, st, n, vals, boxLo, boxHi, U(dC0), F(dL0), F(dH0));
+    // dCnt[3]: merges so far, dCnt[2]: cluster count after a round (the collapse resets both)
+    hipLaunchKernelGGL(gbZeroKernel, dim3(1), dim3(256), 0, st, U(dCnt) + 2, 2u);
+    uint32_t m = n, stalled = 0, rounds = 0;
+    while (m > 1) {
+        if (++rounds > 4096) return fail(ctx, RBR_ERR_UNSUPPORTED, "cluster build did not converge");
+        const unsigned b = (m + 255) / 256;
+        if (stalled) hipLaunchKernelGGL(gbPlocPairKernel, dim3(b), dim3(256), 0, st, m, reinterpret_cast<int32_t*>(U(dNn)));
+        else hipLaunchKernelGGL(gbPlocNearestKernel, dim3(b), dim3(256), 0, st, m, F(dL0), F(dH0),
+                                reinterpret_cast<int32_t*>(U(dNn)));
+        hipLaunchKernelGGL(gbPlocMergeKernel, dim3(b), dim3(256), 0, st, m, n, U(dC0), F(dL0), F(dH0),
+                           reinterpret_cast<const int32_t*>(U(dNn)), U(dCnt) + 3, U(dLeft), U(dRight), U(dPInt),
+                           U(dPLeaf2), U(dCount), U(dC1), F(dL1), F(dH1), U(dKeep));
+        e = hipcub::DeviceScan::ExclusiveSum(dScanTmp.ptr, scanBytes, U(dKeep), U(dPos), int(m), st);
+        if (e != hipSuccess) return failHip(ctx, e, "cluster scan");
+        hipLaunchKernelGGL(gbPlocCompactKernel, dim3(b), dim3(256), 0, st, m, U(dKeep), U(dPos), U(dC1), F(dL1), F(dH1),
+                           U(dC0), F(dL0), F(dH0), U(dCnt) + 2);
+        uint32_t next = 0;
+        e = hipMemcpyAsync(&next, U(dCnt) + 2, 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return failHip(ctx, e, "cluster round");
+        stalled = next == m ? 1u : 0u;
+        m = next;
+    }
+    hipLaunchKernelGGL(gbPlocRangesKernel, dim3(blocks), dim3(256), 0, st, n, U(dLeft), U(dRight), U(dPInt), U(dPLeaf2),
+                       U(dCount), U(dLeafPos), U(dFirst), U(dLast));
+    hipLaunchKernelGGL(gbPlocRenumberKernel, dim3(blocks), dim3(256), 0, st, n, U(dLeafPos), vals, U(dPLeaf2),
+                       U(dVals3), U(dPLeaf), U(dLeft), U(dRight));
+    hipLaunchKernelGGL(gbZeroKernel, dim3(1), dim3(256), 0, st, U(dCnt) + 2, 2u);
+    e = hipGetLastError();
+    return e == hipSuccess ? RBR_OK : failHip(ctx, e, "cluster build");
+}
+
+int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool quant, bool optimal, bool ploc) {
     using Clock = std::chrono::steady_clock;
     const auto t0 = Clock::now();
     const HostGeometry& geo = s->geom;
@@ -514,13 +763,17 @@ int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool 
     hipStream_t st0 = ctx->stream;
     // Device temporaries: the context's, grown as needed and kept for the next build.
     std::vector<DevBuf>& tmp = ctx->buildTmp;
-    if (tmp.size() < 28) tmp.resize(28);
+    if (tmp.size() < 42) tmp.resize(42);
     DevBuf &dTri = tmp[0], &dTriId = tmp[1], &dSph = tmp[2], &dSphId = tmp[3], &dQuad = tmp[4], &dQuadId = tmp[5],
            &dLo = tmp[6], &dHi = tmp[7], &dRed = tmp[8], &dKeys = tmp[9], &dVals = tmp[10], &dKeys2 = tmp[11],
            &dVals2 = tmp[12], &dSortTmp = tmp[13], &dLeft = tmp[14], &dRight = tmp[15], &dFirst = tmp[16],
            &dLast = tmp[17], &dPInt = tmp[18], &dPLeaf = tmp[19], &dArr = tmp[20], &dNodeBox = tmp[21],
            &dItemsA = tmp[22], &dItemsB = tmp[23], &dCnt = tmp[24], &dCost = tmp[25], &dDec = tmp[26],
            &dFloatNodes = tmp[27];
+    // clustering (ploc): two cluster arrays, nearest neighbours, survivors, ranks, counts
+    DevBuf &dC0 = tmp[28], &dC1 = tmp[29], &dL0 = tmp[30], &dL1 = tmp[31], &dH0 = tmp[32], &dH1 = tmp[33],
+           &dNn = tmp[34], &dKeep = tmp[35], &dPos = tmp[36], &dCount = tmp[37], &dScanTmp = tmp[38],
+           &dLeafPos = tmp[39], &dVals3 = tmp[40], &dPLeaf2 = tmp[41];
     auto chk = [&](int r) {
         if (r && !rc) rc = r;
         return rc == RBR_OK;
@@ -574,9 +827,17 @@ int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool 
     }
     const uint32_t* keys = u32(dKeys2);
     const uint32_t* vals = u32(dVals2);
-    if (n >= 2) {
+    if (n >= 2 && ploc) {
+        if ((rc = buildClusterTree(ctx, n, u32(dVals2), f4(dLo), f4(dHi), dC0, dC1, dL0, dL1, dH0, dH1, dNn, dKeep, dPos,
+                                   dCount, dScanTmp, dLeafPos, dVals3, dPLeaf2, dCnt, dLeft, dRight, dFirst, dLast, dPInt,
+                                   dPLeaf)))
+            return rc;
+        vals = u32(dVals3);
+    } else if (n >= 2) {
         hipLaunchKernelGGL(gbKarrasKernel, dim3((n - 1 + 255) / 256), dim3(256), 0, st0, n, keys, u32(dLeft), u32(dRight),
                            u32(dFirst), u32(dLast), u32(dPInt), u32(dPLeaf));
+    }
+    if (n >= 2) {
         hipLaunchKernelGGL(gbZeroKernel, dim3((n - 1 + 255) / 256), dim3(256), 0, st0, u32(dArr), n - 1);
         const DpArgs dp{uint32_t(W), uint32_t(leafMax), kCollapsePrimCost, u32(dFirst), u32(dLast),
                         static_cast<float*>(dCost.ptr), static_cast<float*>(dDec.ptr)};

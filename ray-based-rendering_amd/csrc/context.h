@@ -106,7 +106,7 @@ int prepareTraversal(rbr_ctx* ctx, uint32_t maxStack);
 // Device build of the scene's wide BVH (bvh_gpu.hip): fills the scene's device node, float
 // node and primitive buffers and s->bvh's counts and bounds; the host arrays of s->bvh stay
 // empty until ensureHostBvh.
-int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool quant, bool optimal);
+int buildWideBVHDevice(rbr_ctx* ctx, rbr_scene* s, int width, int leafMax, bool quant, bool optimal, bool ploc);
 
 // Makes s->bvh's host arrays (float nodes, device-layout nodes, records) of a device-built
 // tree from its device buffers, once; a no-op for host-built trees.  Safe to call from

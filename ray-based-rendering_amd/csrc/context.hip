@@ -525,15 +525,17 @@ int rbr_scene_build_bvh_device(rbr_scene* s, int width, int leaf_max) {
     if (!s) return RBR_ERR_INVALID;
     rbr_ctx* ctx = s->ctx;
     const int W = width & 0xFF;
-    if ((W != 4 && W != 8) || (width & ~(RBR_BVH_QUANTIZED | RBR_BVH_GREEDY | 0xFF)))
-        return fail(ctx, RBR_ERR_INVALID, "BVH width must be 4 or 8, optionally | RBR_BVH_QUANTIZED | RBR_BVH_GREEDY");
+    if ((W != 4 && W != 8) || (width & ~(RBR_BVH_QUANTIZED | RBR_BVH_GREEDY | RBR_BVH_PLOC | 0xFF)))
+        return fail(ctx, RBR_ERR_INVALID,
+                    "BVH width must be 4 or 8, optionally | RBR_BVH_QUANTIZED | RBR_BVH_GREEDY | RBR_BVH_PLOC");
     if (leaf_max < 1 || leaf_max > 8) return fail(ctx, RBR_ERR_INVALID, "leaf_max must be in 1..8");
     if (ctx->device < 0) return fail(ctx, RBR_ERR_STATE, "the device build needs a GPU context");
     if (int rc = useDevice(ctx)) return rc;
     s->built = false;
     bumpVersion(s);
     s->hostStale = false;
-    if (int rc = buildWideBVHDevice(ctx, s, W, leaf_max, (width & kQuantFlag) != 0, (width & RBR_BVH_GREEDY) == 0))
+    if (int rc = buildWideBVHDevice(ctx, s, W, leaf_max, (width & kQuantFlag) != 0, (width & RBR_BVH_GREEDY) == 0,
+                                    (width & RBR_BVH_PLOC) != 0))
         return rc;
     s->hostStale = true;
     if (!s->bvh.empty() && uint64_t(W - 1) * s->bvh.maxDepth + 1 > uint64_t(kMaxStack))

@@ -83,6 +83,7 @@ _sig(_lib.rbr_scene_device_nodes, [_vp, _vp, _sz, C.POINTER(_sz)])
 _sig(_lib.rbr_scene_bvh_arrays, [_vp, _vp, _sz, _vp, _sz, C.POINTER(_sz), C.POINTER(_sz)])
 BVH_QUANTIZED = 0x100
 BVH_GREEDY = 0x200
+BVH_PLOC = 0x400   # device build: clustering tree instead of the radix tree
 _sig(_lib.rbr_scene_world_bounds, [_vp, _vp, _vp])
 _sig(_lib.rbr_trace_closest, [_vp, _vp, _sz, _vp])
 _sig(_lib.rbr_trace_any, [_vp, _vp, _sz, _vp])

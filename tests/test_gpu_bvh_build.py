@@ -38,7 +38,7 @@ def dev_scene(ctx, width=4, leaf=4, **kw):
     return sc
 
 
-@pytest.mark.parametrize("collapse", [0, R.BVH_GREEDY])
+@pytest.mark.parametrize("collapse", [0, R.BVH_GREEDY, R.BVH_PLOC, R.BVH_PLOC | R.BVH_GREEDY])
 @pytest.mark.parametrize("width", WIDTHS)
 @pytest.mark.parametrize("cfg", [(30, 30, False, 3000), (400, 1500, True, 6000), (0, 2, False, 500),
                                  (1, 0, False, 200)])
@@ -67,11 +67,12 @@ def test_duplicates_and_leaf_sizes(ctx):
     r = R.rays_array([[0.2, 0.2, 1.0]], [[0, 0, -1]])
     for leaf in range(1, 9):
         for width in WIDTHS:
-            h = dev_scene(ctx, width, leaf, tris=tris, tri_ids=ids).trace_closest(r)
-            assert h["id"][0] == 100 and h["t"][0] == np.float32(1.0), (leaf, width)
+            for tree in (0, R.BVH_PLOC):
+                h = dev_scene(ctx, width | tree, leaf, tris=tris, tri_ids=ids).trace_closest(r)
+                assert h["id"][0] == 100 and h["t"][0] == np.float32(1.0), (leaf, width, tree)
 
 
-@pytest.mark.parametrize("collapse", [0, R.BVH_GREEDY])
+@pytest.mark.parametrize("collapse", [0, R.BVH_GREEDY, R.BVH_PLOC])
 @pytest.mark.parametrize("width", [4, 8])
 def test_structure_and_quantization(ctx, width, collapse):
     """Quantized nodes carry the float nodes' refs and decode (fma(1024 + q, 2^e, p), one
@@ -137,7 +138,12 @@ def test_c1_size_build_and_hits(ctx):
     sd, sh = d.stats(), h.stats()
     say(f"  device build {sd['build_ms']:.1f} ms, {sd['node_count']} nodes, depth {sd['max_depth']}; "
         f"host {sh['build_ms']:.1f} ms, {sh['node_count']} nodes, depth {sh['max_depth']}")
-    assert_same(d.trace_closest(rays), h.trace_closest(rays))
+    ref = h.trace_closest(rays)
+    assert_same(d.trace_closest(rays), ref)
+    c = dev_scene(ctx, 4 | R.BVH_QUANTIZED | R.BVH_PLOC, 4, tris=tris)
+    sc = c.stats()
+    say(f"  clustering build {sc['build_ms']:.1f} ms, {sc['node_count']} nodes, depth {sc['max_depth']}")
+    assert_same(c.trace_closest(rays), ref)
 
 
 def test_render_on_device_built_tree(ctx):
