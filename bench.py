@@ -140,6 +140,8 @@ def main() -> int:
     ap.add_argument("--no-paths", action="store_true", help="skip the path-tracing lines (C0, C2, C3)")
     ap.add_argument("--no-c4", action="store_true", help="skip guided vs plain time-to-relMSE (C4)")
     ap.add_argument("--path-cpu-seconds", type=float, default=5.0)
+    ap.add_argument("--no-device-build", action="store_true",
+                    help="skip the same mesh built on the GPU (build time, its trace rate, hits against the host tree)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {args.gpus} is not supported "
@@ -261,6 +263,29 @@ def main() -> int:
     if not args.no_cpu:
         log("bench: CPU baseline")
         rec["cpu_baseline"] = cpu_baseline(tris, rays, hits, args.cpu_seconds)
+    if not args.no_device_build:
+        # The same mesh and layout built on the GPU (rbr_scene_build_bvh_device), outside the timed
+        # steps: build time (best of three), closest-hit rate on its tree, and its hits, which must
+        # equal the host tree's (hits do not depend on the tree).
+        log("bench: device BVH build")
+        dsc = R.Scene(ctx)
+        dsc.set_geometry(tris=tris)
+        build_ms = []
+        for _ in range(3):
+            dsc.build_bvh(args.width, args.leaf, device=True)
+            build_ms.append(dsc.stats()["build_ms"])
+        for _ in range(args.warmup):
+            dsc.trace_closest_device(d_rays, N_RAYS, d_hits)
+        ms_dev = dsc.time_trace(0, d_rays, N_RAYS, d_hits, max(1, args.steps // 2))
+        dh = np.empty(N_RAYS, R.HIT_DTYPE)
+        d_hits.download(dh)
+        dst = dsc.stats()
+        rec["device_build"] = {
+            "build_ms": min(build_ms), "nodes": dst["node_count"], "max_depth": dst["max_depth"],
+            "closest_mrays_s": N_RAYS / (ms_dev * 1e-3) / 1e6,
+            "hits_equal_host_tree": bool(np.array_equal(dh["id"], hits["id"]) and np.array_equal(dh["t"], hits["t"])),
+        }
+        dsc.close()
     for b in (d_rays, d_shadow, d_hits, d_occ):
         b.free()
     sc.close()
