@@ -1,21 +1,21 @@
 // Quantization of one axis of a wide node's child boxes, shared by the host quantizer
 // (bvh_build.cpp) and the device builder (bvh_gpu.hip) so both produce the same node bits.
-// RBR_HD compiles the functions for the device as well when the file is built by hipcc.
+// RBR_QUANT_HD compiles the functions for the device as well when the file is built by hipcc.
 #pragma once
 
 #include <math.h>
 #include <stdint.h>
 
 #ifdef __HIPCC__
-#define RBR_HD __host__ __device__
+#define RBR_QUANT_HD __host__ __device__
 #else
-#define RBR_HD
+#define RBR_QUANT_HD
 #endif
 
 namespace rbr {
 
 // ceil(log2(v)) for finite v > 0, exact: v = m 2^x with m in [0.5, 1).
-RBR_HD inline int ceilLog2(double v) {
+RBR_QUANT_HD inline int ceilLog2(double v) {
     int x = 0;
     const double m = frexp(v, &x);
     return m == 0.5 ? x - 1 : x;
@@ -28,7 +28,7 @@ RBR_HD inline int ceilLog2(double v) {
 constexpr int kQuantBias = 1024;
 
 // The largest float p' with fma(1024, s, p') <= lo, so that code 0 decodes to at most lo.
-RBR_HD inline float biasedOrigin(float lo, float s) {
+RBR_QUANT_HD inline float biasedOrigin(float lo, float s) {
     float p = float(double(lo) - double(kQuantBias) * double(s));
     while (fmaf(float(kQuantBias), s, p) > lo) p = nextafterf(p, -INFINITY);
     return p;
@@ -40,7 +40,7 @@ RBR_HD inline float biasedOrigin(float lo, float s) {
 // rounding (each plane checked with the single rounding the device uses: lo' = fma(1024 +
 // q_lo, s, p') <= lo, hi' = fma(1024 + q_hi, s, p') >= hi).  Dead children get q_lo = 255,
 // q_hi = 0.
-RBR_HD inline void quantAxis(const float* lo, const float* hi, const bool* live, int W, float& p, int& ebias,
+RBR_QUANT_HD inline void quantAxis(const float* lo, const float* hi, const bool* live, int W, float& p, int& ebias,
                              uint8_t* qlo, uint8_t* qhi) {
     float plo = INFINITY, phi = -INFINITY;
     for (int k = 0; k < W; ++k)
@@ -91,13 +91,13 @@ RBR_HD inline void quantAxis(const float* lo, const float* hi, const bool* live,
 
 // Dword of the plane codes of child k on axis ax: children 2g and 2g + 1 share dword
 // 4 + ax * W/2 + g, bytes (q_lo[2g], q_lo[2g+1], q_hi[2g], q_hi[2g+1]).
-RBR_HD inline int qPlaneDword(int W, int ax, int k) { return 4 + ax * (W / 2) + (k >> 1); }
-RBR_HD inline int qLoShift(int k) { return 8 * (k & 1); }
-RBR_HD inline int qHiShift(int k) { return 8 * (2 + (k & 1)); }
+RBR_QUANT_HD inline int qPlaneDword(int W, int ax, int k) { return 4 + ax * (W / 2) + (k >> 1); }
+RBR_QUANT_HD inline int qLoShift(int k) { return 8 * (k & 1); }
+RBR_QUANT_HD inline int qHiShift(int k) { return 8 * (2 + (k & 1)); }
 
 // One quantized node (qNodeDwords(W) dwords, zero-filled by the caller) from a float node
 // (bvh.h layouts).
-RBR_HD inline void quantizeNode(const float* f, int W, uint32_t* q) {
+RBR_QUANT_HD inline void quantizeNode(const float* f, int W, uint32_t* q) {
     const int q4 = W / 4;
     uint32_t ref[8];
     bool live[8];
