@@ -7,9 +7,9 @@
 //                        layer's B operand; weights are read from LDS in fragment order (one
 //                        16-byte read per lane per fragment, shared by the wave's 2 tiles of 16
 //                        queries); one 1,024-thread workgroup per CU shares one LDS copy of the
-//                        weights, so 4 waves per SIMD fit (register file and LDS both); outputs
-//                        go through a small LDS stage to be turned into 8 vMF lobes per query
-//                        (FP16 record)
+//                        weights, so 4 waves per SIMD fit (register file and LDS both); the
+//                        last layer's rows are permuted so that each lane holds the outputs of
+//                        the two lobes it decodes (8 vMF lobes per query, FP16 record)
 //   guideFwdBwdKernel    training, 64 samples per wave: the same register-resident transposed
 //                        forward, the loss -c*log p(dir) and its gradient w.r.t. the 40 outputs
 //                        (through a small LDS stage), and the backward pass in the same
@@ -66,11 +66,21 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kIT = 2;                         // 16-query tiles per wave
-// Floats per query row of a wave's LDS output stage: 52, not kGOutPad = 48, so that the 16 rows
+// Floats per query row of a training wave's LDS output stage: 52, not kGOutPad = 48, so that the 16 rows
 // of a tile start in 16 different banks (with 48 a column of the tile fell in only 4 of the
 // 64 banks, and every lane-parallel read of one output across the tile's queries conflicted).
 constexpr int kStageRow = 52;
 constexpr int kInferStage = kTile * kStageRow;  // floats of LDS output stage per wave
+
+// Inference computes layer 4 with its rows permuted so that a lane's four outputs of each
+// 16-row block (rows 16 mb + 4 g + i of query r in lane (r, g)) are the ten outputs of lobes
+// 2g, 2g + 1 and the selection logit: row 16 mb + 4 g + i carries output outOfRow(...) of
+// slot 4 mb + i.  The lobe decode then reads its inputs from registers, with no LDS stage.
+__device__ __forceinline__ int outOfRow(int row) {
+    const int g = (row & 15) >> 2, slot = 4 * (row >> 4) + (row & 3);
+    return slot < 10 ? 10 * g + slot : slot == 10 ? kGSel : -1;
+}
+constexpr int kInferW4Frags = 6;   // layer-4 fragments: 3 row blocks x 2 k-blocks
 
 __device__ __forceinline__ v4f mfma32(u32x4 a, u32x4 b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
@@ -121,10 +131,21 @@ __global__ __launch_bounds__(64 * kWaves, 1) void guideInferKernel(GuideInferArg
     extern __shared__ __align__(16) unsigned char smem[];
     u32x4* lwf = reinterpret_cast<u32x4*>(smem);
     float* lb = reinterpret_cast<float*>(smem + kGWeights * 2);
+    u32x4* lw4 = reinterpret_cast<u32x4*>(smem + kGWeights * 2 + kGBias * 4);                       // permuted layer 4
+    float* lb4 = reinterpret_cast<float*>(smem + kGWeights * 2 + kGBias * 4 + kInferW4Frags * 64 * 16);   // its biases
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    float* stage = reinterpret_cast<float*>(smem + kGWeights * 2 + kGBias * 4) + wave * kInferStage;
     for (int i = threadIdx.x; i < kGWeights / 8; i += blockDim.x) lwf[i] = reinterpret_cast<const u32x4*>(a.g.wf)[i];
     for (int i = threadIdx.x; i < kGBias; i += blockDim.x) lb[i] = a.g.b[i];
+    __syncthreads();
+    // the permuted layer 4: fragment (mb, kb), lane (row, k-group) <- output outOfRow(16 mb + row)
+    for (int i = threadIdx.x; i < kInferW4Frags * 64; i += blockDim.x) {
+        const int f = i >> 6, l = i & 63, o = outOfRow(16 * (f >> 1) + (l & 15));
+        lw4[i] = o < 0 ? u32x4{0u, 0u, 0u, 0u} : lwf[(kGFragOff[4] + (o >> 4) * 2 + (f & 1)) * 64 + (o & 15) + 16 * (l >> 4)];
+    }
+    for (int i = threadIdx.x; i < kGOutPad; i += blockDim.x) {
+        const int o = outOfRow(i);
+        lb4[i] = o < 0 ? 0.0f : lb[4 * kGHid + o];
+    }
     __syncthreads();
     const uint32_t n = a.nDev ? *a.nDev : a.n;
     const uint32_t groups = (n + kTile * kIT - 1) / (kTile * kIT);
@@ -174,36 +195,30 @@ __global__ __launch_bounds__(64 * kWaves, 1) void guideInferKernel(GuideInferArg
                 h[t][0] = reluPack(acc[t][0], acc[t][1]);
                 h[t][1] = reluPack(acc[t][2], acc[t][3]);
 #pragma unroll
-                for (int mb = 0; mb < 3; ++mb) o[t][mb] = biasInit(lb, 256, mb, g);
+                for (int mb = 0; mb < 3; ++mb) o[t][mb] = biasInit(lb4, 0, mb, g);
             }
 #pragma unroll
             for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
                 for (int mb = 0; mb < 3; ++mb) {
-                    const u32x4 af = lwf[(kGFragOff[4] + mb * 2 + kb) * 64 + lane];
+                    const u32x4 af = lw4[(mb * 2 + kb) * 64 + lane];
 #pragma unroll
                     for (int t = 0; t < kIT; ++t) o[t][mb] = mfma32(af, h[t][kb], o[t][mb]);
                 }
         }
-        // outputs -> lobes, one tile at a time through the wave's LDS stage: lane (r, g)
-        // writes lobes 2g, 2g+1 of query r.
+        // outputs -> lobes: lane (r, g) holds lobes 2g, 2g+1 of query r (ten outputs, slots 0-9 of
+        // the permuted layer 4) and the selection logit (slot 10).
 #pragma unroll
         for (int t = 0; t < kIT; ++t) {
-#pragma unroll
-            for (int mb = 0; mb < 3; ++mb)
-                *reinterpret_cast<float4*>(stage + r * kStageRow + mb * 16 + 4 * g) =
-                    make_float4(o[t][mb][0], o[t][mb][1], o[t][mb][2], o[t][mb][3]);
-            waveSync();
             {
-                // Lane (r, g) decodes lobes 2g, 2g+1 of query r (ten consecutive outputs).  The
-                // softmax max and sum over the query's 8 weights are taken across its four lanes
-                // (r, r + 16, r + 32, r + 48) with two exchanges each, so each weight is
+                // The softmax max and sum over the query's 8 weights are taken across its four
+                // lanes (r, r + 16, r + 32, r + 48) with two exchanges each, so each weight is
                 // exponentiated once.  Every lane takes part: the four lanes of a query agree on
                 // valid[t].  Hardware exp / rsq / rcp: the lobes are stored in FP16.
-                const float* ov = stage + r * kStageRow;
                 float v[10];
 #pragma unroll
-                for (int q = 0; q < 10; ++q) v[q] = ov[10 * g + q];
+                for (int q = 0; q < 10; ++q) v[q] = o[t][q >> 2][q & 3];
+                const float selLogit = o[t][2][2];
                 float mx = fmaxf(v[4], v[9]);
                 mx = fmaxf(mx, __shfl_xor(mx, 16));
                 mx = fmaxf(mx, __shfl_xor(mx, 32));
@@ -214,7 +229,7 @@ __global__ __launch_bounds__(64 * kWaves, 1) void guideInferKernel(GuideInferArg
                 // The stored weights carry the selection probability s (kGSel): they sum to s,
                 // which the renderer reads back as the guide's share of the samples.
                 const float rse = __builtin_amdgcn_rcpf(se) *
-                                  (kGSelMin + (kGSelMax - kGSelMin) * __builtin_amdgcn_rcpf(1.0f + __expf(-ov[kGSel])));
+                                  (kGSelMin + (kGSelMax - kGSelMin) * __builtin_amdgcn_rcpf(1.0f + __expf(-selLogit)));
                 float vals[2][5];
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
@@ -233,7 +248,6 @@ __global__ __launch_bounds__(64 * kWaves, 1) void guideInferKernel(GuideInferArg
                     for (int a5 = 0; a5 < 5; ++a5) dst[4 * a5 + g] = packH2(vals[0][a5], vals[1][a5]);
                 }
             }
-            waveSync();
         }
     }
 }
@@ -712,7 +726,7 @@ GuideState& state(rbr_ctx* ctx) {
 size_t trainSmem() {
     return size_t(kGWeights) * 2 + size_t(kGTWeights) * 2 + kGBias * 4 + size_t(kInferStage) * 4 * kTrainWaves;
 }
-size_t inferSmem() { return size_t(kGWeights) * 2 + kGBias * 4 + size_t(kInferStage) * 4 * kWaves; }
+size_t inferSmem() { return size_t(kGWeights) * 2 + kGBias * 4 + size_t(kInferW4Frags) * 64 * 16 + kGOutPad * 4; }
 
 int refreshBf16(rbr_ctx* ctx, GuideState& g) {
     std::vector<uint16_t> bf(kGWeights), frag(kGWeights), trans(kGTWeights, 0);
