@@ -14,14 +14,16 @@
 //                        forward, the loss -c*log p(dir) and its gradient w.r.t. the 40 outputs
 //                        (through a small LDS stage), and the backward pass in the same
 //                        transposed form with a second, transposed fragment copy of the weights
-//                        (dA^T = W^T dZ^T; the ReLU masks are kept as bits).  Every layer input
-//                        A_l and output gradient dZ_l is written feature-major to HBM; the
-//                        register operands (lane = sample) are turned into rows (lane =
-//                        feature) by a product with an identity fragment on the matrix cores,
-//                        which is exact for BF16 values
+//                        (dA^T = W^T dZ^T; the ReLU masks are kept as bits).  Every hidden
+//                        layer input A_l (l >= 1) and output gradient dZ_l is written
+//                        feature-major to HBM; the register operands (lane = sample) are turned
+//                        into rows (lane = feature) by a product with an identity fragment on
+//                        the matrix cores, which is exact for BF16 values.  For A_0 only the 9
+//                        input scalars are written (36 of its 448 bytes per sample)
 //   guideWeightGradKernel  dW_l = dZ_l^T A_l and db_l = column sums of dZ_l: one 64 x 64 tile of
 //                        one layer per wave over a chunk of 1024 samples (16x16x32 MFMA with
-//                        the samples as the reduction index), into a per-chunk partial gradient
+//                        the samples as the reduction index), into a per-chunk partial
+//                        gradient; A_0 is recomputed from the scalars bit for bit
 //   guideGradReduceKernel  sums the partial gradients in chunk order (deterministic)
 //   guideAdamKernel      Adam on the FP32 master weights, refreshed BF16 copies
 // All matrix-core work uses v_mfma_f32_16x16x32_bf16: lane l holds the A rows / B columns
@@ -259,7 +261,8 @@ namespace {
 constexpr int kTT = 2;                 // 16-sample tiles per training wave group: 32 samples
 constexpr int kTrainWaves = 8;         // waves per training workgroup
 constexpr uint32_t kGradChunk = 1024;  // samples per partial weight gradient
-constexpr int kGradTiles = 8;          // weight-gradient tiles: layer 0 in 4 column blocks of 64
+constexpr int kGradL0Tiles = kGInPad / 32;  // weight-gradient tiles of layer 0: 64 x 32 column blocks
+constexpr int kGradTiles = kGradL0Tiles + 4;  // and one 64 x 64 (48 x 64) tile for each of layers 1-4
                                        // (the last has 32 columns), layers 1-4 one each
 
 // Identity B operand that turns a forward operand (lane = sample, slots = 32 features of a
@@ -431,7 +434,7 @@ __global__ __launch_bounds__(512) void guideFwdBwdKernel(GuideTrainArgs a) {
             for (int j = 0; j < 9; ++j) s9[t][j] = 0.0f;
             if (valid[t]) guideSample(a, si, s9[t], dir[t], c[t], pb[t], pd[t]);
         }
-        // layer 0: features are produced one k-block at a time and stored as A_0
+        // layer 0: features are produced one k-block at a time
         v4f acc[kTT][4];
         float2 st[kTT];
         ReluMask mask[kTT];
@@ -453,7 +456,15 @@ __global__ __launch_bounds__(512) void guideFwdBwdKernel(GuideTrainArgs a) {
 #pragma unroll
                 for (int t = 0; t < kTT; ++t) acc[t][mb] = mfma32(af, bx[t], acc[t][mb]);
             }
-            storeBlock(a.act[0], kGInPad, grp, kb, bx, idLo, idHi, lane);
+        }
+        // the weight gradient re-derives A_0 from the input scalars (36 B per sample instead of
+        // the 448 B of the BF16 features): lane group g writes scalars g, g + 4, g + 8
+#pragma unroll
+        for (int t = 0; t < kTT; ++t) {
+            float* sb = a.scal + size_t(grp) * 9 * 32 + kTile * t + r;
+#pragma unroll
+            for (int j = 0; j < 9; ++j)
+                if ((j & 3) == g) sb[j * 32] = s9[t][j];
         }
         trainLayer<1, 4>(acc, mask, lwf, lb, a, grp, idLo, idHi, lane);
         trainLayer<2, 4>(acc, mask, lwf, lb, a, grp, idLo, idHi, lane);
@@ -530,17 +541,34 @@ __global__ __launch_bounds__(64) void guideWeightGradKernel(GuideTrainArgs a, ui
     const int tile = int(slot % kGradTiles);
     const uint32_t ch = (slot / kGradTiles) * 8u + xcd;
     if (ch >= chunks) return;
-    const int layer = tile < 4 ? 0 : tile - 3;
-    const int kt = tile < 4 ? tile : 0;
+    const int layer = tile < kGradL0Tiles ? 0 : tile - (kGradL0Tiles - 1);
+    const int kt = tile < kGradL0Tiles ? tile : 0;
     const int K = layer == 0 ? kGInPad : kGHid;
     const int nrb = layer == 4 ? kGOutPad / 16 : 4;
-    const int ncb = min(4, (K - 64 * kt) / 16);
+    const int ncb = layer == 0 ? 2 : 4;      // 16-column blocks of the tile
+    const int c0 = layer == 0 ? 32 * kt : 0; // its first column
     const int N = layer == 4 ? kGOutPad : kGHid;
     const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
     const uint32_t groups = (a.n + 31u) / 32u;
     const uint32_t q0 = ch * (kGradChunk / 32), q1 = min(groups, q0 + kGradChunk / 32);
     const uint16_t* dz = a.dz[layer] + r * 32 + 8 * g;
-    const uint16_t* act = a.act[layer] + (64 * kt + r) * 32 + 8 * g;
+    const uint16_t* act = layer == 0 ? nullptr : a.act[layer] + r * 32 + 8 * g;
+    // Layer 0: the lane's B operand of column block cb is feature c0 + 16 cb + r of samples
+    // 8 g .. 8 g + 7, computed from the stored input scalars (guidePair order; the 4 padding
+    // pairs are zero, as in the forward).  Lanes r and r ^ 1 hold the sine and the cosine of
+    // one pair: each evaluates both for four of the eight samples (the even lane samples 0-3,
+    // the odd lane 4-7) and they swap the half the other needs.
+    int fj[4], fk[4];
+    bool fcos[4], flive[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        const int f = c0 + 16 * cb + r;
+        int j = 0, k = 0;
+        flive[cb] = layer == 0 && cb < ncb && guidePair(f >> 1, j, k);
+        fj[cb] = j;
+        fk[cb] = k;
+        fcos[cb] = (f & 1) != 0;
+    }
     v4f acc[4][4];
     float bs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -549,14 +577,37 @@ __global__ __launch_bounds__(64) void guideWeightGradKernel(GuideTrainArgs a, ui
         for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     for (uint32_t q = q0; q < q1; ++q) {
         const uint16_t* dq = dz + size_t(q) * N * 32;
-        const uint16_t* aq = act + size_t(q) * K * 32;
         u32x4 A[4], B[4];
 #pragma unroll
         for (int rb = 0; rb < 4; ++rb)
             A[rb] = rb < nrb ? *reinterpret_cast<const u32x4*>(dq + 16 * rb * 32) : u32x4{0u, 0u, 0u, 0u};
+        if (layer == 0) {
+            const float* sq = a.scal + size_t(q) * 9 * 32 + 8 * g + 4 * (r & 1);
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb)
-            B[cb] = cb < ncb ? *reinterpret_cast<const u32x4*>(aq + 16 * cb * 32) : u32x4{0u, 0u, 0u, 0u};
+            for (int cb = 0; cb < 2; ++cb) {
+                const float4 x = *reinterpret_cast<const float4*>(sq + fj[cb] * 32);
+                const float xs[4] = {x.x, x.y, x.z, x.w};
+                float sn[4], cs[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float2 sc = guideFeaturePair(xs[i], fk[cb]);
+                    sn[i] = sc.x;
+                    cs[i] = sc.y;
+                }
+                const uint32_t own0 = fcos[cb] ? packBf2(cs[0], cs[1]) : packBf2(sn[0], sn[1]);
+                const uint32_t own1 = fcos[cb] ? packBf2(cs[2], cs[3]) : packBf2(sn[2], sn[3]);
+                const uint32_t oth0 = __shfl_xor(fcos[cb] ? packBf2(sn[0], sn[1]) : packBf2(cs[0], cs[1]), 1);
+                const uint32_t oth1 = __shfl_xor(fcos[cb] ? packBf2(sn[2], sn[3]) : packBf2(cs[2], cs[3]), 1);
+                B[cb] = fcos[cb] ? u32x4{oth0, oth1, own0, own1} : u32x4{own0, own1, oth0, oth1};
+                if (!flive[cb]) B[cb] = u32x4{0u, 0u, 0u, 0u};
+            }
+            B[2] = B[3] = u32x4{0u, 0u, 0u, 0u};
+        } else {
+            const uint16_t* aq = act + size_t(q) * K * 32;
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+                B[cb] = cb < ncb ? *reinterpret_cast<const u32x4*>(aq + 16 * cb * 32) : u32x4{0u, 0u, 0u, 0u};
+        }
 #pragma unroll
         for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
@@ -575,7 +626,7 @@ __global__ __launch_bounds__(64) void guideWeightGradKernel(GuideTrainArgs a, ui
         for (int cb = 0; cb < 4; ++cb)
             if (rb < nrb && cb < ncb)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) out[kGOffW[layer] + (16 * rb + 4 * g + j) * K + 64 * kt + 16 * cb + r] = acc[rb][cb][j];
+                for (int j = 0; j < 4; ++j) out[kGOffW[layer] + (16 * rb + 4 * g + j) * K + c0 + 16 * cb + r] = acc[rb][cb][j];
     if (kt == 0)
 #pragma unroll
         for (int rb = 0; rb < 4; ++rb) {
@@ -753,9 +804,9 @@ int ensureTrain(rbr_ctx* ctx, GuideState& g, size_t n) {
     const size_t cap = (n + kTile * kTT - 1) / (kTile * kTT) * (kTile * kTT);
     if (cap > g.cap) {
         for (int l = 0; l < 5; ++l) {
-            const int K = l == 0 ? kGInPad : kGHid;
             const int N = l == 4 ? kGOutPad : kGHid;
-            if (int rc = ensureBuffer(ctx, g.act[l], cap * K * 2)) return rc;
+            // act[0] holds the 9 FP32 input scalars per sample, the others BF16 activations
+            if (int rc = ensureBuffer(ctx, g.act[l], l == 0 ? cap * 9 * 4 : cap * kGHid * 2)) return rc;
             if (int rc = ensureBuffer(ctx, g.dz[l], cap * N * 2)) return rc;
         }
         g.cap = cap;
@@ -782,8 +833,10 @@ GuideTrainArgs trainArgs(rbr_ctx* ctx, GuideState& g, const float4* samples, uin
     a.cap = g.cap;
     a.scale = scale;
     a.cmax = INFINITY;
+    a.scal = static_cast<float*>(g.act[0].ptr);
+    a.act[0] = nullptr;
     for (int l = 0; l < 5; ++l) {
-        a.act[l] = static_cast<uint16_t*>(g.act[l].ptr);
+        if (l > 0) a.act[l] = static_cast<uint16_t*>(g.act[l].ptr);
         a.dz[l] = static_cast<uint16_t*>(g.dz[l].ptr);
     }
     a.grad = static_cast<float*>(g.dGrad.ptr);

@@ -84,6 +84,23 @@ __device__ __forceinline__ u32x4 guideEncodeBlock(const float* s9, int g, int kb
     return out;
 }
 
+// Features (sin, cos)(2 pi 2^(k-1) x) of input scalar x (frequency k = 0..11), by the operations
+// guideEncodeBlock applies for them: the direct evaluation at frequency 3 (k / 3), then k % 3
+// angle doublings.  Equal to that function's FP32 values bit for bit (same operations, same
+// order), so the BF16 features match too.  Both doublings are evaluated and one result
+// selected, so lanes with different k do not diverge.
+__device__ __forceinline__ float2 guideFeaturePair(float x, int k) {
+    const int g = k / 3, d = k - 3 * g;
+    const float r = x * __builtin_ldexpf(1.0f, 3 * g - 1);
+    const float f = __builtin_amdgcn_fractf(r);
+    const float s0 = __builtin_amdgcn_sinf(f), c0 = __builtin_amdgcn_cosf(f);
+    const float t0 = s0 + s0;
+    const float s1 = t0 * c0, c1 = __builtin_fmaf(-t0, s0, 1.0f);
+    const float t1 = s1 + s1;
+    const float s2 = t1 * c1, c2 = __builtin_fmaf(-t1, s1, 1.0f);
+    return make_float2(d == 0 ? s0 : d == 1 ? s1 : s2, d == 0 ? c0 : d == 1 ? c1 : c2);
+}
+
 // Per-query guide record written by inference for the shading kernel: 5 x 8 FP16,
 // [mu.x of lobes 0..7][mu.y][mu.z][kappa][weight].
 constexpr uint32_t kGuideRecBytes = 80;
@@ -117,7 +134,9 @@ struct GuideTrainArgs {
     float cmax;               // training targets are clamped to cmax before scaling
     size_t cap;               // samples the activation arrays hold (multiple of 32)
     float scale;              // multiplies c (1 / (batch * mean c))
-    uint16_t* act[5];         // A_l, BF16, blocked [cap / 32][K_l][32 samples]
+    uint16_t* act[5];         // A_l (l >= 1), BF16, blocked [cap / 32][K_l][32 samples]
+    float* scal;              // the 9 input scalars, FP32, blocked [cap / 32][9][32 samples]:
+                              // the weight gradient derives A_0 from them (guideFeaturePair)
     uint16_t* dz[5];          // dZ_l, BF16, blocked [cap / 32][N_l][32 samples]
     float* grad;              // FP32 gradient [kGParams]
     double* loss;
