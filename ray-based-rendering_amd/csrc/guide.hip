@@ -21,7 +21,7 @@
 //                        the matrix cores, which is exact for BF16 values.  For A_0 only the 9
 //                        input scalars are written (36 of its 448 bytes per sample)
 //   guideWeightGradKernel  dW_l = dZ_l^T A_l and db_l = column sums of dZ_l: one 64 x 64 tile of
-//                        one layer per wave over a chunk of 1024 samples (16x16x32 MFMA with
+//                        one layer per wave over a chunk of 512 or 1024 samples (16x16x32 MFMA with
 //                        the samples as the reduction index), into a per-chunk partial
 //                        gradient; A_0 is recomputed from the scalars bit for bit
 //   guideGradReduceKernel  sums the partial gradients in chunk order (deterministic)
@@ -260,7 +260,9 @@ namespace {
 
 constexpr int kTT = 2;                 // 16-sample tiles per training wave group: 32 samples
 constexpr int kTrainWaves = 8;         // waves per training workgroup
-constexpr uint32_t kGradChunk = 1024;  // samples per partial weight gradient
+// Samples per partial weight gradient: 1,024, or 512 for batches of at most 2^17 samples, whose
+// 1,024-sample chunks leave too few waves to fill the chip (measured optimum of each size).
+__host__ __device__ constexpr uint32_t gradChunk(size_t n) { return n <= (size_t(1) << 17) ? 512u : 1024u; }
 constexpr int kGradL0Tiles = kGInPad / 32;  // weight-gradient tiles of layer 0: 64 x 32 column blocks
 constexpr int kGradTiles = kGradL0Tiles + 4;  // and one 64 x 64 (48 x 64) tile for each of layers 1-4
                                        // (the last has 32 columns), layers 1-4 one each
@@ -536,7 +538,7 @@ __global__ __launch_bounds__(512) void guideFwdBwdKernel(GuideTrainArgs a) {
 // 16-byte load; a block is 1 KB contiguous), so the sample index is the MFMA's reduction index.  Workgroups are dealt to the 8 XCDs in turn; the
 // mapping keeps the 8 tiles of a chunk on one XCD, whose L2 then serves the dZ_0 rows that the
 // four layer-0 tiles share.
-__global__ __launch_bounds__(64) void guideWeightGradKernel(GuideTrainArgs a, uint32_t chunks, float* __restrict__ partial) {
+__global__ __launch_bounds__(64) void guideWeightGradKernel(GuideTrainArgs a, uint32_t chunks, uint32_t chunkGroups, float* __restrict__ partial) {
     const uint32_t w = blockIdx.x, xcd = w % 8u, slot = w / 8u;
     const int tile = int(slot % kGradTiles);
     const uint32_t ch = (slot / kGradTiles) * 8u + xcd;
@@ -550,7 +552,7 @@ __global__ __launch_bounds__(64) void guideWeightGradKernel(GuideTrainArgs a, ui
     const int N = layer == 4 ? kGOutPad : kGHid;
     const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
     const uint32_t groups = (a.n + 31u) / 32u;
-    const uint32_t q0 = ch * (kGradChunk / 32), q1 = min(groups, q0 + kGradChunk / 32);
+    const uint32_t q0 = ch * chunkGroups, q1 = min(groups, q0 + chunkGroups);
     const uint16_t* dz = a.dz[layer] + r * 32 + 8 * g;
     const uint16_t* act = layer == 0 ? nullptr : a.act[layer] + r * 32 + 8 * g;
     // Layer 0: the lane's B operand of column block cb is feature c0 + 16 cb + r of samples
@@ -798,7 +800,7 @@ int refreshBf16(rbr_ctx* ctx, GuideState& g) {
     return RBR_OK;
 }
 
-uint32_t gradChunks(size_t n) { return uint32_t(((n + 31) / 32 * 32 + kGradChunk - 1) / kGradChunk); }
+uint32_t gradChunks(size_t n) { return uint32_t(((n + 31) / 32 * 32 + gradChunk(n) - 1) / gradChunk(n)); }
 
 int ensureTrain(rbr_ctx* ctx, GuideState& g, size_t n) {
     const size_t cap = (n + kTile * kTT - 1) / (kTile * kTT) * (kTile * kTT);
@@ -854,7 +856,7 @@ int launchLossGrad(rbr_ctx* ctx, GuideState& g, const GuideTrainArgs& a) {
     const uint32_t chunks = gradChunks(a.n);
     float* partial = static_cast<float*>(g.dPartial.ptr);
     hipLaunchKernelGGL(guideWeightGradKernel, dim3((chunks + 7) / 8 * 8 * kGradTiles), dim3(64), 0, ctx->stream, a, chunks,
-                       partial);
+                       gradChunk(a.n) / 32, partial);
     hipLaunchKernelGGL(guideGradReduceKernel, dim3((kGParams + 255) / 256), dim3(256), 0, ctx->stream, partial, chunks,
                        a.grad);
     hipError_t e = hipGetLastError();
